@@ -1,0 +1,867 @@
+// Host side of the C ABI declared in include/deme/deme.h: device memory ownership, stage orchestration, error mapping
+// and per-kernel event timing. One context == one device + one HIP stream; no global mutable state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/deme/deme.h"
+#include "deme_internal.h"
+
+using namespace deme_int;
+
+namespace {
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n, double grow = 1.25) {
+        if (n <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>((size_t)(n * grow), 16);
+        hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct TimedLaunch {
+    int kid;
+    hipEvent_t s, e;
+};
+
+}  // namespace
+
+struct deme_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+    std::string err;
+
+    deme_sim_params sp{};
+    KParams kp{};
+    bool haveParams = false, haveTemplates = false, haveState = false, haveMaterials = false, haveFamilies = false;
+    uint32_t nS = 0, nO = 0, nClumps = 0, nA = 0, nM = 0, nAnalOwners = 0;
+    uint32_t maxMatSph = 0, maxMatAnal = 0, maxMat = 0;
+    uint64_t nBins = 0;
+    std::vector<float> hostMass;
+
+    // static data
+    DBuf<SphereRec> sph;
+    DBuf<OwnerRec> own;
+    DBuf<float4> moi;
+    DBuf<uint32_t> ownerSphStart;
+    DBuf<deme_anal_obj> anal;
+    DBuf<PlanePose> pose;
+    DBuf<deme_material> mats;
+    DBuf<deme_family_rule> rules;
+    DBuf<uint8_t> mask;
+    DBuf<float> acc;  // 6 x nO
+
+    // contact detection
+    DBuf<uint32_t> binCount, binStart;
+    DBuf<uint16_t> numBins;
+    DBuf<uint32_t> numAnal;
+    DBuf<CdRec> rec;
+    DBuf<uint2> binEnt;
+    DBuf<uint2> pairs;
+    DBuf<uint32_t> rowCount, bCount, bStart, bList;
+    DBuf<uint64_t> scanScratch;
+    DBuf<DevCounters> ctr;
+    DevCounters* hctr = nullptr;  // pinned mirror
+
+    // double-buffered contact arrays (index cur = current list, cur ^ 1 = previous list)
+    DBuf<uint32_t> rowStart[2], idB[2];
+    DBuf<uint8_t> ctype[2];
+    DBuf<float4> hist[2];
+    DBuf<uint32_t> idA, map;
+    DBuf<ForceRec> frc;
+    DBuf<float> analPartials, analAcc;
+    int cur = 0;
+    bool havePrev = false, contactsReady = false, fresh = false;
+    uint64_t P = 0, C = 0, Cprev = 0, nPairs = 0, nActive = 0, maxOcc = 0, nGeo = 0;
+    int cdStage = 0;  // 0 none, 1 counted, 2 filled, 3 pairs, 4 done
+
+    double time = 0.0;
+    uint64_t nSteps = 0;
+
+    // per-kernel timing
+    bool timing = false;
+    std::vector<std::string> knames;
+    std::map<std::string, int> kidx;
+    std::vector<double> ktotal;
+    std::vector<uint64_t> kcount;
+    std::vector<hipEvent_t> evPool;
+    std::vector<TimedLaunch> pending;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    int fail_hip(hipError_t e, const char* where) {
+        err = std::string("HIP error in ") + where + ": " + hipGetErrorString(e);
+        return DEME_ERR_HIP;
+    }
+};
+
+namespace {
+
+hipEvent_t take_event(deme_ctx* c) {
+    if (c->evPool.empty()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    hipEvent_t e = c->evPool.back();
+    c->evPool.pop_back();
+    return e;
+}
+
+int kernel_id(deme_ctx* c, const char* name) {
+    auto it = c->kidx.find(name);
+    if (it != c->kidx.end()) return it->second;
+    int id = (int)c->knames.size();
+    c->knames.push_back(name);
+    c->kidx[name] = id;
+    c->ktotal.push_back(0.0);
+    c->kcount.push_back(0);
+    return id;
+}
+
+hipError_t drain_timing(deme_ctx* c) {
+    if (c->pending.empty()) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    for (auto& t : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.s, t.e) == hipSuccess) {
+            c->ktotal[t.kid] += ms;
+            c->kcount[t.kid]++;
+        }
+        c->evPool.push_back(t.s);
+        c->evPool.push_back(t.e);
+    }
+    c->pending.clear();
+    return hipSuccess;
+}
+
+struct KTimer {
+    deme_ctx* c;
+    int kid = -1;
+    hipEvent_t s = nullptr, e = nullptr;
+    KTimer(deme_ctx* ctx, const char* name) : c(ctx) {
+        if (!c->timing) return;
+        kid = kernel_id(c, name);
+        s = take_event(c);
+        e = take_event(c);
+        if (s && e) (void)hipEventRecord(s, c->stream);
+    }
+    ~KTimer() {
+        if (kid < 0 || !s || !e) return;
+        (void)hipEventRecord(e, c->stream);
+        c->pending.push_back({kid, s, e});
+        if (c->pending.size() > 8192) (void)drain_timing(c);
+    }
+};
+
+#define DEME_LAUNCH(ctx, name, expr)                          \
+    do {                                                       \
+        KTimer kt_(ctx, name);                                 \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) return (ctx)->fail_hip(e_, name); \
+    } while (0)
+
+#define DEME_HIP(ctx, expr)                                       \
+    do {                                                           \
+        hipError_t e_ = (expr);                                    \
+        if (e_ != hipSuccess) return (ctx)->fail_hip(e_, #expr);   \
+    } while (0)
+
+int check_ready(deme_ctx* c) {
+    if (!c->haveParams) return c->fail(DEME_ERR_NOT_READY, "simulation parameters not set (deme_set_params)");
+    if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "templates not uploaded (deme_upload_templates)");
+    if (!c->haveState) return c->fail(DEME_ERR_NOT_READY, "owner state not uploaded (deme_upload_state)");
+    if (!c->haveMaterials) return c->fail(DEME_ERR_NOT_READY, "materials not uploaded (deme_upload_materials)");
+    if (!c->haveFamilies) return c->fail(DEME_ERR_NOT_READY, "family data not uploaded (deme_upload_families)");
+    if (c->maxMat >= c->nM) return c->fail(DEME_ERR_INVALID_ARGUMENT, "a material index exceeds the material table");
+    return DEME_OK;
+}
+
+int readback_counters(deme_ctx* c) {
+    DEME_HIP(c, hipMemcpyAsync(c->hctr, c->ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    return DEME_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int deme_device_count(int* n) {
+    int k = 0;
+    hipError_t e = hipGetDeviceCount(&k);
+    if (e != hipSuccess) k = 0;
+    *n = k;
+    return k > 0 ? DEME_OK : DEME_ERR_NO_DEVICE;
+}
+
+int deme_ctx_create(int device, void* stream, deme_ctx** out) {
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DEME_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return DEME_ERR_INVALID_ARGUMENT;
+    deme_ctx* c = new (std::nothrow) deme_ctx();
+    if (!c) return DEME_ERR_INVALID_ARGUMENT;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return DEME_ERR_HIP;
+    }
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return DEME_ERR_HIP;
+        }
+        c->ownStream = true;
+    }
+    if (c->ctr.ensure(1, 1.0) != hipSuccess ||
+        hipHostMalloc((void**)&c->hctr, sizeof(DevCounters), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return DEME_ERR_HIP;
+    }
+    std::memset(c->hctr, 0, sizeof(DevCounters));
+    *out = c;
+    return DEME_OK;
+}
+
+int deme_ctx_destroy(deme_ctx* c) {
+    if (!c) return DEME_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& t : c->pending) {
+        (void)hipEventDestroy(t.s);
+        (void)hipEventDestroy(t.e);
+    }
+    for (auto e : c->evPool) (void)hipEventDestroy(e);
+    c->sph.release(); c->own.release(); c->moi.release(); c->ownerSphStart.release(); c->anal.release();
+    c->pose.release(); c->mats.release(); c->rules.release(); c->mask.release(); c->acc.release();
+    c->binCount.release(); c->binStart.release(); c->numBins.release(); c->numAnal.release(); c->rec.release();
+    c->binEnt.release(); c->pairs.release(); c->rowCount.release(); c->bCount.release(); c->bStart.release();
+    c->bList.release(); c->scanScratch.release(); c->ctr.release();
+    for (int k = 0; k < 2; k++) {
+        c->rowStart[k].release(); c->idB[k].release(); c->ctype[k].release(); c->hist[k].release();
+    }
+    c->idA.release(); c->map.release(); c->frc.release(); c->analPartials.release(); c->analAcc.release();
+    if (c->hctr) (void)hipHostFree(c->hctr);
+    if (c->ownStream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DEME_OK;
+}
+
+const char* deme_last_error(const deme_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int deme_get_stream(const deme_ctx* c, void** s) {
+    *s = (void*)c->stream;
+    return DEME_OK;
+}
+
+int deme_sync(deme_ctx* c) {
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    return DEME_OK;
+}
+
+int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
+    if (p->nvXp2 + p->nvYp2 + p->nvZp2 != 64) return c->fail(DEME_ERR_INVALID_ARGUMENT, "nvXp2 + nvYp2 + nvZp2 != 64");
+    if (!(p->binSize > 0.0)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "bin size must be positive");
+    if (!(p->l > 0.0) || !(p->voxelSize > 0.0)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "l / voxelSize must be > 0");
+    const uint64_t nb = (uint64_t)p->nbX * p->nbY * p->nbZ;
+    if (nb == 0 || nb > 0xFFFFFFFFull)
+        return c->fail(DEME_ERR_CAPACITY, "number of bins exceeds binID_t (2^32 - 1); use a larger bin size");
+    c->sp = *p;
+    KParams& k = c->kp;
+    k.l = p->l;
+    k.voxelSize = p->voxelSize;
+    k.binSize = p->binSize;
+    k.nbX = p->nbX;
+    k.nbY = p->nbY;
+    k.nbZ = p->nbZ;
+    k.nBins = (uint32_t)nb;
+    k.nvX = p->nvXp2;
+    k.nvY = p->nvYp2;
+    k.beta = p->beta;
+    k.h = p->h;
+    k.Gx = p->G[0];
+    k.Gy = p->G[1];
+    k.Gz = p->G[2];
+    k.integrator = p->integrator;
+    if (nb != c->nBins) {
+        c->nBins = nb;
+        DEME_HIP(c, c->binCount.ensure(nb, 1.0));
+        DEME_HIP(c, c->binStart.ensure(nb + 1, 1.0));
+        DEME_HIP(c, hipMemsetAsync(c->binCount.p, 0, nb * sizeof(uint32_t), c->stream));
+        DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>(nb, 1u << 22)), 1.0));
+    }
+    c->haveParams = true;
+    return DEME_OK;
+}
+
+int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, const float* sphRadius,
+                          const float* sphRelPos, const uint8_t* sphMat, uint32_t nO, uint32_t nClumps,
+                          const float* ownerMass, const float* ownerMoi) {
+    if (nClumps > nO) return c->fail(DEME_ERR_INVALID_ARGUMENT, "nClumps > nOwners");
+    if (nO - nClumps > MAX_ANAL_OWNERS)
+        return c->fail(DEME_ERR_UNSUPPORTED, "more than 8 external-object owners are not supported");
+    std::vector<SphereRec> hs(nS);
+    std::vector<uint32_t> start(nClumps + 1, 0);
+    uint32_t maxMat = 0;
+    for (uint32_t s = 0; s < nS; s++) {
+        maxMat = std::max<uint32_t>(maxMat, sphMat[s]);
+        if (sphOwner[s] >= nClumps) return c->fail(DEME_ERR_INVALID_ARGUMENT, "sphere owner is not a clump");
+        if (s > 0 && sphOwner[s] < sphOwner[s - 1])
+            return c->fail(DEME_ERR_INVALID_ARGUMENT, "spheres must be grouped by owner in ascending order");
+        hs[s].owner = sphOwner[s];
+        hs[s].r = sphRadius[s];
+        hs[s].rx = sphRelPos[3 * s];
+        hs[s].ry = sphRelPos[3 * s + 1];
+        hs[s].rz = sphRelPos[3 * s + 2];
+        hs[s].mat = sphMat[s];
+        hs[s].pad0 = hs[s].pad1 = 0;
+        start[sphOwner[s] + 1]++;
+    }
+    for (uint32_t o = 0; o < nClumps; o++) start[o + 1] += start[o];
+    std::vector<float4> hm(nO);
+    for (uint32_t o = 0; o < nO; o++) hm[o] = make_float4(ownerMoi[3 * o], ownerMoi[3 * o + 1], ownerMoi[3 * o + 2], 0.f);
+    c->hostMass.assign(ownerMass, ownerMass + nO);
+    DEME_HIP(c, c->sph.ensure(nS, 1.0));
+    DEME_HIP(c, c->moi.ensure(nO, 1.0));
+    DEME_HIP(c, c->own.ensure(nO, 1.0));
+    DEME_HIP(c, c->acc.ensure((size_t)6 * nO, 1.0));
+    DEME_HIP(c, c->ownerSphStart.ensure(nClumps + 1, 1.0));
+    DEME_HIP(c, c->numBins.ensure(nS, 1.0));
+    DEME_HIP(c, c->numAnal.ensure(nS, 1.0));
+    DEME_HIP(c, c->rec.ensure(nS, 1.0));
+    DEME_HIP(c, c->rowCount.ensure(nS, 1.0));
+    DEME_HIP(c, c->bCount.ensure(nClumps + 1, 1.0));
+    DEME_HIP(c, c->bStart.ensure(nClumps + 1, 1.0));
+    for (int k = 0; k < 2; k++) DEME_HIP(c, c->rowStart[k].ensure(nS + 1, 1.0));
+    DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>({c->nBins, (uint64_t)nS, 1u << 22})), 1.0));
+    if (nS) DEME_HIP(c, hipMemcpyAsync(c->sph.p, hs.data(), nS * sizeof(SphereRec), hipMemcpyHostToDevice, c->stream));
+    if (nO) DEME_HIP(c, hipMemcpyAsync(c->moi.p, hm.data(), nO * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    DEME_HIP(c, hipMemcpyAsync(c->ownerSphStart.p, start.data(), (nClumps + 1) * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, c->stream));
+    DEME_HIP(c, hipMemsetAsync(c->rowCount.p, 0, std::max<size_t>(nS, 1) * sizeof(uint32_t), c->stream));
+    DEME_HIP(c, hipMemsetAsync(c->bCount.p, 0, (nClumps + 1) * sizeof(uint32_t), c->stream));
+    DEME_HIP(c, hipMemsetAsync(c->acc.p, 0, std::max<size_t>((size_t)6 * nO, 1) * sizeof(float), c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->nS = nS;
+    c->nO = nO;
+    c->nClumps = nClumps;
+    c->nAnalOwners = nO - nClumps;
+    c->maxMatSph = maxMat;
+    c->maxMat = std::max(c->maxMatSph, c->maxMatAnal);
+    c->haveTemplates = true;
+    c->haveState = false;
+    c->havePrev = c->contactsReady = c->fresh = false;
+    c->C = c->Cprev = 0;
+    return DEME_OK;
+}
+
+int deme_upload_materials(deme_ctx* c, uint32_t n, const deme_material* m) {
+    if (n == 0 || n > 256) return c->fail(DEME_ERR_INVALID_ARGUMENT, "1..256 materials required");
+    DEME_HIP(c, c->mats.ensure(n, 1.0));
+    DEME_HIP(c, hipMemcpyAsync(c->mats.p, m, n * sizeof(deme_material), hipMemcpyHostToDevice, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->nM = n;
+    c->haveMaterials = true;
+    return DEME_OK;
+}
+
+int deme_upload_analytical(deme_ctx* c, uint32_t n, const deme_anal_obj* objs) {
+    if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before analytical objects");
+    if (n > 64) return c->fail(DEME_ERR_UNSUPPORTED, "more than 64 analytical components");
+    uint32_t maxMat = 0;
+    for (uint32_t a = 0; a < n; a++) {
+        if (objs[a].owner < c->nClumps || objs[a].owner >= c->nO)
+            return c->fail(DEME_ERR_INVALID_ARGUMENT, "analytical component owner must be an external-object owner");
+        maxMat = std::max<uint32_t>(maxMat, objs[a].material);
+    }
+    c->maxMatAnal = maxMat;
+    c->maxMat = std::max(c->maxMatSph, c->maxMatAnal);
+    DEME_HIP(c, c->anal.ensure(std::max<uint32_t>(n, 1), 1.0));
+    DEME_HIP(c, c->pose.ensure(std::max<uint32_t>(n, 1), 1.0));
+    if (n) DEME_HIP(c, hipMemcpyAsync(c->anal.p, objs, n * sizeof(deme_anal_obj), hipMemcpyHostToDevice, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->nA = n;
+    return DEME_OK;
+}
+
+int deme_upload_families(deme_ctx* c, const uint8_t* mask, const deme_family_rule* rules) {
+    DEME_HIP(c, c->mask.ensure(DEME_NUM_MASK_ENTRIES, 1.0));
+    DEME_HIP(c, c->rules.ensure(DEME_NUM_FAMILIES, 1.0));
+    if (!rules && !c->haveFamilies) return c->fail(DEME_ERR_INVALID_ARGUMENT, "family rules required on first upload");
+    DEME_HIP(c, hipMemcpyAsync(c->mask.p, mask, DEME_NUM_MASK_ENTRIES, hipMemcpyHostToDevice, c->stream));
+    if (rules)
+        DEME_HIP(c, hipMemcpyAsync(c->rules.p, rules, DEME_NUM_FAMILIES * sizeof(deme_family_rule),
+                                   hipMemcpyHostToDevice, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->haveFamilies = true;
+    return DEME_OK;
+}
+
+int deme_upload_state(deme_ctx* c, const deme_owner_state* s) {
+    if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before the owner state");
+    const uint32_t n = c->nO;
+    std::vector<OwnerRec> h(n);
+    if (c->haveState && n) {
+        DEME_HIP(c, hipMemcpyAsync(h.data(), c->own.p, n * sizeof(OwnerRec), hipMemcpyDeviceToHost, c->stream));
+        DEME_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::memset(h.data(), 0, n * sizeof(OwnerRec));
+        for (uint32_t o = 0; o < n; o++) h[o].qw = 1.f;
+    }
+    for (uint32_t o = 0; o < n; o++) {
+        OwnerRec& r = h[o];
+        if (s->voxelID) r.voxel = s->voxelID[o];
+        if (s->locX) r.lx = s->locX[o];
+        if (s->locY) r.ly = s->locY[o];
+        if (s->locZ) r.lz = s->locZ[o];
+        if (s->family) r.fam = s->family[o];
+        if (s->oriQw) r.qw = s->oriQw[o];
+        if (s->oriQx) r.qx = s->oriQx[o];
+        if (s->oriQy) r.qy = s->oriQy[o];
+        if (s->oriQz) r.qz = s->oriQz[o];
+        if (s->vX) r.vx = s->vX[o];
+        if (s->vY) r.vy = s->vY[o];
+        if (s->vZ) r.vz = s->vZ[o];
+        if (s->omgX) r.wx = s->omgX[o];
+        if (s->omgY) r.wy = s->omgY[o];
+        if (s->omgZ) r.wz = s->omgZ[o];
+        r.mass = c->hostMass[o];
+        r.pad0 = 0;
+        r.pad1 = 0.f;
+    }
+    if (n) DEME_HIP(c, hipMemcpyAsync(c->own.p, h.data(), n * sizeof(OwnerRec), hipMemcpyHostToDevice, c->stream));
+    // accelerations, when given
+    float* accs[6] = {s->aX, s->aY, s->aZ, s->alphaX, s->alphaY, s->alphaZ};
+    for (int k = 0; k < 6; k++)
+        if (accs[k] && n)
+            DEME_HIP(c, hipMemcpyAsync(c->acc.p + (size_t)k * n, accs[k], n * sizeof(float), hipMemcpyHostToDevice,
+                                       c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->haveState = true;
+    return DEME_OK;
+}
+
+int deme_download_state(deme_ctx* c, deme_owner_state* s) {
+    if (!c->haveState) return c->fail(DEME_ERR_NOT_READY, "no owner state");
+    const uint32_t n = c->nO;
+    std::vector<OwnerRec> h(n);
+    std::vector<float> a((size_t)6 * n);
+    if (n) {
+        DEME_HIP(c, hipMemcpyAsync(h.data(), c->own.p, n * sizeof(OwnerRec), hipMemcpyDeviceToHost, c->stream));
+        DEME_HIP(c, hipMemcpyAsync(a.data(), c->acc.p, (size_t)6 * n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    for (uint32_t o = 0; o < n; o++) {
+        const OwnerRec& r = h[o];
+        if (s->voxelID) s->voxelID[o] = r.voxel;
+        if (s->locX) s->locX[o] = r.lx;
+        if (s->locY) s->locY[o] = r.ly;
+        if (s->locZ) s->locZ[o] = r.lz;
+        if (s->family) s->family[o] = r.fam;
+        if (s->oriQw) s->oriQw[o] = r.qw;
+        if (s->oriQx) s->oriQx[o] = r.qx;
+        if (s->oriQy) s->oriQy[o] = r.qy;
+        if (s->oriQz) s->oriQz[o] = r.qz;
+        if (s->vX) s->vX[o] = r.vx;
+        if (s->vY) s->vY[o] = r.vy;
+        if (s->vZ) s->vZ[o] = r.vz;
+        if (s->omgX) s->omgX[o] = r.wx;
+        if (s->omgY) s->omgY[o] = r.wy;
+        if (s->omgZ) s->omgZ[o] = r.wz;
+    }
+    float* accs[6] = {s->aX, s->aY, s->aZ, s->alphaX, s->alphaY, s->alphaZ};
+    for (int k = 0; k < 6; k++)
+        if (accs[k]) std::memcpy(accs[k], a.data() + (size_t)k * n, n * sizeof(float));
+    return DEME_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Contact detection stages
+// ---------------------------------------------------------------------------------------------------------------
+int deme_cd_bin_count(deme_ctx* c) {
+    int r = check_ready(c);
+    if (r) return r;
+    DEME_HIP(c, hipMemsetAsync(c->ctr.p, 0, sizeof(DevCounters), c->stream));
+    DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
+    DEME_LAUNCH(c, "sphere_bins",
+                launch_sphere_bins(c->kp, c->sph.p, c->own.p, c->nS, c->pose.p, c->nA, c->mask.p, c->binCount.p,
+                                   c->numBins.p, c->numAnal.p, c->rec.p, c->ctr.p, c->stream));
+    DEME_LAUNCH(c, "bin_stats", launch_bin_stats(c->binCount.p, c->nBins, c->ctr.p, c->stream));
+    c->cdStage = 1;
+    return DEME_OK;
+}
+
+int deme_cd_bin_fill(deme_ctx* c) {
+    if (c->cdStage != 1) return c->fail(DEME_ERR_NOT_READY, "deme_cd_bin_count must run first");
+    DEME_LAUNCH(c, "scan_bins",
+                scan_u32(c->binCount.p, nullptr, c->nBins, c->binStart.p, &c->ctr.p->total[0], c->scanScratch.p,
+                         c->stream));
+    int r = readback_counters(c);
+    if (r) return r;
+    const DevCounters& h = *c->hctr;
+    if (h.err || h.maxOcc > DEME_MAX_SPHERES_PER_BIN) {
+        // discard the histogram so the next pass starts from zero
+        DEME_LAUNCH(c, "fill", launch_fill_u32(c->binCount.p, c->nBins, 0, c->stream));
+        DEME_HIP(c, hipStreamSynchronize(c->stream));
+        c->cdStage = 0;
+        if (h.err & ERRBIT_CAPACITY)
+            return c->fail(DEME_ERR_CAPACITY, "a sphere touches more than 65535 bins (binsSphereTouches_t)");
+        if (h.err & ERRBIT_OUTSIDE) return c->fail(DEME_ERR_CAPACITY, "a sphere left the bin grid of the simulation world");
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "a bin contains %u sphere components, exceeding maximum allowance (%u)",
+                      h.maxOcc, DEME_MAX_SPHERES_PER_BIN);
+        return c->fail(DEME_ERR_BIN_OVERFLOW, buf);
+    }
+    c->P = h.total[0];
+    c->nActive = h.nActive;
+    c->maxOcc = h.maxOcc;
+    if (c->P > 0xFFFFFFFFull) {
+        DEME_LAUNCH(c, "fill", launch_fill_u32(c->binCount.p, c->nBins, 0, c->stream));
+        c->cdStage = 0;
+        return c->fail(DEME_ERR_CAPACITY, "bin-sphere touch count exceeds 2^32 - 1");
+    }
+    DEME_HIP(c, c->binEnt.ensure(c->P));
+    DEME_LAUNCH(c, "bin_scatter",
+                launch_bin_scatter(c->kp, c->rec.p, c->sph.p, c->nS, c->binStart.p, c->binCount.p, c->binEnt.p,
+                                   c->ctr.p, c->stream));
+    c->cdStage = 2;
+    return DEME_OK;
+}
+
+int deme_cd_pairs(deme_ctx* c) {
+    if (c->cdStage != 2) return c->fail(DEME_ERR_NOT_READY, "deme_cd_bin_fill must run first");
+    uint64_t cap = std::max<uint64_t>({(uint64_t)(c->nPairs * 1.25) + 4096, c->pairs.cap, c->P / 2 + 4096});
+    for (int attempt = 0; attempt < 3; attempt++) {
+        DEME_HIP(c, c->pairs.ensure(cap, 1.0));
+        DEME_HIP(c, hipMemsetAsync(&c->ctr.p->pairCount, 0, sizeof(uint32_t), c->stream));
+        DEME_LAUNCH(c, "bin_pairs",
+                    launch_bin_pairs(c->kp, c->binEnt.p, c->P, c->binStart.p, c->rec.p, c->sph.p, c->own.p, c->mask.p,
+                                     c->pairs.p, (uint32_t)std::min<uint64_t>(c->pairs.cap, 0xFFFFFFFFull), c->ctr.p,
+                                     c->rowCount.p, c->bCount.p, c->stream));
+        int r = readback_counters(c);
+        if (r) return r;
+        const uint64_t found = c->hctr->pairCount;
+        if (found <= c->pairs.cap) {
+            c->nPairs = found;
+            c->cdStage = 3;
+            return DEME_OK;
+        }
+        // capacity exceeded: clear the per-row counts and redo the pass with room for every hit
+        DEME_LAUNCH(c, "fill", launch_fill_u32(c->rowCount.p, c->nS, 0, c->stream));
+        DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCount.p, c->nClumps + 1, 0, c->stream));
+        cap = found + found / 4 + 4096;
+    }
+    c->cdStage = 0;
+    return c->fail(DEME_ERR_CAPACITY, "pair buffer could not be sized");
+}
+
+int deme_cd_sort_remap(deme_ctx* c) {
+    if (c->cdStage != 3) return c->fail(DEME_ERR_NOT_READY, "deme_cd_pairs must run first");
+    // the current list becomes the previous one
+    if (c->contactsReady) {
+        c->cur ^= 1;
+        c->havePrev = true;
+        c->Cprev = c->C;
+    }
+    const int cu = c->cur, pv = cu ^ 1;
+    DEME_LAUNCH(c, "scan_rows",
+                scan_u32(c->rowCount.p, c->numAnal.p, c->nS, c->rowStart[cu].p, &c->ctr.p->total[1], c->scanScratch.p,
+                         c->stream));
+    int r = readback_counters(c);
+    if (r) return r;
+    const uint64_t C = c->hctr->total[1];
+    if (C > 0xFFFFFFFFull) return c->fail(DEME_ERR_CAPACITY, "contact count exceeds contactPairs_t (2^32 - 1)");
+    c->C = C;
+    c->nGeo = C - c->nPairs;
+    DEME_HIP(c, c->idB[cu].ensure(C));
+    DEME_HIP(c, c->ctype[cu].ensure(C));
+    DEME_HIP(c, c->hist[cu].ensure(C));
+    DEME_HIP(c, c->idA.ensure(C));
+    DEME_HIP(c, c->map.ensure(C));
+    DEME_HIP(c, c->frc.ensure(C));
+    DEME_HIP(c, c->bList.ensure(c->nPairs));
+    DEME_LAUNCH(c, "row_fill",
+                launch_row_fill(c->pairs.p, (uint32_t)c->nPairs, c->rowStart[cu].p, c->numAnal.p, c->rowCount.p,
+                                c->idB[cu].p, c->ctype[cu].p, c->stream));
+    DEME_LAUNCH(c, "geo_fill",
+                launch_geo_fill(c->kp, c->rec.p, c->sph.p, c->own.p, c->nS, c->pose.p, c->nA, c->mask.p, c->numAnal.p,
+                                c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p, c->stream));
+    DEME_LAUNCH(c, "row_finalize",
+                launch_row_finalize(c->nS, c->rowStart[cu].p, c->numAnal.p, c->idA.p, c->idB[cu].p, c->stream));
+    DEME_LAUNCH(c, "history_map",
+                launch_history_map(c->nS, c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p,
+                                   c->havePrev ? c->rowStart[pv].p : nullptr, c->havePrev ? c->idB[pv].p : nullptr,
+                                   c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->stream));
+    DEME_LAUNCH(c, "scan_owners",
+                scan_u32(c->bCount.p, nullptr, c->nClumps, c->bStart.p, &c->ctr.p->total[2], c->scanScratch.p,
+                         c->stream));
+    DEME_LAUNCH(c, "bside_fill",
+                launch_bside_fill((uint32_t)C, c->idB[cu].p, c->ctype[cu].p, c->sph.p, c->bStart.p, c->bCount.p,
+                                  c->bList.p, c->stream));
+    DEME_LAUNCH(c, "bside_sort", launch_bside_sort(c->nClumps, c->bStart.p, c->bList.p, c->stream));
+    c->contactsReady = true;
+    c->fresh = true;
+    c->cdStage = 4;
+    return DEME_OK;
+}
+
+int deme_cd_run(deme_ctx* c) {
+    int r = deme_cd_bin_count(c);
+    if (!r) r = deme_cd_bin_fill(c);
+    if (!r) r = deme_cd_pairs(c);
+    if (!r) r = deme_cd_sort_remap(c);
+    return r;
+}
+
+int deme_download_bin_counts(deme_ctx* c, uint16_t* nb, uint8_t* na) {
+    if (c->cdStage < 1) return c->fail(DEME_ERR_NOT_READY, "no detection pass");
+    if (nb && c->nS) DEME_HIP(c, hipMemcpyAsync(nb, c->numBins.p, c->nS * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> tmp(c->nS);
+    if (na && c->nS)
+        DEME_HIP(c, hipMemcpyAsync(tmp.data(), c->numAnal.p, c->nS * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    if (na)
+        for (uint32_t s = 0; s < c->nS; s++) na[s] = (uint8_t)tmp[s];
+    return DEME_OK;
+}
+
+int deme_download_bins(deme_ctx* c, uint32_t* binStart, uint32_t* binSphere) {
+    if (c->cdStage < 2) return c->fail(DEME_ERR_NOT_READY, "bins not filled");
+    if (binStart)
+        DEME_HIP(c, hipMemcpyAsync(binStart, c->binStart.p, (c->nBins + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   c->stream));
+    std::vector<uint2> e(c->P);
+    if (binSphere && c->P)
+        DEME_HIP(c, hipMemcpyAsync(e.data(), c->binEnt.p, c->P * sizeof(uint2), hipMemcpyDeviceToHost, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    if (binSphere)
+        for (uint64_t k = 0; k < c->P; k++) binSphere[k] = e[k].x;
+    return DEME_OK;
+}
+
+int deme_download_contacts(deme_ctx* c, uint32_t* idA, uint32_t* idB, uint8_t* type, uint32_t* map) {
+    if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "no contact list");
+    const size_t C = c->C;
+    const int cu = c->cur;
+    if (C) {
+        if (idA) DEME_HIP(c, hipMemcpyAsync(idA, c->idA.p, C * 4, hipMemcpyDeviceToHost, c->stream));
+        if (idB) DEME_HIP(c, hipMemcpyAsync(idB, c->idB[cu].p, C * 4, hipMemcpyDeviceToHost, c->stream));
+        if (type) DEME_HIP(c, hipMemcpyAsync(type, c->ctype[cu].p, C, hipMemcpyDeviceToHost, c->stream));
+        if (map) DEME_HIP(c, hipMemcpyAsync(map, c->map.p, C * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    return DEME_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Dynamics
+// ---------------------------------------------------------------------------------------------------------------
+int deme_force(deme_ctx* c) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "contact detection must run before the force pass");
+    const int cu = c->cur, pv = cu ^ 1;
+    const bool remap = c->fresh;
+    DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
+    DEME_LAUNCH(c, "force",
+                launch_force(c->kp, (uint32_t)c->C, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->sph.p, c->own.p,
+                             c->anal.p, c->pose.p, c->mats.p, remap ? (c->havePrev ? c->hist[pv].p : c->hist[cu].p)
+                                                                    : c->hist[cu].p,
+                             remap ? c->map.p : nullptr, c->hist[cu].p, c->frc.p, c->stream));
+    c->fresh = false;
+    return DEME_OK;
+}
+
+static int collect_and_update(deme_ctx* c, int integrate) {
+    const int cu = c->cur;
+    uint32_t nb = 0;
+    const uint32_t needBlocks = (uint32_t)((c->C + 4095) / 4096) + 1;
+    DEME_HIP(c, c->analPartials.ensure((size_t)needBlocks * MAX_ANAL_OWNERS * 6));
+    DEME_HIP(c, c->analAcc.ensure(MAX_ANAL_OWNERS * 6, 1.0));
+    DEME_LAUNCH(c, "anal_collect",
+                launch_anal_collect((uint32_t)c->C, c->idB[cu].p, c->ctype[cu].p, c->frc.p, c->anal.p, c->own.p,
+                                    c->moi.p, c->nClumps, c->nAnalOwners, c->analPartials.p, &nb, c->stream));
+    DEME_LAUNCH(c, "anal_final", launch_anal_final(c->analPartials.p, nb, c->nAnalOwners, c->analAcc.p, c->stream));
+    DEME_LAUNCH(c, integrate ? "owner_update" : "owner_reduce",
+                launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
+                                    c->rowStart[cu].p, c->bStart.p, c->bList.p, c->frc.p, c->analAcc.p, c->moi.p,
+                                    c->rules.p, c->own.p, c->acc.p, integrate, (float)c->time, c->stream));
+    return DEME_OK;
+}
+
+int deme_reduce_owner(deme_ctx* c) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "no contact list");
+    return collect_and_update(c, 0);
+}
+
+int deme_integrate(deme_ctx* c) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "no contact list");
+    return collect_and_update(c, 1);
+}
+
+int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
+    int r = check_ready(c);
+    if (r) return r;
+    const uint64_t f = updateFreq > 0 ? (uint64_t)updateFreq : 1;
+    for (int k = 0; k < nSteps; k++) {
+        if (c->nSteps % f == 0 || !c->contactsReady) {
+            r = deme_cd_run(c);
+            if (r) return r;
+        }
+        r = deme_force(c);
+        if (r) return r;
+        r = collect_and_update(c, 1);
+        if (r) return r;
+        c->time += (double)c->sp.h;
+        c->nSteps++;
+    }
+    return DEME_OK;
+}
+
+int deme_download_forces(deme_ctx* c, float* F, float* T, float* cA, float* cB) {
+    if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "no contact list");
+    std::vector<ForceRec> h(c->C);
+    if (c->C) DEME_HIP(c, hipMemcpyAsync(h.data(), c->frc.p, c->C * sizeof(ForceRec), hipMemcpyDeviceToHost, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->C; i++) {
+        const ForceRec& f = h[i];
+        if (F) { F[3 * i] = f.fx; F[3 * i + 1] = f.fy; F[3 * i + 2] = f.fz; }
+        if (T) { T[3 * i] = f.tx; T[3 * i + 1] = f.ty; T[3 * i + 2] = f.tz; }
+        if (cA) { cA[3 * i] = f.ax; cA[3 * i + 1] = f.ay; cA[3 * i + 2] = f.az; }
+        if (cB) { cB[3 * i] = f.bx; cB[3 * i + 1] = f.by; cB[3 * i + 2] = f.bz; }
+    }
+    return DEME_OK;
+}
+
+int deme_download_wildcards(deme_ctx* c, float* w) {
+    if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "no contact list");
+    const size_t C = c->C;
+    std::vector<float4> h(C);
+    if (c->fresh) {
+        // Between a detection pass and the next force pass the history still lives in the previous order; the force
+        // kernel gathers it through the persistent-contact map. Apply the same gather here.
+        std::vector<uint32_t> m(C);
+        std::vector<float4> prev(c->havePrev ? c->Cprev : 0);
+        if (C) DEME_HIP(c, hipMemcpyAsync(m.data(), c->map.p, C * 4, hipMemcpyDeviceToHost, c->stream));
+        if (!prev.empty())
+            DEME_HIP(c, hipMemcpyAsync(prev.data(), c->hist[c->cur ^ 1].p, prev.size() * sizeof(float4),
+                                       hipMemcpyDeviceToHost, c->stream));
+        DEME_HIP(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < C; i++)
+            h[i] = (m[i] == DEME_NULL_MAPPING_PARTNER || m[i] >= prev.size()) ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                                                             : prev[m[i]];
+    } else {
+        if (C)
+            DEME_HIP(c, hipMemcpyAsync(h.data(), c->hist[c->cur].p, C * sizeof(float4), hipMemcpyDeviceToHost,
+                                       c->stream));
+        DEME_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    for (size_t i = 0; i < C; i++) {
+        w[i] = h[i].x;
+        w[C + i] = h[i].y;
+        w[2 * C + i] = h[i].z;
+        w[3 * C + i] = h[i].w;
+    }
+    return DEME_OK;
+}
+
+int deme_upload_wildcards(deme_ctx* c, const float* w) {
+    if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "no contact list");
+    const size_t C = c->C;
+    std::vector<float4> h(C);
+    for (size_t i = 0; i < C; i++) h[i] = make_float4(w[i], w[C + i], w[2 * C + i], w[3 * C + i]);
+    if (C) DEME_HIP(c, hipMemcpyAsync(c->hist[c->cur].p, h.data(), C * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->fresh = false;  // the uploaded history is already in the current order
+    return DEME_OK;
+}
+
+int deme_get_counts(deme_ctx* c, deme_counts* out) {
+    std::memset(out, 0, sizeof(*out));
+    out->nSpheres = c->nS;
+    out->nOwners = c->nO;
+    out->nAnal = c->nA;
+    out->nBinSphereTouches = c->P;
+    out->nActiveBins = c->nActive;
+    out->nContacts = c->C;
+    out->nPrevContacts = c->Cprev;
+    out->nSphereGeoContacts = c->nGeo;
+    out->maxSpheresInBin = c->maxOcc;
+    out->nSteps = c->nSteps;
+    out->time = c->time;
+    return DEME_OK;
+}
+
+int deme_set_time(deme_ctx* c, double t) {
+    c->time = t;
+    return DEME_OK;
+}
+
+int deme_enable_kernel_timing(deme_ctx* c, int on) {
+    if (!on) {
+        hipError_t e = drain_timing(c);
+        if (e != hipSuccess) return c->fail_hip(e, "timing");
+    }
+    c->timing = on != 0;
+    return DEME_OK;
+}
+
+int deme_reset_kernel_timing(deme_ctx* c) {
+    hipError_t e = drain_timing(c);
+    if (e != hipSuccess) return c->fail_hip(e, "timing");
+    std::fill(c->ktotal.begin(), c->ktotal.end(), 0.0);
+    std::fill(c->kcount.begin(), c->kcount.end(), 0);
+    return DEME_OK;
+}
+
+int deme_kernel_timing(deme_ctx* c, int max, const char** names, double* total_ms, uint64_t* launches, int* n) {
+    hipError_t e = drain_timing(c);
+    if (e != hipSuccess) return c->fail_hip(e, "timing");
+    const int k = (int)c->knames.size();
+    for (int i = 0; i < k && i < max; i++) {
+        if (names) names[i] = c->knames[i].c_str();
+        if (total_ms) total_ms[i] = c->ktotal[i];
+        if (launches) launches[i] = c->kcount[i];
+    }
+    *n = k;
+    return DEME_OK;
+}
+
+}  // extern "C"

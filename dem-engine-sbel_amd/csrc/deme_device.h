@@ -1,0 +1,174 @@
+// Device-side records and arithmetic helpers for the DEM hot path on gfx950.
+//
+// All device code is compiled with -ffp-contract=off and IEEE division/sqrt, so every double/float operation rounds
+// exactly like the reference's arithmetic (no fused multiply-adds are introduced).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/deme/deme_types.h"
+
+namespace deme_dev {
+
+// --------------------------------------------------------------------------------------------------------------
+// HBM-resident records. Owner state is kept as one 64-byte record per owner so that the force kernel's random
+// owner gathers touch a single 64-byte segment instead of a dozen SoA cache lines.
+// --------------------------------------------------------------------------------------------------------------
+struct __attribute__((aligned(16))) OwnerRec {
+    uint64_t voxel;       // voxelID_t
+    uint16_t lx, ly, lz;  // subVoxelPos_t
+    uint8_t fam;          // family_t
+    uint8_t pad0;
+    float qw, qx, qy, qz;  // oriQ
+    float vx, vy, vz;      // linear velocity
+    float wx, wy, wz;      // body-frame angular velocity (omgBar)
+    float mass;
+    float pad1;
+};
+static_assert(sizeof(OwnerRec) == 64, "OwnerRec must be 64 bytes");
+
+// Sphere component template, flattened per sphere (ClumpCompAcqStratAllFlatten.cu).
+struct __attribute__((aligned(16))) SphereRec {
+    uint32_t owner;
+    float r;
+    float rx, ry, rz;
+    uint32_t mat;
+    uint32_t pad0, pad1;
+};
+static_assert(sizeof(SphereRec) == 32, "SphereRec must be 32 bytes");
+
+// Per-sphere contact-detection record written once per detection pass: world position (relative to the world's
+// left-bottom-front corner) in double, the CD radius r + beta in float, and the sphere id.
+struct __attribute__((aligned(16))) CdRec {
+    double x, y, z;
+    float r;
+    uint32_t id;
+};
+static_assert(sizeof(CdRec) == 32, "CdRec must be 32 bytes");
+
+// Per-contact outputs of the force kernel (contactForces, contactTorque_convToForce, contactPointGeometryA/B).
+struct __attribute__((aligned(16))) ForceRec {
+    float fx, fy, fz;
+    float tx, ty, tz;
+    float ax, ay, az;
+    float bx, by, bz;
+};
+static_assert(sizeof(ForceRec) == 48, "ForceRec must be 48 bytes");
+
+// Kernel-visible simulation constants.
+struct KParams {
+    double l, voxelSize, binSize;
+    uint32_t nbX, nbY, nbZ;
+    uint32_t nBins;  // nbX * nbY * nbZ (< 2^32, checked on the host)
+    uint32_t nvX, nvY;
+    float beta, h;
+    float Gx, Gy, Gz;
+    uint32_t integrator;
+};
+
+// --------------------------------------------------------------------------------------------------------------
+// Saturating float->integer conversions (the reference's device conversions saturate; NaN -> 0).
+// --------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t sat_u32(double x) {
+    if (!(x > 0.0)) return 0u;
+    if (x >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+__device__ __forceinline__ uint16_t sat_u16(double x) {
+    if (!(x > 0.0)) return 0;
+    if (x >= 65535.0) return 65535;
+    return (uint16_t)(uint32_t)x;
+}
+__device__ __forceinline__ uint64_t sat_u64(double x) {
+    if (!(x > 0.0)) return 0ull;
+    if (x >= 18446744073709551616.0) return ~0ull;
+    return (uint64_t)x;
+}
+
+// voxelIDToPosition<double, voxelID_t, subVoxelPos_t> (DEMHelperKernels.cu:93-110)
+__device__ __forceinline__ void owner_position(const KParams& P, uint64_t id, uint16_t lx, uint16_t ly, uint16_t lz,
+                                               double& X, double& Y, double& Z) {
+    const uint64_t vx = id & (((uint64_t)1 << P.nvX) - 1);
+    const uint64_t vy = (id >> P.nvX) & (((uint64_t)1 << P.nvY) - 1);
+    const uint64_t vz = id >> (P.nvX + P.nvY);
+    X = (double)vx * P.voxelSize + (double)lx * P.l;
+    Y = (double)vy * P.voxelSize + (double)ly * P.l;
+    Z = (double)vz * P.voxelSize + (double)lz * P.l;
+}
+
+// positionToVoxelID<voxelID_t, subVoxelPos_t, double> (DEMHelperKernels.cu:113-135)
+__device__ __forceinline__ void position_to_voxel(const KParams& P, double X, double Y, double Z, uint64_t& id,
+                                                  uint16_t& lx, uint16_t& ly, uint16_t& lz) {
+    const uint64_t nx = sat_u64(X / P.voxelSize), ny = sat_u64(Y / P.voxelSize), nz = sat_u64(Z / P.voxelSize);
+    lx = sat_u16((X - (double)nx * P.voxelSize) / P.l);
+    ly = sat_u16((Y - (double)ny * P.voxelSize) / P.l);
+    lz = sat_u16((Z - (double)nz * P.voxelSize) / P.l);
+    id = nx + (ny << P.nvX) + (nz << (P.nvX + P.nvY));
+}
+
+// applyOriQToVector3<float, float> (DEMHelperKernels.cu:137-148)
+__device__ __forceinline__ void rotate(float& X, float& Y, float& Z, float Qw, float Qx, float Qy, float Qz) {
+    const float oX = X, oY = Y, oZ = Z;
+    X = (2.0f * (Qw * Qw + Qx * Qx) - 1.0f) * oX + (2.0f * (Qx * Qy - Qw * Qz)) * oY + (2.0f * (Qx * Qz + Qw * Qy)) * oZ;
+    Y = (2.0f * (Qx * Qy + Qw * Qz)) * oX + (2.0f * (Qw * Qw + Qy * Qy) - 1.0f) * oY + (2.0f * (Qy * Qz - Qw * Qx)) * oZ;
+    Z = (2.0f * (Qx * Qz - Qw * Qy)) * oX + (2.0f * (Qy * Qz + Qw * Qx)) * oY + (2.0f * (Qw * Qw + Qz * Qz) - 1.0f) * oZ;
+}
+
+// locateMaskPair (DEMHelperKernels.cu:51-56)
+__device__ __forceinline__ uint32_t mask_index(uint32_t i, uint32_t j) {
+    if (i > j) {
+        const uint32_t t = i;
+        i = j;
+        j = t;
+    }
+    return (1 + j) * j / 2 + i;
+}
+
+// Sphere world position as every reference kernel derives it: owner position in double plus the float-rotated
+// component offset (DEMBinSphereKernels.cu:39-51).
+__device__ __forceinline__ void sphere_position(const KParams& P, const OwnerRec& o, const SphereRec& s, double& X,
+                                                double& Y, double& Z) {
+    double ox, oy, oz;
+    owner_position(P, o.voxel, o.lx, o.ly, o.lz, ox, oy, oz);
+    float rx = s.rx, ry = s.ry, rz = s.rz;
+    rotate(rx, ry, rz, o.qw, o.qx, o.qy, o.qz);
+    X = ox + (double)rx;
+    Y = oy + (double)ry;
+    Z = oz + (double)rz;
+}
+
+__device__ __forceinline__ OwnerRec load_owner(const OwnerRec* __restrict__ p, uint32_t i) {
+    // four 16-byte loads
+    const uint4* q = reinterpret_cast<const uint4*>(p + i);
+    uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+    OwnerRec r;
+    uint4* w = reinterpret_cast<uint4*>(&r);
+    w[0] = a;
+    w[1] = b;
+    w[2] = c;
+    w[3] = d;
+    return r;
+}
+
+__device__ __forceinline__ SphereRec load_sphere(const SphereRec* __restrict__ p, uint32_t i) {
+    const uint4* q = reinterpret_cast<const uint4*>(p + i);
+    uint4 a = q[0], b = q[1];
+    SphereRec r;
+    uint4* w = reinterpret_cast<uint4*>(&r);
+    w[0] = a;
+    w[1] = b;
+    return r;
+}
+
+__device__ __forceinline__ CdRec load_cd(const CdRec* __restrict__ p, uint32_t i) {
+    const uint4* q = reinterpret_cast<const uint4*>(p + i);
+    uint4 a = q[0], b = q[1];
+    CdRec r;
+    uint4* w = reinterpret_cast<uint4*>(&r);
+    w[0] = a;
+    w[1] = b;
+    return r;
+}
+
+}  // namespace deme_dev

@@ -1,0 +1,100 @@
+// Internal declarations shared by the HIP translation units of libdeme (not part of the public ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "deme_device.h"
+
+namespace deme_int {
+
+using deme_dev::CdRec;
+using deme_dev::ForceRec;
+using deme_dev::KParams;
+using deme_dev::OwnerRec;
+using deme_dev::SphereRec;
+
+// World pose of one analytical component, recomputed whenever owner states change.
+struct __attribute__((aligned(16))) PlanePose {
+    double px, py, pz;
+    float nx, ny, nz;
+    uint32_t owner;
+    uint32_t fam;
+    uint32_t type;
+};
+static_assert(sizeof(PlanePose) == 48, "PlanePose must be 48 bytes");
+
+// Device-side counters read back by the host after a detection pass.
+struct DevCounters {
+    uint32_t pairCount;
+    uint32_t err;
+    uint32_t maxOcc;
+    uint32_t pad;
+    unsigned long long nActive;
+    unsigned long long total[4];
+};
+
+enum : uint32_t { ERRBIT_CAPACITY = 1u, ERRBIT_PAIR_OVERFLOW = 2u, ERRBIT_OUTSIDE = 4u };
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr uint64_t SCAN_TILE = (uint64_t)SCAN_BLOCK * SCAN_ITEMS;
+constexpr uint32_t MAX_ANAL_OWNERS = 8;
+
+// scan.hip: out[0..n] = exclusive prefix sum of (a[i] + (b ? b[i] : 0)); out[n] = total; *totalDev = total (u64).
+// scratch must hold scan_scratch_elems(n) u64 values.
+uint64_t scan_scratch_elems(uint64_t n);
+hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
+                    uint64_t* scratch, hipStream_t st);
+
+// cd.hip
+hipError_t launch_plane_pose(const KParams& P, const deme_anal_obj* anal, uint32_t nA, const OwnerRec* own,
+                             PlanePose* pose, hipStream_t st);
+hipError_t launch_sphere_bins(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                              const PlanePose* pose, uint32_t nA, const uint8_t* mask, uint32_t* binCount,
+                              uint16_t* numBins, uint32_t* numAnal, CdRec* rec, DevCounters* ctr, hipStream_t st);
+hipError_t launch_bin_stats(const uint32_t* binCount, uint64_t nBins, DevCounters* ctr, hipStream_t st);
+hipError_t launch_bin_scatter(const KParams& P, const CdRec* rec, const SphereRec* sph, uint32_t nS,
+                              const uint32_t* binStart, uint32_t* binCount, uint2* binEnt, const DevCounters* ctr,
+                              hipStream_t st);
+hipError_t launch_bin_pairs(const KParams& P, const uint2* binEnt, uint64_t nEnt, const uint32_t* binStart,
+                            const CdRec* rec, const SphereRec* sph, const OwnerRec* own, const uint8_t* mask,
+                            uint2* pairs, uint32_t pairCap, DevCounters* ctr, uint32_t* rowCount, uint32_t* bCount,
+                            hipStream_t st);
+hipError_t launch_row_fill(const uint2* pairs, uint32_t nPairs, const uint32_t* rowStart, const uint32_t* numAnal,
+                           uint32_t* rowCount, uint32_t* idB, uint8_t* ctype, hipStream_t st);
+hipError_t launch_geo_fill(const KParams& P, const CdRec* rec, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                           const PlanePose* pose, uint32_t nA, const uint8_t* mask, const uint32_t* numAnal,
+                           const uint32_t* rowStart, uint32_t* idB, uint8_t* ctype, hipStream_t st);
+hipError_t launch_row_finalize(uint32_t nS, const uint32_t* rowStart, const uint32_t* numAnal, uint32_t* idA,
+                               uint32_t* idB, hipStream_t st);
+hipError_t launch_history_map(uint32_t nS, const uint32_t* rowStart, const uint32_t* idB, const uint8_t* ctype,
+                              const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
+                              uint32_t* map, hipStream_t st);
+hipError_t launch_bside_fill(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const SphereRec* sph,
+                             const uint32_t* bStart, uint32_t* bCount, uint32_t* bList, hipStream_t st);
+hipError_t launch_bside_sort(uint32_t nClumps, const uint32_t* bStart, uint32_t* bList, hipStream_t st);
+
+// dyn.hip
+hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, const uint32_t* idB, const uint8_t* ctype,
+                        const SphereRec* sph, const OwnerRec* own, const deme_anal_obj* anal, const PlanePose* pose,
+                        const deme_material* mats, const float4* histIn, const uint32_t* map, float4* histOut,
+                        ForceRec* out, hipStream_t st);
+hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const ForceRec* frc,
+                               const deme_anal_obj* anal, const OwnerRec* own, const float4* moi,
+                               uint32_t firstAnalOwner, uint32_t nAnalOwners, float* blockPartials,
+                               uint32_t* nBlocksOut, hipStream_t st);
+hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint32_t nAnalOwners, float* analAcc,
+                             hipStream_t st);
+hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnalOwner,
+                               uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
+                               const uint32_t* bStart, const uint32_t* bList, const ForceRec* frc,
+                               const float* analAcc, const float4* moi, const deme_family_rule* rules, OwnerRec* own,
+                               float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate, float t,
+                               hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+
+}  // namespace deme_int

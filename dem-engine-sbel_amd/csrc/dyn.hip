@@ -1,0 +1,529 @@
+// Dynamics for gfx950: frictional Hertz-Mindlin contact force with tangential history, owner-keyed force/torque
+// reduction without global atomics, and clump integration.
+//
+// Reference: src/DEM/dT.cpp:1349-1432 and the kernels it launches (DEMPrepForceKernels.cu, DEMCalcForceKernels.cu +
+// DEMCustomizablePolicies/FullHertzianForceModel.cu, DEMCollectForceKernels.cu + DEMCubForceCollection.cu,
+// DEMIntegrationKernels.cu + IntegrationVelPassOn*.cu).
+//
+// Every float/double promotion of the reference's expressions is kept (e.g. `2. * E_cnt * sqrt_Rd` is evaluated in
+// double and rounded to float), and the device code is compiled without contraction.
+#include "deme_internal.h"
+
+namespace deme_int {
+using namespace deme_dev;
+namespace {
+
+constexpr double TINY_FLOAT = 1e-12;
+constexpr double HUGE_FLOAT = 1e15;
+constexpr double TWO_OVER_THREE = 0.666666666666667;
+constexpr double FOUR_OVER_THREE = 1.333333333333333;
+constexpr double FIVE_OVER_THREE = 1.666666666666667;
+constexpr double TWO_TIMES_SQRT_FIVE_OVER_SIX = 1.825741858350554;
+constexpr double PI = 3.1415926535897932385;
+constexpr double PI_SQUARED = 9.869604401089358;
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float b) { return mk(a.x * b, a.y * b, a.z * b); }
+__device__ __forceinline__ f3 operator*(float b, f3 a) { return mk(b * a.x, b * a.y, b * a.z); }
+__device__ __forceinline__ f3 operator/(f3 a, float b) { return mk(a.x / b, a.y / b, a.z / b); }
+__device__ __forceinline__ f3 operator/(f3 a, f3 b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
+__device__ __forceinline__ void operator+=(f3& a, f3 b) {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+}
+__device__ __forceinline__ void operator-=(f3& a, f3 b) {
+    a.x -= b.x;
+    a.y -= b.y;
+    a.z -= b.z;
+}
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length(f3 v) { return sqrtf(dot(v, v)); }
+__device__ __forceinline__ void rotq(f3& v, float w, float x, float y, float z) { rotate(v.x, v.y, v.z, w, x, y, z); }
+
+// matProxy2ContactParam<float> (DEMHelperKernels.cu:316-339)
+__device__ __forceinline__ void mat_proxy(const deme_material& A, const deme_material& B, float& E, float& G,
+                                          float& CoR, float& mu, float& Crr) {
+    const float invE = (1. - A.nu * A.nu) / A.E + (1. - B.nu * B.nu) / B.E;
+    E = 1. / invE;
+    const float invG = 2. * (2. - A.nu) * (1. + A.nu) / A.E + 2. * (2. - B.nu) * (1. + B.nu) / B.E;
+    G = 1. / invG;
+    CoR = fminf(A.CoR, B.CoR);
+    mu = fmaxf(A.mu, B.mu);
+    Crr = fmaxf(A.Crr, B.Crr);
+}
+
+// Rows a14-a15: calculateContactForces (DEMCalcForceKernels.cu:14-179) with FullHertzianForceModel.cu.
+// History (delta_tan_x, delta_tan_y, delta_tan_z, delta_time) is one float4 per contact. On the first step after a
+// detection pass the previous history is gathered through the persistent-contact map (rearrangeContactWildcards,
+// DEMPrepForceKernels.cu:33-55) inside this kernel instead of in a separate pass.
+__global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uint32_t* __restrict__ idA,
+                                               const uint32_t* __restrict__ idB, const uint8_t* __restrict__ ctype,
+                                               const SphereRec* __restrict__ sph, const OwnerRec* __restrict__ own,
+                                               const deme_anal_obj* __restrict__ anal,
+                                               const PlanePose* __restrict__ pose,
+                                               const deme_material* __restrict__ mats, const float4* histIn,
+                                               const uint32_t* __restrict__ map, float4* histOut,
+                                               ForceRec* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nC) return;
+    uint32_t type = ctype[i];
+    const float ts = P.h;
+
+    // ---- body A (always a sphere) ----
+    const SphereRec sa = load_sphere(sph, idA[i]);
+    const OwnerRec oa = load_owner(own, sa.owner);
+    double aox, aoy, aoz;
+    owner_position(P, oa.voxel, oa.lx, oa.ly, oa.lz, aox, aoy, aoz);
+    float rx = sa.rx, ry = sa.ry, rz = sa.rz;
+    rotate(rx, ry, rz, oa.qw, oa.qx, oa.qy, oa.qz);
+    const double XA = aox + (double)rx, YA = aoy + (double)ry, ZA = aoz + (double)rz;
+    const float ARadius = sa.r, AMass = oa.mass;
+    const uint32_t matA = sa.mat;
+
+    // ---- body B ----
+    double box, boy, boz;
+    float BRadius, BMass;
+    uint32_t matB;
+    float Bqw, Bqx, Bqy, Bqz;
+    f3 BLinVel, BRotVel;
+    double cpx = 0, cpy = 0, cpz = 0, overlapDepth = 0;
+    f3 B2A = mk(0, 0, 0);
+    const uint32_t bId = idB[i];
+    if (type == DEME_SPHERE_SPHERE_CONTACT) {
+        const SphereRec sb = load_sphere(sph, bId);
+        const OwnerRec ob = load_owner(own, sb.owner);
+        owner_position(P, ob.voxel, ob.lx, ob.ly, ob.lz, box, boy, boz);
+        float qx = sb.rx, qy = sb.ry, qz = sb.rz;
+        rotate(qx, qy, qz, ob.qw, ob.qx, ob.qy, ob.qz);
+        const double XB = box + (double)qx, YB = boy + (double)qy, ZB = boz + (double)qz;
+        BRadius = sb.r;
+        BMass = ob.mass;
+        matB = sb.mat;
+        Bqw = ob.qw;
+        Bqx = ob.qx;
+        Bqy = ob.qy;
+        Bqz = ob.qz;
+        BLinVel = mk(ob.vx, ob.vy, ob.vz);
+        BRotVel = mk(ob.wx, ob.wy, ob.wz);
+        // checkSpheresOverlap<double, float> (DEMHelperKernels.cu:233-264)
+        const double radA = ARadius, radB = BRadius;
+        const double d2 = (XA - XB) * (XA - XB) + (YA - YB) * (YA - YB) + (ZA - ZB) * (ZA - ZB);
+        if (d2 > (radA + radB) * (radA + radB)) {
+            type = DEME_NOT_A_CONTACT;
+        } else {
+            B2A.x = (float)(XA - XB);
+            B2A.y = (float)(YA - YB);
+            B2A.z = (float)(ZA - ZB);
+            const float mag = sqrtf(B2A.x * B2A.x + B2A.y * B2A.y + B2A.z * B2A.z);
+            B2A.x /= mag;
+            B2A.y /= mag;
+            B2A.z /= mag;
+            overlapDepth = radA + radB - sqrt(d2);
+            cpx = XB + (radB - overlapDepth / 2.0) * (double)B2A.x;
+            cpy = YB + (radB - overlapDepth / 2.0) * (double)B2A.y;
+            cpz = ZB + (radB - overlapDepth / 2.0) * (double)B2A.z;
+        }
+    } else {
+        // analytical component (DEMCalcForceKernels.cu:112-151); pose precomputed with the same arithmetic
+        const deme_anal_obj ab = anal[bId];
+        const PlanePose pp = pose[bId];
+        const OwnerRec ob = load_owner(own, ab.owner);
+        owner_position(P, ob.voxel, ob.lx, ob.ly, ob.lz, box, boy, boz);
+        BRadius = (float)HUGE_FLOAT;
+        BMass = ab.mass;
+        matB = ab.material;
+        Bqw = ob.qw;
+        Bqx = ob.qx;
+        Bqy = ob.qy;
+        Bqz = ob.qz;
+        BLinVel = mk(ob.vx, ob.vy, ob.vz);
+        BRotVel = mk(ob.wx, ob.wy, ob.wz);
+        if (ab.type == DEME_ANAL_OBJ_TYPE_PLANE) {
+            // checkSphereEntityOverlap<double, float> (DEMHelperKernels.cu:394-445), beta4Entity = 0
+            const double radA = ARadius;
+            const double dist = (XA - pp.px) * (double)pp.nx + (YA - pp.py) * (double)pp.ny + (ZA - pp.pz) * (double)pp.nz;
+            overlapDepth = (radA + (double)0.0f - dist) / 2.0;
+            if (overlapDepth < 0.0) {
+                type = DEME_NOT_A_CONTACT;
+            } else {
+                cpx = XA - (double)pp.nx * (dist + overlapDepth);
+                cpy = YA - (double)pp.ny * (dist + overlapDepth);
+                cpz = ZA - (double)pp.nz * (dist + overlapDepth);
+                B2A = mk(pp.nx, pp.ny, pp.nz);
+            }
+        } else {
+            type = DEME_NOT_A_CONTACT;
+        }
+    }
+
+    // ---- contact history ----
+    float4 w;
+    if (map) {
+        const uint32_t m = map[i];
+        w = (m == DEME_NULL_MAPPING_PARTNER) ? make_float4(0.f, 0.f, 0.f, 0.f) : histIn[m];
+    } else {
+        w = histIn[i];
+    }
+    float delta_tan_x = w.x, delta_tan_y = w.y, delta_tan_z = w.z, delta_time = w.w;
+
+    f3 force = mk(0, 0, 0), tof = mk(0, 0, 0);
+    ForceRec fr;
+    if (type != DEME_NOT_A_CONTACT) {
+        f3 locCPA = mk((float)(cpx - aox), (float)(cpy - aoy), (float)(cpz - aoz));
+        f3 locCPB = mk((float)(cpx - box), (float)(cpy - boy), (float)(cpz - boz));
+        rotq(locCPA, oa.qw, -oa.qx, -oa.qy, -oa.qz);
+        rotq(locCPB, Bqw, -Bqx, -Bqy, -Bqz);
+        const f3 ALinVel = mk(oa.vx, oa.vy, oa.vz), ARotVel = mk(oa.wx, oa.wy, oa.wz);
+        // ---- FullHertzianForceModel.cu ----
+        float E_cnt, G_cnt, CoR_cnt, mu_cnt, Crr_cnt;
+        mat_proxy(mats[matA], mats[matB], E_cnt, G_cnt, CoR_cnt, mu_cnt, Crr_cnt);
+        f3 rotVelCPA = cross(ARotVel, locCPA);
+        f3 rotVelCPB = cross(BRotVel, locCPB);
+        rotq(rotVelCPA, oa.qw, oa.qx, oa.qy, oa.qz);
+        rotq(rotVelCPB, Bqw, Bqx, Bqy, Bqz);
+        float mass_eff, sqrt_Rd, beta;
+        f3 vrel_tan;
+        f3 delta_tan = mk(delta_tan_x, delta_tan_y, delta_tan_z);
+        {
+            const f3 velB2A = (ALinVel + rotVelCPA) - (BLinVel + rotVelCPB);
+            const float projection = dot(velB2A, B2A);
+            vrel_tan = velB2A - projection * B2A;
+            {
+                delta_tan += ts * vrel_tan;
+                const float disp_proj = dot(delta_tan, B2A);
+                delta_tan -= disp_proj * B2A;
+                delta_time += ts;
+            }
+            mass_eff = (AMass * BMass) / (AMass + BMass);
+            sqrt_Rd = sqrt(overlapDepth * (ARadius * BRadius) / (ARadius + BRadius));
+            const float Sn = 2. * E_cnt * sqrt_Rd;
+            const float loge = (CoR_cnt < TINY_FLOAT) ? (double)(float)log(TINY_FLOAT) : (double)logf(CoR_cnt);
+            beta = loge / sqrt(loge * loge + PI_SQUARED);
+            const float k_n = TWO_OVER_THREE * Sn;
+            const float gamma_n = TWO_TIMES_SQRT_FIVE_OVER_SIX * beta * sqrtf(Sn * mass_eff);
+            force += (float)(k_n * overlapDepth + gamma_n * projection) * B2A;
+        }
+        if (Crr_cnt > 0.0) {
+            bool should_add_rolling_resistance = true;
+            {
+                const float R_eff = sqrtf((ARadius * BRadius) / (ARadius + BRadius));
+                const float kn_simple = FOUR_OVER_THREE * E_cnt * sqrtf(R_eff);
+                const float gn_simple =
+                    -2.f * sqrtf((float)(FIVE_OVER_THREE * mass_eff * E_cnt)) * beta * powf(R_eff, 0.25f);
+                const float d_coeff = gn_simple / (2.f * sqrtf(kn_simple * mass_eff));
+                if (d_coeff < 1.0) {
+                    const float t_collision = PI * sqrtf(mass_eff / (kn_simple * (1.f - d_coeff * d_coeff)));
+                    if (delta_time <= t_collision) should_add_rolling_resistance = false;
+                }
+            }
+            if (should_add_rolling_resistance) {
+                const f3 v_rot = rotVelCPB - rotVelCPA;
+                const float v_rot_mag = length(v_rot);
+                if (v_rot_mag > TINY_FLOAT) tof = (v_rot / v_rot_mag) * (Crr_cnt * length(force));
+            }
+        }
+        if (mu_cnt > 0.0) {
+            const float kt = 8. * G_cnt * sqrt_Rd;
+            const float gt = -TWO_TIMES_SQRT_FIVE_OVER_SIX * beta * sqrtf(mass_eff * kt);
+            f3 tangent_force = (-kt) * delta_tan - gt * vrel_tan;
+            const float ft = length(tangent_force);
+            if (ft > TINY_FLOAT) {
+                const float ft_max = length(force) * mu_cnt;
+                if (ft > ft_max) {
+                    tangent_force = (ft_max / ft) * tangent_force;
+                    delta_tan = (tangent_force + gt * vrel_tan) / (-kt);
+                }
+            } else {
+                tangent_force = mk(0, 0, 0);
+            }
+            force += tangent_force;
+        }
+        delta_tan_x = delta_tan.x;
+        delta_tan_y = delta_tan.y;
+        delta_tan_z = delta_tan.z;
+        fr.ax = locCPA.x;
+        fr.ay = locCPA.y;
+        fr.az = locCPA.z;
+        fr.bx = locCPB.x;
+        fr.by = locCPB.y;
+        fr.bz = locCPB.z;
+    } else {
+        delta_tan_x = delta_tan_y = delta_tan_z = delta_time = 0.f;
+        fr.ax = fr.ay = fr.az = fr.bx = fr.by = fr.bz = 0.f;
+    }
+    fr.fx = force.x;
+    fr.fy = force.y;
+    fr.fz = force.z;
+    fr.tx = tof.x;
+    fr.ty = tof.y;
+    fr.tz = tof.z;
+    float4* o4 = reinterpret_cast<float4*>(out + i);
+    const float4* f4 = reinterpret_cast<const float4*>(&fr);
+    o4[0] = f4[0];
+    o4[1] = f4[1];
+    o4[2] = f4[2];
+    histOut[i] = make_float4(delta_tan_x, delta_tan_y, delta_tan_z, delta_time);
+}
+
+// forceToAngAcc (DEMCollectForceKernels.cu:87-120): cross(cp, R^T((F + Ftof) * modifier)) / MOI
+__device__ __forceinline__ f3 ang_term(const OwnerRec& o, const float4& I, f3 cp, f3 F, f3 T, float mod) {
+    f3 myF = (F + T) * mod;
+    rotq(myF, o.qw, -o.qx, -o.qy, -o.qz);
+    return cross(cp, myF) / mk(I.x, I.y, I.z);
+}
+
+// Row a16 for analytical owners (planes): every sphere-plane contact contributes its B-side term to the plane's
+// owner. Deterministic two-level reduction: each lane sums 16 consecutive contacts in order, lanes combine in a fixed
+// tree, tiles combine sequentially in k_anal_final.
+constexpr uint32_t ANAL_ITEMS = 16;
+constexpr uint32_t ANAL_TILE = 256 * ANAL_ITEMS;
+
+__global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_t* __restrict__ idB,
+                                                      const uint8_t* __restrict__ ctype,
+                                                      const ForceRec* __restrict__ frc,
+                                                      const deme_anal_obj* __restrict__ anal,
+                                                      const OwnerRec* __restrict__ own, const float4* __restrict__ moi,
+                                                      uint32_t firstAnal, uint32_t nAnalOwners,
+                                                      float* __restrict__ partials) {
+    __shared__ float red[256][6];
+    float acc[MAX_ANAL_OWNERS][6];
+#pragma unroll
+    for (uint32_t k = 0; k < MAX_ANAL_OWNERS; k++)
+#pragma unroll
+        for (int c = 0; c < 6; c++) acc[k][c] = 0.f;
+    const uint64_t c0 = (uint64_t)blockIdx.x * ANAL_TILE + (uint64_t)threadIdx.x * ANAL_ITEMS;
+    for (uint32_t it = 0; it < ANAL_ITEMS; it++) {
+        const uint64_t c = c0 + it;
+        if (c >= nC) break;
+        if (ctype[c] != DEME_SPHERE_PLANE_CONTACT) continue;
+        const uint32_t o = anal[idB[c]].owner;
+        const uint32_t k = o - firstAnal;
+        const OwnerRec orc = load_owner(own, o);
+        const ForceRec f = frc[c];
+        const f3 F = mk(f.fx, f.fy, f.fz), T = mk(f.tx, f.ty, f.tz), cp = mk(f.bx, f.by, f.bz);
+        const f3 la = F * -1.f / orc.mass;
+        const f3 lg = ang_term(orc, moi[o], cp, F, T, -1.f);
+#pragma unroll
+        for (uint32_t kk = 0; kk < MAX_ANAL_OWNERS; kk++) {
+            if (kk == k) {
+                acc[kk][0] += la.x;
+                acc[kk][1] += la.y;
+                acc[kk][2] += la.z;
+                acc[kk][3] += lg.x;
+                acc[kk][4] += lg.y;
+                acc[kk][5] += lg.z;
+            }
+        }
+    }
+    for (uint32_t k = 0; k < nAnalOwners; k++) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) red[threadIdx.x][c] = acc[k][c];
+        __syncthreads();
+        for (uint32_t s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s)
+#pragma unroll
+                for (int c = 0; c < 6; c++) red[threadIdx.x][c] += red[threadIdx.x + s][c];
+            __syncthreads();
+        }
+        if (threadIdx.x < 6) partials[((uint64_t)blockIdx.x * nAnalOwners + k) * 6 + threadIdx.x] = red[0][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+__global__ void k_anal_final(const float* __restrict__ partials, uint32_t nBlocks, uint32_t nAnalOwners,
+                             float* __restrict__ analAcc) {
+    const uint32_t t = threadIdx.x;
+    if (t >= nAnalOwners * 6) return;
+    float s = 0.f;
+    for (uint32_t b = 0; b < nBlocks; b++) s += partials[(uint64_t)b * nAnalOwners * 6 + t];
+    analAcc[t] = s;
+}
+
+// Rows a16 (clump owners) + a17: owner reduction fused with integrateOwners (DEMIntegrationKernels.cu:170-179).
+// A clump's A-side contacts are the contiguous rows of its spheres; its B-side contacts are listed in bList in
+// ascending contact order. Terms are summed sequentially in that order (A side, then B side) in float.
+__global__ __launch_bounds__(256) void k_owner_update(
+    KParams P, uint32_t nO, uint32_t nClumps, uint32_t firstAnal, uint32_t nAnalOwners,
+    const uint32_t* __restrict__ ownerSphStart, const uint32_t* __restrict__ rowStart,
+    const uint32_t* __restrict__ bStart, const uint32_t* __restrict__ bList, const ForceRec* __restrict__ frc,
+    const float* __restrict__ analAcc, const float4* __restrict__ moi, const deme_family_rule* __restrict__ rules,
+    OwnerRec* __restrict__ own, float* __restrict__ accOut, int doIntegrate, float t) {
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nO) return;
+    OwnerRec r = load_owner(own, o);
+    f3 a = mk(0, 0, 0), al = mk(0, 0, 0);
+    if (o < nClumps) {
+        const float4 I = moi[o];
+        const float m = r.mass;
+        const uint32_t c0 = rowStart[ownerSphStart[o]], c1 = rowStart[ownerSphStart[o + 1]];
+        for (uint32_t c = c0; c < c1; c++) {
+            const ForceRec f = frc[c];
+            const f3 F = mk(f.fx, f.fy, f.fz);
+            a += F * 1.f / m;
+            al += ang_term(r, I, mk(f.ax, f.ay, f.az), F, mk(f.tx, f.ty, f.tz), 1.f);
+        }
+        const uint32_t b0 = bStart[o], b1 = bStart[o + 1];
+        for (uint32_t k = b0; k < b1; k++) {
+            const ForceRec f = frc[bList[k]];
+            const f3 F = mk(f.fx, f.fy, f.fz);
+            a += F * -1.f / m;
+            al += ang_term(r, I, mk(f.bx, f.by, f.bz), F, mk(f.tx, f.ty, f.tz), -1.f);
+        }
+        a = mk(0.f + a.x, 0.f + a.y, 0.f + a.z);
+        al = mk(0.f + al.x, 0.f + al.y, 0.f + al.z);
+    } else if (o - firstAnal < nAnalOwners) {
+        const float* s = analAcc + (uint64_t)(o - firstAnal) * 6;
+        a = mk(s[0], s[1], s[2]);
+        al = mk(s[3], s[4], s[5]);
+    }
+    accOut[o] = a.x;
+    accOut[(uint64_t)nO + o] = a.y;
+    accOut[2ull * nO + o] = a.z;
+    accOut[3ull * nO + o] = al.x;
+    accOut[4ull * nO + o] = al.y;
+    accOut[5ull * nO + o] = al.z;
+    if (!doIntegrate) return;
+
+    // ---- integrateVel (DEMIntegrationKernels.cu:179-221) ----
+    const deme_family_rule R = rules[r.fam];
+    bool LinPrescribed = false, RotPrescribed = false;
+    const f3 old_v = mk(r.vx, r.vy, r.vz), old_w = mk(r.wx, r.wy, r.wz);
+    if (R.used) {
+        if (R.velSet & 1) r.vx = R.vel[0];
+        if (R.velSet & 2) r.vy = R.vel[1];
+        if (R.velSet & 4) r.vz = R.vel[2];
+        if (R.velSet & 8) r.wx = R.vel[3];
+        if (R.velSet & 16) r.wy = R.vel[4];
+        if (R.velSet & 32) r.wz = R.vel[5];
+        LinPrescribed = R.linVelPrescribed;
+        RotPrescribed = R.rotVelPrescribed;
+    }
+    f3 v_update = mk(0, 0, 0), w_update = mk(0, 0, 0);
+    if (!LinPrescribed) {
+        v_update.x = (a.x + P.Gx) * P.h;
+        v_update.y = (a.y + P.Gy) * P.h;
+        v_update.z = (a.z + P.Gz) * P.h;
+        r.vx += v_update.x;
+        r.vy += v_update.y;
+        r.vz += v_update.z;
+    }
+    if (!RotPrescribed) {
+        w_update.x = al.x * P.h;
+        w_update.y = al.y * P.h;
+        w_update.z = al.z * P.h;
+        r.wx += w_update.x;
+        r.wy += w_update.y;
+        r.wz += w_update.z;
+    }
+    f3 v, w;
+    if (P.integrator == DEME_INTEGRATOR_FORWARD_EULER) {
+        v = old_v;
+        w = old_w;
+    } else if (P.integrator == DEME_INTEGRATOR_CENTERED_DIFFERENCE) {
+        v = old_v + v_update;
+        w = old_w + w_update;
+    } else {
+        v = old_v + v_update * 0.5f;
+        w = old_w + w_update * 0.5f;
+    }
+    // ---- integratePos (DEMIntegrationKernels.cu:241-301) ----
+    double X, Y, Z;
+    owner_position(P, r.voxel, r.lx, r.ly, r.lz, X, Y, Z);
+    LinPrescribed = false;
+    RotPrescribed = false;
+    if (R.used) {
+        if (R.posSet & 1) X = R.pos[0];
+        if (R.posSet & 2) Y = R.pos[1];
+        if (R.posSet & 4) Z = R.pos[2];
+        LinPrescribed = R.linPosPrescribed;
+        RotPrescribed = R.rotPosPrescribed;
+    }
+    if (!LinPrescribed) {
+        X += (double)v.x * P.h;
+        Y += (double)v.y * P.h;
+        Z += (double)v.z * P.h;
+    }
+    position_to_voxel(P, X, Y, Z, r.voxel, r.lx, r.ly, r.lz);
+    if (!RotPrescribed) {
+        double dq0 = 1, dq1 = w.x, dq2 = w.y, dq3 = w.z;
+        const double len = sqrt(dq1 * dq1 + dq2 * dq2 + dq3 * dq3);
+        const double theta = 0.5 * P.h * len;
+        if (len > 0) {
+            dq0 = cos(theta);
+            const double s = sin(theta) / len;
+            dq1 *= s;
+            dq2 *= s;
+            dq3 *= s;
+        }
+        // HamiltonProduct<float> (DEMHelperKernels.cu:169-186)
+        const float a1 = r.qw, b1 = r.qx, c1 = r.qy, d1 = r.qz;
+        const float a2 = (float)dq0, b2 = (float)dq1, c2 = (float)dq2, d2 = (float)dq3;
+        r.qw = a1 * a2 - b1 * b2 - c1 * c2 - d1 * d2;
+        r.qx = a1 * b2 + b1 * a2 + c1 * d2 - d1 * c2;
+        r.qy = a1 * c2 - b1 * d2 + c1 * a2 + d1 * b2;
+        r.qz = a1 * d2 + b1 * c2 - c1 * b2 + d1 * a2;
+    }
+    (void)t;  // numeric prescriptions do not depend on time
+    uint4* dst = reinterpret_cast<uint4*>(own + o);
+    const uint4* src = reinterpret_cast<const uint4*>(&r);
+    dst[0] = src[0];
+    dst[1] = src[1];
+    dst[2] = src[2];
+    dst[3] = src[3];
+}
+
+inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, const uint32_t* idB, const uint8_t* ctype,
+                        const SphereRec* sph, const OwnerRec* own, const deme_anal_obj* anal, const PlanePose* pose,
+                        const deme_material* mats, const float4* histIn, const uint32_t* map, float4* histOut,
+                        ForceRec* out, hipStream_t st) {
+    if (nC == 0) return hipSuccess;
+    k_force<<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, anal, pose, mats, histIn, map, histOut,
+                                             out);
+    return hipGetLastError();
+}
+
+hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const ForceRec* frc,
+                               const deme_anal_obj* anal, const OwnerRec* own, const float4* moi,
+                               uint32_t firstAnalOwner, uint32_t nAnalOwners, float* blockPartials,
+                               uint32_t* nBlocksOut, hipStream_t st) {
+    const uint32_t nb = nC == 0 ? 0 : blocks(nC, ANAL_TILE);
+    *nBlocksOut = nb;
+    if (nb == 0 || nAnalOwners == 0) return hipSuccess;
+    k_anal_collect<<<nb, 256, 0, st>>>(nC, idB, ctype, frc, anal, own, moi, firstAnalOwner, nAnalOwners,
+                                       blockPartials);
+    return hipGetLastError();
+}
+
+hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint32_t nAnalOwners, float* analAcc,
+                             hipStream_t st) {
+    if (nAnalOwners == 0) return hipSuccess;
+    k_anal_final<<<1, 64, 0, st>>>(blockPartials, nBlocks, nAnalOwners, analAcc);
+    return hipGetLastError();
+}
+
+hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnalOwner,
+                               uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
+                               const uint32_t* bStart, const uint32_t* bList, const ForceRec* frc,
+                               const float* analAcc, const float4* moi, const deme_family_rule* rules, OwnerRec* own,
+                               float* acc, int doIntegrate, float t, hipStream_t st) {
+    if (nO == 0) return hipSuccess;
+    k_owner_update<<<blocks(nO, 256), 256, 0, st>>>(P, nO, nClumps, firstAnalOwner, nAnalOwners, ownerSphStart,
+                                                    rowStart, bStart, bList, frc, analAcc, moi, rules, own, acc,
+                                                    doIntegrate, t);
+    return hipGetLastError();
+}
+
+}  // namespace deme_int

@@ -1,0 +1,156 @@
+// Device-wide exclusive prefix sums (the occupancy scans of contact detection: per-bin occupancy -> bin start,
+// per-sphere contact counts -> row start, per-owner counts -> list start).
+//
+// Three phases per level, tiles of 4096 elements (256 lanes x 16 items, four wave64s): tile reduce -> scan of tile
+// sums (recursively) -> tile scan plus offset. Deterministic; sums are carried in 64 bits so overflow of the 32-bit
+// outputs is detectable from the returned total.
+#include "deme_internal.h"
+
+namespace deme_int {
+namespace {
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per lane across a 256-lane block; returns exclusive prefix, writes block total.
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t& total) {
+    __shared__ uint64_t waveTot[SCAN_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t incl = wave_incl_scan(v, lane);
+    if (lane == 63) waveTot[wid] = incl;
+    __syncthreads();
+    uint64_t pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; w++) {
+        const uint64_t x = waveTot[w];
+        if (w < wid) pre += x;
+        tot += x;
+    }
+    __syncthreads();
+    total = tot;
+    return pre + incl - v;
+}
+
+template <typename TIn>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const TIn* __restrict__ a, const TIn* __restrict__ b,
+                                                            uint64_t n, uint64_t* __restrict__ sums) {
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint64_t idx = base + (uint64_t)i * SCAN_BLOCK + threadIdx.x;
+        if (idx < n) {
+            s += (uint64_t)a[idx];
+            if (b) s += (uint64_t)b[idx];
+        }
+    }
+    uint64_t tot;
+    block_excl_scan(s, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// tile scan: loads striped (coalesced), scans per-lane contiguous runs, stores striped.
+template <typename TIn, typename TOut>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const TIn* __restrict__ b, uint64_t n,
+                                                           const uint64_t* __restrict__ offsets, TOut* out,
+                                                           unsigned long long* totalOut) {
+    // +1 padding per 16 entries keeps the per-lane contiguous walk free of 64-bit bank conflicts
+    __shared__ uint64_t tile[SCAN_TILE + SCAN_TILE / 16];
+    auto at = [](uint32_t k) { return k + (k >> 4); };
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
+        const uint64_t idx = base + k;
+        uint64_t v = 0;
+        if (idx < n) {
+            v = (uint64_t)a[idx];
+            if (b) v += (uint64_t)b[idx];
+        }
+        tile[at(k)] = v;
+    }
+    __syncthreads();
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) s += tile[at(threadIdx.x * SCAN_ITEMS + i)];
+    uint64_t tot;
+    uint64_t run = block_excl_scan(s, tot);
+    const uint64_t off = offsets ? offsets[blockIdx.x] : 0;
+    run += off;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint32_t k = threadIdx.x * SCAN_ITEMS + i;
+        const uint64_t v = tile[at(k)];
+        tile[at(k)] = run;
+        run += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
+        const uint64_t idx = base + k;
+        if (idx < n) out[idx] = (TOut)tile[at(k)];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint64_t total = off + tot;
+        out[n] = (TOut)total;
+        if (totalOut) *totalOut = total;
+    }
+}
+
+// In-place exclusive scan of u64 values (the per-tile sums); out has n+1 slots.
+hipError_t scan_u64_inplace(uint64_t* v, uint64_t n, uint64_t* scratch, hipStream_t st) {
+    const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb <= 1) {
+        k_scan_apply<uint64_t, uint64_t><<<1, SCAN_BLOCK, 0, st>>>(v, nullptr, n, nullptr, v, nullptr);
+        return hipGetLastError();
+    }
+    uint64_t* sums = scratch;
+    k_scan_reduce<uint64_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(v, nullptr, n, sums);
+    hipError_t e = scan_u64_inplace(sums, nb, scratch + nb + 1, st);
+    if (e != hipSuccess) return e;
+    k_scan_apply<uint64_t, uint64_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(v, nullptr, n, sums, v, nullptr);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+uint64_t scan_scratch_elems(uint64_t n) {
+    uint64_t total = 0;
+    uint64_t m = n;
+    while (true) {
+        const uint64_t nb = (m + SCAN_TILE - 1) / SCAN_TILE;
+        total += nb + 1;
+        if (nb <= 1) break;
+        m = nb;
+    }
+    return total + 16;
+}
+
+hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
+                    uint64_t* scratch, hipStream_t st) {
+    const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (n == 0) {
+        // out[0] = 0 and total = 0
+        k_scan_apply<uint32_t, uint32_t><<<1, SCAN_BLOCK, 0, st>>>(a, b, 0, nullptr, out, totalDev);
+        return hipGetLastError();
+    }
+    if (nb <= 1) {
+        k_scan_apply<uint32_t, uint32_t><<<1, SCAN_BLOCK, 0, st>>>(a, b, n, nullptr, out, totalDev);
+        return hipGetLastError();
+    }
+    uint64_t* sums = scratch;
+    k_scan_reduce<uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, b, n, sums);
+    hipError_t e = scan_u64_inplace(sums, nb, scratch + nb + 1, st);
+    if (e != hipSuccess) return e;
+    k_scan_apply<uint32_t, uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, b, n, sums, out, totalDev);
+    return hipGetLastError();
+}
+
+}  // namespace deme_int

@@ -1,0 +1,323 @@
+"""ctypes declarations of the C ABI in include/deme/deme.h (libdeme.so) and a numpy-facing device system.
+
+The product path is libdeme.so; nothing here computes physics. Loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdeme.so")
+
+# status codes / constants (include/deme/deme_types.h)
+DEME_OK = 0
+DEME_ERR_INVALID_ARGUMENT = 1
+DEME_ERR_HIP = 2
+DEME_ERR_BIN_OVERFLOW = 3
+DEME_ERR_CAPACITY = 4
+DEME_ERR_NOT_READY = 5
+DEME_ERR_UNSUPPORTED = 7
+DEME_ERR_NO_DEVICE = 8
+SPHERE_SPHERE = 1
+SPHERE_PLANE = 3
+NULL_MAP = 0xFFFFFFFF
+NUM_MASK_ENTRIES = 256 * 257 // 2
+NUM_FAMILIES = 256
+INTEGRATOR = {"forward_euler": 0, "centered_difference": 1, "extended_taylor": 2}
+
+
+class SimParams(C.Structure):
+    _fields_ = [
+        ("nvXp2", C.c_uint8), ("nvYp2", C.c_uint8), ("nvZp2", C.c_uint8), ("integrator", C.c_uint8),
+        ("nbX", C.c_uint32), ("nbY", C.c_uint32), ("nbZ", C.c_uint32), ("pad0", C.c_uint32),
+        ("l", C.c_double), ("voxelSize", C.c_double), ("l_exact", C.c_double), ("voxelSize_exact", C.c_double),
+        ("binSize", C.c_double),
+        ("LBF", C.c_float * 3), ("G", C.c_float * 3), ("h", C.c_float), ("beta", C.c_float),
+    ]
+
+
+class Counts(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "nSpheres", "nOwners", "nAnal", "nBinSphereTouches", "nActiveBins", "nContacts", "nPrevContacts",
+        "nSphereGeoContacts", "maxSpheresInBin", "nSteps")] + [("time", C.c_double)]
+
+
+class OwnerState(C.Structure):
+    _fields_ = [("voxelID", C.c_void_p)] + [(n, C.c_void_p) for n in (
+        "locX", "locY", "locZ", "oriQw", "oriQx", "oriQy", "oriQz", "vX", "vY", "vZ", "omgX", "omgY", "omgZ",
+        "aX", "aY", "aZ", "alphaX", "alphaY", "alphaZ", "family")]
+
+
+# numpy mirrors of the POD structs
+MATERIAL_DTYPE = np.dtype([("E", "<f4"), ("nu", "<f4"), ("CoR", "<f4"), ("mu", "<f4"), ("Crr", "<f4")])
+ANAL_DTYPE = np.dtype([("owner", "<u4"), ("type", "u1"), ("material", "u1"), ("normal", "u1"), ("pad", "u1"),
+                       ("relPos", "<f4", (3,)), ("rot", "<f4", (3,)), ("size", "<f4", (3,)), ("mass", "<f4")])
+RULE_DTYPE = np.dtype([("used", "u1"), ("linVelPrescribed", "u1"), ("rotVelPrescribed", "u1"),
+                       ("linPosPrescribed", "u1"), ("rotPosPrescribed", "u1"), ("velSet", "u1"), ("posSet", "u1"),
+                       ("pad", "u1"), ("vel", "<f4", (6,)), ("pos", "<f8", (3,))])
+assert MATERIAL_DTYPE.itemsize == 20 and ANAL_DTYPE.itemsize == 48 and RULE_DTYPE.itemsize == 56
+
+STATE_FIELDS = {
+    "voxelID": np.uint64, "locX": np.uint16, "locY": np.uint16, "locZ": np.uint16,
+    "oriQw": np.float32, "oriQx": np.float32, "oriQy": np.float32, "oriQz": np.float32,
+    "vX": np.float32, "vY": np.float32, "vZ": np.float32, "omgX": np.float32, "omgY": np.float32,
+    "omgZ": np.float32, "aX": np.float32, "aY": np.float32, "aZ": np.float32, "alphaX": np.float32,
+    "alphaY": np.float32, "alphaZ": np.float32, "family": np.uint8,
+}
+
+EXPORTED = [
+    "deme_ctx_create", "deme_ctx_destroy", "deme_last_error", "deme_device_count", "deme_get_stream", "deme_sync",
+    "deme_figure_out_nv", "deme_kernel_literal", "deme_build_sim_params", "deme_encode_positions",
+    "deme_decode_positions", "deme_set_params", "deme_upload_templates", "deme_upload_materials",
+    "deme_upload_analytical", "deme_upload_families", "deme_upload_state", "deme_download_state",
+    "deme_cd_bin_count", "deme_cd_bin_fill", "deme_cd_pairs", "deme_cd_sort_remap", "deme_cd_run",
+    "deme_download_bin_counts", "deme_download_bins", "deme_download_contacts", "deme_force", "deme_reduce_owner",
+    "deme_integrate", "deme_step", "deme_download_forces", "deme_download_wildcards", "deme_upload_wildcards",
+    "deme_get_counts", "deme_set_time", "deme_enable_kernel_timing", "deme_reset_kernel_timing",
+    "deme_kernel_timing",
+]
+
+_lib = None
+
+
+def ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def load(path: str = LIB_PATH):
+    """Load libdeme.so (raises OSError if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"libdeme.so not found at {path}; run __graft_entry__.build() first")
+    lib = C.CDLL(path)
+    vp, i32, u32, u64, f32, f64 = C.c_void_p, C.c_int, C.c_uint32, C.c_uint64, C.c_float, C.c_double
+    sig = {
+        "deme_ctx_create": ([i32, vp, C.POINTER(vp)], i32),
+        "deme_ctx_destroy": ([vp], i32),
+        "deme_last_error": ([vp], C.c_char_p),
+        "deme_device_count": ([C.POINTER(i32)], i32),
+        "deme_get_stream": ([vp, C.POINTER(vp)], i32),
+        "deme_sync": ([vp], i32),
+        "deme_figure_out_nv": ([f32, f32, f32, i32, vp, C.POINTER(f32)], i32),
+        "deme_kernel_literal": ([f64], f64),
+        "deme_build_sim_params": ([f32, f32, f32, i32, vp, f32, vp, f32, f64, vp, f32, f32, i32,
+                                   C.POINTER(SimParams)], i32),
+        "deme_encode_positions": ([C.POINTER(SimParams), u64, vp, vp, vp, vp, vp], i32),
+        "deme_decode_positions": ([C.POINTER(SimParams), u64, vp, vp, vp, vp, vp], i32),
+        "deme_set_params": ([vp, C.POINTER(SimParams)], i32),
+        "deme_upload_templates": ([vp, u32, vp, vp, vp, vp, u32, u32, vp, vp], i32),
+        "deme_upload_materials": ([vp, u32, vp], i32),
+        "deme_upload_analytical": ([vp, u32, vp], i32),
+        "deme_upload_families": ([vp, vp, vp], i32),
+        "deme_upload_state": ([vp, C.POINTER(OwnerState)], i32),
+        "deme_download_state": ([vp, C.POINTER(OwnerState)], i32),
+        "deme_cd_bin_count": ([vp], i32), "deme_cd_bin_fill": ([vp], i32), "deme_cd_pairs": ([vp], i32),
+        "deme_cd_sort_remap": ([vp], i32), "deme_cd_run": ([vp], i32),
+        "deme_download_bin_counts": ([vp, vp, vp], i32),
+        "deme_download_bins": ([vp, vp, vp], i32),
+        "deme_download_contacts": ([vp, vp, vp, vp, vp], i32),
+        "deme_force": ([vp], i32), "deme_reduce_owner": ([vp], i32), "deme_integrate": ([vp], i32),
+        "deme_step": ([vp, i32, i32], i32),
+        "deme_download_forces": ([vp, vp, vp, vp, vp], i32),
+        "deme_download_wildcards": ([vp, vp], i32), "deme_upload_wildcards": ([vp, vp], i32),
+        "deme_get_counts": ([vp, C.POINTER(Counts)], i32),
+        "deme_set_time": ([vp, f64], i32),
+        "deme_enable_kernel_timing": ([vp, i32], i32), "deme_reset_kernel_timing": ([vp], i32),
+        "deme_kernel_timing": ([vp, i32, vp, vp, vp, C.POINTER(i32)], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+class DemeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"DEME status {code}: {msg}")
+        self.code = code
+
+
+def build_sim_params(box, lbf, min_radius, h, beta=0.0, g=(0.0, 0.0, -9.81), bin_size=0.0,
+                     integrator="extended_taylor", exact_dir=-1, nv=None, l=0.0) -> SimParams:
+    lib = load()
+    out = SimParams()
+    # keep every array bound to a name until the call returns (ctypes does not own the memory)
+    nv_arr = None if nv is None else np.asarray(nv, np.uint8)
+    lbf_arr = np.asarray(lbf, np.float32)
+    g_arr = np.asarray(g, np.float32)
+    st = lib.deme_build_sim_params(C.c_float(box[0]), C.c_float(box[1]), C.c_float(box[2]), exact_dir, ptr(nv_arr),
+                                   C.c_float(l), ptr(lbf_arr), C.c_float(min_radius), C.c_double(bin_size),
+                                   ptr(g_arr), C.c_float(h), C.c_float(beta), INTEGRATOR[integrator], C.byref(out))
+    if st != DEME_OK:
+        raise DemeError(st, "deme_build_sim_params failed")
+    return out
+
+
+def encode_positions(params: SimParams, xyz: np.ndarray):
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    n = xyz.shape[0]
+    vox = np.zeros(n, np.uint64)
+    lx, ly, lz = (np.zeros(n, np.uint16) for _ in range(3))
+    lib.deme_encode_positions(C.byref(params), n, ptr(xyz), ptr(vox), ptr(lx), ptr(ly), ptr(lz))
+    return vox, lx, ly, lz
+
+
+def decode_positions(params: SimParams, vox, lx, ly, lz) -> np.ndarray:
+    lib = load()
+    n = len(vox)
+    out = np.zeros((n, 3), np.float32)
+    v = np.ascontiguousarray(vox, np.uint64)
+    x, y, z = (np.ascontiguousarray(a, np.uint16) for a in (lx, ly, lz))
+    lib.deme_decode_positions(C.byref(params), n, ptr(v), ptr(x), ptr(y), ptr(z), ptr(out))
+    return out
+
+
+def make_state_struct(arrays: dict) -> OwnerState:
+    s = OwnerState()
+    for k in STATE_FIELDS:
+        a = arrays.get(k)
+        setattr(s, k, None if a is None else a.ctypes.data)
+    return s
+
+
+def empty_state(n: int) -> dict:
+    return {k: np.zeros(n, t) for k, t in STATE_FIELDS.items()}
+
+
+class DeviceSystem:
+    """A scene resident on the GPU behind the C ABI. Every method maps to one ABI call."""
+
+    def __init__(self, scene, device: int = 0):
+        self.lib = load()
+        self.scene = scene
+        h = C.c_void_p()
+        st = self.lib.deme_ctx_create(device, None, C.byref(h))
+        if st != DEME_OK:
+            raise DemeError(st, "deme_ctx_create failed (no HIP device?)")
+        self.ctx = h
+        sc = scene
+        self._chk(self.lib.deme_set_params(self.ctx, C.byref(sc.params)))
+        self._chk(self.lib.deme_upload_templates(
+            self.ctx, sc.n_spheres, ptr(sc.sph_owner), ptr(sc.sph_radius), ptr(sc.sph_relpos), ptr(sc.sph_mat),
+            sc.n_owners, sc.n_clumps, ptr(sc.owner_mass), ptr(sc.owner_moi)))
+        self._chk(self.lib.deme_upload_materials(self.ctx, len(sc.materials), ptr(sc.materials)))
+        self._chk(self.lib.deme_upload_analytical(self.ctx, len(sc.anal), ptr(sc.anal)))
+        self._chk(self.lib.deme_upload_families(self.ctx, ptr(sc.mask), ptr(sc.rules)))
+        self.set_state(sc.state)
+
+    def _chk(self, st):
+        if st != DEME_OK:
+            raise DemeError(st, self.lib.deme_last_error(self.ctx).decode())
+        return st
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.deme_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, state: dict):
+        arrs = {k: (np.ascontiguousarray(v, STATE_FIELDS[k]) if v is not None else None) for k, v in state.items()}
+        s = make_state_struct(arrs)
+        self._chk(self.lib.deme_upload_state(self.ctx, C.byref(s)))
+
+    def get_state(self) -> dict:
+        arrs = empty_state(self.scene.n_owners)
+        s = make_state_struct(arrs)
+        self._chk(self.lib.deme_download_state(self.ctx, C.byref(s)))
+        return arrs
+
+    def counts(self) -> Counts:
+        c = Counts()
+        self._chk(self.lib.deme_get_counts(self.ctx, C.byref(c)))
+        return c
+
+    def cd_bin_count(self): self._chk(self.lib.deme_cd_bin_count(self.ctx))
+    def cd_bin_fill(self): self._chk(self.lib.deme_cd_bin_fill(self.ctx))
+    def cd_pairs(self): self._chk(self.lib.deme_cd_pairs(self.ctx))
+    def cd_sort_remap(self): self._chk(self.lib.deme_cd_sort_remap(self.ctx))
+    def cd_run(self): self._chk(self.lib.deme_cd_run(self.ctx))
+    def force(self): self._chk(self.lib.deme_force(self.ctx))
+    def reduce_owner(self): self._chk(self.lib.deme_reduce_owner(self.ctx))
+    def integrate(self): self._chk(self.lib.deme_integrate(self.ctx))
+    def step(self, n, update_freq=1): self._chk(self.lib.deme_step(self.ctx, int(n), int(update_freq)))
+    def sync(self): self._chk(self.lib.deme_sync(self.ctx))
+    def set_time(self, t): self._chk(self.lib.deme_set_time(self.ctx, float(t)))
+
+    def bin_counts(self):
+        n = self.scene.n_spheres
+        nb = np.zeros(n, np.uint16)
+        na = np.zeros(n, np.uint8)
+        self._chk(self.lib.deme_download_bin_counts(self.ctx, ptr(nb), ptr(na)))
+        return nb, na
+
+    def bins(self):
+        p = self.scene.params
+        nbins = p.nbX * p.nbY * p.nbZ
+        start = np.zeros(nbins + 1, np.uint32)
+        P = self.counts().nBinSphereTouches
+        sph = np.zeros(P, np.uint32)
+        self._chk(self.lib.deme_download_bins(self.ctx, ptr(start), ptr(sph)))
+        return start, sph
+
+    def contacts(self):
+        n = self.counts().nContacts
+        a, b, m = (np.zeros(n, np.uint32) for _ in range(3))
+        t = np.zeros(n, np.uint8)
+        self._chk(self.lib.deme_download_contacts(self.ctx, ptr(a), ptr(b), ptr(t), ptr(m)))
+        return a, b, t, m
+
+    def forces(self):
+        n = self.counts().nContacts
+        F, T, cA, cB = (np.zeros((n, 3), np.float32) for _ in range(4))
+        self._chk(self.lib.deme_download_forces(self.ctx, ptr(F), ptr(T), ptr(cA), ptr(cB)))
+        return F, T, cA, cB
+
+    def wildcards(self):
+        n = self.counts().nContacts
+        w = np.zeros((4, n), np.float32)
+        self._chk(self.lib.deme_download_wildcards(self.ctx, ptr(w)))
+        return w
+
+    def set_wildcards(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._chk(self.lib.deme_upload_wildcards(self.ctx, ptr(w)))
+
+    def stream_handle(self) -> int:
+        s = C.c_void_p()
+        self._chk(self.lib.deme_get_stream(self.ctx, C.byref(s)))
+        return s.value or 0
+
+    def enable_timing(self, on=True): self._chk(self.lib.deme_enable_kernel_timing(self.ctx, 1 if on else 0))
+    def reset_timing(self): self._chk(self.lib.deme_reset_kernel_timing(self.ctx))
+
+    def kernel_timing(self) -> dict:
+        mx = 64
+        names = (C.c_char_p * mx)()
+        ms = np.zeros(mx, np.float64)
+        cnt = np.zeros(mx, np.uint64)
+        n = C.c_int()
+        self._chk(self.lib.deme_kernel_timing(self.ctx, mx, C.cast(names, C.c_void_p), ptr(ms), ptr(cnt),
+                                              C.byref(n)))
+        return {names[i].decode(): (float(ms[i]), int(cnt[i])) for i in range(min(n.value, mx))}
+
+
+def device_count() -> int:
+    lib = load()
+    n = C.c_int(0)
+    lib.deme_device_count(C.byref(n))
+    return n.value

@@ -1,0 +1,760 @@
+// DEMSolver facade (see DEMSolver.h). Host-side set-up restates DEMSolver::Initialize
+// (src/DEM/APIPublic.cpp:659-693, APIPrivate.cpp) and hands every per-step operation to the C ABI.
+#include "DEMSolver.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+
+namespace deme {
+
+namespace {
+
+float3 f3sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+
+// helper_math normalize on the host: v * (1 / sqrtf(dot(v, v)))
+float3 normalize3(float3 v) {
+    const float invLen = 1.0f / std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    return make_float3(v.x * invLen, v.y * invLen, v.z * invLen);
+}
+
+// hostApplyOriQToVector3<float, float> (HostSideHelpers.hpp:373-384)
+void host_rotate(float& X, float& Y, float& Z, float Qw, float Qx, float Qy, float Qz) {
+    const float oX = X, oY = Y, oZ = Z;
+    X = (2.0f * (Qw * Qw + Qx * Qx) - 1.0f) * oX + (2.0f * (Qx * Qy - Qw * Qz)) * oY + (2.0f * (Qx * Qz + Qw * Qy)) * oZ;
+    Y = (2.0f * (Qx * Qy + Qw * Qz)) * oX + (2.0f * (Qw * Qw + Qy * Qy) - 1.0f) * oY + (2.0f * (Qy * Qz - Qw * Qx)) * oZ;
+    Z = (2.0f * (Qx * Qz - Qw * Qy)) * oX + (2.0f * (Qy * Qz + Qw * Qx)) * oY + (2.0f * (Qw * Qw + Qz * Qz) - 1.0f) * oZ;
+}
+
+uint64_t sat_u64(double x) {
+    if (!(x > 0.0)) return 0;
+    if (x >= 18446744073709551616.0) return ~0ull;
+    return (uint64_t)x;
+}
+uint16_t sat_u16(double x) {
+    if (!(x > 0.0)) return 0;
+    if (x >= 65535.0) return 65535;
+    return (uint16_t)(uint32_t)x;
+}
+
+// hostPositionToVoxelID (HostSideHelpers.hpp:327-349) with the exact l / voxelSize
+void host_pos_to_voxel(const deme_sim_params& p, double X, double Y, double Z, uint64_t& id, uint16_t& sx, uint16_t& sy,
+                       uint16_t& sz) {
+    const uint64_t nx = sat_u64(X / p.voxelSize_exact), ny = sat_u64(Y / p.voxelSize_exact),
+                   nz = sat_u64(Z / p.voxelSize_exact);
+    sx = sat_u16((X - (double)nx * p.voxelSize_exact) / p.l_exact);
+    sy = sat_u16((Y - (double)ny * p.voxelSize_exact) / p.l_exact);
+    sz = sat_u16((Z - (double)nz * p.voxelSize_exact) / p.l_exact);
+    id = nx;
+    id += ny << p.nvXp2;
+    id += nz << (p.nvXp2 + p.nvYp2);
+}
+
+// hostVoxelIDToPosition (HostSideHelpers.hpp:307-324)
+void host_voxel_to_pos(const deme_sim_params& p, uint64_t id, uint16_t sx, uint16_t sy, uint16_t sz, double& X,
+                       double& Y, double& Z) {
+    const uint64_t vx = id & (((uint64_t)1 << p.nvXp2) - 1);
+    const uint64_t vy = (id >> p.nvXp2) & (((uint64_t)1 << p.nvYp2) - 1);
+    const uint64_t vz = id >> (p.nvXp2 + p.nvYp2);
+    X = (double)vx * p.voxelSize_exact + (double)sx * p.l_exact;
+    Y = (double)vy * p.voxelSize_exact + (double)sy * p.l_exact;
+    Z = (double)vz * p.voxelSize_exact + (double)sz * p.l_exact;
+}
+
+uint32_t locate_mask_pair(uint32_t i, uint32_t j) {
+    if (i > j) std::swap(i, j);
+    return (1 + j) * j / 2 + i;
+}
+
+void check_family(unsigned int ID, const char* what) {
+    if (ID > 255) {
+        std::stringstream ss;
+        ss << what << ": family number " << ID << " is larger than the max allowance 255";
+        throw std::runtime_error(ss.str());
+    }
+}
+
+// A prescription string of the reference is a C expression in t; this path takes numeric constants.
+bool parse_numeric(const std::string& s, double& v) {
+    const char* b = s.c_str();
+    char* e = nullptr;
+    v = std::strtod(b, &e);
+    if (e == b) return false;
+    while (*e == ' ' || *e == '\t' || *e == 'f') e++;
+    return *e == '\0';
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------------
+// templates / batches / objects
+// ---------------------------------------------------------------------------------------------------------------
+void DEMClumpTemplate::Scale(float s) {
+    for (auto& p : relPos) {
+        p.x *= s;
+        p.y *= s;
+        p.z *= s;
+    }
+    for (auto& r : radii) r *= s;
+}
+void DEMClumpTemplate::InformCentroidPrincipal(float3 center, float4 prin_Q) {
+    for (auto& p : relPos) {
+        host_rotate(p.x, p.y, p.z, prin_Q.w, -prin_Q.x, -prin_Q.y, -prin_Q.z);
+        p.x -= center.x;
+        p.y -= center.y;
+        p.z -= center.z;
+    }
+}
+void DEMClumpTemplate::Move(float3 vec, float4 rot_Q) {
+    for (auto& p : relPos) {
+        host_rotate(p.x, p.y, p.z, rot_Q.w, rot_Q.x, rot_Q.y, rot_Q.z);
+        p.x += vec.x;
+        p.y += vec.y;
+        p.z += vec.z;
+    }
+}
+
+DEMClumpBatch::DEMClumpBatch(size_t num) : nClumps(num) {
+    types.resize(num);
+    families.resize(num, DEFAULT_CLUMP_FAMILY_NUM);
+    vel.resize(num, make_float3(0, 0, 0));
+    angVel.resize(num, make_float3(0, 0, 0));
+    xyz.resize(num);
+    oriQ.resize(num, make_float4(0, 0, 0, 1));
+}
+void DEMClumpBatch::assertLength(size_t len, const std::string& name) const {
+    if (len != nClumps) {
+        std::stringstream ss;
+        ss << name << " input argument must have length " << nClumps << " (not " << len
+           << "), same as the number of clumps you originally added via AddClumps.";
+        throw std::runtime_error(ss.str());
+    }
+}
+void DEMClumpBatch::SetTypes(const std::vector<std::shared_ptr<DEMClumpTemplate>>& input) {
+    assertLength(input.size(), "SetTypes");
+    types = input;
+}
+void DEMClumpBatch::SetTypes(const std::shared_ptr<DEMClumpTemplate>& input) {
+    SetTypes(std::vector<std::shared_ptr<DEMClumpTemplate>>(nClumps, input));
+}
+void DEMClumpBatch::SetPos(const std::vector<float3>& input) {
+    assertLength(input.size(), "SetPos");
+    xyz = input;
+}
+void DEMClumpBatch::SetVel(const std::vector<float3>& input) {
+    assertLength(input.size(), "SetVel");
+    vel = input;
+}
+void DEMClumpBatch::SetAngVel(const std::vector<float3>& input) {
+    assertLength(input.size(), "SetAngVel");
+    angVel = input;
+}
+void DEMClumpBatch::SetOriQ(const std::vector<float4>& input) {
+    assertLength(input.size(), "SetOriQ");
+    oriQ = input;
+}
+void DEMClumpBatch::SetFamilies(const std::vector<unsigned int>& input) {
+    assertLength(input.size(), "SetFamilies");
+    for (unsigned int f : input) check_family(f, "SetFamilies");
+    families = input;
+    family_isSpecified = true;
+}
+
+void DEMExternObj::SetFamily(unsigned int code) {
+    check_family(code, "DEMExternObj::SetFamily");
+    family_code = code;
+}
+void DEMExternObj::AddPlane(const float3 pos, const float3 normal, const std::shared_ptr<DEMMaterial>& material) {
+    planes.push_back({pos, normalize3(normal), material});
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------------------------------------------------
+DEMSolver::DEMSolver(unsigned int) { m_family_mask.assign(DEME_NUM_MASK_ENTRIES, DEME_DONT_PREVENT_CONTACT); }
+
+DEMSolver::~DEMSolver() {
+    if (m_ctx) deme_ctx_destroy(m_ctx);
+}
+
+void DEMSolver::check(int status, const char* what) const {
+    if (status != DEME_OK) {
+        std::stringstream ss;
+        ss << what << " failed (status " << status << ")";
+        if (m_ctx) ss << ": " << deme_last_error(m_ctx);
+        throw std::runtime_error(ss.str());
+    }
+}
+
+void DEMSolver::InstructBoxDomainDimension(float x, float y, float z, SPATIAL_DIR dir_exact) {
+    m_user_boxSize = make_float3(x, y, z);
+    m_box_dir_length_is_exact = dir_exact;
+    explicit_nv_override = false;
+}
+
+void DEMSolver::InstructBoxDomainNumVoxel(unsigned char x, unsigned char y, unsigned char z, float len_unit) {
+    if (x + y + z != 64) throw std::runtime_error("Voxel numbers (as powers of 2) must add up to 64.");
+    l = len_unit;
+    nvXp2 = x;
+    nvYp2 = y;
+    nvZp2 = z;
+    const double vs = (double)((size_t)1 << 16) * (double)l;
+    m_user_boxSize = make_float3((float)(vs * (double)((size_t)1 << x)), (float)(vs * (double)((size_t)1 << y)),
+                                 (float)(vs * (double)((size_t)1 << z)));
+    explicit_nv_override = true;
+}
+
+std::shared_ptr<DEMForceModel> DEMSolver::UseFrictionalHertzianModel() {
+    m_force_model->type = FORCE_MODEL::HERTZIAN;
+    return m_force_model;
+}
+std::shared_ptr<DEMForceModel> DEMSolver::UseFrictionlessHertzianModel() {
+    throw std::runtime_error("The frictionless Hertzian model is not provided on this path (FORCE_MODEL::HERTZIAN only).");
+}
+std::shared_ptr<DEMForceModel> DEMSolver::DefineContactForceModel(const std::string&) {
+    throw std::runtime_error("Custom force-model strings need runtime specialisation, which this path does not provide.");
+}
+
+void DEMSolver::SetFamilyFixed(unsigned int ID) {
+    check_family(ID, "SetFamilyFixed");
+    FamilyPrescription p;
+    p.family = ID;
+    p.linVelX = p.linVelY = p.linVelZ = "0";
+    p.rotVelX = p.rotVelY = p.rotVelZ = "0";
+    p.linVelPrescribed = p.rotVelPrescribed = p.rotPosPrescribed = p.linPosPrescribed = true;
+    p.used = true;
+    m_input_family_prescription.push_back(p);
+}
+
+void DEMSolver::SetFamilyPrescribedLinVel(unsigned int ID, const std::string& vx, const std::string& vy,
+                                          const std::string& vz, bool dictate) {
+    check_family(ID, "SetFamilyPrescribedLinVel");
+    FamilyPrescription p;
+    p.family = ID;
+    p.linVelX = vx;
+    p.linVelY = vy;
+    p.linVelZ = vz;
+    p.linVelPrescribed = dictate;
+    p.rotVelPrescribed = dictate;
+    p.used = true;
+    m_input_family_prescription.push_back(p);
+}
+
+void DEMSolver::SetFamilyPrescribedAngVel(unsigned int ID, const std::string& wx, const std::string& wy,
+                                          const std::string& wz, bool dictate) {
+    check_family(ID, "SetFamilyPrescribedAngVel");
+    FamilyPrescription p;
+    p.family = ID;
+    p.rotVelX = wx;
+    p.rotVelY = wy;
+    p.rotVelZ = wz;
+    p.linVelPrescribed = dictate;
+    p.rotVelPrescribed = dictate;
+    p.used = true;
+    m_input_family_prescription.push_back(p);
+}
+
+void DEMSolver::SetFamilyPrescribedPosition(unsigned int ID, const std::string& X, const std::string& Y,
+                                            const std::string& Z) {
+    check_family(ID, "SetFamilyPrescribedPosition");
+    FamilyPrescription p;
+    p.family = ID;
+    p.linPosX = X;
+    p.linPosY = Y;
+    p.linPosZ = Z;
+    p.rotPosPrescribed = true;
+    p.linPosPrescribed = true;
+    p.used = true;
+    m_input_family_prescription.push_back(p);
+}
+
+void DEMSolver::DisableContactBetweenFamilies(unsigned int ID1, unsigned int ID2) {
+    check_family(ID1, "DisableContactBetweenFamilies");
+    check_family(ID2, "DisableContactBetweenFamilies");
+    m_family_mask[locate_mask_pair(ID1, ID2)] = DEME_PREVENT_CONTACT;
+    m_mask_dirty = sys_initialized;
+    m_input_no_contact_pairs.push_back({ID1, ID2});
+}
+
+void DEMSolver::EnableContactBetweenFamilies(unsigned int ID1, unsigned int ID2) {
+    check_family(ID1, "EnableContactBetweenFamilies");
+    check_family(ID2, "EnableContactBetweenFamilies");
+    if (!sys_initialized)
+        throw std::runtime_error("There is no need to call EnableContactBetweenFamilies before system initialization.");
+    m_family_mask[locate_mask_pair(ID1, ID2)] = DEME_DONT_PREVENT_CONTACT;
+    m_mask_dirty = true;
+}
+
+std::shared_ptr<DEMMaterial> DEMSolver::LoadMaterial(const std::unordered_map<std::string, float>& mat_prop) {
+    for (const auto& kv : mat_prop)
+        if (kv.first.find(' ') != std::string::npos)
+            throw std::runtime_error("Material property " + kv.first + " is not valid: no spaces allowed in its name.");
+    auto ptr = std::make_shared<DEMMaterial>(mat_prop);
+    ptr->load_order = (unsigned int)m_loaded_materials.size();
+    m_loaded_materials.push_back(ptr);
+    return ptr;
+}
+
+std::shared_ptr<DEMClumpTemplate> DEMSolver::LoadClumpType(DEMClumpTemplate& clump) {
+    if (clump.nComp != clump.radii.size() || clump.nComp != clump.relPos.size() ||
+        clump.nComp != clump.materials.size())
+        throw std::runtime_error(
+            "Radii, relative positions and material arrays defining a clump topology, must all have the same length.");
+    clump.mark = (unsigned int)m_templates.size();
+    if (clump.m_name == "NULL") {
+        char name[32];
+        std::snprintf(name, sizeof(name), "%04u", clump.mark);
+        clump.m_name = name;
+    }
+    m_templates.push_back(std::make_shared<DEMClumpTemplate>(clump));
+    return m_templates.back();
+}
+
+std::shared_ptr<DEMClumpTemplate> DEMSolver::LoadClumpType(float mass, float3 moi, const std::vector<float>& r,
+                                                           const std::vector<float3>& pos,
+                                                           const std::vector<std::shared_ptr<DEMMaterial>>& mats) {
+    DEMClumpTemplate c;
+    c.mass = mass;
+    c.MOI = moi;
+    c.radii = r;
+    c.relPos = pos;
+    c.materials = mats;
+    c.nComp = (unsigned int)r.size();
+    return LoadClumpType(c);
+}
+
+std::shared_ptr<DEMClumpTemplate> DEMSolver::LoadClumpType(float mass, float3 moi, const std::vector<float>& r,
+                                                           const std::vector<float3>& pos,
+                                                           const std::shared_ptr<DEMMaterial>& mat) {
+    return LoadClumpType(mass, moi, r, pos, std::vector<std::shared_ptr<DEMMaterial>>(r.size(), mat));
+}
+
+std::shared_ptr<DEMClumpTemplate> DEMSolver::LoadSphereType(float mass, float radius,
+                                                            const std::shared_ptr<DEMMaterial>& material) {
+    const float I = (float)(2.0 / 5.0 * mass * radius * radius);  // APIPublic.cpp:402-409
+    return LoadClumpType(mass, make_float3(I, I, I), std::vector<float>(1, radius),
+                         std::vector<float3>(1, make_float3(0, 0, 0)),
+                         std::vector<std::shared_ptr<DEMMaterial>>(1, material));
+}
+
+std::shared_ptr<DEMClumpBatch> DEMSolver::AddClumps(DEMClumpBatch& input_batch) {
+    input_batch.load_order = cached_input_clump_batches.size();
+    cached_input_clump_batches.push_back(std::make_shared<DEMClumpBatch>(input_batch));
+    return cached_input_clump_batches.back();
+}
+
+std::shared_ptr<DEMClumpBatch> DEMSolver::AddClumps(const std::vector<std::shared_ptr<DEMClumpTemplate>>& types,
+                                                    const std::vector<float3>& xyz) {
+    if (types.size() != xyz.size()) throw std::runtime_error("Arrays in the call AddClumps must all have the same length.");
+    DEMClumpBatch b(types.size());
+    b.SetTypes(types);
+    b.SetPos(xyz);
+    return AddClumps(b);
+}
+
+std::shared_ptr<DEMExternObj> DEMSolver::AddExternalObject() {
+    auto p = std::make_shared<DEMExternObj>();
+    p->load_order = (unsigned int)cached_extern_objs.size();
+    cached_extern_objs.push_back(p);
+    return p;
+}
+
+std::shared_ptr<DEMExternObj> DEMSolver::AddBCPlane(const float3 pos, const float3 normal,
+                                                    const std::shared_ptr<DEMMaterial>& material) {
+    auto p = AddExternalObject();
+    p->AddPlane(pos, normal, material);
+    return p;
+}
+
+// addWorldBoundingBox (APIPrivate.cpp:660-687)
+void DEMSolver::addWorldBoundingBox() {
+    if (m_user_add_bounding_box == "none") return;
+    if (!m_bounding_box_material) throw std::runtime_error("InstructBoxDomainBoundingBC needs a material.");
+    const float3 L = m_boxLBF, S = m_user_boxSize;
+    auto mat = m_bounding_box_material;
+    auto box = AddExternalObject();
+    box->AddPlane(make_float3(L.x + S.x / 2., L.y + S.y / 2., L.z), make_float3(0, 0, 1), mat);
+    if (m_user_add_bounding_box == "only_bottom") return;
+    box->AddPlane(make_float3(L.x, L.y + S.y / 2., L.z + S.z / 2.), make_float3(1, 0, 0), mat);
+    box->AddPlane(make_float3(L.x + S.x, L.y + S.y / 2., L.z + S.z / 2.), make_float3(-1, 0, 0), mat);
+    box->AddPlane(make_float3(L.x + S.x / 2., L.y, L.z + S.z / 2.), make_float3(0, 1, 0), mat);
+    box->AddPlane(make_float3(L.x + S.x / 2., L.y + S.y, L.z + S.z / 2.), make_float3(0, -1, 0), mat);
+    if (m_user_add_bounding_box == "top_open") return;
+    box->AddPlane(make_float3(L.x + S.x / 2., L.y + S.y / 2., L.z + S.z), make_float3(0, 0, -1), mat);
+}
+
+void DEMSolver::Initialize() {
+    if (sys_initialized) throw std::runtime_error("Re-initialization is not supported on this path.");
+    // validateUserInputs (APIPrivate.cpp:878-907)
+    if (m_user_boxSize.x <= 0.f || m_user_boxSize.y <= 0.f || m_user_boxSize.z <= 0.f)
+        throw std::runtime_error("The size of the simulation world is set to be (or default to be) impossibly small.");
+    if (!(m_ts_size > 0.0)) throw std::runtime_error("A positive time step size is required (SetInitTimeStep).");
+    if (m_force_model->type != FORCE_MODEL::HERTZIAN) throw std::runtime_error("Only the frictional Hertzian model is provided.");
+    if (m_integrator == TIME_INTEGRATOR::CHUNG)
+        throw std::runtime_error("The integration type is unknown or not implemented. Please select another via SetIntegrator.");
+    SetFamilyFixed(RESERVED_FAMILY_NUM);
+
+    // figureOutOrigin (APIPrivate.cpp:647-658)
+    if (m_user_instructed_origin == "center") {
+        m_boxLBF = make_float3(-m_user_boxSize.x / 2.0f, -m_user_boxSize.y / 2.0f, -m_user_boxSize.z / 2.0f);
+    } else if (m_user_instructed_origin != "explicit") {
+        throw std::runtime_error("Unrecognized location of system origin.");
+    }
+    addWorldBoundingBox();
+
+    // smallest radius (decideBinSize, APIPrivate.cpp:335-344)
+    float minR = std::numeric_limits<float>::max();
+    for (const auto& t : m_templates)
+        for (float r : t->radii) minR = std::min(minR, r);
+
+    // expand factor (transferSimParams, APIPrivate.cpp:730-737)
+    if (!use_user_defined_expand_factor)
+        m_expand_factor = (float)(m_approx_max_vel * m_ts_size * m_updateFreq * m_expand_safety_param);
+    if (m_expand_factor < 0.f) m_expand_factor = 0.f;
+
+    int exact = -1;
+    if (m_box_dir_length_is_exact == SPATIAL_DIR::X) exact = 0;
+    if (m_box_dir_length_is_exact == SPATIAL_DIR::Y) exact = 1;
+    if (m_box_dir_length_is_exact == SPATIAL_DIR::Z) exact = 2;
+    const float LBF[3] = {m_boxLBF.x, m_boxLBF.y, m_boxLBF.z};
+    const float g[3] = {G.x, G.y, G.z};
+    int integ = m_integrator == TIME_INTEGRATOR::FORWARD_EULER         ? DEME_INTEGRATOR_FORWARD_EULER
+                : m_integrator == TIME_INTEGRATOR::CENTERED_DIFFERENCE ? DEME_INTEGRATOR_CENTERED_DIFFERENCE
+                                                                        : DEME_INTEGRATOR_EXTENDED_TAYLOR;
+    uint8_t nv[3] = {nvXp2, nvYp2, nvZp2};
+    int st = deme_build_sim_params(m_user_boxSize.x, m_user_boxSize.y, m_user_boxSize.z, exact,
+                                   explicit_nv_override ? nv : nullptr, l, LBF,
+                                   minR == std::numeric_limits<float>::max() ? 0.f : minR,
+                                   use_user_defined_bin_size ? m_binSize : 0.0, g, (float)m_ts_size, m_expand_factor,
+                                   integ, &m_params);
+    if (st != DEME_OK) {
+        if (st == DEME_ERR_INVALID_ARGUMENT)
+            throw std::runtime_error("There are spheres with 0 radii, and the user did not specify the bin size.");
+        throw std::runtime_error("The number of bins exceeds binID_t; please use a larger bin size.");
+    }
+    m_binSize = m_params.binSize;
+
+    // entities: clumps first, then external objects (dT.cpp:436-620)
+    std::vector<uint32_t> sphOwner;
+    std::vector<float> sphR, sphRel;
+    std::vector<uint8_t> sphMat;
+    std::vector<float> mass, moi;
+    size_t nClumps = 0;
+    for (const auto& b : cached_input_clump_batches) nClumps += b->GetNumClumps();
+    nOwnerClumps = nClumps;
+    nExtObj = cached_extern_objs.size();
+    nOwnerBodies = nClumps + nExtObj;
+    const size_t nO = nOwnerBodies;
+    h_voxel.assign(nO, 0);
+    h_lx.assign(nO, 0);
+    h_ly.assign(nO, 0);
+    h_lz.assign(nO, 0);
+    h_qw.assign(nO, 1.f);
+    for (auto* v : {&h_qx, &h_qy, &h_qz, &h_vx, &h_vy, &h_vz, &h_wx, &h_wy, &h_wz, &h_ax, &h_ay, &h_az, &h_alx, &h_aly,
+                    &h_alz})
+        v->assign(nO, 0.f);
+    h_fam.assign(nO, 0);
+    mass.resize(nO);
+    moi.resize(3 * nO);
+    size_t o = 0;
+    for (const auto& b : cached_input_clump_batches) {
+        for (size_t i = 0; i < b->GetNumClumps(); i++, o++) {
+            const auto& t = b->types.at(i);
+            if (!t) throw std::runtime_error("A clump in an AddClumps batch has no template.");
+            mass[o] = t->mass;
+            moi[3 * o] = t->MOI.x;
+            moi[3 * o + 1] = t->MOI.y;
+            moi[3 * o + 2] = t->MOI.z;
+            for (unsigned int j = 0; j < t->nComp; j++) {
+                sphOwner.push_back((uint32_t)o);
+                sphR.push_back(t->radii[j]);
+                sphRel.push_back(t->relPos[j].x);
+                sphRel.push_back(t->relPos[j].y);
+                sphRel.push_back(t->relPos[j].z);
+                sphMat.push_back((uint8_t)t->materials[j]->load_order);
+            }
+            const float3 c = f3sub(b->xyz[i], m_boxLBF);
+            host_pos_to_voxel(m_params, (double)c.x, (double)c.y, (double)c.z, h_voxel[o], h_lx[o], h_ly[o], h_lz[o]);
+            h_qw[o] = b->oriQ[i].w;
+            h_qx[o] = b->oriQ[i].x;
+            h_qy[o] = b->oriQ[i].y;
+            h_qz[o] = b->oriQ[i].z;
+            h_vx[o] = b->vel[i].x;
+            h_vy[o] = b->vel[i].y;
+            h_vz[o] = b->vel[i].z;
+            h_wx[o] = b->angVel[i].x;
+            h_wy[o] = b->angVel[i].y;
+            h_wz[o] = b->angVel[i].z;
+            h_fam[o] = (uint8_t)b->families[i];
+        }
+    }
+    nSpheresGM = sphOwner.size();
+    std::vector<deme_anal_obj> anal;
+    for (size_t k = 0; k < nExtObj; k++, o++) {
+        const auto& e = cached_extern_objs[k];
+        mass[o] = e->mass;
+        moi[3 * o] = e->MOI.x;
+        moi[3 * o + 1] = e->MOI.y;
+        moi[3 * o + 2] = e->MOI.z;
+        const float3 c = f3sub(e->init_pos, m_boxLBF);
+        host_pos_to_voxel(m_params, (double)c.x, (double)c.y, (double)c.z, h_voxel[o], h_lx[o], h_ly[o], h_lz[o]);
+        h_fam[o] = (uint8_t)e->family_code;
+        for (const auto& pl : e->planes) {
+            deme_anal_obj a{};
+            a.owner = (uint32_t)o;
+            a.type = DEME_ANAL_OBJ_TYPE_PLANE;
+            a.material = (uint8_t)pl.material->load_order;
+            a.normal = 0;
+            a.relPos[0] = pl.position.x;
+            a.relPos[1] = pl.position.y;
+            a.relPos[2] = pl.position.z;
+            a.rot[0] = pl.normal.x;
+            a.rot[1] = pl.normal.y;
+            a.rot[2] = pl.normal.z;
+            a.mass = e->mass;
+            anal.push_back(a);
+        }
+    }
+    nAnalGM = anal.size();
+
+    // materials (equipMaterials, APIPrivate.cpp:1089-1140): missing properties default to 0
+    std::vector<deme_material> mats;
+    for (const auto& m : m_loaded_materials) {
+        auto get = [&](const char* k) {
+            auto it = m->mat_prop.find(k);
+            return it == m->mat_prop.end() ? 0.f : it->second;
+        };
+        mats.push_back(deme_material{get("E"), get("nu"), get("CoR"), get("mu"), get("Crr")});
+    }
+    if (mats.empty()) mats.push_back(deme_material{0, 0, 0, 0, 0});
+
+    // family prescriptions (figureOutFamilyMasks APIPrivate.cpp:574-645, equipFamilyPrescribedMotions :1005-1048)
+    std::vector<FamilyPrescription> merged(256);
+    for (unsigned int f = 0; f < 256; f++) merged[f].family = f;
+    for (const auto& p : m_input_family_prescription) {
+        auto& m = merged[p.family];
+        m.used = true;
+        auto take = [](std::string& dst, const std::string& src) {
+            if (src != "none") dst = src;
+        };
+        take(m.linPosX, p.linPosX);
+        take(m.linPosY, p.linPosY);
+        take(m.linPosZ, p.linPosZ);
+        take(m.linVelX, p.linVelX);
+        take(m.linVelY, p.linVelY);
+        take(m.linVelZ, p.linVelZ);
+        take(m.rotVelX, p.rotVelX);
+        take(m.rotVelY, p.rotVelY);
+        take(m.rotVelZ, p.rotVelZ);
+        m.linVelPrescribed = m.linVelPrescribed || p.linVelPrescribed;
+        m.rotVelPrescribed = m.rotVelPrescribed || p.rotVelPrescribed;
+        m.rotPosPrescribed = m.rotPosPrescribed || p.rotPosPrescribed;
+        m.linPosPrescribed = m.linPosPrescribed || p.linPosPrescribed;
+    }
+    std::vector<deme_family_rule> rules(256);
+    for (unsigned int f = 0; f < 256; f++) {
+        const auto& m = merged[f];
+        deme_family_rule& r = rules[f];
+        std::memset(&r, 0, sizeof(r));
+        if (!m.used) continue;
+        r.used = 1;
+        r.linVelPrescribed = m.linVelPrescribed;
+        r.rotVelPrescribed = m.rotVelPrescribed;
+        r.linPosPrescribed = m.linPosPrescribed;
+        r.rotPosPrescribed = m.rotPosPrescribed;
+        const std::string* vels[6] = {&m.linVelX, &m.linVelY, &m.linVelZ, &m.rotVelX, &m.rotVelY, &m.rotVelZ};
+        for (int k = 0; k < 6; k++) {
+            if (*vels[k] == "none") continue;
+            double v;
+            if (!parse_numeric(*vels[k], v))
+                throw std::runtime_error("Prescribed velocity expression '" + *vels[k] +
+                                         "' is not a numeric constant; time-dependent expressions need runtime "
+                                         "specialisation, which this path does not provide.");
+            r.vel[k] = (float)v;
+            r.velSet |= (uint8_t)(1u << k);
+        }
+        const std::string* pos[3] = {&m.linPosX, &m.linPosY, &m.linPosZ};
+        for (int k = 0; k < 3; k++) {
+            if (*pos[k] == "none") continue;
+            double v;
+            if (!parse_numeric(*pos[k], v))
+                throw std::runtime_error("Prescribed position expression '" + *pos[k] + "' is not a numeric constant.");
+            r.pos[k] = v;
+            r.posSet |= (uint8_t)(1u << k);
+        }
+    }
+
+    // device context
+    int dev = 0;
+    check(deme_ctx_create(dev, nullptr, &m_ctx), "deme_ctx_create");
+    check(deme_set_params(m_ctx, &m_params), "deme_set_params");
+    check(deme_upload_templates(m_ctx, (uint32_t)nSpheresGM, sphOwner.data(), sphR.data(), sphRel.data(), sphMat.data(),
+                                (uint32_t)nO, (uint32_t)nClumps, mass.data(), moi.data()),
+          "deme_upload_templates");
+    check(deme_upload_materials(m_ctx, (uint32_t)mats.size(), mats.data()), "deme_upload_materials");
+    check(deme_upload_analytical(m_ctx, (uint32_t)anal.size(), anal.data()), "deme_upload_analytical");
+    check(deme_upload_families(m_ctx, m_family_mask.data(), rules.data()), "deme_upload_families");
+    m_mirror_valid = true;
+    m_mirror_dirty = true;
+    pushHostState();
+    sys_initialized = true;
+}
+
+void DEMSolver::pushHostState() {
+    if (!m_mirror_dirty) return;
+    deme_owner_state s{};
+    s.voxelID = h_voxel.data();
+    s.locX = h_lx.data();
+    s.locY = h_ly.data();
+    s.locZ = h_lz.data();
+    s.oriQw = h_qw.data();
+    s.oriQx = h_qx.data();
+    s.oriQy = h_qy.data();
+    s.oriQz = h_qz.data();
+    s.vX = h_vx.data();
+    s.vY = h_vy.data();
+    s.vZ = h_vz.data();
+    s.omgX = h_wx.data();
+    s.omgY = h_wy.data();
+    s.omgZ = h_wz.data();
+    s.family = h_fam.data();
+    check(deme_upload_state(m_ctx, &s), "deme_upload_state");
+    m_mirror_dirty = false;
+}
+
+void DEMSolver::syncHostState() {
+    if (!sys_initialized) throw std::runtime_error("The system is not initialized.");
+    if (m_mirror_valid) return;
+    deme_owner_state s{};
+    s.voxelID = h_voxel.data();
+    s.locX = h_lx.data();
+    s.locY = h_ly.data();
+    s.locZ = h_lz.data();
+    s.oriQw = h_qw.data();
+    s.oriQx = h_qx.data();
+    s.oriQy = h_qy.data();
+    s.oriQz = h_qz.data();
+    s.vX = h_vx.data();
+    s.vY = h_vy.data();
+    s.vZ = h_vz.data();
+    s.omgX = h_wx.data();
+    s.omgY = h_wy.data();
+    s.omgZ = h_wz.data();
+    s.aX = h_ax.data();
+    s.aY = h_ay.data();
+    s.aZ = h_az.data();
+    s.alphaX = h_alx.data();
+    s.alphaY = h_aly.data();
+    s.alphaZ = h_alz.data();
+    s.family = h_fam.data();
+    check(deme_download_state(m_ctx, &s), "deme_download_state");
+    m_mirror_valid = true;
+}
+
+void DEMSolver::DoDynamics(double duration) {
+    if (!sys_initialized) throw std::runtime_error("Please call Initialize before DoDynamics.");
+    pushHostState();
+    // same step count as the dT loop `for (cycle = 0; cycle < duration; cycle += h)` (dT.cpp:1527)
+    const float h = m_params.h;
+    int n = 0;
+    for (double cyc = 0.0; cyc < duration; cyc += (double)h) n++;
+    if (m_mask_dirty) {
+        check(deme_upload_families(m_ctx, m_family_mask.data(), nullptr), "deme_upload_families");
+        m_mask_dirty = false;
+    }
+    check(deme_step(m_ctx, n, m_updateFreq), "deme_step");
+    m_mirror_valid = false;
+}
+
+void DEMSolver::DoDynamicsThenSync(double duration) {
+    DoDynamics(duration);
+    check(deme_sync(m_ctx), "deme_sync");
+}
+
+float3 DEMSolver::GetOwnerPosition(uint32_t i) {
+    syncHostState();
+    double X, Y, Z;
+    host_voxel_to_pos(m_params, h_voxel.at(i), h_lx.at(i), h_ly.at(i), h_lz.at(i), X, Y, Z);
+    return make_float3((float)(X + m_params.LBF[0]), (float)(Y + m_params.LBF[1]), (float)(Z + m_params.LBF[2]));
+}
+float3 DEMSolver::GetOwnerVelocity(uint32_t i) {
+    syncHostState();
+    return make_float3(h_vx.at(i), h_vy.at(i), h_vz.at(i));
+}
+float3 DEMSolver::GetOwnerAngVel(uint32_t i) {
+    syncHostState();
+    return make_float3(h_wx.at(i), h_wy.at(i), h_wz.at(i));
+}
+float4 DEMSolver::GetOwnerOriQ(uint32_t i) {
+    syncHostState();
+    float4 q;
+    q.w = h_qw.at(i);
+    q.x = h_qx.at(i);
+    q.y = h_qy.at(i);
+    q.z = h_qz.at(i);
+    return q;
+}
+float3 DEMSolver::GetOwnerAcc(uint32_t i) {
+    syncHostState();
+    return make_float3(h_ax.at(i), h_ay.at(i), h_az.at(i));
+}
+float3 DEMSolver::GetOwnerAngAcc(uint32_t i) {
+    syncHostState();
+    return make_float3(h_alx.at(i), h_aly.at(i), h_alz.at(i));
+}
+void DEMSolver::SetOwnerPosition(uint32_t i, float3 pos) {
+    syncHostState();
+    // float subtraction first, as dT::setOwnerPos does (dT.cpp:1783-1790)
+    const double X = pos.x - m_params.LBF[0], Y = pos.y - m_params.LBF[1], Z = pos.z - m_params.LBF[2];
+    host_pos_to_voxel(m_params, X, Y, Z, h_voxel.at(i), h_lx.at(i), h_ly.at(i), h_lz.at(i));
+    m_mirror_dirty = true;
+}
+void DEMSolver::SetOwnerVelocity(uint32_t i, float3 v) {
+    syncHostState();
+    h_vx.at(i) = v.x;
+    h_vy.at(i) = v.y;
+    h_vz.at(i) = v.z;
+    m_mirror_dirty = true;
+}
+void DEMSolver::SetOwnerAngVel(uint32_t i, float3 w) {
+    syncHostState();
+    h_wx.at(i) = w.x;
+    h_wy.at(i) = w.y;
+    h_wz.at(i) = w.z;
+    m_mirror_dirty = true;
+}
+void DEMSolver::SetOwnerOriQ(uint32_t i, float4 q) {
+    syncHostState();
+    h_qw.at(i) = q.w;
+    h_qx.at(i) = q.x;
+    h_qy.at(i) = q.y;
+    h_qz.at(i) = q.z;
+    m_mirror_dirty = true;
+}
+
+size_t DEMSolver::GetNumContacts() {
+    deme_counts c;
+    check(deme_get_counts(m_ctx, &c), "deme_get_counts");
+    return c.nContacts;
+}
+
+double DEMSolver::GetTimeElapsed() {
+    deme_counts c;
+    check(deme_get_counts(m_ctx, &c), "deme_get_counts");
+    return c.time;
+}
+
+void DEMSolver::ShowTimingStats() {
+    const char* names[64];
+    double ms[64];
+    uint64_t cnt[64];
+    int n = 0;
+    check(deme_kernel_timing(m_ctx, 64, names, ms, cnt, &n), "deme_kernel_timing");
+    std::printf("\n~~ DEVICE KERNEL TIMING ~~\n");
+    for (int i = 0; i < n && i < 64; i++)
+        std::printf("%-16s %12.3f ms over %8llu launches\n", names[i], ms[i], (unsigned long long)cnt[i]);
+}
+
+}  // namespace deme

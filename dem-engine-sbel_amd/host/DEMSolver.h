@@ -1,0 +1,282 @@
+// DEMSolver: C++ facade with the reference's user-facing API (src/DEM/API.h:44-887) for the per-step hot path,
+// running on the MI355X C ABI (include/deme/deme.h). Names, argument meaning and error behaviour (std::runtime_error
+// on invalid input) follow the reference; entry points outside the hot path are absent.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/deme/deme.h"
+
+namespace deme {
+
+// Verbosity levels (src/DEM/Defines.h:104)
+enum VERBOSITY { QUIET = 0, ERROR = 10, WARNING = 20, INFO = 30, STEP_STATS = 32, STEP_METRIC = 35, DEBUG = 40 };
+// Stepping methods (src/DEM/Defines.h:106)
+enum class TIME_INTEGRATOR { FORWARD_EULER, CENTERED_DIFFERENCE, EXTENDED_TAYLOR, CHUNG };
+// Spatial directions (src/DEM/Defines.h:133)
+enum class SPATIAL_DIR { X, Y, Z, NONE };
+enum class FORCE_MODEL { HERTZIAN, HERTZIAN_FRICTIONLESS, CUSTOM };
+
+constexpr unsigned int RESERVED_FAMILY_NUM = 255;
+constexpr unsigned int DEFAULT_CLUMP_FAMILY_NUM = 0;
+
+// Material (src/DEM/Structs.h, DEMMaterial): named float properties; load order is the material index.
+struct DEMMaterial {
+    std::unordered_map<std::string, float> mat_prop;
+    unsigned int load_order = 0;
+    explicit DEMMaterial(const std::unordered_map<std::string, float>& props) : mat_prop(props) {}
+};
+
+// Clump template (src/DEM/Structs.h:389-477).
+class DEMClumpTemplate {
+  public:
+    float mass = 0;
+    float3 MOI = make_float3(0, 0, 0);
+    std::vector<float> radii;
+    std::vector<float3> relPos;
+    std::vector<std::shared_ptr<DEMMaterial>> materials;
+    unsigned int nComp = 0;
+    unsigned int mark = 0;
+    std::string m_name = "NULL";
+    float volume = 0.0;
+    void SetVolume(float vol) { volume = vol; }
+    void AssignName(const std::string& name) { m_name = name; }
+    // Scale all geometry components (Structs.h:465-474)
+    void Scale(float s);
+    // Re-express component positions in the centroid/principal frame (Structs.h:439-449)
+    void InformCentroidPrincipal(float3 center, float4 prin_Q);
+    // Rotate then move the components (Structs.h:452-461)
+    void Move(float3 vec, float4 rot_Q);
+};
+
+// Batch of clumps to add (src/DEM/Structs.h:480-559).
+class DEMClumpBatch {
+  private:
+    const size_t nClumps;
+    void assertLength(size_t len, const std::string& name) const;
+
+  public:
+    bool family_isSpecified = false;
+    std::vector<std::shared_ptr<DEMClumpTemplate>> types;
+    std::vector<unsigned int> families;
+    std::vector<float3> vel;
+    std::vector<float3> angVel;
+    std::vector<float3> xyz;
+    std::vector<float4> oriQ;
+    size_t load_order = 0;
+    explicit DEMClumpBatch(size_t num);
+    size_t GetNumClumps() const { return nClumps; }
+    void SetTypes(const std::vector<std::shared_ptr<DEMClumpTemplate>>& input);
+    void SetTypes(const std::shared_ptr<DEMClumpTemplate>& input);
+    void SetType(const std::shared_ptr<DEMClumpTemplate>& input) { SetTypes(input); }
+    void SetPos(const std::vector<float3>& input);
+    void SetPos(float3 input) { SetPos(std::vector<float3>(nClumps, input)); }
+    void SetVel(const std::vector<float3>& input);
+    void SetVel(float3 input) { SetVel(std::vector<float3>(nClumps, input)); }
+    void SetAngVel(const std::vector<float3>& input);
+    void SetAngVel(float3 input) { SetAngVel(std::vector<float3>(nClumps, input)); }
+    void SetOriQ(const std::vector<float4>& input);
+    void SetOriQ(float4 input) { SetOriQ(std::vector<float4>(nClumps, input)); }
+    void SetFamilies(const std::vector<unsigned int>& input);
+    void SetFamilies(unsigned int input) { SetFamilies(std::vector<unsigned int>(nClumps, input)); }
+    void SetFamily(unsigned int input) { SetFamilies(input); }
+};
+
+// External object made of analytical components (src/DEM/BdrsAndObjs.h:100-170); planes only on this path.
+class DEMExternObj {
+  public:
+    struct Plane {
+        float3 position;
+        float3 normal;
+        std::shared_ptr<DEMMaterial> material;
+    };
+    unsigned int family_code = RESERVED_FAMILY_NUM;
+    float3 init_pos = make_float3(0, 0, 0);
+    float mass = 1e15f;
+    float3 MOI = make_float3(1e15f, 1e15f, 1e15f);
+    unsigned int load_order = 0;
+    std::vector<Plane> planes;
+    void SetFamily(unsigned int code);
+    void SetMass(float m) { mass = m; }
+    void SetMOI(float3 m) { MOI = m; }
+    void SetInitPos(float3 p) { init_pos = p; }
+    void AddPlane(const float3 pos, const float3 normal, const std::shared_ptr<DEMMaterial>& material);
+};
+
+struct DEMForceModel {
+    FORCE_MODEL type = FORCE_MODEL::HERTZIAN;
+};
+
+class DEMSolver {
+  public:
+    explicit DEMSolver(unsigned int nGPUs = 1);
+    ~DEMSolver();
+    DEMSolver(const DEMSolver&) = delete;
+    DEMSolver& operator=(const DEMSolver&) = delete;
+
+    void SetVerbosity(VERBOSITY v) { verbosity = v; }
+
+    // ---- world (API.h:55-92) ----
+    void InstructBoxDomainDimension(float x, float y, float z, SPATIAL_DIR dir_exact = SPATIAL_DIR::NONE);
+    void InstructBoxDomainNumVoxel(unsigned char x, unsigned char y, unsigned char z, float len_unit = 1e-10f);
+    void InstructBoxDomainBoundingBC(const std::string& inst, const std::shared_ptr<DEMMaterial>& mat) {
+        m_user_add_bounding_box = inst;
+        m_bounding_box_material = mat;
+    }
+    void SetCoordSysOrigin(const std::string& where) { m_user_instructed_origin = where; }
+    void SetCoordSysOrigin(float3 O) {
+        m_boxLBF = O;
+        m_user_instructed_origin = "explicit";
+    }
+    void SetGravitationalAcceleration(float3 g) { G = g; }
+    void SetInitTimeStep(double ts_size) { m_ts_size = ts_size; }
+    void SetCDUpdateFreq(int freq) { m_updateFreq = freq; }
+    void SetIntegrator(TIME_INTEGRATOR intg) { m_integrator = intg; }
+    void SetInitBinSize(double bin_size) {
+        use_user_defined_bin_size = true;
+        m_binSize = bin_size;
+    }
+    void SetExpandFactor(float beta, bool fix = true) {
+        m_expand_factor = beta;
+        use_user_defined_expand_factor = fix;
+    }
+    void SetMaxVelocity(float max_vel) { m_approx_max_vel = max_vel; }
+    void SetExpandSafetyParam(float param) { m_expand_safety_param = param; }
+
+    // ---- force model (API.h:120-130) ----
+    std::shared_ptr<DEMForceModel> UseFrictionalHertzianModel();
+    std::shared_ptr<DEMForceModel> UseFrictionlessHertzianModel();
+    std::shared_ptr<DEMForceModel> DefineContactForceModel(const std::string& model);
+
+    // ---- families (API.h:255-298) ----
+    void SetFamilyFixed(unsigned int ID);
+    void SetFamilyPrescribedLinVel(unsigned int ID, const std::string& velX, const std::string& velY,
+                                   const std::string& velZ, bool dictate = true);
+    void SetFamilyPrescribedAngVel(unsigned int ID, const std::string& velX, const std::string& velY,
+                                   const std::string& velZ, bool dictate = true);
+    void SetFamilyPrescribedPosition(unsigned int ID, const std::string& X, const std::string& Y, const std::string& Z);
+    void DisableContactBetweenFamilies(unsigned int ID1, unsigned int ID2);
+    void EnableContactBetweenFamilies(unsigned int ID1, unsigned int ID2);
+
+    // ---- entities (API.h:160-310) ----
+    std::shared_ptr<DEMMaterial> LoadMaterial(const std::unordered_map<std::string, float>& mat_prop);
+    std::shared_ptr<DEMClumpTemplate> LoadClumpType(DEMClumpTemplate& clump);
+    std::shared_ptr<DEMClumpTemplate> LoadClumpType(float mass, float3 moi, const std::vector<float>& sp_radii,
+                                                    const std::vector<float3>& sp_locations_xyz,
+                                                    const std::vector<std::shared_ptr<DEMMaterial>>& sp_materials);
+    std::shared_ptr<DEMClumpTemplate> LoadClumpType(float mass, float3 moi, const std::vector<float>& sp_radii,
+                                                    const std::vector<float3>& sp_locations_xyz,
+                                                    const std::shared_ptr<DEMMaterial>& sp_material);
+    std::shared_ptr<DEMClumpTemplate> LoadSphereType(float mass, float radius,
+                                                     const std::shared_ptr<DEMMaterial>& material);
+    std::shared_ptr<DEMClumpBatch> AddClumps(DEMClumpBatch& input_batch);
+    std::shared_ptr<DEMClumpBatch> AddClumps(const std::vector<std::shared_ptr<DEMClumpTemplate>>& input_types,
+                                             const std::vector<float3>& input_xyz);
+    std::shared_ptr<DEMExternObj> AddExternalObject();
+    std::shared_ptr<DEMExternObj> AddBCPlane(const float3 pos, const float3 normal,
+                                             const std::shared_ptr<DEMMaterial>& material);
+
+    // ---- running (API.h:374-386) ----
+    void Initialize();
+    void DoDynamics(double thisCallDuration);
+    void DoDynamicsThenSync(double thisCallDuration);
+    void DoStepDynamics() { DoDynamics(m_ts_size); }
+
+    // ---- state (API.h:196-214) ----
+    float3 GetOwnerPosition(uint32_t ownerID);
+    float3 GetOwnerVelocity(uint32_t ownerID);
+    float3 GetOwnerAngVel(uint32_t ownerID);
+    float4 GetOwnerOriQ(uint32_t ownerID);
+    float3 GetOwnerAcc(uint32_t ownerID);
+    float3 GetOwnerAngAcc(uint32_t ownerID);
+    void SetOwnerPosition(uint32_t ownerID, float3 pos);
+    void SetOwnerVelocity(uint32_t ownerID, float3 vel);
+    void SetOwnerAngVel(uint32_t ownerID, float3 angVel);
+    void SetOwnerOriQ(uint32_t ownerID, float4 oriQ);
+
+    // ---- queries and timing ----
+    size_t GetNumOwners() const { return nOwnerBodies; }
+    size_t GetNumClumps() const { return nOwnerClumps; }
+    size_t GetNumSpheres() const { return nSpheresGM; }
+    size_t GetNumContacts();
+    double GetTimeElapsed();
+    float GetUpdatedExpandFactor() const { return m_expand_factor; }
+    double GetBinSize() const { return m_binSize; }
+    const deme_sim_params& GetSimParams() const { return m_params; }
+    void ShowTimingStats();
+    // Raw context for callers that drive the C ABI directly (e.g. bench harnesses).
+    deme_ctx* GetContext() { return m_ctx; }
+
+  private:
+    struct FamilyPrescription {
+        unsigned int family;
+        std::string linPosX = "none", linPosY = "none", linPosZ = "none";
+        std::string linVelX = "none", linVelY = "none", linVelZ = "none";
+        std::string rotVelX = "none", rotVelY = "none", rotVelZ = "none";
+        bool linVelPrescribed = false, rotVelPrescribed = false, rotPosPrescribed = false, linPosPrescribed = false;
+        bool used = false;
+    };
+
+    void check(int status, const char* what) const;
+    void syncHostState();
+    void pushHostState();
+    void addWorldBoundingBox();
+
+    VERBOSITY verbosity = INFO;
+    deme_ctx* m_ctx = nullptr;
+    bool sys_initialized = false;
+
+    // world
+    float3 m_user_boxSize = make_float3(-1.f, -1.f, -1.f);
+    SPATIAL_DIR m_box_dir_length_is_exact = SPATIAL_DIR::NONE;
+    bool explicit_nv_override = false;
+    unsigned char nvXp2 = 0, nvYp2 = 0, nvZp2 = 0;
+    float l = std::numeric_limits<float>::max();
+    float3 m_boxLBF = make_float3(0, 0, 0);
+    std::string m_user_instructed_origin = "explicit";
+    std::string m_user_add_bounding_box = "none";
+    std::shared_ptr<DEMMaterial> m_bounding_box_material;
+    float3 G = make_float3(0, 0, 0);
+    double m_ts_size = -1.0;
+    int m_updateFreq = 0;
+    TIME_INTEGRATOR m_integrator = TIME_INTEGRATOR::EXTENDED_TAYLOR;
+    bool use_user_defined_bin_size = false;
+    double m_binSize = 0.0;
+    bool use_user_defined_expand_factor = false;
+    float m_expand_factor = 0.f;
+    float m_approx_max_vel = -1.f;
+    float m_expand_safety_param = 1.f;
+    std::shared_ptr<DEMForceModel> m_force_model = std::make_shared<DEMForceModel>();
+
+    // entities
+    std::vector<std::shared_ptr<DEMMaterial>> m_loaded_materials;
+    std::vector<std::shared_ptr<DEMClumpTemplate>> m_templates;
+    std::vector<std::shared_ptr<DEMClumpBatch>> cached_input_clump_batches;
+    std::vector<std::shared_ptr<DEMExternObj>> cached_extern_objs;
+    std::vector<FamilyPrescription> m_input_family_prescription;
+    std::vector<std::pair<unsigned int, unsigned int>> m_input_no_contact_pairs;
+    std::vector<uint8_t> m_family_mask;
+    bool m_mask_dirty = false;
+
+    // derived
+    deme_sim_params m_params{};
+    size_t nOwnerClumps = 0, nOwnerBodies = 0, nSpheresGM = 0, nExtObj = 0, nAnalGM = 0;
+
+    // host mirror of the owner state (refreshed lazily after dynamics)
+    bool m_mirror_valid = false, m_mirror_dirty = false;
+    std::vector<uint64_t> h_voxel;
+    std::vector<uint16_t> h_lx, h_ly, h_lz;
+    std::vector<float> h_qw, h_qx, h_qy, h_qz, h_vx, h_vy, h_vz, h_wx, h_wy, h_wz, h_ax, h_ay, h_az, h_alx, h_aly,
+        h_alz;
+    std::vector<uint8_t> h_fam;
+};
+
+}  // namespace deme

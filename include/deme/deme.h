@@ -1,0 +1,133 @@
+/*
+ * C ABI of the MI355X-native DEM hot path (libdeme.so).
+ *
+ * The reference drives this path from two worker threads through internal C++ seams:
+ *   contactDetection(...)      src/algorithms/DEMCubBasedSubroutines.h:48-65 (impl DEMCubContactDetection.cu:39-397)
+ *   collectContactForces(...)  src/algorithms/DEMCubBasedSubroutines.h:67-74 (impl DEMCubForceCollection.cu:20-158)
+ *   jitify kernel launches     e.g. src/DEM/dT.cpp:1385-1388 (calculateContactForces), dT.cpp:1424-1431 (integrate)
+ * This ABI replaces those seams. A context owns all device memory (SoA/AoS in HBM) and one HIP stream; every call
+ * runs on that stream. All entry points return a DEME_* status (0 == DEME_OK); deme_last_error() gives the message.
+ * No C++ exception and no device trap crosses this boundary.
+ *
+ * Index conventions follow the reference: owners are clumps [0, nClumps) followed by external objects; spheres are
+ * numbered clump by clump (ownerClumpBody is non-decreasing); analytical components are numbered in load order.
+ */
+#ifndef DEME_H
+#define DEME_H
+
+#include "deme_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct deme_ctx deme_ctx;
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Lifecycle. Replaces DEMSolver's construction of kT/dT and their streams (src/DEM/APIPublic.cpp:22-41,
+ * src/core/utils/GpuManager.cpp:8-33); one context == one device + one stream (stream may be NULL: the context then
+ * creates its own default-priority stream).
+ * ------------------------------------------------------------------------------------------------------------- */
+int deme_ctx_create(int device, void* hip_stream, deme_ctx** out);
+int deme_ctx_destroy(deme_ctx* ctx);
+const char* deme_last_error(const deme_ctx* ctx);
+int deme_device_count(int* n);
+int deme_get_stream(const deme_ctx* ctx, void** hip_stream);
+int deme_sync(deme_ctx* ctx);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Host-side sizing (no device work). Restate DEMSolver::figureOutNV (src/DEM/APIPrivate.cpp:222-333),
+ * decideBinSize (APIPrivate.cpp:335-369), transferSimParams' expand factor (APIPrivate.cpp:730-737) and the JIT
+ * literal formatting of l / voxelSize (HostSideHelpers.hpp:397-402 used at APIPrivate.cpp:1358-1359).
+ * exact_dir: -1 none, 0 X, 1 Y, 2 Z (SPATIAL_DIR, InstructBoxDomainDimension at APIPublic.cpp:92-99).
+ * ------------------------------------------------------------------------------------------------------------- */
+int deme_figure_out_nv(float boxX, float boxY, float boxZ, int exact_dir, uint8_t nv_out[3], float* l_out);
+double deme_kernel_literal(double value);
+/* Fill a deme_sim_params: nv/l from the box (or explicit nv + l when nv_in != NULL, InstructBoxDomainNumVoxel at
+ * APIPublic.cpp:101-119), bin size = 2 * min_radius unless bin_size > 0 (SetInitBinSize), nb = floor(world/bin)+1. */
+int deme_build_sim_params(float boxX, float boxY, float boxZ, int exact_dir, const uint8_t* nv_in, float l_in,
+                          const float LBF[3], float min_radius, double bin_size, const float G[3], float h,
+                          float beta, int integrator, deme_sim_params* out);
+/* World coordinates (float xyz, n x 3) <-> voxel + sub-voxel integers with the exact l / voxelSize, as the host does
+ * at initialisation (dT.cpp:528-575: coordinate - LBF in float, then hostPositionToVoxelID) and in getOwnerPos
+ * (dT.cpp:1770-1781: hostVoxelIDToPosition + LBF). */
+int deme_encode_positions(const deme_sim_params* p, uint64_t n, const float* xyz, uint64_t* voxelID, uint16_t* locX,
+                          uint16_t* locY, uint16_t* locZ);
+int deme_decode_positions(const deme_sim_params* p, uint64_t n, const uint64_t* voxelID, const uint16_t* locX,
+                          const uint16_t* locY, const uint16_t* locZ, float* xyz);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Static data. Replaces allocateManagedArrays/populateEntityArrays (src/DEM/kT.cpp:372-626, dT.cpp:214-693) and the
+ * JIT definitions of materials, analytical objects and family prescriptions (APIPrivate.cpp:1005-1347).
+ * ------------------------------------------------------------------------------------------------------------- */
+int deme_set_params(deme_ctx* ctx, const deme_sim_params* p);
+/* Per sphere: owner (clump index), radius, relPos (3 floats), material. Per owner (clumps and external objects):
+ * mass and principal MOI (3 floats). nClumps <= nOwners; spheres must be grouped by owner in ascending order. */
+int deme_upload_templates(deme_ctx* ctx, uint32_t nSpheres, const uint32_t* sphOwner, const float* sphRadius,
+                          const float* sphRelPos, const uint8_t* sphMat, uint32_t nOwners, uint32_t nClumps,
+                          const float* ownerMass, const float* ownerMoi);
+int deme_upload_materials(deme_ctx* ctx, uint32_t n, const deme_material* mats);
+int deme_upload_analytical(deme_ctx* ctx, uint32_t n, const deme_anal_obj* objs);
+/* mask: DEME_NUM_MASK_ENTRIES entries (locateMaskPair layout); rules: DEME_NUM_FAMILIES entries, or NULL to update
+ * only the mask after the first upload (DisableContactBetweenFamilies after initialisation, APIPublic.cpp:448-466). */
+int deme_upload_families(deme_ctx* ctx, const uint8_t* mask, const deme_family_rule* rules);
+
+/* State I/O (dT getters/setters, src/DEM/dT.cpp:1722-1806; initial encoding dT.cpp:572-575). */
+int deme_upload_state(deme_ctx* ctx, const deme_owner_state* s);
+int deme_download_state(deme_ctx* ctx, deme_owner_state* s);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Contact detection (contactDetection, DEMCubContactDetection.cu:39-397). The four stages may be called in order,
+ * or all at once with deme_cd_run.
+ * ------------------------------------------------------------------------------------------------------------- */
+/* getNumberOfBinsEachSphereTouches (DEMBinSphereKernels.cu:12-115): per-sphere bin and plane counts, occupancy. */
+int deme_cd_bin_count(deme_ctx* ctx);
+/* prefix scan + populateBinSphereTouchingPairs + SortPairs/RLE (DEMCubContactDetection.cu:77-166). */
+int deme_cd_bin_fill(deme_ctx* ctx);
+/* getNumberOfContactsEachBin / populateContactPairsEachBin (DEMContactKernels_Blockwise.cu:15-309). */
+int deme_cd_pairs(deme_ctx* ctx);
+/* contact sort + history map (DEMCubContactDetection.cu:236-372, DEMHistoryMappingKernels.cu:5-60). */
+int deme_cd_sort_remap(deme_ctx* ctx);
+int deme_cd_run(deme_ctx* ctx);
+int deme_download_bin_counts(deme_ctx* ctx, uint16_t* numBinsSphereTouches, uint8_t* numAnalGeoSphereTouches);
+/* binStart: nBins + 1 entries; binSphere: P entries (sphere ids of each bin, unordered within a bin). */
+int deme_download_bins(deme_ctx* ctx, uint32_t* binStart, uint32_t* binSphere);
+/* Canonical contact list: by idA; plane contacts before sphere contacts; sphere partners by ascending idB. */
+int deme_download_contacts(deme_ctx* ctx, uint32_t* idA, uint32_t* idB, uint8_t* type, uint32_t* map);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Dynamics (DEMDynamicThread::calculateForces / integrateOwnerMotions, src/DEM/dT.cpp:1349-1432).
+ * ------------------------------------------------------------------------------------------------------------- */
+/* prepareForceArrays + rearrangeContactWildcards + calculateContactForces (DEMPrepForceKernels.cu:23-55,
+ * DEMCalcForceKernels.cu:14-179 with FullHertzianForceModel.cu). */
+int deme_force(deme_ctx* ctx);
+/* collectContactForces (DEMCubForceCollection.cu:20-158): per-owner linear and angular accelerations. */
+int deme_reduce_owner(deme_ctx* ctx);
+/* integrateOwners (DEMIntegrationKernels.cu:170-179), preceded by the owner reduction of the last forces. */
+int deme_integrate(deme_ctx* ctx);
+/* The step loop (DEMDynamicThread::workerThread, dT.cpp:1527-1585) with a deterministic contact-update schedule:
+ * detection on the state at steps k*f (f = max(update_freq, 1)), force -> collect -> integrate each step. */
+int deme_step(deme_ctx* ctx, int n_steps, int update_freq);
+
+/* Per-contact outputs (contactForces, contactTorque_convToForce, contactPointGeometryA/B): 3 floats each. */
+int deme_download_forces(deme_ctx* ctx, float* force, float* torque_only_force, float* cpA, float* cpB);
+/* Contact wildcards delta_tan_x, delta_tan_y, delta_tan_z, delta_time (Models.h:237-252), 4 x C floats. */
+int deme_download_wildcards(deme_ctx* ctx, float* w);
+int deme_upload_wildcards(deme_ctx* ctx, const float* w);
+int deme_get_counts(deme_ctx* ctx, deme_counts* out);
+int deme_set_time(deme_ctx* ctx, double t);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Measurement: per-kernel HIP-event timing on the context's stream (no counterpart in the reference, which only has
+ * wall-clock phase timers, src/core/utils/Timer.hpp:40-100). Names are the kernels' names.
+ * ------------------------------------------------------------------------------------------------------------- */
+int deme_enable_kernel_timing(deme_ctx* ctx, int on);
+int deme_reset_kernel_timing(deme_ctx* ctx);
+/* Writes up to max entries; returns the number of kernel names in *n. */
+int deme_kernel_timing(deme_ctx* ctx, int max, const char** names, double* total_ms, uint64_t* launches, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

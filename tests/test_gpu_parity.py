@@ -1,0 +1,195 @@
+"""Parity of the HIP hot path (through the C ABI) with the CPU oracle on identical states.
+
+Bars: integer outputs (bin counts, bin tables, contact lists, history maps, voxel ids) bit-exact; per-contact forces
+within 1e-5 of the system force scale (max |F|) of an FP64 evaluation and within 1e-6 of the FP32 oracle; owner
+accelerations within 1e-5 relative of the oracle; N-step positions within 1e-7 m.
+"""
+import numpy as np
+import pytest
+
+from deme_amd import scenes
+from deme_amd.abi import DEME_ERR_BIN_OVERFLOW, DemeError, DeviceSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def packed_spheres():
+    return scenes.sphere_settling(n=3000, spacing=1.98, velocity_sigma=0.05, omega_sigma=5.0, Crr=0.02)
+
+
+def packed_clumps():
+    return scenes.clump_settling(n_clumps=3000, spacing=(0.0070, 0.0046, 0.0046), velocity_sigma=0.05,
+                                 omega_sigma=5.0, random_orientation=True)
+
+
+def loose_box():
+    return scenes.sphere_settling(n=2000, velocity_sigma=0.3)
+
+
+def expanded_clumps():
+    return scenes.clump_settling(n_clumps=2000, spacing=(0.0072, 0.0049, 0.0049), beta_frac=0.2,
+                                 velocity_sigma=0.05, omega_sigma=3.0, random_orientation=True)
+
+
+SCENES = [packed_spheres, packed_clumps, loose_box, expanded_clumps]
+
+
+def systems(make, pre_steps=0):
+    import pyoracle
+
+    sc = make()
+    dev = DeviceSystem(sc)
+    if pre_steps:
+        dev.step(pre_steps, 1)
+    ora = pyoracle.OracleSystem(sc)
+    ora.set_state(dev.get_state())
+    return sc, dev, ora
+
+
+@pytest.mark.parametrize("make", SCENES)
+def test_bin_counts_and_table_bit_exact(make):
+    sc, dev, ora = systems(make, pre_steps=20)
+    dev.cd_bin_count()
+    ora.cd_bins()
+    nb_d, na_d = dev.bin_counts()
+    nb_o, na_o = ora.bin_counts()
+    assert np.array_equal(nb_d, nb_o)
+    assert np.array_equal(na_d, na_o)
+    dev.cd_bin_fill()
+    start, sph = dev.bins()
+    ob, osph = ora.bin_pairs()
+    p = sc.params
+    bins = np.repeat(np.arange(p.nbX * p.nbY * p.nbZ, dtype=np.uint32), np.diff(start.astype(np.int64)))
+    order = np.lexsort((sph, bins))
+    assert np.array_equal(bins[order], ob)
+    assert np.array_equal(sph[order], osph)
+    cd, co = dev.counts(), ora.counts()
+    assert cd.nBinSphereTouches == co.nBinSphereTouches
+    assert cd.nActiveBins == co.nActiveBins
+    assert cd.maxSpheresInBin == co.maxSpheresInBin
+
+
+@pytest.mark.parametrize("make", SCENES)
+def test_contacts_and_history_map_bit_exact(make):
+    sc, dev, ora = systems(make)
+    total = 0
+    for k in range(6):
+        ora.set_state(dev.get_state())
+        dev.cd_run()
+        ora.cd_run()
+        da, db, dt, dm = dev.contacts()
+        oa, ob, ot, om = ora.contacts()
+        assert np.array_equal(da, oa), f"idA differs at update {k}"
+        assert np.array_equal(db, ob), f"idB differs at update {k}"
+        assert np.array_equal(dt, ot), f"contact types differ at update {k}"
+        assert np.array_equal(dm, om), f"history map differs at update {k}"
+        total += len(da)
+        for _ in range(3):
+            dev.force()
+            dev.integrate()
+    assert total > 0
+
+
+def aligned_force_pass(make):
+    sc, dev, ora = systems(make)
+    ora.step(15, 1)
+    dev.set_state(ora.get_state())
+    ora.cd_run()
+    dev.cd_run()
+    da, db, dt, _ = dev.contacts()
+    oa, ob, ot, _ = ora.contacts()
+    assert np.array_equal(da, oa) and np.array_equal(db, ob) and np.array_equal(dt, ot)
+    dev.set_wildcards(ora.wildcards())
+    F64, T64 = ora.forces_fp64()
+    ora.force()
+    dev.force()
+    return sc, dev, ora, F64, T64
+
+
+@pytest.mark.parametrize("make", SCENES)
+def test_force_matches_fp64_and_fp32_oracle(make):
+    sc, dev, ora, F64, T64 = aligned_force_pass(make)
+    F, T, cA, cB = dev.forces()
+    Fo, To, cAo, cBo = ora.forces()
+    scale = max(np.abs(F64).max(), 1e-30)
+    assert np.abs(F - F64).max() <= 1e-5 * scale
+    assert np.abs(F - Fo).max() <= 1e-6 * scale
+    tscale = max(np.abs(T64).max(), 1e-30)
+    assert np.abs(T - T64).max() <= 1e-5 * tscale + 1e-12
+    assert np.abs(cA - cAo).max() <= 1e-6 * max(np.abs(cAo).max(), 1e-30)
+    assert np.abs(cB[:, :] - cBo).max() <= 1e-6 * max(np.abs(cBo).max(), 1e-30)
+    w, wo = dev.wildcards(), ora.wildcards()
+    assert np.abs(w[3] - wo[3]).max() <= 1e-7 * max(np.abs(wo[3]).max(), 1e-30)
+    assert np.abs(w[:3] - wo[:3]).max() <= 1e-5 * max(np.abs(wo[:3]).max(), 1e-30)
+
+
+@pytest.mark.parametrize("make", SCENES)
+def test_owner_reduction_matches_oracle(make):
+    sc, dev, ora, _, _ = aligned_force_pass(make)
+    dev.reduce_owner()
+    ora.reduce_owner()
+    sd, so = dev.get_state(), ora.get_state()
+    for k in ("aX", "aY", "aZ", "alphaX", "alphaY", "alphaZ"):
+        d, o = sd[k].astype(np.float64), so[k].astype(np.float64)
+        scale = max(np.abs(o).max(), 1e-30)
+        assert np.abs(d - o).max() <= 1e-5 * scale, k
+
+
+@pytest.mark.parametrize("make", SCENES)
+def test_integration_matches_oracle(make):
+    sc, dev, ora, _, _ = aligned_force_pass(make)
+    dev.integrate()
+    ora.reduce_owner()
+    ora.integrate()
+    sd, so = dev.get_state(), ora.get_state()
+    assert np.array_equal(sd["family"], so["family"])
+    from deme_amd.abi import decode_positions
+
+    pd = decode_positions(sc.params, sd["voxelID"], sd["locX"], sd["locY"], sd["locZ"]).astype(np.float64)
+    po = decode_positions(sc.params, so["voxelID"], so["locX"], so["locY"], so["locZ"]).astype(np.float64)
+    assert np.abs(pd - po).max() <= 1e-7
+    # most owners agree to the last sub-voxel
+    same = (sd["voxelID"] == so["voxelID"]) & (sd["locX"] == so["locX"]) & (sd["locY"] == so["locY"]) & (
+        sd["locZ"] == so["locZ"])
+    assert same.mean() > 0.99
+    for k in ("vX", "vY", "vZ", "omgX", "omgY", "omgZ", "oriQw", "oriQx", "oriQy", "oriQz"):
+        scale = max(np.abs(so[k]).max(), 1e-30)
+        assert np.abs(sd[k].astype(np.float64) - so[k]).max() <= 1e-5 * scale, k
+
+
+@pytest.mark.parametrize("make,freq", [(packed_spheres, 1), (loose_box, 1), (expanded_clumps, 5)])
+def test_trajectory_parity(make, freq):
+    import pyoracle
+
+    sc = make()
+    dev = DeviceSystem(sc)
+    ora = pyoracle.OracleSystem(sc)
+    n = 20
+    dev.step(n, freq)
+    ora.step(n, freq)
+    sd, so = dev.get_state(), ora.get_state()
+    from deme_amd.abi import decode_positions
+
+    pd = decode_positions(sc.params, sd["voxelID"], sd["locX"], sd["locY"], sd["locZ"]).astype(np.float64)
+    po = decode_positions(sc.params, so["voxelID"], so["locX"], so["locY"], so["locZ"]).astype(np.float64)
+    assert np.abs(pd - po).max() <= 1e-7
+    vscale = max(np.abs(so["vZ"]).max(), 1e-30)
+    assert np.abs(sd["vZ"] - so["vZ"]).max() <= 1e-4 * vscale
+    assert dev.counts().nContacts == ora.counts().nContacts
+    assert dev.counts().nSteps == n
+
+
+def test_bin_overflow_is_reported_not_trapped():
+    sc = scenes.sphere_settling(n=3000, spacing=1.98)
+    # a bin as large as the box puts every sphere in a handful of bins (> 256 per bin)
+    from deme_amd.abi import build_sim_params
+
+    p = sc.params
+    sc.params = build_sim_params((0.3, 0.3, 0.6), tuple(p.LBF), 0.005, p.h, bin_size=0.2)
+    dev = DeviceSystem(sc)
+    with pytest.raises(DemeError) as e:
+        dev.cd_run()
+    assert e.value.code == DEME_ERR_BIN_OVERFLOW
+    # the context stays usable after the error
+    with pytest.raises(DemeError):
+        dev.cd_run()
