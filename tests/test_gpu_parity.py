@@ -114,8 +114,11 @@ def test_force_matches_fp64_and_fp32_oracle(make):
     scale = max(np.abs(F64).max(), 1e-30)
     assert np.abs(F - F64).max() <= 1e-5 * scale
     assert np.abs(F - Fo).max() <= 1e-6 * scale
+    # rolling resistance: direction of v_rot = rotVelCPB - rotVelCPA, a normalised difference of nearly equal float
+    # vectors, so the reference's own FP32 math is only ~1e-5 from FP64 there; held to the FP32 oracle tightly
     tscale = max(np.abs(T64).max(), 1e-30)
-    assert np.abs(T - T64).max() <= 1e-5 * tscale + 1e-12
+    assert np.abs(T - T64).max() <= 1e-4 * tscale + 1e-12
+    assert np.abs(T - To).max() <= 1e-6 * tscale + 1e-12
     assert np.abs(cA - cAo).max() <= 1e-6 * max(np.abs(cAo).max(), 1e-30)
     assert np.abs(cB[:, :] - cBo).max() <= 1e-6 * max(np.abs(cBo).max(), 1e-30)
     w, wo = dev.wildcards(), ora.wildcards()
