@@ -1,0 +1,183 @@
+"""Benchmark of the DEM per-step hot path on one MI355X.
+
+Workload: 1,000,000 three-sphere clumps (3,000,000 spheres) settling in a 0.5 x 0.5 x 1.0 m plane box, FP32 state
+with FP64 positions, E=5e7, nu=0.3, CoR=0.6, mu=0.4, dt=5e-6 s. Two runs: contact detection every step (headline
+`value`) and every 5 steps with a drift margin of 0.2 r_min. A step = contact detection (when due) + Hertz-Mindlin
+force + owner reduction + integration, all on device; state stays resident in HBM.
+
+Prints ONE JSON line. `roofline` describes the kernel with the largest device time in the timed region:
+algorithmic bytes per launch (unique compulsory traffic of the arrays it touches, actual counts of that step) over
+its average HIP-event duration. `cpu_baseline` times the oracle (host C++/OpenMP restatement) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dem-engine-sbel_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def bytes_model(name, k):
+    """Algorithmic bytes per launch of each kernel (counts k of the step it ran in)."""
+    S, O, P, C, Np, Ba, nb = k["S"], k["O"], k["P"], k["C"], k["pairs"], k["Ba"], k["nbins"]
+    owner_cd = 8 + 6 + 16 + 1          # voxel id, sub-voxel position, quaternion, family
+    owner_dyn = 8 + 6 + 16 + 12 + 12 + 4  # + velocity, angular velocity, mass
+    sph = 4 + 4 + 12 + 4               # owner, radius, offset, material
+    model = {
+        # sphere -> bins: templates + owner poses in, counts + CD records out, one 4-byte RMW per touched bin
+        "sphere_bins": S * sph + O * owner_cd + S * (2 + 4 + 32) + P * 8,
+        "bin_stats": nb * 4,
+        "scan_bins": nb * 4 * 3,
+        # second pass over spheres: CD records in, RMW per touch, (sphere, bin) entries out
+        "bin_scatter": S * (32 + 4) + P * (4 + 8 + 8),
+        # per-bin pair test: entries + bin ends + CD records (unique) + owner/family of hit pairs, pairs out
+        "bin_pairs": P * (8 + 4) + S * 32 + Np * (8 + 8 + 8),
+        "scan_rows": S * (4 + 4 + 4),
+        "row_fill": Np * (8 + 4 + 4 + 8 + 5),
+        "geo_fill": S * (4 + 4 + 32) + k["geo"] * 5,
+        "row_finalize": S * 8 + C * 8,
+        "history_map": S * 8 + C * 5 + k["Cprev"] * 5 + C * 4,
+        "scan_owners": O * 12,
+        "bside_fill": C * 5 + Np * (4 + 8 + 4),
+        "bside_sort": O * 8 + Np * 8,
+        # per contact: ids + type, history in/out, force record out; owner and sphere records (unique footprint)
+        "force": C * (9 + 16 + 16 + 48) + S * sph + O * owner_dyn + (C * 4 if k.get("fresh") else 0),
+        "anal_collect": C * 5 + k["geo"] * (4 + 48),
+        # per owner: own record in/out, MOI, accelerations out; A rows contiguous, B lists gathered
+        "owner_update": O * (64 + 64 + 16 + 24 + 8) + C * 48 + Np * (4 + 48),
+    }
+    return model.get(name)
+
+
+def build_scene(n_clumps, update_freq, beta_frac):
+    from deme_amd import scenes
+
+    return scenes.clump_settling(n_clumps=n_clumps, update_freq=update_freq, beta_frac=beta_frac)
+
+
+def run_device(sc, steps, warmup, update_freq):
+    from deme_amd.abi import DeviceSystem
+
+    dev = DeviceSystem(sc)
+    dev.step(warmup, update_freq)
+    dev.sync()
+    # kernel-level HIP events on the library's stream over the timed region
+    dev.reset_timing()
+    dev.enable_timing(True)
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.step(steps, update_freq)
+    dev.sync()
+    t1 = time.perf_counter()
+    dev.enable_timing(False)
+    timing = dev.kernel_timing()
+    c = dev.counts()
+    info = dict(S=sc.n_spheres, O=sc.n_owners, P=c.nBinSphereTouches, C=c.nContacts, Cprev=c.nPrevContacts,
+                pairs=c.nContacts - c.nSphereGeoContacts, geo=c.nSphereGeoContacts, Ba=c.nActiveBins,
+                nbins=sc.params.nbX * sc.params.nbY * sc.params.nbZ, fresh=update_freq == 1)
+    dev.close()
+    return t1 - t0, timing, info
+
+
+def cpu_baseline(sc, steps, update_freq, threads):
+    import pyoracle  # noqa: E402  (the oracle is the reported CPU baseline only)
+
+    ora = pyoracle.OracleSystem(sc, threads=threads)
+    ora.step(1, update_freq)  # warm-up step
+    t0 = time.perf_counter()
+    ora.step(steps, update_freq)
+    dt = time.perf_counter() - t0
+    ora.close()
+    return dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--clumps", type=int, default=1_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-run-b", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+    scA = build_scene(args.clumps, 1, 0.0)
+    tA, timingA, infoA = run_device(scA, args.steps, args.warmup, 1)
+    value = args.clumps * args.steps / tA
+
+    runB = None
+    if not args.skip_run_b:
+        scB = build_scene(args.clumps, 5, 0.2)
+        tB, timingB, infoB = run_device(scB, args.steps, args.warmup, 5)
+        runB = {"clump_steps_per_s": args.clumps * args.steps / tB, "ms_per_step": 1e3 * tB / args.steps,
+                "contacts": infoB["C"], "beta_m": scB.params.beta}
+
+    # dominant kernel of the headline run
+    dom = max(timingA.items(), key=lambda kv: kv[1][0])
+    name, (ms_total, launches) = dom
+    avg_ms = ms_total / max(launches, 1)
+    bytes_per = bytes_model(name, infoA)
+    achieved = (bytes_per / (avg_ms * 1e-3) / 1e9) if bytes_per else None
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", f"pmc_{name}_clump_settling_1M.json")
+    if os.path.exists(pmc_path):
+        try:
+            traffic = json.load(open(pmc_path)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    kernels = {k: {"avg_us": round(1e3 * v[0] / max(v[1], 1), 2), "launches": v[1],
+                   "share": round(v[0] / sum(x[0] for x in timingA.values()), 4),
+                   "GBps": (round(bytes_model(k, infoA) / (v[0] / max(v[1], 1) * 1e-3) / 1e9, 1)
+                            if bytes_model(k, infoA) else None)}
+               for k, v in sorted(timingA.items(), key=lambda kv: -kv[1][0])}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        os.environ.setdefault("OMP_PROC_BIND", "close")
+        dt = cpu_baseline(scA, args.cpu_steps, 1, threads)
+        cpu = {"value": args.clumps * args.cpu_steps / dt, "unit": "clump-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"oracle, same 1M-clump workload, {args.cpu_steps} steps after 1 warm-up step, "
+                         f"contact detection every step, {threads} OpenMP threads"}
+
+    line = {
+        "metric": "particle-steps/s (1M clumps)",
+        "value": value,
+        "unit": "clump-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * tA / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 positions)",
+        "data": "synthetic (seeded compacted lattice)",
+        "config": {"workload": "clump_settling: 1M three-sphere clumps, 0.5x0.5x1.0 m plane box, dt=5e-6, "
+                               "contact detection every step",
+                   "clumps": args.clumps, "spheres": scA.n_spheres, "contacts": infoA["C"],
+                   "bin_sphere_touches": infoA["P"], "active_bins": infoA["Ba"],
+                   "run_every5_beta0.2r": runB, "kernels": kernels},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": traffic,
+                     "bytes_per_launch": bytes_per, "avg_launch_us": 1e3 * avg_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -352,15 +352,18 @@ __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t
 }
 
 // B-side owner lists for the owner reduction: contact indices of sphere-sphere contacts grouped by owner of B.
-__global__ __launch_bounds__(256) void k_bside_fill(uint32_t nC, const uint32_t* __restrict__ idB,
+__global__ __launch_bounds__(256) void k_bside_fill(uint32_t nC, uint32_t nS, const uint32_t* __restrict__ idB,
                                                     const uint8_t* __restrict__ ctype,
                                                     const SphereRec* __restrict__ sph,
                                                     const uint32_t* __restrict__ bStart, uint32_t* bCount,
                                                     uint32_t* __restrict__ bList) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nC || ctype[c] != DEME_SPHERE_SPHERE_CONTACT) return;
-    const uint32_t o = sph[idB[c]].owner;
+    const uint32_t b = idB[c];
+    if (b >= nS) return;  // guard against a malformed list (never gathers past the sphere table)
+    const uint32_t o = sph[b].owner;
     const uint32_t slot = atomicSub(&bCount[o], 1u) - 1u;
+    if (slot >= bStart[o + 1] - bStart[o]) return;
     bList[bStart[o] + slot] = c;
 }
 
@@ -469,10 +472,11 @@ hipError_t launch_history_map(uint32_t nS, const uint32_t* rowStart, const uint3
     return hipGetLastError();
 }
 
-hipError_t launch_bside_fill(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const SphereRec* sph,
-                             const uint32_t* bStart, uint32_t* bCount, uint32_t* bList, hipStream_t st) {
+hipError_t launch_bside_fill(uint32_t nC, uint32_t nS, const uint32_t* idB, const uint8_t* ctype,
+                             const SphereRec* sph, const uint32_t* bStart, uint32_t* bCount, uint32_t* bList,
+                             hipStream_t st) {
     if (nC == 0) return hipSuccess;
-    k_bside_fill<<<blocks(nC, 256), 256, 0, st>>>(nC, idB, ctype, sph, bStart, bCount, bList);
+    k_bside_fill<<<blocks(nC, 256), 256, 0, st>>>(nC, nS, idB, ctype, sph, bStart, bCount, bList);
     return hipGetLastError();
 }
 

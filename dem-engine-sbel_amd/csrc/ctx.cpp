@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -97,6 +98,15 @@ struct deme_ctx {
     DBuf<uint32_t> idA, map;
     DBuf<ForceRec> frc;
     DBuf<float> analPartials, analAcc;
+    // cell-list detection
+    CellGrid grid{};
+    uint64_t nCells = 0;
+    float maxRadius = 0.f;
+    uint32_t slabCap = 32;
+    bool countsStale = false;
+    DBuf<uint32_t> cellCount, cellStart, cellOf, slotOf, slab;
+    DBuf<CdRec> sorted;
+    DBuf<unsigned long long> planeBits;
     int cur = 0;
     bool havePrev = false, contactsReady = false, fresh = false;
     uint64_t P = 0, C = 0, Cprev = 0, nPairs = 0, nActive = 0, maxOcc = 0, nGeo = 0;
@@ -316,6 +326,8 @@ int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
     k.Gy = p->G[1];
     k.Gz = p->G[2];
     k.integrator = p->integrator;
+    k.nS = c->nS;
+    k.nA = c->nA;
     if (nb != c->nBins) {
         c->nBins = nb;
         DEME_HIP(c, c->binCount.ensure(nb, 1.0));
@@ -336,6 +348,7 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     std::vector<SphereRec> hs(nS);
     std::vector<uint32_t> start(nClumps + 1, 0);
     uint32_t maxMat = 0;
+    float maxR = 0.f;
     for (uint32_t s = 0; s < nS; s++) {
         maxMat = std::max<uint32_t>(maxMat, sphMat[s]);
         if (sphOwner[s] >= nClumps) return c->fail(DEME_ERR_INVALID_ARGUMENT, "sphere owner is not a clump");
@@ -343,6 +356,9 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
             return c->fail(DEME_ERR_INVALID_ARGUMENT, "spheres must be grouped by owner in ascending order");
         hs[s].owner = sphOwner[s];
         hs[s].r = sphRadius[s];
+        if (!std::isfinite(sphRadius[s]) || sphRadius[s] < 0.f)
+            return c->fail(DEME_ERR_INVALID_ARGUMENT, "sphere radii must be finite and non-negative");
+        maxR = std::max(maxR, sphRadius[s]);
         hs[s].rx = sphRelPos[3 * s];
         hs[s].ry = sphRelPos[3 * s + 1];
         hs[s].rz = sphRelPos[3 * s + 2];
@@ -376,11 +392,13 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     DEME_HIP(c, hipMemsetAsync(c->acc.p, 0, std::max<size_t>((size_t)6 * nO, 1) * sizeof(float), c->stream));
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     c->nS = nS;
+    c->kp.nS = nS;
     c->nO = nO;
     c->nClumps = nClumps;
     c->nAnalOwners = nO - nClumps;
     c->maxMatSph = maxMat;
     c->maxMat = std::max(c->maxMatSph, c->maxMatAnal);
+    c->maxRadius = maxR;
     c->haveTemplates = true;
     c->haveState = false;
     c->havePrev = c->contactsReady = c->fresh = false;
@@ -414,6 +432,7 @@ int deme_upload_analytical(deme_ctx* c, uint32_t n, const deme_anal_obj* objs) {
     if (n) DEME_HIP(c, hipMemcpyAsync(c->anal.p, objs, n * sizeof(deme_anal_obj), hipMemcpyHostToDevice, c->stream));
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     c->nA = n;
+    c->kp.nA = n;
     return DEME_OK;
 }
 
@@ -516,6 +535,8 @@ int deme_cd_bin_count(deme_ctx* c) {
     if (r) return r;
     DEME_HIP(c, hipMemsetAsync(c->ctr.p, 0, sizeof(DevCounters), c->stream));
     DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
+    // the occupancy histogram must start at zero whatever ran before (an aborted pass leaves counts behind)
+    DEME_LAUNCH(c, "fill", launch_fill_u32(c->binCount.p, c->nBins, 0, c->stream));
     DEME_LAUNCH(c, "sphere_bins",
                 launch_sphere_bins(c->kp, c->sph.p, c->own.p, c->nS, c->pose.p, c->nA, c->mask.p, c->binCount.p,
                                    c->numBins.p, c->numAnal.p, c->rec.p, c->ctr.p, c->stream));
@@ -564,6 +585,10 @@ int deme_cd_bin_fill(deme_ctx* c) {
 int deme_cd_pairs(deme_ctx* c) {
     if (c->cdStage != 2) return c->fail(DEME_ERR_NOT_READY, "deme_cd_bin_fill must run first");
     uint64_t cap = std::max<uint64_t>({(uint64_t)(c->nPairs * 1.25) + 4096, c->pairs.cap, c->P / 2 + 4096});
+    // This stage accumulates per-row and per-owner counts with atomics; the cell-list pass stores its own row counts
+    // into rowCount, so both counters are cleared here rather than assumed to be zero.
+    DEME_LAUNCH(c, "fill", launch_fill_u32(c->rowCount.p, c->nS, 0, c->stream));
+    DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCount.p, c->nClumps + 1, 0, c->stream));
     for (int attempt = 0; attempt < 3; attempt++) {
         DEME_HIP(c, c->pairs.ensure(cap, 1.0));
         DEME_HIP(c, hipMemsetAsync(&c->ctr.p->pairCount, 0, sizeof(uint32_t), c->stream));
@@ -629,7 +654,7 @@ int deme_cd_sort_remap(deme_ctx* c) {
                 scan_u32(c->bCount.p, nullptr, c->nClumps, c->bStart.p, &c->ctr.p->total[2], c->scanScratch.p,
                          c->stream));
     DEME_LAUNCH(c, "bside_fill",
-                launch_bside_fill((uint32_t)C, c->idB[cu].p, c->ctype[cu].p, c->sph.p, c->bStart.p, c->bCount.p,
+                launch_bside_fill((uint32_t)C, c->nS, c->idB[cu].p, c->ctype[cu].p, c->sph.p, c->bStart.p, c->bCount.p,
                                   c->bList.p, c->stream));
     DEME_LAUNCH(c, "bside_sort", launch_bside_sort(c->nClumps, c->bStart.p, c->bList.p, c->stream));
     c->contactsReady = true;
@@ -638,12 +663,123 @@ int deme_cd_sort_remap(deme_ctx* c) {
     return DEME_OK;
 }
 
+// Cell grid for candidate generation: edge = largest interaction distance 2 * (r_max + beta) (slightly padded for
+// the float rounding of r + beta), covering the voxel world.
+static int setup_grid(deme_ctx* c) {
+    if (!std::isfinite(c->maxRadius) || !(c->maxRadius > 0.f))
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, "sphere radii must be positive and finite");
+    double edge = 2.0 * ((double)c->maxRadius + (double)c->sp.beta) * 1.0001;
+    const double wx = c->sp.voxelSize_exact * std::ldexp(1.0, c->sp.nvXp2);
+    const double wy = c->sp.voxelSize_exact * std::ldexp(1.0, c->sp.nvYp2);
+    const double wz = c->sp.voxelSize_exact * std::ldexp(1.0, c->sp.nvZp2);
+    for (;;) {
+        const double nx = std::floor(wx / edge) + 1, ny = std::floor(wy / edge) + 1, nz = std::floor(wz / edge) + 1;
+        if (nx * ny * nz <= (double)(1u << 28)) {
+            c->grid.inv = 1.0 / edge;
+            c->grid.nx = (uint32_t)nx;
+            c->grid.ny = (uint32_t)ny;
+            c->grid.nz = (uint32_t)nz;
+            break;
+        }
+        edge *= 1.25;
+    }
+    const uint64_t nCells = (uint64_t)c->grid.nx * c->grid.ny * c->grid.nz;
+    if (nCells != c->nCells) {
+        DEME_HIP(c, c->cellCount.ensure(nCells, 1.0));
+        DEME_HIP(c, c->cellStart.ensure(nCells + 1, 1.0));
+        DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>({nCells, c->nBins, c->nS, 1u << 22})), 1.0));
+        c->nCells = nCells;
+    }
+    return DEME_OK;
+}
+
 int deme_cd_run(deme_ctx* c) {
-    int r = deme_cd_bin_count(c);
-    if (!r) r = deme_cd_bin_fill(c);
-    if (!r) r = deme_cd_pairs(c);
-    if (!r) r = deme_cd_sort_remap(c);
-    return r;
+    int r = check_ready(c);
+    if (r) return r;
+    r = setup_grid(c);
+    if (r) return r;
+    const uint32_t nS = c->nS;
+    DEME_HIP(c, c->cellOf.ensure(nS));
+    DEME_HIP(c, c->slotOf.ensure(nS));
+    DEME_HIP(c, c->sorted.ensure(nS));
+    DEME_HIP(c, c->planeBits.ensure(nS));
+    DEME_HIP(c, c->slab.ensure((size_t)nS * c->slabCap, 1.0));
+    DEME_HIP(c, hipMemsetAsync(c->ctr.p, 0, sizeof(DevCounters), c->stream));
+    DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
+    DEME_LAUNCH(c, "fill", launch_fill_u32(c->cellCount.p, c->nCells, 0, c->stream));
+    // per-owner B-side counts are accumulated below; clear them whatever the previous pass left
+    DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCount.p, c->nClumps + 1, 0, c->stream));
+    DEME_LAUNCH(c, "cell_assign",
+                launch_cell_assign(c->kp, c->grid, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->mask.p,
+                                   c->cellCount.p, c->cellOf.p, c->slotOf.p, c->numBins.p, c->numAnal.p,
+                                   c->planeBits.p, c->rec.p, c->ctr.p, c->stream));
+    DEME_LAUNCH(c, "scan_cells",
+                scan_u32(c->cellCount.p, nullptr, c->nCells, c->cellStart.p, &c->ctr.p->total[3], c->scanScratch.p,
+                         c->stream));
+    DEME_LAUNCH(c, "cell_scatter",
+                launch_cell_scatter(nS, c->cellOf.p, c->slotOf.p, c->cellStart.p, c->rec.p, c->sorted.p, c->stream));
+    // the current list becomes the previous one
+    if (c->contactsReady) {
+        c->cur ^= 1;
+        c->havePrev = true;
+        c->Cprev = c->C;
+    }
+    c->contactsReady = false;
+    c->cdStage = 0;
+    const int cu = c->cur, pv = cu ^ 1;
+    for (int attempt = 0;; attempt++) {
+        DEME_LAUNCH(c, "cell_pairs",
+                    launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, c->cellStart.p, c->sph.p, c->own.p, c->mask.p,
+                                      c->slabCap, c->slab.p, c->rowCount.p, c->bCount.p, c->ctr.p, c->stream));
+        DEME_LAUNCH(c, "scan_rows",
+                    scan_u32(c->rowCount.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1],
+                             c->scanScratch.p, c->stream));
+        r = readback_counters(c);
+        if (r) return r;
+        const DevCounters& h = *c->hctr;
+        if (h.err) {
+            c->havePrev = false;
+            if (h.err & ERRBIT_CAPACITY)
+                return c->fail(DEME_ERR_CAPACITY, "a sphere touches more than 65535 bins (binsSphereTouches_t)");
+            return c->fail(DEME_ERR_CAPACITY, "a sphere left the bin grid of the simulation world");
+        }
+        if (h.maxOcc <= c->slabCap) break;
+        if (attempt >= 2) return c->fail(DEME_ERR_CAPACITY, "partner rows could not be sized");
+        // a sphere has more partners than the slab row holds: widen the rows and redo the pass
+        c->slabCap = (uint32_t)std::min<uint64_t>(((uint64_t)h.maxOcc * 5 / 4 + 7) & ~7ull, 65535);
+        DEME_HIP(c, c->slab.ensure((size_t)nS * c->slabCap, 1.0));
+        DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCount.p, c->nClumps + 1, 0, c->stream));
+        DEME_HIP(c, hipMemsetAsync(&c->ctr.p->maxOcc, 0, sizeof(uint32_t), c->stream));
+    }
+    const uint64_t C = c->hctr->total[1];
+    if (C > 0xFFFFFFFFull) return c->fail(DEME_ERR_CAPACITY, "contact count exceeds contactPairs_t (2^32 - 1)");
+    c->C = C;
+    DEME_HIP(c, c->idB[cu].ensure(C));
+    DEME_HIP(c, c->ctype[cu].ensure(C));
+    DEME_HIP(c, c->hist[cu].ensure(C));
+    DEME_HIP(c, c->idA.ensure(C));
+    DEME_HIP(c, c->map.ensure(C));
+    DEME_HIP(c, c->frc.ensure(C));
+    DEME_HIP(c, c->bList.ensure(C));
+    DEME_LAUNCH(c, "cell_rows",
+                launch_cell_rows(nS, c->slabCap, c->slab.p, c->rowStart[cu].p, c->numAnal.p, c->planeBits.p, c->idA.p,
+                                 c->idB[cu].p, c->ctype[cu].p, c->stream));
+    DEME_LAUNCH(c, "history_map",
+                launch_history_map(nS, c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p,
+                                   c->havePrev ? c->rowStart[pv].p : nullptr, c->havePrev ? c->idB[pv].p : nullptr,
+                                   c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->stream));
+    DEME_LAUNCH(c, "scan_owners",
+                scan_u32(c->bCount.p, nullptr, c->nClumps, c->bStart.p, &c->ctr.p->total[2], c->scanScratch.p,
+                         c->stream));
+    DEME_LAUNCH(c, "bside_fill",
+                launch_bside_fill((uint32_t)C, c->nS, c->idB[cu].p, c->ctype[cu].p, c->sph.p, c->bStart.p, c->bCount.p,
+                                  c->bList.p, c->stream));
+    DEME_LAUNCH(c, "bside_sort", launch_bside_sort(c->nClumps, c->bStart.p, c->bList.p, c->stream));
+    c->contactsReady = true;
+    c->fresh = true;
+    c->countsStale = true;  // pair / plane split is read back lazily
+    c->cdStage = 0;
+    return DEME_OK;
 }
 
 int deme_download_bin_counts(deme_ctx* c, uint16_t* nb, uint8_t* na) {
@@ -815,6 +951,13 @@ int deme_upload_wildcards(deme_ctx* c, const float* w) {
 
 int deme_get_counts(deme_ctx* c, deme_counts* out) {
     std::memset(out, 0, sizeof(*out));
+    if (c->countsStale) {
+        int r = readback_counters(c);
+        if (r) return r;
+        c->nPairs = c->hctr->total[2];
+        c->nGeo = c->C - c->nPairs;
+        c->countsStale = false;
+    }
     out->nSpheres = c->nS;
     out->nOwners = c->nO;
     out->nAnal = c->nA;

@@ -65,6 +65,7 @@ struct KParams {
     float beta, h;
     float Gx, Gy, Gz;
     uint32_t integrator;
+    uint32_t nS, nA;  // sphere and analytical-component counts (bounds of data-dependent gathers)
 };
 
 // --------------------------------------------------------------------------------------------------------------

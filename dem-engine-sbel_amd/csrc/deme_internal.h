@@ -27,6 +27,13 @@ struct __attribute__((aligned(16))) PlanePose {
 };
 static_assert(sizeof(PlanePose) == 48, "PlanePose must be 48 bytes");
 
+// Uniform cell grid over the simulation world used to generate candidate pairs (edge >= largest interaction
+// distance, so partners lie in the 27 neighbouring cells).
+struct CellGrid {
+    double inv;  // 1 / cell edge
+    uint32_t nx, ny, nz, pad;
+};
+
 // Device-side counters read back by the host after a detection pass.
 struct DevCounters {
     uint32_t pairCount;
@@ -74,9 +81,26 @@ hipError_t launch_row_finalize(uint32_t nS, const uint32_t* rowStart, const uint
 hipError_t launch_history_map(uint32_t nS, const uint32_t* rowStart, const uint32_t* idB, const uint8_t* ctype,
                               const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
                               uint32_t* map, hipStream_t st);
-hipError_t launch_bside_fill(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const SphereRec* sph,
-                             const uint32_t* bStart, uint32_t* bCount, uint32_t* bList, hipStream_t st);
+hipError_t launch_bside_fill(uint32_t nC, uint32_t nS, const uint32_t* idB, const uint8_t* ctype,
+                             const SphereRec* sph, const uint32_t* bStart, uint32_t* bCount, uint32_t* bList,
+                             hipStream_t st);
 hipError_t launch_bside_sort(uint32_t nClumps, const uint32_t* bStart, uint32_t* bList, hipStream_t st);
+
+// cd_fast.hip
+hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereRec* sph, const OwnerRec* own,
+                              uint32_t nS, const PlanePose* pose, uint32_t nA, const uint8_t* mask,
+                              uint32_t* cellCount, uint32_t* cellOf, uint32_t* slotOf, uint16_t* numBins,
+                              uint32_t* numAnal, unsigned long long* planeBits, CdRec* rec, DevCounters* ctr,
+                              hipStream_t st);
+hipError_t launch_cell_scatter(uint32_t nS, const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
+                               const CdRec* rec, CdRec* sorted, hipStream_t st);
+hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
+                             const uint32_t* cellStart, const SphereRec* sph, const OwnerRec* own, const uint8_t* mask,
+                             uint32_t slabCap, uint32_t* slab, uint32_t* rowCount, uint32_t* bCount, DevCounters* ctr,
+                             hipStream_t st);
+hipError_t launch_cell_rows(uint32_t nS, uint32_t slabCap, const uint32_t* slab, const uint32_t* rowStart,
+                            const uint32_t* numAnal, const unsigned long long* planeBits, uint32_t* idA, uint32_t* idB,
+                            uint8_t* ctype, hipStream_t st);
 
 // dyn.hip
 hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, const uint32_t* idB, const uint8_t* ctype,

@@ -77,6 +77,21 @@ __global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uin
     if (i >= nC) return;
     uint32_t type = ctype[i];
     const float ts = P.h;
+    {
+        // guard: ids outside their tables never reach a gather (a malformed list yields a zero record)
+        const uint32_t a = idA[i], b = idB[i];
+        const bool ok = a < P.nS && ((type == DEME_SPHERE_SPHERE_CONTACT && b < P.nS) ||
+                                     (type == DEME_SPHERE_PLANE_CONTACT && b < P.nA));
+        if (!ok) {
+            const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4* o4 = reinterpret_cast<float4*>(out + i);
+            o4[0] = z;
+            o4[1] = z;
+            o4[2] = z;
+            histOut[i] = z;
+            return;
+        }
+    }
 
     // ---- body A (always a sphere) ----
     const SphereRec sa = load_sphere(sph, idA[i]);

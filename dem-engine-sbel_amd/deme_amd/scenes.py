@@ -174,29 +174,39 @@ def clump_template(seed=1002, r0=0.002, offset_frac=0.6, samples=2_000_000):
         inside |= ((pts[:, 0] - c) ** 2 + pts[:, 1] ** 2 + pts[:, 2] ** 2) <= float(rr) ** 2
     vol = np.prod(hi - lo) * inside.mean()
     p = pts[inside]
-    com = p.mean(0)
+    # the union is symmetric about the x axis: centroid on the axis, Iyy = Izz
+    com = np.array([p[:, 0].mean(), 0.0, 0.0])
     q = p - com
     mass = RHO * vol
     Ixx = mass * np.mean(q[:, 1] ** 2 + q[:, 2] ** 2)
-    Iyy = mass * np.mean(q[:, 0] ** 2 + q[:, 2] ** 2)
-    Izz = mass * np.mean(q[:, 0] ** 2 + q[:, 1] ** 2)
-    rel = np.stack([cx - com[0], np.full(3, -com[1]), np.full(3, -com[2])], 1).astype(np.float32)
-    return dict(radii=radii, relpos=rel, mass=np.float32(mass), moi=np.array([Ixx, Iyy, Izz], np.float32))
+    Iyz = mass * np.mean(q[:, 0] ** 2 + 0.5 * (q[:, 1] ** 2 + q[:, 2] ** 2))
+    rel = np.stack([cx - com[0], np.zeros(3), np.zeros(3)], 1).astype(np.float32)
+    return dict(radii=radii, relpos=rel, mass=np.float32(mass), moi=np.array([Ixx, Iyz, Iyz], np.float32))
 
 
 def clump_settling(n_clumps=1_000_000, seed=1002, box=(0.5, 0.5, 1.0), h=5e-6, steps=2000, update_freq=1,
-                   beta_frac=0.0, E=5e7, nu=0.3, CoR=0.6, mu=0.4, Crr=0.0, spacing=(0.0078, 0.0053, 0.0053),
+                   beta_frac=0.0, E=5e7, nu=0.3, CoR=0.6, mu=0.4, Crr=0.0, spacing=None, compaction=0.998,
                    jitter=0.1, velocity_sigma=0.0, omega_sigma=0.0, random_orientation=False):
     """Three-sphere clumps on an orthorhombic lattice settling in a plane box. update_freq > 1 runs with a drift
-    margin beta = beta_frac * smallest radius."""
+    margin beta = beta_frac * smallest radius.
+
+    Default lattice: a compacted bed -- neighbour spacing = compaction x the clumps' contact distance along each
+    axis, bottom layer resting on the floor -- so the contact path is loaded from the first step (a loose lattice
+    would free-fall for the whole run)."""
     tpl = clump_template(seed=seed)
     rng = np.random.default_rng(seed + 17)
     box = tuple(float(f32(b)) for b in box)
     lbf = tuple(float(f32(-f32(b) / f32(2.0))) for b in box)
-    half_x = float(np.abs(tpl["relpos"][:, 0]).max() + tpl["radii"].max())
+    xs = tpl["relpos"][:, 0]
+    x_lo = float((xs - tpl["radii"]).min())
+    x_hi = float((xs + tpl["radii"]).max())
+    half_x = max(-x_lo, x_hi)
     half_yz = float(tpl["radii"].max())
+    if spacing is None:
+        spacing = ((x_hi - x_lo) * compaction, 2 * half_yz * compaction, 2 * half_yz * compaction)
     sx, sy, sz = spacing
-    margin = max(half_x, half_yz) + 0.0004
+    margin = max(half_x, half_yz) + 0.0002
+    z_floor = half_yz * compaction
     nx = int(np.floor((box[0] - 2 * margin) / sx)) + 1
     ny = int(np.floor((box[1] - 2 * margin) / sy)) + 1
     nz = int(np.ceil(n_clumps / (nx * ny)))
@@ -205,10 +215,10 @@ def clump_settling(n_clumps=1_000_000, seed=1002, box=(0.5, 0.5, 1.0), h=5e-6, s
     i, j, k = ii.ravel()[order][:n_clumps], jj.ravel()[order][:n_clumps], kk.ravel()[order][:n_clumps]
     x0 = lbf[0] + (box[0] - (nx - 1) * sx) / 2
     y0 = lbf[1] + (box[1] - (ny - 1) * sy) / 2
-    z0 = lbf[2] + margin
+    z0 = lbf[2] + z_floor
     xyz = np.stack([x0 + i * sx, y0 + j * sy, z0 + k * sz], 1)
     r_mean = float(tpl["radii"].mean())
-    xyz += rng.uniform(-jitter * 0.2 * r_mean, jitter * 0.2 * r_mean, size=xyz.shape)
+    xyz += rng.uniform(-jitter * 0.02 * r_mean, jitter * 0.02 * r_mean, size=xyz.shape)
     if xyz[:, 2].max() + half_yz > lbf[2] + box[2]:
         raise ValueError("lattice does not fit the box")
     rmin = float(tpl["radii"].min())

@@ -90,6 +90,27 @@ def test_contacts_and_history_map_bit_exact(make):
     assert total > 0
 
 
+@pytest.mark.parametrize("make", SCENES)
+def test_staged_bin_path_matches_cell_path_and_oracle(make):
+    """The per-bin stages (count, scan+fill, per-bin pairs, sort+remap) and the cell-list detection give the same
+    canonical contact list, equal to the oracle's."""
+    sc, dev, ora = systems(make, pre_steps=10)
+    state = dev.get_state()
+    ora.set_state(state)
+    ora.cd_run()
+    oa, ob, ot, _ = ora.contacts()
+    dev.cd_bin_count()
+    dev.cd_bin_fill()
+    dev.cd_pairs()
+    dev.cd_sort_remap()
+    sa, sb, st, _ = dev.contacts()
+    dev.set_state(state)
+    dev.cd_run()
+    fa, fb, ft, _ = dev.contacts()
+    for x, y in ((sa, oa), (sb, ob), (st, ot), (fa, oa), (fb, ob), (ft, ot)):
+        assert np.array_equal(x, y)
+
+
 def aligned_force_pass(make):
     sc, dev, ora = systems(make)
     ora.step(15, 1)
@@ -183,16 +204,25 @@ def test_trajectory_parity(make, freq):
 
 
 def test_bin_overflow_is_reported_not_trapped():
-    sc = scenes.sphere_settling(n=3000, spacing=1.98)
-    # a bin as large as the box puts every sphere in a handful of bins (> 256 per bin)
+    """More than 256 spheres in one bin: the staged per-bin path reports DEME_ERR_BIN_OVERFLOW like the reference
+    (DEMContactKernels_Blockwise.cu:42-45, minus the trap); the cell-list path has no per-bin capacity and returns
+    the brute-force contact set."""
+    import pyoracle
     from deme_amd.abi import build_sim_params
 
+    sc = scenes.sphere_settling(n=3000, spacing=1.98)
     p = sc.params
     sc.params = build_sim_params((0.3, 0.3, 0.6), tuple(p.LBF), 0.005, p.h, bin_size=0.2)
     dev = DeviceSystem(sc)
+    dev.cd_bin_count()
     with pytest.raises(DemeError) as e:
-        dev.cd_run()
+        dev.cd_bin_fill()
     assert e.value.code == DEME_ERR_BIN_OVERFLOW
     # the context stays usable after the error
-    with pytest.raises(DemeError):
-        dev.cd_run()
+    dev.cd_run()
+    a, b, t, _ = dev.contacts()
+    ora = pyoracle.OracleSystem(sc)
+    ba, bb = ora.brute_force_pairs()
+    got = set(zip(a[t == 1].tolist(), b[t == 1].tolist()))
+    assert got == set(zip(ba.tolist(), bb.tolist()))
+    assert len(got) > 0
