@@ -25,7 +25,7 @@ HBM_PEAK_GBPS = 8000.0
 
 def bytes_model(name, k):
     """Algorithmic bytes per launch of each kernel (counts k of the step it ran in)."""
-    S, O, P, C, Np, Ba, nb = k["S"], k["O"], k["P"], k["C"], k["pairs"], k["Ba"], k["nbins"]
+    S, O, P, C, Np, Ba, nb, Nc = k["S"], k["O"], k["P"], k["C"], k["pairs"], k["Ba"], k["nbins"], k["cells"]
     owner_cd = 8 + 6 + 16 + 1          # voxel id, sub-voxel position, quaternion, family
     owner_dyn = 8 + 6 + 16 + 12 + 12 + 4  # + velocity, angular velocity, mass
     sph = 4 + 4 + 12 + 4               # owner, radius, offset, material
@@ -49,6 +49,17 @@ def bytes_model(name, k):
         # per contact: ids + type, history in/out, force record out; owner and sphere records (unique footprint)
         "force": C * (9 + 16 + 16 + 48) + S * sph + O * owner_dyn + (C * 4 if k.get("fresh") else 0),
         "anal_collect": C * 5 + k["geo"] * (4 + 48),
+        # cell-list detection: per sphere template + owner pose in; CD record, cell, slot, bin count, plane count
+        # and plane bits out; one 4-byte RMW per occupied cell
+        "cell_assign": S * sph + O * owner_cd + S * (32 + 4 + 4 + 2 + 4 + 8) + Nc * 8,
+        "scan_cells": Nc * 8,
+        "cell_scatter": S * (4 + 4 + 32 + 32) + Nc * 4,
+        # each CD record and cell boundary once, the lane's owner/family, rowCount out; per accepted partner a slab
+        # entry, its owner/family and a 4-byte RMW of the owner's B-side count
+        "cell_pairs": S * (32 + 4 + 4) + O * 4 + Nc * 4 + Np * (4 + 4 + 8),
+        "cell_rows": S * (4 + 8) + Np * 4 + C * 9,
+        "fill": (Nc + O + 1) * 4 / 2,  # two launches per pass: cell counts and owner B-side counts
+        "anal_final": k["anal_blocks"] * 48 * 4,
         # per owner: own record in/out, MOI, accelerations out; A rows contiguous, B lists gathered
         "owner_update": O * (64 + 64 + 16 + 24 + 8) + C * 48 + Np * (4 + 48),
     }
@@ -80,7 +91,8 @@ def run_device(sc, steps, warmup, update_freq):
     c = dev.counts()
     info = dict(S=sc.n_spheres, O=sc.n_owners, P=c.nBinSphereTouches, C=c.nContacts, Cprev=c.nPrevContacts,
                 pairs=c.nContacts - c.nSphereGeoContacts, geo=c.nSphereGeoContacts, Ba=c.nActiveBins,
-                nbins=sc.params.nbX * sc.params.nbY * sc.params.nbZ, fresh=update_freq == 1)
+                nbins=sc.params.nbX * sc.params.nbY * sc.params.nbZ, cells=c.nCells, fresh=update_freq == 1,
+                anal_blocks=(c.nContacts + 4095) // 4096)
     dev.close()
     return t1 - t0, timing, info
 
@@ -103,15 +115,27 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--clumps", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-run-b", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the workload, copy a device buffer of this size once (a known-size reference "
+                         "kernel for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+        print(json.dumps({"error": "this benchmark runs on one GPU only; sharding over several GPUs is not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    if args.copy_probe_mib > 0:
+        import torch
+
+        n = args.copy_probe_mib << 18  # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     scA = build_scene(args.clumps, 1, 0.0)
     tA, timingA, infoA = run_device(scA, args.steps, args.warmup, 1)
@@ -121,8 +145,12 @@ def main():
     if not args.skip_run_b:
         scB = build_scene(args.clumps, 5, 0.2)
         tB, timingB, infoB = run_device(scB, args.steps, args.warmup, 5)
+        totB = sum(v[0] for v in timingB.values())
         runB = {"clump_steps_per_s": args.clumps * args.steps / tB, "ms_per_step": 1e3 * tB / args.steps,
-                "contacts": infoB["C"], "beta_m": scB.params.beta}
+                "contacts": infoB["C"], "beta_m": scB.params.beta,
+                "kernels_us_per_step": {k: round(1e3 * v[0] / args.steps, 2)
+                                        for k, v in sorted(timingB.items(), key=lambda kv: -kv[1][0])},
+                "device_ms_per_step": round(totB / args.steps, 4)}
 
     # dominant kernel of the headline run
     dom = max(timingA.items(), key=lambda kv: kv[1][0])
@@ -130,13 +158,13 @@ def main():
     avg_ms = ms_total / max(launches, 1)
     bytes_per = bytes_model(name, infoA)
     achieved = (bytes_per / (avg_ms * 1e-3) / 1e9) if bytes_per else None
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a known-size copy)
     traffic = None
-    pmc_path = os.path.join(ROOT, "profiles", f"pmc_{name}_clump_settling_1M.json")
+    pmc_path = os.path.join(ROOT, "profiles", "hbm_traffic_clump_settling_1M.json")
     if os.path.exists(pmc_path):
-        try:
-            traffic = json.load(open(pmc_path)).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+        entry = json.load(open(pmc_path)).get("kernels", {}).get(name)
+        if entry:
+            traffic = entry["hbm_bytes_per_launch"]
     kernels = {k: {"avg_us": round(1e3 * v[0] / max(v[1], 1), 2), "launches": v[1],
                    "share": round(v[0] / sum(x[0] for x in timingA.values()), 4),
                    "GBps": (round(bytes_model(k, infoA) / (v[0] / max(v[1], 1) * 1e-3) / 1e9, 1)

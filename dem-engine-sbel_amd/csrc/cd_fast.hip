@@ -76,7 +76,8 @@ __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, cons
                                                      uint32_t* __restrict__ cellOf, uint32_t* __restrict__ slotOf,
                                                      uint16_t* __restrict__ numBins, uint32_t* __restrict__ numAnal,
                                                      unsigned long long* __restrict__ planeBits,
-                                                     CdRec* __restrict__ rec, DevCounters* ctr) {
+                                                     CdRec* __restrict__ rec, uint2* __restrict__ meta,
+                                                     DevCounters* ctr) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
     const SphereRec sr = load_sphere(sph, s);
@@ -91,12 +92,18 @@ __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, cons
     if ((uint64_t)nx * ny * nz > 65535u) {
         atomicOr(&ctr->err, ERRBIT_CAPACITY);
     } else {
+        // The linear id grows with each of i, j, k, so without 32-bit wrap the range's largest id is its last corner.
         const uint32_t nbXY = P.nbX * P.nbY;
-        bool outside = false;
-        for (uint32_t dk = 0; dk < nz; dk++)
-            for (uint32_t dj = 0; dj < ny; dj++)
-                for (uint32_t di = 0; di < nx; di++)
-                    outside |= ((r.x0 + di) + (r.y0 + dj) * P.nbX + (r.z0 + dk) * nbXY) >= P.nBins;
+        const uint64_t last = (uint64_t)r.x1 + (uint64_t)r.y1 * P.nbX + (uint64_t)r.z1 * nbXY;
+        bool outside = last >= P.nBins;
+        if (last > 0xFFFFFFFFull) {
+            // ids wrap: enumerate them as the reference forms them
+            outside = false;
+            for (uint32_t dk = 0; dk < nz; dk++)
+                for (uint32_t dj = 0; dj < ny; dj++)
+                    for (uint32_t di = 0; di < nx; di++)
+                        outside |= ((r.x0 + di) + (r.y0 + dj) * P.nbX + (r.z0 + dk) * nbXY) >= P.nBins;
+        }
         if (outside) atomicOr(&ctr->err, ERRBIT_OUTSIDE);
     }
     uint32_t cnt = 0;
@@ -123,33 +130,49 @@ __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, cons
     cr.r = sr.r;  // raw radius: the bin rule needs both (double)r + beta and float(r + beta)
     cr.id = s;
     rec[s] = cr;
+    meta[s] = make_uint2(sr.owner, o.fam);
 }
 
 __global__ __launch_bounds__(256) void k_cell_scatter(uint32_t nS, const uint32_t* __restrict__ cellOf,
                                                       const uint32_t* __restrict__ slotOf,
                                                       const uint32_t* __restrict__ cellStart,
-                                                      const CdRec* __restrict__ rec, CdRec* __restrict__ sorted) {
+                                                      const CdRec* __restrict__ rec, const uint2* __restrict__ meta,
+                                                      CdRec* __restrict__ sorted, uint2* __restrict__ sortedMeta,
+                                                      uint32_t* __restrict__ posOf) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
     const CdRec cr = load_cd(rec, s);
-    CdRec* dst = sorted + cellStart[cellOf[s]] + slotOf[s];
+    const uint32_t pos = cellStart[cellOf[s]] + slotOf[s];
+    posOf[s] = pos;
+    sortedMeta[pos] = meta[s];
+    CdRec* dst = sorted + pos;
     uint4* d4 = reinterpret_cast<uint4*>(dst);
     const uint4* s4 = reinterpret_cast<const uint4*>(&cr);
     d4[0] = s4[0];
     d4[1] = s4[1];
 }
 
-// The reference's bin rule for A (lower id) and B: expanded overlap (float radii r + beta), contact point
-// (checkSpheresOverlap<double>, DEMHelperKernels.cu:195-222), and the contact point's bin inside both AABB ranges.
-__device__ __forceinline__ bool bin_rule(const KParams& P, const CdRec& A, const CdRec& B) {
-    float rfa = A.r;
-    rfa += P.beta;
-    float rfb = B.r;
-    rfb += P.beta;
-    const double ra = (double)rfa, rb = (double)rfb;
+// Expanded radius as the reference's detection kernel forms it: float(r + beta), widened to double.
+__device__ __forceinline__ double expanded_radius(const KParams& P, float r) {
+    float rf = r;
+    rf += P.beta;
+    return (double)rf;
+}
+
+// First half of the reference's sphere-pair test (checkSpheresOverlap<double>, DEMHelperKernels.cu:195-205): the
+// expanded spheres overlap.
+__device__ __forceinline__ bool expanded_overlap(double ra, const CdRec& A, double rb, const CdRec& B) {
     const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
     const double d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 > (ra + rb) * (ra + rb)) return false;
+    return !(d2 > (ra + rb) * (ra + rb));
+}
+
+// Second half, for a pair that overlaps: the contact point (DEMHelperKernels.cu:206-222) computed with A the lower id,
+// and its bin inside both spheres' AABB bin ranges (ga: A's range, precomputed by the caller).
+__device__ __forceinline__ bool bin_rule_point(const KParams& P, const CdRec& A, const BinRange& ga, const CdRec& B) {
+    const double ra = expanded_radius(P, A.r), rb = expanded_radius(P, B.r);
+    const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+    const double d2 = dx * dx + dy * dy + dz * dz;
     float nX = (float)dx, nY = (float)dy, nZ = (float)dz;
     const float mag = sqrtf(nX * nX + nY * nY + nZ * nZ);
     nX /= mag;
@@ -160,11 +183,8 @@ __device__ __forceinline__ bool bin_rule(const KParams& P, const CdRec& A, const
     const double cy = B.y + (rb - half) * (double)nY;
     const double cz = B.z + (rb - half) * (double)nZ;
     const uint32_t ix = sat_u32(cx / P.binSize), iy = sat_u32(cy / P.binSize), iz = sat_u32(cz / P.binSize);
-    double rcdA = (double)A.r;
-    rcdA += (double)P.beta;
     double rcdB = (double)B.r;
     rcdB += (double)P.beta;
-    const BinRange ga = bin_range(P, A.x, A.y, A.z, rcdA);
     const BinRange gb = bin_range(P, B.x, B.y, B.z, rcdB);
     if (range_in_grid(P, ga) && range_in_grid(P, gb)) {
         // no aliasing: a linear id is one (i, j, k) -- unless the point itself lies past the grid, where the
@@ -178,50 +198,168 @@ __device__ __forceinline__ bool bin_rule(const KParams& P, const CdRec& A, const
 }
 
 // One lane per sphere in cell order: scan 27 neighbour cells, keep partners with larger ids.
-__global__ __launch_bounds__(256) void k_cell_pairs(KParams P, CellGrid G, uint32_t nS,
-                                                    const CdRec* __restrict__ sorted,
-                                                    const uint32_t* __restrict__ cellStart,
-                                                    const SphereRec* __restrict__ sph,
-                                                    const OwnerRec* __restrict__ own,
-                                                    const uint8_t* __restrict__ mask, uint32_t slabCap,
-                                                    uint32_t* __restrict__ slab, uint32_t* __restrict__ rowCount,
-                                                    uint32_t* bCount, DevCounters* ctr) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    // a flagged assignment error (oversized or out-of-world ranges) aborts the pass before any range is walked
-    if (i >= nS || ctr->err) return;
-    const CdRec me = load_cd(sorted, i);
-    int cx, cy, cz;
-    cell_of(G, me.x, me.y, me.z, cx, cy, cz);
-    const uint32_t myOwner = sph[me.id].owner;
-    const uint32_t myFam = own[myOwner].fam;
-    uint32_t n = 0;
-    uint32_t* row = slab + (uint64_t)me.id * slabCap;
+//
+// Cells are numbered x fastest, so for a block of consecutive sorted spheres (cells cLo..cHi) the candidates of all
+// its lanes for one (dy, dz) neighbour row are the sorted range of cells cLo - 1 + off .. cHi + 1 + off, off =
+// dy * nx + dz * nx * ny: nine contiguous segments. The block stages them in LDS as block-relative float positions
+// and sweeps its lanes' 27-cell windows there.
+//
+// Two passes keep a wave's lanes in step. The sweep runs only a single-precision prefilter, conservative by
+// CellGrid::tol and the largest expanded radius, and parks the survivors (sorted-array indices) in the lane's slab
+// column: entry k of sorted sphere i sits at slab[k * nS + i], so a wave's k-th entries share cache lines.
+// The second pass re-tests each survivor with the reference's double-precision overlap and contact-point bin rule
+// and compacts the row in place into accepted partner ids. A row whose survivors exceed the slab is reported through
+// maxOcc (an upper bound of its partners) and the pass is redone with wider rows. A block whose segments exceed the
+// LDS budget sweeps global memory with the exact test instead.
+constexpr uint32_t PAIR_BLOCK = 128;
+constexpr uint32_t PAIR_LDS_RECS = 1536;
+
+__global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G, uint32_t nS, uint32_t nBlocks,
+                                                           const CdRec* __restrict__ sorted,
+                                                           const uint2* __restrict__ sortedMeta,
+                                                           const uint32_t* __restrict__ cellStart,
+                                                           const uint8_t* __restrict__ mask, uint32_t slabCap,
+                                                           uint32_t* __restrict__ slab,
+                                                           uint32_t* __restrict__ pairCnt, uint32_t* bCount,
+                                                           DevCounters* ctr) {
+    __shared__ float4 recs[PAIR_LDS_RECS];  // x, y, z relative to the block origin; id bits
+    __shared__ uint32_t segGlob[9], segLen[9], segBase[9];
+    __shared__ uint32_t cellLo, cellHi;
+    __shared__ double org[3];
+    // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so XCD k gets a contiguous share of the
+    // sphere range and the rows its blocks share stay in its L2
+    const uint32_t per = (nBlocks + 7) / 8;
+    const uint32_t lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    // a flagged assignment error (oversized or out-of-world ranges) aborts the pass; both tests are block-uniform
+    if (lb >= nBlocks || ctr->err) return;
+    const uint32_t i0 = lb * PAIR_BLOCK;
+    const uint32_t iEnd = min(i0 + PAIR_BLOCK, nS);
+    const uint32_t i = i0 + threadIdx.x;
+    const bool valid = i < iEnd;
+    CdRec me;
+    int cx = 0, cy = 0, cz = 0;
+    if (valid) {
+        me = load_cd(sorted, i);
+        cell_of(G, me.x, me.y, me.z, cx, cy, cz);
+        const uint32_t myCell = (uint32_t)cx + (uint32_t)cy * G.nx + (uint32_t)cz * G.nx * G.ny;
+        // sorted by cell: the block's cells are those of its first and last sphere and everything between
+        if (i == i0) {
+            cellLo = myCell;
+            org[0] = me.x;
+            org[1] = me.y;
+            org[2] = me.z;
+        }
+        if (i == iEnd - 1) cellHi = myCell;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        const int64_t dz = (int64_t)(threadIdx.x / 3) - 1, dy = (int64_t)(threadIdx.x % 3) - 1;
+        const int64_t off = dy * G.nx + dz * (int64_t)G.nx * G.ny;
+        const int64_t lo = max((int64_t)cellLo + off - 1, (int64_t)0);
+        const int64_t hi = min((int64_t)cellHi + off + 1, (int64_t)G.nCells - 1);
+        uint32_t gs = 0, ge = 0;
+        if (lo <= hi) {
+            gs = cellStart[lo];
+            ge = cellStart[hi + 1];
+        }
+        segGlob[threadIdx.x] = gs;
+        segLen[threadIdx.x] = ge - gs;
+    }
+    __syncthreads();
+    uint32_t total = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) total += segLen[k];
+    const bool staged = total <= PAIR_LDS_RECS;  // block-uniform
+    if (staged) {
+        const double ox = org[0], oy = org[1], oz = org[2];
+        uint32_t base = 0;
+        for (int sg = 0; sg < 9; sg++) {
+            const uint32_t len = segLen[sg], g = segGlob[sg];
+            if (threadIdx.x == 0) segBase[sg] = base;
+            for (uint32_t k = threadIdx.x; k < len; k += PAIR_BLOCK) {
+                const CdRec r = load_cd(sorted, g + k);
+                recs[base + k] =
+                    make_float4((float)(r.x - ox), (float)(r.y - oy), (float)(r.z - oz), __uint_as_float(r.id));
+            }
+            base += len;
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    const double ra = expanded_radius(P, me.r);
+    uint32_t* row = slab + i;  // entry k at row[k * nS]
+    uint32_t nc = 0;
     const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < (int)G.nz - 1 ? cz + 1 : cz;
     const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < (int)G.ny - 1 ? cy + 1 : cy;
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < (int)G.nx - 1 ? cx + 1 : cx;
-    for (int z = z0; z <= z1; z++)
-        for (int y = y0; y <= y1; y++) {
-            // cells x0..x1 of one (y, z) row are contiguous in the sorted array
-            const uint32_t c0 = (uint32_t)x0 + (uint32_t)y * G.nx + (uint32_t)z * G.nx * G.ny;
-            const uint32_t b = cellStart[c0], e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
-            for (uint32_t j = b; j < e; j++) {
-                const CdRec ot = load_cd(sorted, j);
-                if (ot.id <= me.id) continue;
-                if (!bin_rule(P, me, ot)) continue;
-                const uint32_t ob = sph[ot.id].owner;
-                if (ob == myOwner) continue;
-                if (mask[mask_index(myFam, own[ob].fam)] != DEME_DONT_PREVENT_CONTACT) continue;
-                if (n < slabCap) row[n] = ot.id;
-                n++;
-                atomicAdd(&bCount[ob], 1u);
+    if (staged) {
+        const float px = (float)(me.x - org[0]), py = (float)(me.y - org[1]), pz = (float)(me.z - org[2]);
+        float R = (float)ra;
+        R += G.rfMax;
+        R += G.tol;
+        const float thr = R * R;
+        for (int z = z0; z <= z1; z++)
+            for (int y = y0; y <= y1; y++) {
+                const uint32_t c0 = (uint32_t)x0 + (uint32_t)y * G.nx + (uint32_t)z * G.nx * G.ny;
+                const uint32_t b = cellStart[c0], e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
+                const int sgi = (z - cz + 1) * 3 + (y - cy + 1);
+                // this row's cells lie inside segment sgi, so b..e index into its staged records
+                const uint32_t base = segBase[sgi] - segGlob[sgi];  // modulo 2^32: base + j = segBase + (j - segGlob)
+                for (uint32_t j = b; j < e; j++) {
+                    const float4 q = recs[base + j];
+                    if (__float_as_uint(q.w) <= me.id) continue;
+                    const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+                    if (dx * dx + dy * dy + dz * dz > thr) continue;
+                    if (nc < slabCap) row[(uint64_t)nc * nS] = j;
+                    nc++;
+                }
             }
+    } else {
+        for (int z = z0; z <= z1; z++)
+            for (int y = y0; y <= y1; y++) {
+                const uint32_t c0 = (uint32_t)x0 + (uint32_t)y * G.nx + (uint32_t)z * G.nx * G.ny;
+                const uint32_t b = cellStart[c0], e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
+                for (uint32_t j = b; j < e; j++) {
+                    const CdRec ot = load_cd(sorted, j);
+                    if (ot.id <= me.id) continue;
+                    if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) continue;
+                    if (nc < slabCap) row[(uint64_t)nc * nS] = j;
+                    nc++;
+                }
+            }
+    }
+    if (nc > slabCap) {
+        pairCnt[i] = nc;
+        atomicMax(&ctr->maxOcc, nc);
+        return;
+    }
+    uint32_t n = 0;
+    if (nc) {
+        double rcdA = (double)me.r;
+        rcdA += (double)P.beta;
+        const BinRange ga = bin_range(P, me.x, me.y, me.z, rcdA);
+        const uint2 mine = sortedMeta[i];  // owner, family
+        for (uint32_t k = 0; k < nc; k++) {
+            const uint32_t j = row[(uint64_t)k * nS];
+            const CdRec ot = load_cd(sorted, j);
+            if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) continue;
+            if (!bin_rule_point(P, me, ga, ot)) continue;
+            const uint2 theirs = sortedMeta[j];
+            const uint32_t ob = theirs.x;
+            if (ob == mine.x) continue;
+            if (mask[mask_index(mine.y, theirs.y)] != DEME_DONT_PREVENT_CONTACT) continue;
+            row[(uint64_t)n * nS] = ot.id;  // n <= k: compaction never overwrites an unread survivor
+            n++;
+            atomicAdd(&bCount[ob], 1u);
         }
-    rowCount[me.id] = n;
-    if (n > slabCap) atomicMax(&ctr->maxOcc, n);
+    }
+    pairCnt[i] = n;
 }
 
 // Canonical rows: plane contacts (ascending plane index) then partners (ascending id); idA for the row.
-__global__ __launch_bounds__(256) void k_cell_rows(uint32_t nS, uint32_t slabCap, const uint32_t* __restrict__ slab,
+__global__ __launch_bounds__(256) void k_cell_rows(uint32_t nS, const uint32_t* __restrict__ slab,
+                                                   const uint32_t* __restrict__ posOf,
                                                    const uint32_t* __restrict__ rowStart,
                                                    const uint32_t* __restrict__ numAnal,
                                                    const unsigned long long* __restrict__ planeBits,
@@ -240,9 +378,9 @@ __global__ __launch_bounds__(256) void k_cell_rows(uint32_t nS, uint32_t slabCap
         pos++;
     }
     const uint32_t sp = pos;
-    const uint32_t* src = slab + (uint64_t)s * slabCap;
+    const uint32_t* src = slab + posOf[s];  // the sphere's slab column (sorted position)
     for (uint32_t k = 0; sp + k < en; k++) {
-        const uint32_t v = src[k];
+        const uint32_t v = src[(uint64_t)k * nS];
         uint32_t q = sp + k;
         while (q > sp && idB[q - 1] > v) {
             idB[q] = idB[q - 1];
@@ -261,36 +399,38 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereRec* sph, const OwnerRec* own,
                               uint32_t nS, const PlanePose* pose, uint32_t nA, const uint8_t* mask,
                               uint32_t* cellCount, uint32_t* cellOf, uint32_t* slotOf, uint16_t* numBins,
-                              uint32_t* numAnal, unsigned long long* planeBits, CdRec* rec, DevCounters* ctr,
-                              hipStream_t st) {
+                              uint32_t* numAnal, unsigned long long* planeBits, CdRec* rec, uint2* meta,
+                              DevCounters* ctr, hipStream_t st) {
     if (nS == 0) return hipSuccess;
     k_cell_assign<<<blocks(nS, 256), 256, 0, st>>>(P, G, sph, own, nS, pose, nA, mask, cellCount, cellOf, slotOf,
-                                                   numBins, numAnal, planeBits, rec, ctr);
+                                                   numBins, numAnal, planeBits, rec, meta, ctr);
     return hipGetLastError();
 }
 
 hipError_t launch_cell_scatter(uint32_t nS, const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
-                               const CdRec* rec, CdRec* sorted, hipStream_t st) {
+                               const CdRec* rec, const uint2* meta, CdRec* sorted, uint2* sortedMeta, uint32_t* posOf,
+                               hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cell_scatter<<<blocks(nS, 256), 256, 0, st>>>(nS, cellOf, slotOf, cellStart, rec, sorted);
+    k_cell_scatter<<<blocks(nS, 256), 256, 0, st>>>(nS, cellOf, slotOf, cellStart, rec, meta, sorted, sortedMeta,
+                                                    posOf);
     return hipGetLastError();
 }
 
 hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
-                             const uint32_t* cellStart, const SphereRec* sph, const OwnerRec* own, const uint8_t* mask,
-                             uint32_t slabCap, uint32_t* slab, uint32_t* rowCount, uint32_t* bCount, DevCounters* ctr,
-                             hipStream_t st) {
+                             const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
+                             uint32_t* slab, uint32_t* pairCnt, uint32_t* bCount, DevCounters* ctr, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cell_pairs<<<blocks(nS, 256), 256, 0, st>>>(P, G, nS, sorted, cellStart, sph, own, mask, slabCap, slab, rowCount,
-                                                  bCount, ctr);
+    const uint32_t nb = blocks(nS, PAIR_BLOCK);
+    k_cell_pairs<<<8 * ((nb + 7) / 8), PAIR_BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedMeta, cellStart, mask, slabCap,
+                                                            slab, pairCnt, bCount, ctr);
     return hipGetLastError();
 }
 
-hipError_t launch_cell_rows(uint32_t nS, uint32_t slabCap, const uint32_t* slab, const uint32_t* rowStart,
+hipError_t launch_cell_rows(uint32_t nS, const uint32_t* slab, const uint32_t* posOf, const uint32_t* rowStart,
                             const uint32_t* numAnal, const unsigned long long* planeBits, uint32_t* idA, uint32_t* idB,
                             uint8_t* ctype, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cell_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, slabCap, slab, rowStart, numAnal, planeBits, idA, idB, ctype);
+    k_cell_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, slab, posOf, rowStart, numAnal, planeBits, idA, idB, ctype);
     return hipGetLastError();
 }
 

@@ -104,8 +104,9 @@ struct deme_ctx {
     float maxRadius = 0.f;
     uint32_t slabCap = 32;
     bool countsStale = false;
-    DBuf<uint32_t> cellCount, cellStart, cellOf, slotOf, slab;
+    DBuf<uint32_t> cellCount, cellStart, cellOf, slotOf, posOf, pairCnt, slab;
     DBuf<CdRec> sorted;
+    DBuf<uint2> meta, sortedMeta;
     DBuf<unsigned long long> planeBits;
     int cur = 0;
     bool havePrev = false, contactsReady = false, fresh = false;
@@ -284,6 +285,9 @@ int deme_ctx_destroy(deme_ctx* c) {
         c->rowStart[k].release(); c->idB[k].release(); c->ctype[k].release(); c->hist[k].release();
     }
     c->idA.release(); c->map.release(); c->frc.release(); c->analPartials.release(); c->analAcc.release();
+    c->cellCount.release(); c->cellStart.release(); c->cellOf.release(); c->slotOf.release(); c->posOf.release();
+    c->slab.release(); c->sorted.release(); c->planeBits.release(); c->pairCnt.release(); c->meta.release();
+    c->sortedMeta.release();
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -679,9 +683,20 @@ static int setup_grid(deme_ctx* c) {
             c->grid.nx = (uint32_t)nx;
             c->grid.ny = (uint32_t)ny;
             c->grid.nz = (uint32_t)nz;
+            c->grid.nCells = (uint32_t)(nx * ny * nz);
             break;
         }
         edge *= 1.25;
+    }
+    {
+        // Prefilter slack: block-relative float coordinates carry at most ~2.1e-7 * extent + 1.8e-7 * distance of
+        // rounding error in a pair distance; 4e-6 * (extent + cutoff) leaves a wide margin, also for spheres
+        // sticking out of the voxel world by their clump offsets.
+        float rf = c->maxRadius;
+        rf += c->sp.beta;
+        c->grid.rfMax = rf;
+        const double extent = std::max({wx, wy, wz});
+        c->grid.tol = (float)(4e-6 * (extent + 2.0 * (double)rf));
     }
     const uint64_t nCells = (uint64_t)c->grid.nx * c->grid.ny * c->grid.nz;
     if (nCells != c->nCells) {
@@ -701,6 +716,10 @@ int deme_cd_run(deme_ctx* c) {
     const uint32_t nS = c->nS;
     DEME_HIP(c, c->cellOf.ensure(nS));
     DEME_HIP(c, c->slotOf.ensure(nS));
+    DEME_HIP(c, c->posOf.ensure(nS));
+    DEME_HIP(c, c->pairCnt.ensure(nS));
+    DEME_HIP(c, c->meta.ensure(nS));
+    DEME_HIP(c, c->sortedMeta.ensure(nS));
     DEME_HIP(c, c->sorted.ensure(nS));
     DEME_HIP(c, c->planeBits.ensure(nS));
     DEME_HIP(c, c->slab.ensure((size_t)nS * c->slabCap, 1.0));
@@ -712,12 +731,13 @@ int deme_cd_run(deme_ctx* c) {
     DEME_LAUNCH(c, "cell_assign",
                 launch_cell_assign(c->kp, c->grid, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->mask.p,
                                    c->cellCount.p, c->cellOf.p, c->slotOf.p, c->numBins.p, c->numAnal.p,
-                                   c->planeBits.p, c->rec.p, c->ctr.p, c->stream));
+                                   c->planeBits.p, c->rec.p, c->meta.p, c->ctr.p, c->stream));
     DEME_LAUNCH(c, "scan_cells",
                 scan_u32(c->cellCount.p, nullptr, c->nCells, c->cellStart.p, &c->ctr.p->total[3], c->scanScratch.p,
                          c->stream));
     DEME_LAUNCH(c, "cell_scatter",
-                launch_cell_scatter(nS, c->cellOf.p, c->slotOf.p, c->cellStart.p, c->rec.p, c->sorted.p, c->stream));
+                launch_cell_scatter(nS, c->cellOf.p, c->slotOf.p, c->cellStart.p, c->rec.p, c->meta.p, c->sorted.p,
+                                    c->sortedMeta.p, c->posOf.p, c->stream));
     // the current list becomes the previous one
     if (c->contactsReady) {
         c->cur ^= 1;
@@ -729,11 +749,12 @@ int deme_cd_run(deme_ctx* c) {
     const int cu = c->cur, pv = cu ^ 1;
     for (int attempt = 0;; attempt++) {
         DEME_LAUNCH(c, "cell_pairs",
-                    launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, c->cellStart.p, c->sph.p, c->own.p, c->mask.p,
-                                      c->slabCap, c->slab.p, c->rowCount.p, c->bCount.p, c->ctr.p, c->stream));
+                    launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, c->sortedMeta.p, c->cellStart.p, c->mask.p,
+                                      c->slabCap, c->slab.p, c->pairCnt.p, c->bCount.p, c->ctr.p, c->stream));
+        // row lengths in id order: partner counts gathered from sorted positions, plus plane contacts
         DEME_LAUNCH(c, "scan_rows",
-                    scan_u32(c->rowCount.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1],
-                             c->scanScratch.p, c->stream));
+                    scan_u32(c->pairCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1],
+                             c->scanScratch.p, c->stream, c->posOf.p));
         r = readback_counters(c);
         if (r) return r;
         const DevCounters& h = *c->hctr;
@@ -762,7 +783,7 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->frc.ensure(C));
     DEME_HIP(c, c->bList.ensure(C));
     DEME_LAUNCH(c, "cell_rows",
-                launch_cell_rows(nS, c->slabCap, c->slab.p, c->rowStart[cu].p, c->numAnal.p, c->planeBits.p, c->idA.p,
+                launch_cell_rows(nS, c->slab.p, c->posOf.p, c->rowStart[cu].p, c->numAnal.p, c->planeBits.p, c->idA.p,
                                  c->idB[cu].p, c->ctype[cu].p, c->stream));
     DEME_LAUNCH(c, "history_map",
                 launch_history_map(nS, c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p,
@@ -969,6 +990,7 @@ int deme_get_counts(deme_ctx* c, deme_counts* out) {
     out->maxSpheresInBin = c->maxOcc;
     out->nSteps = c->nSteps;
     out->time = c->time;
+    out->nCells = c->nCells;
     return DEME_OK;
 }
 

@@ -39,7 +39,7 @@ struct __attribute__((aligned(16))) SphereRec {
 static_assert(sizeof(SphereRec) == 32, "SphereRec must be 32 bytes");
 
 // Per-sphere contact-detection record written once per detection pass: world position (relative to the world's
-// left-bottom-front corner) in double, the CD radius r + beta in float, and the sphere id.
+// left-bottom-front corner) in double, the sphere radius in float, and the sphere id.
 struct __attribute__((aligned(16))) CdRec {
     double x, y, z;
     float r;

@@ -31,7 +31,9 @@ static_assert(sizeof(PlanePose) == 48, "PlanePose must be 48 bytes");
 // distance, so partners lie in the 27 neighbouring cells).
 struct CellGrid {
     double inv;  // 1 / cell edge
-    uint32_t nx, ny, nz, pad;
+    uint32_t nx, ny, nz, nCells;
+    float rfMax;  // largest expanded radius float(r + beta)
+    float tol;    // slack of the single-precision candidate prefilter (metres), above its rounding error
 };
 
 // Device-side counters read back by the host after a detection pass.
@@ -51,11 +53,11 @@ constexpr int SCAN_ITEMS = 16;
 constexpr uint64_t SCAN_TILE = (uint64_t)SCAN_BLOCK * SCAN_ITEMS;
 constexpr uint32_t MAX_ANAL_OWNERS = 8;
 
-// scan.hip: out[0..n] = exclusive prefix sum of (a[i] + (b ? b[i] : 0)); out[n] = total; *totalDev = total (u64).
-// scratch must hold scan_scratch_elems(n) u64 values.
+// scan.hip: out[0..n] = exclusive prefix sum of (a[idx ? idx[i] : i] + (b ? b[i] : 0)); out[n] = total;
+// *totalDev = total (u64). scratch must hold scan_scratch_elems(n) u64 values.
 uint64_t scan_scratch_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
-                    uint64_t* scratch, hipStream_t st);
+                    uint64_t* scratch, hipStream_t st, const uint32_t* idx = nullptr);
 
 // cd.hip
 hipError_t launch_plane_pose(const KParams& P, const deme_anal_obj* anal, uint32_t nA, const OwnerRec* own,
@@ -90,15 +92,18 @@ hipError_t launch_bside_sort(uint32_t nClumps, const uint32_t* bStart, uint32_t*
 hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereRec* sph, const OwnerRec* own,
                               uint32_t nS, const PlanePose* pose, uint32_t nA, const uint8_t* mask,
                               uint32_t* cellCount, uint32_t* cellOf, uint32_t* slotOf, uint16_t* numBins,
-                              uint32_t* numAnal, unsigned long long* planeBits, CdRec* rec, DevCounters* ctr,
-                              hipStream_t st);
+                              uint32_t* numAnal, unsigned long long* planeBits, CdRec* rec, uint2* meta,
+                              DevCounters* ctr, hipStream_t st);
+// meta: (owner, family) per sphere; sortedMeta / sorted / posOf: the same in cell order
 hipError_t launch_cell_scatter(uint32_t nS, const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
-                               const CdRec* rec, CdRec* sorted, hipStream_t st);
+                               const CdRec* rec, const uint2* meta, CdRec* sorted, uint2* sortedMeta, uint32_t* posOf,
+                               hipStream_t st);
+// pairCnt: accepted partners per sorted position
 hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
-                             const uint32_t* cellStart, const SphereRec* sph, const OwnerRec* own, const uint8_t* mask,
-                             uint32_t slabCap, uint32_t* slab, uint32_t* rowCount, uint32_t* bCount, DevCounters* ctr,
-                             hipStream_t st);
-hipError_t launch_cell_rows(uint32_t nS, uint32_t slabCap, const uint32_t* slab, const uint32_t* rowStart,
+                             const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
+                             uint32_t* slab, uint32_t* pairCnt, uint32_t* bCount, DevCounters* ctr, hipStream_t st);
+// slab: column-major by sorted position (entry k of sorted sphere i at slab[k * nS + i]); posOf: sorted position
+hipError_t launch_cell_rows(uint32_t nS, const uint32_t* slab, const uint32_t* posOf, const uint32_t* rowStart,
                             const uint32_t* numAnal, const unsigned long long* planeBits, uint32_t* idA, uint32_t* idB,
                             uint8_t* ctype, hipStream_t st);
 

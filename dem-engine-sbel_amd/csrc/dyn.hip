@@ -316,10 +316,12 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
 #pragma unroll
         for (int c = 0; c < 6; c++) acc[k][c] = 0.f;
     const uint64_t c0 = (uint64_t)blockIdx.x * ANAL_TILE + (uint64_t)threadIdx.x * ANAL_ITEMS;
+    bool any = false;
     for (uint32_t it = 0; it < ANAL_ITEMS; it++) {
         const uint64_t c = c0 + it;
         if (c >= nC) break;
         if (ctype[c] != DEME_SPHERE_PLANE_CONTACT) continue;
+        any = true;
         const uint32_t o = anal[idB[c]].owner;
         const uint32_t k = o - firstAnal;
         const OwnerRec orc = load_owner(own, o);
@@ -339,6 +341,11 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
             }
         }
     }
+    if (!__syncthreads_or(any)) {
+        // a tile without plane contacts contributes exact zeros; skip the tree
+        if (threadIdx.x < nAnalOwners * 6) partials[(uint64_t)blockIdx.x * nAnalOwners * 6 + threadIdx.x] = 0.f;
+        return;
+    }
     for (uint32_t k = 0; k < nAnalOwners; k++) {
 #pragma unroll
         for (int c = 0; c < 6; c++) red[threadIdx.x][c] = acc[k][c];
@@ -354,13 +361,20 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
     }
 }
 
-__global__ void k_anal_final(const float* __restrict__ partials, uint32_t nBlocks, uint32_t nAnalOwners,
-                             float* __restrict__ analAcc) {
-    const uint32_t t = threadIdx.x;
-    if (t >= nAnalOwners * 6) return;
+// One workgroup per (owner, component): lane t sums tiles t, t + 256, ... in order, then a fixed LDS tree.
+__global__ __launch_bounds__(256) void k_anal_final(const float* __restrict__ partials, uint32_t nBlocks,
+                                                    uint32_t nAnalOwners, float* __restrict__ analAcc) {
+    __shared__ float red[256];
+    const uint32_t v = blockIdx.x, t = threadIdx.x, stride = nAnalOwners * 6;
     float s = 0.f;
-    for (uint32_t b = 0; b < nBlocks; b++) s += partials[(uint64_t)b * nAnalOwners * 6 + t];
-    analAcc[t] = s;
+    for (uint32_t b = t; b < nBlocks; b += 256) s += partials[(uint64_t)b * stride + v];
+    red[t] = s;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) analAcc[v] = red[0];
 }
 
 // Rows a16 (clump owners) + a17: owner reduction fused with integrateOwners (DEMIntegrationKernels.cu:170-179).
@@ -525,7 +539,8 @@ hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* 
 hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint32_t nAnalOwners, float* analAcc,
                              hipStream_t st) {
     if (nAnalOwners == 0) return hipSuccess;
-    k_anal_final<<<1, 64, 0, st>>>(blockPartials, nBlocks, nAnalOwners, analAcc);
+    if (nAnalOwners > MAX_ANAL_OWNERS) return hipErrorInvalidValue;
+    k_anal_final<<<nAnalOwners * 6, 256, 0, st>>>(blockPartials, nBlocks, nAnalOwners, analAcc);
     return hipGetLastError();
 }
 

@@ -37,18 +37,25 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t& total)
     return pre + incl - v;
 }
 
+// Element i of the scanned sequence: a[idx ? idx[i] : i] + (b ? b[i] : 0).
 template <typename TIn>
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const TIn* __restrict__ a, const TIn* __restrict__ b,
-                                                            uint64_t n, uint64_t* __restrict__ sums) {
+__device__ __forceinline__ uint64_t scan_elem(const TIn* __restrict__ a, const uint32_t* __restrict__ idx,
+                                              const TIn* __restrict__ b, uint64_t i) {
+    uint64_t v = (uint64_t)a[idx ? (uint64_t)idx[i] : i];
+    if (b) v += (uint64_t)b[i];
+    return v;
+}
+
+template <typename TIn>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const TIn* __restrict__ a, const uint32_t* __restrict__ gidx,
+                                                            const TIn* __restrict__ b, uint64_t n,
+                                                            uint64_t* __restrict__ sums) {
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
     uint64_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) {
         const uint64_t idx = base + (uint64_t)i * SCAN_BLOCK + threadIdx.x;
-        if (idx < n) {
-            s += (uint64_t)a[idx];
-            if (b) s += (uint64_t)b[idx];
-        }
+        if (idx < n) s += scan_elem(a, gidx, b, idx);
     }
     uint64_t tot;
     block_excl_scan(s, tot);
@@ -57,7 +64,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const TIn* __restric
 
 // tile scan: loads striped (coalesced), scans per-lane contiguous runs, stores striped.
 template <typename TIn, typename TOut>
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const TIn* __restrict__ b, uint64_t n,
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const uint32_t* __restrict__ gidx,
+                                                           const TIn* __restrict__ b, uint64_t n,
                                                            const uint64_t* __restrict__ offsets, TOut* out,
                                                            unsigned long long* totalOut) {
     // +1 padding per 16 entries keeps the per-lane contiguous walk free of 64-bit bank conflicts
@@ -68,12 +76,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const T
     for (int i = 0; i < SCAN_ITEMS; i++) {
         const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
         const uint64_t idx = base + k;
-        uint64_t v = 0;
-        if (idx < n) {
-            v = (uint64_t)a[idx];
-            if (b) v += (uint64_t)b[idx];
-        }
-        tile[at(k)] = v;
+        tile[at(k)] = idx < n ? scan_elem(a, gidx, b, idx) : 0;
     }
     __syncthreads();
     uint64_t s = 0;
@@ -108,14 +111,14 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const T
 hipError_t scan_u64_inplace(uint64_t* v, uint64_t n, uint64_t* scratch, hipStream_t st) {
     const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb <= 1) {
-        k_scan_apply<uint64_t, uint64_t><<<1, SCAN_BLOCK, 0, st>>>(v, nullptr, n, nullptr, v, nullptr);
+        k_scan_apply<uint64_t, uint64_t><<<1, SCAN_BLOCK, 0, st>>>(v, nullptr, nullptr, n, nullptr, v, nullptr);
         return hipGetLastError();
     }
     uint64_t* sums = scratch;
-    k_scan_reduce<uint64_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(v, nullptr, n, sums);
+    k_scan_reduce<uint64_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(v, nullptr, nullptr, n, sums);
     hipError_t e = scan_u64_inplace(sums, nb, scratch + nb + 1, st);
     if (e != hipSuccess) return e;
-    k_scan_apply<uint64_t, uint64_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(v, nullptr, n, sums, v, nullptr);
+    k_scan_apply<uint64_t, uint64_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(v, nullptr, nullptr, n, sums, v, nullptr);
     return hipGetLastError();
 }
 
@@ -134,22 +137,22 @@ uint64_t scan_scratch_elems(uint64_t n) {
 }
 
 hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
-                    uint64_t* scratch, hipStream_t st) {
+                    uint64_t* scratch, hipStream_t st, const uint32_t* idx) {
     const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (n == 0) {
         // out[0] = 0 and total = 0
-        k_scan_apply<uint32_t, uint32_t><<<1, SCAN_BLOCK, 0, st>>>(a, b, 0, nullptr, out, totalDev);
+        k_scan_apply<uint32_t, uint32_t><<<1, SCAN_BLOCK, 0, st>>>(a, idx, b, 0, nullptr, out, totalDev);
         return hipGetLastError();
     }
     if (nb <= 1) {
-        k_scan_apply<uint32_t, uint32_t><<<1, SCAN_BLOCK, 0, st>>>(a, b, n, nullptr, out, totalDev);
+        k_scan_apply<uint32_t, uint32_t><<<1, SCAN_BLOCK, 0, st>>>(a, idx, b, n, nullptr, out, totalDev);
         return hipGetLastError();
     }
     uint64_t* sums = scratch;
-    k_scan_reduce<uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, b, n, sums);
+    k_scan_reduce<uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, idx, b, n, sums);
     hipError_t e = scan_u64_inplace(sums, nb, scratch + nb + 1, st);
     if (e != hipSuccess) return e;
-    k_scan_apply<uint32_t, uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, b, n, sums, out, totalDev);
+    k_scan_apply<uint32_t, uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, idx, b, n, sums, out, totalDev);
     return hipGetLastError();
 }
 

@@ -43,7 +43,7 @@ class SimParams(C.Structure):
 class Counts(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "nSpheres", "nOwners", "nAnal", "nBinSphereTouches", "nActiveBins", "nContacts", "nPrevContacts",
-        "nSphereGeoContacts", "maxSpheresInBin", "nSteps")] + [("time", C.c_double)]
+        "nSphereGeoContacts", "maxSpheresInBin", "nSteps")] + [("time", C.c_double), ("nCells", C.c_uint64)]
 
 
 class OwnerState(C.Structure):

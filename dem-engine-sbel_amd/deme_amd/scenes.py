@@ -129,9 +129,10 @@ def _finish(name, params, xyz_clumps, sph_owner, sph_radius, sph_relpos, sph_mat
 
 def sphere_settling(n=20000, seed=1001, r=0.005, box=(0.3, 0.3, 0.6), h=1e-5, steps=1000, update_freq=1,
                     E=1e7, nu=0.3, CoR=0.5, mu=0.5, Crr=0.0, jitter=0.1, spacing=2.2, velocity_sigma=0.0,
-                    omega_sigma=0.0, random_orientation=False, beta=0.0):
+                    omega_sigma=0.0, random_orientation=False, beta=0.0, n_big=0, r_big=None):
     """Monodisperse FP32 spheres seeded on a cubic lattice (spacing 2.2 r, jitter +-0.1 r), settling under gravity
-    in a box of analytical planes."""
+    in a box of analytical planes. Optionally n_big spheres of radius r_big sit on a row above the lattice (clear of
+    it), which widens the detection cells so that many small spheres share one cell."""
     rng = np.random.default_rng(seed)
     box = tuple(float(f32(b)) for b in box)
     lbf = tuple(float(f32(-f32(b) / f32(2.0))) for b in box)
@@ -148,16 +149,27 @@ def sphere_settling(n=20000, seed=1001, r=0.005, box=(0.3, 0.3, 0.6), h=1e-5, st
     z0 = lbf[2] + 1.1 * r
     xyz = np.stack([x0 + i * s, y0 + j * s, z0 + k * s], 1)
     xyz += rng.uniform(-jitter * r, jitter * r, size=xyz.shape)
-    mass = f32(RHO * 4.0 / 3.0 * np.pi * r ** 3)
-    moi = f32(2.0 / 5.0 * float(mass) * float(f32(r)) * float(f32(r)))
+    radius = np.full(n, r, np.float32)
+    if n_big:
+        rb = float(f32(r_big))
+        zb = float(xyz[:, 2].max()) + r + 1.5 * rb
+        xb = lbf[0] + rb * 1.5 + np.arange(n_big) * 2.5 * rb
+        if xb[-1] + rb > lbf[0] + box[0] or zb + rb > lbf[2] + box[2]:
+            raise ValueError("large spheres do not fit above the lattice")
+        big = np.stack([xb, np.full(n_big, lbf[1] + box[1] / 2), np.full(n_big, zb)], 1)
+        xyz = np.concatenate([xyz, big])
+        radius = np.concatenate([radius, np.full(n_big, rb, np.float32)])
+    m = n + n_big
+    mass = (RHO * 4.0 / 3.0 * np.pi * radius.astype(np.float64) ** 3).astype(np.float32)
+    moi = (2.0 / 5.0 * mass.astype(np.float64) * radius.astype(np.float64) ** 2).astype(np.float32)
     params = build_sim_params(box, lbf, float(f32(r)), h, beta=beta)
     mats = np.zeros(1, MATERIAL_DTYPE)
     mats[0] = (E, nu, CoR, mu, Crr)
-    anal = box_planes(lbf, box, owner=n, material=0)
-    vel, omg, q = random_motion(rng, n, velocity_sigma, omega_sigma, random_orientation)
-    return _finish("sphere_settling", params, xyz, np.arange(n), np.full(n, r, np.float32), np.zeros((n, 3)),
-                   np.zeros(n), np.full(n, mass), np.full((n, 3), moi), mats, anal, 1, update_freq, steps, vel=vel,
-                   omg=omg, quat=q, info=dict(r=r, box=box))
+    anal = box_planes(lbf, box, owner=m, material=0)
+    vel, omg, q = random_motion(rng, m, velocity_sigma, omega_sigma, random_orientation)
+    return _finish("sphere_settling", params, xyz, np.arange(m), radius, np.zeros((m, 3)), np.zeros(m), mass,
+                   np.repeat(moi[:, None], 3, 1), mats, anal, 1, update_freq, steps, vel=vel, omg=omg, quat=q,
+                   info=dict(r=r, box=box))
 
 
 def clump_template(seed=1002, r0=0.002, offset_frac=0.6, samples=2_000_000):
@@ -235,3 +247,35 @@ def clump_settling(n_clumps=1_000_000, seed=1002, box=(0.5, 0.5, 1.0), h=5e-6, s
     return _finish("clump_settling", params, xyz, sph_owner, sph_radius, sph_relpos, np.zeros(n_s),
                    np.full(n_clumps, tpl["mass"]), np.tile(tpl["moi"], (n_clumps, 1)), mats, anal, 1, update_freq,
                    steps, vel=vel, omg=omg, quat=q, info=dict(template=tpl, box=box, beta=beta))
+
+
+def spheres_scene(xyz, radius, box, lbf=None, h=1e-5, beta=0.0, g=(0.0, 0.0, -9.81), E=1e7, nu=0.3, CoR=0.5,
+                  mu=0.5, Crr=0.0, planes=True, vel=None, omg=None, quat=None, families=None, rules=None, mask=None,
+                  integrator="extended_taylor", name="spheres"):
+    """Single-sphere owners at given centres (world coordinates) in a box whose left-bottom-front corner defaults to
+    (-box/2); planes=True adds the six bounding planes (family 255), planes=False none."""
+    xyz = np.atleast_2d(np.asarray(xyz, np.float64))
+    n = xyz.shape[0]
+    radius = np.broadcast_to(np.asarray(radius, np.float32), (n,)).copy()
+    box = tuple(float(f32(b)) for b in box)
+    if lbf is None:
+        lbf = tuple(float(f32(-f32(b) / f32(2.0))) for b in box)
+    params = build_sim_params(box, lbf, float(radius.min()), h, beta=beta, g=g, integrator=integrator)
+    mats = np.zeros(1, MATERIAL_DTYPE)
+    mats[0] = (E, nu, CoR, mu, Crr)
+    mass = (RHO * 4.0 / 3.0 * np.pi * radius.astype(np.float64) ** 3).astype(np.float32)
+    moi = (2.0 / 5.0 * mass.astype(np.float64) * radius.astype(np.float64) ** 2).astype(np.float32)
+    if planes:
+        anal = box_planes(lbf, box, owner=n, material=0)
+        n_ext = 1
+    else:
+        anal = np.zeros(0, ANAL_DTYPE)
+        n_ext = 0
+    if mask is None or rules is None:
+        m0, r0 = default_families()
+        mask = m0 if mask is None else mask
+        rules = r0 if rules is None else rules
+    to32 = (lambda a: None if a is None else np.atleast_2d(np.asarray(a, np.float32)))
+    return _finish(name, params, xyz, np.arange(n), radius, np.zeros((n, 3)), np.zeros(n), mass,
+                   np.repeat(moi[:, None], 3, 1), mats, anal, n_ext, 1, 1, vel=to32(vel), omg=to32(omg),
+                   quat=to32(quat), families=families, mask=mask, rules=rules, info=dict(box=box))

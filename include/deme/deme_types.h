@@ -144,6 +144,7 @@ typedef struct deme_counts {
     uint64_t maxSpheresInBin;
     uint64_t nSteps;
     double time;
+    uint64_t nCells; /* cells of the contact-detection cell grid (0 before the first cell-list pass) */
 } deme_counts;
 
 #ifdef __cplusplus

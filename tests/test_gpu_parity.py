@@ -31,7 +31,14 @@ def expanded_clumps():
                                  velocity_sigma=0.05, omega_sigma=3.0, random_orientation=True)
 
 
-SCENES = [packed_spheres, packed_clumps, loose_box, expanded_clumps]
+def crowded_cells():
+    # three large spheres widen the detection cells to ~90 small spheres each: blocks whose neighbour segments
+    # exceed the staging budget sweep global memory instead of LDS
+    return scenes.sphere_settling(n=3000, r=0.002, box=(0.1, 0.1, 0.2), spacing=2.0, velocity_sigma=0.05,
+                                  n_big=3, r_big=0.01)
+
+
+SCENES = [packed_spheres, packed_clumps, loose_box, expanded_clumps, crowded_cells]
 
 
 def systems(make, pre_steps=0):
