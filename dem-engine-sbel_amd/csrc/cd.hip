@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_bin_pairs(KParams P, const uint2* __res
                                                    const SphereRec* __restrict__ sph, const OwnerRec* __restrict__ own,
                                                    const uint8_t* __restrict__ mask, uint2* __restrict__ pairs,
                                                    uint32_t pairCap, DevCounters* ctr, uint32_t* rowCount,
-                                                   uint32_t* bCount) {
+                                                   uint32_t* bCnt) {
     __shared__ CdRec tile[256];
     const uint64_t base = (uint64_t)blockIdx.x * 256u;
     const uint64_t e = base + threadIdx.x;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_bin_pairs(KParams P, const uint2* __res
         const bool active = j <= nIter;
         if (!__any(active)) break;
         bool hit = false;
-        uint32_t ia = 0, ib = 0, ob = 0;
+        uint32_t ia = 0, ib = 0;
         if (active) {
             const uint64_t ep = e + j;
             const CdRec ot = (ep < tileEnd) ? tile[ep - base] : load_cd(rec, binEnt[ep].x);
@@ -239,7 +239,6 @@ __global__ __launch_bounds__(256) void k_bin_pairs(KParams P, const uint2* __res
                         hit = true;
                         ia = A.id;
                         ib = B.id;
-                        ob = obb;
                     }
                 }
             }
@@ -254,7 +253,7 @@ __global__ __launch_bounds__(256) void k_bin_pairs(KParams P, const uint2* __res
                 const uint32_t idx = first + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
                 if (idx < pairCap) pairs[idx] = make_uint2(ia, ib);
                 atomicAdd(&rowCount[ia], 1u);
-                atomicAdd(&bCount[ob], 1u);
+                atomicAdd(&bCnt[ib], 1u);
             }
         }
     }
@@ -321,21 +320,27 @@ __global__ __launch_bounds__(256) void k_row_finalize(uint32_t nS, const uint32_
 }
 
 // Row a12: persistent-contact map (buildPersistentMap, DEMHistoryMappingKernels.cu:16-60) as a merge join of the
-// new and previous rows of the same sphere (both ordered by (type group, idB)).
+// new and previous rows of the same sphere (both ordered by (type group, idB)). The same pass links every
+// sphere-sphere contact (s, b) to its slot in b's mirror row (the contacts where b is the B body, ordered by A id):
+// mirror[i] = bRowStart[b] + rank of s in that row.
 __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t* __restrict__ rowStart,
                                                      const uint32_t* __restrict__ idB,
                                                      const uint8_t* __restrict__ ctype,
                                                      const uint32_t* __restrict__ prevRowStart,
                                                      const uint32_t* __restrict__ prevIdB,
-                                                     const uint8_t* __restrict__ prevType, uint32_t* __restrict__ map) {
+                                                     const uint8_t* __restrict__ prevType, uint32_t* __restrict__ map,
+                                                     const uint32_t* __restrict__ bRowStart,
+                                                     const uint32_t* __restrict__ bPart,
+                                                     uint32_t* __restrict__ mirror) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
     const uint32_t ns = rowStart[s], ne = rowStart[s + 1];
     uint32_t j = prevRowStart ? prevRowStart[s] : 0;
     const uint32_t je = prevRowStart ? prevRowStart[s + 1] : 0;
     for (uint32_t i = ns; i < ne; i++) {
-        const uint32_t gi = ctype[i] == DEME_SPHERE_SPHERE_CONTACT ? 1u : 0u;
-        const uint64_t ki = ((uint64_t)gi << 32) | idB[i];
+        const uint32_t t = ctype[i], b = idB[i];
+        const uint32_t gi = t == DEME_SPHERE_SPHERE_CONTACT ? 1u : 0u;
+        const uint64_t ki = ((uint64_t)gi << 32) | b;
         uint32_t m = DEME_NULL_MAPPING_PARTNER;
         while (j < je) {
             const uint32_t gj = prevType[j] == DEME_SPHERE_SPHERE_CONTACT ? 1u : 0u;
@@ -344,42 +349,53 @@ __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t
                 j++;
                 continue;
             }
-            if (kj == ki && prevType[j] == ctype[i]) m = j;
+            if (kj == ki && prevType[j] == t) m = j;
             break;
         }
         map[i] = m;
+        uint32_t link = DEME_NULL_MAPPING_PARTNER;
+        if (t == DEME_SPHERE_SPHERE_CONTACT && b < nS) {
+            uint32_t lo = bRowStart[b], hi = bRowStart[b + 1];
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (bPart[mid] < s) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < bRowStart[b + 1] && bPart[lo] == s) link = lo;
+        }
+        mirror[i] = link;
     }
 }
 
-// B-side owner lists for the owner reduction: contact indices of sphere-sphere contacts grouped by owner of B.
-__global__ __launch_bounds__(256) void k_bside_fill(uint32_t nC, uint32_t nS, const uint32_t* __restrict__ idB,
-                                                    const uint8_t* __restrict__ ctype,
-                                                    const SphereRec* __restrict__ sph,
-                                                    const uint32_t* __restrict__ bStart, uint32_t* bCount,
-                                                    uint32_t* __restrict__ bList) {
+// Mirror rows for a contact list built without them (the staged path): per B sphere, the A ids of its sphere-sphere
+// contacts. bCnt holds the per-sphere counts and is consumed (counted down) to hand out slots.
+__global__ __launch_bounds__(256) void k_brow_fill(uint32_t nC, uint32_t nS, const uint32_t* __restrict__ idA,
+                                                   const uint32_t* __restrict__ idB,
+                                                   const uint8_t* __restrict__ ctype,
+                                                   const uint32_t* __restrict__ bRowStart, uint32_t* bCnt,
+                                                   uint32_t* __restrict__ bPart) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nC || ctype[c] != DEME_SPHERE_SPHERE_CONTACT) return;
     const uint32_t b = idB[c];
-    if (b >= nS) return;  // guard against a malformed list (never gathers past the sphere table)
-    const uint32_t o = sph[b].owner;
-    const uint32_t slot = atomicSub(&bCount[o], 1u) - 1u;
-    if (slot >= bStart[o + 1] - bStart[o]) return;
-    bList[bStart[o] + slot] = c;
+    if (b >= nS) return;  // guard against a malformed list (never indexes past the row table)
+    const uint32_t slot = atomicSub(&bCnt[b], 1u) - 1u;
+    if (slot >= bRowStart[b + 1] - bRowStart[b]) return;
+    bPart[bRowStart[b] + slot] = idA[c];
 }
 
-__global__ __launch_bounds__(256) void k_bside_sort(uint32_t nO, const uint32_t* __restrict__ bStart,
-                                                    uint32_t* __restrict__ bList) {
-    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= nO) return;
-    const uint32_t st = bStart[o], en = bStart[o + 1];
+__global__ __launch_bounds__(256) void k_brow_sort(uint32_t nS, const uint32_t* __restrict__ bRowStart,
+                                                   uint32_t* __restrict__ bPart) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nS) return;
+    const uint32_t st = bRowStart[s], en = bRowStart[s + 1];
     for (uint32_t i = st + 1; i < en; i++) {
-        const uint32_t v = bList[i];
+        const uint32_t v = bPart[i];
         uint32_t k = i;
-        while (k > st && bList[k - 1] > v) {
-            bList[k] = bList[k - 1];
+        while (k > st && bPart[k - 1] > v) {
+            bPart[k] = bPart[k - 1];
             k--;
         }
-        bList[k] = v;
+        bPart[k] = v;
     }
 }
 
@@ -434,11 +450,11 @@ hipError_t launch_bin_scatter(const KParams& P, const CdRec* rec, const SphereRe
 
 hipError_t launch_bin_pairs(const KParams& P, const uint2* binEnt, uint64_t nEnt, const uint32_t* binStart,
                             const CdRec* rec, const SphereRec* sph, const OwnerRec* own, const uint8_t* mask,
-                            uint2* pairs, uint32_t pairCap, DevCounters* ctr, uint32_t* rowCount, uint32_t* bCount,
+                            uint2* pairs, uint32_t pairCap, DevCounters* ctr, uint32_t* rowCount, uint32_t* bCnt,
                             hipStream_t st) {
     if (nEnt == 0) return hipSuccess;
     k_bin_pairs<<<blocks(nEnt, 256), 256, 0, st>>>(P, binEnt, nEnt, binStart, rec, sph, own, mask, pairs, pairCap, ctr,
-                                                   rowCount, bCount);
+                                                   rowCount, bCnt);
     return hipGetLastError();
 }
 
@@ -466,23 +482,24 @@ hipError_t launch_row_finalize(uint32_t nS, const uint32_t* rowStart, const uint
 
 hipError_t launch_history_map(uint32_t nS, const uint32_t* rowStart, const uint32_t* idB, const uint8_t* ctype,
                               const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
-                              uint32_t* map, hipStream_t st) {
+                              uint32_t* map, const uint32_t* bRowStart, const uint32_t* bPart, uint32_t* mirror,
+                              hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_history_map<<<blocks(nS, 256), 256, 0, st>>>(nS, rowStart, idB, ctype, prevRowStart, prevIdB, prevType, map);
+    k_history_map<<<blocks(nS, 256), 256, 0, st>>>(nS, rowStart, idB, ctype, prevRowStart, prevIdB, prevType, map,
+                                                   bRowStart, bPart, mirror);
     return hipGetLastError();
 }
 
-hipError_t launch_bside_fill(uint32_t nC, uint32_t nS, const uint32_t* idB, const uint8_t* ctype,
-                             const SphereRec* sph, const uint32_t* bStart, uint32_t* bCount, uint32_t* bList,
-                             hipStream_t st) {
+hipError_t launch_brow_fill(uint32_t nC, uint32_t nS, const uint32_t* idA, const uint32_t* idB, const uint8_t* ctype,
+                            const uint32_t* bRowStart, uint32_t* bCnt, uint32_t* bPart, hipStream_t st) {
     if (nC == 0) return hipSuccess;
-    k_bside_fill<<<blocks(nC, 256), 256, 0, st>>>(nC, nS, idB, ctype, sph, bStart, bCount, bList);
+    k_brow_fill<<<blocks(nC, 256), 256, 0, st>>>(nC, nS, idA, idB, ctype, bRowStart, bCnt, bPart);
     return hipGetLastError();
 }
 
-hipError_t launch_bside_sort(uint32_t nClumps, const uint32_t* bStart, uint32_t* bList, hipStream_t st) {
-    if (nClumps == 0) return hipSuccess;
-    k_bside_sort<<<blocks(nClumps, 256), 256, 0, st>>>(nClumps, bStart, bList);
+hipError_t launch_brow_sort(uint32_t nS, const uint32_t* bRowStart, uint32_t* bPart, hipStream_t st) {
+    if (nS == 0) return hipSuccess;
+    k_brow_sort<<<blocks(nS, 256), 256, 0, st>>>(nS, bRowStart, bPart);
     return hipGetLastError();
 }
 

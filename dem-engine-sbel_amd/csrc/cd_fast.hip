@@ -138,13 +138,17 @@ __global__ __launch_bounds__(256) void k_cell_scatter(uint32_t nS, const uint32_
                                                       const uint32_t* __restrict__ cellStart,
                                                       const CdRec* __restrict__ rec, const uint2* __restrict__ meta,
                                                       CdRec* __restrict__ sorted, uint2* __restrict__ sortedMeta,
-                                                      uint32_t* __restrict__ posOf) {
+                                                      uint32_t* __restrict__ posOf, uint32_t* __restrict__ aCnt,
+                                                      uint32_t* __restrict__ bCnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
     const CdRec cr = load_cd(rec, s);
     const uint32_t pos = cellStart[cellOf[s]] + slotOf[s];
     posOf[s] = pos;
     sortedMeta[pos] = meta[s];
+    // the pair pass accumulates its per-sphere counts with atomics
+    aCnt[pos] = 0u;
+    bCnt[pos] = 0u;
     CdRec* dst = sorted + pos;
     uint4* d4 = reinterpret_cast<uint4*>(dst);
     const uint4* s4 = reinterpret_cast<const uint4*>(&cr);
@@ -197,33 +201,49 @@ __device__ __forceinline__ bool bin_rule_point(const KParams& P, const CdRec& A,
     return range_has_linear(P, ga, lin) && range_has_linear(P, gb, lin);
 }
 
-// One lane per sphere in cell order: scan 27 neighbour cells, keep partners with larger ids.
+// One lane per sphere in cell order, half-shell: each lane pairs its sphere with the later spheres of its own cell
+// and with the spheres of 13 of its 26 neighbour cells (the cell after it in x; the three cells of the next row in y;
+// the nine cells of the next layer in z), so every candidate pair is met exactly once.
 //
 // Cells are numbered x fastest, so for a block of consecutive sorted spheres (cells cLo..cHi) the candidates of all
 // its lanes for one (dy, dz) neighbour row are the sorted range of cells cLo - 1 + off .. cHi + 1 + off, off =
-// dy * nx + dz * nx * ny: nine contiguous segments. The block stages them in LDS as block-relative float positions
-// and sweeps its lanes' 27-cell windows there.
+// dy * nx + dz * nx * ny: five contiguous segments. The block stages them in LDS as block-relative float positions
+// and sweeps its lanes' windows there.
 //
 // Two passes keep a wave's lanes in step. The sweep runs only a single-precision prefilter, conservative by
-// CellGrid::tol and the largest expanded radius, and parks the survivors (sorted-array indices) in the lane's slab
-// column: entry k of sorted sphere i sits at slab[k * nS + i], so a wave's k-th entries share cache lines.
-// The second pass re-tests each survivor with the reference's double-precision overlap and contact-point bin rule
-// and compacts the row in place into accepted partner ids. A row whose survivors exceed the slab is reported through
-// maxOcc (an upper bound of its partners) and the pass is redone with wider rows. A block whose segments exceed the
-// LDS budget sweeps global memory with the exact test instead.
+// CellGrid::tol and the largest expanded radius, and parks the survivors (sorted-array indices) in the lane's column
+// of `surv`: entry k of sorted sphere i sits at surv[k * nS + i], so a wave's k-th entries share cache lines. The
+// second pass re-tests each survivor with the reference's double-precision overlap and contact-point bin rule
+// (evaluated with the lower id as A, whichever lane meets the pair) and appends every accepted pair to both spheres'
+// columns: the higher id to the lower-id sphere's contact row (rowSlab, count aCnt), the lower id to the higher-id
+// sphere's mirror row (mirSlab, count bCnt). Both counters are indexed by sorted position, so the appends of a block
+// hit neighbouring addresses. A column that overflows the slab is reported through maxOcc and the pass is redone
+// with wider columns. A block whose segments exceed the LDS budget sweeps global memory with the exact test instead.
 constexpr uint32_t PAIR_BLOCK = 128;
-constexpr uint32_t PAIR_LDS_RECS = 1536;
+constexpr uint32_t PAIR_LDS_RECS = 1024;
+constexpr int PAIR_SEGS = 5;
+
+// segment k of the half shell: (dy, dz) = (0, 0), (1, 0), (-1, 1), (0, 1), (1, 1)
+__device__ __forceinline__ int seg_dy(int k) { return k == 0 ? 0 : (k == 1 ? 1 : k - 3); }
+__device__ __forceinline__ int seg_dz(int k) { return k < 2 ? 0 : 1; }
+
+__device__ __forceinline__ void append_pair(uint32_t* __restrict__ cnt, uint32_t* __restrict__ slab, uint32_t idx,
+                                            uint32_t val, uint32_t nS, uint32_t slabCap, DevCounters* ctr) {
+    const uint32_t slot = atomicAdd(&cnt[idx], 1u);
+    if (slot < slabCap) slab[(uint64_t)slot * nS + idx] = val;
+    else atomicMax(&ctr->maxOcc, slot + 1);
+}
 
 __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G, uint32_t nS, uint32_t nBlocks,
                                                            const CdRec* __restrict__ sorted,
                                                            const uint2* __restrict__ sortedMeta,
                                                            const uint32_t* __restrict__ cellStart,
                                                            const uint8_t* __restrict__ mask, uint32_t slabCap,
-                                                           uint32_t* __restrict__ slab,
-                                                           uint32_t* __restrict__ pairCnt, uint32_t* bCount,
-                                                           DevCounters* ctr) {
+                                                           uint32_t* __restrict__ surv, uint32_t* __restrict__ rowSlab,
+                                                           uint32_t* __restrict__ mirSlab, uint32_t* aCnt,
+                                                           uint32_t* bCnt, DevCounters* ctr) {
     __shared__ float4 recs[PAIR_LDS_RECS];  // x, y, z relative to the block origin; id bits
-    __shared__ uint32_t segGlob[9], segLen[9], segBase[9];
+    __shared__ uint32_t segGlob[PAIR_SEGS], segLen[PAIR_SEGS], segBase[PAIR_SEGS];
     __shared__ uint32_t cellLo, cellHi;
     __shared__ double org[3];
     // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so XCD k gets a contiguous share of the
@@ -238,10 +258,11 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
     const bool valid = i < iEnd;
     CdRec me;
     int cx = 0, cy = 0, cz = 0;
+    uint32_t myCell = 0;
     if (valid) {
         me = load_cd(sorted, i);
         cell_of(G, me.x, me.y, me.z, cx, cy, cz);
-        const uint32_t myCell = (uint32_t)cx + (uint32_t)cy * G.nx + (uint32_t)cz * G.nx * G.ny;
+        myCell = (uint32_t)cx + (uint32_t)cy * G.nx + (uint32_t)cz * G.nx * G.ny;
         // sorted by cell: the block's cells are those of its first and last sphere and everything between
         if (i == i0) {
             cellLo = myCell;
@@ -252,9 +273,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
         if (i == iEnd - 1) cellHi = myCell;
     }
     __syncthreads();
-    if (threadIdx.x < 9) {
-        const int64_t dz = (int64_t)(threadIdx.x / 3) - 1, dy = (int64_t)(threadIdx.x % 3) - 1;
-        const int64_t off = dy * G.nx + dz * (int64_t)G.nx * G.ny;
+    if (threadIdx.x < PAIR_SEGS) {
+        const int64_t off = (int64_t)seg_dy(threadIdx.x) * G.nx + (int64_t)seg_dz(threadIdx.x) * G.nx * G.ny;
         const int64_t lo = max((int64_t)cellLo + off - 1, (int64_t)0);
         const int64_t hi = min((int64_t)cellHi + off + 1, (int64_t)G.nCells - 1);
         uint32_t gs = 0, ge = 0;
@@ -268,12 +288,12 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
     __syncthreads();
     uint32_t total = 0;
 #pragma unroll
-    for (int k = 0; k < 9; k++) total += segLen[k];
+    for (int k = 0; k < PAIR_SEGS; k++) total += segLen[k];
     const bool staged = total <= PAIR_LDS_RECS;  // block-uniform
     if (staged) {
         const double ox = org[0], oy = org[1], oz = org[2];
         uint32_t base = 0;
-        for (int sg = 0; sg < 9; sg++) {
+        for (int sg = 0; sg < PAIR_SEGS; sg++) {
             const uint32_t len = segLen[sg], g = segGlob[sg];
             if (threadIdx.x == 0) segBase[sg] = base;
             for (uint32_t k = threadIdx.x; k < len; k += PAIR_BLOCK) {
@@ -288,89 +308,110 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
     if (!valid) return;
 
     const double ra = expanded_radius(P, me.r);
-    uint32_t* row = slab + i;  // entry k at row[k * nS]
+    uint32_t* col = surv + i;  // survivor k at col[k * nS]
     uint32_t nc = 0;
-    const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < (int)G.nz - 1 ? cz + 1 : cz;
-    const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < (int)G.ny - 1 ? cy + 1 : cy;
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < (int)G.nx - 1 ? cx + 1 : cx;
-    if (staged) {
-        const float px = (float)(me.x - org[0]), py = (float)(me.y - org[1]), pz = (float)(me.z - org[2]);
-        float R = (float)ra;
-        R += G.rfMax;
-        R += G.tol;
-        const float thr = R * R;
-        for (int z = z0; z <= z1; z++)
-            for (int y = y0; y <= y1; y++) {
-                const uint32_t c0 = (uint32_t)x0 + (uint32_t)y * G.nx + (uint32_t)z * G.nx * G.ny;
-                const uint32_t b = cellStart[c0], e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
-                const int sgi = (z - cz + 1) * 3 + (y - cy + 1);
-                // this row's cells lie inside segment sgi, so b..e index into its staged records
-                const uint32_t base = segBase[sgi] - segGlob[sgi];  // modulo 2^32: base + j = segBase + (j - segGlob)
-                for (uint32_t j = b; j < e; j++) {
-                    const float4 q = recs[base + j];
-                    if (__float_as_uint(q.w) <= me.id) continue;
-                    const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
-                    if (dx * dx + dy * dy + dz * dz > thr) continue;
-                    if (nc < slabCap) row[(uint64_t)nc * nS] = j;
-                    nc++;
-                }
+    const uint32_t nxny = G.nx * G.ny;
+    for (int sg = 0; sg < PAIR_SEGS; sg++) {
+        const int y = cy + seg_dy(sg), z = cz + seg_dz(sg);
+        if (y < 0 || y >= (int)G.ny || z >= (int)G.nz) continue;
+        uint32_t b, e;
+        if (sg == 0) {
+            // own cell after this sphere, then the next cell in x
+            b = i + 1;
+            e = cellStart[myCell + (uint32_t)(x1 - cx) + 1];
+        } else {
+            const uint32_t c0 = (uint32_t)x0 + (uint32_t)y * G.nx + (uint32_t)z * nxny;
+            b = cellStart[c0];
+            e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
+        }
+        if (staged) {
+            const float px = (float)(me.x - org[0]), py = (float)(me.y - org[1]), pz = (float)(me.z - org[2]);
+            float R = (float)ra;
+            R += G.rfMax;
+            R += G.tol;
+            const float thr = R * R;
+            // this row's cells lie inside segment sg, so b..e index into its staged records
+            const uint32_t base = segBase[sg] - segGlob[sg];  // modulo 2^32: base + j = segBase + (j - segGlob)
+            for (uint32_t j = b; j < e; j++) {
+                const float4 q = recs[base + j];
+                const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+                if (dx * dx + dy * dy + dz * dz > thr) continue;
+                if (nc < slabCap) col[(uint64_t)nc * nS] = j;
+                nc++;
             }
-    } else {
-        for (int z = z0; z <= z1; z++)
-            for (int y = y0; y <= y1; y++) {
-                const uint32_t c0 = (uint32_t)x0 + (uint32_t)y * G.nx + (uint32_t)z * G.nx * G.ny;
-                const uint32_t b = cellStart[c0], e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
-                for (uint32_t j = b; j < e; j++) {
-                    const CdRec ot = load_cd(sorted, j);
-                    if (ot.id <= me.id) continue;
-                    if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) continue;
-                    if (nc < slabCap) row[(uint64_t)nc * nS] = j;
-                    nc++;
-                }
+        } else {
+            for (uint32_t j = b; j < e; j++) {
+                const CdRec ot = load_cd(sorted, j);
+                if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) continue;
+                if (nc < slabCap) col[(uint64_t)nc * nS] = j;
+                nc++;
             }
+        }
     }
     if (nc > slabCap) {
-        pairCnt[i] = nc;
         atomicMax(&ctr->maxOcc, nc);
         return;
     }
-    uint32_t n = 0;
-    if (nc) {
-        double rcdA = (double)me.r;
-        rcdA += (double)P.beta;
-        const BinRange ga = bin_range(P, me.x, me.y, me.z, rcdA);
-        const uint2 mine = sortedMeta[i];  // owner, family
-        for (uint32_t k = 0; k < nc; k++) {
-            const uint32_t j = row[(uint64_t)k * nS];
-            const CdRec ot = load_cd(sorted, j);
-            if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) continue;
-            if (!bin_rule_point(P, me, ga, ot)) continue;
-            const uint2 theirs = sortedMeta[j];
-            const uint32_t ob = theirs.x;
-            if (ob == mine.x) continue;
-            if (mask[mask_index(mine.y, theirs.y)] != DEME_DONT_PREVENT_CONTACT) continue;
-            row[(uint64_t)n * nS] = ot.id;  // n <= k: compaction never overwrites an unread survivor
-            n++;
-            atomicAdd(&bCount[ob], 1u);
+    if (nc == 0) return;
+    double rcdMe = (double)me.r;
+    rcdMe += (double)P.beta;
+    const BinRange gMe = bin_range(P, me.x, me.y, me.z, rcdMe);
+    const uint2 mine = sortedMeta[i];  // owner, family
+    for (uint32_t k = 0; k < nc; k++) {
+        const uint32_t j = col[(uint64_t)k * nS];
+        const CdRec ot = load_cd(sorted, j);
+        if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) continue;
+        // the rule is evaluated with the lower id as A, whichever lane meets the pair
+        const bool meLow = me.id < ot.id;
+        bool hit;
+        if (meLow) {
+            hit = bin_rule_point(P, me, gMe, ot);
+        } else {
+            double rcdO = (double)ot.r;
+            rcdO += (double)P.beta;
+            hit = bin_rule_point(P, ot, bin_range(P, ot.x, ot.y, ot.z, rcdO), me);
         }
+        if (!hit) continue;
+        const uint2 theirs = sortedMeta[j];
+        if (theirs.x == mine.x) continue;
+        if (mask[mask_index(mine.y, theirs.y)] != DEME_DONT_PREVENT_CONTACT) continue;
+        const uint32_t loIdx = meLow ? i : j, hiIdx = meLow ? j : i;
+        const uint32_t loId = meLow ? me.id : ot.id, hiId = meLow ? ot.id : me.id;
+        append_pair(aCnt, rowSlab, loIdx, hiId, nS, slabCap, ctr);
+        append_pair(bCnt, mirSlab, hiIdx, loId, nS, slabCap, ctr);
     }
-    pairCnt[i] = n;
 }
 
-// Canonical rows: plane contacts (ascending plane index) then partners (ascending id); idA for the row.
-__global__ __launch_bounds__(256) void k_cell_rows(uint32_t nS, const uint32_t* __restrict__ slab,
+// Canonical rows: plane contacts (ascending plane index) then partners (ascending id); idA for the row. Mirror rows:
+// the sphere's lower-id partners, ascending. Both come from the sphere's slab columns (sorted position).
+__device__ __forceinline__ void sorted_copy(const uint32_t* __restrict__ src, uint32_t n, uint32_t nS,
+                                            uint32_t* __restrict__ dst) {
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t v = src[(uint64_t)k * nS];
+        uint32_t q = k;
+        while (q > 0 && dst[q - 1] > v) {
+            dst[q] = dst[q - 1];
+            q--;
+        }
+        dst[q] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_rows(uint32_t nS, const uint32_t* __restrict__ rowSlab,
+                                                   const uint32_t* __restrict__ mirSlab,
                                                    const uint32_t* __restrict__ posOf,
                                                    const uint32_t* __restrict__ rowStart,
-                                                   const uint32_t* __restrict__ numAnal,
                                                    const unsigned long long* __restrict__ planeBits,
-                                                   uint32_t* __restrict__ idA, uint32_t* __restrict__ idB,
-                                                   uint8_t* __restrict__ ctype) {
+                                                   const uint32_t* __restrict__ bRowStart, uint32_t* __restrict__ idA,
+                                                   uint32_t* __restrict__ idB, uint8_t* __restrict__ ctype,
+                                                   uint32_t* __restrict__ bPart) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
     const uint32_t st = rowStart[s], en = rowStart[s + 1];
     uint32_t pos = st;
     unsigned long long bits = planeBits[s];
-    while (bits) {
+    while (bits && pos < en) {
         const int a = __ffsll(bits) - 1;
         bits &= bits - 1;
         idB[pos] = (uint32_t)a;
@@ -378,16 +419,10 @@ __global__ __launch_bounds__(256) void k_cell_rows(uint32_t nS, const uint32_t* 
         pos++;
     }
     const uint32_t sp = pos;
-    const uint32_t* src = slab + posOf[s];  // the sphere's slab column (sorted position)
-    for (uint32_t k = 0; sp + k < en; k++) {
-        const uint32_t v = src[(uint64_t)k * nS];
-        uint32_t q = sp + k;
-        while (q > sp && idB[q - 1] > v) {
-            idB[q] = idB[q - 1];
-            q--;
-        }
-        idB[q] = v;
-    }
+    const uint32_t p = posOf[s];
+    sorted_copy(rowSlab + p, en - sp, nS, idB + sp);
+    const uint32_t bs = bRowStart[s];
+    sorted_copy(mirSlab + p, bRowStart[s + 1] - bs, nS, bPart + bs);
     for (uint32_t q = sp; q < en; q++) ctype[q] = DEME_SPHERE_SPHERE_CONTACT;
     for (uint32_t q = st; q < en; q++) idA[q] = s;
 }
@@ -409,28 +444,30 @@ hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereR
 
 hipError_t launch_cell_scatter(uint32_t nS, const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
                                const CdRec* rec, const uint2* meta, CdRec* sorted, uint2* sortedMeta, uint32_t* posOf,
-                               hipStream_t st) {
+                               uint32_t* aCnt, uint32_t* bCnt, hipStream_t st) {
     if (nS == 0) return hipSuccess;
     k_cell_scatter<<<blocks(nS, 256), 256, 0, st>>>(nS, cellOf, slotOf, cellStart, rec, meta, sorted, sortedMeta,
-                                                    posOf);
+                                                    posOf, aCnt, bCnt);
     return hipGetLastError();
 }
 
 hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
                              const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
-                             uint32_t* slab, uint32_t* pairCnt, uint32_t* bCount, DevCounters* ctr, hipStream_t st) {
+                             uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
+                             DevCounters* ctr, hipStream_t st) {
     if (nS == 0) return hipSuccess;
     const uint32_t nb = blocks(nS, PAIR_BLOCK);
     k_cell_pairs<<<8 * ((nb + 7) / 8), PAIR_BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedMeta, cellStart, mask, slabCap,
-                                                            slab, pairCnt, bCount, ctr);
+                                                            surv, rowSlab, mirSlab, aCnt, bCnt, ctr);
     return hipGetLastError();
 }
 
-hipError_t launch_cell_rows(uint32_t nS, const uint32_t* slab, const uint32_t* posOf, const uint32_t* rowStart,
-                            const uint32_t* numAnal, const unsigned long long* planeBits, uint32_t* idA, uint32_t* idB,
-                            uint8_t* ctype, hipStream_t st) {
+hipError_t launch_cell_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* mirSlab, const uint32_t* posOf,
+                            const uint32_t* rowStart, const unsigned long long* planeBits, const uint32_t* bRowStart,
+                            uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* bPart, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cell_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, slab, posOf, rowStart, numAnal, planeBits, idA, idB, ctype);
+    k_cell_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, rowSlab, mirSlab, posOf, rowStart, planeBits, bRowStart, idA, idB,
+                                                 ctype, bPart);
     return hipGetLastError();
 }
 

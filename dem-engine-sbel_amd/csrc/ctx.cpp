@@ -86,7 +86,10 @@ struct deme_ctx {
     DBuf<CdRec> rec;
     DBuf<uint2> binEnt;
     DBuf<uint2> pairs;
-    DBuf<uint32_t> rowCount, bCount, bStart, bList;
+    DBuf<uint32_t> rowCount;
+    // mirror rows: per sphere, the A ids of the contacts in which it is body B (ascending), and each sphere-sphere
+    // contact's slot there
+    DBuf<uint32_t> bCnt, bRowStart, bPart, mirror;
     DBuf<uint64_t> scanScratch;
     DBuf<DevCounters> ctr;
     DevCounters* hctr = nullptr;  // pinned mirror
@@ -96,15 +99,17 @@ struct deme_ctx {
     DBuf<uint8_t> ctype[2];
     DBuf<float4> hist[2];
     DBuf<uint32_t> idA, map;
-    DBuf<ForceRec> frc;
+    DBuf<ForceRec> frc;           // full contact records, kept by deme_force only
+    DBuf<float> aTerm, bTerm, gTerm;  // owner terms: A side per contact, B side per mirror slot / per plane contact
+    bool forcesFull = false;
+    bool termsReady = false;  // owner terms belong to the current contact list
     DBuf<float> analPartials, analAcc;
     // cell-list detection
     CellGrid grid{};
     uint64_t nCells = 0;
     float maxRadius = 0.f;
     uint32_t slabCap = 32;
-    bool countsStale = false;
-    DBuf<uint32_t> cellCount, cellStart, cellOf, slotOf, posOf, pairCnt, slab;
+    DBuf<uint32_t> cellCount, cellStart, cellOf, slotOf, posOf, aCnt, survSlab, rowSlab, mirSlab;
     DBuf<CdRec> sorted;
     DBuf<uint2> meta, sortedMeta;
     DBuf<unsigned long long> planeBits;
@@ -279,15 +284,16 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->sph.release(); c->own.release(); c->moi.release(); c->ownerSphStart.release(); c->anal.release();
     c->pose.release(); c->mats.release(); c->rules.release(); c->mask.release(); c->acc.release();
     c->binCount.release(); c->binStart.release(); c->numBins.release(); c->numAnal.release(); c->rec.release();
-    c->binEnt.release(); c->pairs.release(); c->rowCount.release(); c->bCount.release(); c->bStart.release();
-    c->bList.release(); c->scanScratch.release(); c->ctr.release();
+    c->binEnt.release(); c->pairs.release(); c->rowCount.release(); c->bCnt.release(); c->bRowStart.release();
+    c->bPart.release(); c->mirror.release(); c->scanScratch.release(); c->ctr.release();
+    c->aTerm.release(); c->bTerm.release(); c->gTerm.release();
     for (int k = 0; k < 2; k++) {
         c->rowStart[k].release(); c->idB[k].release(); c->ctype[k].release(); c->hist[k].release();
     }
     c->idA.release(); c->map.release(); c->frc.release(); c->analPartials.release(); c->analAcc.release();
     c->cellCount.release(); c->cellStart.release(); c->cellOf.release(); c->slotOf.release(); c->posOf.release();
-    c->slab.release(); c->sorted.release(); c->planeBits.release(); c->pairCnt.release(); c->meta.release();
-    c->sortedMeta.release();
+    c->survSlab.release(); c->rowSlab.release(); c->mirSlab.release(); c->sorted.release(); c->planeBits.release();
+    c->aCnt.release(); c->meta.release(); c->sortedMeta.release();
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -383,8 +389,8 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     DEME_HIP(c, c->numAnal.ensure(nS, 1.0));
     DEME_HIP(c, c->rec.ensure(nS, 1.0));
     DEME_HIP(c, c->rowCount.ensure(nS, 1.0));
-    DEME_HIP(c, c->bCount.ensure(nClumps + 1, 1.0));
-    DEME_HIP(c, c->bStart.ensure(nClumps + 1, 1.0));
+    DEME_HIP(c, c->bCnt.ensure(nS + 1, 1.0));
+    DEME_HIP(c, c->bRowStart.ensure(nS + 1, 1.0));
     for (int k = 0; k < 2; k++) DEME_HIP(c, c->rowStart[k].ensure(nS + 1, 1.0));
     DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>({c->nBins, (uint64_t)nS, 1u << 22})), 1.0));
     if (nS) DEME_HIP(c, hipMemcpyAsync(c->sph.p, hs.data(), nS * sizeof(SphereRec), hipMemcpyHostToDevice, c->stream));
@@ -392,7 +398,7 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     DEME_HIP(c, hipMemcpyAsync(c->ownerSphStart.p, start.data(), (nClumps + 1) * sizeof(uint32_t),
                                hipMemcpyHostToDevice, c->stream));
     DEME_HIP(c, hipMemsetAsync(c->rowCount.p, 0, std::max<size_t>(nS, 1) * sizeof(uint32_t), c->stream));
-    DEME_HIP(c, hipMemsetAsync(c->bCount.p, 0, (nClumps + 1) * sizeof(uint32_t), c->stream));
+    DEME_HIP(c, hipMemsetAsync(c->bCnt.p, 0, (nS + 1) * sizeof(uint32_t), c->stream));
     DEME_HIP(c, hipMemsetAsync(c->acc.p, 0, std::max<size_t>((size_t)6 * nO, 1) * sizeof(float), c->stream));
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     c->nS = nS;
@@ -589,17 +595,17 @@ int deme_cd_bin_fill(deme_ctx* c) {
 int deme_cd_pairs(deme_ctx* c) {
     if (c->cdStage != 2) return c->fail(DEME_ERR_NOT_READY, "deme_cd_bin_fill must run first");
     uint64_t cap = std::max<uint64_t>({(uint64_t)(c->nPairs * 1.25) + 4096, c->pairs.cap, c->P / 2 + 4096});
-    // This stage accumulates per-row and per-owner counts with atomics; the cell-list pass stores its own row counts
-    // into rowCount, so both counters are cleared here rather than assumed to be zero.
+    // This stage accumulates per-sphere row and mirror-row counts with atomics; other passes leave their own values
+    // in both arrays, so they are cleared here rather than assumed to be zero.
     DEME_LAUNCH(c, "fill", launch_fill_u32(c->rowCount.p, c->nS, 0, c->stream));
-    DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCount.p, c->nClumps + 1, 0, c->stream));
+    DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, c->nS, 0, c->stream));
     for (int attempt = 0; attempt < 3; attempt++) {
         DEME_HIP(c, c->pairs.ensure(cap, 1.0));
         DEME_HIP(c, hipMemsetAsync(&c->ctr.p->pairCount, 0, sizeof(uint32_t), c->stream));
         DEME_LAUNCH(c, "bin_pairs",
                     launch_bin_pairs(c->kp, c->binEnt.p, c->P, c->binStart.p, c->rec.p, c->sph.p, c->own.p, c->mask.p,
                                      c->pairs.p, (uint32_t)std::min<uint64_t>(c->pairs.cap, 0xFFFFFFFFull), c->ctr.p,
-                                     c->rowCount.p, c->bCount.p, c->stream));
+                                     c->rowCount.p, c->bCnt.p, c->stream));
         int r = readback_counters(c);
         if (r) return r;
         const uint64_t found = c->hctr->pairCount;
@@ -610,7 +616,7 @@ int deme_cd_pairs(deme_ctx* c) {
         }
         // capacity exceeded: clear the per-row counts and redo the pass with room for every hit
         DEME_LAUNCH(c, "fill", launch_fill_u32(c->rowCount.p, c->nS, 0, c->stream));
-        DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCount.p, c->nClumps + 1, 0, c->stream));
+        DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, c->nS, 0, c->stream));
         cap = found + found / 4 + 4096;
     }
     c->cdStage = 0;
@@ -640,8 +646,12 @@ int deme_cd_sort_remap(deme_ctx* c) {
     DEME_HIP(c, c->hist[cu].ensure(C));
     DEME_HIP(c, c->idA.ensure(C));
     DEME_HIP(c, c->map.ensure(C));
-    DEME_HIP(c, c->frc.ensure(C));
-    DEME_HIP(c, c->bList.ensure(c->nPairs));
+    DEME_HIP(c, c->mirror.ensure(C));
+    DEME_HIP(c, c->bPart.ensure(c->nPairs));
+    DEME_HIP(c, c->aTerm.ensure((size_t)6 * C));
+    DEME_HIP(c, c->gTerm.ensure((size_t)6 * C));
+    DEME_HIP(c, c->bTerm.ensure((size_t)6 * c->nPairs));
+    c->termsReady = false;
     DEME_LAUNCH(c, "row_fill",
                 launch_row_fill(c->pairs.p, (uint32_t)c->nPairs, c->rowStart[cu].p, c->numAnal.p, c->rowCount.p,
                                 c->idB[cu].p, c->ctype[cu].p, c->stream));
@@ -650,19 +660,21 @@ int deme_cd_sort_remap(deme_ctx* c) {
                                 c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p, c->stream));
     DEME_LAUNCH(c, "row_finalize",
                 launch_row_finalize(c->nS, c->rowStart[cu].p, c->numAnal.p, c->idA.p, c->idB[cu].p, c->stream));
+    DEME_LAUNCH(c, "scan_brows",
+                scan_u32(c->bCnt.p, nullptr, c->nS, c->bRowStart.p, &c->ctr.p->total[2], c->scanScratch.p,
+                         c->stream));
+    DEME_LAUNCH(c, "brow_fill",
+                launch_brow_fill((uint32_t)C, c->nS, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->bRowStart.p,
+                                 c->bCnt.p, c->bPart.p, c->stream));
+    DEME_LAUNCH(c, "brow_sort", launch_brow_sort(c->nS, c->bRowStart.p, c->bPart.p, c->stream));
     DEME_LAUNCH(c, "history_map",
                 launch_history_map(c->nS, c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p,
                                    c->havePrev ? c->rowStart[pv].p : nullptr, c->havePrev ? c->idB[pv].p : nullptr,
-                                   c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->stream));
-    DEME_LAUNCH(c, "scan_owners",
-                scan_u32(c->bCount.p, nullptr, c->nClumps, c->bStart.p, &c->ctr.p->total[2], c->scanScratch.p,
-                         c->stream));
-    DEME_LAUNCH(c, "bside_fill",
-                launch_bside_fill((uint32_t)C, c->nS, c->idB[cu].p, c->ctype[cu].p, c->sph.p, c->bStart.p, c->bCount.p,
-                                  c->bList.p, c->stream));
-    DEME_LAUNCH(c, "bside_sort", launch_bside_sort(c->nClumps, c->bStart.p, c->bList.p, c->stream));
+                                   c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->bRowStart.p, c->bPart.p,
+                                   c->mirror.p, c->stream));
     c->contactsReady = true;
     c->fresh = true;
+    c->forcesFull = false;
     c->cdStage = 4;
     return DEME_OK;
 }
@@ -717,17 +729,17 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->cellOf.ensure(nS));
     DEME_HIP(c, c->slotOf.ensure(nS));
     DEME_HIP(c, c->posOf.ensure(nS));
-    DEME_HIP(c, c->pairCnt.ensure(nS));
+    DEME_HIP(c, c->aCnt.ensure(nS));
     DEME_HIP(c, c->meta.ensure(nS));
     DEME_HIP(c, c->sortedMeta.ensure(nS));
     DEME_HIP(c, c->sorted.ensure(nS));
     DEME_HIP(c, c->planeBits.ensure(nS));
-    DEME_HIP(c, c->slab.ensure((size_t)nS * c->slabCap, 1.0));
+    DEME_HIP(c, c->survSlab.ensure((size_t)nS * c->slabCap, 1.0));
+    DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
+    DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, hipMemsetAsync(c->ctr.p, 0, sizeof(DevCounters), c->stream));
     DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
     DEME_LAUNCH(c, "fill", launch_fill_u32(c->cellCount.p, c->nCells, 0, c->stream));
-    // per-owner B-side counts are accumulated below; clear them whatever the previous pass left
-    DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCount.p, c->nClumps + 1, 0, c->stream));
     DEME_LAUNCH(c, "cell_assign",
                 launch_cell_assign(c->kp, c->grid, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->mask.p,
                                    c->cellCount.p, c->cellOf.p, c->slotOf.p, c->numBins.p, c->numAnal.p,
@@ -737,7 +749,7 @@ int deme_cd_run(deme_ctx* c) {
                          c->stream));
     DEME_LAUNCH(c, "cell_scatter",
                 launch_cell_scatter(nS, c->cellOf.p, c->slotOf.p, c->cellStart.p, c->rec.p, c->meta.p, c->sorted.p,
-                                    c->sortedMeta.p, c->posOf.p, c->stream));
+                                    c->sortedMeta.p, c->posOf.p, c->aCnt.p, c->bCnt.p, c->stream));
     // the current list becomes the previous one
     if (c->contactsReady) {
         c->cur ^= 1;
@@ -750,11 +762,15 @@ int deme_cd_run(deme_ctx* c) {
     for (int attempt = 0;; attempt++) {
         DEME_LAUNCH(c, "cell_pairs",
                     launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, c->sortedMeta.p, c->cellStart.p, c->mask.p,
-                                      c->slabCap, c->slab.p, c->pairCnt.p, c->bCount.p, c->ctr.p, c->stream));
-        // row lengths in id order: partner counts gathered from sorted positions, plus plane contacts
+                                      c->slabCap, c->survSlab.p, c->rowSlab.p, c->mirSlab.p, c->aCnt.p, c->bCnt.p,
+                                      c->ctr.p, c->stream));
+        // row lengths in id order: partner counts gathered from sorted positions, plus plane contacts; mirror rows
         DEME_LAUNCH(c, "scan_rows",
-                    scan_u32(c->pairCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1],
+                    scan_u32(c->aCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1],
                              c->scanScratch.p, c->stream, c->posOf.p));
+        DEME_LAUNCH(c, "scan_brows",
+                    scan_u32(c->bCnt.p, nullptr, nS, c->bRowStart.p, &c->ctr.p->total[2], c->scanScratch.p,
+                             c->stream, c->posOf.p));
         r = readback_counters(c);
         if (r) return r;
         const DevCounters& h = *c->hctr;
@@ -768,37 +784,41 @@ int deme_cd_run(deme_ctx* c) {
         if (attempt >= 2) return c->fail(DEME_ERR_CAPACITY, "partner rows could not be sized");
         // a sphere has more partners than the slab row holds: widen the rows and redo the pass
         c->slabCap = (uint32_t)std::min<uint64_t>(((uint64_t)h.maxOcc * 5 / 4 + 7) & ~7ull, 65535);
-        DEME_HIP(c, c->slab.ensure((size_t)nS * c->slabCap, 1.0));
-        DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCount.p, c->nClumps + 1, 0, c->stream));
+        DEME_HIP(c, c->survSlab.ensure((size_t)nS * c->slabCap, 1.0));
+        DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
+        DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
+        DEME_LAUNCH(c, "fill", launch_fill_u32(c->aCnt.p, nS, 0, c->stream));
+        DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, nS, 0, c->stream));
         DEME_HIP(c, hipMemsetAsync(&c->ctr.p->maxOcc, 0, sizeof(uint32_t), c->stream));
     }
-    const uint64_t C = c->hctr->total[1];
+    const uint64_t C = c->hctr->total[1], nPairs = c->hctr->total[2];
     if (C > 0xFFFFFFFFull) return c->fail(DEME_ERR_CAPACITY, "contact count exceeds contactPairs_t (2^32 - 1)");
+    if (nPairs > C) return c->fail(DEME_ERR_HIP, "inconsistent contact and mirror-row counts");
     c->C = C;
+    c->nPairs = nPairs;
+    c->nGeo = C - nPairs;
     DEME_HIP(c, c->idB[cu].ensure(C));
     DEME_HIP(c, c->ctype[cu].ensure(C));
     DEME_HIP(c, c->hist[cu].ensure(C));
     DEME_HIP(c, c->idA.ensure(C));
     DEME_HIP(c, c->map.ensure(C));
-    DEME_HIP(c, c->frc.ensure(C));
-    DEME_HIP(c, c->bList.ensure(C));
+    DEME_HIP(c, c->mirror.ensure(C));
+    DEME_HIP(c, c->bPart.ensure(nPairs));
+    DEME_HIP(c, c->aTerm.ensure((size_t)6 * C));
+    DEME_HIP(c, c->gTerm.ensure((size_t)6 * C));
+    DEME_HIP(c, c->bTerm.ensure((size_t)6 * nPairs));
+    c->termsReady = false;
     DEME_LAUNCH(c, "cell_rows",
-                launch_cell_rows(nS, c->slab.p, c->posOf.p, c->rowStart[cu].p, c->numAnal.p, c->planeBits.p, c->idA.p,
-                                 c->idB[cu].p, c->ctype[cu].p, c->stream));
+                launch_cell_rows(nS, c->rowSlab.p, c->mirSlab.p, c->posOf.p, c->rowStart[cu].p, c->planeBits.p,
+                                 c->bRowStart.p, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->bPart.p, c->stream));
     DEME_LAUNCH(c, "history_map",
                 launch_history_map(nS, c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p,
                                    c->havePrev ? c->rowStart[pv].p : nullptr, c->havePrev ? c->idB[pv].p : nullptr,
-                                   c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->stream));
-    DEME_LAUNCH(c, "scan_owners",
-                scan_u32(c->bCount.p, nullptr, c->nClumps, c->bStart.p, &c->ctr.p->total[2], c->scanScratch.p,
-                         c->stream));
-    DEME_LAUNCH(c, "bside_fill",
-                launch_bside_fill((uint32_t)C, c->nS, c->idB[cu].p, c->ctype[cu].p, c->sph.p, c->bStart.p, c->bCount.p,
-                                  c->bList.p, c->stream));
-    DEME_LAUNCH(c, "bside_sort", launch_bside_sort(c->nClumps, c->bStart.p, c->bList.p, c->stream));
+                                   c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->bRowStart.p, c->bPart.p,
+                                   c->mirror.p, c->stream));
     c->contactsReady = true;
     c->fresh = true;
-    c->countsStale = true;  // pair / plane split is read back lazily
+    c->forcesFull = false;
     c->cdStage = 0;
     return DEME_OK;
 }
@@ -846,35 +866,42 @@ int deme_download_contacts(deme_ctx* c, uint32_t* idA, uint32_t* idB, uint8_t* t
 // ---------------------------------------------------------------------------------------------------------------
 // Dynamics
 // ---------------------------------------------------------------------------------------------------------------
-int deme_force(deme_ctx* c) {
+static int force_pass(deme_ctx* c, bool full) {
     int r = check_ready(c);
     if (r) return r;
     if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "contact detection must run before the force pass");
     const int cu = c->cur, pv = cu ^ 1;
     const bool remap = c->fresh;
+    if (full) DEME_HIP(c, c->frc.ensure(c->C));
     DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
     DEME_LAUNCH(c, "force",
                 launch_force(c->kp, (uint32_t)c->C, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->sph.p, c->own.p,
-                             c->anal.p, c->pose.p, c->mats.p, remap ? (c->havePrev ? c->hist[pv].p : c->hist[cu].p)
-                                                                    : c->hist[cu].p,
-                             remap ? c->map.p : nullptr, c->hist[cu].p, c->frc.p, c->stream));
+                             c->moi.p, c->anal.p, c->pose.p, c->mats.p,
+                             remap ? (c->havePrev ? c->hist[pv].p : c->hist[cu].p) : c->hist[cu].p,
+                             remap ? c->map.p : nullptr, c->hist[cu].p, c->mirror.p, c->aTerm.p, c->bTerm.p,
+                             c->gTerm.p, full ? c->frc.p : nullptr, c->stream));
     c->fresh = false;
+    c->forcesFull = full;
+    c->termsReady = true;
     return DEME_OK;
 }
 
+int deme_force(deme_ctx* c) { return force_pass(c, true); }
+
 static int collect_and_update(deme_ctx* c, int integrate) {
+    if (!c->termsReady) return c->fail(DEME_ERR_NOT_READY, "the force pass must run on the current contact list first");
     const int cu = c->cur;
     uint32_t nb = 0;
     const uint32_t needBlocks = (uint32_t)((c->C + 4095) / 4096) + 1;
     DEME_HIP(c, c->analPartials.ensure((size_t)needBlocks * MAX_ANAL_OWNERS * 6));
     DEME_HIP(c, c->analAcc.ensure(MAX_ANAL_OWNERS * 6, 1.0));
     DEME_LAUNCH(c, "anal_collect",
-                launch_anal_collect((uint32_t)c->C, c->idB[cu].p, c->ctype[cu].p, c->frc.p, c->anal.p, c->own.p,
-                                    c->moi.p, c->nClumps, c->nAnalOwners, c->analPartials.p, &nb, c->stream));
+                launch_anal_collect((uint32_t)c->C, c->idB[cu].p, c->ctype[cu].p, c->gTerm.p, c->anal.p, c->nA,
+                                    c->nClumps, c->nAnalOwners, c->analPartials.p, &nb, c->stream));
     DEME_LAUNCH(c, "anal_final", launch_anal_final(c->analPartials.p, nb, c->nAnalOwners, c->analAcc.p, c->stream));
     DEME_LAUNCH(c, integrate ? "owner_update" : "owner_reduce",
                 launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
-                                    c->rowStart[cu].p, c->bStart.p, c->bList.p, c->frc.p, c->analAcc.p, c->moi.p,
+                                    c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, c->analAcc.p,
                                     c->rules.p, c->own.p, c->acc.p, integrate, (float)c->time, c->stream));
     return DEME_OK;
 }
@@ -902,7 +929,7 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
             r = deme_cd_run(c);
             if (r) return r;
         }
-        r = deme_force(c);
+        r = force_pass(c, false);
         if (r) return r;
         r = collect_and_update(c, 1);
         if (r) return r;
@@ -914,6 +941,8 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
 
 int deme_download_forces(deme_ctx* c, float* F, float* T, float* cA, float* cB) {
     if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "no contact list");
+    if (!c->forcesFull)
+        return c->fail(DEME_ERR_NOT_READY, "contact records are kept by deme_force only (deme_step stores owner terms)");
     std::vector<ForceRec> h(c->C);
     if (c->C) DEME_HIP(c, hipMemcpyAsync(h.data(), c->frc.p, c->C * sizeof(ForceRec), hipMemcpyDeviceToHost, c->stream));
     DEME_HIP(c, hipStreamSynchronize(c->stream));
@@ -972,13 +1001,6 @@ int deme_upload_wildcards(deme_ctx* c, const float* w) {
 
 int deme_get_counts(deme_ctx* c, deme_counts* out) {
     std::memset(out, 0, sizeof(*out));
-    if (c->countsStale) {
-        int r = readback_counters(c);
-        if (r) return r;
-        c->nPairs = c->hctr->total[2];
-        c->nGeo = c->C - c->nPairs;
-        c->countsStale = false;
-    }
     out->nSpheres = c->nS;
     out->nOwners = c->nO;
     out->nAnal = c->nA;

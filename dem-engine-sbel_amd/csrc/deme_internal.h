@@ -69,9 +69,10 @@ hipError_t launch_bin_stats(const uint32_t* binCount, uint64_t nBins, DevCounter
 hipError_t launch_bin_scatter(const KParams& P, const CdRec* rec, const SphereRec* sph, uint32_t nS,
                               const uint32_t* binStart, uint32_t* binCount, uint2* binEnt, const DevCounters* ctr,
                               hipStream_t st);
+// bCnt: per-sphere count of contacts in which the sphere is body B (sizes the mirror rows)
 hipError_t launch_bin_pairs(const KParams& P, const uint2* binEnt, uint64_t nEnt, const uint32_t* binStart,
                             const CdRec* rec, const SphereRec* sph, const OwnerRec* own, const uint8_t* mask,
-                            uint2* pairs, uint32_t pairCap, DevCounters* ctr, uint32_t* rowCount, uint32_t* bCount,
+                            uint2* pairs, uint32_t pairCap, DevCounters* ctr, uint32_t* rowCount, uint32_t* bCnt,
                             hipStream_t st);
 hipError_t launch_row_fill(const uint2* pairs, uint32_t nPairs, const uint32_t* rowStart, const uint32_t* numAnal,
                            uint32_t* rowCount, uint32_t* idB, uint8_t* ctype, hipStream_t st);
@@ -80,13 +81,16 @@ hipError_t launch_geo_fill(const KParams& P, const CdRec* rec, const SphereRec* 
                            const uint32_t* rowStart, uint32_t* idB, uint8_t* ctype, hipStream_t st);
 hipError_t launch_row_finalize(uint32_t nS, const uint32_t* rowStart, const uint32_t* numAnal, uint32_t* idA,
                                uint32_t* idB, hipStream_t st);
+// map: persistent-contact map to the previous list; mirror: slot of each sphere-sphere contact in its B sphere's
+// mirror row (bRowStart / bPart: per sphere, the A ids of its contacts as body B, ascending)
 hipError_t launch_history_map(uint32_t nS, const uint32_t* rowStart, const uint32_t* idB, const uint8_t* ctype,
                               const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
-                              uint32_t* map, hipStream_t st);
-hipError_t launch_bside_fill(uint32_t nC, uint32_t nS, const uint32_t* idB, const uint8_t* ctype,
-                             const SphereRec* sph, const uint32_t* bStart, uint32_t* bCount, uint32_t* bList,
-                             hipStream_t st);
-hipError_t launch_bside_sort(uint32_t nClumps, const uint32_t* bStart, uint32_t* bList, hipStream_t st);
+                              uint32_t* map, const uint32_t* bRowStart, const uint32_t* bPart, uint32_t* mirror,
+                              hipStream_t st);
+// mirror rows from a finished contact list (counts in bCnt, consumed)
+hipError_t launch_brow_fill(uint32_t nC, uint32_t nS, const uint32_t* idA, const uint32_t* idB, const uint8_t* ctype,
+                            const uint32_t* bRowStart, uint32_t* bCnt, uint32_t* bPart, hipStream_t st);
+hipError_t launch_brow_sort(uint32_t nS, const uint32_t* bRowStart, uint32_t* bPart, hipStream_t st);
 
 // cd_fast.hip
 hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereRec* sph, const OwnerRec* own,
@@ -94,34 +98,37 @@ hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereR
                               uint32_t* cellCount, uint32_t* cellOf, uint32_t* slotOf, uint16_t* numBins,
                               uint32_t* numAnal, unsigned long long* planeBits, CdRec* rec, uint2* meta,
                               DevCounters* ctr, hipStream_t st);
-// meta: (owner, family) per sphere; sortedMeta / sorted / posOf: the same in cell order
+// meta: (owner, family) per sphere; sortedMeta / sorted / posOf: the same in cell order; aCnt / bCnt (sorted order)
+// are cleared for the pair pass
 hipError_t launch_cell_scatter(uint32_t nS, const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
                                const CdRec* rec, const uint2* meta, CdRec* sorted, uint2* sortedMeta, uint32_t* posOf,
-                               hipStream_t st);
-// pairCnt: accepted partners per sorted position
+                               uint32_t* aCnt, uint32_t* bCnt, hipStream_t st);
+// Slabs are column-major by sorted position (entry k of sorted sphere i at slab[k * nS + i]). surv: prefilter
+// survivors; rowSlab / aCnt: higher-id partners (contact row); mirSlab / bCnt: lower-id partners (mirror row).
 hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
                              const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
-                             uint32_t* slab, uint32_t* pairCnt, uint32_t* bCount, DevCounters* ctr, hipStream_t st);
-// slab: column-major by sorted position (entry k of sorted sphere i at slab[k * nS + i]); posOf: sorted position
-hipError_t launch_cell_rows(uint32_t nS, const uint32_t* slab, const uint32_t* posOf, const uint32_t* rowStart,
-                            const uint32_t* numAnal, const unsigned long long* planeBits, uint32_t* idA, uint32_t* idB,
-                            uint8_t* ctype, hipStream_t st);
+                             uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
+                             DevCounters* ctr, hipStream_t st);
+hipError_t launch_cell_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* mirSlab, const uint32_t* posOf,
+                            const uint32_t* rowStart, const unsigned long long* planeBits, const uint32_t* bRowStart,
+                            uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* bPart, hipStream_t st);
 
 // dyn.hip
+// aTerm / gTerm: 6 floats per contact; bTerm: 6 floats per mirror entry; out (optional): full contact records
 hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, const uint32_t* idB, const uint8_t* ctype,
-                        const SphereRec* sph, const OwnerRec* own, const deme_anal_obj* anal, const PlanePose* pose,
-                        const deme_material* mats, const float4* histIn, const uint32_t* map, float4* histOut,
+                        const SphereRec* sph, const OwnerRec* own, const float4* moi, const deme_anal_obj* anal,
+                        const PlanePose* pose, const deme_material* mats, const float4* histIn, const uint32_t* map,
+                        float4* histOut, const uint32_t* mirror, float* aTerm, float* bTerm, float* gTerm,
                         ForceRec* out, hipStream_t st);
-hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const ForceRec* frc,
-                               const deme_anal_obj* anal, const OwnerRec* own, const float4* moi,
-                               uint32_t firstAnalOwner, uint32_t nAnalOwners, float* blockPartials,
-                               uint32_t* nBlocksOut, hipStream_t st);
+hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const float* gTerm,
+                               const deme_anal_obj* anal, uint32_t nA, uint32_t firstAnalOwner, uint32_t nAnalOwners,
+                               float* blockPartials, uint32_t* nBlocksOut, hipStream_t st);
 hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint32_t nAnalOwners, float* analAcc,
                              hipStream_t st);
 hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnalOwner,
                                uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
-                               const uint32_t* bStart, const uint32_t* bList, const ForceRec* frc,
-                               const float* analAcc, const float4* moi, const deme_family_rule* rules, OwnerRec* own,
+                               const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
+                               const float* analAcc, const deme_family_rule* rules, OwnerRec* own,
                                float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate, float t,
                                hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);

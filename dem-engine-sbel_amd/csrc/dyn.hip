@@ -61,21 +61,48 @@ __device__ __forceinline__ void mat_proxy(const deme_material& A, const deme_mat
     Crr = fmaxf(A.Crr, B.Crr);
 }
 
+// forceToAcc and forceToAngAcc (DEMCollectForceKernels.cu:68-120) of one contact on one owner: F * modifier / mass
+// and cross(cp, R^T((F + Ftof) * modifier)) / MOI, six floats.
+__device__ __forceinline__ void store_term(float* __restrict__ dst, f3 F, f3 T, f3 cp, float mass, float qw, float qx,
+                                           float qy, float qz, float4 I, float mod) {
+    const f3 lin = F * mod / mass;
+    f3 myF = (F + T) * mod;
+    rotq(myF, qw, -qx, -qy, -qz);
+    const f3 ang = cross(cp, myF) / mk(I.x, I.y, I.z);
+    float2* d = reinterpret_cast<float2*>(dst);
+    d[0] = make_float2(lin.x, lin.y);
+    d[1] = make_float2(lin.z, ang.x);
+    d[2] = make_float2(ang.y, ang.z);
+}
+
+__device__ __forceinline__ void zero_term(float* __restrict__ dst) {
+    float2* d = reinterpret_cast<float2*>(dst);
+    d[0] = d[1] = d[2] = make_float2(0.f, 0.f);
+}
+
 // Rows a14-a15: calculateContactForces (DEMCalcForceKernels.cu:14-179) with FullHertzianForceModel.cu.
 // History (delta_tan_x, delta_tan_y, delta_tan_z, delta_time) is one float4 per contact. On the first step after a
 // detection pass the previous history is gathered through the persistent-contact map (rearrangeContactWildcards,
 // DEMPrepForceKernels.cu:33-55) inside this kernel instead of in a separate pass.
+// The kernel also stores each contact's owner terms (the per-contact half of collectContactForces): the A body's at
+// aTerm[i], the B body's at bTerm[mirror[i]] (its slot in the B sphere's mirror row) for sphere-sphere contacts and
+// at gTerm[i] for sphere-plane contacts. FULL additionally keeps the contact record (force, torque-only force,
+// contact points in both body frames) for inspection.
+template <bool FULL>
 __global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uint32_t* __restrict__ idA,
                                                const uint32_t* __restrict__ idB, const uint8_t* __restrict__ ctype,
                                                const SphereRec* __restrict__ sph, const OwnerRec* __restrict__ own,
-                                               const deme_anal_obj* __restrict__ anal,
+                                               const float4* __restrict__ moi, const deme_anal_obj* __restrict__ anal,
                                                const PlanePose* __restrict__ pose,
                                                const deme_material* __restrict__ mats, const float4* histIn,
                                                const uint32_t* __restrict__ map, float4* histOut,
+                                               const uint32_t* __restrict__ mirror, float* __restrict__ aTerm,
+                                               float* __restrict__ bTerm, float* __restrict__ gTerm,
                                                ForceRec* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nC) return;
-    uint32_t type = ctype[i];
+    const uint32_t listType = ctype[i];
+    uint32_t type = listType;
     const float ts = P.h;
     {
         // guard: ids outside their tables never reach a gather (a malformed list yields a zero record)
@@ -84,11 +111,15 @@ __global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uin
                                      (type == DEME_SPHERE_PLANE_CONTACT && b < P.nA));
         if (!ok) {
             const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-            float4* o4 = reinterpret_cast<float4*>(out + i);
-            o4[0] = z;
-            o4[1] = z;
-            o4[2] = z;
+            if (FULL) {
+                float4* o4 = reinterpret_cast<float4*>(out + i);
+                o4[0] = z;
+                o4[1] = z;
+                o4[2] = z;
+            }
             histOut[i] = z;
+            zero_term(aTerm + (uint64_t)i * 6);
+            if (type == DEME_SPHERE_PLANE_CONTACT) zero_term(gTerm + (uint64_t)i * 6);
             return;
         }
     }
@@ -106,8 +137,8 @@ __global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uin
 
     // ---- body B ----
     double box, boy, boz;
-    float BRadius, BMass;
-    uint32_t matB;
+    float BRadius, BMass, BOwnerMass;
+    uint32_t matB, BOwner;
     float Bqw, Bqx, Bqy, Bqz;
     f3 BLinVel, BRotVel;
     double cpx = 0, cpy = 0, cpz = 0, overlapDepth = 0;
@@ -122,6 +153,8 @@ __global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uin
         const double XB = box + (double)qx, YB = boy + (double)qy, ZB = boz + (double)qz;
         BRadius = sb.r;
         BMass = ob.mass;
+        BOwnerMass = ob.mass;
+        BOwner = sb.owner;
         matB = sb.mat;
         Bqw = ob.qw;
         Bqx = ob.qx;
@@ -155,6 +188,8 @@ __global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uin
         owner_position(P, ob.voxel, ob.lx, ob.ly, ob.lz, box, boy, boz);
         BRadius = (float)HUGE_FLOAT;
         BMass = ab.mass;
+        BOwnerMass = ob.mass;
+        BOwner = ab.owner;
         matB = ab.material;
         Bqw = ob.qw;
         Bqx = ob.qx;
@@ -191,10 +226,10 @@ __global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uin
     float delta_tan_x = w.x, delta_tan_y = w.y, delta_tan_z = w.z, delta_time = w.w;
 
     f3 force = mk(0, 0, 0), tof = mk(0, 0, 0);
-    ForceRec fr;
+    f3 locCPA = mk(0, 0, 0), locCPB = mk(0, 0, 0);
     if (type != DEME_NOT_A_CONTACT) {
-        f3 locCPA = mk((float)(cpx - aox), (float)(cpy - aoy), (float)(cpz - aoz));
-        f3 locCPB = mk((float)(cpx - box), (float)(cpy - boy), (float)(cpz - boz));
+        locCPA = mk((float)(cpx - aox), (float)(cpy - aoy), (float)(cpz - aoz));
+        locCPB = mk((float)(cpx - box), (float)(cpy - boy), (float)(cpz - boz));
         rotq(locCPA, oa.qw, -oa.qx, -oa.qy, -oa.qz);
         rotq(locCPB, Bqw, -Bqx, -Bqy, -Bqz);
         const f3 ALinVel = mk(oa.vx, oa.vy, oa.vz), ARotVel = mk(oa.wx, oa.wy, oa.wz);
@@ -265,48 +300,51 @@ __global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uin
         delta_tan_x = delta_tan.x;
         delta_tan_y = delta_tan.y;
         delta_tan_z = delta_tan.z;
+    } else {
+        delta_tan_x = delta_tan_y = delta_tan_z = delta_time = 0.f;
+    }
+    if (FULL) {
+        ForceRec fr;
+        fr.fx = force.x;
+        fr.fy = force.y;
+        fr.fz = force.z;
+        fr.tx = tof.x;
+        fr.ty = tof.y;
+        fr.tz = tof.z;
         fr.ax = locCPA.x;
         fr.ay = locCPA.y;
         fr.az = locCPA.z;
         fr.bx = locCPB.x;
         fr.by = locCPB.y;
         fr.bz = locCPB.z;
-    } else {
-        delta_tan_x = delta_tan_y = delta_tan_z = delta_time = 0.f;
-        fr.ax = fr.ay = fr.az = fr.bx = fr.by = fr.bz = 0.f;
+        float4* o4 = reinterpret_cast<float4*>(out + i);
+        const float4* f4 = reinterpret_cast<const float4*>(&fr);
+        o4[0] = f4[0];
+        o4[1] = f4[1];
+        o4[2] = f4[2];
     }
-    fr.fx = force.x;
-    fr.fy = force.y;
-    fr.fz = force.z;
-    fr.tx = tof.x;
-    fr.ty = tof.y;
-    fr.tz = tof.z;
-    float4* o4 = reinterpret_cast<float4*>(out + i);
-    const float4* f4 = reinterpret_cast<const float4*>(&fr);
-    o4[0] = f4[0];
-    o4[1] = f4[1];
-    o4[2] = f4[2];
     histOut[i] = make_float4(delta_tan_x, delta_tan_y, delta_tan_z, delta_time);
+    // owner terms: A side +1, B side -1
+    store_term(aTerm + (uint64_t)i * 6, force, tof, locCPA, AMass, oa.qw, oa.qx, oa.qy, oa.qz, moi[sa.owner], 1.f);
+    if (listType == DEME_SPHERE_SPHERE_CONTACT) {
+        const uint32_t m = mirror[i];
+        if (m != DEME_NULL_MAPPING_PARTNER)
+            store_term(bTerm + (uint64_t)m * 6, force, tof, locCPB, BOwnerMass, Bqw, Bqx, Bqy, Bqz, moi[BOwner], -1.f);
+    } else {
+        store_term(gTerm + (uint64_t)i * 6, force, tof, locCPB, BOwnerMass, Bqw, Bqx, Bqy, Bqz, moi[BOwner], -1.f);
+    }
 }
 
-// forceToAngAcc (DEMCollectForceKernels.cu:87-120): cross(cp, R^T((F + Ftof) * modifier)) / MOI
-__device__ __forceinline__ f3 ang_term(const OwnerRec& o, const float4& I, f3 cp, f3 F, f3 T, float mod) {
-    f3 myF = (F + T) * mod;
-    rotq(myF, o.qw, -o.qx, -o.qy, -o.qz);
-    return cross(cp, myF) / mk(I.x, I.y, I.z);
-}
-
-// Row a16 for analytical owners (planes): every sphere-plane contact contributes its B-side term to the plane's
-// owner. Deterministic two-level reduction: each lane sums 16 consecutive contacts in order, lanes combine in a fixed
-// tree, tiles combine sequentially in k_anal_final.
+// Row a16 for analytical owners (planes): every sphere-plane contact contributes its B-side term (stored by the
+// force kernel) to the plane's owner. Deterministic two-level reduction: each lane sums 16 consecutive contacts in
+// order, lanes combine in a fixed tree, tiles combine in a fixed tree in k_anal_final.
 constexpr uint32_t ANAL_ITEMS = 16;
 constexpr uint32_t ANAL_TILE = 256 * ANAL_ITEMS;
 
 __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_t* __restrict__ idB,
                                                       const uint8_t* __restrict__ ctype,
-                                                      const ForceRec* __restrict__ frc,
-                                                      const deme_anal_obj* __restrict__ anal,
-                                                      const OwnerRec* __restrict__ own, const float4* __restrict__ moi,
+                                                      const float* __restrict__ gTerm,
+                                                      const deme_anal_obj* __restrict__ anal, uint32_t nA,
                                                       uint32_t firstAnal, uint32_t nAnalOwners,
                                                       float* __restrict__ partials) {
     __shared__ float red[256][6];
@@ -321,23 +359,21 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
         const uint64_t c = c0 + it;
         if (c >= nC) break;
         if (ctype[c] != DEME_SPHERE_PLANE_CONTACT) continue;
+        const uint32_t b = idB[c];
+        if (b >= nA) continue;
+        const uint32_t k = anal[b].owner - firstAnal;
+        const float2* g = reinterpret_cast<const float2*>(gTerm + c * 6);
+        const float2 g0 = g[0], g1 = g[1], g2 = g[2];
         any = true;
-        const uint32_t o = anal[idB[c]].owner;
-        const uint32_t k = o - firstAnal;
-        const OwnerRec orc = load_owner(own, o);
-        const ForceRec f = frc[c];
-        const f3 F = mk(f.fx, f.fy, f.fz), T = mk(f.tx, f.ty, f.tz), cp = mk(f.bx, f.by, f.bz);
-        const f3 la = F * -1.f / orc.mass;
-        const f3 lg = ang_term(orc, moi[o], cp, F, T, -1.f);
 #pragma unroll
         for (uint32_t kk = 0; kk < MAX_ANAL_OWNERS; kk++) {
             if (kk == k) {
-                acc[kk][0] += la.x;
-                acc[kk][1] += la.y;
-                acc[kk][2] += la.z;
-                acc[kk][3] += lg.x;
-                acc[kk][4] += lg.y;
-                acc[kk][5] += lg.z;
+                acc[kk][0] += g0.x;
+                acc[kk][1] += g0.y;
+                acc[kk][2] += g1.x;
+                acc[kk][3] += g1.y;
+                acc[kk][4] += g2.x;
+                acc[kk][5] += g2.y;
             }
         }
     }
@@ -346,14 +382,16 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
         if (threadIdx.x < nAnalOwners * 6) partials[(uint64_t)blockIdx.x * nAnalOwners * 6 + threadIdx.x] = 0.f;
         return;
     }
-    for (uint32_t k = 0; k < nAnalOwners; k++) {
+#pragma unroll
+    for (uint32_t k = 0; k < MAX_ANAL_OWNERS; k++) {
+        if (k >= nAnalOwners) break;
 #pragma unroll
         for (int c = 0; c < 6; c++) red[threadIdx.x][c] = acc[k][c];
         __syncthreads();
-        for (uint32_t s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s)
+        for (uint32_t st = 128; st > 0; st >>= 1) {
+            if (threadIdx.x < st)
 #pragma unroll
-                for (int c = 0; c < 6; c++) red[threadIdx.x][c] += red[threadIdx.x + s][c];
+                for (int c = 0; c < 6; c++) red[threadIdx.x][c] += red[threadIdx.x + st][c];
             __syncthreads();
         }
         if (threadIdx.x < 6) partials[((uint64_t)blockIdx.x * nAnalOwners + k) * 6 + threadIdx.x] = red[0][threadIdx.x];
@@ -377,36 +415,33 @@ __global__ __launch_bounds__(256) void k_anal_final(const float* __restrict__ pa
     if (t == 0) analAcc[v] = red[0];
 }
 
+__device__ __forceinline__ void add_term(f3& a, f3& al, const float* __restrict__ src) {
+    const float2* g = reinterpret_cast<const float2*>(src);
+    const float2 g0 = g[0], g1 = g[1], g2 = g[2];
+    a += mk(g0.x, g0.y, g1.x);
+    al += mk(g1.y, g2.x, g2.y);
+}
+
 // Rows a16 (clump owners) + a17: owner reduction fused with integrateOwners (DEMIntegrationKernels.cu:170-179).
-// A clump's A-side contacts are the contiguous rows of its spheres; its B-side contacts are listed in bList in
-// ascending contact order. Terms are summed sequentially in that order (A side, then B side) in float.
+// A clump's A-side terms are the contiguous rows of its spheres; its B-side terms are the contiguous mirror rows of
+// its spheres (ordered by B sphere, then A sphere). Terms are summed sequentially in that order (A side, then B side)
+// in float.
 __global__ __launch_bounds__(256) void k_owner_update(
     KParams P, uint32_t nO, uint32_t nClumps, uint32_t firstAnal, uint32_t nAnalOwners,
     const uint32_t* __restrict__ ownerSphStart, const uint32_t* __restrict__ rowStart,
-    const uint32_t* __restrict__ bStart, const uint32_t* __restrict__ bList, const ForceRec* __restrict__ frc,
-    const float* __restrict__ analAcc, const float4* __restrict__ moi, const deme_family_rule* __restrict__ rules,
-    OwnerRec* __restrict__ own, float* __restrict__ accOut, int doIntegrate, float t) {
+    const uint32_t* __restrict__ bRowStart, const float* __restrict__ aTerm, const float* __restrict__ bTerm,
+    const float* __restrict__ analAcc, const deme_family_rule* __restrict__ rules, OwnerRec* __restrict__ own,
+    float* __restrict__ accOut, int doIntegrate, float t) {
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nO) return;
     OwnerRec r = load_owner(own, o);
     f3 a = mk(0, 0, 0), al = mk(0, 0, 0);
     if (o < nClumps) {
-        const float4 I = moi[o];
-        const float m = r.mass;
-        const uint32_t c0 = rowStart[ownerSphStart[o]], c1 = rowStart[ownerSphStart[o + 1]];
-        for (uint32_t c = c0; c < c1; c++) {
-            const ForceRec f = frc[c];
-            const f3 F = mk(f.fx, f.fy, f.fz);
-            a += F * 1.f / m;
-            al += ang_term(r, I, mk(f.ax, f.ay, f.az), F, mk(f.tx, f.ty, f.tz), 1.f);
-        }
-        const uint32_t b0 = bStart[o], b1 = bStart[o + 1];
-        for (uint32_t k = b0; k < b1; k++) {
-            const ForceRec f = frc[bList[k]];
-            const f3 F = mk(f.fx, f.fy, f.fz);
-            a += F * -1.f / m;
-            al += ang_term(r, I, mk(f.bx, f.by, f.bz), F, mk(f.tx, f.ty, f.tz), -1.f);
-        }
+        const uint32_t s0 = ownerSphStart[o], s1 = ownerSphStart[o + 1];
+        const uint32_t c1 = rowStart[s1];
+        for (uint32_t c = rowStart[s0]; c < c1; c++) add_term(a, al, aTerm + (uint64_t)c * 6);
+        const uint32_t m1 = bRowStart[s1];
+        for (uint32_t m = bRowStart[s0]; m < m1; m++) add_term(a, al, bTerm + (uint64_t)m * 6);
         a = mk(0.f + a.x, 0.f + a.y, 0.f + a.z);
         al = mk(0.f + al.x, 0.f + al.y, 0.f + al.z);
     } else if (o - firstAnal < nAnalOwners) {
@@ -515,24 +550,28 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 }  // namespace
 
 hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, const uint32_t* idB, const uint8_t* ctype,
-                        const SphereRec* sph, const OwnerRec* own, const deme_anal_obj* anal, const PlanePose* pose,
-                        const deme_material* mats, const float4* histIn, const uint32_t* map, float4* histOut,
+                        const SphereRec* sph, const OwnerRec* own, const float4* moi, const deme_anal_obj* anal,
+                        const PlanePose* pose, const deme_material* mats, const float4* histIn, const uint32_t* map,
+                        float4* histOut, const uint32_t* mirror, float* aTerm, float* bTerm, float* gTerm,
                         ForceRec* out, hipStream_t st) {
     if (nC == 0) return hipSuccess;
-    k_force<<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, anal, pose, mats, histIn, map, histOut,
-                                             out);
+    if (out)
+        k_force<true><<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats, histIn,
+                                                       map, histOut, mirror, aTerm, bTerm, gTerm, out);
+    else
+        k_force<false><<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats,
+                                                        histIn, map, histOut, mirror, aTerm, bTerm, gTerm, nullptr);
     return hipGetLastError();
 }
 
-hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const ForceRec* frc,
-                               const deme_anal_obj* anal, const OwnerRec* own, const float4* moi,
-                               uint32_t firstAnalOwner, uint32_t nAnalOwners, float* blockPartials,
-                               uint32_t* nBlocksOut, hipStream_t st) {
+hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const float* gTerm,
+                               const deme_anal_obj* anal, uint32_t nA, uint32_t firstAnalOwner, uint32_t nAnalOwners,
+                               float* blockPartials, uint32_t* nBlocksOut, hipStream_t st) {
     const uint32_t nb = nC == 0 ? 0 : blocks(nC, ANAL_TILE);
     *nBlocksOut = nb;
     if (nb == 0 || nAnalOwners == 0) return hipSuccess;
-    k_anal_collect<<<nb, 256, 0, st>>>(nC, idB, ctype, frc, anal, own, moi, firstAnalOwner, nAnalOwners,
-                                       blockPartials);
+    if (nAnalOwners > MAX_ANAL_OWNERS) return hipErrorInvalidValue;
+    k_anal_collect<<<nb, 256, 0, st>>>(nC, idB, ctype, gTerm, anal, nA, firstAnalOwner, nAnalOwners, blockPartials);
     return hipGetLastError();
 }
 
@@ -546,12 +585,12 @@ hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint3
 
 hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnalOwner,
                                uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
-                               const uint32_t* bStart, const uint32_t* bList, const ForceRec* frc,
-                               const float* analAcc, const float4* moi, const deme_family_rule* rules, OwnerRec* own,
-                               float* acc, int doIntegrate, float t, hipStream_t st) {
+                               const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
+                               const float* analAcc, const deme_family_rule* rules, OwnerRec* own, float* acc,
+                               int doIntegrate, float t, hipStream_t st) {
     if (nO == 0) return hipSuccess;
     k_owner_update<<<blocks(nO, 256), 256, 0, st>>>(P, nO, nClumps, firstAnalOwner, nAnalOwners, ownerSphStart,
-                                                    rowStart, bStart, bList, frc, analAcc, moi, rules, own, acc,
+                                                    rowStart, bRowStart, aTerm, bTerm, analAcc, rules, own, acc,
                                                     doIntegrate, t);
     return hipGetLastError();
 }

@@ -1162,8 +1162,9 @@ void ora_force_fp64(const ora_ctx* c, double* Fout, double* Tout) {
 //
 // Per-contact terms are forceToAcc (F * modifier / mass) and forceToAngAcc (cross(cp, R^T((F + Ftof) * modifier)) /
 // MOI). The reference sums them with CUB ReduceByKey after a stable sort of [A-side entries; B-side entries] by owner.
-// Order fixed here: each clump owner sums its A-side terms in ascending contact index, then its B-side terms in
-// ascending contact index, sequentially in float. Analytical owners (planes) are summed in double and rounded once.
+// Order fixed here: each clump owner sums its A-side terms in ascending contact index, then its B-side terms ordered
+// by (B sphere id, A sphere id), sequentially in float. Analytical owners (planes) are summed in double and rounded
+// once.
 // ---------------------------------------------------------------------------------------------------------------
 namespace {
 
@@ -1199,6 +1200,14 @@ int ora_collect(ora_ctx* c) {
             aList[ca[c->sphOwner[c->idA[i]]]++] = (uint32_t)i;
             bList[cb[ownerB_of(c, i)]++] = (uint32_t)i;
         }
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t oo = 0; oo < (int64_t)nO; oo++) {
+        const uint32_t o = (uint32_t)oo;
+        if (c->isAnalOwner[o]) continue;
+        std::sort(bList.begin() + bCnt[o], bList.begin() + bCnt[o + 1], [c](uint32_t x, uint32_t y) {
+            return c->idB[x] != c->idB[y] ? c->idB[x] < c->idB[y] : c->idA[x] < c->idA[y];
+        });
     }
 #pragma omp parallel for schedule(static)
     for (int64_t oo = 0; oo < (int64_t)nO; oo++) {
