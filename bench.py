@@ -97,6 +97,54 @@ def run_device(sc, steps, warmup, update_freq):
     return t1 - t0, timing, info
 
 
+def run_batch(n_scenes, steps, warmup, update_freq, beta):
+    """Batched independent settling columns: one context stepping all scenes together, against the plain loop that
+    steps each scene in a context of its own (same world, same physics: each scene's trajectory is identical)."""
+    from deme_amd import scenes
+    from deme_amd.abi import DeviceSystem
+
+    sc = scenes.batched_columns(n_scenes=n_scenes, update_freq=update_freq, beta=beta)
+    dev = DeviceSystem(sc)
+    dev.step(warmup, update_freq)
+    # autotuner: contact-update interval (with its drift margin for the free-fall speed bound of the columns) and
+    # cell scale, timed on the live batch
+    vmax = float(np.sqrt(2 * 9.81 * 0.3))
+    update_freq, table = dev.autotune(freqs=(1, 2, 5, 10), cell_scales=(1.0, 1.25), trial_steps=10, max_vel=vmax,
+                                      safety=1.2)
+    f32 = np.float32
+    beta = float(f32(vmax) * f32(sc.params.h) * f32(update_freq) * f32(1.2)) if update_freq > 1 else 0.0
+    scale = (1.0, 1.25)[int(np.argmin(table[[1, 2, 5, 10].index(update_freq)]))]
+    dev.step(warmup, update_freq)
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.step(steps, update_freq)
+    dev.sync()
+    t_batch = time.perf_counter() - t0
+    contacts = dev.counts().nContacts
+    dev.close()
+    parts = [DeviceSystem(p) for p in scenes.split_batch(sc, own_world=True)]
+    for d in parts:
+        d.set_beta(beta)
+        d.set_cell_scale(scale)
+        d.step(warmup, update_freq)
+    for d in parts:
+        d.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for d in parts:
+            d.step(1, update_freq)
+    for d in parts:
+        d.sync()
+    t_loop = time.perf_counter() - t0
+    for d in parts:
+        d.close()
+    return {"scenes": n_scenes, "spheres": sc.n_spheres, "contacts": contacts, "update_freq": update_freq,
+            "beta_m": beta, "cell_scale": scale, "autotune_ms_per_step": table.round(4).tolist(), "steps": steps,
+            "scene_steps_per_s": n_scenes * steps / t_batch, "ms_per_step": 1e3 * t_batch / steps,
+            "loop_scene_steps_per_s": n_scenes * steps / t_loop, "loop_ms_per_step": 1e3 * t_loop / steps,
+            "speedup_vs_loop": t_loop / t_batch}
+
+
 def cpu_baseline(sc, steps, update_freq, threads):
     import pyoracle  # noqa: E402  (the oracle is the reported CPU baseline only)
 
@@ -118,6 +166,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-run-b", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the workload, copy a device buffer of this size once (a known-size reference "
                          "kernel for scaling hardware byte counters)")
@@ -151,6 +202,10 @@ def main():
                 "kernels_us_per_step": {k: round(1e3 * v[0] / args.steps, 2)
                                         for k, v in sorted(timingB.items(), key=lambda kv: -kv[1][0])},
                 "device_ms_per_step": round(totB / args.steps, 4)}
+
+    batch = None
+    if not args.skip_batch:
+        batch = run_batch(args.batch_scenes, args.batch_steps, 3, 1, 0.0)
 
     # dominant kernel of the headline run
     dom = max(timingA.items(), key=lambda kv: kv[1][0])
@@ -198,7 +253,7 @@ def main():
                                "contact detection every step",
                    "clumps": args.clumps, "spheres": scA.n_spheres, "contacts": infoA["C"],
                    "bin_sphere_touches": infoA["P"], "active_bins": infoA["Ba"],
-                   "run_every5_beta0.2r": runB, "kernels": kernels},
+                   "run_every5_beta0.2r": runB, "batched_columns": batch, "kernels": kernels},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": traffic,
                      "bytes_per_launch": bytes_per, "avg_launch_us": 1e3 * avg_ms},

@@ -114,7 +114,9 @@ __global__ __launch_bounds__(256) void k_sphere_bins(KParams P, const SphereRec*
     }
     uint32_t cnt = 0;
     const uint32_t famS = o.fam;
-    for (uint32_t a = 0; a < nA; a++) {
+    uint32_t a0, a1;
+    scene_planes(P, o.scene, a0, a1);
+    for (uint32_t a = a0; a < a1 && a < nA; a++) {
         const PlanePose pp = pose[a];
         if (mask[mask_index(famS, pp.fam)] != DEME_DONT_PREVENT_CONTACT) continue;
         if (plane_cd(P, pp, X, Y, Z, rCD)) cnt++;
@@ -287,9 +289,12 @@ __global__ __launch_bounds__(256) void k_geo_fill(KParams P, const CdRec* __rest
     const SphereRec sr = load_sphere(sph, s);
     double rCD = (double)sr.r;
     rCD += (double)P.beta;
-    const uint32_t famS = own[sr.owner].fam;
+    const OwnerRec o = load_owner(own, sr.owner);
+    const uint32_t famS = o.fam;
     uint32_t pos = rowStart[s];
-    for (uint32_t a = 0; a < nA; a++) {
+    uint32_t a0, a1;
+    scene_planes(P, o.scene, a0, a1);
+    for (uint32_t a = a0; a < a1 && a < nA; a++) {
         const PlanePose pp = pose[a];
         if (mask[mask_index(famS, pp.fam)] != DEME_DONT_PREVENT_CONTACT) continue;
         if (plane_cd(P, pp, cr.x, cr.y, cr.z, rCD)) {

@@ -68,6 +68,28 @@ __device__ __forceinline__ void cell_of(const CellGrid& G, double X, double Y, d
     cz = (int)fz;
 }
 
+// Slot of this lane in counter[key] (atomic fetch-add of 1), with one atomic per run of equal keys among adjacent
+// active lanes: the spheres of a clump are consecutive and usually share a cell, and same-address atomics of one
+// wave serialize. Lanes of a run get consecutive slots in lane order.
+__device__ __forceinline__ uint32_t run_slot(uint32_t* counter, uint32_t key) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long active = __ballot(1);
+    const unsigned long long upTo = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+    const uint32_t prevKey = __shfl_up(key, 1, 64);
+    const bool prevActive = lane > 0 && ((active >> (lane - 1)) & 1ull);
+    const bool starts = !prevActive || prevKey != key;
+    const unsigned long long starters = __ballot(starts);
+    // run: from the highest starter at or below this lane up to (not including) the next starter or inactive lane
+    const int start = 63 - __clzll(starters & upTo);
+    const unsigned long long fromStart = ~((1ull << start) - 1ull);  // lanes start..63
+    const unsigned long long stops = (starters | ~active) & ~upTo;  // first of these above this lane ends the run
+    const unsigned long long runMask = fromStart & (stops ? ((1ull << (__ffsll(stops) - 1)) - 1ull) : ~0ull);
+    uint32_t base = 0;
+    if (lane == start) base = atomicAdd(&counter[key], (uint32_t)__popcll(runMask & active));
+    base = __shfl(base, start, 64);
+    return base + (uint32_t)__popcll(runMask & active & (upTo >> 1));
+}
+
 // Per sphere: world position, plane contacts (a10), bin counts (a6), cell assignment with its slot.
 __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, const SphereRec* __restrict__ sph,
                                                      const OwnerRec* __restrict__ own, uint32_t nS,
@@ -107,13 +129,15 @@ __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, cons
         if (outside) atomicOr(&ctr->err, ERRBIT_OUTSIDE);
     }
     uint32_t cnt = 0;
-    unsigned long long bits = 0;
-    for (uint32_t a = 0; a < nA; a++) {
+    unsigned long long bits = 0;  // bit k: component a0 + k of the sphere's scene
+    uint32_t a0, a1;
+    scene_planes(P, o.scene, a0, a1);
+    for (uint32_t a = a0; a < a1 && a < nA && a - a0 < 64; a++) {
         const PlanePose pp = pose[a];
         if (mask[mask_index(o.fam, pp.fam)] != DEME_DONT_PREVENT_CONTACT) continue;
         if (plane_cd(P, pp, X, Y, Z, rCD)) {
             cnt++;
-            bits |= 1ull << a;
+            bits |= 1ull << (a - a0);
         }
     }
     numAnal[s] = cnt;
@@ -122,7 +146,7 @@ __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, cons
     cell_of(G, X, Y, Z, cx, cy, cz);
     const uint32_t cell = (uint32_t)cx + (uint32_t)cy * G.nx + (uint32_t)cz * G.nx * G.ny;
     cellOf[s] = cell;
-    slotOf[s] = atomicAdd(&cellCount[cell], 1u);
+    slotOf[s] = run_slot(cellCount, cell);
     CdRec cr;
     cr.x = X;
     cr.y = Y;
@@ -172,8 +196,14 @@ __device__ __forceinline__ bool expanded_overlap(double ra, const CdRec& A, doub
 }
 
 // Second half, for a pair that overlaps: the contact point (DEMHelperKernels.cu:206-222) computed with A the lower id,
-// and its bin inside both spheres' AABB bin ranges (ga: A's range, precomputed by the caller).
-__device__ __forceinline__ bool bin_rule_point(const KParams& P, const CdRec& A, const BinRange& ga, const CdRec& B) {
+// and its bin inside both spheres' AABB bin ranges.
+struct RuleBins {
+    uint32_t ix, iy, iz;
+    BinRange ga, gb;
+};
+
+__device__ __forceinline__ void contact_point(const KParams& P, const CdRec& A, const CdRec& B, double& cx, double& cy,
+                                              double& cz) {
     const double ra = expanded_radius(P, A.r), rb = expanded_radius(P, B.r);
     const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
     const double d2 = dx * dx + dy * dy + dz * dz;
@@ -183,44 +213,64 @@ __device__ __forceinline__ bool bin_rule_point(const KParams& P, const CdRec& A,
     nY /= mag;
     nZ /= mag;
     const double half = (ra + rb - sqrt(d2)) / 2.0;
-    const double cx = B.x + (rb - half) * (double)nX;
-    const double cy = B.y + (rb - half) * (double)nY;
-    const double cz = B.z + (rb - half) * (double)nZ;
-    const uint32_t ix = sat_u32(cx / P.binSize), iy = sat_u32(cy / P.binSize), iz = sat_u32(cz / P.binSize);
-    double rcdB = (double)B.r;
-    rcdB += (double)P.beta;
-    const BinRange gb = bin_range(P, B.x, B.y, B.z, rcdB);
-    if (range_in_grid(P, ga) && range_in_grid(P, gb)) {
+    cx = B.x + (rb - half) * (double)nX;
+    cy = B.y + (rb - half) * (double)nY;
+    cz = B.z + (rb - half) * (double)nZ;
+}
+
+__device__ __forceinline__ double cd_radius(const KParams& P, float r) {
+    double v = (double)r;
+    v += (double)P.beta;
+    return v;
+}
+
+// The reference's bin indices, with its divisions by binSize.
+__device__ __forceinline__ void rule_bins_exact(const KParams& P, const CdRec& A, const CdRec& B, double cx, double cy,
+                                                double cz, RuleBins& R) {
+    R.ix = sat_u32(cx / P.binSize);
+    R.iy = sat_u32(cy / P.binSize);
+    R.iz = sat_u32(cz / P.binSize);
+    R.ga = bin_range(P, A.x, A.y, A.z, cd_radius(P, A.r));
+    R.gb = bin_range(P, B.x, B.y, B.z, cd_radius(P, B.r));
+}
+
+__device__ __forceinline__ bool rule_member(const KParams& P, const RuleBins& R) {
+    if (range_in_grid(P, R.ga) && range_in_grid(P, R.gb)) {
         // no aliasing: a linear id is one (i, j, k) -- unless the point itself lies past the grid, where the
         // reference's linear id aliases; fall through to the exact enumeration then
-        if (ix < P.nbX && iy < P.nbY && iz < P.nbZ)
-            return ix >= ga.x0 && ix <= ga.x1 && iy >= ga.y0 && iy <= ga.y1 && iz >= ga.z0 && iz <= ga.z1 &&
-                   ix >= gb.x0 && ix <= gb.x1 && iy >= gb.y0 && iy <= gb.y1 && iz >= gb.z0 && iz <= gb.z1;
+        if (R.ix < P.nbX && R.iy < P.nbY && R.iz < P.nbZ)
+            return R.ix >= R.ga.x0 && R.ix <= R.ga.x1 && R.iy >= R.ga.y0 && R.iy <= R.ga.y1 && R.iz >= R.ga.z0 &&
+                   R.iz <= R.ga.z1 && R.ix >= R.gb.x0 && R.ix <= R.gb.x1 && R.iy >= R.gb.y0 && R.iy <= R.gb.y1 &&
+                   R.iz >= R.gb.z0 && R.iz <= R.gb.z1;
     }
-    const uint32_t lin = ix + iy * P.nbX + iz * P.nbX * P.nbY;
-    return range_has_linear(P, ga, lin) && range_has_linear(P, gb, lin);
+    const uint32_t lin = R.ix + R.iy * P.nbX + R.iz * P.nbX * P.nbY;
+    return range_has_linear(P, R.ga, lin) && range_has_linear(P, R.gb, lin);
+}
+
+// A: the lower id of an overlapping pair.
+__device__ __forceinline__ bool bin_rule_point(const KParams& P, const CdRec& A, const CdRec& B) {
+    double cx, cy, cz;
+    contact_point(P, A, B, cx, cy, cz);
+    RuleBins R;
+    rule_bins_exact(P, A, B, cx, cy, cz, R);
+    return rule_member(P, R);
 }
 
 // One lane per sphere in cell order, half-shell: each lane pairs its sphere with the later spheres of its own cell
 // and with the spheres of 13 of its 26 neighbour cells (the cell after it in x; the three cells of the next row in y;
-// the nine cells of the next layer in z), so every candidate pair is met exactly once.
+// the nine cells of the next layer in z), so every candidate pair is met exactly once. Cells are numbered x fastest,
+// so each neighbour row of three cells is one contiguous range of the cell-sorted records.
 //
-// Cells are numbered x fastest, so for a block of consecutive sorted spheres (cells cLo..cHi) the candidates of all
-// its lanes for one (dy, dz) neighbour row are the sorted range of cells cLo - 1 + off .. cHi + 1 + off, off =
-// dy * nx + dz * nx * ny: five contiguous segments. The block stages them in LDS as block-relative float positions
-// and sweeps its lanes' windows there.
-//
-// Two passes keep a wave's lanes in step. The sweep runs only a single-precision prefilter, conservative by
-// CellGrid::tol and the largest expanded radius, and parks the survivors (sorted-array indices) in the lane's column
-// of `surv`: entry k of sorted sphere i sits at surv[k * nS + i], so a wave's k-th entries share cache lines. The
-// second pass re-tests each survivor with the reference's double-precision overlap and contact-point bin rule
-// (evaluated with the lower id as A, whichever lane meets the pair) and appends every accepted pair to both spheres'
-// columns: the higher id to the lower-id sphere's contact row (rowSlab, count aCnt), the lower id to the higher-id
-// sphere's mirror row (mirSlab, count bCnt). Both counters are indexed by sorted position, so the appends of a block
-// hit neighbouring addresses. A column that overflows the slab is reported through maxOcc and the pass is redone
-// with wider columns. A block whose segments exceed the LDS budget sweeps global memory with the exact test instead.
+// Two phases keep the block's lanes busy. The sweep runs the reference's double-precision expanded-overlap test and
+// the same-owner check and queues every surviving pair in LDS. The block then shares the queue out evenly: the family
+// mask and the contact-point bin rule (evaluated with the lower id as A, whichever lane met the pair), and each
+// accepted pair is appended to both spheres' columns: the higher id to the lower-id sphere's contact row (rowSlab,
+// count aCnt), the lower id to the higher-id sphere's mirror row (mirSlab, count bCnt). Slabs and counters are indexed
+// by sorted position (entry k of sorted sphere i at slab[k * nS + i]), so the appends of a block hit neighbouring
+// addresses. Pairs beyond the queue's capacity are parked in the lane's column of `surv` and finished by that lane.
+// A column that overflows a slab is reported through maxOcc and the pass is redone with wider columns.
 constexpr uint32_t PAIR_BLOCK = 128;
-constexpr uint32_t PAIR_LDS_RECS = 1024;
+constexpr uint32_t PAIR_QUEUE = 1024;
 constexpr int PAIR_SEGS = 5;
 
 // segment k of the half shell: (dy, dz) = (0, 0), (1, 0), (-1, 1), (0, 1), (1, 1)
@@ -234,6 +284,21 @@ __device__ __forceinline__ void append_pair(uint32_t* __restrict__ cnt, uint32_t
     else atomicMax(&ctr->maxOcc, slot + 1);
 }
 
+// Second phase for one overlapping pair of different owners (sorted positions i, j).
+__device__ __forceinline__ void finish_pair(const KParams& P, uint32_t nS, const CdRec* __restrict__ sorted,
+                                            const uint2* __restrict__ sortedMeta, const uint8_t* __restrict__ mask,
+                                            uint32_t i, uint32_t j, uint32_t slabCap, uint32_t* rowSlab,
+                                            uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt, DevCounters* ctr) {
+    if (mask[mask_index(sortedMeta[i].y, sortedMeta[j].y)] != DEME_DONT_PREVENT_CONTACT) return;
+    const CdRec a = load_cd(sorted, i), b = load_cd(sorted, j);
+    const bool aLow = a.id < b.id;
+    if (!(aLow ? bin_rule_point(P, a, b) : bin_rule_point(P, b, a))) return;
+    const uint32_t loIdx = aLow ? i : j, hiIdx = aLow ? j : i;
+    const uint32_t loId = aLow ? a.id : b.id, hiId = aLow ? b.id : a.id;
+    append_pair(aCnt, rowSlab, loIdx, hiId, nS, slabCap, ctr);
+    append_pair(bCnt, mirSlab, hiIdx, loId, nS, slabCap, ctr);
+}
+
 __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G, uint32_t nS, uint32_t nBlocks,
                                                            const CdRec* __restrict__ sorted,
                                                            const uint2* __restrict__ sortedMeta,
@@ -242,145 +307,67 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
                                                            uint32_t* __restrict__ surv, uint32_t* __restrict__ rowSlab,
                                                            uint32_t* __restrict__ mirSlab, uint32_t* aCnt,
                                                            uint32_t* bCnt, DevCounters* ctr) {
-    __shared__ float4 recs[PAIR_LDS_RECS];  // x, y, z relative to the block origin; id bits
-    __shared__ uint32_t segGlob[PAIR_SEGS], segLen[PAIR_SEGS], segBase[PAIR_SEGS];
-    __shared__ uint32_t cellLo, cellHi;
-    __shared__ double org[3];
+    __shared__ uint32_t qI[PAIR_QUEUE], qJ[PAIR_QUEUE];
+    __shared__ uint32_t qLen;
     // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so XCD k gets a contiguous share of the
     // sphere range and the rows its blocks share stay in its L2
     const uint32_t per = (nBlocks + 7) / 8;
     const uint32_t lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
     // a flagged assignment error (oversized or out-of-world ranges) aborts the pass; both tests are block-uniform
     if (lb >= nBlocks || ctr->err) return;
-    const uint32_t i0 = lb * PAIR_BLOCK;
-    const uint32_t iEnd = min(i0 + PAIR_BLOCK, nS);
-    const uint32_t i = i0 + threadIdx.x;
-    const bool valid = i < iEnd;
-    CdRec me;
-    int cx = 0, cy = 0, cz = 0;
-    uint32_t myCell = 0;
-    if (valid) {
-        me = load_cd(sorted, i);
+    if (threadIdx.x == 0) qLen = 0;
+    __syncthreads();
+    const uint32_t i = lb * PAIR_BLOCK + threadIdx.x;
+    uint32_t nc = 0;  // pairs parked in surv
+    if (i < nS) {
+        const CdRec me = load_cd(sorted, i);
+        const uint32_t myOwner = sortedMeta[i].x;
+        int cx, cy, cz;
         cell_of(G, me.x, me.y, me.z, cx, cy, cz);
-        myCell = (uint32_t)cx + (uint32_t)cy * G.nx + (uint32_t)cz * G.nx * G.ny;
-        // sorted by cell: the block's cells are those of its first and last sphere and everything between
-        if (i == i0) {
-            cellLo = myCell;
-            org[0] = me.x;
-            org[1] = me.y;
-            org[2] = me.z;
-        }
-        if (i == iEnd - 1) cellHi = myCell;
-    }
-    __syncthreads();
-    if (threadIdx.x < PAIR_SEGS) {
-        const int64_t off = (int64_t)seg_dy(threadIdx.x) * G.nx + (int64_t)seg_dz(threadIdx.x) * G.nx * G.ny;
-        const int64_t lo = max((int64_t)cellLo + off - 1, (int64_t)0);
-        const int64_t hi = min((int64_t)cellHi + off + 1, (int64_t)G.nCells - 1);
-        uint32_t gs = 0, ge = 0;
-        if (lo <= hi) {
-            gs = cellStart[lo];
-            ge = cellStart[hi + 1];
-        }
-        segGlob[threadIdx.x] = gs;
-        segLen[threadIdx.x] = ge - gs;
-    }
-    __syncthreads();
-    uint32_t total = 0;
-#pragma unroll
-    for (int k = 0; k < PAIR_SEGS; k++) total += segLen[k];
-    const bool staged = total <= PAIR_LDS_RECS;  // block-uniform
-    if (staged) {
-        const double ox = org[0], oy = org[1], oz = org[2];
-        uint32_t base = 0;
+        const uint32_t nxny = G.nx * G.ny;
+        const uint32_t myCell = (uint32_t)cx + (uint32_t)cy * G.nx + (uint32_t)cz * nxny;
+        const double ra = expanded_radius(P, me.r);
+        const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < (int)G.nx - 1 ? cx + 1 : cx;
+        uint32_t* col = surv + i;
         for (int sg = 0; sg < PAIR_SEGS; sg++) {
-            const uint32_t len = segLen[sg], g = segGlob[sg];
-            if (threadIdx.x == 0) segBase[sg] = base;
-            for (uint32_t k = threadIdx.x; k < len; k += PAIR_BLOCK) {
-                const CdRec r = load_cd(sorted, g + k);
-                recs[base + k] =
-                    make_float4((float)(r.x - ox), (float)(r.y - oy), (float)(r.z - oz), __uint_as_float(r.id));
+            const int y = cy + seg_dy(sg), z = cz + seg_dz(sg);
+            if (y < 0 || y >= (int)G.ny || z >= (int)G.nz) continue;
+            uint32_t b, e;
+            if (sg == 0) {
+                // own cell after this sphere, then the next cell in x
+                b = i + 1;
+                e = cellStart[myCell + (uint32_t)(x1 - cx) + 1];
+            } else {
+                const uint32_t c0 = (uint32_t)x0 + (uint32_t)y * G.nx + (uint32_t)z * nxny;
+                b = cellStart[c0];
+                e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
             }
-            base += len;
-        }
-    }
-    __syncthreads();
-    if (!valid) return;
-
-    const double ra = expanded_radius(P, me.r);
-    uint32_t* col = surv + i;  // survivor k at col[k * nS]
-    uint32_t nc = 0;
-    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < (int)G.nx - 1 ? cx + 1 : cx;
-    const uint32_t nxny = G.nx * G.ny;
-    for (int sg = 0; sg < PAIR_SEGS; sg++) {
-        const int y = cy + seg_dy(sg), z = cz + seg_dz(sg);
-        if (y < 0 || y >= (int)G.ny || z >= (int)G.nz) continue;
-        uint32_t b, e;
-        if (sg == 0) {
-            // own cell after this sphere, then the next cell in x
-            b = i + 1;
-            e = cellStart[myCell + (uint32_t)(x1 - cx) + 1];
-        } else {
-            const uint32_t c0 = (uint32_t)x0 + (uint32_t)y * G.nx + (uint32_t)z * nxny;
-            b = cellStart[c0];
-            e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
-        }
-        if (staged) {
-            const float px = (float)(me.x - org[0]), py = (float)(me.y - org[1]), pz = (float)(me.z - org[2]);
-            float R = (float)ra;
-            R += G.rfMax;
-            R += G.tol;
-            const float thr = R * R;
-            // this row's cells lie inside segment sg, so b..e index into its staged records
-            const uint32_t base = segBase[sg] - segGlob[sg];  // modulo 2^32: base + j = segBase + (j - segGlob)
-            for (uint32_t j = b; j < e; j++) {
-                const float4 q = recs[base + j];
-                const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
-                if (dx * dx + dy * dy + dz * dz > thr) continue;
-                if (nc < slabCap) col[(uint64_t)nc * nS] = j;
-                nc++;
-            }
-        } else {
             for (uint32_t j = b; j < e; j++) {
                 const CdRec ot = load_cd(sorted, j);
                 if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) continue;
-                if (nc < slabCap) col[(uint64_t)nc * nS] = j;
-                nc++;
+                if (sortedMeta[j].x == myOwner) continue;  // same clump: never a contact
+                const uint32_t q = atomicAdd(&qLen, 1u);
+                if (q < PAIR_QUEUE) {
+                    qI[q] = i;
+                    qJ[q] = j;
+                } else {
+                    if (nc < slabCap) col[(uint64_t)nc * nS] = j;
+                    nc++;
+                }
             }
         }
     }
+    __syncthreads();
+    const uint32_t nq = min(qLen, PAIR_QUEUE);
+    for (uint32_t q = threadIdx.x; q < nq; q += PAIR_BLOCK)
+        finish_pair(P, nS, sorted, sortedMeta, mask, qI[q], qJ[q], slabCap, rowSlab, mirSlab, aCnt, bCnt, ctr);
     if (nc > slabCap) {
         atomicMax(&ctr->maxOcc, nc);
         return;
     }
-    if (nc == 0) return;
-    double rcdMe = (double)me.r;
-    rcdMe += (double)P.beta;
-    const BinRange gMe = bin_range(P, me.x, me.y, me.z, rcdMe);
-    const uint2 mine = sortedMeta[i];  // owner, family
-    for (uint32_t k = 0; k < nc; k++) {
-        const uint32_t j = col[(uint64_t)k * nS];
-        const CdRec ot = load_cd(sorted, j);
-        if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) continue;
-        // the rule is evaluated with the lower id as A, whichever lane meets the pair
-        const bool meLow = me.id < ot.id;
-        bool hit;
-        if (meLow) {
-            hit = bin_rule_point(P, me, gMe, ot);
-        } else {
-            double rcdO = (double)ot.r;
-            rcdO += (double)P.beta;
-            hit = bin_rule_point(P, ot, bin_range(P, ot.x, ot.y, ot.z, rcdO), me);
-        }
-        if (!hit) continue;
-        const uint2 theirs = sortedMeta[j];
-        if (theirs.x == mine.x) continue;
-        if (mask[mask_index(mine.y, theirs.y)] != DEME_DONT_PREVENT_CONTACT) continue;
-        const uint32_t loIdx = meLow ? i : j, hiIdx = meLow ? j : i;
-        const uint32_t loId = meLow ? me.id : ot.id, hiId = meLow ? ot.id : me.id;
-        append_pair(aCnt, rowSlab, loIdx, hiId, nS, slabCap, ctr);
-        append_pair(bCnt, mirSlab, hiIdx, loId, nS, slabCap, ctr);
-    }
+    for (uint32_t k = 0; k < nc; k++)
+        finish_pair(P, nS, sorted, sortedMeta, mask, i, surv[(uint64_t)k * nS + i], slabCap, rowSlab, mirSlab, aCnt,
+                    bCnt, ctr);
 }
 
 // Canonical rows: plane contacts (ascending plane index) then partners (ascending id); idA for the row. Mirror rows:
@@ -398,7 +385,9 @@ __device__ __forceinline__ void sorted_copy(const uint32_t* __restrict__ src, ui
     }
 }
 
-__global__ __launch_bounds__(256) void k_cell_rows(uint32_t nS, const uint32_t* __restrict__ rowSlab,
+__global__ __launch_bounds__(256) void k_cell_rows(KParams P, const SphereRec* __restrict__ sph,
+                                                   const OwnerRec* __restrict__ own, uint32_t nS,
+                                                   const uint32_t* __restrict__ rowSlab,
                                                    const uint32_t* __restrict__ mirSlab,
                                                    const uint32_t* __restrict__ posOf,
                                                    const uint32_t* __restrict__ rowStart,
@@ -411,10 +400,12 @@ __global__ __launch_bounds__(256) void k_cell_rows(uint32_t nS, const uint32_t* 
     const uint32_t st = rowStart[s], en = rowStart[s + 1];
     uint32_t pos = st;
     unsigned long long bits = planeBits[s];
+    uint32_t a0 = 0, a1 = 0;
+    if (bits) scene_planes(P, own[sph[s].owner].scene, a0, a1);
     while (bits && pos < en) {
         const int a = __ffsll(bits) - 1;
         bits &= bits - 1;
-        idB[pos] = (uint32_t)a;
+        idB[pos] = a0 + (uint32_t)a;
         ctype[pos] = DEME_SPHERE_PLANE_CONTACT;
         pos++;
     }
@@ -462,12 +453,13 @@ hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, c
     return hipGetLastError();
 }
 
-hipError_t launch_cell_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* mirSlab, const uint32_t* posOf,
+hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                            const uint32_t* rowSlab, const uint32_t* mirSlab, const uint32_t* posOf,
                             const uint32_t* rowStart, const unsigned long long* planeBits, const uint32_t* bRowStart,
                             uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* bPart, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cell_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, rowSlab, mirSlab, posOf, rowStart, planeBits, bRowStart, idA, idB,
-                                                 ctype, bPart);
+    k_cell_rows<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, rowSlab, mirSlab, posOf, rowStart, planeBits,
+                                                 bRowStart, idA, idB, ctype, bPart);
     return hipGetLastError();
 }
 

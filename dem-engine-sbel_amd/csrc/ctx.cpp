@@ -66,6 +66,11 @@ struct deme_ctx {
     uint32_t maxMatSph = 0, maxMatAnal = 0, maxMat = 0;
     uint64_t nBins = 0;
     std::vector<float> hostMass;
+    // scenes of a batch: per owner scene index; analytical components grouped by scene (planeStart)
+    std::vector<uint32_t> hostScene, hostAnalOwner, hostAnalScene;  // analScene empty: the owner's scene
+    uint32_t nScenes = 1;
+    bool scenesDirty = true;
+    DBuf<uint32_t> planeStart;
 
     // static data
     DBuf<SphereRec> sph;
@@ -109,6 +114,7 @@ struct deme_ctx {
     uint64_t nCells = 0;
     float maxRadius = 0.f;
     uint32_t slabCap = 32;
+    double cellScale = 1.0;  // cell edge = cellScale x the largest interaction distance (>= 1)
     DBuf<uint32_t> cellCount, cellStart, cellOf, slotOf, posOf, aCnt, survSlab, rowSlab, mirSlab;
     DBuf<CdRec> sorted;
     DBuf<uint2> meta, sortedMeta;
@@ -213,6 +219,35 @@ struct KTimer {
         if (e_ != hipSuccess) return (ctx)->fail_hip(e_, #expr);   \
     } while (0)
 
+// Analytical components of scene k are planeStart[k] .. planeStart[k + 1] - 1: components must come grouped by the
+// scene of their owner, at most 64 per scene (the per-sphere plane bitset).
+int build_scene_planes(deme_ctx* c) {
+    if (!c->scenesDirty) return DEME_OK;
+    const uint32_t nSc = c->nScenes;
+    std::vector<uint32_t> start(nSc + 1, 0);
+    uint32_t prev = 0;
+    for (uint32_t a = 0; a < c->nA; a++) {
+        const uint32_t o = c->hostAnalOwner[a];
+        const uint32_t sc = a < c->hostAnalScene.size() ? c->hostAnalScene[a] : (o < c->hostScene.size() ? c->hostScene[o] : 0);
+        if (sc >= nSc) return c->fail(DEME_ERR_INVALID_ARGUMENT, "an analytical component's owner has no scene");
+        if (sc < prev) return c->fail(DEME_ERR_INVALID_ARGUMENT, "analytical components must be grouped by scene");
+        prev = sc;
+        start[sc + 1]++;
+    }
+    for (uint32_t k = 0; k < nSc; k++) {
+        if (start[k + 1] > 64) return c->fail(DEME_ERR_UNSUPPORTED, "more than 64 analytical components in one scene");
+        start[k + 1] += start[k];
+    }
+    DEME_HIP(c, c->planeStart.ensure(nSc + 1, 1.0));
+    DEME_HIP(c, hipMemcpyAsync(c->planeStart.p, start.data(), (nSc + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->kp.planeStart = c->planeStart.p;
+    c->kp.nScenes = nSc;
+    c->scenesDirty = false;
+    return DEME_OK;
+}
+
 int check_ready(deme_ctx* c) {
     if (!c->haveParams) return c->fail(DEME_ERR_NOT_READY, "simulation parameters not set (deme_set_params)");
     if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "templates not uploaded (deme_upload_templates)");
@@ -220,7 +255,7 @@ int check_ready(deme_ctx* c) {
     if (!c->haveMaterials) return c->fail(DEME_ERR_NOT_READY, "materials not uploaded (deme_upload_materials)");
     if (!c->haveFamilies) return c->fail(DEME_ERR_NOT_READY, "family data not uploaded (deme_upload_families)");
     if (c->maxMat >= c->nM) return c->fail(DEME_ERR_INVALID_ARGUMENT, "a material index exceeds the material table");
-    return DEME_OK;
+    return build_scene_planes(c);
 }
 
 int readback_counters(deme_ctx* c) {
@@ -338,13 +373,8 @@ int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
     k.integrator = p->integrator;
     k.nS = c->nS;
     k.nA = c->nA;
-    if (nb != c->nBins) {
-        c->nBins = nb;
-        DEME_HIP(c, c->binCount.ensure(nb, 1.0));
-        DEME_HIP(c, c->binStart.ensure(nb + 1, 1.0));
-        DEME_HIP(c, hipMemsetAsync(c->binCount.p, 0, nb * sizeof(uint32_t), c->stream));
-        DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>(nb, 1u << 22)), 1.0));
-    }
+    // the per-bin arrays of the staged path are sized on its first use (deme_cd_bin_count)
+    c->nBins = nb;
     c->haveParams = true;
     return DEME_OK;
 }
@@ -392,7 +422,7 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     DEME_HIP(c, c->bCnt.ensure(nS + 1, 1.0));
     DEME_HIP(c, c->bRowStart.ensure(nS + 1, 1.0));
     for (int k = 0; k < 2; k++) DEME_HIP(c, c->rowStart[k].ensure(nS + 1, 1.0));
-    DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>({c->nBins, (uint64_t)nS, 1u << 22})), 1.0));
+    DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>((uint64_t)nS, 1u << 22)), 1.0));
     if (nS) DEME_HIP(c, hipMemcpyAsync(c->sph.p, hs.data(), nS * sizeof(SphereRec), hipMemcpyHostToDevice, c->stream));
     if (nO) DEME_HIP(c, hipMemcpyAsync(c->moi.p, hm.data(), nO * sizeof(float4), hipMemcpyHostToDevice, c->stream));
     DEME_HIP(c, hipMemcpyAsync(c->ownerSphStart.p, start.data(), (nClumps + 1) * sizeof(uint32_t),
@@ -406,6 +436,9 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     c->nO = nO;
     c->nClumps = nClumps;
     c->nAnalOwners = nO - nClumps;
+    c->hostScene.assign(nO, 0u);
+    c->nScenes = 1;
+    c->scenesDirty = true;
     c->maxMatSph = maxMat;
     c->maxMat = std::max(c->maxMatSph, c->maxMatAnal);
     c->maxRadius = maxR;
@@ -428,7 +461,7 @@ int deme_upload_materials(deme_ctx* c, uint32_t n, const deme_material* m) {
 
 int deme_upload_analytical(deme_ctx* c, uint32_t n, const deme_anal_obj* objs) {
     if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before analytical objects");
-    if (n > 64) return c->fail(DEME_ERR_UNSUPPORTED, "more than 64 analytical components");
+    if (n > 65535) return c->fail(DEME_ERR_UNSUPPORTED, "more than 65535 analytical components");
     uint32_t maxMat = 0;
     for (uint32_t a = 0; a < n; a++) {
         if (objs[a].owner < c->nClumps || objs[a].owner >= c->nO)
@@ -443,6 +476,35 @@ int deme_upload_analytical(deme_ctx* c, uint32_t n, const deme_anal_obj* objs) {
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     c->nA = n;
     c->kp.nA = n;
+    c->hostAnalOwner.resize(n);
+    for (uint32_t a = 0; a < n; a++) c->hostAnalOwner[a] = objs[a].owner;
+    c->hostAnalScene.clear();  // components changed: their scenes follow their owners until set again
+    c->scenesDirty = true;
+    return DEME_OK;
+}
+
+int deme_upload_scenes(deme_ctx* c, uint32_t nScenes, const uint32_t* ownerScene, const uint32_t* analScene) {
+    if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before the scenes");
+    if (nScenes == 0 || !ownerScene) return c->fail(DEME_ERR_INVALID_ARGUMENT, "need at least one scene");
+    for (uint32_t o = 0; o < c->nO; o++)
+        if (ownerScene[o] >= nScenes) return c->fail(DEME_ERR_INVALID_ARGUMENT, "owner scene index out of range");
+    if (analScene)
+        for (uint32_t a = 0; a < c->nA; a++)
+            if (analScene[a] >= nScenes) return c->fail(DEME_ERR_INVALID_ARGUMENT, "component scene index out of range");
+    c->hostScene.assign(ownerScene, ownerScene + c->nO);
+    if (analScene) c->hostAnalScene.assign(analScene, analScene + c->nA);
+    else c->hostAnalScene.clear();
+    c->nScenes = nScenes;
+    c->scenesDirty = true;
+    if (c->haveState && c->nO) {
+        // carry the scene into the resident owner records
+        std::vector<OwnerRec> h(c->nO);
+        DEME_HIP(c, hipMemcpyAsync(h.data(), c->own.p, c->nO * sizeof(OwnerRec), hipMemcpyDeviceToHost, c->stream));
+        DEME_HIP(c, hipStreamSynchronize(c->stream));
+        for (uint32_t o = 0; o < c->nO; o++) h[o].scene = ownerScene[o];
+        DEME_HIP(c, hipMemcpyAsync(c->own.p, h.data(), c->nO * sizeof(OwnerRec), hipMemcpyHostToDevice, c->stream));
+        DEME_HIP(c, hipStreamSynchronize(c->stream));
+    }
     return DEME_OK;
 }
 
@@ -489,7 +551,7 @@ int deme_upload_state(deme_ctx* c, const deme_owner_state* s) {
         if (s->omgZ) r.wz = s->omgZ[o];
         r.mass = c->hostMass[o];
         r.pad0 = 0;
-        r.pad1 = 0.f;
+        r.scene = c->hostScene[o];
     }
     if (n) DEME_HIP(c, hipMemcpyAsync(c->own.p, h.data(), n * sizeof(OwnerRec), hipMemcpyHostToDevice, c->stream));
     // accelerations, when given
@@ -543,6 +605,9 @@ int deme_download_state(deme_ctx* c, deme_owner_state* s) {
 int deme_cd_bin_count(deme_ctx* c) {
     int r = check_ready(c);
     if (r) return r;
+    DEME_HIP(c, c->binCount.ensure(c->nBins, 1.0));
+    DEME_HIP(c, c->binStart.ensure(c->nBins + 1, 1.0));
+    DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>({c->nBins, (uint64_t)c->nS, 1u << 22})), 1.0));
     DEME_HIP(c, hipMemsetAsync(c->ctr.p, 0, sizeof(DevCounters), c->stream));
     DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
     // the occupancy histogram must start at zero whatever ran before (an aborted pass leaves counts behind)
@@ -684,10 +749,14 @@ int deme_cd_sort_remap(deme_ctx* c) {
 static int setup_grid(deme_ctx* c) {
     if (!std::isfinite(c->maxRadius) || !(c->maxRadius > 0.f))
         return c->fail(DEME_ERR_INVALID_ARGUMENT, "sphere radii must be positive and finite");
-    double edge = 2.0 * ((double)c->maxRadius + (double)c->sp.beta) * 1.0001;
-    const double wx = c->sp.voxelSize_exact * std::ldexp(1.0, c->sp.nvXp2);
-    const double wy = c->sp.voxelSize_exact * std::ldexp(1.0, c->sp.nvYp2);
-    const double wz = c->sp.voxelSize_exact * std::ldexp(1.0, c->sp.nvZp2);
+    double edge = 2.0 * ((double)c->maxRadius + (double)c->sp.beta) * 1.0001 * c->cellScale;
+    // the grid spans the user's domain (positions beyond it fall into edge cells), at most the voxel world
+    auto span = [](double voxelWorld, float box) {
+        return (box > 0.f && (double)box < voxelWorld) ? (double)box : voxelWorld;
+    };
+    const double wx = span(c->sp.voxelSize_exact * std::ldexp(1.0, c->sp.nvXp2), c->sp.boxSize[0]);
+    const double wy = span(c->sp.voxelSize_exact * std::ldexp(1.0, c->sp.nvYp2), c->sp.boxSize[1]);
+    const double wz = span(c->sp.voxelSize_exact * std::ldexp(1.0, c->sp.nvZp2), c->sp.boxSize[2]);
     for (;;) {
         const double nx = std::floor(wx / edge) + 1, ny = std::floor(wy / edge) + 1, nz = std::floor(wz / edge) + 1;
         if (nx * ny * nz <= (double)(1u << 28)) {
@@ -700,21 +769,11 @@ static int setup_grid(deme_ctx* c) {
         }
         edge *= 1.25;
     }
-    {
-        // Prefilter slack: block-relative float coordinates carry at most ~2.1e-7 * extent + 1.8e-7 * distance of
-        // rounding error in a pair distance; 4e-6 * (extent + cutoff) leaves a wide margin, also for spheres
-        // sticking out of the voxel world by their clump offsets.
-        float rf = c->maxRadius;
-        rf += c->sp.beta;
-        c->grid.rfMax = rf;
-        const double extent = std::max({wx, wy, wz});
-        c->grid.tol = (float)(4e-6 * (extent + 2.0 * (double)rf));
-    }
     const uint64_t nCells = (uint64_t)c->grid.nx * c->grid.ny * c->grid.nz;
     if (nCells != c->nCells) {
         DEME_HIP(c, c->cellCount.ensure(nCells, 1.0));
         DEME_HIP(c, c->cellStart.ensure(nCells + 1, 1.0));
-        DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>({nCells, c->nBins, c->nS, 1u << 22})), 1.0));
+        DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>({nCells, (uint64_t)c->nS, 1u << 22})), 1.0));
         c->nCells = nCells;
     }
     return DEME_OK;
@@ -809,8 +868,9 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->bTerm.ensure((size_t)6 * nPairs));
     c->termsReady = false;
     DEME_LAUNCH(c, "cell_rows",
-                launch_cell_rows(nS, c->rowSlab.p, c->mirSlab.p, c->posOf.p, c->rowStart[cu].p, c->planeBits.p,
-                                 c->bRowStart.p, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->bPart.p, c->stream));
+                launch_cell_rows(c->kp, c->sph.p, c->own.p, nS, c->rowSlab.p, c->mirSlab.p, c->posOf.p,
+                                 c->rowStart[cu].p, c->planeBits.p, c->bRowStart.p, c->idA.p, c->idB[cu].p,
+                                 c->ctype[cu].p, c->bPart.p, c->stream));
     DEME_LAUNCH(c, "history_map",
                 launch_history_map(nS, c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p,
                                    c->havePrev ? c->rowStart[pv].p : nullptr, c->havePrev ? c->idB[pv].p : nullptr,
@@ -936,6 +996,78 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
         c->time += (double)c->sp.h;
         c->nSteps++;
     }
+    return DEME_OK;
+}
+
+int deme_set_cell_scale(deme_ctx* c, double scale) {
+    if (!(scale >= 1.0) || scale > 8.0) return c->fail(DEME_ERR_INVALID_ARGUMENT, "cell scale must lie in [1, 8]");
+    c->cellScale = scale;
+    c->nCells = 0;  // re-derive the grid at the next detection pass
+    return DEME_OK;
+}
+
+int deme_set_beta(deme_ctx* c, float beta) {
+    if (!(beta >= 0.f) || !std::isfinite(beta)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "beta must be >= 0");
+    c->sp.beta = beta;
+    c->kp.beta = beta;
+    return DEME_OK;
+}
+
+// Trial runs of candidate (contact-update interval, cell scale) pairs on the live system; the drift margin of an
+// interval f is beta = maxVel * h * f * safety (transferSimParams, APIPrivate.cpp:731-733). Each trial starts with a
+// detection pass and runs trialSteps steps (rounded up to a multiple of f). The fastest candidate stays configured.
+int deme_autotune(deme_ctx* c, const int* freqs, int nFreqs, const double* cellScales, int nScales, int trialSteps,
+                  float maxVel, float safety, int* bestFreq, double* msPerStep) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (nFreqs <= 0 || nScales <= 0 || trialSteps <= 0 || !freqs || !cellScales)
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, "need candidates and trial steps");
+    if (!(maxVel >= 0.f) || !(safety > 0.f)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "bad velocity or safety");
+    if (maxVel == 0.f && c->nO) {
+        // measured: the largest owner speed of the current state
+        std::vector<OwnerRec> h(c->nO);
+        DEME_HIP(c, hipMemcpyAsync(h.data(), c->own.p, c->nO * sizeof(OwnerRec), hipMemcpyDeviceToHost, c->stream));
+        DEME_HIP(c, hipStreamSynchronize(c->stream));
+        for (const OwnerRec& o : h) maxVel = std::max(maxVel, std::sqrt(o.vx * o.vx + o.vy * o.vy + o.vz * o.vz));
+    }
+    hipEvent_t e0, e1;
+    DEME_HIP(c, hipEventCreate(&e0));
+    DEME_HIP(c, hipEventCreate(&e1));
+    double best = 1e300;
+    int bf = freqs[0];
+    double bs = cellScales[0];
+    float bb = c->sp.beta;
+    for (int i = 0; i < nFreqs; i++) {
+        for (int j = 0; j < nScales; j++) {
+            const int f = std::max(freqs[i], 1);
+            const float beta = f > 1 ? maxVel * c->sp.h * (float)f * safety : 0.f;
+            if ((r = deme_set_beta(c, beta)) || (r = deme_set_cell_scale(c, cellScales[j]))) break;
+            const int n = ((trialSteps + f - 1) / f) * f;
+            c->contactsReady = false;  // the trial starts with a detection pass
+            DEME_HIP(c, hipEventRecord(e0, c->stream));
+            if ((r = deme_step(c, n, f))) break;
+            DEME_HIP(c, hipEventRecord(e1, c->stream));
+            DEME_HIP(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            DEME_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+            const double per = (double)ms / n;
+            if (msPerStep) msPerStep[i * nScales + j] = per;
+            if (per < best) {
+                best = per;
+                bf = f;
+                bs = cellScales[j];
+                bb = beta;
+            }
+        }
+        if (r) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r) return r;
+    deme_set_beta(c, bb);
+    deme_set_cell_scale(c, bs);
+    c->contactsReady = false;
+    *bestFreq = bf;
     return DEME_OK;
 }
 

@@ -24,7 +24,7 @@ struct __attribute__((aligned(16))) OwnerRec {
     float vx, vy, vz;      // linear velocity
     float wx, wy, wz;      // body-frame angular velocity (omgBar)
     float mass;
-    float pad1;
+    uint32_t scene;  // independent scene of a batch (0 for a single scene)
 };
 static_assert(sizeof(OwnerRec) == 64, "OwnerRec must be 64 bytes");
 
@@ -66,7 +66,21 @@ struct KParams {
     float Gx, Gy, Gz;
     uint32_t integrator;
     uint32_t nS, nA;  // sphere and analytical-component counts (bounds of data-dependent gathers)
+    // Analytical components are grouped by scene: scene k owns components planeStart[k] .. planeStart[k + 1] - 1
+    // (at most 64). A single scene has planeStart = {0, nA}.
+    const uint32_t* planeStart;
+    uint32_t nScenes;
 };
+
+// Range of analytical components a body of the given scene can touch.
+__device__ __forceinline__ void scene_planes(const KParams& P, uint32_t scene, uint32_t& a0, uint32_t& a1) {
+    if (scene < P.nScenes) {
+        a0 = P.planeStart[scene];
+        a1 = P.planeStart[scene + 1];
+    } else {
+        a0 = a1 = 0;
+    }
+}
 
 // --------------------------------------------------------------------------------------------------------------
 // Saturating float->integer conversions (the reference's device conversions saturate; NaN -> 0).

@@ -32,8 +32,6 @@ static_assert(sizeof(PlanePose) == 48, "PlanePose must be 48 bytes");
 struct CellGrid {
     double inv;  // 1 / cell edge
     uint32_t nx, ny, nz, nCells;
-    float rfMax;  // largest expanded radius float(r + beta)
-    float tol;    // slack of the single-precision candidate prefilter (metres), above its rounding error
 };
 
 // Device-side counters read back by the host after a detection pass.
@@ -109,7 +107,8 @@ hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, c
                              const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
                              uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
                              DevCounters* ctr, hipStream_t st);
-hipError_t launch_cell_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* mirSlab, const uint32_t* posOf,
+hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                            const uint32_t* rowSlab, const uint32_t* mirSlab, const uint32_t* posOf,
                             const uint32_t* rowStart, const unsigned long long* planeBits, const uint32_t* bRowStart,
                             uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* bPart, hipStream_t st);
 

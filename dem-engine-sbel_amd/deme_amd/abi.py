@@ -37,6 +37,7 @@ class SimParams(C.Structure):
         ("l", C.c_double), ("voxelSize", C.c_double), ("l_exact", C.c_double), ("voxelSize_exact", C.c_double),
         ("binSize", C.c_double),
         ("LBF", C.c_float * 3), ("G", C.c_float * 3), ("h", C.c_float), ("beta", C.c_float),
+        ("boxSize", C.c_float * 3), ("pad1", C.c_float),
     ]
 
 
@@ -73,12 +74,12 @@ EXPORTED = [
     "deme_ctx_create", "deme_ctx_destroy", "deme_last_error", "deme_device_count", "deme_get_stream", "deme_sync",
     "deme_figure_out_nv", "deme_kernel_literal", "deme_build_sim_params", "deme_encode_positions",
     "deme_decode_positions", "deme_set_params", "deme_upload_templates", "deme_upload_materials",
-    "deme_upload_analytical", "deme_upload_families", "deme_upload_state", "deme_download_state",
+    "deme_upload_analytical", "deme_upload_families", "deme_upload_scenes", "deme_upload_state", "deme_download_state",
     "deme_cd_bin_count", "deme_cd_bin_fill", "deme_cd_pairs", "deme_cd_sort_remap", "deme_cd_run",
     "deme_download_bin_counts", "deme_download_bins", "deme_download_contacts", "deme_force", "deme_reduce_owner",
     "deme_integrate", "deme_step", "deme_download_forces", "deme_download_wildcards", "deme_upload_wildcards",
     "deme_get_counts", "deme_set_time", "deme_enable_kernel_timing", "deme_reset_kernel_timing",
-    "deme_kernel_timing",
+    "deme_kernel_timing", "deme_set_beta", "deme_set_cell_scale", "deme_autotune",
 ]
 
 _lib = None
@@ -115,6 +116,7 @@ def load(path: str = LIB_PATH):
         "deme_upload_materials": ([vp, u32, vp], i32),
         "deme_upload_analytical": ([vp, u32, vp], i32),
         "deme_upload_families": ([vp, vp, vp], i32),
+        "deme_upload_scenes": ([vp, u32, vp, vp], i32),
         "deme_upload_state": ([vp, C.POINTER(OwnerState)], i32),
         "deme_download_state": ([vp, C.POINTER(OwnerState)], i32),
         "deme_cd_bin_count": ([vp], i32), "deme_cd_bin_fill": ([vp], i32), "deme_cd_pairs": ([vp], i32),
@@ -130,6 +132,9 @@ def load(path: str = LIB_PATH):
         "deme_set_time": ([vp, f64], i32),
         "deme_enable_kernel_timing": ([vp, i32], i32), "deme_reset_kernel_timing": ([vp], i32),
         "deme_kernel_timing": ([vp, i32, vp, vp, vp, C.POINTER(i32)], i32),
+        "deme_set_beta": ([vp, f32], i32),
+        "deme_set_cell_scale": ([vp, f64], i32),
+        "deme_autotune": ([vp, vp, i32, vp, i32, i32, f32, f32, C.POINTER(i32), vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -212,6 +217,9 @@ class DeviceSystem:
         self._chk(self.lib.deme_upload_materials(self.ctx, len(sc.materials), ptr(sc.materials)))
         self._chk(self.lib.deme_upload_analytical(self.ctx, len(sc.anal), ptr(sc.anal)))
         self._chk(self.lib.deme_upload_families(self.ctx, ptr(sc.mask), ptr(sc.rules)))
+        if getattr(sc, "owner_scene", None) is not None:
+            self._chk(self.lib.deme_upload_scenes(self.ctx, int(sc.n_scenes), ptr(sc.owner_scene),
+                                                  ptr(getattr(sc, "anal_scene", None))))
         self.set_state(sc.state)
 
     def _chk(self, st):
@@ -304,6 +312,20 @@ class DeviceSystem:
 
     def enable_timing(self, on=True): self._chk(self.lib.deme_enable_kernel_timing(self.ctx, 1 if on else 0))
     def reset_timing(self): self._chk(self.lib.deme_reset_kernel_timing(self.ctx))
+
+    def set_beta(self, beta): self._chk(self.lib.deme_set_beta(self.ctx, float(beta)))
+
+    def set_cell_scale(self, scale): self._chk(self.lib.deme_set_cell_scale(self.ctx, float(scale)))
+
+    def autotune(self, freqs=(1, 2, 5, 10), cell_scales=(1.0, 1.25), trial_steps=20, max_vel=0.0, safety=1.5):
+        """Times each (update interval, cell scale) on the live system; keeps the fastest. Returns (interval, table)."""
+        f = np.asarray(freqs, np.int32)
+        cs = np.asarray(cell_scales, np.float64)
+        ms = np.zeros(len(f) * len(cs), np.float64)
+        best = C.c_int()
+        self._chk(self.lib.deme_autotune(self.ctx, ptr(f), len(f), ptr(cs), len(cs), int(trial_steps),
+                                         float(max_vel), float(safety), C.byref(best), ptr(ms)))
+        return best.value, ms.reshape(len(f), len(cs))
 
     def kernel_timing(self) -> dict:
         mx = 64

@@ -13,7 +13,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .abi import (ANAL_DTYPE, MATERIAL_DTYPE, NUM_FAMILIES, NUM_MASK_ENTRIES, RULE_DTYPE, SimParams,
-                  build_sim_params, empty_state, encode_positions)
+                  build_sim_params, decode_positions, empty_state, encode_positions)
 
 RHO = 2600.0
 
@@ -39,6 +39,9 @@ class Scene:
     update_freq: int = 1
     steps: int = 1
     info: dict = field(default_factory=dict)
+    owner_scene: np.ndarray = None  # batch: scene index of every owner (None: one scene)
+    anal_scene: np.ndarray = None   # batch: scene index of every analytical component (None: its owner's)
+    n_scenes: int = 1
 
 
 def f32(x):
@@ -279,3 +282,192 @@ def spheres_scene(xyz, radius, box, lbf=None, h=1e-5, beta=0.0, g=(0.0, 0.0, -9.
     return _finish(name, params, xyz, np.arange(n), radius, np.zeros((n, 3)), np.zeros(n), mass,
                    np.repeat(moi[:, None], 3, 1), mats, anal, n_ext, 1, 1, vel=to32(vel), omg=to32(omg),
                    quat=to32(quat), families=families, mask=mask, rules=rules, info=dict(box=box))
+
+
+def polydisperse_bed(n=8_000_000, seed=1003, median_r=0.001, sigma=0.2, r_clip=(0.0005, 0.002), n_mat=4,
+                     h=2e-6, plate_speed=0.05, beta=0.0, velocity_sigma=0.0, steps=500):
+    """Lognormal spheres (median 1 mm, sigma 0.2, clipped to [0.5, 2] mm) with four materials assigned at random, on a
+    jittered cubic lattice of pitch 2 x the 90th-percentile radius in an open-top box, under a top plate (its own
+    owner, a prescribed family moving down at plate_speed) one pitch above the bed."""
+    rng = np.random.default_rng(seed)
+    radius = np.clip(median_r * np.exp(sigma * rng.standard_normal(n)), *r_clip).astype(np.float32)
+    pitch = 2.0 * float(np.quantile(radius, 0.9))  # the largest tenth overlap a neighbour or the floor
+    side = max(4, int(np.ceil(0.8 * n ** (1.0 / 3.0))))  # a column about twice as tall as wide
+    layers = int(np.ceil(n / (side * side)))
+    bx = side * pitch + 2 * pitch
+    bz = (layers + 3) * pitch
+    box = (float(f32(bx)), float(f32(bx)), float(f32(bz)))
+    lbf = tuple(float(f32(-f32(b) / f32(2.0))) for b in box)
+    ii, jj, kk = np.meshgrid(np.arange(side), np.arange(side), np.arange(layers), indexing="ij")
+    order = np.lexsort((ii.ravel(), jj.ravel(), kk.ravel()))
+    i, j, k = ii.ravel()[order][:n], jj.ravel()[order][:n], kk.ravel()[order][:n]
+    xyz = np.stack([lbf[0] + 1.5 * pitch + i * pitch, lbf[1] + 1.5 * pitch + j * pitch,
+                    lbf[2] + 0.5 * pitch + k * pitch], 1)
+    xyz += rng.uniform(-0.05 * pitch, 0.05 * pitch, size=xyz.shape)
+    mat = rng.integers(0, n_mat, n).astype(np.uint8)
+    mats = np.zeros(n_mat, MATERIAL_DTYPE)
+    for m in range(n_mat):
+        mats[m] = (1e7 * (1 + m), 0.25 + 0.02 * m, 0.4 + 0.1 * m, 0.3 + 0.1 * m, 0.0)
+    mass = (RHO * 4.0 / 3.0 * np.pi * radius.astype(np.float64) ** 3).astype(np.float32)
+    moi = (2.0 / 5.0 * mass.astype(np.float64) * radius.astype(np.float64) ** 2).astype(np.float32)
+    params = build_sim_params(box, lbf, float(radius.min()), h, beta=beta)
+    walls = box_planes(lbf, box, owner=n, material=0, top=False)
+    plate = np.zeros(1, ANAL_DTYPE)
+    plate[0]["owner"] = n + 1
+    plate[0]["type"] = 0
+    plate[0]["material"] = n_mat - 1
+    plate[0]["relPos"] = np.array([0.0, 0.0, 0.0], np.float32)
+    plate[0]["rot"] = np.array([0.0, 0.0, -1.0], np.float32)
+    plate[0]["mass"] = f32(1e15)
+    anal = np.concatenate([walls, plate])
+    rule = np.zeros(1, RULE_DTYPE)[0]
+    rule["used"] = 1
+    rule["linVelPrescribed"] = rule["rotVelPrescribed"] = 1
+    rule["velSet"] = 0x3F
+    rule["vel"][:] = (0.0, 0.0, -plate_speed, 0.0, 0.0, 0.0)
+    mask, rules = default_families({10: rule})
+    vel, omg, q = random_motion(rng, n, velocity_sigma)
+    sc = _finish("polydisperse_bed", params, xyz, np.arange(n), radius, np.zeros((n, 3)), mat, mass,
+                 np.repeat(moi[:, None], 3, 1), mats, anal, 2, 1, steps, vel=vel, mask=mask, rules=rules,
+                 info=dict(box=box, pitch=pitch))
+    # the plate owner: centred above the bed, in the prescribed family
+    top = float(xyz[:, 2].max()) + float(radius.max()) + pitch
+    plate_xyz = np.array([[lbf[0] + box[0] / 2, lbf[1] + box[1] / 2, top]], np.float32)
+    v, lx, ly, lz = encode_positions(params, plate_xyz)
+    st = sc.state
+    st["voxelID"][n + 1], st["locX"][n + 1], st["locY"][n + 1], st["locZ"][n + 1] = v[0], lx[0], ly[0], lz[0]
+    st["family"][n + 1] = 10
+    st["vZ"][n + 1] = -plate_speed
+    return sc
+
+
+def voxel_split(box):
+    """Bits per axis (summing to 64) and sub-voxel unit l covering `box` with the smallest l."""
+    best = None
+    for a in range(8, 41):
+        for b in range(8, 41):
+            c = 64 - a - b
+            if c < 8 or c > 40:
+                continue
+            l = max(box[0] / 2.0 ** (a + 16), box[1] / 2.0 ** (b + 16), box[2] / 2.0 ** (c + 16))
+            if best is None or l < best[1]:
+                best = ((a, b, c), l)
+    nv, l = best
+    return nv, float(np.nextafter(f32(l), f32(np.inf)))
+
+
+def batched_columns(n_scenes=256, seed=1004, s_range=(2_000, 200_000), r_range=(0.001, 0.003), height=0.3,
+                    h=5e-6, steps=1000, update_freq=1, beta=0.0, total_scale=1.0, pitch_scale=1.05):
+    """Independent settling columns batched into one system. Scene k has S_k spheres (log-uniform in s_range, times
+    total_scale), radii uniform in r_range, its own material (E log-uniform in [1e6, 1e8], mu in [0.2, 0.8], CoR in
+    [0.3, 0.9], nu 0.3) and its own open-top box of five planes. All columns are about `height` tall; their bases grow
+    with S_k, and they sit on shelves in the xy plane, two lattice pitches apart, so no pair of spheres from different
+    scenes can ever be in range. The walls of every scene belong to one fixed owner; each plane is tagged with its
+    scene."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(s_range[0]), np.log(s_range[1])
+    counts = np.maximum(1, np.round(np.exp(rng.uniform(lo, hi, n_scenes)) * total_scale)).astype(np.int64)
+    pitch = 2.0 * r_range[1] * pitch_scale
+    layers = max(1, int(height / pitch))
+    sides = np.ceil(np.sqrt(np.ceil(counts / layers))).astype(np.int64)
+    widths = sides * pitch + pitch  # half a pitch of wall clearance on each side
+    gap = 2.0 * pitch
+    # shelf packing: widest first, rows of bounded length
+    order = np.argsort(-widths, kind="stable")
+    row_len = float(np.sqrt((widths + gap) ** 2 @ np.ones(n_scenes))) * 1.05
+    origin = np.zeros((n_scenes, 2))
+    x = y = row_h = 0.0
+    for k in order:
+        if x > 0 and x + widths[k] > row_len:
+            x = 0.0
+            y += row_h + gap
+            row_h = 0.0
+        origin[k] = (x, y)
+        x += widths[k] + gap
+        row_h = max(row_h, widths[k])
+    world = (float(origin[:, 0].max() + widths.max() + gap), float(y + row_h + gap), float(height + 4 * pitch))
+    box = tuple(float(f32(b)) for b in world)
+    lbf = (0.0, 0.0, 0.0)
+    xyz, rad, mat, owner_scene = [], [], [], []
+    anal = []
+    anal_scene = []
+    mats = np.zeros(n_scenes, MATERIAL_DTYPE)
+    n_total = int(counts.sum())
+    for k in range(n_scenes):
+        E = float(np.exp(rng.uniform(np.log(1e6), np.log(1e8))))
+        mats[k] = (E, 0.3, rng.uniform(0.3, 0.9), rng.uniform(0.2, 0.8), 0.0)
+        n = int(counts[k])
+        side = int(sides[k])
+        i = np.arange(n) % side
+        j = (np.arange(n) // side) % side
+        l = np.arange(n) // (side * side)
+        p = np.stack([origin[k, 0] + pitch * (i + 1.0), origin[k, 1] + pitch * (j + 1.0), pitch * (l + 0.55)], 1)
+        p += rng.uniform(-0.05 * pitch, 0.05 * pitch, size=p.shape)
+        xyz.append(p)
+        rad.append(rng.uniform(r_range[0], r_range[1], n))
+        mat.append(np.full(n, k))
+        owner_scene.append(np.full(n, k))
+        w = float(widths[k])
+        walls = box_planes(lbf, (w, w, world[2]), owner=n_total, material=k, top=False)
+        walls["relPos"][:, 0] += np.float32(origin[k, 0])
+        walls["relPos"][:, 1] += np.float32(origin[k, 1])
+        anal.append(walls)
+        anal_scene.append(np.full(len(walls), k))
+    xyz = np.concatenate(xyz)
+    radius = np.concatenate(rad).astype(np.float32)
+    mass = (RHO * 4.0 / 3.0 * np.pi * radius.astype(np.float64) ** 3).astype(np.float32)
+    moi = (2.0 / 5.0 * mass.astype(np.float64) * radius.astype(np.float64) ** 2).astype(np.float32)
+    # explicit voxel split (InstructBoxDomainNumVoxel): a flat world gets more bits along x and y
+    nv, l = voxel_split(box)
+    params = build_sim_params(box, lbf, float(radius.min()), h, beta=beta, nv=nv, l=l)
+    sc = _finish("batched_columns", params, xyz, np.arange(n_total), radius, np.zeros((n_total, 3)),
+                 np.concatenate(mat).astype(np.uint8), mass, np.repeat(moi[:, None], 3, 1), mats,
+                 np.concatenate(anal), 1, update_freq, steps, info=dict(counts=counts, box=box, pitch=pitch))
+    sc.owner_scene = np.concatenate(owner_scene + [np.zeros(1, np.int64)]).astype(np.uint32)
+    sc.anal_scene = np.concatenate(anal_scene).astype(np.uint32)
+    sc.n_scenes = n_scenes
+    return sc
+
+
+def split_batch(batch, own_world=False):
+    """The scenes of a batch as separate systems. By default each keeps the batch's world (same voxel grid, materials
+    and planes), so a scene run alone follows exactly the trajectory it follows inside the batch; own_world=True gives
+    each scene a world of its own column's size instead (as a scene set up on its own would have)."""
+    out = []
+    n_sph = batch.n_spheres
+    if own_world:
+        world_pos = decode_positions(batch.params, batch.state["voxelID"], batch.state["locX"], batch.state["locY"],
+                                     batch.state["locZ"]).astype(np.float64)
+    for k in range(batch.n_scenes):
+        sph = np.nonzero(batch.owner_scene[:n_sph] == k)[0]
+        n = len(sph)
+        walls = batch.anal[batch.anal_scene == k].copy()
+        walls["owner"] = n
+        owners = np.concatenate([batch.sph_owner[sph], [batch.n_owners - 1]])  # single-sphere owners, then the walls
+        st = {key: v[owners].copy() for key, v in batch.state.items()}
+        params = batch.params
+        if own_world:
+            # the column's own world: its walls' extent, origin at the column's corner
+            x0 = float(walls["relPos"][:, 0].min())
+            y0 = float(walls["relPos"][:, 1].min())
+            w = float(walls["relPos"][:, 0].max()) - x0
+            box = (float(f32(w)), float(f32(w)), float(batch.info["box"][2]))
+            nv, l = voxel_split(box)
+            params = build_sim_params(box, (0.0, 0.0, 0.0), float(batch.sph_radius[sph].min()), batch.params.h,
+                                      beta=batch.params.beta, nv=nv, l=l)
+            xyz = world_pos[owners].copy()
+            xyz[:, 0] -= x0
+            xyz[:, 1] -= y0
+            xyz[-1] = 0.0
+            v, lx, ly, lz = encode_positions(params, xyz.astype(np.float32))
+            st.update(voxelID=v, locX=lx, locY=ly, locZ=lz)
+            walls["relPos"][:, 0] -= np.float32(x0)
+            walls["relPos"][:, 1] -= np.float32(y0)
+        sc = Scene(name=f"{batch.name}[{k}]", params=params, n_spheres=n, n_owners=n + 1, n_clumps=n,
+                   sph_owner=np.arange(n, dtype=np.uint32), sph_radius=batch.sph_radius[sph].copy(),
+                   sph_relpos=batch.sph_relpos[sph].copy(), sph_mat=batch.sph_mat[sph].copy(),
+                   owner_mass=batch.owner_mass[owners].copy(), owner_moi=batch.owner_moi[owners].copy(),
+                   materials=batch.materials, anal=walls, mask=batch.mask, rules=batch.rules, state=st,
+                   update_freq=batch.update_freq, steps=batch.steps, info=dict(scene=k))
+        out.append(sc)
+    return out

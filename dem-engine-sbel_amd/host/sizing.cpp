@@ -150,6 +150,9 @@ int deme_build_sim_params(float bx, float by, float bz, int exact_dir, const uin
     }
     p.h = h;
     p.beta = beta;
+    p.boxSize[0] = bx;
+    p.boxSize[1] = by;
+    p.boxSize[2] = bz;
     *out = p;
     return DEME_OK;
 }

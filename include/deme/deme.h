@@ -72,6 +72,11 @@ int deme_upload_analytical(deme_ctx* ctx, uint32_t n, const deme_anal_obj* objs)
  * only the mask after the first upload (DisableContactBetweenFamilies after initialisation, APIPublic.cpp:448-466). */
 int deme_upload_families(deme_ctx* ctx, const uint8_t* mask, const deme_family_rule* rules);
 
+/* Batches of independent scenes in one context (no counterpart in the reference, which runs one scene per solver):
+ * ownerScene[o] in [0, nScenes) for every owner; analScene[a] for every analytical component, or NULL to use its
+ * owner's scene. Spheres only meet analytical components of their owner's scene; components must come grouped by
+ * scene, at most 64 per scene. Without this call everything is scene 0. Call after deme_upload_analytical. */
+int deme_upload_scenes(deme_ctx* ctx, uint32_t nScenes, const uint32_t* ownerScene, const uint32_t* analScene);
 /* State I/O (dT getters/setters, src/DEM/dT.cpp:1722-1806; initial encoding dT.cpp:572-575). */
 int deme_upload_state(deme_ctx* ctx, const deme_owner_state* s);
 int deme_download_state(deme_ctx* ctx, deme_owner_state* s);
@@ -109,7 +114,21 @@ int deme_integrate(deme_ctx* ctx);
  * detection on the state at steps k*f (f = max(update_freq, 1)), force -> collect -> integrate each step. */
 int deme_step(deme_ctx* ctx, int n_steps, int update_freq);
 
-/* Per-contact outputs (contactForces, contactTorque_convToForce, contactPointGeometryA/B): 3 floats each. */
+/* Drift margin beta (SetExpandFactor, API.h:100-104): detection uses r + beta. Takes effect at the next detection. */
+int deme_set_beta(deme_ctx* ctx, float beta);
+/* Cell edge of the cell-list detection as a multiple (>= 1) of the largest interaction distance 2 (r_max + beta)
+ * (no counterpart in the reference, whose bin size plays this role for its per-bin pass). */
+int deme_set_cell_scale(deme_ctx* ctx, double scale);
+/* Autotuner (no counterpart in the reference): times trialSteps steps of the live system for every pair of
+ * contact-update interval freqs[i] (drift margin maxVel * h * f * safety, as transferSimParams sets it,
+ * APIPrivate.cpp:731-733; maxVel = 0 measures the largest owner speed) and cell scale cellScales[j], keeps the
+ * fastest configured and returns its interval. msPerStep (optional, nFreqs x nScales) receives the timings. The trials
+ * advance the simulation. */
+int deme_autotune(deme_ctx* ctx, const int* freqs, int nFreqs, const double* cellScales, int nScales, int trialSteps,
+                  float maxVel, float safety, int* bestFreq, double* msPerStep);
+
+/* Per-contact outputs (contactForces, contactTorque_convToForce, contactPointGeometryA/B): 3 floats each. Kept by
+ * deme_force; deme_step stores per-owner terms instead, and a download after it returns DEME_ERR_NOT_READY. */
 int deme_download_forces(deme_ctx* ctx, float* force, float* torque_only_force, float* cpA, float* cpB);
 /* Contact wildcards delta_tan_x, delta_tan_y, delta_tan_z, delta_time (Models.h:237-252), 4 x C floats. */
 int deme_download_wildcards(deme_ctx* ctx, float* w);

@@ -87,6 +87,8 @@ typedef struct deme_sim_params {
     float G[3];
     float h;
     float beta;
+    float boxSize[3]; /* the user's domain (InstructBoxDomainDimension); 0 = the whole voxel world */
+    float pad1;
 } deme_sim_params;
 
 /* Material properties used by the frictional Hertz-Mindlin model (FullHertzianForceModel.cu:5-19). */

@@ -140,6 +140,9 @@ struct ora_ctx {
     std::vector<uint16_t> locX, locY, locZ;
     std::vector<float> qw, qx, qy, qz, vX, vY, vZ, wX, wY, wZ, aX, aY, aZ, alX, alY, alZ;
     std::vector<uint8_t> fam;
+    std::vector<uint32_t> ownerScene;  // batch scene per owner (all 0 for one scene)
+    std::vector<uint32_t> analScene;   // batch scene per analytical component (empty: its owner's)
+    uint32_t nScenes = 1;
     // CD intermediates
     std::vector<uint16_t> numBins;
     std::vector<uint8_t> numAnal;
@@ -259,6 +262,13 @@ inline bool sphere_anal_cd(const ora_ctx* c, uint32_t a, double X, double Y, dou
 
 inline bool mask_allows(const ora_ctx* c, uint32_t famA, uint32_t famB) {
     return c->mask[locate_mask_pair(famA, famB)] == DEME_DONT_PREVENT_CONTACT;
+}
+
+// A batch keeps scenes apart: a sphere meets only the analytical components of its own scene.
+inline bool same_scene(const ora_ctx* c, uint32_t sphereOwner, uint32_t a) {
+    if (c->ownerScene.empty()) return true;
+    const uint32_t sa = c->analScene.empty() ? c->ownerScene[c->anal[a].owner] : c->analScene[a];
+    return c->ownerScene[sphereOwner] == sa;
 }
 
 // checkSpheresOverlap<double> (DEMHelperKernels.cu:195-222) -- CD flavour (float normal, half depth)
@@ -463,6 +473,7 @@ int ora_cd_bins(ora_ctx* c) {
         const uint32_t famS = c->fam[c->sphOwner[s]];
         uint32_t cnt = 0;
         for (uint32_t a = 0; a < c->nA; a++) {
+            if (!same_scene(c, c->sphOwner[s], a)) continue;
             if (!mask_allows(c, famS, c->fam[c->anal[a].owner])) continue;
             if (sphere_anal_cd(c, a, X, Y, Z, rCD)) cnt++;
         }
@@ -685,6 +696,7 @@ int ora_cd_pairs(ora_ctx* c) {
         rCD += (double)c->p.beta;
         const uint32_t famS = c->fam[c->sphOwner[s]];
         for (uint32_t a = 0; a < c->nA; a++) {
+            if (!same_scene(c, c->sphOwner[s], a)) continue;
             if (!mask_allows(c, famS, c->fam[c->anal[a].owner])) continue;
             if (sphere_anal_cd(c, a, X, Y, Z, rCD)) {
                 A.push_back(s);
@@ -1377,3 +1389,10 @@ void ora_get_time(const ora_ctx* c, double* t, uint64_t* n) {
 void ora_set_time(ora_ctx* c, double t) { c->time = t; }
 
 }  // extern "C"
+
+void ora_set_scenes(ora_ctx* c, uint32_t nScenes, const uint32_t* ownerScene, const uint32_t* analScene) {
+    c->nScenes = nScenes;
+    c->ownerScene.assign(ownerScene, ownerScene + c->nO);
+    if (analScene) c->analScene.assign(analScene, analScene + c->nA);
+    else c->analScene.clear();
+}

@@ -36,6 +36,8 @@ ora_ctx* ora_create(const deme_sim_params* p, uint32_t nSpheres, const uint32_t*
                     const deme_family_rule* rules /* 256 */);
 void ora_destroy(ora_ctx* c);
 void ora_set_threads(int n);
+/* Batch scenes: a sphere meets only the analytical components of its scene (analScene NULL: the owner's scene). */
+void ora_set_scenes(ora_ctx* c, uint32_t nScenes, const uint32_t* ownerScene, const uint32_t* analScene);
 void ora_set_state(ora_ctx* c, const deme_owner_state* s);
 void ora_get_state(const ora_ctx* c, deme_owner_state* s);
 void ora_get_counts(const ora_ctx* c, deme_counts* out);

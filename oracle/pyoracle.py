@@ -32,6 +32,7 @@ def load():
     lib.ora_destroy.argtypes = [vp]
     lib.ora_set_threads.argtypes = [i32]
     lib.ora_set_state.argtypes = [vp, C.POINTER(OwnerState)]
+    lib.ora_set_scenes.argtypes = [vp, u32, vp, vp]
     lib.ora_get_state.argtypes = [vp, C.POINTER(OwnerState)]
     lib.ora_get_counts.argtypes = [vp, C.POINTER(Counts)]
     for n in ("ora_cd_bins", "ora_cd_pairs", "ora_cd", "ora_force", "ora_collect", "ora_integrate"):
@@ -81,6 +82,9 @@ class OracleSystem:
                                      ptr(sc.sph_relpos), ptr(sc.sph_mat), sc.n_owners, ptr(sc.owner_mass),
                                      ptr(sc.owner_moi), len(sc.anal), ptr(sc.anal), len(sc.materials),
                                      ptr(sc.materials), ptr(sc.mask), ptr(sc.rules))
+        if getattr(sc, "owner_scene", None) is not None:
+            self._keep += [sc.owner_scene, getattr(sc, "anal_scene", None)]
+            self.lib.ora_set_scenes(self.h, int(sc.n_scenes), ptr(sc.owner_scene), ptr(getattr(sc, "anal_scene", None)))
         self.set_state(sc.state)
 
     def close(self):

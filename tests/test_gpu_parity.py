@@ -38,7 +38,12 @@ def crowded_cells():
                                   n_big=3, r_big=0.01)
 
 
-SCENES = [packed_spheres, packed_clumps, loose_box, expanded_clumps, crowded_cells]
+def polydisperse():
+    # lognormal radii, four materials, a top plate in a prescribed family moving down
+    return scenes.polydisperse_bed(n=3000, velocity_sigma=0.05)
+
+
+SCENES = [packed_spheres, packed_clumps, loose_box, expanded_clumps, crowded_cells, polydisperse]
 
 
 def systems(make, pre_steps=0):
@@ -188,7 +193,7 @@ def test_integration_matches_oracle(make):
         assert np.abs(sd[k].astype(np.float64) - so[k]).max() <= 1e-5 * scale, k
 
 
-@pytest.mark.parametrize("make,freq", [(packed_spheres, 1), (loose_box, 1), (expanded_clumps, 5)])
+@pytest.mark.parametrize("make,freq", [(packed_spheres, 1), (loose_box, 1), (expanded_clumps, 5), (polydisperse, 1)])
 def test_trajectory_parity(make, freq):
     import pyoracle
 
@@ -233,3 +238,17 @@ def test_bin_overflow_is_reported_not_trapped():
     got = set(zip(a[t == 1].tolist(), b[t == 1].tolist()))
     assert got == set(zip(ba.tolist(), bb.tolist()))
     assert len(got) > 0
+
+
+def test_prescribed_plate_moves_at_its_velocity():
+    from deme_amd.abi import decode_positions
+
+    sc = polydisperse()
+    dev = DeviceSystem(sc)
+    p0 = decode_positions(sc.params, *(dev.get_state()[k] for k in ("voxelID", "locX", "locY", "locZ")))[-1]
+    n = 50
+    dev.step(n, 1)
+    p1 = decode_positions(sc.params, *(dev.get_state()[k] for k in ("voxelID", "locX", "locY", "locZ")))[-1]
+    dz = float(p1[2]) - float(p0[2])
+    assert abs(dz - (-0.05 * n * float(np.float32(sc.params.h)))) < 1e-7
+    assert abs(float(p1[0]) - float(p0[0])) < 1e-9
