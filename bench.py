@@ -145,6 +145,60 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta):
             "speedup_vs_loop": t_loop / t_batch}
 
 
+def run_adjoint(n, steps, window, warmup_windows=1):
+    """Differentiation workload: spheres dropped with seeded velocities onto a plane. Forward = the plain step loop
+    with the contact list rebuilt every `window` steps; adjoint = deme_adjoint_forward (the same steps with a
+    checkpoint per window) and deme_adjoint_backward (per window: recompute with per-step snapshots, then the reverse
+    sweep), for the MSE of final positions against a target made with perturbed materials."""
+    from deme_amd import scenes
+    from deme_amd.abi import DeviceSystem, positions_f64
+
+    sc = scenes.adjoint_drop(n=n, steps=steps, window=window)
+    dev = DeviceSystem(sc)
+    # target state: the same simulator with E x 1.1, CoR x 0.9, mu x 1.1
+    pert = sc.materials.copy()
+    pert["E"] *= 1.1
+    pert["CoR"] *= 0.9
+    pert["mu"] *= 1.1
+    dev.upload_materials(pert)
+    dev.step(steps, window)  # from the initial state of a fresh context
+    s = dev.get_state()
+    target = positions_f64(sc.params, s["voxelID"], s["locX"], s["locY"], s["locZ"])[:n]
+    dev.upload_materials(sc.materials)
+    # warm-up (buffers sized), then the timed legs, each from the initial state
+    dev.set_state(sc.state)
+    dev.clear_contacts()
+    dev.adjoint_forward(warmup_windows * window, window)
+    dev.adjoint_backward(np.zeros((n, 3)))
+    dev.set_state(sc.state)
+    dev.clear_contacts()
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.step(steps, window)
+    dev.sync()
+    t_fwd = time.perf_counter() - t0
+    contacts = dev.counts().nContacts
+    dev.set_state(sc.state)
+    dev.clear_contacts()
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.adjoint_forward(steps, window)
+    dev.sync()
+    t_afwd = time.perf_counter() - t0
+    s = dev.get_state()
+    X = positions_f64(sc.params, s["voxelID"], s["locX"], s["locY"], s["locZ"])[:n]
+    loss = float(((X - target) ** 2).sum(1).mean())
+    t0 = time.perf_counter()
+    gV, gM = dev.adjoint_backward(2.0 * (X - target) / n)
+    t_bwd = time.perf_counter() - t0
+    dev.close()
+    return {"spheres": n, "materials": len(sc.materials), "steps": steps, "window": window, "contacts": contacts,
+            "forward_ms_per_step": 1e3 * t_fwd / steps, "checkpointed_forward_ms_per_step": 1e3 * t_afwd / steps,
+            "backward_ms_per_step": 1e3 * t_bwd / steps, "adjoint_over_forward": t_bwd / t_fwd,
+            "gradient_eval_over_forward": (t_afwd + t_bwd) / t_fwd, "loss": loss,
+            "grad_E": gM[:, 0].tolist(), "grad_v0_norm": float(np.linalg.norm(gV))}
+
+
 def cpu_baseline(sc, steps, update_freq, threads):
     import pyoracle  # noqa: E402  (the oracle is the reported CPU baseline only)
 
@@ -169,6 +223,10 @@ def main():
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--adjoint-spheres", type=int, default=50_000)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-window", type=int, default=20)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the workload, copy a device buffer of this size once (a known-size reference "
                          "kernel for scaling hardware byte counters)")
@@ -206,6 +264,10 @@ def main():
     batch = None
     if not args.skip_batch:
         batch = run_batch(args.batch_scenes, args.batch_steps, 3, 1, 0.0)
+
+    adjoint = None
+    if not args.skip_adjoint:
+        adjoint = run_adjoint(args.adjoint_spheres, args.adjoint_steps, args.adjoint_window)
 
     # dominant kernel of the headline run
     dom = max(timingA.items(), key=lambda kv: kv[1][0])
@@ -253,7 +315,8 @@ def main():
                                "contact detection every step",
                    "clumps": args.clumps, "spheres": scA.n_spheres, "contacts": infoA["C"],
                    "bin_sphere_touches": infoA["P"], "active_bins": infoA["Ba"],
-                   "run_every5_beta0.2r": runB, "batched_columns": batch, "kernels": kernels},
+                   "run_every5_beta0.2r": runB, "batched_columns": batch, "adjoint_drop": adjoint,
+                   "kernels": kernels},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": traffic,
                      "bytes_per_launch": bytes_per, "avg_launch_us": 1e3 * avg_ms},

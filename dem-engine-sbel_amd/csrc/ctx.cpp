@@ -107,6 +107,28 @@ struct deme_ctx {
     DBuf<ForceRec> frc;           // full contact records, kept by deme_force only
     DBuf<float> aTerm, bTerm, gTerm;  // owner terms: A side per contact, B side per mirror slot / per plane contact
     bool forcesFull = false;
+    // adjoint: checkpoints at window starts (+ the final state) and the recorded steps of one window
+    struct AdjCk {
+        DBuf<OwnerRec> own;
+        DBuf<uint32_t> rowStart, idB;
+        DBuf<uint8_t> ctype;
+        DBuf<float4> hist;
+        DBuf<uint32_t> idA, mirror, bRowStart;  // the list's derived rows, kept for the final checkpoint only
+        DBuf<float> acc;                          // accelerations of the last step, final checkpoint only
+        uint64_t C = 0, nPairs = 0;
+        bool haveList = false, full = false;
+        double time = 0.0;
+        uint64_t nSteps = 0;
+    };
+    std::vector<AdjCk> adjCk;
+    std::vector<DBuf<OwnerRec>> adjOwn;
+    std::vector<DBuf<float4>> adjHist;
+    std::vector<DBuf<float>> adjAcc;
+    int adjWindow = 0, adjStepsTotal = 0;
+    bool adjReady = false, singleSpheres = false;
+    std::vector<deme_material> hostMats;
+    DBuf<float> lam, lAcc, adjA, adjB, matPart, matGrad;
+    DBuf<float4> lDt[2], lDtPrev;
     bool termsReady = false;  // owner terms belong to the current contact list
     DBuf<float> analPartials, analAcc;
     // cell-list detection
@@ -329,6 +351,15 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->cellCount.release(); c->cellStart.release(); c->cellOf.release(); c->slotOf.release(); c->posOf.release();
     c->survSlab.release(); c->rowSlab.release(); c->mirSlab.release(); c->sorted.release(); c->planeBits.release();
     c->aCnt.release(); c->meta.release(); c->sortedMeta.release();
+    for (auto& k : c->adjCk) {
+        k.own.release(); k.rowStart.release(); k.idB.release(); k.ctype.release(); k.hist.release();
+        k.idA.release(); k.mirror.release(); k.bRowStart.release(); k.acc.release();
+    }
+    for (auto& b : c->adjOwn) b.release();
+    for (auto& b : c->adjHist) b.release();
+    for (auto& b : c->adjAcc) b.release();
+    c->lam.release(); c->lAcc.release(); c->adjA.release(); c->adjB.release(); c->matPart.release();
+    c->matGrad.release(); c->lDt[0].release(); c->lDt[1].release(); c->lDtPrev.release();
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -348,6 +379,7 @@ int deme_sync(deme_ctx* c) {
 }
 
 int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
+    c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (p->nvXp2 + p->nvYp2 + p->nvZp2 != 64) return c->fail(DEME_ERR_INVALID_ARGUMENT, "nvXp2 + nvYp2 + nvZp2 != 64");
     if (!(p->binSize > 0.0)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "bin size must be positive");
     if (!(p->l > 0.0) || !(p->voxelSize > 0.0)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "l / voxelSize must be > 0");
@@ -382,6 +414,7 @@ int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
 int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, const float* sphRadius,
                           const float* sphRelPos, const uint8_t* sphMat, uint32_t nO, uint32_t nClumps,
                           const float* ownerMass, const float* ownerMoi) {
+    c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (nClumps > nO) return c->fail(DEME_ERR_INVALID_ARGUMENT, "nClumps > nOwners");
     if (nO - nClumps > MAX_ANAL_OWNERS)
         return c->fail(DEME_ERR_UNSUPPORTED, "more than 8 external-object owners are not supported");
@@ -410,6 +443,11 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     std::vector<float4> hm(nO);
     for (uint32_t o = 0; o < nO; o++) hm[o] = make_float4(ownerMoi[3 * o], ownerMoi[3 * o + 1], ownerMoi[3 * o + 2], 0.f);
     c->hostMass.assign(ownerMass, ownerMass + nO);
+    // the adjoint handles single-sphere owners (each sphere its own owner, centred)
+    c->singleSpheres = nS == nClumps;
+    for (uint32_t s = 0; s < nS && c->singleSpheres; s++)
+        c->singleSpheres = sphOwner[s] == s && sphRelPos[3 * s] == 0.f && sphRelPos[3 * s + 1] == 0.f &&
+                           sphRelPos[3 * s + 2] == 0.f;
     DEME_HIP(c, c->sph.ensure(nS, 1.0));
     DEME_HIP(c, c->moi.ensure(nO, 1.0));
     DEME_HIP(c, c->own.ensure(nO, 1.0));
@@ -450,16 +488,19 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
 }
 
 int deme_upload_materials(deme_ctx* c, uint32_t n, const deme_material* m) {
+    c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (n == 0 || n > 256) return c->fail(DEME_ERR_INVALID_ARGUMENT, "1..256 materials required");
     DEME_HIP(c, c->mats.ensure(n, 1.0));
     DEME_HIP(c, hipMemcpyAsync(c->mats.p, m, n * sizeof(deme_material), hipMemcpyHostToDevice, c->stream));
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     c->nM = n;
+    c->hostMats.assign(m, m + n);
     c->haveMaterials = true;
     return DEME_OK;
 }
 
 int deme_upload_analytical(deme_ctx* c, uint32_t n, const deme_anal_obj* objs) {
+    c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before analytical objects");
     if (n > 65535) return c->fail(DEME_ERR_UNSUPPORTED, "more than 65535 analytical components");
     uint32_t maxMat = 0;
@@ -509,6 +550,7 @@ int deme_upload_scenes(deme_ctx* c, uint32_t nScenes, const uint32_t* ownerScene
 }
 
 int deme_upload_families(deme_ctx* c, const uint8_t* mask, const deme_family_rule* rules) {
+    c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     DEME_HIP(c, c->mask.ensure(DEME_NUM_MASK_ENTRIES, 1.0));
     DEME_HIP(c, c->rules.ensure(DEME_NUM_FAMILIES, 1.0));
     if (!rules && !c->haveFamilies) return c->fail(DEME_ERR_INVALID_ARGUMENT, "family rules required on first upload");
@@ -522,6 +564,7 @@ int deme_upload_families(deme_ctx* c, const uint8_t* mask, const deme_family_rul
 }
 
 int deme_upload_state(deme_ctx* c, const deme_owner_state* s) {
+    c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before the owner state");
     const uint32_t n = c->nO;
     std::vector<OwnerRec> h(n);
@@ -562,6 +605,16 @@ int deme_upload_state(deme_ctx* c, const deme_owner_state* s) {
                                        c->stream));
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     c->haveState = true;
+    return DEME_OK;
+}
+
+int deme_clear_contacts(deme_ctx* c) {
+    c->adjReady = false;
+    c->havePrev = c->contactsReady = c->fresh = false;
+    c->termsReady = c->forcesFull = false;
+    c->C = c->Cprev = c->nPairs = 0;
+    c->nSteps = 0;
+    c->time = 0.0;
     return DEME_OK;
 }
 
@@ -981,6 +1034,7 @@ int deme_integrate(deme_ctx* c) {
 }
 
 int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
+    c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     int r = check_ready(c);
     if (r) return r;
     const uint64_t f = updateFreq > 0 ? (uint64_t)updateFreq : 1;
@@ -1007,6 +1061,7 @@ int deme_set_cell_scale(deme_ctx* c, double scale) {
 }
 
 int deme_set_beta(deme_ctx* c, float beta) {
+    c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (!(beta >= 0.f) || !std::isfinite(beta)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "beta must be >= 0");
     c->sp.beta = beta;
     c->kp.beta = beta;
@@ -1068,6 +1123,257 @@ int deme_autotune(deme_ctx* c, const int* freqs, int nFreqs, const double* cellS
     deme_set_cell_scale(c, bs);
     c->contactsReady = false;
     *bestFreq = bf;
+    return DEME_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Adjoint: forward with checkpoints, backward with per-window recomputation.
+// ---------------------------------------------------------------------------------------------------------------
+static int adj_save(deme_ctx* c, deme_ctx::AdjCk& k, bool full) {
+    DEME_HIP(c, k.own.ensure(c->nO, 1.0));
+    DEME_HIP(c, hipMemcpyAsync(k.own.p, c->own.p, c->nO * sizeof(OwnerRec), hipMemcpyDeviceToDevice, c->stream));
+    k.haveList = c->contactsReady;
+    k.C = c->contactsReady ? c->C : 0;
+    if (k.haveList) {
+        const int cu = c->cur;
+        DEME_HIP(c, k.rowStart.ensure(c->nS + 1, 1.0));
+        DEME_HIP(c, k.idB.ensure(std::max<uint64_t>(k.C, 1), 1.0));
+        DEME_HIP(c, k.ctype.ensure(std::max<uint64_t>(k.C, 1), 1.0));
+        DEME_HIP(c, k.hist.ensure(std::max<uint64_t>(k.C, 1), 1.0));
+        DEME_HIP(c, hipMemcpyAsync(k.rowStart.p, c->rowStart[cu].p, (c->nS + 1) * 4, hipMemcpyDeviceToDevice, c->stream));
+        if (k.C) {
+            DEME_HIP(c, hipMemcpyAsync(k.idB.p, c->idB[cu].p, k.C * 4, hipMemcpyDeviceToDevice, c->stream));
+            DEME_HIP(c, hipMemcpyAsync(k.ctype.p, c->ctype[cu].p, k.C, hipMemcpyDeviceToDevice, c->stream));
+            DEME_HIP(c, hipMemcpyAsync(k.hist.p, c->hist[cu].p, k.C * sizeof(float4), hipMemcpyDeviceToDevice,
+                                       c->stream));
+        }
+        if (full) {
+            // idA, mirror slots and mirror rows, so the restored list can be stepped on without a detection pass
+            k.nPairs = c->nPairs;
+            DEME_HIP(c, k.idA.ensure(std::max<uint64_t>(k.C, 1), 1.0));
+            DEME_HIP(c, k.mirror.ensure(std::max<uint64_t>(k.C, 1), 1.0));
+            DEME_HIP(c, k.bRowStart.ensure(c->nS + 1, 1.0));
+            if (k.C) {
+                DEME_HIP(c, hipMemcpyAsync(k.idA.p, c->idA.p, k.C * 4, hipMemcpyDeviceToDevice, c->stream));
+                DEME_HIP(c, hipMemcpyAsync(k.mirror.p, c->mirror.p, k.C * 4, hipMemcpyDeviceToDevice, c->stream));
+            }
+            DEME_HIP(c, hipMemcpyAsync(k.bRowStart.p, c->bRowStart.p, (c->nS + 1) * 4, hipMemcpyDeviceToDevice,
+                                       c->stream));
+        }
+    }
+    if (full) {
+        DEME_HIP(c, k.acc.ensure((size_t)6 * c->nO, 1.0));
+        DEME_HIP(c, hipMemcpyAsync(k.acc.p, c->acc.p, (size_t)6 * c->nO * sizeof(float), hipMemcpyDeviceToDevice,
+                                   c->stream));
+    }
+    k.full = full;
+    k.time = c->time;
+    k.nSteps = c->nSteps;
+    return DEME_OK;
+}
+
+static int adj_restore(deme_ctx* c, const deme_ctx::AdjCk& k) {
+    DEME_HIP(c, hipMemcpyAsync(c->own.p, k.own.p, c->nO * sizeof(OwnerRec), hipMemcpyDeviceToDevice, c->stream));
+    const int cu = c->cur;
+    if (k.haveList) {
+        DEME_HIP(c, c->idB[cu].ensure(std::max<uint64_t>(k.C, 1)));
+        DEME_HIP(c, c->ctype[cu].ensure(std::max<uint64_t>(k.C, 1)));
+        DEME_HIP(c, c->hist[cu].ensure(std::max<uint64_t>(k.C, 1)));
+        DEME_HIP(c, hipMemcpyAsync(c->rowStart[cu].p, k.rowStart.p, (c->nS + 1) * 4, hipMemcpyDeviceToDevice, c->stream));
+        if (k.C) {
+            DEME_HIP(c, hipMemcpyAsync(c->idB[cu].p, k.idB.p, k.C * 4, hipMemcpyDeviceToDevice, c->stream));
+            DEME_HIP(c, hipMemcpyAsync(c->ctype[cu].p, k.ctype.p, k.C, hipMemcpyDeviceToDevice, c->stream));
+            DEME_HIP(c, hipMemcpyAsync(c->hist[cu].p, k.hist.p, k.C * sizeof(float4), hipMemcpyDeviceToDevice,
+                                       c->stream));
+        }
+        c->C = k.C;
+        if (k.full) {
+            c->nPairs = k.nPairs;
+            DEME_HIP(c, c->idA.ensure(std::max<uint64_t>(k.C, 1)));
+            DEME_HIP(c, c->mirror.ensure(std::max<uint64_t>(k.C, 1)));
+            DEME_HIP(c, c->bTerm.ensure(std::max<uint64_t>((uint64_t)6 * k.nPairs, 1)));
+            DEME_HIP(c, c->aTerm.ensure(std::max<uint64_t>((uint64_t)6 * k.C, 1)));
+            DEME_HIP(c, c->gTerm.ensure(std::max<uint64_t>((uint64_t)6 * k.C, 1)));
+            if (k.C) {
+                DEME_HIP(c, hipMemcpyAsync(c->idA.p, k.idA.p, k.C * 4, hipMemcpyDeviceToDevice, c->stream));
+                DEME_HIP(c, hipMemcpyAsync(c->mirror.p, k.mirror.p, k.C * 4, hipMemcpyDeviceToDevice, c->stream));
+            }
+            DEME_HIP(c, hipMemcpyAsync(c->bRowStart.p, k.bRowStart.p, (c->nS + 1) * 4, hipMemcpyDeviceToDevice,
+                                       c->stream));
+        }
+    }
+    if (k.full)
+        DEME_HIP(c, hipMemcpyAsync(c->acc.p, k.acc.p, (size_t)6 * c->nO * sizeof(float), hipMemcpyDeviceToDevice,
+                                   c->stream));
+    c->contactsReady = k.haveList;
+    c->havePrev = false;
+    c->fresh = false;
+    c->termsReady = false;
+    c->time = k.time;
+    c->nSteps = k.nSteps;
+    return DEME_OK;
+}
+
+// One window: a detection pass, then n steps of force -> reduce -> integrate. With record, the steps run on per-step
+// snapshot buffers instead of in place: owner records adjOwn[k] (before step k) -> adjOwn[k + 1], history
+// adjHist[k] -> adjHist[k + 1], accelerations into adjAcc[k]; the reverse sweep reads them back. The analytical
+// owners are fixed walls on this path, so their force sums (which no sphere reads) and poses are not recomputed per
+// step.
+static int adj_window(deme_ctx* c, int n, bool record) {
+    int r = deme_cd_run(c);
+    if (r) return r;
+    if (!record) {
+        for (int k = 0; k < n; k++) {
+            if ((r = force_pass(c, false))) return r;
+            if ((r = collect_and_update(c, 1))) return r;
+            c->time += (double)c->sp.h;
+            c->nSteps++;
+        }
+        return DEME_OK;
+    }
+    if ((int)c->adjOwn.size() < n + 1) {
+        c->adjOwn.resize(n + 1);
+        c->adjHist.resize(n + 1);
+        c->adjAcc.resize(n + 1);
+    }
+    const uint64_t C = c->C;
+    for (int k = 0; k <= n; k++) {
+        DEME_HIP(c, c->adjOwn[k].ensure(c->nO));
+        DEME_HIP(c, c->adjHist[k].ensure(std::max<uint64_t>(C, 1)));
+        DEME_HIP(c, c->adjAcc[k].ensure((size_t)6 * c->nO));
+    }
+    DEME_HIP(c, c->analAcc.ensure(MAX_ANAL_OWNERS * 6, 1.0));
+    const int cu = c->cur, pv = cu ^ 1;
+    DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
+    DEME_LAUNCH(c, "adj_gather", launch_adj_gather_hist((uint32_t)C, c->havePrev ? c->map.p : nullptr,
+                                                         (uint32_t)c->Cprev, c->hist[pv].p, c->adjHist[0].p,
+                                                         c->stream));
+    DEME_HIP(c, hipMemcpyAsync(c->adjOwn[0].p, c->own.p, c->nO * sizeof(OwnerRec), hipMemcpyDeviceToDevice,
+                               c->stream));
+    for (int k = 0; k < n; k++) {
+        DEME_LAUNCH(c, "force",
+                    launch_force(c->kp, (uint32_t)C, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->sph.p,
+                                 c->adjOwn[k].p, c->moi.p, c->anal.p, c->pose.p, c->mats.p, c->adjHist[k].p, nullptr,
+                                 c->adjHist[k + 1].p, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p, nullptr,
+                                 c->stream));
+        DEME_LAUNCH(c, "owner_update",
+                    launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
+                                        c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, c->analAcc.p,
+                                        c->rules.p, c->adjOwn[k].p, c->adjAcc[k].p, 1, (float)c->time, c->stream,
+                                        c->adjOwn[k + 1].p));
+        c->time += (double)c->sp.h;
+        c->nSteps++;
+    }
+    c->fresh = false;
+    c->termsReady = false;
+    return DEME_OK;
+}
+
+static int adj_check(deme_ctx* c) {
+    if (!c->singleSpheres)
+        return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint handles single-sphere owners (spheres as their own owners)");
+    if (c->nM * 4 > 256) return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint handles at most 64 materials");
+    for (const deme_material& m : c->hostMats)
+        if (m.Crr != 0.f) return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint has no rolling-resistance term (Crr = 0)");
+    if (c->sp.integrator != DEME_INTEGRATOR_EXTENDED_TAYLOR)
+        return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint differentiates the extended-Taylor integrator");
+    return DEME_OK;
+}
+
+int deme_adjoint_forward(deme_ctx* c, int nSteps, int window) {
+    int r = check_ready(c);
+    if (r) return r;
+    if ((r = adj_check(c))) return r;
+    if (nSteps <= 0 || window <= 0) return c->fail(DEME_ERR_INVALID_ARGUMENT, "need positive steps and window");
+    const int nWin = (nSteps + window - 1) / window;
+    c->adjCk.resize(nWin + 1);
+    for (int w = 0; w < nWin; w++) {
+        if ((r = adj_save(c, c->adjCk[w], false))) return r;
+        if ((r = adj_window(c, std::min(window, nSteps - w * window), false))) return r;
+    }
+    if ((r = adj_save(c, c->adjCk[nWin], true))) return r;
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->adjWindow = window;
+    c->adjStepsTotal = nSteps;
+    c->adjReady = true;
+    return DEME_OK;
+}
+
+int deme_adjoint_backward(deme_ctx* c, const float* dLdPos, const float* dLdVel, float* gradVel0, float* gradMat) {
+    if (!c->adjReady) return c->fail(DEME_ERR_NOT_READY, "deme_adjoint_forward must run first");
+    int r;
+    const uint32_t nO = c->nO;
+    const int window = c->adjWindow, nSteps = c->adjStepsTotal;
+    const int nWin = (nSteps + window - 1) / window;
+    const uint32_t nMatV = c->nM * 4;
+    DEME_HIP(c, c->lam.ensure((size_t)9 * nO, 1.0));
+    DEME_HIP(c, c->lAcc.ensure((size_t)6 * nO, 1.0));
+    DEME_HIP(c, c->matGrad.ensure(std::max<uint32_t>(nMatV, 1), 1.0));
+    DEME_HIP(c, hipMemsetAsync(c->lam.p, 0, (size_t)9 * nO * sizeof(float), c->stream));
+    DEME_HIP(c, hipMemsetAsync(c->matGrad.p, 0, nMatV * sizeof(float), c->stream));
+    if (dLdPos) DEME_HIP(c, hipMemcpyAsync(c->lam.p, dLdPos, (size_t)3 * nO * sizeof(float), hipMemcpyHostToDevice,
+                                           c->stream));
+    if (dLdVel) DEME_HIP(c, hipMemcpyAsync(c->lam.p + (size_t)3 * nO, dLdVel, (size_t)3 * nO * sizeof(float),
+                                           hipMemcpyHostToDevice, c->stream));
+    // the last step's history out feeds nothing
+    int cur = 0;
+    bool lDtSet = false;
+    for (int w = nWin - 1; w >= 0; w--) {
+        const int n = std::min(window, nSteps - w * window);
+        if ((r = adj_restore(c, c->adjCk[w]))) return r;
+        if ((r = adj_window(c, n, true))) return r;
+        const uint64_t C = c->C;
+        const int cu = c->cur;
+        DEME_HIP(c, c->lDt[0].ensure(std::max<uint64_t>(C, 1)));
+        DEME_HIP(c, c->lDt[1].ensure(std::max<uint64_t>(C, 1)));
+        DEME_HIP(c, c->adjA.ensure(std::max<uint64_t>((uint64_t)9 * C, 1)));
+        DEME_HIP(c, c->adjB.ensure(std::max<uint64_t>((uint64_t)9 * c->nPairs, 1)));
+        const uint32_t nb = adj_force_blocks((uint32_t)C);
+        DEME_HIP(c, c->matPart.ensure(std::max<uint64_t>((uint64_t)nb * nMatV, 1)));
+        if (!lDtSet) {
+            DEME_LAUNCH(c, "adj_fill", launch_fill_f4(c->lDt[cur].p, C, c->stream));
+            lDtSet = true;
+        }
+        // the window's last step through the integration; then per step the contacts, and the owners' contact sums
+        // fused with the integration of the step before
+        DEME_LAUNCH(c, "adj_owner_pre", launch_adj_owner_pre(nO, c->nClumps, c->sp.h, c->adjOwn[n - 1].p,
+                                                             c->adjAcc[n - 1].p, c->lam.p, c->lAcc.p, c->stream));
+        for (int k = n - 1; k >= 0; k--) {
+            DEME_LAUNCH(c, "adj_force",
+                        launch_adj_force(c->kp, (uint32_t)C, c->nClumps, c->idA.p, c->idB[cu].p, c->ctype[cu].p,
+                                         c->sph.p, c->adjOwn[k].p, c->moi.p, c->anal.p, c->pose.p, c->mats.p, c->nM,
+                                         c->adjHist[k].p, c->mirror.p, nO, c->lAcc.p, c->lDt[cur].p, c->adjA.p,
+                                         c->adjB.p, c->lDt[cur ^ 1].p, c->matPart.p, k != n - 1, c->stream));
+            DEME_LAUNCH(c, "adj_owner_post",
+                        launch_adj_owner_post(nO, c->nClumps, c->ownerSphStart.p, c->rowStart[cu].p, c->bRowStart.p,
+                                              c->adjA.p, c->adjB.p, c->lam.p, c->sp.h,
+                                              k > 0 ? c->adjOwn[k - 1].p : nullptr,
+                                              k > 0 ? c->adjAcc[k - 1].p : nullptr, c->lAcc.p, c->stream));
+            cur ^= 1;  // the history-in adjoint of step k is the history-out adjoint of step k - 1
+        }
+        DEME_LAUNCH(c, "adj_mat", launch_adj_mat_final(c->matPart.p, C ? nb : 0, nMatV, c->matGrad.p, c->stream));
+        if (w > 0) {
+            // back through the gather of the window's first step onto the previous window's final list
+            const uint64_t Cprev = c->adjCk[w].C;
+            DEME_HIP(c, c->lDtPrev.ensure(std::max<uint64_t>(Cprev, 1)));
+            DEME_LAUNCH(c, "adj_scatter", launch_adj_scatter_hist((uint32_t)C, c->havePrev ? c->map.p : nullptr,
+                                                                  (uint32_t)Cprev, c->lDt[cur].p, c->lDtPrev.p,
+                                                                  c->stream));
+            DEME_HIP(c, c->lDt[cur].ensure(std::max<uint64_t>(Cprev, 1)));
+            DEME_HIP(c, c->lDt[cur ^ 1].ensure(std::max<uint64_t>(Cprev, 1)));
+            if (Cprev)
+                DEME_HIP(c, hipMemcpyAsync(c->lDt[cur].p, c->lDtPrev.p, Cprev * sizeof(float4),
+                                           hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    if (gradVel0)
+        DEME_HIP(c, hipMemcpyAsync(gradVel0, c->lam.p + (size_t)3 * nO, (size_t)3 * nO * sizeof(float),
+                                   hipMemcpyDeviceToHost, c->stream));
+    if (gradMat && nMatV)
+        DEME_HIP(c, hipMemcpyAsync(gradMat, c->matGrad.p, nMatV * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    // leave the system where the forward run ended
+    if ((r = adj_restore(c, c->adjCk[nWin]))) return r;
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
     return DEME_OK;
 }
 

@@ -129,7 +129,27 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
                                const float* analAcc, const deme_family_rule* rules, OwnerRec* own,
                                float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate, float t,
-                               hipStream_t st);
+                               hipStream_t st, OwnerRec* ownOut = nullptr /* integrated records; null: in place */);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+
+// adj.hip (adjoint of the step for single-sphere owners; lam: 9 x nO SoA adjoints of X, v, w)
+uint32_t adj_force_blocks(uint32_t nC);
+hipError_t launch_adj_owner_pre(uint32_t nO, uint32_t nClumps, float h, const OwnerRec* ownSnap, const float* accSnap,
+                                float* lam, float* lAcc, hipStream_t st);
+hipError_t launch_adj_force(const KParams& P, uint32_t nC, uint32_t nClumps, const uint32_t* idA, const uint32_t* idB,
+                            const uint8_t* ctype, const SphereRec* sph, const OwnerRec* ownSnap, const float4* moi,
+                            const deme_anal_obj* anal, const PlanePose* pose, const deme_material* mats, uint32_t nMat,
+                            const float4* histIn, const uint32_t* mirror, uint32_t nO, const float* lAcc,
+                            const float4* lDtOut, float* adjA, float* adjB, float4* lDtIn, float* matPart,
+                            int accumulate, hipStream_t st);
+hipError_t launch_adj_owner_post(uint32_t nO, uint32_t nClumps, const uint32_t* ownerSphStart, const uint32_t* rowStart,
+                                 const uint32_t* bRowStart, const float* adjA, const float* adjB, float* lam,
+                                 float h, const OwnerRec* ownPrev, const float* accPrev, float* lAcc, hipStream_t st);
+hipError_t launch_adj_mat_final(const float* matPart, uint32_t nBlocks, uint32_t n, float* grad, hipStream_t st);
+hipError_t launch_adj_gather_hist(uint32_t nC, const uint32_t* map, uint32_t nPrev, const float4* prev, float4* out,
+                                  hipStream_t st);
+hipError_t launch_adj_scatter_hist(uint32_t nC, const uint32_t* map, uint32_t nPrev, const float4* lIn, float4* lPrev,
+                                   hipStream_t st);
+hipError_t launch_fill_f4(float4* p, uint64_t n, hipStream_t st);
 
 }  // namespace deme_int

@@ -430,8 +430,8 @@ __global__ __launch_bounds__(256) void k_owner_update(
     KParams P, uint32_t nO, uint32_t nClumps, uint32_t firstAnal, uint32_t nAnalOwners,
     const uint32_t* __restrict__ ownerSphStart, const uint32_t* __restrict__ rowStart,
     const uint32_t* __restrict__ bRowStart, const float* __restrict__ aTerm, const float* __restrict__ bTerm,
-    const float* __restrict__ analAcc, const deme_family_rule* __restrict__ rules, OwnerRec* __restrict__ own,
-    float* __restrict__ accOut, int doIntegrate, float t) {
+    const float* __restrict__ analAcc, const deme_family_rule* __restrict__ rules, const OwnerRec* own,
+    OwnerRec* ownOut, float* __restrict__ accOut, int doIntegrate, float t) {
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nO) return;
     OwnerRec r = load_owner(own, o);
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256) void k_owner_update(
         r.qz = a1 * d2 + b1 * c2 - c1 * b2 + d1 * a2;
     }
     (void)t;  // numeric prescriptions do not depend on time
-    uint4* dst = reinterpret_cast<uint4*>(own + o);
+    uint4* dst = reinterpret_cast<uint4*>(ownOut + o);
     const uint4* src = reinterpret_cast<const uint4*>(&r);
     dst[0] = src[0];
     dst[1] = src[1];
@@ -587,11 +587,11 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
                                const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
                                const float* analAcc, const deme_family_rule* rules, OwnerRec* own, float* acc,
-                               int doIntegrate, float t, hipStream_t st) {
+                               int doIntegrate, float t, hipStream_t st, OwnerRec* ownOut) {
     if (nO == 0) return hipSuccess;
     k_owner_update<<<blocks(nO, 256), 256, 0, st>>>(P, nO, nClumps, firstAnalOwner, nAnalOwners, ownerSphStart,
-                                                    rowStart, bRowStart, aTerm, bTerm, analAcc, rules, own, acc,
-                                                    doIntegrate, t);
+                                                    rowStart, bRowStart, aTerm, bTerm, analAcc, rules, own,
+                                                    ownOut ? ownOut : own, acc, doIntegrate, t);
     return hipGetLastError();
 }
 

@@ -79,7 +79,8 @@ EXPORTED = [
     "deme_download_bin_counts", "deme_download_bins", "deme_download_contacts", "deme_force", "deme_reduce_owner",
     "deme_integrate", "deme_step", "deme_download_forces", "deme_download_wildcards", "deme_upload_wildcards",
     "deme_get_counts", "deme_set_time", "deme_enable_kernel_timing", "deme_reset_kernel_timing",
-    "deme_kernel_timing", "deme_set_beta", "deme_set_cell_scale", "deme_autotune",
+    "deme_kernel_timing", "deme_set_beta", "deme_set_cell_scale", "deme_autotune", "deme_adjoint_forward",
+    "deme_adjoint_backward", "deme_clear_contacts",
 ]
 
 _lib = None
@@ -135,6 +136,9 @@ def load(path: str = LIB_PATH):
         "deme_set_beta": ([vp, f32], i32),
         "deme_set_cell_scale": ([vp, f64], i32),
         "deme_autotune": ([vp, vp, i32, vp, i32, i32, f32, f32, C.POINTER(i32), vp], i32),
+        "deme_adjoint_forward": ([vp, i32, i32], i32),
+        "deme_clear_contacts": ([vp], i32),
+        "deme_adjoint_backward": ([vp, vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -184,6 +188,18 @@ def decode_positions(params: SimParams, vox, lx, ly, lz) -> np.ndarray:
     x, y, z = (np.ascontiguousarray(a, np.uint16) for a in (lx, ly, lz))
     lib.deme_decode_positions(C.byref(params), n, ptr(v), ptr(x), ptr(y), ptr(z), ptr(out))
     return out
+
+
+def positions_f64(params: SimParams, vox, lx, ly, lz) -> np.ndarray:
+    """Owner positions in double in the voxel-grid frame (the world frame offset by LBF): voxel index times the voxel
+    size plus sub-voxel count times l per axis (ownerPosition, DEMHelperKernels.cu)."""
+    vox = np.asarray(vox, np.uint64)
+    nx, ny = int(params.nvXp2), int(params.nvYp2)
+    idx = (vox & np.uint64((1 << nx) - 1), (vox >> np.uint64(nx)) & np.uint64((1 << ny) - 1),
+           vox >> np.uint64(nx + ny))
+    loc = (lx, ly, lz)
+    return np.stack([idx[k].astype(np.float64) * params.voxelSize + np.asarray(loc[k], np.float64) * params.l
+                     for k in range(3)], 1)
 
 
 def make_state_struct(arrays: dict) -> OwnerState:
@@ -242,6 +258,10 @@ class DeviceSystem:
         arrs = {k: (np.ascontiguousarray(v, STATE_FIELDS[k]) if v is not None else None) for k, v in state.items()}
         s = make_state_struct(arrs)
         self._chk(self.lib.deme_upload_state(self.ctx, C.byref(s)))
+
+    def clear_contacts(self):
+        """Forget the contact list and histories; the step count and time restart at zero."""
+        self._chk(self.lib.deme_clear_contacts(self.ctx))
 
     def get_state(self) -> dict:
         arrs = empty_state(self.scene.n_owners)
@@ -315,6 +335,10 @@ class DeviceSystem:
 
     def set_beta(self, beta): self._chk(self.lib.deme_set_beta(self.ctx, float(beta)))
 
+    def upload_materials(self, mats):
+        mats = np.ascontiguousarray(mats, MATERIAL_DTYPE)
+        self._chk(self.lib.deme_upload_materials(self.ctx, len(mats), ptr(mats)))
+
     def set_cell_scale(self, scale): self._chk(self.lib.deme_set_cell_scale(self.ctx, float(scale)))
 
     def autotune(self, freqs=(1, 2, 5, 10), cell_scales=(1.0, 1.25), trial_steps=20, max_vel=0.0, safety=1.5):
@@ -326,6 +350,29 @@ class DeviceSystem:
         self._chk(self.lib.deme_autotune(self.ctx, ptr(f), len(f), ptr(cs), len(cs), int(trial_steps),
                                          float(max_vel), float(safety), C.byref(best), ptr(ms)))
         return best.value, ms.reshape(len(f), len(cs))
+
+    def adjoint_forward(self, n_steps, window):
+        """Forward run of n_steps with the contact list rebuilt every `window` steps, checkpointed for the adjoint."""
+        self._chk(self.lib.deme_adjoint_forward(self.ctx, int(n_steps), int(window)))
+
+    def adjoint_backward(self, dLdX=None, dLdV=None):
+        """Gradients of a loss with final-state derivatives dLdX, dLdV (n_owners x 3, float) with respect to the initial
+        velocities (n_owners x 3) and the materials (n_materials x 4: E, nu, CoR, mu)."""
+        no, nm = self.scene.n_owners, len(self.scene.materials)
+
+        def soa(a):
+            if a is None:
+                return None
+            a = np.asarray(a, np.float32).reshape(-1, 3)
+            full = np.zeros((no, 3), np.float32)
+            full[:a.shape[0]] = a
+            return np.ascontiguousarray(full.T)
+
+        x, v = soa(dLdX), soa(dLdV)
+        gV = np.zeros((3, no), np.float32)
+        gM = np.zeros((nm, 4), np.float32)
+        self._chk(self.lib.deme_adjoint_backward(self.ctx, ptr(x), ptr(v), ptr(gV), ptr(gM)))
+        return gV.T.copy(), gM
 
     def kernel_timing(self) -> dict:
         mx = 64

@@ -341,6 +341,50 @@ def polydisperse_bed(n=8_000_000, seed=1003, median_r=0.001, sigma=0.2, r_clip=(
     return sc
 
 
+def adjoint_drop(n=50_000, seed=1005, r_range=(0.002, 0.004), n_mat=4, velocity_sigma=0.5, h=1e-5, steps=200,
+                 window=20, beta=None, gap=0.02, E0=2e7):
+    """Spheres of radii uniform in r_range with n_mat materials assigned at random, on a cubic lattice of pitch
+    2 r_max (1 + gap) whose bottom layer rests on a floor plane, with seeded initial velocities N(0, velocity_sigma):
+    the differentiation workload (steps at h, contact list rebuilt every `window` steps with drift margin beta,
+    default 1.2 x a 4.5-sigma speed over one window)."""
+    rng = np.random.default_rng(seed)
+    radius = rng.uniform(r_range[0], r_range[1], n).astype(np.float32)
+    pitch = 2.0 * r_range[1] * (1.0 + gap)
+    side = max(2, int(np.ceil((2.0 * n) ** (1.0 / 3.0))))
+    layers = int(np.ceil(n / (side * side)))
+    bx = (side + 2) * pitch
+    bz = max((layers + 4) * pitch, bx)  # a cube: the voxel split of the reference sizing takes near-cubic boxes
+    box = (float(f32(bx)), float(f32(bx)), float(f32(bz)))
+    lbf = tuple(float(f32(-f32(b) / f32(2.0))) for b in box)
+    ii, jj, kk = np.meshgrid(np.arange(side), np.arange(side), np.arange(layers), indexing="ij")
+    order = np.lexsort((ii.ravel(), jj.ravel(), kk.ravel()))
+    i, j, k = ii.ravel()[order][:n], jj.ravel()[order][:n], kk.ravel()[order][:n]
+    floor = lbf[2] + pitch
+    xyz = np.stack([lbf[0] + 1.5 * pitch + i * pitch, lbf[1] + 1.5 * pitch + j * pitch,
+                    floor + radius.astype(np.float64) * (1.0 - 1e-3) * (k == 0) + (k > 0) * (0.5 * pitch + k * pitch)],
+                   1)  # the bottom layer presses 0.1% of its radius into the floor: no contact starts at zero overlap
+    mat = rng.integers(0, n_mat, n).astype(np.uint8)
+    mats = np.zeros(n_mat, MATERIAL_DTYPE)
+    for m in range(n_mat):
+        mats[m] = (E0 * (1 + m), 0.25 + 0.03 * m, 0.5 + 0.1 * m, 0.3 + 0.1 * m, 0.0)
+    mass = (RHO * 4.0 / 3.0 * np.pi * radius.astype(np.float64) ** 3).astype(np.float32)
+    moi = (2.0 / 5.0 * mass.astype(np.float64) * radius.astype(np.float64) ** 2).astype(np.float32)
+    if beta is None:
+        beta = 1.2 * 4.5 * velocity_sigma * h * window
+    params = build_sim_params(box, lbf, float(radius.min()), h, beta=float(f32(beta)))
+    plane = np.zeros(1, ANAL_DTYPE)
+    plane[0]["owner"] = n
+    plane[0]["type"] = 0
+    plane[0]["material"] = 0
+    plane[0]["relPos"] = np.array([0.0, 0.0, floor], np.float32)
+    plane[0]["rot"] = np.array([0.0, 0.0, 1.0], np.float32)
+    plane[0]["mass"] = f32(1e15)
+    vel, _, _ = random_motion(rng, n, velocity_sigma)
+    return _finish("adjoint_drop", params, xyz, np.arange(n), radius, np.zeros((n, 3)), mat, mass,
+                   np.repeat(moi[:, None], 3, 1), mats, plane, 1, window, steps, vel=vel,
+                   info=dict(box=box, pitch=pitch, window=window))
+
+
 def voxel_split(box):
     """Bits per axis (summing to 64) and sub-voxel unit l covering `box` with the smallest l."""
     best = None

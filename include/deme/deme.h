@@ -80,6 +80,9 @@ int deme_upload_scenes(deme_ctx* ctx, uint32_t nScenes, const uint32_t* ownerSce
 /* State I/O (dT getters/setters, src/DEM/dT.cpp:1722-1806; initial encoding dT.cpp:572-575). */
 int deme_upload_state(deme_ctx* ctx, const deme_owner_state* s);
 int deme_download_state(deme_ctx* ctx, deme_owner_state* s);
+/* Drops the contact list with its tangential histories and restarts the step count and time at zero, so the next step
+ * begins with a detection pass on the uploaded state, as the first step after Initialize (API.h:620) does. */
+int deme_clear_contacts(deme_ctx* ctx);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Contact detection (contactDetection, DEMCubContactDetection.cu:39-397). The four stages may be called in order,
@@ -126,6 +129,22 @@ int deme_set_cell_scale(deme_ctx* ctx, double scale);
  * advance the simulation. */
 int deme_autotune(deme_ctx* ctx, const int* freqs, int nFreqs, const double* cellScales, int nScales, int trialSteps,
                   float maxVel, float safety, int* bestFreq, double* msPerStep);
+
+/* Adjoint of the step (no counterpart in the reference, which has no differentiation; for calibration callers that
+ * fit material parameters or initial velocities to observed states). Systems of single-sphere owners (each sphere
+ * its own, centred owner) with free families, fixed analytical walls, Crr = 0, the extended-Taylor integrator and at
+ * most 64 materials; DEME_ERR_UNSUPPORTED otherwise.
+ *
+ * deme_adjoint_forward runs nSteps steps from the current state with the contact list rebuilt every `window` steps
+ * (the drift margin set by deme_set_params / deme_set_beta must cover a window) and checkpoints the state and contact
+ * list at every window start. deme_adjoint_backward then returns the gradients of a loss L on the final state:
+ * dLdPos and dLdVel (nullable) hold dL/dX and dL/dv of the final owner positions and velocities, gradVel0 (nullable)
+ * receives dL/dv of the initial velocities, each 3 x nOwners floats component-major (all x, then all y, then all z;
+ * wall owners are ignored), and gradMat (nullable) receives dL/d(E, nu, CoR, mu) of every material (4 floats each).
+ * The backward pass recomputes each window from its checkpoint, holding the contact set of the window fixed, and
+ * leaves the system in the final state of the forward run. */
+int deme_adjoint_forward(deme_ctx* ctx, int nSteps, int window);
+int deme_adjoint_backward(deme_ctx* ctx, const float* dLdPos, const float* dLdVel, float* gradVel0, float* gradMat);
 
 /* Per-contact outputs (contactForces, contactTorque_convToForce, contactPointGeometryA/B): 3 floats each. Kept by
  * deme_force; deme_step stores per-owner terms instead, and a download after it returns DEME_ERR_NOT_READY. */

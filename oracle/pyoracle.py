@@ -199,3 +199,127 @@ def figure_out_nv(bx, by, bz):
 
 def literal17(v):
     return load().ora_literal17(v)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# FP64 host adjoint (oracle/adjoint_ref.cpp): sphere owners, fixed planes, contact set frozen per window.
+# ---------------------------------------------------------------------------------------------------------------------
+class AdjProblem(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("X0", C.c_void_p), ("V0", C.c_void_p), ("W0", C.c_void_p), ("r", C.c_void_p),
+                ("m", C.c_void_p), ("I", C.c_void_p), ("mat", C.c_void_p), ("nP", C.c_uint32),
+                ("planeP", C.c_void_p), ("planeN", C.c_void_p), ("planeMat", C.c_void_p), ("planeMass", C.c_void_p),
+                ("nMat", C.c_uint32), ("E", C.c_void_p), ("nu", C.c_void_p), ("CoR", C.c_void_p), ("mu", C.c_void_p),
+                ("h", C.c_double), ("gx", C.c_double), ("gy", C.c_double), ("gz", C.c_double),
+                ("beta", C.c_double), ("nSteps", C.c_int), ("window", C.c_int)]
+
+
+def positions_fp64(params, vox, lx, ly, lz):
+    """Owner positions in double in the frame of the voxel grid (offset LBF), as ownerPosition computes them."""
+    vox = np.asarray(vox, np.uint64)
+    nx, ny = int(params.nvXp2), int(params.nvYp2)
+    ix = (vox & np.uint64((1 << nx) - 1)).astype(np.float64)
+    iy = ((vox >> np.uint64(nx)) & np.uint64((1 << ny) - 1)).astype(np.float64)
+    iz = (vox >> np.uint64(nx + ny)).astype(np.float64)
+    vs, l = params.voxelSize, params.l
+    return np.stack([ix * vs + np.asarray(lx, np.float64) * l, iy * vs + np.asarray(ly, np.float64) * l,
+                     iz * vs + np.asarray(lz, np.float64) * l], 1)
+
+
+def _rotate(q, v):
+    """Rotate rows of v by unit quaternions q (w, x, y, z) in double."""
+    w, x, y, z = (q[:, k].astype(np.float64) for k in range(4))
+    u = np.stack([x, y, z], 1)
+    t = 2.0 * np.cross(u, v)
+    return v + w[:, None] * t + np.cross(u, t)
+
+
+class AdjointFP64:
+    """The FP64 forward simulator and reverse sweep of a scene of single-sphere owners (each sphere its own owner,
+    centred) with fixed planes, starting from `state` (default: the scene's)."""
+
+    def __init__(self, scene, n_steps, window, state=None):
+        self.lib = load()
+        lib = self.lib
+        lib.ora_adj_forward.argtypes = [C.POINTER(AdjProblem)] + [C.c_void_p] * 6 + [C.POINTER(C.c_uint64)]
+        lib.ora_adj_forward.restype = C.c_int
+        lib.ora_adj_gradient.argtypes = [C.POINTER(AdjProblem)] + [C.c_void_p] * 6
+        lib.ora_adj_gradient.restype = C.c_int
+        st = scene.state if state is None else state
+        n = scene.n_spheres
+        assert scene.n_clumps == n and np.array_equal(scene.sph_owner, np.arange(n)), "single-sphere owners only"
+        p = scene.params
+        X = positions_fp64(p, st["voxelID"], st["locX"], st["locY"], st["locZ"])
+        q = np.stack([st["oriQw"], st["oriQx"], st["oriQy"], st["oriQz"]], 1)
+        self.X0 = np.ascontiguousarray(X[:n])
+        self.V0 = np.ascontiguousarray(np.stack([st["vX"], st["vY"], st["vZ"]], 1)[:n].astype(np.float64))
+        wb = np.stack([st["omgX"], st["omgY"], st["omgZ"]], 1).astype(np.float64)
+        self.W0 = np.ascontiguousarray(_rotate(q[:n], wb[:n]))
+        self.r = np.ascontiguousarray(scene.sph_radius.astype(np.float64))
+        self.m = np.ascontiguousarray(scene.owner_mass[:n].astype(np.float64))
+        self.I = np.ascontiguousarray(scene.owner_moi[:n, 0].astype(np.float64))
+        self.mat = np.ascontiguousarray(scene.sph_mat.astype(np.uint32))
+        planes = [a for a in scene.anal if a["type"] == 0]
+        assert len(planes) == len(scene.anal), "planes only"
+        P, N = [], []
+        for a in planes:
+            o = int(a["owner"])
+            qo = q[o:o + 1]
+            P.append(X[o] + _rotate(qo, a["relPos"][None].astype(np.float64))[0])
+            N.append(_rotate(qo, a["rot"][None].astype(np.float64))[0])
+        self.planeP = np.ascontiguousarray(np.array(P, np.float64).reshape(-1, 3))
+        self.planeN = np.ascontiguousarray(np.array(N, np.float64).reshape(-1, 3))
+        self.planeMat = np.ascontiguousarray(np.array([a["material"] for a in planes], np.uint32))
+        self.planeMass = np.ascontiguousarray(np.array([a["mass"] for a in planes], np.float64))
+        mats = scene.materials
+        self.mats = {k: np.ascontiguousarray(mats[k].astype(np.float64)) for k in ("E", "nu", "CoR", "mu")}
+        pr = AdjProblem()
+        pr.n, pr.nP, pr.nMat = n, len(planes), len(mats)
+        for k in ("X0", "V0", "W0", "r", "m", "I", "mat", "planeP", "planeN", "planeMat", "planeMass"):
+            setattr(pr, k, getattr(self, k).ctypes.data)
+        for k in ("E", "nu", "CoR", "mu"):
+            setattr(pr, k, self.mats[k].ctypes.data)
+        pr.h = float(p.h)
+        pr.gx, pr.gy, pr.gz = (float(g) for g in p.G)
+        pr.beta = float(p.beta)
+        pr.nSteps, pr.window = int(n_steps), int(window)
+        self.pr, self.n, self.n_mat = pr, n, len(mats)
+
+    def forward(self, E=None, nu=None, CoR=None, mu=None, V0=None):
+        """Final positions and velocities (n x 3 each) with optionally perturbed materials or initial velocities."""
+        keep = []
+
+        def arr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64)
+            keep.append(a)
+            return a.ctypes.data
+
+        pr = self.pr
+        if V0 is not None:
+            pr = AdjProblem.from_buffer_copy(self.pr)
+            v0 = np.ascontiguousarray(V0, np.float64)
+            keep.append(v0)
+            pr.V0 = v0.ctypes.data
+        X = np.zeros((self.n, 3))
+        V = np.zeros((self.n, 3))
+        nc = C.c_uint64()
+        st = self.lib.ora_adj_forward(C.byref(pr), arr(E), arr(nu), arr(CoR), arr(mu), ptr(X), ptr(V), C.byref(nc))
+        if st:
+            raise OracleError(f"ora_adj_forward: {st}")
+        return X, V
+
+    def gradient(self, dLdX=None, dLdV=None):
+        """Gradients with respect to the initial velocities (n x 3) and the materials (nMat x 4: E, nu, CoR, mu) of the
+        loss whose derivatives on the final positions and velocities are dLdX, dLdV; also the final state."""
+        gV = np.zeros((self.n, 3))
+        gM = np.zeros((self.n_mat, 4))
+        X = np.zeros((self.n, 3))
+        V = np.zeros((self.n, 3))
+        a = None if dLdX is None else np.ascontiguousarray(dLdX, np.float64)
+        b = None if dLdV is None else np.ascontiguousarray(dLdV, np.float64)
+        st = self.lib.ora_adj_gradient(C.byref(self.pr), None if a is None else ptr(a), None if b is None else ptr(b),
+                                       ptr(gV), ptr(gM), ptr(X), ptr(V))
+        if st:
+            raise OracleError(f"ora_adj_gradient: {st}")
+        return gV, gM, X, V
