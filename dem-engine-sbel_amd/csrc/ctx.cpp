@@ -62,6 +62,12 @@ struct deme_ctx {
     deme_sim_params sp{};
     KParams kp{};
     bool haveParams = false, haveTemplates = false, haveState = false, haveMaterials = false, haveFamilies = false;
+    // force kernel specialised at run time (hipRTC); inactive: the ahead-of-time kernel of the built-in model
+    JitForce jit;
+    bool jitActive = false, jitStale = false, jitCustom = false;
+    std::string jitModel;
+    std::vector<std::string> jitWildcards;
+    std::vector<std::pair<std::string, std::string>> jitOverrides;
     uint32_t nS = 0, nO = 0, nClumps = 0, nA = 0, nM = 0, nAnalOwners = 0;
     uint32_t maxMatSph = 0, maxMatAnal = 0, maxMat = 0;
     uint64_t nBins = 0;
@@ -360,6 +366,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     for (auto& b : c->adjAcc) b.release();
     c->lam.release(); c->lAcc.release(); c->adjA.release(); c->adjB.release(); c->matPart.release();
     c->matGrad.release(); c->lDt[0].release(); c->lDt[1].release(); c->lDtPrev.release();
+    c->jit.release();
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -408,6 +415,7 @@ int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
     // the per-bin arrays of the staged path are sized on its first use (deme_cd_bin_count)
     c->nBins = nb;
     c->haveParams = true;
+    if (c->jitActive) c->jitStale = true;  // the specialised kernel holds the old constants
     return DEME_OK;
 }
 
@@ -979,20 +987,47 @@ int deme_download_contacts(deme_ctx* c, uint32_t* idA, uint32_t* idB, uint8_t* t
 // ---------------------------------------------------------------------------------------------------------------
 // Dynamics
 // ---------------------------------------------------------------------------------------------------------------
+// Compiles the specialised force kernel for the context's model, placeholder overrides and constants.
+static int jit_rebuild(deme_ctx* c) {
+    std::string src, err, log;
+    if (!jit_force_source(c->jitModel, c->jitWildcards, c->haveParams ? &c->sp : nullptr, c->jitOverrides, src, err))
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, err);
+    hipDeviceProp_t prop;
+    DEME_HIP(c, hipGetDeviceProperties(&prop, c->device));
+    std::vector<char> code;
+    if (jit_compile(src, prop.gcnArchName, code, log))
+        return c->fail(DEME_ERR_JIT, "hipRTC compilation of the force model failed:\n" + log);
+    DEME_HIP(c, c->jit.load(code));
+    c->jitStale = false;
+    return DEME_OK;
+}
+
+// The force kernel of the context: the specialised one when a model or placeholder values were given, else the
+// ahead-of-time kernel of the built-in model.
+static hipError_t launch_force_ctx(deme_ctx* c, bool full, uint32_t nC, const uint32_t* idB, const uint8_t* ctype,
+                                   const OwnerRec* own, const float4* histIn, const uint32_t* map, float4* histOut,
+                                   ForceRec* out) {
+    if (c->jitActive)
+        return c->jit.launch(full, c->kp, nC, c->idA.p, idB, ctype, c->sph.p, own, c->moi.p, c->anal.p, c->pose.p,
+                             c->mats.p, histIn, map, histOut, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p, out,
+                             c->stream);
+    return launch_force(c->kp, nC, c->idA.p, idB, ctype, c->sph.p, own, c->moi.p, c->anal.p, c->pose.p, c->mats.p,
+                        histIn, map, histOut, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p, out, c->stream);
+}
+
 static int force_pass(deme_ctx* c, bool full) {
     int r = check_ready(c);
     if (r) return r;
+    if (c->jitActive && c->jitStale && (r = jit_rebuild(c))) return r;
     if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "contact detection must run before the force pass");
     const int cu = c->cur, pv = cu ^ 1;
     const bool remap = c->fresh;
     if (full) DEME_HIP(c, c->frc.ensure(c->C));
     DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
     DEME_LAUNCH(c, "force",
-                launch_force(c->kp, (uint32_t)c->C, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->sph.p, c->own.p,
-                             c->moi.p, c->anal.p, c->pose.p, c->mats.p,
-                             remap ? (c->havePrev ? c->hist[pv].p : c->hist[cu].p) : c->hist[cu].p,
-                             remap ? c->map.p : nullptr, c->hist[cu].p, c->mirror.p, c->aTerm.p, c->bTerm.p,
-                             c->gTerm.p, full ? c->frc.p : nullptr, c->stream));
+                launch_force_ctx(c, full, (uint32_t)c->C, c->idB[cu].p, c->ctype[cu].p, c->own.p,
+                                 remap ? (c->havePrev ? c->hist[pv].p : c->hist[cu].p) : c->hist[cu].p,
+                                 remap ? c->map.p : nullptr, c->hist[cu].p, full ? c->frc.p : nullptr));
     c->fresh = false;
     c->forcesFull = full;
     c->termsReady = true;
@@ -1251,11 +1286,8 @@ static int adj_window(deme_ctx* c, int n, bool record) {
     DEME_HIP(c, hipMemcpyAsync(c->adjOwn[0].p, c->own.p, c->nO * sizeof(OwnerRec), hipMemcpyDeviceToDevice,
                                c->stream));
     for (int k = 0; k < n; k++) {
-        DEME_LAUNCH(c, "force",
-                    launch_force(c->kp, (uint32_t)C, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->sph.p,
-                                 c->adjOwn[k].p, c->moi.p, c->anal.p, c->pose.p, c->mats.p, c->adjHist[k].p, nullptr,
-                                 c->adjHist[k + 1].p, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p, nullptr,
-                                 c->stream));
+        DEME_LAUNCH(c, "force", launch_force_ctx(c, false, (uint32_t)C, c->idB[cu].p, c->ctype[cu].p, c->adjOwn[k].p,
+                                                 c->adjHist[k].p, nullptr, c->adjHist[k + 1].p, nullptr));
         DEME_LAUNCH(c, "owner_update",
                     launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
                                         c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, c->analAcc.p,
@@ -1270,6 +1302,8 @@ static int adj_window(deme_ctx* c, int n, bool record) {
 }
 
 static int adj_check(deme_ctx* c) {
+    if (c->jitCustom)
+        return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint differentiates the built-in frictional Hertz-Mindlin model");
     if (!c->singleSpheres)
         return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint handles single-sphere owners (spheres as their own owners)");
     if (c->nM * 4 > 256) return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint handles at most 64 materials");
@@ -1375,6 +1409,58 @@ int deme_adjoint_backward(deme_ctx* c, const float* dLdPos, const float* dLdVel,
     if ((r = adj_restore(c, c->adjCk[nWin]))) return r;
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     return DEME_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Force model and runtime specialisation.
+// ---------------------------------------------------------------------------------------------------------------
+const char* deme_builtin_force_model(int which) {
+    return (which == 0 || which == 1) ? jit_builtin_model(which) : nullptr;
+}
+
+int deme_set_force_model(deme_ctx* c, const char* model, const char* const* wildcards, int nWildcards) {
+    c->adjReady = false;
+    if (!model) {
+        c->jit.release();
+        c->jitActive = c->jitStale = c->jitCustom = false;
+        c->jitOverrides.clear();
+        return DEME_OK;
+    }
+    if (nWildcards < 0 || (nWildcards > 0 && !wildcards))
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, "wildcard names missing");
+    std::vector<std::string> w;
+    for (int k = 0; k < nWildcards; k++) {
+        if (!wildcards[k]) return c->fail(DEME_ERR_INVALID_ARGUMENT, "null wildcard name");
+        w.emplace_back(wildcards[k]);
+    }
+    c->jitModel = model;
+    c->jitWildcards = w;
+    c->jitOverrides.clear();
+    c->jitActive = true;
+    c->jitStale = true;
+    c->jitCustom = true;
+    return jit_rebuild(c);
+}
+
+int deme_jit_build(deme_ctx* c, const char* const* keys, const char* const* values, int n) {
+    c->adjReady = false;
+    if (n < 0 || (n > 0 && (!keys || !values))) return c->fail(DEME_ERR_INVALID_ARGUMENT, "keys/values missing");
+    std::vector<std::pair<std::string, std::string>> ov;
+    for (int k = 0; k < n; k++) {
+        if (!keys[k] || !values[k]) return c->fail(DEME_ERR_INVALID_ARGUMENT, "null placeholder key or value");
+        ov.emplace_back(keys[k], values[k]);
+    }
+    if (!c->jitActive) {
+        c->jitModel = jit_builtin_model(0);
+        c->jitWildcards = {"delta_tan_x", "delta_tan_y", "delta_tan_z", "delta_time"};
+        c->jitCustom = false;
+    }
+    for (const auto& kv : ov)
+        if (kv.first == "_DEMForceModel_" || kv.first.rfind("_forceModel", 0) == 0) c->jitCustom = true;
+    c->jitOverrides = ov;
+    c->jitActive = true;
+    c->jitStale = true;
+    return jit_rebuild(c);
 }
 
 int deme_download_forces(deme_ctx* c, float* F, float* T, float* cA, float* cB) {

@@ -4,8 +4,10 @@
 // exactly like the reference's arithmetic (no fused multiply-adds are introduced).
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "../../include/deme/deme_types.h"
 
@@ -55,6 +57,16 @@ struct __attribute__((aligned(16))) ForceRec {
     float bx, by, bz;
 };
 static_assert(sizeof(ForceRec) == 48, "ForceRec must be 48 bytes");
+
+// World pose of one analytical component, recomputed whenever owner states change.
+struct __attribute__((aligned(16))) PlanePose {
+    double px, py, pz;
+    float nx, ny, nz;
+    uint32_t owner;
+    uint32_t fam;
+    uint32_t type;
+};
+static_assert(sizeof(PlanePose) == 48, "PlanePose must be 48 bytes");
 
 // Kernel-visible simulation constants.
 struct KParams {

@@ -15,17 +15,8 @@ using deme_dev::CdRec;
 using deme_dev::ForceRec;
 using deme_dev::KParams;
 using deme_dev::OwnerRec;
+using deme_dev::PlanePose;
 using deme_dev::SphereRec;
-
-// World pose of one analytical component, recomputed whenever owner states change.
-struct __attribute__((aligned(16))) PlanePose {
-    double px, py, pz;
-    float nx, ny, nz;
-    uint32_t owner;
-    uint32_t fam;
-    uint32_t type;
-};
-static_assert(sizeof(PlanePose) == 48, "PlanePose must be 48 bytes");
 
 // Uniform cell grid over the simulation world used to generate candidate pairs (edge >= largest interaction
 // distance, so partners lie in the 27 neighbouring cells).
@@ -131,6 +122,25 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate, float t,
                                hipStream_t st, OwnerRec* ownOut = nullptr /* integrated records; null: in place */);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+
+// jit_rt.cpp: the force kernel specialised at run time by hipRTC
+struct JitForce {
+    hipModule_t mod = nullptr;
+    hipFunction_t full = nullptr, terms = nullptr;
+    void release();
+    hipError_t load(const std::vector<char>& code);
+    hipError_t launch(bool wantFull, const KParams& P, uint32_t nC, const uint32_t* idA, const uint32_t* idB,
+                      const uint8_t* ctype, const SphereRec* sph, const OwnerRec* own, const float4* moi,
+                      const deme_anal_obj* anal, const PlanePose* pose, const deme_material* mats,
+                      const float4* histIn, const uint32_t* map, float4* histOut, const uint32_t* mirror,
+                      float* aTerm, float* bTerm, float* gTerm, ForceRec* out, hipStream_t st) const;
+};
+// built-in model bodies: 0 frictional Hertz-Mindlin (the ahead-of-time kernel's), 1 frictionless Hertz
+const char* jit_builtin_model(int which);
+bool jit_force_source(const std::string& model, const std::vector<std::string>& wildcards, const deme_sim_params* p,
+                      const std::vector<std::pair<std::string, std::string>>& overrides, std::string& src,
+                      std::string& err);
+int jit_compile(const std::string& src, const std::string& arch, std::vector<char>& code, std::string& log);
 
 // adj.hip (adjoint of the step for single-sphere owners; lam: 9 x nO SoA adjoints of X, v, w)
 uint32_t adj_force_blocks(uint32_t nC);

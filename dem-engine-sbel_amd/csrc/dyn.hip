@@ -9,18 +9,13 @@
 // double and rounded to float), and the device code is compiled without contraction.
 #include "deme_internal.h"
 
+// Rows a14-a15, the contact force kernel: generated at build time from jit/force_kernel.tpl with the default
+// frictional Hertz-Mindlin model (jit/hertz_frictional.model), the same source the runtime hipRTC path specialises.
+#include "force_aot.hip"
+
 namespace deme_int {
 using namespace deme_dev;
 namespace {
-
-constexpr double TINY_FLOAT = 1e-12;
-constexpr double HUGE_FLOAT = 1e15;
-constexpr double TWO_OVER_THREE = 0.666666666666667;
-constexpr double FOUR_OVER_THREE = 1.333333333333333;
-constexpr double FIVE_OVER_THREE = 1.666666666666667;
-constexpr double TWO_TIMES_SQRT_FIVE_OVER_SIX = 1.825741858350554;
-constexpr double PI = 3.1415926535897932385;
-constexpr double PI_SQUARED = 9.869604401089358;
 
 struct f3 {
     float x, y, z;
@@ -48,292 +43,6 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
 __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ float length(f3 v) { return sqrtf(dot(v, v)); }
 __device__ __forceinline__ void rotq(f3& v, float w, float x, float y, float z) { rotate(v.x, v.y, v.z, w, x, y, z); }
-
-// matProxy2ContactParam<float> (DEMHelperKernels.cu:316-339)
-__device__ __forceinline__ void mat_proxy(const deme_material& A, const deme_material& B, float& E, float& G,
-                                          float& CoR, float& mu, float& Crr) {
-    const float invE = (1. - A.nu * A.nu) / A.E + (1. - B.nu * B.nu) / B.E;
-    E = 1. / invE;
-    const float invG = 2. * (2. - A.nu) * (1. + A.nu) / A.E + 2. * (2. - B.nu) * (1. + B.nu) / B.E;
-    G = 1. / invG;
-    CoR = fminf(A.CoR, B.CoR);
-    mu = fmaxf(A.mu, B.mu);
-    Crr = fmaxf(A.Crr, B.Crr);
-}
-
-// forceToAcc and forceToAngAcc (DEMCollectForceKernels.cu:68-120) of one contact on one owner: F * modifier / mass
-// and cross(cp, R^T((F + Ftof) * modifier)) / MOI, six floats.
-__device__ __forceinline__ void store_term(float* __restrict__ dst, f3 F, f3 T, f3 cp, float mass, float qw, float qx,
-                                           float qy, float qz, float4 I, float mod) {
-    const f3 lin = F * mod / mass;
-    f3 myF = (F + T) * mod;
-    rotq(myF, qw, -qx, -qy, -qz);
-    const f3 ang = cross(cp, myF) / mk(I.x, I.y, I.z);
-    float2* d = reinterpret_cast<float2*>(dst);
-    d[0] = make_float2(lin.x, lin.y);
-    d[1] = make_float2(lin.z, ang.x);
-    d[2] = make_float2(ang.y, ang.z);
-}
-
-__device__ __forceinline__ void zero_term(float* __restrict__ dst) {
-    float2* d = reinterpret_cast<float2*>(dst);
-    d[0] = d[1] = d[2] = make_float2(0.f, 0.f);
-}
-
-// Rows a14-a15: calculateContactForces (DEMCalcForceKernels.cu:14-179) with FullHertzianForceModel.cu.
-// History (delta_tan_x, delta_tan_y, delta_tan_z, delta_time) is one float4 per contact. On the first step after a
-// detection pass the previous history is gathered through the persistent-contact map (rearrangeContactWildcards,
-// DEMPrepForceKernels.cu:33-55) inside this kernel instead of in a separate pass.
-// The kernel also stores each contact's owner terms (the per-contact half of collectContactForces): the A body's at
-// aTerm[i], the B body's at bTerm[mirror[i]] (its slot in the B sphere's mirror row) for sphere-sphere contacts and
-// at gTerm[i] for sphere-plane contacts. FULL additionally keeps the contact record (force, torque-only force,
-// contact points in both body frames) for inspection.
-template <bool FULL>
-__global__ __launch_bounds__(256) void k_force(KParams P, uint32_t nC, const uint32_t* __restrict__ idA,
-                                               const uint32_t* __restrict__ idB, const uint8_t* __restrict__ ctype,
-                                               const SphereRec* __restrict__ sph, const OwnerRec* __restrict__ own,
-                                               const float4* __restrict__ moi, const deme_anal_obj* __restrict__ anal,
-                                               const PlanePose* __restrict__ pose,
-                                               const deme_material* __restrict__ mats, const float4* histIn,
-                                               const uint32_t* __restrict__ map, float4* histOut,
-                                               const uint32_t* __restrict__ mirror, float* __restrict__ aTerm,
-                                               float* __restrict__ bTerm, float* __restrict__ gTerm,
-                                               ForceRec* __restrict__ out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nC) return;
-    const uint32_t listType = ctype[i];
-    uint32_t type = listType;
-    const float ts = P.h;
-    {
-        // guard: ids outside their tables never reach a gather (a malformed list yields a zero record)
-        const uint32_t a = idA[i], b = idB[i];
-        const bool ok = a < P.nS && ((type == DEME_SPHERE_SPHERE_CONTACT && b < P.nS) ||
-                                     (type == DEME_SPHERE_PLANE_CONTACT && b < P.nA));
-        if (!ok) {
-            const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (FULL) {
-                float4* o4 = reinterpret_cast<float4*>(out + i);
-                o4[0] = z;
-                o4[1] = z;
-                o4[2] = z;
-            }
-            histOut[i] = z;
-            zero_term(aTerm + (uint64_t)i * 6);
-            if (type == DEME_SPHERE_PLANE_CONTACT) zero_term(gTerm + (uint64_t)i * 6);
-            return;
-        }
-    }
-
-    // ---- body A (always a sphere) ----
-    const SphereRec sa = load_sphere(sph, idA[i]);
-    const OwnerRec oa = load_owner(own, sa.owner);
-    double aox, aoy, aoz;
-    owner_position(P, oa.voxel, oa.lx, oa.ly, oa.lz, aox, aoy, aoz);
-    float rx = sa.rx, ry = sa.ry, rz = sa.rz;
-    rotate(rx, ry, rz, oa.qw, oa.qx, oa.qy, oa.qz);
-    const double XA = aox + (double)rx, YA = aoy + (double)ry, ZA = aoz + (double)rz;
-    const float ARadius = sa.r, AMass = oa.mass;
-    const uint32_t matA = sa.mat;
-
-    // ---- body B ----
-    double box, boy, boz;
-    float BRadius, BMass, BOwnerMass;
-    uint32_t matB, BOwner;
-    float Bqw, Bqx, Bqy, Bqz;
-    f3 BLinVel, BRotVel;
-    double cpx = 0, cpy = 0, cpz = 0, overlapDepth = 0;
-    f3 B2A = mk(0, 0, 0);
-    const uint32_t bId = idB[i];
-    if (type == DEME_SPHERE_SPHERE_CONTACT) {
-        const SphereRec sb = load_sphere(sph, bId);
-        const OwnerRec ob = load_owner(own, sb.owner);
-        owner_position(P, ob.voxel, ob.lx, ob.ly, ob.lz, box, boy, boz);
-        float qx = sb.rx, qy = sb.ry, qz = sb.rz;
-        rotate(qx, qy, qz, ob.qw, ob.qx, ob.qy, ob.qz);
-        const double XB = box + (double)qx, YB = boy + (double)qy, ZB = boz + (double)qz;
-        BRadius = sb.r;
-        BMass = ob.mass;
-        BOwnerMass = ob.mass;
-        BOwner = sb.owner;
-        matB = sb.mat;
-        Bqw = ob.qw;
-        Bqx = ob.qx;
-        Bqy = ob.qy;
-        Bqz = ob.qz;
-        BLinVel = mk(ob.vx, ob.vy, ob.vz);
-        BRotVel = mk(ob.wx, ob.wy, ob.wz);
-        // checkSpheresOverlap<double, float> (DEMHelperKernels.cu:233-264)
-        const double radA = ARadius, radB = BRadius;
-        const double d2 = (XA - XB) * (XA - XB) + (YA - YB) * (YA - YB) + (ZA - ZB) * (ZA - ZB);
-        if (d2 > (radA + radB) * (radA + radB)) {
-            type = DEME_NOT_A_CONTACT;
-        } else {
-            B2A.x = (float)(XA - XB);
-            B2A.y = (float)(YA - YB);
-            B2A.z = (float)(ZA - ZB);
-            const float mag = sqrtf(B2A.x * B2A.x + B2A.y * B2A.y + B2A.z * B2A.z);
-            B2A.x /= mag;
-            B2A.y /= mag;
-            B2A.z /= mag;
-            overlapDepth = radA + radB - sqrt(d2);
-            cpx = XB + (radB - overlapDepth / 2.0) * (double)B2A.x;
-            cpy = YB + (radB - overlapDepth / 2.0) * (double)B2A.y;
-            cpz = ZB + (radB - overlapDepth / 2.0) * (double)B2A.z;
-        }
-    } else {
-        // analytical component (DEMCalcForceKernels.cu:112-151); pose precomputed with the same arithmetic
-        const deme_anal_obj ab = anal[bId];
-        const PlanePose pp = pose[bId];
-        const OwnerRec ob = load_owner(own, ab.owner);
-        owner_position(P, ob.voxel, ob.lx, ob.ly, ob.lz, box, boy, boz);
-        BRadius = (float)HUGE_FLOAT;
-        BMass = ab.mass;
-        BOwnerMass = ob.mass;
-        BOwner = ab.owner;
-        matB = ab.material;
-        Bqw = ob.qw;
-        Bqx = ob.qx;
-        Bqy = ob.qy;
-        Bqz = ob.qz;
-        BLinVel = mk(ob.vx, ob.vy, ob.vz);
-        BRotVel = mk(ob.wx, ob.wy, ob.wz);
-        if (ab.type == DEME_ANAL_OBJ_TYPE_PLANE) {
-            // checkSphereEntityOverlap<double, float> (DEMHelperKernels.cu:394-445), beta4Entity = 0
-            const double radA = ARadius;
-            const double dist = (XA - pp.px) * (double)pp.nx + (YA - pp.py) * (double)pp.ny + (ZA - pp.pz) * (double)pp.nz;
-            overlapDepth = (radA + (double)0.0f - dist) / 2.0;
-            if (overlapDepth < 0.0) {
-                type = DEME_NOT_A_CONTACT;
-            } else {
-                cpx = XA - (double)pp.nx * (dist + overlapDepth);
-                cpy = YA - (double)pp.ny * (dist + overlapDepth);
-                cpz = ZA - (double)pp.nz * (dist + overlapDepth);
-                B2A = mk(pp.nx, pp.ny, pp.nz);
-            }
-        } else {
-            type = DEME_NOT_A_CONTACT;
-        }
-    }
-
-    // ---- contact history ----
-    float4 w;
-    if (map) {
-        const uint32_t m = map[i];
-        w = (m == DEME_NULL_MAPPING_PARTNER) ? make_float4(0.f, 0.f, 0.f, 0.f) : histIn[m];
-    } else {
-        w = histIn[i];
-    }
-    float delta_tan_x = w.x, delta_tan_y = w.y, delta_tan_z = w.z, delta_time = w.w;
-
-    f3 force = mk(0, 0, 0), tof = mk(0, 0, 0);
-    f3 locCPA = mk(0, 0, 0), locCPB = mk(0, 0, 0);
-    if (type != DEME_NOT_A_CONTACT) {
-        locCPA = mk((float)(cpx - aox), (float)(cpy - aoy), (float)(cpz - aoz));
-        locCPB = mk((float)(cpx - box), (float)(cpy - boy), (float)(cpz - boz));
-        rotq(locCPA, oa.qw, -oa.qx, -oa.qy, -oa.qz);
-        rotq(locCPB, Bqw, -Bqx, -Bqy, -Bqz);
-        const f3 ALinVel = mk(oa.vx, oa.vy, oa.vz), ARotVel = mk(oa.wx, oa.wy, oa.wz);
-        // ---- FullHertzianForceModel.cu ----
-        float E_cnt, G_cnt, CoR_cnt, mu_cnt, Crr_cnt;
-        mat_proxy(mats[matA], mats[matB], E_cnt, G_cnt, CoR_cnt, mu_cnt, Crr_cnt);
-        f3 rotVelCPA = cross(ARotVel, locCPA);
-        f3 rotVelCPB = cross(BRotVel, locCPB);
-        rotq(rotVelCPA, oa.qw, oa.qx, oa.qy, oa.qz);
-        rotq(rotVelCPB, Bqw, Bqx, Bqy, Bqz);
-        float mass_eff, sqrt_Rd, beta;
-        f3 vrel_tan;
-        f3 delta_tan = mk(delta_tan_x, delta_tan_y, delta_tan_z);
-        {
-            const f3 velB2A = (ALinVel + rotVelCPA) - (BLinVel + rotVelCPB);
-            const float projection = dot(velB2A, B2A);
-            vrel_tan = velB2A - projection * B2A;
-            {
-                delta_tan += ts * vrel_tan;
-                const float disp_proj = dot(delta_tan, B2A);
-                delta_tan -= disp_proj * B2A;
-                delta_time += ts;
-            }
-            mass_eff = (AMass * BMass) / (AMass + BMass);
-            sqrt_Rd = sqrt(overlapDepth * (ARadius * BRadius) / (ARadius + BRadius));
-            const float Sn = 2. * E_cnt * sqrt_Rd;
-            const float loge = (CoR_cnt < TINY_FLOAT) ? (double)(float)log(TINY_FLOAT) : (double)logf(CoR_cnt);
-            beta = loge / sqrt(loge * loge + PI_SQUARED);
-            const float k_n = TWO_OVER_THREE * Sn;
-            const float gamma_n = TWO_TIMES_SQRT_FIVE_OVER_SIX * beta * sqrtf(Sn * mass_eff);
-            force += (float)(k_n * overlapDepth + gamma_n * projection) * B2A;
-        }
-        if (Crr_cnt > 0.0) {
-            bool should_add_rolling_resistance = true;
-            {
-                const float R_eff = sqrtf((ARadius * BRadius) / (ARadius + BRadius));
-                const float kn_simple = FOUR_OVER_THREE * E_cnt * sqrtf(R_eff);
-                const float gn_simple =
-                    -2.f * sqrtf((float)(FIVE_OVER_THREE * mass_eff * E_cnt)) * beta * powf(R_eff, 0.25f);
-                const float d_coeff = gn_simple / (2.f * sqrtf(kn_simple * mass_eff));
-                if (d_coeff < 1.0) {
-                    const float t_collision = PI * sqrtf(mass_eff / (kn_simple * (1.f - d_coeff * d_coeff)));
-                    if (delta_time <= t_collision) should_add_rolling_resistance = false;
-                }
-            }
-            if (should_add_rolling_resistance) {
-                const f3 v_rot = rotVelCPB - rotVelCPA;
-                const float v_rot_mag = length(v_rot);
-                if (v_rot_mag > TINY_FLOAT) tof = (v_rot / v_rot_mag) * (Crr_cnt * length(force));
-            }
-        }
-        if (mu_cnt > 0.0) {
-            const float kt = 8. * G_cnt * sqrt_Rd;
-            const float gt = -TWO_TIMES_SQRT_FIVE_OVER_SIX * beta * sqrtf(mass_eff * kt);
-            f3 tangent_force = (-kt) * delta_tan - gt * vrel_tan;
-            const float ft = length(tangent_force);
-            if (ft > TINY_FLOAT) {
-                const float ft_max = length(force) * mu_cnt;
-                if (ft > ft_max) {
-                    tangent_force = (ft_max / ft) * tangent_force;
-                    delta_tan = (tangent_force + gt * vrel_tan) / (-kt);
-                }
-            } else {
-                tangent_force = mk(0, 0, 0);
-            }
-            force += tangent_force;
-        }
-        delta_tan_x = delta_tan.x;
-        delta_tan_y = delta_tan.y;
-        delta_tan_z = delta_tan.z;
-    } else {
-        delta_tan_x = delta_tan_y = delta_tan_z = delta_time = 0.f;
-    }
-    if (FULL) {
-        ForceRec fr;
-        fr.fx = force.x;
-        fr.fy = force.y;
-        fr.fz = force.z;
-        fr.tx = tof.x;
-        fr.ty = tof.y;
-        fr.tz = tof.z;
-        fr.ax = locCPA.x;
-        fr.ay = locCPA.y;
-        fr.az = locCPA.z;
-        fr.bx = locCPB.x;
-        fr.by = locCPB.y;
-        fr.bz = locCPB.z;
-        float4* o4 = reinterpret_cast<float4*>(out + i);
-        const float4* f4 = reinterpret_cast<const float4*>(&fr);
-        o4[0] = f4[0];
-        o4[1] = f4[1];
-        o4[2] = f4[2];
-    }
-    histOut[i] = make_float4(delta_tan_x, delta_tan_y, delta_tan_z, delta_time);
-    // owner terms: A side +1, B side -1
-    store_term(aTerm + (uint64_t)i * 6, force, tof, locCPA, AMass, oa.qw, oa.qx, oa.qy, oa.qz, moi[sa.owner], 1.f);
-    if (listType == DEME_SPHERE_SPHERE_CONTACT) {
-        const uint32_t m = mirror[i];
-        if (m != DEME_NULL_MAPPING_PARTNER)
-            store_term(bTerm + (uint64_t)m * 6, force, tof, locCPB, BOwnerMass, Bqw, Bqx, Bqy, Bqz, moi[BOwner], -1.f);
-    } else {
-        store_term(gTerm + (uint64_t)i * 6, force, tof, locCPB, BOwnerMass, Bqw, Bqx, Bqy, Bqz, moi[BOwner], -1.f);
-    }
-}
 
 // Row a16 for analytical owners (planes): every sphere-plane contact contributes its B-side term (stored by the
 // force kernel) to the plane's owner. Deterministic two-level reduction: each lane sums 16 consecutive contacts in
@@ -556,10 +265,10 @@ hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, cons
                         ForceRec* out, hipStream_t st) {
     if (nC == 0) return hipSuccess;
     if (out)
-        k_force<true><<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats, histIn,
+        deme_force_full<<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats, histIn,
                                                        map, histOut, mirror, aTerm, bTerm, gTerm, out);
     else
-        k_force<false><<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats,
+        deme_force_terms<<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats,
                                                         histIn, map, histOut, mirror, aTerm, bTerm, gTerm, nullptr);
     return hipGetLastError();
 }

@@ -80,7 +80,8 @@ EXPORTED = [
     "deme_integrate", "deme_step", "deme_download_forces", "deme_download_wildcards", "deme_upload_wildcards",
     "deme_get_counts", "deme_set_time", "deme_enable_kernel_timing", "deme_reset_kernel_timing",
     "deme_kernel_timing", "deme_set_beta", "deme_set_cell_scale", "deme_autotune", "deme_adjoint_forward",
-    "deme_adjoint_backward", "deme_clear_contacts",
+    "deme_adjoint_backward", "deme_clear_contacts", "deme_set_force_model", "deme_jit_build",
+    "deme_builtin_force_model", "deme_jit_render", "deme_jit_compile",
 ]
 
 _lib = None
@@ -138,6 +139,11 @@ def load(path: str = LIB_PATH):
         "deme_autotune": ([vp, vp, i32, vp, i32, i32, f32, f32, C.POINTER(i32), vp], i32),
         "deme_adjoint_forward": ([vp, i32, i32], i32),
         "deme_clear_contacts": ([vp], i32),
+        "deme_set_force_model": ([vp, C.c_char_p, vp, i32], i32),
+        "deme_jit_build": ([vp, vp, vp, i32], i32),
+        "deme_builtin_force_model": ([i32], C.c_char_p),
+        "deme_jit_render": ([C.c_char_p, vp, i32, vp, vp, u64, C.POINTER(u64)], i32),
+        "deme_jit_compile": ([C.c_char_p, vp, u64], i32),
         "deme_adjoint_backward": ([vp, vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
@@ -202,6 +208,39 @@ def positions_f64(params: SimParams, vox, lx, ly, lz) -> np.ndarray:
                      for k in range(3)], 1)
 
 
+def _cstr_array(items):
+    if not items:
+        return None
+    return (C.c_char_p * len(items))(*[x.encode() for x in items])
+
+
+def builtin_force_model(which=0) -> str:
+    """Source of a built-in force model body: 0 frictional Hertz-Mindlin, 1 frictionless Hertz."""
+    return load().deme_builtin_force_model(int(which)).decode()
+
+
+def jit_render(model=None, wildcards=(), params=None) -> str:
+    """The force kernel source specialised for a model (None: the built-in one) and constants (None: runtime)."""
+    lib = load()
+    arr = _cstr_array(list(wildcards))
+    m = None if model is None else model.encode()
+    need = C.c_uint64()
+    args = (m, C.cast(arr, C.c_void_p) if arr else None, len(wildcards), C.byref(params) if params else None)
+    st = lib.deme_jit_render(*args, None, 0, C.byref(need))
+    if st:
+        raise DemeError(st, "deme_jit_render failed")
+    buf = C.create_string_buffer(need.value)
+    lib.deme_jit_render(*args, buf, need.value, C.byref(need))
+    return buf.value.decode()
+
+
+def jit_compile(source: str):
+    """hipRTC compile of a force kernel source for gfx950: (status, log)."""
+    log = C.create_string_buffer(1 << 16)
+    st = load().deme_jit_compile(source.encode(), log, len(log))
+    return st, log.value.decode(errors="replace")
+
+
 def make_state_struct(arrays: dict) -> OwnerState:
     s = OwnerState()
     for k in STATE_FIELDS:
@@ -258,6 +297,22 @@ class DeviceSystem:
         arrs = {k: (np.ascontiguousarray(v, STATE_FIELDS[k]) if v is not None else None) for k, v in state.items()}
         s = make_state_struct(arrs)
         self._chk(self.lib.deme_upload_state(self.ctx, C.byref(s)))
+
+    def set_force_model(self, model, wildcards=()):
+        """Force-model body (reference model-string names) compiled by hipRTC; None returns to the built-in kernel."""
+        if model is None:
+            self._chk(self.lib.deme_set_force_model(self.ctx, None, None, 0))
+            return
+        arr = _cstr_array(wildcards)
+        self._chk(self.lib.deme_set_force_model(self.ctx, model.encode(), C.cast(arr, C.c_void_p) if arr else None,
+                                                len(wildcards)))
+
+    def jit_build(self, substitutions=None):
+        """Specialise the force kernel at run time with placeholder values {key: value} (none: the constants only)."""
+        subs = dict(substitutions or {})
+        keys, vals = _cstr_array(list(subs)), _cstr_array(list(subs.values()))
+        self._chk(self.lib.deme_jit_build(self.ctx, C.cast(keys, C.c_void_p) if keys else None,
+                                          C.cast(vals, C.c_void_p) if vals else None, len(subs)))
 
     def clear_contacts(self):
         """Forget the contact list and histories; the step count and time restart at zero."""

@@ -130,6 +130,33 @@ int deme_set_cell_scale(deme_ctx* ctx, double scale);
 int deme_autotune(deme_ctx* ctx, const int* freqs, int nFreqs, const double* cellScales, int nScales, int trialSteps,
                   float maxVel, float safety, int* bestFreq, double* msPerStep);
 
+/* ---- Force model and runtime specialisation (hipRTC in place of jitify) ----
+ * The force kernel is one template with the placeholders of the reference's JIT programs (DEMCalcForceKernels.cu:
+ * _DEMForceModel_, _forceModelIngredientDefinition_, _forceModelIngredientAcqForA_/_B_, _forceModelContactWildcardAcq_
+ * /_Write_/_Destroy_, _materialDefs_, _nvXp2_, _nvYp2_, _voxelSize_, _l_), substituted textually as JitHelper.cpp:44-48
+ * does. The library ships it compiled with the built-in frictional Hertz-Mindlin model.
+ *
+ * deme_set_force_model: a force-model body written against the names of the reference's model strings (force,
+ * torque_only_force, locCPA, locCPB, B2A, overlapDepth, AOwnerMass, BOwnerMass, ARadius, BRadius, AoriQ*, BoriQ*,
+ * bodyAMatType, bodyBMatType, the material arrays E, nu, CoR, mu, Crr, the ingredients ts, ALinVel, BLinVel,
+ * ARotVel, BRotVel, AOwnerFamily, BOwnerFamily, and its contact wildcards by name; DefineContactForceModel /
+ * SetPerContactWildcards, API.h:115-122, AuxClasses.h:123-160), compiled now by hipRTC with the simulation constants
+ * written in; DEME_ERR_JIT with the compiler log in deme_last_error on failure. At most 4 contact wildcards (kept as
+ * one float4 per contact, in sorted name order, zero on creation). model = NULL returns to the built-in kernel.
+ * deme_jit_build: the built-in model (or the model set last) with explicit placeholder values (keys / values, n
+ * pairs) overriding the generated ones; with n = 0 it specialises the built-in kernel on the simulation constants.
+ * deme_builtin_force_model: the source of built-in model 0 (frictional Hertz-Mindlin, wildcards delta_tan_x,
+ * delta_tan_y, delta_tan_z, delta_time) or 1 (frictionless Hertz, no wildcards), NULL otherwise. */
+int deme_set_force_model(deme_ctx* ctx, const char* model, const char* const* wildcards, int nWildcards);
+int deme_jit_build(deme_ctx* ctx, const char* const* keys, const char* const* values, int n);
+const char* deme_builtin_force_model(int which);
+/* Context-free helpers: the specialised source for a model (model NULL: built-in model 0; p NULL: constants read
+ * from the kernel parameters), written to out (cap bytes, NUL-terminated; *needed = full size + 1), and a hipRTC
+ * compile of a source for gfx950 (log written to log, cap bytes). Neither needs a device. */
+int deme_jit_render(const char* model, const char* const* wildcards, int nWildcards, const deme_sim_params* p,
+                    char* out, uint64_t cap, uint64_t* needed);
+int deme_jit_compile(const char* source, char* log, uint64_t cap);
+
 /* Adjoint of the step (no counterpart in the reference, which has no differentiation; for calibration callers that
  * fit material parameters or initial velocities to observed states). Systems of single-sphere owners (each sphere
  * its own, centred owner) with free families, fixed analytical walls, Crr = 0, the extended-Taylor integrator and at

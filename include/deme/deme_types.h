@@ -12,8 +12,18 @@
 #ifndef DEME_TYPES_H
 #define DEME_TYPES_H
 
+#ifdef __HIPCC_RTC__
+/* compiled by hipRTC (runtime specialisation of the force kernel): the fixed-width types come from its runtime */
+typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint16_t uint16_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::int64_t int64_t;
+#else
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {

@@ -1,0 +1,76 @@
+"""Force-model specialisation without a device: rendering of the force kernel template for models and constants,
+and hipRTC compilation of the results (hipRTC compiles for gfx950 on the host)."""
+import os
+
+import numpy as np
+import pytest
+
+from deme_amd import abi, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PLACEHOLDERS = ("_DEMForceModel_", "_forceModelIngredientDefinition_", "_forceModelIngredientAcqForA_",
+                "_forceModelIngredientAcqForB_", "_forceModelContactWildcardAcq_", "_forceModelContactWildcardWrite_",
+                "_forceModelContactWildcardDestroy_", "_materialDefs_", "_nvXp2_", "_nvYp2_", "_voxelSize_", "_l_")
+
+
+def test_shipped_kernel_is_the_template_with_the_builtin_model():
+    # the ahead-of-time kernel is generated from the same template and model the runtime path renders
+    gen = open(os.path.join(ROOT, "dem-engine-sbel_amd", "_build", "force_aot.hip")).read()
+    body = gen.split("\n", 1)[1]
+    assert body == abi.jit_render()
+
+
+@pytest.mark.parametrize("baked", [False, True])
+def test_rendered_sources_have_no_placeholders_left(baked):
+    sc = scenes.sphere_settling(n=50)
+    src = abi.jit_render(params=sc.params if baked else None)
+    for p in PLACEHOLDERS:
+        assert p not in src, p
+    if baked:
+        # the voxel split and units are written in as literals that parse back exactly
+        assert f"const uint32_t nvX = {sc.params.nvXp2}u, nvY = {sc.params.nvYp2}u;" in src
+        lit = src.split("subUnit = ")[1].split(";")[0]
+        assert float(lit) == sc.params.l
+    else:
+        assert "const uint32_t nvX = P.nvX, nvY = P.nvY;" in src
+
+
+def test_ingredients_follow_whole_word_use():
+    src = abi.jit_render("force = (float)(overlapDepth * ts) * B2A;", ())
+    assert "const float ts = P.h;" in src
+    assert "ALinVel = make_float3" not in src and "AOwnerFamily" not in src
+    src = abi.jit_render("float tsx = 1.f; force = tsx * B2A * (float)(AOwnerFamily + BOwnerFamily);", ())
+    assert "const float ts = P.h;" not in src
+    assert "AOwnerFamily = myOwnerRec.fam;" in src and "BOwnerFamily = myOwnerRec.fam;" in src
+
+
+def test_wildcards_map_to_history_components_in_sorted_order():
+    src = abi.jit_render("force = B2A * (float)overlapDepth; zeta += 1.f; alpha = 2.f;", ("zeta", "alpha"))
+    assert "float alpha = wc.x;" in src and "float zeta = wc.y;" in src
+    assert "wc = make_float4(alpha, zeta, 0.f, 0.f);" in src
+    with pytest.raises(abi.DemeError):
+        abi.jit_render("force = B2A;", ("a", "b", "c", "d", "e"))
+    with pytest.raises(abi.DemeError):
+        abi.jit_render("force = B2A;", ("two words",))
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_builtin_models_compile_with_baked_constants(which):
+    sc = scenes.clump_settling(n_clumps=1000)
+    wc = ("delta_tan_x", "delta_tan_y", "delta_tan_z", "delta_time") if which == 0 else ()
+    st, log = abi.jit_compile(abi.jit_render(abi.builtin_force_model(which), wc, sc.params))
+    assert st == 0, log
+
+
+def test_custom_model_compiles_and_errors_are_reported():
+    model = """
+    // linear spring-dashpot with a contact age wildcard
+    const float k = 2e4f, c = 0.5f;
+    const float vn = dot(ALinVel - BLinVel, B2A);
+    force = (float)(k * overlapDepth) * B2A - (c * vn) * B2A;
+    contact_age += ts;
+    """
+    st, log = abi.jit_compile(abi.jit_render(model, ("contact_age",)))
+    assert st == 0, log
+    st, log = abi.jit_compile(abi.jit_render("force = make_float3(overlapDepth, ;", ()))
+    assert st == 6 and "error" in log  # DEME_ERR_JIT, with the compiler's message
