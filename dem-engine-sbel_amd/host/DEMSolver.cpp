@@ -206,15 +206,36 @@ void DEMSolver::InstructBoxDomainNumVoxel(unsigned char x, unsigned char y, unsi
     explicit_nv_override = true;
 }
 
+void DEMForceModel::SetForceModelType(FORCE_MODEL t) {
+    type = t;
+    model.clear();
+    if (t == FORCE_MODEL::HERTZIAN)
+        contact_wildcards = {"delta_tan_x", "delta_tan_y", "delta_tan_z", "delta_time"};
+    else if (t == FORCE_MODEL::HERTZIAN_FRICTIONLESS)
+        contact_wildcards.clear();
+}
+void DEMForceModel::DefineCustomModel(const std::string& m) {
+    type = FORCE_MODEL::CUSTOM;
+    model = m;
+}
+void DEMForceModel::SetPerContactWildcards(const std::set<std::string>& names) {
+    for (const auto& n : names)
+        if (n.find(' ') != std::string::npos)
+            throw std::runtime_error("Contact wildcard " + n + " is not valid: no spaces allowed in its name.");
+    contact_wildcards = names;
+}
+
 std::shared_ptr<DEMForceModel> DEMSolver::UseFrictionalHertzianModel() {
-    m_force_model->type = FORCE_MODEL::HERTZIAN;
+    m_force_model->SetForceModelType(FORCE_MODEL::HERTZIAN);
     return m_force_model;
 }
 std::shared_ptr<DEMForceModel> DEMSolver::UseFrictionlessHertzianModel() {
-    throw std::runtime_error("The frictionless Hertzian model is not provided on this path (FORCE_MODEL::HERTZIAN only).");
+    m_force_model->SetForceModelType(FORCE_MODEL::HERTZIAN_FRICTIONLESS);
+    return m_force_model;
 }
-std::shared_ptr<DEMForceModel> DEMSolver::DefineContactForceModel(const std::string&) {
-    throw std::runtime_error("Custom force-model strings need runtime specialisation, which this path does not provide.");
+std::shared_ptr<DEMForceModel> DEMSolver::DefineContactForceModel(const std::string& model) {
+    m_force_model->DefineCustomModel(model);
+    return m_force_model;
 }
 
 void DEMSolver::SetFamilyFixed(unsigned int ID) {
@@ -391,7 +412,6 @@ void DEMSolver::Initialize() {
     if (m_user_boxSize.x <= 0.f || m_user_boxSize.y <= 0.f || m_user_boxSize.z <= 0.f)
         throw std::runtime_error("The size of the simulation world is set to be (or default to be) impossibly small.");
     if (!(m_ts_size > 0.0)) throw std::runtime_error("A positive time step size is required (SetInitTimeStep).");
-    if (m_force_model->type != FORCE_MODEL::HERTZIAN) throw std::runtime_error("Only the frictional Hertzian model is provided.");
     if (m_integrator == TIME_INTEGRATOR::CHUNG)
         throw std::runtime_error("The integration type is unknown or not implemented. Please select another via SetIntegrator.");
     SetFamilyFixed(RESERVED_FAMILY_NUM);
@@ -596,6 +616,16 @@ void DEMSolver::Initialize() {
     check(deme_upload_materials(m_ctx, (uint32_t)mats.size(), mats.data()), "deme_upload_materials");
     check(deme_upload_analytical(m_ctx, (uint32_t)anal.size(), anal.data()), "deme_upload_analytical");
     check(deme_upload_families(m_ctx, m_family_mask.data(), rules.data()), "deme_upload_families");
+    if (m_force_model->type != FORCE_MODEL::HERTZIAN) {
+        // other models are compiled now by hipRTC (the built-in frictional model ships precompiled)
+        const std::string body = m_force_model->type == FORCE_MODEL::HERTZIAN_FRICTIONLESS
+                                     ? std::string(deme_builtin_force_model(1))
+                                     : m_force_model->model;
+        std::vector<const char*> names;
+        for (const auto& n : m_force_model->contact_wildcards) names.push_back(n.c_str());
+        check(deme_set_force_model(m_ctx, body.c_str(), names.empty() ? nullptr : names.data(), (int)names.size()),
+              "deme_set_force_model");
+    }
     m_mirror_valid = true;
     m_mirror_dirty = true;
     pushHostState();

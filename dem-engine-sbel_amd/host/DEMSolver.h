@@ -9,6 +9,7 @@
 #include <limits>
 #include <memory>
 #include <stdexcept>
+#include <set>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -111,8 +112,15 @@ class DEMExternObj {
     void AddPlane(const float3 pos, const float3 normal, const std::shared_ptr<DEMMaterial>& material);
 };
 
+// Force model in use (AuxClasses.h:123-160): a built-in type, or a custom model body (the reference's model-string
+// names; compiled at Initialize by hipRTC) with the names of its per-contact wildcards.
 struct DEMForceModel {
     FORCE_MODEL type = FORCE_MODEL::HERTZIAN;
+    std::string model;
+    std::set<std::string> contact_wildcards = {"delta_tan_x", "delta_tan_y", "delta_tan_z", "delta_time"};
+    void SetForceModelType(FORCE_MODEL t);
+    void DefineCustomModel(const std::string& m);
+    void SetPerContactWildcards(const std::set<std::string>& names);
 };
 
 class DEMSolver {
