@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, cons
     scene_planes(P, o.scene, a0, a1);
     for (uint32_t a = a0; a < a1 && a < nA && a - a0 < 64; a++) {
         const PlanePose pp = pose[a];
-        if (mask[mask_index(o.fam, pp.fam)] != DEME_DONT_PREVENT_CONTACT) continue;
+        if (mask && mask[mask_index(o.fam, pp.fam)] != DEME_DONT_PREVENT_CONTACT) continue;
         if (plane_cd(P, pp, X, Y, Z, rCD)) {
             cnt++;
             bits |= 1ull << (a - a0);
@@ -234,6 +234,43 @@ __device__ __forceinline__ void rule_bins_exact(const KParams& P, const CdRec& A
     R.gb = bin_range(P, B.x, B.y, B.z, cd_radius(P, B.r));
 }
 
+// The same bin indices from multiplications by the reciprocal of the bin size, used when every floor input lies
+// farther from an integer than the product and the quotient can differ (a few ulps; the margin is 2^-40 relative):
+// then each floor, and so the rule, equals the division's. Returns false when a value is too close to call.
+__device__ __forceinline__ bool far_from_int(double t, double scale) {
+    const double f = t - floor(t);
+    const double eps = 0x1p-40 * (scale + 1.0);
+    return f > eps && f < 1.0 - eps;
+}
+
+__device__ __forceinline__ bool range_fast(const KParams& P, double inv, double X, double Y, double Z, double rCD,
+                                           BinRange& r) {
+    const double bx = X * inv, by = Y * inv, bz = Z * inv, span = rCD * inv;
+    const double ax = fabs(bx) + span, ay = fabs(by) + span, az = fabs(bz) + span;
+    const double x0 = bx - span, x1 = bx + span, y0 = by - span, y1 = by + span, z0 = bz - span, z1 = bz + span;
+    if (!(far_from_int(x0, ax) && far_from_int(x1, ax) && far_from_int(y0, ay) && far_from_int(y1, ay) &&
+          far_from_int(z0, az) && far_from_int(z1, az)))
+        return false;
+    r.x0 = sat_u32(x0);
+    r.x1 = sat_u32(x1);
+    r.y0 = sat_u32(y0);
+    r.y1 = sat_u32(y1);
+    r.z0 = sat_u32(z0);
+    r.z1 = sat_u32(z1);
+    return true;
+}
+
+__device__ __forceinline__ bool rule_bins_fast(const KParams& P, double inv, const CdRec& A, const CdRec& B, double cx,
+                                               double cy, double cz, RuleBins& R) {
+    const double qx = cx * inv, qy = cy * inv, qz = cz * inv;
+    if (!(far_from_int(qx, fabs(qx)) && far_from_int(qy, fabs(qy)) && far_from_int(qz, fabs(qz)))) return false;
+    R.ix = sat_u32(qx);
+    R.iy = sat_u32(qy);
+    R.iz = sat_u32(qz);
+    return range_fast(P, inv, A.x, A.y, A.z, cd_radius(P, A.r), R.ga) &&
+           range_fast(P, inv, B.x, B.y, B.z, cd_radius(P, B.r), R.gb);
+}
+
 __device__ __forceinline__ bool rule_member(const KParams& P, const RuleBins& R) {
     if (range_in_grid(P, R.ga) && range_in_grid(P, R.gb)) {
         // no aliasing: a linear id is one (i, j, k) -- unless the point itself lies past the grid, where the
@@ -247,12 +284,12 @@ __device__ __forceinline__ bool rule_member(const KParams& P, const RuleBins& R)
     return range_has_linear(P, R.ga, lin) && range_has_linear(P, R.gb, lin);
 }
 
-// A: the lower id of an overlapping pair.
-__device__ __forceinline__ bool bin_rule_point(const KParams& P, const CdRec& A, const CdRec& B) {
+// A: the lower id of an overlapping pair. inv: 1 / binSize.
+__device__ __forceinline__ bool bin_rule_point(const KParams& P, double inv, const CdRec& A, const CdRec& B) {
     double cx, cy, cz;
     contact_point(P, A, B, cx, cy, cz);
     RuleBins R;
-    rule_bins_exact(P, A, B, cx, cy, cz, R);
+    if (!rule_bins_fast(P, inv, A, B, cx, cy, cz, R)) rule_bins_exact(P, A, B, cx, cy, cz, R);
     return rule_member(P, R);
 }
 
@@ -285,14 +322,15 @@ __device__ __forceinline__ void append_pair(uint32_t* __restrict__ cnt, uint32_t
 }
 
 // Second phase for one overlapping pair of different owners (sorted positions i, j).
-__device__ __forceinline__ void finish_pair(const KParams& P, uint32_t nS, const CdRec* __restrict__ sorted,
+__device__ __forceinline__ void finish_pair(const KParams& P, double inv, uint32_t nS, const CdRec* __restrict__ sorted,
                                             const uint2* __restrict__ sortedMeta, const uint8_t* __restrict__ mask,
                                             uint32_t i, uint32_t j, uint32_t slabCap, uint32_t* rowSlab,
                                             uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt, DevCounters* ctr) {
-    if (mask[mask_index(sortedMeta[i].y, sortedMeta[j].y)] != DEME_DONT_PREVENT_CONTACT) return;
+    // (mask == nullptr: no family pair is masked)
+    if (mask && mask[mask_index(sortedMeta[i].y, sortedMeta[j].y)] != DEME_DONT_PREVENT_CONTACT) return;
     const CdRec a = load_cd(sorted, i), b = load_cd(sorted, j);
     const bool aLow = a.id < b.id;
-    if (!(aLow ? bin_rule_point(P, a, b) : bin_rule_point(P, b, a))) return;
+    if (!(aLow ? bin_rule_point(P, inv, a, b) : bin_rule_point(P, inv, b, a))) return;
     const uint32_t loIdx = aLow ? i : j, hiIdx = aLow ? j : i;
     const uint32_t loId = aLow ? a.id : b.id, hiId = aLow ? b.id : a.id;
     append_pair(aCnt, rowSlab, loIdx, hiId, nS, slabCap, ctr);
@@ -342,10 +380,10 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
                 b = cellStart[c0];
                 e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
             }
-            for (uint32_t j = b; j < e; j++) {
-                const CdRec ot = load_cd(sorted, j);
-                if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) continue;
-                if (sortedMeta[j].x == myOwner) continue;  // same clump: never a contact
+            // two candidates per iteration: both loads are in flight before either test
+            auto test = [&](const CdRec& ot, uint32_t j) {
+                if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) return;
+                if (sortedMeta[j].x == myOwner) return;  // same clump: never a contact
                 const uint32_t q = atomicAdd(&qLen, 1u);
                 if (q < PAIR_QUEUE) {
                     qI[q] = i;
@@ -354,20 +392,28 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
                     if (nc < slabCap) col[(uint64_t)nc * nS] = j;
                     nc++;
                 }
+            };
+            uint32_t j = b;
+            for (; j + 1 < e; j += 2) {
+                const CdRec o0 = load_cd(sorted, j), o1 = load_cd(sorted, j + 1);
+                test(o0, j);
+                test(o1, j + 1);
             }
+            if (j < e) test(load_cd(sorted, j), j);
         }
     }
     __syncthreads();
+    const double inv = 1.0 / P.binSize;
     const uint32_t nq = min(qLen, PAIR_QUEUE);
     for (uint32_t q = threadIdx.x; q < nq; q += PAIR_BLOCK)
-        finish_pair(P, nS, sorted, sortedMeta, mask, qI[q], qJ[q], slabCap, rowSlab, mirSlab, aCnt, bCnt, ctr);
+        finish_pair(P, inv, nS, sorted, sortedMeta, mask, qI[q], qJ[q], slabCap, rowSlab, mirSlab, aCnt, bCnt, ctr);
     if (nc > slabCap) {
         atomicMax(&ctr->maxOcc, nc);
         return;
     }
     for (uint32_t k = 0; k < nc; k++)
-        finish_pair(P, nS, sorted, sortedMeta, mask, i, surv[(uint64_t)k * nS + i], slabCap, rowSlab, mirSlab, aCnt,
-                    bCnt, ctr);
+        finish_pair(P, inv, nS, sorted, sortedMeta, mask, i, surv[(uint64_t)k * nS + i], slabCap, rowSlab, mirSlab,
+                    aCnt, bCnt, ctr);
 }
 
 // Canonical rows: plane contacts (ascending plane index) then partners (ascending id); idA for the row. Mirror rows:

@@ -62,6 +62,7 @@ struct deme_ctx {
     deme_sim_params sp{};
     KParams kp{};
     bool haveParams = false, haveTemplates = false, haveState = false, haveMaterials = false, haveFamilies = false;
+    bool maskAny = true;  // some family pair is masked off
     // force kernel specialised at run time (hipRTC); inactive: the ahead-of-time kernel of the built-in model
     JitForce jit;
     bool jitActive = false, jitStale = false, jitCustom = false;
@@ -567,6 +568,8 @@ int deme_upload_families(deme_ctx* c, const uint8_t* mask, const deme_family_rul
         DEME_HIP(c, hipMemcpyAsync(c->rules.p, rules, DEME_NUM_FAMILIES * sizeof(deme_family_rule),
                                    hipMemcpyHostToDevice, c->stream));
     DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->maskAny = false;  // an all-allow mask lets the pair pass skip the family lookups
+    for (uint32_t k = 0; k < DEME_NUM_MASK_ENTRIES; k++) c->maskAny |= mask[k] != DEME_DONT_PREVENT_CONTACT;
     c->haveFamilies = true;
     return DEME_OK;
 }
@@ -861,7 +864,8 @@ int deme_cd_run(deme_ctx* c) {
     DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
     DEME_LAUNCH(c, "fill", launch_fill_u32(c->cellCount.p, c->nCells, 0, c->stream));
     DEME_LAUNCH(c, "cell_assign",
-                launch_cell_assign(c->kp, c->grid, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->mask.p,
+                launch_cell_assign(c->kp, c->grid, c->sph.p, c->own.p, nS, c->pose.p, c->nA,
+                                   c->maskAny ? c->mask.p : nullptr,
                                    c->cellCount.p, c->cellOf.p, c->slotOf.p, c->numBins.p, c->numAnal.p,
                                    c->planeBits.p, c->rec.p, c->meta.p, c->ctr.p, c->stream));
     DEME_LAUNCH(c, "scan_cells",
@@ -881,7 +885,8 @@ int deme_cd_run(deme_ctx* c) {
     const int cu = c->cur, pv = cu ^ 1;
     for (int attempt = 0;; attempt++) {
         DEME_LAUNCH(c, "cell_pairs",
-                    launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, c->sortedMeta.p, c->cellStart.p, c->mask.p,
+                    launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, c->sortedMeta.p, c->cellStart.p,
+                                      c->maskAny ? c->mask.p : nullptr,
                                       c->slabCap, c->survSlab.p, c->rowSlab.p, c->mirSlab.p, c->aCnt.p, c->bCnt.p,
                                       c->ctr.p, c->stream));
         // row lengths in id order: partner counts gathered from sorted positions, plus plane contacts; mirror rows

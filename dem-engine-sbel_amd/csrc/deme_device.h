@@ -86,7 +86,11 @@ struct KParams {
 
 // Range of analytical components a body of the given scene can touch.
 __device__ __forceinline__ void scene_planes(const KParams& P, uint32_t scene, uint32_t& a0, uint32_t& a1) {
-    if (scene < P.nScenes) {
+    if (P.nScenes == 1) {
+        // one scene: every component (no table lookup on the dependent-load chain)
+        a0 = 0;
+        a1 = scene == 0 ? P.nA : 0;
+    } else if (scene < P.nScenes) {
         a0 = P.planeStart[scene];
         a1 = P.planeStart[scene + 1];
     } else {
