@@ -308,6 +308,7 @@ __device__ __forceinline__ bool bin_rule_point(const KParams& P, double inv, con
 // A column that overflows a slab is reported through maxOcc and the pass is redone with wider columns.
 constexpr uint32_t PAIR_BLOCK = 128;
 constexpr uint32_t PAIR_QUEUE = 1024;
+static_assert(PAIR_BLOCK <= 128 && PAIR_QUEUE < (1u << 24), "queue codes hold a 7-bit lane and a 24-bit slot");
 constexpr int PAIR_SEGS = 5;
 
 // segment k of the half shell: (dy, dz) = (0, 0), (1, 0), (-1, 1), (0, 1), (1, 1)
@@ -319,6 +320,18 @@ __device__ __forceinline__ void append_pair(uint32_t* __restrict__ cnt, uint32_t
     const uint32_t slot = atomicAdd(&cnt[idx], 1u);
     if (slot < slabCap) slab[(uint64_t)slot * nS + idx] = val;
     else atomicMax(&ctr->maxOcc, slot + 1);
+}
+
+// Second phase, test half: the family mask and the bin rule of an overlapping pair of different owners (sorted
+// positions i, j); on acceptance the ids of i and j.
+__device__ __forceinline__ bool accept_pair(const KParams& P, double inv, const CdRec* __restrict__ sorted,
+                                            const uint2* __restrict__ sortedMeta, const uint8_t* __restrict__ mask,
+                                            uint32_t i, uint32_t j, uint32_t& idI, uint32_t& idJ) {
+    if (mask && mask[mask_index(sortedMeta[i].y, sortedMeta[j].y)] != DEME_DONT_PREVENT_CONTACT) return false;
+    const CdRec a = load_cd(sorted, i), b = load_cd(sorted, j);
+    idI = a.id;
+    idJ = b.id;
+    return a.id < b.id ? bin_rule_point(P, inv, a, b) : bin_rule_point(P, inv, b, a);
 }
 
 // Second phase for one overlapping pair of different owners (sorted positions i, j).
@@ -347,6 +360,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
                                                            uint32_t* bCnt, DevCounters* ctr) {
     __shared__ uint32_t qI[PAIR_QUEUE], qJ[PAIR_QUEUE];
     __shared__ uint32_t qLen;
+    __shared__ uint32_t selfCnt[2][PAIR_BLOCK], selfBase[2][PAIR_BLOCK];
     // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so XCD k gets a contiguous share of the
     // sphere range and the rows its blocks share stay in its L2
     const uint32_t per = (nBlocks + 7) / 8;
@@ -405,8 +419,44 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
     __syncthreads();
     const double inv = 1.0 / P.binSize;
     const uint32_t nq = min(qLen, PAIR_QUEUE);
-    for (uint32_t q = threadIdx.x; q < nq; q += PAIR_BLOCK)
-        finish_pair(P, inv, nS, sorted, sortedMeta, mask, qI[q], qJ[q], slabCap, rowSlab, mirSlab, aCnt, bCnt, ctr);
+    const uint32_t blockBase = lb * PAIR_BLOCK;
+    selfCnt[0][threadIdx.x] = selfCnt[1][threadIdx.x] = 0;
+    __syncthreads();
+    // Every accepted pair makes one entry in each of the two spheres' columns. The partner's entry is appended at
+    // once (global atomic); the finder's entry -- always a sphere of this block -- is counted in LDS, so each of the
+    // block's spheres takes all its own entries with a single global atomic and they are written together.
+    constexpr uint32_t REJECTED = 0xFFFFFFFFu;
+    for (uint32_t q = threadIdx.x; q < nq; q += PAIR_BLOCK) {
+        const uint32_t si = qI[q], sj = qJ[q];
+        uint32_t idI, idJ;
+        if (!accept_pair(P, inv, sorted, sortedMeta, mask, si, sj, idI, idJ)) {
+            qI[q] = REJECTED;
+            continue;
+        }
+        // i lower id: i's contact row gets j and j's mirror row gets i; else the other way round
+        const uint32_t kind = idI < idJ ? 0u : 1u;
+        if (kind == 0) append_pair(bCnt, mirSlab, sj, idI, nS, slabCap, ctr);
+        else append_pair(aCnt, rowSlab, sj, idI, nS, slabCap, ctr);
+        const uint32_t local = si - blockBase;
+        const uint32_t ls = atomicAdd(&selfCnt[kind][local], 1u);
+        qI[q] = (kind << 31) | (local << 24) | ls;
+        qJ[q] = idJ;
+    }
+    __syncthreads();
+    {
+        const uint32_t t = threadIdx.x;
+        if (selfCnt[0][t]) selfBase[0][t] = atomicAdd(&aCnt[blockBase + t], selfCnt[0][t]);
+        if (selfCnt[1][t]) selfBase[1][t] = atomicAdd(&bCnt[blockBase + t], selfCnt[1][t]);
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < nq; q += PAIR_BLOCK) {
+        const uint32_t code = qI[q];
+        if (code == REJECTED) continue;
+        const uint32_t kind = code >> 31, local = (code >> 24) & 0x7Fu;
+        const uint32_t slot = selfBase[kind][local] + (code & 0xFFFFFFu);
+        if (slot < slabCap) (kind == 0 ? rowSlab : mirSlab)[(uint64_t)slot * nS + blockBase + local] = qJ[q];
+        else atomicMax(&ctr->maxOcc, slot + 1);
+    }
     if (nc > slabCap) {
         atomicMax(&ctr->maxOcc, nc);
         return;
