@@ -103,8 +103,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_force(
     const uint32_t* __restrict__ mirror, uint32_t nO, const float* __restrict__ lAcc,
     const float4* __restrict__ lDtOut, float* __restrict__ adjA, float* __restrict__ adjB, float4* __restrict__ lDtIn,
     float* __restrict__ matPart, int accumulate) {
-    __shared__ float gsh[ADJ_BLOCK][8];
-    __shared__ uint32_t msh[ADJ_BLOCK][2];
+    __shared__ float wsum[ADJ_BLOCK / 64][256];
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t mA = 0, mB = 0;
@@ -223,19 +222,24 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_force(
             g[7] = adj.gB.mu;
         }
     }
-    for (int k = 0; k < 8; k++) gsh[threadIdx.x][k] = g[k];
-    msh[threadIdx.x][0] = mA;
-    msh[threadIdx.x][1] = mB;
-    __syncthreads();
-    // per-block material sums in lane order (A side then B side of each lane), one thread per (material, property);
-    // a block keeps its slot over the steps of a window, so the slots accumulate in step order
-    if (threadIdx.x < nMat * 4) {
-        const uint32_t m = threadIdx.x / 4, q = threadIdx.x % 4;
-        float s = 0.f;
-        for (uint32_t l = 0; l < ADJ_BLOCK; l++) {
-            if (msh[l][0] == m) s += gsh[l][q];
-            if (msh[l][1] == m) s += gsh[l][4 + q];
+    // per-block material sums: for each (material, property) a butterfly sum over each wave's lanes (each lane's A
+    // side plus its B side), then the block's waves in order -- a fixed order, so the result is deterministic. A block
+    // keeps its slot over the steps of a window, so the slots accumulate in step order.
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t m = 0; m < nMat; m++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float v = (mA == m ? g[q] : 0.f) + (mB == m ? g[4 + q] : 0.f);
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+            if (lane == 0) wsum[wave][m * 4 + q] = v;
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < nMat * 4) {
+        float s = 0.f;
+#pragma unroll
+        for (uint32_t w = 0; w < ADJ_BLOCK / 64; w++) s += wsum[w][threadIdx.x];
         float* slot = matPart + (uint64_t)blockIdx.x * nMat * 4 + threadIdx.x;
         *slot = accumulate ? *slot + s : s;
     }

@@ -45,8 +45,10 @@ __device__ __forceinline__ float length(f3 v) { return sqrtf(dot(v, v)); }
 __device__ __forceinline__ void rotq(f3& v, float w, float x, float y, float z) { rotate(v.x, v.y, v.z, w, x, y, z); }
 
 // Row a16 for analytical owners (planes): every sphere-plane contact contributes its B-side term (stored by the
-// force kernel) to the plane's owner. Deterministic two-level reduction: each lane sums 16 consecutive contacts in
-// order, lanes combine in a fixed tree, tiles combine in a fixed tree in k_anal_final.
+// force kernel) to the plane's owner. Deterministic two-level reduction: lane t of a tile sums contacts t, t + 256, ...
+// (16 of them) in order, lanes combine in a fixed tree, tiles combine in a fixed tree in k_anal_final. The contact
+// types of a lane's 16 contacts are loaded together (coalesced across the wave), so the gathers of the few plane
+// contacts are the only dependent loads.
 constexpr uint32_t ANAL_ITEMS = 16;
 constexpr uint32_t ANAL_TILE = 256 * ANAL_ITEMS;
 
@@ -62,12 +64,18 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
     for (uint32_t k = 0; k < MAX_ANAL_OWNERS; k++)
 #pragma unroll
         for (int c = 0; c < 6; c++) acc[k][c] = 0.f;
-    const uint64_t c0 = (uint64_t)blockIdx.x * ANAL_TILE + (uint64_t)threadIdx.x * ANAL_ITEMS;
+    const uint64_t c0 = (uint64_t)blockIdx.x * ANAL_TILE + threadIdx.x;
     bool any = false;
+    uint8_t ty[ANAL_ITEMS];
+#pragma unroll
     for (uint32_t it = 0; it < ANAL_ITEMS; it++) {
-        const uint64_t c = c0 + it;
-        if (c >= nC) break;
-        if (ctype[c] != DEME_SPHERE_PLANE_CONTACT) continue;
+        const uint64_t c = c0 + (uint64_t)it * 256;
+        ty[it] = c < nC ? ctype[c] : (uint8_t)DEME_NOT_A_CONTACT;
+    }
+#pragma unroll
+    for (uint32_t it = 0; it < ANAL_ITEMS; it++) {
+        const uint64_t c = c0 + (uint64_t)it * 256;
+        if (ty[it] != DEME_SPHERE_PLANE_CONTACT) continue;
         const uint32_t b = idB[c];
         if (b >= nA) continue;
         const uint32_t k = anal[b].owner - firstAnal;
