@@ -123,6 +123,15 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta):
     dev.sync()
     t_batch = time.perf_counter() - t0
     contacts = dev.counts().nContacts
+    # the same steps with the non-detection steps replayed from captured HIP graphs
+    dev.set_step_graphs(True)
+    dev.step(warmup, update_freq)
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.step(steps, update_freq)
+    dev.sync()
+    t_graph = time.perf_counter() - t0
+    dev.set_step_graphs(False)
     dev.close()
     parts = [DeviceSystem(p) for p in scenes.split_batch(sc, own_world=True)]
     for d in parts:
@@ -143,6 +152,7 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta):
     return {"scenes": n_scenes, "spheres": sc.n_spheres, "contacts": contacts, "update_freq": update_freq,
             "beta_m": beta, "cell_scale": scale, "autotune_ms_per_step": table.round(4).tolist(), "steps": steps,
             "scene_steps_per_s": n_scenes * steps / t_batch, "ms_per_step": 1e3 * t_batch / steps,
+            "graph_replay_ms_per_step": 1e3 * t_graph / steps,
             "loop_scene_steps_per_s": n_scenes * steps / t_loop, "loop_ms_per_step": 1e3 * t_loop / steps,
             "speedup_vs_loop": t_loop / t_batch}
 

@@ -63,6 +63,12 @@ struct deme_ctx {
     KParams kp{};
     bool haveParams = false, haveTemplates = false, haveState = false, haveMaterials = false, haveFamilies = false;
     bool maskAny = true;  // some family pair is masked off
+    // the non-detection step replayed from a captured HIP graph (deme_set_step_graphs); any change of buffers,
+    // kernel parameters or kernels bumps graphGen and the next such step captures again
+    bool useGraphs = false;
+    uint64_t graphGen = 1, capturedGen = 0;
+    hipGraphExec_t stepExec = nullptr;
+    KParams capturedKp{};
     // force kernel specialised at run time (hipRTC); inactive: the ahead-of-time kernel of the built-in model
     JitForce jit;
     bool jitActive = false, jitStale = false, jitCustom = false;
@@ -368,6 +374,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->lam.release(); c->lAcc.release(); c->adjA.release(); c->adjB.release(); c->matPart.release();
     c->matGrad.release(); c->lDt[0].release(); c->lDt[1].release(); c->lDtPrev.release();
     c->jit.release();
+    if (c->stepExec) (void)hipGraphExecDestroy(c->stepExec);
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -387,6 +394,7 @@ int deme_sync(deme_ctx* c) {
 }
 
 int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
+    c->graphGen++;
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (p->nvXp2 + p->nvYp2 + p->nvZp2 != 64) return c->fail(DEME_ERR_INVALID_ARGUMENT, "nvXp2 + nvYp2 + nvZp2 != 64");
     if (!(p->binSize > 0.0)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "bin size must be positive");
@@ -423,6 +431,7 @@ int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
 int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, const float* sphRadius,
                           const float* sphRelPos, const uint8_t* sphMat, uint32_t nO, uint32_t nClumps,
                           const float* ownerMass, const float* ownerMoi) {
+    c->graphGen++;
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (nClumps > nO) return c->fail(DEME_ERR_INVALID_ARGUMENT, "nClumps > nOwners");
     if (nO - nClumps > MAX_ANAL_OWNERS)
@@ -497,6 +506,7 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
 }
 
 int deme_upload_materials(deme_ctx* c, uint32_t n, const deme_material* m) {
+    c->graphGen++;
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (n == 0 || n > 256) return c->fail(DEME_ERR_INVALID_ARGUMENT, "1..256 materials required");
     DEME_HIP(c, c->mats.ensure(n, 1.0));
@@ -509,6 +519,7 @@ int deme_upload_materials(deme_ctx* c, uint32_t n, const deme_material* m) {
 }
 
 int deme_upload_analytical(deme_ctx* c, uint32_t n, const deme_anal_obj* objs) {
+    c->graphGen++;
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before analytical objects");
     if (n > 65535) return c->fail(DEME_ERR_UNSUPPORTED, "more than 65535 analytical components");
@@ -534,6 +545,7 @@ int deme_upload_analytical(deme_ctx* c, uint32_t n, const deme_anal_obj* objs) {
 }
 
 int deme_upload_scenes(deme_ctx* c, uint32_t nScenes, const uint32_t* ownerScene, const uint32_t* analScene) {
+    c->graphGen++;
     if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before the scenes");
     if (nScenes == 0 || !ownerScene) return c->fail(DEME_ERR_INVALID_ARGUMENT, "need at least one scene");
     for (uint32_t o = 0; o < c->nO; o++)
@@ -559,6 +571,7 @@ int deme_upload_scenes(deme_ctx* c, uint32_t nScenes, const uint32_t* ownerScene
 }
 
 int deme_upload_families(deme_ctx* c, const uint8_t* mask, const deme_family_rule* rules) {
+    c->graphGen++;
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     DEME_HIP(c, c->mask.ensure(DEME_NUM_MASK_ENTRIES, 1.0));
     DEME_HIP(c, c->rules.ensure(DEME_NUM_FAMILIES, 1.0));
@@ -620,6 +633,7 @@ int deme_upload_state(deme_ctx* c, const deme_owner_state* s) {
 }
 
 int deme_clear_contacts(deme_ctx* c) {
+    c->graphGen++;
     c->adjReady = false;
     c->havePrev = c->contactsReady = c->fresh = false;
     c->termsReady = c->forcesFull = false;
@@ -844,6 +858,7 @@ static int setup_grid(deme_ctx* c) {
 }
 
 int deme_cd_run(deme_ctx* c) {
+    c->graphGen++;
     int r = check_ready(c);
     if (r) return r;
     r = setup_grid(c);
@@ -1073,27 +1088,77 @@ int deme_integrate(deme_ctx* c) {
     return collect_and_update(c, 1);
 }
 
+// One non-detection step from the captured graph, capturing it first when anything it depends on changed.
+static int step_graph(deme_ctx* c) {
+    const bool valid = c->stepExec && c->capturedGen == c->graphGen &&
+                       std::memcmp(&c->capturedKp, &c->kp, sizeof(KParams)) == 0;
+    if (!valid) {
+        hipGraph_t g = nullptr;
+        DEME_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int r = force_pass(c, false);
+        if (!r) r = collect_and_update(c, 1);
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r) {
+            if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        DEME_HIP(c, e);
+        if (c->stepExec) {
+            hipGraphNode_t errNode = nullptr;
+            hipGraphExecUpdateResult res;
+            if (hipGraphExecUpdate(c->stepExec, g, &errNode, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(c->stepExec);
+                c->stepExec = nullptr;
+            }
+        }
+        hipError_t ie = hipSuccess;
+        if (!c->stepExec) ie = hipGraphInstantiate(&c->stepExec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        DEME_HIP(c, ie);
+        c->capturedGen = c->graphGen;
+        c->capturedKp = c->kp;
+    }
+    DEME_HIP(c, hipGraphLaunch(c->stepExec, c->stream));
+    return DEME_OK;
+}
+
 int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     int r = check_ready(c);
     if (r) return r;
     const uint64_t f = updateFreq > 0 ? (uint64_t)updateFreq : 1;
     for (int k = 0; k < nSteps; k++) {
-        if (c->nSteps % f == 0 || !c->contactsReady) {
+        const bool detect = c->nSteps % f == 0 || !c->contactsReady;
+        if (detect) {
             r = deme_cd_run(c);
             if (r) return r;
         }
-        r = force_pass(c, false);
-        if (r) return r;
-        r = collect_and_update(c, 1);
-        if (r) return r;
+        // Between detection passes every step launches the same kernels with the same arguments (the history is
+        // updated in place; the time argument is not read by the integration), so it is captured once per pass and
+        // replayed. The step right after a detection pass (history gathered through the map) runs directly.
+        if (!detect && !c->fresh && c->useGraphs && !c->timing && !c->jitStale && c->stream) {
+            if ((r = step_graph(c))) return r;
+        } else {
+            r = force_pass(c, false);
+            if (r) return r;
+            r = collect_and_update(c, 1);
+            if (r) return r;
+        }
         c->time += (double)c->sp.h;
         c->nSteps++;
     }
     return DEME_OK;
 }
 
+int deme_set_step_graphs(deme_ctx* c, int on) {
+    c->useGraphs = on != 0;
+    c->graphGen++;
+    return DEME_OK;
+}
+
 int deme_set_cell_scale(deme_ctx* c, double scale) {
+    c->graphGen++;
     if (!(scale >= 1.0) || scale > 8.0) return c->fail(DEME_ERR_INVALID_ARGUMENT, "cell scale must lie in [1, 8]");
     c->cellScale = scale;
     c->nCells = 0;  // re-derive the grid at the next detection pass
@@ -1101,6 +1166,7 @@ int deme_set_cell_scale(deme_ctx* c, double scale) {
 }
 
 int deme_set_beta(deme_ctx* c, float beta) {
+    c->graphGen++;
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (!(beta >= 0.f) || !std::isfinite(beta)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "beta must be >= 0");
     c->sp.beta = beta;
@@ -1213,6 +1279,7 @@ static int adj_save(deme_ctx* c, deme_ctx::AdjCk& k, bool full) {
 }
 
 static int adj_restore(deme_ctx* c, const deme_ctx::AdjCk& k) {
+    c->graphGen++;
     DEME_HIP(c, hipMemcpyAsync(c->own.p, k.own.p, c->nO * sizeof(OwnerRec), hipMemcpyDeviceToDevice, c->stream));
     const int cu = c->cur;
     if (k.haveList) {
@@ -1424,6 +1491,7 @@ const char* deme_builtin_force_model(int which) {
 }
 
 int deme_set_force_model(deme_ctx* c, const char* model, const char* const* wildcards, int nWildcards) {
+    c->graphGen++;
     c->adjReady = false;
     if (!model) {
         c->jit.release();
@@ -1448,6 +1516,7 @@ int deme_set_force_model(deme_ctx* c, const char* model, const char* const* wild
 }
 
 int deme_jit_build(deme_ctx* c, const char* const* keys, const char* const* values, int n) {
+    c->graphGen++;
     c->adjReady = false;
     if (n < 0 || (n > 0 && (!keys || !values))) return c->fail(DEME_ERR_INVALID_ARGUMENT, "keys/values missing");
     std::vector<std::pair<std::string, std::string>> ov;
@@ -1518,6 +1587,7 @@ int deme_download_wildcards(deme_ctx* c, float* w) {
 }
 
 int deme_upload_wildcards(deme_ctx* c, const float* w) {
+    c->graphGen++;
     if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "no contact list");
     const size_t C = c->C;
     std::vector<float4> h(C);

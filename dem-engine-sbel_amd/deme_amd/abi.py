@@ -81,7 +81,7 @@ EXPORTED = [
     "deme_get_counts", "deme_set_time", "deme_enable_kernel_timing", "deme_reset_kernel_timing",
     "deme_kernel_timing", "deme_set_beta", "deme_set_cell_scale", "deme_autotune", "deme_adjoint_forward",
     "deme_adjoint_backward", "deme_clear_contacts", "deme_set_force_model", "deme_jit_build",
-    "deme_builtin_force_model", "deme_jit_render", "deme_jit_compile",
+    "deme_builtin_force_model", "deme_jit_render", "deme_jit_compile", "deme_set_step_graphs",
 ]
 
 _lib = None
@@ -139,6 +139,7 @@ def load(path: str = LIB_PATH):
         "deme_autotune": ([vp, vp, i32, vp, i32, i32, f32, f32, C.POINTER(i32), vp], i32),
         "deme_adjoint_forward": ([vp, i32, i32], i32),
         "deme_clear_contacts": ([vp], i32),
+        "deme_set_step_graphs": ([vp, i32], i32),
         "deme_set_force_model": ([vp, C.c_char_p, vp, i32], i32),
         "deme_jit_build": ([vp, vp, vp, i32], i32),
         "deme_builtin_force_model": ([i32], C.c_char_p),
@@ -313,6 +314,10 @@ class DeviceSystem:
         keys, vals = _cstr_array(list(subs)), _cstr_array(list(subs.values()))
         self._chk(self.lib.deme_jit_build(self.ctx, C.cast(keys, C.c_void_p) if keys else None,
                                           C.cast(vals, C.c_void_p) if vals else None, len(subs)))
+
+    def set_step_graphs(self, on=True):
+        """Replay the steps between detection passes from a captured HIP graph (default) or launch them directly."""
+        self._chk(self.lib.deme_set_step_graphs(self.ctx, 1 if on else 0))
 
     def clear_contacts(self):
         """Forget the contact list and histories; the step count and time restart at zero."""

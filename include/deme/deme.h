@@ -116,6 +116,11 @@ int deme_integrate(deme_ctx* ctx);
 /* The step loop (DEMDynamicThread::workerThread, dT.cpp:1527-1585) with a deterministic contact-update schedule:
  * detection on the state at steps k*f (f = max(update_freq, 1)), force -> collect -> integrate each step. */
 int deme_step(deme_ctx* ctx, int n_steps, int update_freq);
+/* Between detection passes deme_step replays each step (plane poses, force, collection, integration) from a HIP graph
+ * captured once per pass (no counterpart in the reference, which launches every kernel). Off by default (direct
+ * launches): the capture per detection pass costs more than the replays save unless passes are far apart. Results
+ * are identical either way. Kernel timing (deme_enable_kernel_timing) runs directly. */
+int deme_set_step_graphs(deme_ctx* ctx, int on);
 
 /* Drift margin beta (SetExpandFactor, API.h:100-104): detection uses r + beta. Takes effect at the next detection. */
 int deme_set_beta(deme_ctx* ctx, float beta);
