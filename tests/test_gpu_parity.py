@@ -43,7 +43,16 @@ def polydisperse():
     return scenes.polydisperse_bed(n=3000, velocity_sigma=0.05)
 
 
-SCENES = [packed_spheres, packed_clumps, loose_box, expanded_clumps, crowded_cells, polydisperse]
+def masked_families():
+    # DisableContactBetweenFamilies(1, 2) on a packed bed of spheres split at random between families 1 and 2
+    sc = scenes.sphere_settling(n=3000, spacing=1.98, velocity_sigma=0.05, omega_sigma=5.0, seed=1010)
+    rng = np.random.default_rng(5)
+    sc.state["family"][:sc.n_clumps] = rng.integers(1, 3, sc.n_clumps)
+    sc.mask[(1 + 2) * 2 // 2 + 1] = 1  # locateMaskPair(1, 2)
+    return sc
+
+
+SCENES = [packed_spheres, packed_clumps, loose_box, expanded_clumps, crowded_cells, polydisperse, masked_families]
 
 
 def systems(make, pre_steps=0):
@@ -252,3 +261,44 @@ def test_prescribed_plate_moves_at_its_velocity():
     dz = float(p1[2]) - float(p0[2])
     assert abs(dz - (-0.05 * n * float(np.float32(sc.params.h)))) < 1e-7
     assert abs(float(p1[0]) - float(p0[0])) < 1e-9
+
+
+def test_masked_family_pairs_never_touch():
+    sc, dev, ora = systems(masked_families, pre_steps=5)
+    dev.cd_run()
+    a, b, t, _ = dev.contacts()
+    fam = sc.state["family"]
+    sph = t == 1
+    fa, fb = fam[a[sph]], fam[b[sph]]
+    assert np.all(fa == fb) and sph.sum() > 500  # only same-family sphere pairs remain
+    ora.cd_run()
+    oa, ob, ot, _ = ora.contacts()
+    assert np.array_equal(a, oa) and np.array_equal(b, ob) and np.array_equal(t, ot)
+
+
+def test_empty_contact_list_steps_and_differentiates():
+    import pyoracle
+
+    # 27 spheres far apart, no walls: every step runs with zero contacts
+    g = np.arange(3) * 0.05 - 0.05
+    xyz = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(2)
+    sc = scenes.spheres_scene(xyz, 0.003, (0.4, 0.4, 0.4), planes=False, vel=rng.normal(0, 0.1, (27, 3)), h=1e-5)
+    dev = DeviceSystem(sc)
+    ora = pyoracle.OracleSystem(sc)
+    dev.step(12, 1)
+    ora.step(12, 1)
+    assert dev.counts().nContacts == 0
+    dev.force()
+    F, T, _, _ = dev.forces()
+    assert F.shape == (0, 3) and T.shape == (0, 3)
+    sd, so = dev.get_state(), ora.get_state()
+    for k in ("voxelID", "locX", "locY", "locZ", "vX", "vY", "vZ"):
+        assert np.array_equal(sd[k], so[k]), k
+    # adjoint of free flight: dX_T/dv_0 = T h
+    dev.clear_contacts()
+    dev.adjoint_forward(10, 5)
+    w = rng.normal(size=(27, 3))
+    gV, gM = dev.adjoint_backward(w)
+    assert np.allclose(gV[:27], w * 10 * np.float32(sc.params.h), rtol=1e-5, atol=0)
+    assert np.all(gM == 0.0)
