@@ -72,6 +72,10 @@ struct deme_ctx {
     // force kernel specialised at run time (hipRTC); inactive: the ahead-of-time kernel of the built-in model
     JitForce jit;
     bool jitActive = false, jitStale = false, jitCustom = false;
+    // owner update specialised with motion strategies / family change rules (inactive: the ahead-of-time kernel)
+    JitOwner jitOwner;
+    bool ownerJitActive = false;
+    std::vector<std::pair<std::string, std::string>> ownerValues;
     std::string jitModel;
     std::vector<std::string> jitWildcards;
     std::vector<std::pair<std::string, std::string>> jitOverrides;
@@ -374,6 +378,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->lam.release(); c->lAcc.release(); c->adjA.release(); c->adjB.release(); c->matPart.release();
     c->matGrad.release(); c->lDt[0].release(); c->lDt[1].release(); c->lDtPrev.release();
     c->jit.release();
+    c->jitOwner.release();
     if (c->stepExec) (void)hipGraphExecDestroy(c->stepExec);
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
@@ -1068,9 +1073,14 @@ static int collect_and_update(deme_ctx* c, int integrate) {
                                     c->nClumps, c->nAnalOwners, c->analPartials.p, &nb, c->stream));
     DEME_LAUNCH(c, "anal_final", launch_anal_final(c->analPartials.p, nb, c->nAnalOwners, c->analAcc.p, c->stream));
     DEME_LAUNCH(c, integrate ? "owner_update" : "owner_reduce",
-                launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
-                                    c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, c->analAcc.p,
-                                    c->rules.p, c->own.p, c->acc.p, integrate, (float)c->time, c->stream));
+                c->ownerJitActive
+                    ? c->jitOwner.launch(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
+                                         c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, c->analAcc.p,
+                                         c->rules.p, c->own.p, c->own.p, c->acc.p, integrate, (float)c->time,
+                                         c->stream)
+                    : launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
+                                          c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, c->analAcc.p,
+                                          c->rules.p, c->own.p, c->acc.p, integrate, (float)c->time, c->stream));
     return DEME_OK;
 }
 
@@ -1376,6 +1386,8 @@ static int adj_window(deme_ctx* c, int n, bool record) {
 static int adj_check(deme_ctx* c) {
     if (c->jitCustom)
         return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint differentiates the built-in frictional Hertz-Mindlin model");
+    if (c->ownerJitActive)
+        return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint has no expression prescriptions or family change rules");
     if (!c->singleSpheres)
         return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint handles single-sphere owners (spheres as their own owners)");
     if (c->nM * 4 > 256) return c->fail(DEME_ERR_UNSUPPORTED, "the adjoint handles at most 64 materials");
@@ -1519,10 +1531,23 @@ int deme_jit_build(deme_ctx* c, const char* const* keys, const char* const* valu
     c->graphGen++;
     c->adjReady = false;
     if (n < 0 || (n > 0 && (!keys || !values))) return c->fail(DEME_ERR_INVALID_ARGUMENT, "keys/values missing");
-    std::vector<std::pair<std::string, std::string>> ov;
+    std::vector<std::pair<std::string, std::string>> ov, own;
     for (int k = 0; k < n; k++) {
         if (!keys[k] || !values[k]) return c->fail(DEME_ERR_INVALID_ARGUMENT, "null placeholder key or value");
-        ov.emplace_back(keys[k], values[k]);
+        (jit_is_owner_key(keys[k]) ? own : ov).emplace_back(keys[k], values[k]);
+    }
+    if (!own.empty()) {
+        // the owner update kernel's placeholders: specialise that kernel
+        std::string log;
+        hipDeviceProp_t prop;
+        DEME_HIP(c, hipGetDeviceProperties(&prop, c->device));
+        std::vector<char> code;
+        if (jit_compile(jit_owner_source(own), prop.gcnArchName, code, log))
+            return c->fail(DEME_ERR_JIT, "hipRTC compilation of the motion rules failed:\n" + log);
+        DEME_HIP(c, c->jitOwner.load(code));
+        c->ownerValues = own;
+        c->ownerJitActive = true;
+        if (ov.empty()) return DEME_OK;
     }
     if (!c->jitActive) {
         c->jitModel = jit_builtin_model(0);
@@ -1535,6 +1560,24 @@ int deme_jit_build(deme_ctx* c, const char* const* keys, const char* const* valu
     c->jitActive = true;
     c->jitStale = true;
     return jit_rebuild(c);
+}
+
+int deme_clear_motion_rules(deme_ctx* c) {
+    c->graphGen++;
+    c->adjReady = false;
+    c->jitOwner.release();
+    c->ownerJitActive = false;
+    c->ownerValues.clear();
+    return DEME_OK;
+}
+
+int deme_change_family(deme_ctx* c, int from, int to) {
+    if (from < 0 || from > 255 || to < 0 || to > 255)
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, "family numbers lie in 0..255");
+    if (!c->haveState) return c->fail(DEME_ERR_NOT_READY, "no owner state");
+    c->adjReady = false;
+    DEME_LAUNCH(c, "change_family", launch_change_family(c->own.p, c->nO, (uint8_t)from, (uint8_t)to, c->stream));
+    return DEME_OK;
 }
 
 int deme_download_forces(deme_ctx* c, float* F, float* T, float* cA, float* cB) {

@@ -122,6 +122,7 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate, float t,
                                hipStream_t st, OwnerRec* ownOut = nullptr /* integrated records; null: in place */);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_change_family(OwnerRec* own, uint32_t nO, uint8_t from, uint8_t to, hipStream_t st);
 
 // jit_rt.cpp: the force kernel specialised at run time by hipRTC
 struct JitForce {
@@ -135,6 +136,20 @@ struct JitForce {
                       const float4* histIn, const uint32_t* map, float4* histOut, const uint32_t* mirror,
                       float* aTerm, float* bTerm, float* gTerm, ForceRec* out, hipStream_t st) const;
 };
+// the owner update kernel specialised with motion strategies / family change rules
+struct JitOwner {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    void release();
+    hipError_t load(const std::vector<char>& code);
+    hipError_t launch(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnal, uint32_t nAnalOwners,
+                      const uint32_t* ownerSphStart, const uint32_t* rowStart, const uint32_t* bRowStart,
+                      const float* aTerm, const float* bTerm, const float* analAcc, const deme_family_rule* rules,
+                      const OwnerRec* own, OwnerRec* ownOut, float* acc, int doIntegrate, float t,
+                      hipStream_t st) const;
+};
+std::string jit_owner_source(const std::vector<std::pair<std::string, std::string>>& values);
+bool jit_is_owner_key(const std::string& key);
 // built-in model bodies: 0 frictional Hertz-Mindlin (the ahead-of-time kernel's), 1 frictionless Hertz
 const char* jit_builtin_model(int which);
 bool jit_force_source(const std::string& model, const std::vector<std::string>& wildcards, const deme_sim_params* p,

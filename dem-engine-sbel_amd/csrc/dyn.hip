@@ -12,37 +12,13 @@
 // Rows a14-a15, the contact force kernel: generated at build time from jit/force_kernel.tpl with the default
 // frictional Hertz-Mindlin model (jit/hertz_frictional.model), the same source the runtime hipRTC path specialises.
 #include "force_aot.hip"
+// Rows a16 (clump owners) + a17, the owner update: generated at build time from jit/owner_kernel.tpl with no motion
+// strategies (numeric prescriptions from the family rule table).
+#include "owner_aot.hip"
 
 namespace deme_int {
 using namespace deme_dev;
 namespace {
-
-struct f3 {
-    float x, y, z;
-};
-__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
-__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ f3 operator*(f3 a, float b) { return mk(a.x * b, a.y * b, a.z * b); }
-__device__ __forceinline__ f3 operator*(float b, f3 a) { return mk(b * a.x, b * a.y, b * a.z); }
-__device__ __forceinline__ f3 operator/(f3 a, float b) { return mk(a.x / b, a.y / b, a.z / b); }
-__device__ __forceinline__ f3 operator/(f3 a, f3 b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
-__device__ __forceinline__ void operator+=(f3& a, f3 b) {
-    a.x += b.x;
-    a.y += b.y;
-    a.z += b.z;
-}
-__device__ __forceinline__ void operator-=(f3& a, f3 b) {
-    a.x -= b.x;
-    a.y -= b.y;
-    a.z -= b.z;
-}
-__device__ __forceinline__ f3 cross(f3 a, f3 b) {
-    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float length(f3 v) { return sqrtf(dot(v, v)); }
-__device__ __forceinline__ void rotq(f3& v, float w, float x, float y, float z) { rotate(v.x, v.y, v.z, w, x, y, z); }
 
 // Row a16 for analytical owners (planes): every sphere-plane contact contributes its B-side term (stored by the
 // force kernel) to the plane's owner. Deterministic two-level reduction: lane t of a tile sums contacts t, t + 256, ...
@@ -132,134 +108,10 @@ __global__ __launch_bounds__(256) void k_anal_final(const float* __restrict__ pa
     if (t == 0) analAcc[v] = red[0];
 }
 
-__device__ __forceinline__ void add_term(f3& a, f3& al, const float* __restrict__ src) {
-    const float2* g = reinterpret_cast<const float2*>(src);
-    const float2 g0 = g[0], g1 = g[1], g2 = g[2];
-    a += mk(g0.x, g0.y, g1.x);
-    al += mk(g1.y, g2.x, g2.y);
-}
-
-// Rows a16 (clump owners) + a17: owner reduction fused with integrateOwners (DEMIntegrationKernels.cu:170-179).
-// A clump's A-side terms are the contiguous rows of its spheres; its B-side terms are the contiguous mirror rows of
-// its spheres (ordered by B sphere, then A sphere). Terms are summed sequentially in that order (A side, then B side)
-// in float.
-__global__ __launch_bounds__(256) void k_owner_update(
-    KParams P, uint32_t nO, uint32_t nClumps, uint32_t firstAnal, uint32_t nAnalOwners,
-    const uint32_t* __restrict__ ownerSphStart, const uint32_t* __restrict__ rowStart,
-    const uint32_t* __restrict__ bRowStart, const float* __restrict__ aTerm, const float* __restrict__ bTerm,
-    const float* __restrict__ analAcc, const deme_family_rule* __restrict__ rules, const OwnerRec* own,
-    OwnerRec* ownOut, float* __restrict__ accOut, int doIntegrate, float t) {
+// ChangeFamily (APIPublic.cpp:184-207): every owner of family `from` moves to family `to`.
+__global__ __launch_bounds__(256) void k_change_family(OwnerRec* own, uint32_t nO, uint8_t from, uint8_t to) {
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= nO) return;
-    OwnerRec r = load_owner(own, o);
-    f3 a = mk(0, 0, 0), al = mk(0, 0, 0);
-    if (o < nClumps) {
-        const uint32_t s0 = ownerSphStart[o], s1 = ownerSphStart[o + 1];
-        const uint32_t c1 = rowStart[s1];
-        for (uint32_t c = rowStart[s0]; c < c1; c++) add_term(a, al, aTerm + (uint64_t)c * 6);
-        const uint32_t m1 = bRowStart[s1];
-        for (uint32_t m = bRowStart[s0]; m < m1; m++) add_term(a, al, bTerm + (uint64_t)m * 6);
-        a = mk(0.f + a.x, 0.f + a.y, 0.f + a.z);
-        al = mk(0.f + al.x, 0.f + al.y, 0.f + al.z);
-    } else if (o - firstAnal < nAnalOwners) {
-        const float* s = analAcc + (uint64_t)(o - firstAnal) * 6;
-        a = mk(s[0], s[1], s[2]);
-        al = mk(s[3], s[4], s[5]);
-    }
-    accOut[o] = a.x;
-    accOut[(uint64_t)nO + o] = a.y;
-    accOut[2ull * nO + o] = a.z;
-    accOut[3ull * nO + o] = al.x;
-    accOut[4ull * nO + o] = al.y;
-    accOut[5ull * nO + o] = al.z;
-    if (!doIntegrate) return;
-
-    // ---- integrateVel (DEMIntegrationKernels.cu:179-221) ----
-    const deme_family_rule R = rules[r.fam];
-    bool LinPrescribed = false, RotPrescribed = false;
-    const f3 old_v = mk(r.vx, r.vy, r.vz), old_w = mk(r.wx, r.wy, r.wz);
-    if (R.used) {
-        if (R.velSet & 1) r.vx = R.vel[0];
-        if (R.velSet & 2) r.vy = R.vel[1];
-        if (R.velSet & 4) r.vz = R.vel[2];
-        if (R.velSet & 8) r.wx = R.vel[3];
-        if (R.velSet & 16) r.wy = R.vel[4];
-        if (R.velSet & 32) r.wz = R.vel[5];
-        LinPrescribed = R.linVelPrescribed;
-        RotPrescribed = R.rotVelPrescribed;
-    }
-    f3 v_update = mk(0, 0, 0), w_update = mk(0, 0, 0);
-    if (!LinPrescribed) {
-        v_update.x = (a.x + P.Gx) * P.h;
-        v_update.y = (a.y + P.Gy) * P.h;
-        v_update.z = (a.z + P.Gz) * P.h;
-        r.vx += v_update.x;
-        r.vy += v_update.y;
-        r.vz += v_update.z;
-    }
-    if (!RotPrescribed) {
-        w_update.x = al.x * P.h;
-        w_update.y = al.y * P.h;
-        w_update.z = al.z * P.h;
-        r.wx += w_update.x;
-        r.wy += w_update.y;
-        r.wz += w_update.z;
-    }
-    f3 v, w;
-    if (P.integrator == DEME_INTEGRATOR_FORWARD_EULER) {
-        v = old_v;
-        w = old_w;
-    } else if (P.integrator == DEME_INTEGRATOR_CENTERED_DIFFERENCE) {
-        v = old_v + v_update;
-        w = old_w + w_update;
-    } else {
-        v = old_v + v_update * 0.5f;
-        w = old_w + w_update * 0.5f;
-    }
-    // ---- integratePos (DEMIntegrationKernels.cu:241-301) ----
-    double X, Y, Z;
-    owner_position(P, r.voxel, r.lx, r.ly, r.lz, X, Y, Z);
-    LinPrescribed = false;
-    RotPrescribed = false;
-    if (R.used) {
-        if (R.posSet & 1) X = R.pos[0];
-        if (R.posSet & 2) Y = R.pos[1];
-        if (R.posSet & 4) Z = R.pos[2];
-        LinPrescribed = R.linPosPrescribed;
-        RotPrescribed = R.rotPosPrescribed;
-    }
-    if (!LinPrescribed) {
-        X += (double)v.x * P.h;
-        Y += (double)v.y * P.h;
-        Z += (double)v.z * P.h;
-    }
-    position_to_voxel(P, X, Y, Z, r.voxel, r.lx, r.ly, r.lz);
-    if (!RotPrescribed) {
-        double dq0 = 1, dq1 = w.x, dq2 = w.y, dq3 = w.z;
-        const double len = sqrt(dq1 * dq1 + dq2 * dq2 + dq3 * dq3);
-        const double theta = 0.5 * P.h * len;
-        if (len > 0) {
-            dq0 = cos(theta);
-            const double s = sin(theta) / len;
-            dq1 *= s;
-            dq2 *= s;
-            dq3 *= s;
-        }
-        // HamiltonProduct<float> (DEMHelperKernels.cu:169-186)
-        const float a1 = r.qw, b1 = r.qx, c1 = r.qy, d1 = r.qz;
-        const float a2 = (float)dq0, b2 = (float)dq1, c2 = (float)dq2, d2 = (float)dq3;
-        r.qw = a1 * a2 - b1 * b2 - c1 * c2 - d1 * d2;
-        r.qx = a1 * b2 + b1 * a2 + c1 * d2 - d1 * c2;
-        r.qy = a1 * c2 - b1 * d2 + c1 * a2 + d1 * b2;
-        r.qz = a1 * d2 + b1 * c2 - c1 * b2 + d1 * a2;
-    }
-    (void)t;  // numeric prescriptions do not depend on time
-    uint4* dst = reinterpret_cast<uint4*>(ownOut + o);
-    const uint4* src = reinterpret_cast<const uint4*>(&r);
-    dst[0] = src[0];
-    dst[1] = src[1];
-    dst[2] = src[2];
-    dst[3] = src[3];
+    if (o < nO && own[o].fam == from) own[o].fam = to;
 }
 
 inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
@@ -306,9 +158,15 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                const float* analAcc, const deme_family_rule* rules, OwnerRec* own, float* acc,
                                int doIntegrate, float t, hipStream_t st, OwnerRec* ownOut) {
     if (nO == 0) return hipSuccess;
-    k_owner_update<<<blocks(nO, 256), 256, 0, st>>>(P, nO, nClumps, firstAnalOwner, nAnalOwners, ownerSphStart,
+    deme_owner_update<<<blocks(nO, 256), 256, 0, st>>>(P, nO, nClumps, firstAnalOwner, nAnalOwners, ownerSphStart,
                                                     rowStart, bRowStart, aTerm, bTerm, analAcc, rules, own,
                                                     ownOut ? ownOut : own, acc, doIntegrate, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_change_family(OwnerRec* own, uint32_t nO, uint8_t from, uint8_t to, hipStream_t st) {
+    if (nO == 0) return hipSuccess;
+    k_change_family<<<blocks(nO, 256), 256, 0, st>>>(own, nO, from, to);
     return hipGetLastError();
 }
 

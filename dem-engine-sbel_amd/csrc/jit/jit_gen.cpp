@@ -83,6 +83,13 @@ bool model_substitutions(const std::string& model, std::vector<std::string> wild
     return true;
 }
 
+void owner_substitutions(const std::string& velStrategy, const std::string& posStrategy,
+                         const std::string& familyChangeRules, Subst& out) {
+    out.push_back({"_familyChangeRules_", familyChangeRules.empty() ? " " : familyChangeRules});
+    out.push_back({"_velPrescriptionStrategy_", velStrategy.empty() ? " " : velStrategy});
+    out.push_back({"_posPrescriptionStrategy_", posStrategy.empty() ? " " : posStrategy});
+}
+
 static std::string exact(double v) {
     char b[64];
     std::snprintf(b, sizeof b, "%.17g", v);

@@ -26,6 +26,12 @@ bool model_substitutions(const std::string& model, std::vector<std::string> wild
 // parse back to the exact values.
 void param_substitutions(const deme_sim_params* p, Subst& out);
 
+// Placeholder values of the owner update kernel: the prescribed-velocity and prescribed-position strategies
+// (switch cases on the family, as equipFamilyPrescribedMotions writes them, APIPrivate.cpp:1005-1048) and the family
+// change rules (statements setting newFamily, APIPrivate.cpp:980-1003). Empty strings: none (the numeric table only).
+void owner_substitutions(const std::string& velStrategy, const std::string& posStrategy,
+                         const std::string& familyChangeRules, Subst& out);
+
 // Replace every occurrence of each key in order; later entries with the same key as an earlier one win (overrides).
 std::string render(const std::string& tpl, const Subst& s);
 

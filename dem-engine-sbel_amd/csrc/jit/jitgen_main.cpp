@@ -1,6 +1,7 @@
-// Build-time generator of the ahead-of-time force kernel: the template with the default (frictional Hertz-Mindlin)
-// model and the simulation constants read from the kernel parameters.
-//   jitgen <template> <model> <out.hip>
+// Build-time generator of the ahead-of-time kernels from their templates:
+//   jitgen force <template> <model> <out.hip>   the force kernel with the default (frictional Hertz-Mindlin) model
+//                                               and the simulation constants read from the kernel parameters
+//   jitgen owner <template> <out.hip>           the owner update kernel with no motion strategies
 #include <fstream>
 #include <iostream>
 #include <sstream>
@@ -8,8 +9,9 @@
 #include "jit_gen.h"
 
 int main(int argc, char** argv) {
-    if (argc != 4) {
-        std::cerr << "usage: jitgen <template> <model> <out.hip>\n";
+    const bool owner = argc == 4 && std::string(argv[1]) == "owner";
+    if (!(owner || (argc == 5 && std::string(argv[1]) == "force"))) {
+        std::cerr << "usage: jitgen force <template> <model> <out.hip> | jitgen owner <template> <out.hip>\n";
         return 2;
     }
     auto slurp = [](const char* path, std::string& s) {
@@ -21,6 +23,18 @@ int main(int argc, char** argv) {
         return true;
     };
     std::string tpl, model, err;
+    if (owner) {
+        if (!slurp(argv[2], tpl)) {
+            std::cerr << "jitgen: cannot read inputs\n";
+            return 1;
+        }
+        deme_jitgen::Subst s;
+        deme_jitgen::owner_substitutions("", "", "", s);
+        std::ofstream o(argv[3]);
+        o << "// generated from " << argv[2] << " by jitgen; do not edit\n" << deme_jitgen::render(tpl, s);
+        return o ? 0 : 1;
+    }
+    argv++;
     if (!slurp(argv[1], tpl) || !slurp(argv[2], model)) {
         std::cerr << "jitgen: cannot read inputs\n";
         return 1;

@@ -15,6 +15,7 @@
 
 namespace deme_jit_text {
 extern const char* const force_kernel_tpl;
+extern const char* const owner_kernel_tpl;
 extern const char* const jit_prelude_h;
 extern const char* const deme_device_h;
 extern const char* const deme_types_h;
@@ -50,6 +51,22 @@ bool jit_force_source(const std::string& model, const std::vector<std::string>& 
     for (const auto& kv : overrides) s.push_back(kv);
     src = deme_jitgen::render(deme_jit_text::force_kernel_tpl, s);
     return true;
+}
+
+std::string jit_owner_source(const std::vector<std::pair<std::string, std::string>>& values) {
+    deme_jitgen::Subst s;
+    std::string vel, pos, fam;
+    for (const auto& kv : values) {
+        if (kv.first == "_velPrescriptionStrategy_") vel = kv.second;
+        else if (kv.first == "_posPrescriptionStrategy_") pos = kv.second;
+        else if (kv.first == "_familyChangeRules_") fam = kv.second;
+    }
+    deme_jitgen::owner_substitutions(vel, pos, fam, s);
+    return deme_jitgen::render(deme_jit_text::owner_kernel_tpl, s);
+}
+
+bool jit_is_owner_key(const std::string& k) {
+    return k == "_velPrescriptionStrategy_" || k == "_posPrescriptionStrategy_" || k == "_familyChangeRules_";
 }
 
 int jit_compile(const std::string& src, const std::string& arch, std::vector<char>& code, std::string& log) {
@@ -126,6 +143,35 @@ hipError_t JitForce::launch(bool wantFull, const KParams& P, uint32_t nC, const 
     return hipModuleLaunchKernel(wantFull ? full : terms, grid, 1, 1, 256, 1, 1, 0, st, args, nullptr);
 }
 
+void JitOwner::release() {
+    if (mod) (void)hipModuleUnload(mod);
+    mod = nullptr;
+    fn = nullptr;
+}
+
+hipError_t JitOwner::load(const std::vector<char>& code) {
+    release();
+    hipError_t e = hipModuleLoadData(&mod, code.data());
+    if (e != hipSuccess) {
+        mod = nullptr;
+        return e;
+    }
+    return hipModuleGetFunction(&fn, mod, "deme_owner_update");
+}
+
+hipError_t JitOwner::launch(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnal, uint32_t nAnalOwners,
+                            const uint32_t* ownerSphStart, const uint32_t* rowStart, const uint32_t* bRowStart,
+                            const float* aTerm, const float* bTerm, const float* analAcc,
+                            const deme_family_rule* rules, const OwnerRec* own, OwnerRec* ownOut, float* acc,
+                            int doIntegrate, float t, hipStream_t st) const {
+    if (nO == 0) return hipSuccess;
+    KParams p = P;
+    void* args[] = {&p,     &nO,    &nClumps, &firstAnal, &nAnalOwners, &ownerSphStart, &rowStart, &bRowStart,
+                    &aTerm, &bTerm, &analAcc, &rules,     &own,         &ownOut,        &acc,      &doIntegrate,
+                    &t};
+    return hipModuleLaunchKernel(fn, (nO + 255) / 256, 1, 1, 256, 1, 1, 0, st, args, nullptr);
+}
+
 }  // namespace deme_int
 
 // ---- context-free entry points (render and compile; no device needed) ----
@@ -137,6 +183,22 @@ extern "C" int deme_jit_render(const char* model, const char* const* wildcards, 
     const std::string m = model ? model : deme_int::jit_builtin_model(0);
     if (!model) w = {"delta_tan_x", "delta_tan_y", "delta_tan_z", "delta_time"};
     if (!deme_int::jit_force_source(m, w, p, {}, src, err)) return DEME_ERR_INVALID_ARGUMENT;
+    if (needed) *needed = src.size() + 1;
+    if (out && cap) {
+        const size_t n = std::min<size_t>(cap - 1, src.size());
+        std::memcpy(out, src.data(), n);
+        out[n] = '\0';
+    }
+    return DEME_OK;
+}
+
+extern "C" int deme_jit_render_owner(const char* velStrategy, const char* posStrategy, const char* familyChangeRules,
+                                     char* out, uint64_t cap, uint64_t* needed) {
+    std::vector<std::pair<std::string, std::string>> v;
+    if (velStrategy) v.emplace_back("_velPrescriptionStrategy_", velStrategy);
+    if (posStrategy) v.emplace_back("_posPrescriptionStrategy_", posStrategy);
+    if (familyChangeRules) v.emplace_back("_familyChangeRules_", familyChangeRules);
+    const std::string src = deme_int::jit_owner_source(v);
     if (needed) *needed = src.size() + 1;
     if (out && cap) {
         const size_t n = std::min<size_t>(cap - 1, src.size());

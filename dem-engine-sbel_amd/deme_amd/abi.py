@@ -82,6 +82,7 @@ EXPORTED = [
     "deme_kernel_timing", "deme_set_beta", "deme_set_cell_scale", "deme_autotune", "deme_adjoint_forward",
     "deme_adjoint_backward", "deme_clear_contacts", "deme_set_force_model", "deme_jit_build",
     "deme_builtin_force_model", "deme_jit_render", "deme_jit_compile", "deme_set_step_graphs",
+    "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner",
 ]
 
 _lib = None
@@ -140,6 +141,9 @@ def load(path: str = LIB_PATH):
         "deme_adjoint_forward": ([vp, i32, i32], i32),
         "deme_clear_contacts": ([vp], i32),
         "deme_set_step_graphs": ([vp, i32], i32),
+        "deme_clear_motion_rules": ([vp], i32),
+        "deme_change_family": ([vp, i32, i32], i32),
+        "deme_jit_render_owner": ([C.c_char_p, C.c_char_p, C.c_char_p, vp, u64, C.POINTER(u64)], i32),
         "deme_set_force_model": ([vp, C.c_char_p, vp, i32], i32),
         "deme_jit_build": ([vp, vp, vp, i32], i32),
         "deme_builtin_force_model": ([i32], C.c_char_p),
@@ -235,6 +239,17 @@ def jit_render(model=None, wildcards=(), params=None) -> str:
     return buf.value.decode()
 
 
+def jit_render_owner(vel_strategy=None, pos_strategy=None, family_change_rules=None) -> str:
+    """The owner update kernel source for the given strategies and rules."""
+    lib = load()
+    args = tuple(None if x is None else x.encode() for x in (vel_strategy, pos_strategy, family_change_rules))
+    need = C.c_uint64()
+    lib.deme_jit_render_owner(*args, None, 0, C.byref(need))
+    buf = C.create_string_buffer(need.value)
+    lib.deme_jit_render_owner(*args, buf, need.value, C.byref(need))
+    return buf.value.decode()
+
+
 def jit_compile(source: str):
     """hipRTC compile of a force kernel source for gfx950: (status, log)."""
     log = C.create_string_buffer(1 << 16)
@@ -314,6 +329,18 @@ class DeviceSystem:
         keys, vals = _cstr_array(list(subs)), _cstr_array(list(subs.values()))
         self._chk(self.lib.deme_jit_build(self.ctx, C.cast(keys, C.c_void_p) if keys else None,
                                           C.cast(vals, C.c_void_p) if vals else None, len(subs)))
+
+    def set_motion_rules(self, vel_strategy=None, pos_strategy=None, family_change_rules=None):
+        """Owner update kernel specialised (hipRTC) with prescription strategies / family change rules."""
+        subs = {k: v for k, v in (("_velPrescriptionStrategy_", vel_strategy), ("_posPrescriptionStrategy_",
+                                   pos_strategy), ("_familyChangeRules_", family_change_rules)) if v is not None}
+        if not subs:
+            self._chk(self.lib.deme_clear_motion_rules(self.ctx))
+            return
+        self.jit_build(subs)
+
+    def change_family(self, src, dst):
+        self._chk(self.lib.deme_change_family(self.ctx, int(src), int(dst)))
 
     def set_step_graphs(self, on=True):
         """Replay the steps between detection passes from a captured HIP graph (default) or launch them directly."""

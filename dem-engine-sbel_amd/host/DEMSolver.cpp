@@ -76,7 +76,7 @@ void check_family(unsigned int ID, const char* what) {
     }
 }
 
-// A prescription string of the reference is a C expression in t; this path takes numeric constants.
+// A prescription string of the reference is a C expression in t; numeric constants go to the rule table.
 bool parse_numeric(const std::string& s, double& v) {
     const char* b = s.c_str();
     char* e = nullptr;
@@ -236,6 +236,24 @@ std::shared_ptr<DEMForceModel> DEMSolver::UseFrictionlessHertzianModel() {
 std::shared_ptr<DEMForceModel> DEMSolver::DefineContactForceModel(const std::string& model) {
     m_force_model->DefineCustomModel(model);
     return m_force_model;
+}
+
+void DEMSolver::ChangeFamilyWhen(unsigned int ID_from, unsigned int ID_to, const std::string& condition) {
+    check_family(ID_from, "ChangeFamilyWhen");
+    check_family(ID_to, "ChangeFamilyWhen");
+    if (sys_initialized) throw std::runtime_error("ChangeFamilyWhen must be called before Initialize on this path.");
+    m_family_changes.push_back({ID_from, ID_to, condition});
+}
+
+void DEMSolver::ChangeFamily(unsigned int ID_from, unsigned int ID_to) {
+    check_family(ID_from, "ChangeFamily");
+    check_family(ID_to, "ChangeFamily");
+    if (!sys_initialized) {
+        m_pending_family_changes.push_back({ID_from, ID_to, ""});  // applied when Initialize assigns families
+        return;
+    }
+    check(deme_change_family(m_ctx, (int)ID_from, (int)ID_to), "deme_change_family");
+    m_mirror_valid = false;
 }
 
 void DEMSolver::SetFamilyFixed(unsigned int ID) {
@@ -550,6 +568,9 @@ void DEMSolver::Initialize() {
     }
     if (mats.empty()) mats.push_back(deme_material{0, 0, 0, 0, 0});
 
+    for (const auto& fc : m_pending_family_changes)
+        for (auto& f : h_fam)
+            if (f == fc.from) f = (uint8_t)fc.to;
     // family prescriptions (figureOutFamilyMasks APIPrivate.cpp:574-645, equipFamilyPrescribedMotions :1005-1048)
     std::vector<FamilyPrescription> merged(256);
     for (unsigned int f = 0; f < 256; f++) merged[f].family = f;
@@ -574,11 +595,36 @@ void DEMSolver::Initialize() {
         m.linPosPrescribed = m.linPosPrescribed || p.linPosPrescribed;
     }
     std::vector<deme_family_rule> rules(256);
+    // families whose prescriptions are expressions (in t) go to the owner update kernel's strategies, compiled by
+    // hipRTC as the reference's equipFamilyPrescribedMotions strings are; numeric ones stay in the rule table
+    std::string velStrategy, posStrategy;
     for (unsigned int f = 0; f < 256; f++) {
         const auto& m = merged[f];
         deme_family_rule& r = rules[f];
         std::memset(&r, 0, sizeof(r));
         if (!m.used) continue;
+        const std::string* all[9] = {&m.linVelX, &m.linVelY, &m.linVelZ, &m.rotVelX, &m.rotVelY,
+                                     &m.rotVelZ, &m.linPosX, &m.linPosY, &m.linPosZ};
+        bool expr = false;
+        for (const std::string* e : all) {
+            double v;
+            if (*e != "none" && !parse_numeric(*e, v)) expr = true;
+        }
+        if (expr) {
+            const char* vn[6] = {"vX", "vY", "vZ", "omgBarX", "omgBarY", "omgBarZ"};
+            const char* pn[3] = {"X", "Y", "Z"};
+            velStrategy += "case " + std::to_string(f) + ": {";
+            for (int k = 0; k < 6; k++)
+                if (*all[k] != "none") velStrategy += std::string(vn[k]) + " = " + *all[k] + ";";
+            velStrategy += "LinPrescribed = " + std::to_string((int)m.linVelPrescribed) + "; RotPrescribed = " +
+                           std::to_string((int)m.rotVelPrescribed) + "; break; }";
+            posStrategy += "case " + std::to_string(f) + ": {";
+            for (int k = 0; k < 3; k++)
+                if (*all[6 + k] != "none") posStrategy += std::string(pn[k]) + " = " + *all[6 + k] + ";";
+            posStrategy += "LinPrescribed = " + std::to_string((int)m.linPosPrescribed) + "; RotPrescribed = " +
+                           std::to_string((int)m.rotPosPrescribed) + "; break; }";
+            continue;
+        }
         r.used = 1;
         r.linVelPrescribed = m.linVelPrescribed;
         r.rotVelPrescribed = m.rotVelPrescribed;
@@ -616,6 +662,20 @@ void DEMSolver::Initialize() {
     check(deme_upload_materials(m_ctx, (uint32_t)mats.size(), mats.data()), "deme_upload_materials");
     check(deme_upload_analytical(m_ctx, (uint32_t)anal.size(), anal.data()), "deme_upload_analytical");
     check(deme_upload_families(m_ctx, m_family_mask.data(), rules.data()), "deme_upload_families");
+    // motion strategies and family change rules (equipFamilyOnFlyChanges, APIPrivate.cpp:980-1003)
+    std::string changeRules;
+    for (const auto& fc : m_family_changes) {
+        std::string cond = fc.condition;
+        for (size_t p = cond.find("return"); p != std::string::npos; p = cond.find("return", p))
+            cond.replace(p, 6, "shouldMakeChange = ");
+        changeRules += "if (family_code == " + std::to_string(fc.from) + ") { bool shouldMakeChange = false; " + cond +
+                       " if (shouldMakeChange) newFamily = " + std::to_string(fc.to) + "; }";
+    }
+    if (!velStrategy.empty() || !changeRules.empty()) {
+        const char* keys[3] = {"_velPrescriptionStrategy_", "_posPrescriptionStrategy_", "_familyChangeRules_"};
+        const char* vals[3] = {velStrategy.c_str(), posStrategy.c_str(), changeRules.c_str()};
+        check(deme_jit_build(m_ctx, keys, vals, 3), "deme_jit_build (motion rules)");
+    }
     if (m_force_model->type != FORCE_MODEL::HERTZIAN) {
         // other models are compiled now by hipRTC (the built-in frictional model ships precompiled)
         const std::string body = m_force_model->type == FORCE_MODEL::HERTZIAN_FRICTIONLESS

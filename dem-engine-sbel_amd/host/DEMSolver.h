@@ -173,6 +173,11 @@ class DEMSolver {
     void SetFamilyPrescribedPosition(unsigned int ID, const std::string& X, const std::string& Y, const std::string& Z);
     void DisableContactBetweenFamilies(unsigned int ID1, unsigned int ID2);
     void EnableContactBetweenFamilies(unsigned int ID1, unsigned int ID2);
+    // ChangeFamilyWhen (API.h:262): each step, owners of family ID_from whose condition (C++ statements reading pos,
+    // vel, acc, mass, family_code, t, h and ending in `return <bool>;`) holds move to family ID_to.
+    void ChangeFamilyWhen(unsigned int ID_from, unsigned int ID_to, const std::string& condition);
+    // ChangeFamily (API.h:268): all owners of family ID_from move to ID_to now.
+    void ChangeFamily(unsigned int ID_from, unsigned int ID_to);
 
     // ---- entities (API.h:160-310) ----
     std::shared_ptr<DEMMaterial> LoadMaterial(const std::unordered_map<std::string, float>& mat_prop);
@@ -270,6 +275,11 @@ class DEMSolver {
     std::vector<std::shared_ptr<DEMClumpBatch>> cached_input_clump_batches;
     std::vector<std::shared_ptr<DEMExternObj>> cached_extern_objs;
     std::vector<FamilyPrescription> m_input_family_prescription;
+    struct FamilyChange {
+        unsigned int from, to;
+        std::string condition;
+    };
+    std::vector<FamilyChange> m_family_changes, m_pending_family_changes;
     std::vector<std::pair<unsigned int, unsigned int>> m_input_no_contact_pairs;
     std::vector<uint8_t> m_family_mask;
     bool m_mask_dirty = false;

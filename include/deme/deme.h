@@ -153,7 +153,18 @@ int deme_autotune(deme_ctx* ctx, const int* freqs, int nFreqs, const double* cel
  * deme_builtin_force_model: the source of built-in model 0 (frictional Hertz-Mindlin, wildcards delta_tan_x,
  * delta_tan_y, delta_tan_z, delta_time) or 1 (frictionless Hertz, no wildcards), NULL otherwise. */
 int deme_set_force_model(deme_ctx* ctx, const char* model, const char* const* wildcards, int nWildcards);
+/* deme_jit_build also takes the owner update kernel's placeholders (DEMIntegrationKernels.cu, DEMModeratorKernels.cu):
+ * _velPrescriptionStrategy_ and _posPrescriptionStrategy_ (switch cases on `family`, as equipFamilyPrescribedMotions
+ * writes them, APIPrivate.cpp:1005-1048: expressions in t setting vX vY vZ omgBarX omgBarY omgBarZ / X Y Z oriQw..z and
+ * LinPrescribed, RotPrescribed) and _familyChangeRules_ (statements reading pos (double3, relative to the world's
+ * lower corner), vel, acc, mass, family_code, t, h and setting newFamily, as equipFamilyOnFlyChanges writes them,
+ * APIPrivate.cpp:980-1003); given any of them, the owner update kernel is compiled with them. Families a strategy
+ * does not name keep the numeric prescriptions of deme_upload_families. */
 int deme_jit_build(deme_ctx* ctx, const char* const* keys, const char* const* values, int n);
+/* Drop the owner update kernel's strategies and rules (back to the shipped kernel). */
+int deme_clear_motion_rules(deme_ctx* ctx);
+/* ChangeFamily (API.h:268): every owner of family `from` moves to family `to` now. */
+int deme_change_family(deme_ctx* ctx, int from, int to);
 const char* deme_builtin_force_model(int which);
 /* Context-free helpers: the specialised source for a model (model NULL: built-in model 0; p NULL: constants read
  * from the kernel parameters), written to out (cap bytes, NUL-terminated; *needed = full size + 1), and a hipRTC
@@ -161,6 +172,9 @@ const char* deme_builtin_force_model(int which);
 int deme_jit_render(const char* model, const char* const* wildcards, int nWildcards, const deme_sim_params* p,
                     char* out, uint64_t cap, uint64_t* needed);
 int deme_jit_compile(const char* source, char* log, uint64_t cap);
+/* The owner update kernel's source for the given strategies and rules (each nullable). */
+int deme_jit_render_owner(const char* velStrategy, const char* posStrategy, const char* familyChangeRules, char* out,
+                          uint64_t cap, uint64_t* needed);
 
 /* Adjoint of the step (no counterpart in the reference, which has no differentiation; for calibration callers that
  * fit material parameters or initial velocities to observed states). Systems of single-sphere owners (each sphere

@@ -74,3 +74,23 @@ def test_custom_model_compiles_and_errors_are_reported():
     assert st == 0, log
     st, log = abi.jit_compile(abi.jit_render("force = make_float3(overlapDepth, ;", ()))
     assert st == 6 and "error" in log  # DEME_ERR_JIT, with the compiler's message
+
+
+def test_shipped_owner_kernel_is_the_template_without_strategies():
+    gen = open(os.path.join(ROOT, "dem-engine-sbel_amd", "_build", "owner_aot.hip")).read()
+    assert gen.split("\n", 1)[1] == abi.jit_render_owner()
+    for p in ("_velPrescriptionStrategy_", "_posPrescriptionStrategy_", "_familyChangeRules_"):
+        assert p not in gen
+
+
+def test_motion_strategies_and_family_rules_compile():
+    vel = "case 7: { vX = 0.f; vY = 0.f; vZ = -0.05f * t; LinPrescribed = 1; RotPrescribed = 1; break; }"
+    pos = "case 8: { X = 0.1 + 0.01 * t; LinPrescribed = 1; RotPrescribed = 0; break; }"
+    fam = ("if (family_code == 1) { bool shouldMakeChange = false; shouldMakeChange = pos.z < 0.1 && "
+           "length(acc * mass) > 0.f; if (shouldMakeChange) newFamily = 2; }")
+    src = abi.jit_render_owner(vel, pos, fam)
+    assert "vZ = -0.05f * t;" in src and "newFamily = 2;" in src
+    st, log = abi.jit_compile(src)
+    assert st == 0, log
+    st, log = abi.jit_compile(abi.jit_render_owner("case 3: { vX = undefined_thing; break; }"))
+    assert st == 6 and "undefined_thing" in log
