@@ -773,6 +773,10 @@ float3 DEMSolver::GetOwnerVelocity(uint32_t i) {
     syncHostState();
     return make_float3(h_vx.at(i), h_vy.at(i), h_vz.at(i));
 }
+unsigned int DEMSolver::GetOwnerFamily(uint32_t i) {
+    syncHostState();
+    return h_fam.at(i);
+}
 float3 DEMSolver::GetOwnerAngVel(uint32_t i) {
     syncHostState();
     return make_float3(h_wx.at(i), h_wy.at(i), h_wz.at(i));

@@ -207,6 +207,7 @@ class DEMSolver {
     float3 GetOwnerPosition(uint32_t ownerID);
     float3 GetOwnerVelocity(uint32_t ownerID);
     float3 GetOwnerAngVel(uint32_t ownerID);
+    unsigned int GetOwnerFamily(uint32_t ownerID);
     float4 GetOwnerOriQ(uint32_t ownerID);
     float3 GetOwnerAcc(uint32_t ownerID);
     float3 GetOwnerAngAcc(uint32_t ownerID);

@@ -40,3 +40,15 @@ def test_custom_linear_model_rests_at_its_spring_depth():
     expect = 2 * d["mass"] * 9.81 / d["k"]  # k * (geometric depth / 2) = m g
     assert d["vmax"] < 1e-3
     assert abs(depth - expect) <= 0.05 * expect, (depth, expect)
+
+
+def test_expression_prescription_and_family_change_through_the_facade():
+    d = run("prescribed", n=9, seconds=0.02)
+    # sphere 0: vx = 0.1 t of the last step (t as the step saw it, float), vy = vz = 0
+    h = 1e-5
+    steps = round(d["time"] / h)
+    t_last = np.float32(sum([h] * (steps - 1)))
+    assert d["v0"][0] == pytest.approx(float(np.float32(0.1) * t_last), rel=1e-6)
+    assert d["v0"][1] == 0.0 and d["v0"][2] == 0.0
+    # the free spheres fell faster than 0.05 m/s within the first 6 ms and moved to family 6; sphere 0 stayed in 5
+    assert d["families"][0] == 5 and all(f == 6 for f in d["families"][1:])
