@@ -66,6 +66,7 @@ struct deme_ctx {
     // the non-detection step replayed from a captured HIP graph (deme_set_step_graphs); any change of buffers,
     // kernel parameters or kernels bumps graphGen and the next such step captures again
     bool useGraphs = false;
+    bool keepRecords = false;  // the last step of each deme_step call stores per-contact records
     uint64_t graphGen = 1, capturedGen = 0;
     hipGraphExec_t stepExec = nullptr;
     KParams capturedKp{};
@@ -1147,10 +1148,11 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
         // Between detection passes every step launches the same kernels with the same arguments (the history is
         // updated in place; the time argument is not read by the integration), so it is captured once per pass and
         // replayed. The step right after a detection pass (history gathered through the map) runs directly.
-        if (!detect && !c->fresh && c->useGraphs && !c->timing && !c->jitStale && c->stream) {
+        const bool records = c->keepRecords && k == nSteps - 1;
+        if (!detect && !c->fresh && !records && c->useGraphs && !c->timing && !c->jitStale && c->stream) {
             if ((r = step_graph(c))) return r;
         } else {
-            r = force_pass(c, false);
+            r = force_pass(c, records);
             if (r) return r;
             r = collect_and_update(c, 1);
             if (r) return r;
@@ -1158,6 +1160,11 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
         c->time += (double)c->sp.h;
         c->nSteps++;
     }
+    return DEME_OK;
+}
+
+int deme_set_keep_contact_records(deme_ctx* c, int on) {
+    c->keepRecords = on != 0;
     return DEME_OK;
 }
 

@@ -82,6 +82,7 @@ EXPORTED = [
     "deme_kernel_timing", "deme_set_beta", "deme_set_cell_scale", "deme_autotune", "deme_adjoint_forward",
     "deme_adjoint_backward", "deme_clear_contacts", "deme_set_force_model", "deme_jit_build",
     "deme_builtin_force_model", "deme_jit_render", "deme_jit_compile", "deme_set_step_graphs",
+    "deme_set_keep_contact_records",
     "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner",
 ]
 
@@ -141,6 +142,7 @@ def load(path: str = LIB_PATH):
         "deme_adjoint_forward": ([vp, i32, i32], i32),
         "deme_clear_contacts": ([vp], i32),
         "deme_set_step_graphs": ([vp, i32], i32),
+        "deme_set_keep_contact_records": ([vp, i32], i32),
         "deme_clear_motion_rules": ([vp], i32),
         "deme_change_family": ([vp, i32, i32], i32),
         "deme_jit_render_owner": ([C.c_char_p, C.c_char_p, C.c_char_p, vp, u64, C.POINTER(u64)], i32),
@@ -343,8 +345,12 @@ class DeviceSystem:
         self._chk(self.lib.deme_change_family(self.ctx, int(src), int(dst)))
 
     def set_step_graphs(self, on=True):
-        """Replay the steps between detection passes from a captured HIP graph (default) or launch them directly."""
+        """Replay the steps between detection passes from a captured HIP graph, or launch them directly (default)."""
         self._chk(self.lib.deme_set_step_graphs(self.ctx, 1 if on else 0))
+
+    def keep_contact_records(self, on=True):
+        """Store the per-contact records (download_forces) on the last step of every step() call."""
+        self._chk(self.lib.deme_set_keep_contact_records(self.ctx, 1 if on else 0))
 
     def clear_contacts(self):
         """Forget the contact list and histories; the step count and time restart at zero."""

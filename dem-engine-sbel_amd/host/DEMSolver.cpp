@@ -1,6 +1,7 @@
 // DEMSolver facade (see DEMSolver.h). Host-side set-up restates DEMSolver::Initialize
 // (src/DEM/APIPublic.cpp:659-693, APIPrivate.cpp) and hands every per-step operation to the C ABI.
 #include "DEMSolver.h"
+#include "host_math.h"
 
 #include <cmath>
 #include <cstdio>
@@ -12,21 +13,10 @@ namespace deme {
 
 namespace {
 
+using hostmath::host_rotate;
+using hostmath::normalize3;
+
 float3 f3sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
-
-// helper_math normalize on the host: v * (1 / sqrtf(dot(v, v)))
-float3 normalize3(float3 v) {
-    const float invLen = 1.0f / std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
-    return make_float3(v.x * invLen, v.y * invLen, v.z * invLen);
-}
-
-// hostApplyOriQToVector3<float, float> (HostSideHelpers.hpp:373-384)
-void host_rotate(float& X, float& Y, float& Z, float Qw, float Qx, float Qy, float Qz) {
-    const float oX = X, oY = Y, oZ = Z;
-    X = (2.0f * (Qw * Qw + Qx * Qx) - 1.0f) * oX + (2.0f * (Qx * Qy - Qw * Qz)) * oY + (2.0f * (Qx * Qz + Qw * Qy)) * oZ;
-    Y = (2.0f * (Qx * Qy + Qw * Qz)) * oX + (2.0f * (Qw * Qw + Qy * Qy) - 1.0f) * oY + (2.0f * (Qy * Qz - Qw * Qx)) * oZ;
-    Z = (2.0f * (Qx * Qz - Qw * Qy)) * oX + (2.0f * (Qy * Qz + Qw * Qx)) * oY + (2.0f * (Qw * Qw + Qz * Qz) - 1.0f) * oZ;
-}
 
 uint64_t sat_u64(double x) {
     if (!(x > 0.0)) return 0;
@@ -254,6 +244,11 @@ void DEMSolver::ChangeFamily(unsigned int ID_from, unsigned int ID_to) {
     }
     check(deme_change_family(m_ctx, (int)ID_from, (int)ID_to), "deme_change_family");
     m_mirror_valid = false;
+}
+
+void DEMSolver::DisableFamilyOutput(unsigned int ID) {
+    check_family(ID, "DisableFamilyOutput");
+    m_no_output_families.insert(ID);
 }
 
 void DEMSolver::SetFamilyFixed(unsigned int ID) {
@@ -494,6 +489,7 @@ void DEMSolver::Initialize() {
                     &h_alz})
         v->assign(nO, 0.f);
     h_fam.assign(nO, 0);
+    m_clump_template.assign(nClumps, 0);
     mass.resize(nO);
     moi.resize(3 * nO);
     size_t o = 0;
@@ -501,6 +497,7 @@ void DEMSolver::Initialize() {
         for (size_t i = 0; i < b->GetNumClumps(); i++, o++) {
             const auto& t = b->types.at(i);
             if (!t) throw std::runtime_error("A clump in an AddClumps batch has no template.");
+            m_clump_template[o] = t->mark;
             mass[o] = t->mass;
             moi[3 * o] = t->MOI.x;
             moi[3 * o + 1] = t->MOI.y;
@@ -529,6 +526,11 @@ void DEMSolver::Initialize() {
         }
     }
     nSpheresGM = sphOwner.size();
+    m_sph_owner = sphOwner;
+    m_sph_r = sphR;
+    m_sph_rel.resize(nSpheresGM);
+    for (size_t i = 0; i < nSpheresGM; i++)
+        m_sph_rel[i] = make_float3(sphRel[3 * i], sphRel[3 * i + 1], sphRel[3 * i + 2]);
     std::vector<deme_anal_obj> anal;
     for (size_t k = 0; k < nExtObj; k++, o++) {
         const auto& e = cached_extern_objs[k];
@@ -556,6 +558,8 @@ void DEMSolver::Initialize() {
         }
     }
     nAnalGM = anal.size();
+    m_anal_owner.resize(nAnalGM);
+    for (size_t a = 0; a < nAnalGM; a++) m_anal_owner[a] = anal[a].owner;
 
     // materials (equipMaterials, APIPrivate.cpp:1089-1140): missing properties default to 0
     std::vector<deme_material> mats;
@@ -662,6 +666,8 @@ void DEMSolver::Initialize() {
     check(deme_upload_materials(m_ctx, (uint32_t)mats.size(), mats.data()), "deme_upload_materials");
     check(deme_upload_analytical(m_ctx, (uint32_t)anal.size(), anal.data()), "deme_upload_analytical");
     check(deme_upload_families(m_ctx, m_family_mask.data(), rules.data()), "deme_upload_families");
+    // the contact file reports the last step's per-contact forces and points, as the reference keeps them
+    check(deme_set_keep_contact_records(m_ctx, 1), "deme_set_keep_contact_records");
     // motion strategies and family change rules (equipFamilyOnFlyChanges, APIPrivate.cpp:980-1003)
     std::string changeRules;
     for (const auto& fc : m_family_changes) {

@@ -25,6 +25,50 @@ enum class TIME_INTEGRATOR { FORWARD_EULER, CENTERED_DIFFERENCE, EXTENDED_TAYLOR
 // Spatial directions (src/DEM/Defines.h:133)
 enum class SPATIAL_DIR { X, Y, Z, NONE };
 enum class FORCE_MODEL { HERTZIAN, HERTZIAN_FRICTIONLESS, CUSTOM };
+// Output file formats (src/DEM/Defines.h:114): CSV is written; BINARY leaves an empty file, as the reference does;
+// CHPF (a third-party binary format) is not built and throws.
+enum class OUTPUT_FORMAT { CSV, BINARY, CHPF };
+// Columns of the sphere and clump files beyond the position (src/DEM/Defines.h:118-131). The reference's writers
+// emit ABSV, VEL and FAMILY; the other flags are accepted and write nothing, as there.
+enum OUTPUT_CONTENT {
+    XYZ = 0,
+    QUAT = 1,
+    ABSV = 2,
+    VEL = 4,
+    ANG_VEL = 8,
+    ACC = 16,
+    ANG_ACC = 32,
+    FAMILY = 64,
+    MAT = 128,
+    EXP_FACTOR = 256
+};
+// Columns of the contact file beyond the owners and the type (src/DEM/Defines.h:135-143); COMPONENT writes nothing.
+enum CNT_OUTPUT_CONTENT { OWNERS = 0, FORCE = 1, POINT = 2, COMPONENT = 4, NORMAL = 8, TORQUE_ONLY_FORCE = 16 };
+
+// Column names of the output files (src/DEM/Structs.h:584-611): the on-disk format, kept so files interchange with
+// the reference's tools and readers.
+const std::string OUTPUT_FILE_X_COL_NAME = "X";
+const std::string OUTPUT_FILE_Y_COL_NAME = "Y";
+const std::string OUTPUT_FILE_Z_COL_NAME = "Z";
+const std::string OUTPUT_FILE_R_COL_NAME = "r";
+const std::string OUTPUT_FILE_CLUMP_TYPE_NAME = "clump_type";
+const std::string OUTPUT_FILE_OWNER_1_NAME = "A";
+const std::string OUTPUT_FILE_OWNER_2_NAME = "B";
+const std::string OUTPUT_FILE_COMP_1_NAME = "compA";
+const std::string OUTPUT_FILE_COMP_2_NAME = "compB";
+const std::string OUTPUT_FILE_CNT_TYPE_NAME = "contact_type";
+const std::string OUTPUT_FILE_FORCE_X_NAME = "f_x";
+const std::string OUTPUT_FILE_FORCE_Y_NAME = "f_y";
+const std::string OUTPUT_FILE_FORCE_Z_NAME = "f_z";
+const std::string OUTPUT_FILE_TOF_X_NAME = "tof_x";
+const std::string OUTPUT_FILE_TOF_Y_NAME = "tof_y";
+const std::string OUTPUT_FILE_TOF_Z_NAME = "tof_z";
+const std::string OUTPUT_FILE_NORMAL_X_NAME = "n_x";
+const std::string OUTPUT_FILE_NORMAL_Y_NAME = "n_y";
+const std::string OUTPUT_FILE_NORMAL_Z_NAME = "n_z";
+const std::string OUTPUT_FILE_SPH_SPH_CONTACT_NAME = "SS";
+const std::string OUTPUT_FILE_SPH_ANAL_CONTACT_NAME = "SA";
+const std::string OUTPUT_FILE_SPH_MESH_CONTACT_NAME = "SM";
 
 constexpr unsigned int RESERVED_FAMILY_NUM = 255;
 constexpr unsigned int DEFAULT_CLUMP_FAMILY_NUM = 0;
@@ -178,6 +222,39 @@ class DEMSolver {
     void ChangeFamilyWhen(unsigned int ID_from, unsigned int ID_to, const std::string& condition);
     // ChangeFamily (API.h:268): all owners of family ID_from move to ID_to now.
     void ChangeFamily(unsigned int ID_from, unsigned int ID_to);
+    // DisableFamilyOutput (API.h:260): owners of this family are left out of the sphere and clump files.
+    void DisableFamilyOutput(unsigned int ID);
+
+    // ---- output files (API.h:318-371, 430-436) ----
+    void SetOutputFormat(OUTPUT_FORMAT format) { m_out_format = format; }
+    void SetOutputContent(unsigned int content) { m_out_content = content; }
+    void SetContactOutputFormat(OUTPUT_FORMAT format) { m_cnt_out_format = format; }
+    void SetContactOutputContent(unsigned int content) { m_cnt_out_content = content; }
+    // One row per clump: X,Y,Z,Qw,Qx,Qy,Qz,clump_type then the requested columns (dT.cpp:990-1052).
+    void WriteClumpFile(const std::string& outfilename) const;
+    // One row per component sphere: X,Y,Z,r then the requested columns (dT.cpp:896-986).
+    void WriteSphereFile(const std::string& outfilename) const;
+    // One row per pair of the contact list of the last step: A,B,contact_type then the requested columns
+    // (dT.cpp:1054-1180). Contacts are written in the library's canonical order (by A's sphere; plane contacts first;
+    // sphere partners by ascending id).
+    void WriteContactFile(const std::string& outfilename) const;
+    // Readers of the clump file (API.h:326-371): clump type name -> positions / quaternions (w, x, y, z), in file
+    // order. Columns are found by name; others are ignored; blank lines are skipped; numbers are read as the
+    // reference's CSV reader reads them. Throws std::runtime_error on a missing or repeated column, a row with too
+    // few or too many fields, or a field that is not a number.
+    static std::unordered_map<std::string, std::vector<float3>> ReadClumpXyzFromCsv(
+        const std::string& infilename,
+        const std::string& clump_header = OUTPUT_FILE_CLUMP_TYPE_NAME,
+        const std::string& x_header = OUTPUT_FILE_X_COL_NAME,
+        const std::string& y_header = OUTPUT_FILE_Y_COL_NAME,
+        const std::string& z_header = OUTPUT_FILE_Z_COL_NAME);
+    static std::unordered_map<std::string, std::vector<float4>> ReadClumpQuatFromCsv(
+        const std::string& infilename,
+        const std::string& clump_header = OUTPUT_FILE_CLUMP_TYPE_NAME,
+        const std::string& qw_header = "Qw",
+        const std::string& qx_header = "Qx",
+        const std::string& qy_header = "Qy",
+        const std::string& qz_header = "Qz");
 
     // ---- entities (API.h:160-310) ----
     std::shared_ptr<DEMMaterial> LoadMaterial(const std::unordered_map<std::string, float>& mat_prop);
@@ -284,10 +361,28 @@ class DEMSolver {
     std::vector<std::pair<unsigned int, unsigned int>> m_input_no_contact_pairs;
     std::vector<uint8_t> m_family_mask;
     bool m_mask_dirty = false;
+    std::set<unsigned int> m_no_output_families;
+
+    // output settings (API.h:474-479 defaults)
+    OUTPUT_FORMAT m_out_format = OUTPUT_FORMAT::CSV;
+    OUTPUT_FORMAT m_cnt_out_format = OUTPUT_FORMAT::CSV;
+    unsigned int m_out_content = OUTPUT_CONTENT::QUAT | OUTPUT_CONTENT::ABSV;
+    unsigned int m_cnt_out_content = CNT_OUTPUT_CONTENT::FORCE | CNT_OUTPUT_CONTENT::POINT;
 
     // derived
     deme_sim_params m_params{};
     size_t nOwnerClumps = 0, nOwnerBodies = 0, nSpheresGM = 0, nExtObj = 0, nAnalGM = 0;
+    // geometry kept for the output files: component spheres (owner, radius, offset in the owner frame), the template
+    // of each clump, the owner of each analytical component
+    std::vector<uint32_t> m_sph_owner;
+    std::vector<float> m_sph_r;
+    std::vector<float3> m_sph_rel;
+    std::vector<uint32_t> m_clump_template;
+    std::vector<uint32_t> m_anal_owner;
+
+    void writeSpheresCsv(std::string& out);
+    void writeClumpsCsv(std::string& out);
+    void writeContactsCsv(std::string& out);
 
     // host mirror of the owner state (refreshed lazily after dynamics)
     bool m_mirror_valid = false, m_mirror_dirty = false;

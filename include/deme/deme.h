@@ -121,6 +121,11 @@ int deme_step(deme_ctx* ctx, int n_steps, int update_freq);
  * launches): the capture per detection pass costs more than the replays save unless passes are far apart. Results
  * are identical either way. Kernel timing (deme_enable_kernel_timing) runs directly. */
 int deme_set_step_graphs(deme_ctx* ctx, int on);
+/* When on, the last step of every deme_step call also stores the per-contact records that deme_download_forces
+ * returns (force, torque-only force, contact points), as the reference's force kernel stores contactForces,
+ * contactTorque_convToForce and contactPointGeometryA/B each step (DEMCalcForceKernels.cu:160-175) for its contact
+ * file writer (dT.cpp:1054-1180). Off by default: deme_step then keeps only the per-owner terms. */
+int deme_set_keep_contact_records(deme_ctx* ctx, int on);
 
 /* Drift margin beta (SetExpandFactor, API.h:100-104): detection uses r + beta. Takes effect at the next detection. */
 int deme_set_beta(deme_ctx* ctx, float beta);

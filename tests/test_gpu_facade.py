@@ -52,3 +52,59 @@ def test_expression_prescription_and_family_change_through_the_facade():
     assert d["v0"][1] == 0.0 and d["v0"][2] == 0.0
     # the free spheres fell faster than 0.05 m/s within the first 6 ms and moved to family 6; sphere 0 stayed in 5
     assert d["families"][0] == 5 and all(f == 6 for f in d["families"][1:])
+
+
+def _state(path):
+    with open(path) as fh:
+        st = json.load(fh)
+    st["voxel"] = [int(v) for v in st["voxel"]]
+    return st
+
+
+def test_output_files_match_the_restated_writers_and_restart(tmp_path):
+    import csv_ref
+
+    d = str(tmp_path)
+    print(f"facade_demo io 96 0.1 {d}", flush=True)
+    out = subprocess.run([DEMO, "io", "96", "0.1", d], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    info = json.loads(out.stdout.strip().splitlines()[-1])
+    st = _state(os.path.join(d, "state.json"))
+    types = np.array(st["type"])
+    assert info["contacts"] == len(types) and (types == 1).sum() > 20 and (types == 3).sum() > 10
+    assert any(f == 1 for f in st["family"])  # some clumps are in the family whose output is disabled
+
+    def text(name):
+        with open(os.path.join(d, name)) as fh:
+            return fh.read()
+
+    allv, dflt = 1 | 2 | 4 | 64, 1 | 2
+    assert text("spheres_all.csv") == csv_ref.spheres_csv(st, allv, no_output={1})
+    assert text("clumps_all.csv") == csv_ref.clumps_csv(st, allv, no_output={1})
+    assert text("spheres_default.csv") == csv_ref.spheres_csv(st, dflt, no_output={1})
+    assert text("clumps_default.csv") == csv_ref.clumps_csv(st, dflt, no_output={1})
+    assert text("contacts_all.csv") == csv_ref.contacts_csv(st, 1 | 2 | 8 | 16)
+    assert text("contacts_default.csv") == csv_ref.contacts_csv(st, 1 | 2)
+    assert text("contacts_owners.csv") == csv_ref.contacts_csv(st, 0)
+
+    # physical sense of the contact file: touching pairs push A away from B, so the force on A points against the
+    # normal from A's sphere centre to the contact point. Damping pulls separating pairs a little (the model does not
+    # clip the normal force): such pulls stay below a few ball weights (~7e-3 N each), pushes reach ~100 of them.
+    rows = [ln.split(",") for ln in text("contacts_all.csv").splitlines()[1:]]
+    F = np.array([[float(x) for x in r[3:6]] for r in rows])
+    n = np.array([[float(x) for x in r[9:12]] for r in rows])
+    touching = np.abs(F).sum(1) > 0
+    assert touching.sum() > 10
+    fn = (F[touching] * n[touching]).sum(1)
+    assert fn.min() < -0.2 and fn.max() < 0.05, (fn.min(), fn.max())
+    assert np.allclose(np.linalg.norm(n[touching], axis=1), 1.0, atol=1e-5)
+
+    # restart: the clumps read back from the clump file start a second simulation at the same places
+    a = [ln.split(",") for ln in text("clumps_all.csv").splitlines()[1:]]
+    b = [ln.split(",") for ln in text("clumps_restart.csv").splitlines()[1:]]
+    key = lambda r: (r[7], float(r[0]), float(r[1]), float(r[2]))  # noqa: E731
+    a, b = sorted(a, key=key), sorted(b, key=key)
+    assert len(a) == len(b) == sum(1 for f in st["family"][:st["n_clumps"]] if f != 1)
+    for ra, rb in zip(a, b):
+        assert ra[7] == rb[7]
+        assert np.allclose([float(x) for x in ra[:7]], [float(x) for x in rb[:7]], rtol=1e-5, atol=1e-6)
