@@ -9,10 +9,12 @@
 #include <map>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/deme/deme.h"
 #include "deme_internal.h"
+#include "jit/jit_gen.h"
 
 using namespace deme_int;
 
@@ -80,6 +82,11 @@ struct deme_ctx {
     std::string jitModel;
     std::vector<std::string> jitWildcards;
     std::vector<std::pair<std::string, std::string>> jitOverrides;
+    // inspectors: one hipRTC module per region code; block partials (+ the result) and the clump volumes
+    std::unordered_map<std::string, JitInspect> inspectJit;
+    DBuf<double> inspPart;
+    DBuf<float> ownerVol;
+    bool haveVolumes = false;
     uint32_t nS = 0, nO = 0, nClumps = 0, nA = 0, nM = 0, nAnalOwners = 0;
     uint32_t maxMatSph = 0, maxMatAnal = 0, maxMat = 0;
     uint64_t nBins = 0;
@@ -380,6 +387,8 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->matGrad.release(); c->lDt[0].release(); c->lDt[1].release(); c->lDtPrev.release();
     c->jit.release();
     c->jitOwner.release();
+    for (auto& kv : c->inspectJit) kv.second.release();
+    c->inspPart.release(); c->ownerVol.release();
     if (c->stepExec) (void)hipGraphExecDestroy(c->stepExec);
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
@@ -495,6 +504,7 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     c->nS = nS;
     c->kp.nS = nS;
+    c->haveVolumes = false;  // per clump: re-uploaded with the templates
     c->nO = nO;
     c->nClumps = nClumps;
     c->nAnalOwners = nO - nClumps;
@@ -1662,6 +1672,87 @@ int deme_get_counts(deme_ctx* c, deme_counts* out) {
     out->nSteps = c->nSteps;
     out->time = c->time;
     out->nCells = c->nCells;
+    return DEME_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Inspectors
+// ---------------------------------------------------------------------------------------------------------------
+int deme_set_owner_volumes(deme_ctx* c, const float* volume) {
+    if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "templates not uploaded (deme_upload_templates)");
+    if (!volume && c->nClumps) return c->fail(DEME_ERR_INVALID_ARGUMENT, "volume is NULL");
+    DEME_HIP(c, c->ownerVol.ensure(std::max<uint32_t>(c->nClumps, 1)));
+    if (c->nClumps)
+        DEME_HIP(c, hipMemcpyAsync(c->ownerVol.p, volume, (size_t)c->nClumps * sizeof(float), hipMemcpyHostToDevice,
+                                   c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    c->haveVolumes = true;
+    return DEME_OK;
+}
+
+int deme_inspect(deme_ctx* c, const char* quantity, const char* region, float* value) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!quantity || !value) return c->fail(DEME_ERR_INVALID_ARGUMENT, "quantity and value must not be NULL");
+    // DEMInspector::switch_quantity_type (AuxClasses.cpp:58-103)
+    const std::string q = quantity;
+    int what, flavor;
+    if (q == "clump_max_z") {
+        what = INSPECT_CLUMP_MAX_Z;
+        flavor = INSPECT_MAX;
+    } else if (q == "clump_min_z") {
+        what = INSPECT_CLUMP_MIN_Z;
+        flavor = INSPECT_MIN;
+    } else if (q == "clump_max_absv") {
+        what = INSPECT_CLUMP_MAX_ABSV;
+        flavor = INSPECT_MAX;
+    } else if (q == "clump_volume") {
+        what = INSPECT_CLUMP_VOLUME;
+        flavor = INSPECT_SUM;
+        if (!c->haveVolumes) return c->fail(DEME_ERR_NOT_READY, "clump volumes are not uploaded");
+    } else {
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, q + " is not a known query type.");
+    }
+    const std::string reg = region ? region : "";
+    const bool all = reg.find_first_not_of(" \t\r\n") == std::string::npos;
+    if (!all && (!(deme_jitgen::whole_word(reg, "X") || deme_jitgen::whole_word(reg, "Y") ||
+                   deme_jitgen::whole_word(reg, "Z")) ||
+                 !deme_jitgen::whole_word(reg, "return")))
+        // the reference's check of the region code (AuxClasses.cpp:121-131)
+        return c->fail(DEME_ERR_INVALID_ARGUMENT,
+                       "the region of an inspector needs to return a bool that is a result of logical operations "
+                       "involving X, Y and Z");
+    const uint32_t n = what == INSPECT_CLUMP_VOLUME ? c->nClumps : c->nS;
+    const uint32_t nb = inspect_blocks(n);
+    DEME_HIP(c, c->inspPart.ensure((size_t)nb + 1));
+    if (what != INSPECT_CLUMP_VOLUME) DEME_HIP(c, c->ownerVol.ensure(1));
+    if (all) {
+        DEME_LAUNCH(c, "inspect",
+                    launch_inspect(c->kp, what, flavor, n, c->sph.p, c->own.p, c->ownerVol.p, c->sp.LBF,
+                                   c->inspPart.p, c->stream));
+    } else {
+        auto it = c->inspectJit.find(reg);
+        if (it == c->inspectJit.end()) {
+            hipDeviceProp_t prop;
+            DEME_HIP(c, hipGetDeviceProperties(&prop, c->device));
+            std::vector<char> code;
+            std::string log;
+            if (jit_compile(jit_inspect_source(reg), prop.gcnArchName, code, log))
+                return c->fail(DEME_ERR_JIT, "hipRTC compilation of the inspector region failed:\n" + log);
+            JitInspect k;
+            DEME_HIP(c, k.load(code));
+            it = c->inspectJit.emplace(reg, k).first;
+        }
+        const JitInspect& k = it->second;
+        DEME_LAUNCH(c, "inspect",
+                    k.launch(c->kp, what, flavor, n, c->sph.p, c->own.p, c->ownerVol.p, c->sp.LBF, c->inspPart.p,
+                             c->stream));
+    }
+    DEME_LAUNCH(c, "inspect_final", launch_inspect_final(flavor, nb, c->inspPart.p, c->inspPart.p + nb, c->stream));
+    double out = 0.0;
+    DEME_HIP(c, hipMemcpyAsync(&out, c->inspPart.p + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    *value = (float)out;
     return DEME_OK;
 }
 

@@ -68,6 +68,12 @@ struct __attribute__((aligned(16))) PlanePose {
 };
 static_assert(sizeof(PlanePose) == 48, "PlanePose must be 48 bytes");
 
+// Inspector quantities (DEMInspector::switch_quantity_type, AuxClasses.cpp:58-103) and reductions; the tile of the
+// inspection kernels (256 threads, 8 coalesced strides each).
+enum { INSPECT_CLUMP_MAX_Z = 0, INSPECT_CLUMP_MIN_Z = 1, INSPECT_CLUMP_MAX_ABSV = 2, INSPECT_CLUMP_VOLUME = 3 };
+enum { INSPECT_MAX = 0, INSPECT_MIN = 1, INSPECT_SUM = 2 };
+constexpr int INSPECT_BLOCK = 256, INSPECT_ITEMS = 8;
+
 // Kernel-visible simulation constants.
 struct KParams {
     double l, voxelSize, binSize;

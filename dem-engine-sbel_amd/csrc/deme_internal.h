@@ -17,6 +17,14 @@ using deme_dev::KParams;
 using deme_dev::OwnerRec;
 using deme_dev::PlanePose;
 using deme_dev::SphereRec;
+using deme_dev::INSPECT_BLOCK;
+using deme_dev::INSPECT_CLUMP_MAX_ABSV;
+using deme_dev::INSPECT_CLUMP_MAX_Z;
+using deme_dev::INSPECT_CLUMP_MIN_Z;
+using deme_dev::INSPECT_CLUMP_VOLUME;
+using deme_dev::INSPECT_MAX;
+using deme_dev::INSPECT_MIN;
+using deme_dev::INSPECT_SUM;
 
 // Uniform cell grid over the simulation world used to generate candidate pairs (edge >= largest interaction
 // distance, so partners lie in the 27 neighbouring cells).
@@ -149,6 +157,22 @@ struct JitOwner {
                       hipStream_t st) const;
 };
 std::string jit_owner_source(const std::vector<std::pair<std::string, std::string>>& values);
+
+// Inspection kernels (query.hip; regions through hipRTC). inspect_blocks(n) partials, then one block folds them.
+uint32_t inspect_blocks(uint32_t n);
+hipError_t launch_inspect(const KParams& P, int quantity, int flavor, uint32_t n, const SphereRec* sph,
+                          const OwnerRec* own, const float* vol, const float lbf[3], double* partial, hipStream_t st);
+hipError_t launch_inspect_final(int flavor, uint32_t nPart, const double* partial, double* out, hipStream_t st);
+struct JitInspect {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    void release();
+    hipError_t load(const std::vector<char>& code);
+    hipError_t launch(const KParams& P, int quantity, int flavor, uint32_t n, const SphereRec* sph,
+                      const OwnerRec* own, const float* vol, const float lbf[3], double* partial,
+                      hipStream_t st) const;
+};
+std::string jit_inspect_source(const std::string& region);
 bool jit_is_owner_key(const std::string& key);
 // built-in model bodies: 0 frictional Hertz-Mindlin (the ahead-of-time kernel's), 1 frictionless Hertz
 const char* jit_builtin_model(int which);

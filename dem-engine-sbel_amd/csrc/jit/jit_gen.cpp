@@ -90,6 +90,10 @@ void owner_substitutions(const std::string& velStrategy, const std::string& posS
     out.push_back({"_posPrescriptionStrategy_", posStrategy.empty() ? " " : posStrategy});
 }
 
+void inspect_substitutions(const std::string& region, Subst& out) {
+    out.push_back({"_inRegionPolicy_", region.empty() ? "return true;" : region});
+}
+
 static std::string exact(double v) {
     char b[64];
     std::snprintf(b, sizeof b, "%.17g", v);

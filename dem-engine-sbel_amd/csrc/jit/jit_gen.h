@@ -32,6 +32,10 @@ void param_substitutions(const deme_sim_params* p, Subst& out);
 void owner_substitutions(const std::string& velStrategy, const std::string& posStrategy,
                          const std::string& familyChangeRules, Subst& out);
 
+// Placeholder value of the inspection kernel: the region code of an inspector (a function body over X, Y, Z that
+// returns a bool, AuxClasses.cpp:121-139); empty: every entity counts.
+void inspect_substitutions(const std::string& region, Subst& out);
+
 // Replace every occurrence of each key in order; later entries with the same key as an earlier one win (overrides).
 std::string render(const std::string& tpl, const Subst& s);
 

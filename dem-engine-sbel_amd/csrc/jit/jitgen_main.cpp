@@ -2,6 +2,7 @@
 //   jitgen force <template> <model> <out.hip>   the force kernel with the default (frictional Hertz-Mindlin) model
 //                                               and the simulation constants read from the kernel parameters
 //   jitgen owner <template> <out.hip>           the owner update kernel with no motion strategies
+//   jitgen inspect <template> <out.hip>         the inspection kernels with a region that counts everything
 #include <fstream>
 #include <iostream>
 #include <sstream>
@@ -10,8 +11,9 @@
 
 int main(int argc, char** argv) {
     const bool owner = argc == 4 && std::string(argv[1]) == "owner";
-    if (!(owner || (argc == 5 && std::string(argv[1]) == "force"))) {
-        std::cerr << "usage: jitgen force <template> <model> <out.hip> | jitgen owner <template> <out.hip>\n";
+    const bool inspect = argc == 4 && std::string(argv[1]) == "inspect";
+    if (!(owner || inspect || (argc == 5 && std::string(argv[1]) == "force"))) {
+        std::cerr << "usage: jitgen force <template> <model> <out.hip> | jitgen owner|inspect <template> <out.hip>\n";
         return 2;
     }
     auto slurp = [](const char* path, std::string& s) {
@@ -23,13 +25,16 @@ int main(int argc, char** argv) {
         return true;
     };
     std::string tpl, model, err;
-    if (owner) {
+    if (owner || inspect) {
         if (!slurp(argv[2], tpl)) {
             std::cerr << "jitgen: cannot read inputs\n";
             return 1;
         }
         deme_jitgen::Subst s;
-        deme_jitgen::owner_substitutions("", "", "", s);
+        if (owner)
+            deme_jitgen::owner_substitutions("", "", "", s);
+        else
+            deme_jitgen::inspect_substitutions("", s);
         std::ofstream o(argv[3]);
         o << "// generated from " << argv[2] << " by jitgen; do not edit\n" << deme_jitgen::render(tpl, s);
         return o ? 0 : 1;

@@ -16,6 +16,7 @@
 namespace deme_jit_text {
 extern const char* const force_kernel_tpl;
 extern const char* const owner_kernel_tpl;
+extern const char* const inspect_kernel_tpl;
 extern const char* const jit_prelude_h;
 extern const char* const deme_device_h;
 extern const char* const deme_types_h;
@@ -63,6 +64,12 @@ std::string jit_owner_source(const std::vector<std::pair<std::string, std::strin
     }
     deme_jitgen::owner_substitutions(vel, pos, fam, s);
     return deme_jitgen::render(deme_jit_text::owner_kernel_tpl, s);
+}
+
+std::string jit_inspect_source(const std::string& region) {
+    deme_jitgen::Subst s;
+    deme_jitgen::inspect_substitutions(region, s);
+    return deme_jitgen::render(deme_jit_text::inspect_kernel_tpl, s);
 }
 
 bool jit_is_owner_key(const std::string& k) {
@@ -172,6 +179,31 @@ hipError_t JitOwner::launch(const KParams& P, uint32_t nO, uint32_t nClumps, uin
     return hipModuleLaunchKernel(fn, (nO + 255) / 256, 1, 1, 256, 1, 1, 0, st, args, nullptr);
 }
 
+void JitInspect::release() {
+    if (mod) (void)hipModuleUnload(mod);
+    mod = nullptr;
+    fn = nullptr;
+}
+
+hipError_t JitInspect::load(const std::vector<char>& code) {
+    release();
+    hipError_t e = hipModuleLoadData(&mod, code.data());
+    if (e != hipSuccess) {
+        mod = nullptr;
+        return e;
+    }
+    return hipModuleGetFunction(&fn, mod, "deme_inspect_kernel");
+}
+
+hipError_t JitInspect::launch(const KParams& P, int quantity, int flavor, uint32_t n, const SphereRec* sph,
+                              const OwnerRec* own, const float* vol, const float lbf[3], double* partial,
+                              hipStream_t st) const {
+    KParams p = P;
+    float lx = lbf[0], ly = lbf[1], lz = lbf[2];
+    void* args[] = {&p, &quantity, &flavor, &n, &sph, &own, &vol, &lx, &ly, &lz, &partial};
+    return hipModuleLaunchKernel(fn, inspect_blocks(n), 1, 1, INSPECT_BLOCK, 1, 1, 0, st, args, nullptr);
+}
+
 }  // namespace deme_int
 
 // ---- context-free entry points (render and compile; no device needed) ----
@@ -199,6 +231,17 @@ extern "C" int deme_jit_render_owner(const char* velStrategy, const char* posStr
     if (posStrategy) v.emplace_back("_posPrescriptionStrategy_", posStrategy);
     if (familyChangeRules) v.emplace_back("_familyChangeRules_", familyChangeRules);
     const std::string src = deme_int::jit_owner_source(v);
+    if (needed) *needed = src.size() + 1;
+    if (out && cap) {
+        const size_t n = std::min<size_t>(cap - 1, src.size());
+        std::memcpy(out, src.data(), n);
+        out[n] = '\0';
+    }
+    return DEME_OK;
+}
+
+extern "C" int deme_jit_render_inspect(const char* region, char* out, uint64_t cap, uint64_t* needed) {
+    const std::string src = deme_int::jit_inspect_source(region ? region : "");
     if (needed) *needed = src.size() + 1;
     if (out && cap) {
         const size_t n = std::min<size_t>(cap - 1, src.size());

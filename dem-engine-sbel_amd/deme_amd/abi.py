@@ -82,8 +82,8 @@ EXPORTED = [
     "deme_kernel_timing", "deme_set_beta", "deme_set_cell_scale", "deme_autotune", "deme_adjoint_forward",
     "deme_adjoint_backward", "deme_clear_contacts", "deme_set_force_model", "deme_jit_build",
     "deme_builtin_force_model", "deme_jit_render", "deme_jit_compile", "deme_set_step_graphs",
-    "deme_set_keep_contact_records",
-    "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner",
+    "deme_set_keep_contact_records", "deme_set_owner_volumes", "deme_inspect",
+    "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner", "deme_jit_render_inspect",
 ]
 
 _lib = None
@@ -143,9 +143,12 @@ def load(path: str = LIB_PATH):
         "deme_clear_contacts": ([vp], i32),
         "deme_set_step_graphs": ([vp, i32], i32),
         "deme_set_keep_contact_records": ([vp, i32], i32),
+        "deme_set_owner_volumes": ([vp, vp], i32),
+        "deme_inspect": ([vp, C.c_char_p, C.c_char_p, vp], i32),
         "deme_clear_motion_rules": ([vp], i32),
         "deme_change_family": ([vp, i32, i32], i32),
         "deme_jit_render_owner": ([C.c_char_p, C.c_char_p, C.c_char_p, vp, u64, C.POINTER(u64)], i32),
+        "deme_jit_render_inspect": ([C.c_char_p, vp, u64, C.POINTER(u64)], i32),
         "deme_set_force_model": ([vp, C.c_char_p, vp, i32], i32),
         "deme_jit_build": ([vp, vp, vp, i32], i32),
         "deme_builtin_force_model": ([i32], C.c_char_p),
@@ -238,6 +241,17 @@ def jit_render(model=None, wildcards=(), params=None) -> str:
         raise DemeError(st, "deme_jit_render failed")
     buf = C.create_string_buffer(need.value)
     lib.deme_jit_render(*args, buf, need.value, C.byref(need))
+    return buf.value.decode()
+
+
+def jit_render_inspect(region=None) -> str:
+    """The inspection kernels' source for a region (None: the whole domain)."""
+    lib = load()
+    arg = None if region is None else region.encode()
+    need = C.c_uint64()
+    lib.deme_jit_render_inspect(arg, None, 0, C.byref(need))
+    buf = C.create_string_buffer(need.value)
+    lib.deme_jit_render_inspect(arg, buf, need.value, C.byref(need))
     return buf.value.decode()
 
 
@@ -347,6 +361,18 @@ class DeviceSystem:
     def set_step_graphs(self, on=True):
         """Replay the steps between detection passes from a captured HIP graph, or launch them directly (default)."""
         self._chk(self.lib.deme_set_step_graphs(self.ctx, 1 if on else 0))
+
+    def set_owner_volumes(self, volume):
+        v = np.ascontiguousarray(volume, dtype=np.float32)
+        self._chk(self.lib.deme_set_owner_volumes(self.ctx, ptr(v)))
+
+    def inspect(self, quantity, region=None):
+        """Value of an inspector: quantity in clump_max_z, clump_min_z, clump_max_absv, clump_volume; region a function
+        body over X, Y, Z returning whether an entity counts (None: the whole domain)."""
+        out = np.zeros(1, np.float32)
+        reg = None if region is None else region.encode()
+        self._chk(self.lib.deme_inspect(self.ctx, quantity.encode(), reg, ptr(out)))
+        return np.float32(out[0])
 
     def keep_contact_records(self, on=True):
         """Store the per-contact records (download_forces) on the last step of every step() call."""

@@ -668,6 +668,26 @@ void DEMSolver::Initialize() {
     check(deme_upload_families(m_ctx, m_family_mask.data(), rules.data()), "deme_upload_families");
     // the contact file reports the last step's per-contact forces and points, as the reference keeps them
     check(deme_set_keep_contact_records(m_ctx, 1), "deme_set_keep_contact_records");
+    // clump volumes for the clump_volume inspector (the template volumes, AuxClasses.cpp:51-55)
+    {
+        std::vector<float> vol(nClumps);
+        for (size_t c = 0; c < nClumps; c++) vol[c] = m_templates.at(m_clump_template[c])->volume;
+        check(deme_set_owner_volumes(m_ctx, vol.data()), "deme_set_owner_volumes");
+    }
+    // owners of the tracked objects: clumps in batch order, then the external objects (dT.cpp:715-741)
+    {
+        std::vector<size_t> batchStart(1, 0);
+        for (const auto& b : cached_input_clump_batches) batchStart.push_back(batchStart.back() + b->GetNumClumps());
+        for (auto& t : m_tracked_objs) {
+            if (t->kind == DEMTrackedObj::Kind::CLUMP_BATCH) {
+                t->ownerID = (uint32_t)batchStart.at(t->load_order);
+                t->nSpanOwners = batchStart.at(t->load_order + 1) - batchStart.at(t->load_order);
+            } else {
+                t->ownerID = (uint32_t)(nClumps + t->load_order);
+                t->nSpanOwners = 1;
+            }
+        }
+    }
     // motion strategies and family change rules (equipFamilyOnFlyChanges, APIPrivate.cpp:980-1003)
     std::string changeRules;
     for (const auto& fc : m_family_changes) {
@@ -833,6 +853,79 @@ void DEMSolver::SetOwnerOriQ(uint32_t i, float4 q) {
     h_qz.at(i) = q.z;
     m_mirror_dirty = true;
 }
+
+// ---------------------------------------------------------------------------------------------------------------
+// inspectors and trackers
+// ---------------------------------------------------------------------------------------------------------------
+DEMInspector::DEMInspector(DEMSolver* s, const std::string& q, const std::string& r, bool all)
+    : sys(s), quantity(q), region(r), all_domain(all) {
+    // DEMInspector::switch_quantity_type (AuxClasses.cpp:58-103)
+    if (q != "clump_max_z" && q != "clump_min_z" && q != "clump_max_absv" && q != "clump_volume")
+        throw std::runtime_error(q + " is not a known query type.");
+}
+
+float DEMInspector::GetValue() {
+    if (!sys->GetInitStatus())
+        throw std::runtime_error(
+            "Inspector should only be initialized or used after the simulation system is initialized (because it "
+            "uses device-side data)!");
+    sys->pushHostState();
+    float v = 0.f;
+    const int st = deme_inspect(sys->GetContext(), quantity.c_str(), all_domain ? nullptr : region.c_str(), &v);
+    if (st != DEME_OK) {
+        std::string what = deme_last_error(sys->GetContext());
+        if (st == DEME_ERR_INVALID_ARGUMENT && !all_domain)
+            what = "One of your inspectors is set to query a specific region, but the domain is not properly defined. "
+                   "It needs to return a bool variable that is a result of logical operations involving X, Y and Z.";
+        throw std::runtime_error("Inspector " + quantity + ": " + what);
+    }
+    return v;
+}
+
+std::shared_ptr<DEMInspector> DEMSolver::CreateInspector(const std::string& quantity) {
+    return std::make_shared<DEMInspector>(this, quantity, " ", true);
+}
+std::shared_ptr<DEMInspector> DEMSolver::CreateInspector(const std::string& quantity, const std::string& region) {
+    return std::make_shared<DEMInspector>(this, quantity, region, false);
+}
+
+std::shared_ptr<DEMTracker> DEMSolver::Track(std::shared_ptr<DEMExternObj>& obj) {
+    auto t = std::make_shared<DEMTrackedObj>();
+    t->kind = DEMTrackedObj::Kind::EXTERNAL;
+    t->load_order = obj->load_order;
+    m_tracked_objs.push_back(t);
+    auto tr = std::make_shared<DEMTracker>(this);
+    tr->obj = t;
+    return tr;
+}
+std::shared_ptr<DEMTracker> DEMSolver::Track(std::shared_ptr<DEMClumpBatch>& obj) {
+    auto t = std::make_shared<DEMTrackedObj>();
+    t->kind = DEMTrackedObj::Kind::CLUMP_BATCH;
+    t->load_order = obj->load_order;
+    m_tracked_objs.push_back(t);
+    auto tr = std::make_shared<DEMTracker>(this);
+    tr->obj = t;
+    return tr;
+}
+
+uint32_t DEMTracker::owner(size_t offset) const {
+    if (!sys->GetInitStatus()) throw std::runtime_error("A tracker is used before the system is initialized.");
+    if (offset >= obj->nSpanOwners)
+        throw std::runtime_error("Tracker offset " + std::to_string(offset) + " is beyond the " +
+                                 std::to_string(obj->nSpanOwners) + " owners it tracks.");
+    return obj->ownerID + (uint32_t)offset;
+}
+float3 DEMTracker::Pos(size_t offset) { return sys->GetOwnerPosition(owner(offset)); }
+float3 DEMTracker::AngVelLocal(size_t offset) { return sys->GetOwnerAngVel(owner(offset)); }
+float3 DEMTracker::Vel(size_t offset) { return sys->GetOwnerVelocity(owner(offset)); }
+float4 DEMTracker::OriQ(size_t offset) { return sys->GetOwnerOriQ(owner(offset)); }
+unsigned int DEMTracker::GetFamily(size_t offset) { return sys->GetOwnerFamily(owner(offset)); }
+float3 DEMTracker::ContactAcc(size_t offset) { return sys->GetOwnerAcc(owner(offset)); }
+float3 DEMTracker::ContactAngAccLocal(size_t offset) { return sys->GetOwnerAngAcc(owner(offset)); }
+void DEMTracker::SetPos(float3 pos, size_t offset) { sys->SetOwnerPosition(owner(offset), pos); }
+void DEMTracker::SetAngVel(float3 w, size_t offset) { sys->SetOwnerAngVel(owner(offset), w); }
+void DEMTracker::SetVel(float3 v, size_t offset) { sys->SetOwnerVelocity(owner(offset), v); }
+void DEMTracker::SetOriQ(float4 q, size_t offset) { sys->SetOwnerOriQ(owner(offset), q); }
 
 size_t DEMSolver::GetNumContacts() {
     deme_counts c;

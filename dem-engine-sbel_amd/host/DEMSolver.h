@@ -167,6 +167,54 @@ struct DEMForceModel {
     void SetPerContactWildcards(const std::set<std::string>& names);
 };
 
+class DEMSolver;
+
+// Inspector (AuxClasses.h:24-74): the reduced value of a quantity over the whole domain or over a region given as C++
+// statements reading the world position X, Y, Z and returning a bool. Quantities: clump_max_z, clump_min_z,
+// clump_max_absv, clump_volume (the sum of template volumes, DEMClumpTemplate::SetVolume). An unknown quantity throws
+// at creation; a malformed region throws at the first GetValue, which compiles it.
+class DEMInspector {
+  public:
+    DEMInspector(DEMSolver* sys, const std::string& quantity, const std::string& region, bool all_domain);
+    float GetValue();
+
+  private:
+    DEMSolver* sys;
+    std::string quantity, region;
+    bool all_domain;
+};
+
+// The owners a tracker follows (Structs.h:562-579): a batch of clumps or one external object.
+struct DEMTrackedObj {
+    enum class Kind { CLUMP_BATCH, EXTERNAL } kind = Kind::CLUMP_BATCH;
+    size_t load_order = 0;
+    uint32_t ownerID = std::numeric_limits<uint32_t>::max();  // assigned by Initialize
+    size_t nSpanOwners = 1;
+};
+
+// Tracker (AuxClasses.h:77-121): get and set the state of the owners of a tracked batch or object; offset picks the
+// owner within a batch.
+class DEMTracker {
+  public:
+    explicit DEMTracker(DEMSolver* sys) : sys(sys) {}
+    std::shared_ptr<DEMTrackedObj> obj;
+    float3 Pos(size_t offset = 0);
+    float3 AngVelLocal(size_t offset = 0);
+    float3 Vel(size_t offset = 0);
+    float4 OriQ(size_t offset = 0);
+    unsigned int GetFamily(size_t offset = 0);
+    float3 ContactAcc(size_t offset = 0);
+    float3 ContactAngAccLocal(size_t offset = 0);
+    void SetPos(float3 pos, size_t offset = 0);
+    void SetAngVel(float3 angVel, size_t offset = 0);
+    void SetVel(float3 vel, size_t offset = 0);
+    void SetOriQ(float4 oriQ, size_t offset = 0);
+
+  private:
+    DEMSolver* sys;
+    uint32_t owner(size_t offset) const;
+};
+
 class DEMSolver {
   public:
     explicit DEMSolver(unsigned int nGPUs = 1);
@@ -274,6 +322,13 @@ class DEMSolver {
     std::shared_ptr<DEMExternObj> AddBCPlane(const float3 pos, const float3 normal,
                                              const std::shared_ptr<DEMMaterial>& material);
 
+    // ---- inspectors and trackers (API.h:240-249) ----
+    std::shared_ptr<DEMInspector> CreateInspector(const std::string& quantity = "clump_max_z");
+    std::shared_ptr<DEMInspector> CreateInspector(const std::string& quantity, const std::string& region);
+    std::shared_ptr<DEMTracker> Track(std::shared_ptr<DEMExternObj>& obj);
+    std::shared_ptr<DEMTracker> Track(std::shared_ptr<DEMClumpBatch>& obj);
+    bool GetInitStatus() const { return sys_initialized; }
+
     // ---- running (API.h:374-386) ----
     void Initialize();
     void DoDynamics(double thisCallDuration);
@@ -307,6 +362,8 @@ class DEMSolver {
     deme_ctx* GetContext() { return m_ctx; }
 
   private:
+    friend class DEMInspector;
+    friend class DEMTracker;
     struct FamilyPrescription {
         unsigned int family;
         std::string linPosX = "none", linPosY = "none", linPosZ = "none";
@@ -352,6 +409,7 @@ class DEMSolver {
     std::vector<std::shared_ptr<DEMClumpTemplate>> m_templates;
     std::vector<std::shared_ptr<DEMClumpBatch>> cached_input_clump_batches;
     std::vector<std::shared_ptr<DEMExternObj>> cached_extern_objs;
+    std::vector<std::shared_ptr<DEMTrackedObj>> m_tracked_objs;
     std::vector<FamilyPrescription> m_input_family_prescription;
     struct FamilyChange {
         unsigned int from, to;

@@ -10,6 +10,10 @@
 //     Single spheres and three-sphere clumps in a box settle; then the sphere, clump and contact files are written
 //     into <dir> with several content settings, with the raw state they were written from (state.json), and the
 //     clump file is read back and used to start a second simulation, whose clump file is written too.
+//   facade_demo inspect
+//     Balls settle on the floor while two more fall from above; inspectors (whole domain and a region) and trackers
+//     (of the falling pair and of an added plane) are read, one tracked ball is moved with SetPos, and the
+//     inspectors are read again. Prints one JSON line.
 //   facade_demo readcsv <file>
 //     Reads <file> with ReadClumpXyzFromCsv and ReadClumpQuatFromCsv and prints the values as float bit patterns.
 #include <cmath>
@@ -242,10 +246,82 @@ int run_io(int n, double seconds, const std::string& dir) {
     return 0;
 }
 
+int run_inspect() {
+    DEMSolver sim;
+    sim.InstructBoxDomainDimension(0.2f, 0.2f, 0.2f);
+    sim.SetCoordSysOrigin("center");
+    auto mat = sim.LoadMaterial({{"E", 1e7f}, {"nu", 0.3f}, {"CoR", 0.3f}, {"mu", 0.4f}, {"Crr", 0.f}});
+    sim.InstructBoxDomainBoundingBC("top_open", mat);
+    const float r = 0.004f, pi = 3.14159265f, vol = 4.f / 3.f * pi * r * r * r;
+    auto ball = sim.LoadSphereType(2600.f * vol, r, mat);
+    ball->SetVolume(vol);
+    std::vector<float3> rest, fall = {make_float3(0.03f, 0.01f, 0.02f), make_float3(-0.05f, 0.02f, 0.04f)};
+    for (int i = 0; i < 18; i++) rest.push_back(make_float3(-0.08f + 0.02f * (i % 9), -0.08f + 0.02f * (i / 9), -0.1f + r));
+    auto b1 = sim.AddClumps(std::vector<std::shared_ptr<DEMClumpTemplate>>(rest.size(), ball), rest);
+    auto b2 = sim.AddClumps(std::vector<std::shared_ptr<DEMClumpTemplate>>(fall.size(), ball), fall);
+    auto wall = sim.AddBCPlane(make_float3(0, 0.09f, 0), make_float3(0, -1, 0), mat);
+    wall->SetInitPos(make_float3(0.01f, 0.02f, 0.03f));
+    auto tFall = sim.Track(b2);
+    auto tWall = sim.Track(wall);
+    sim.SetInitTimeStep(1e-5);
+    sim.SetCDUpdateFreq(1);
+    sim.SetGravitationalAcceleration(make_float3(0, 0, -9.81f));
+    auto maxZ = sim.CreateInspector("clump_max_z");
+    auto minZ = sim.CreateInspector("clump_min_z");
+    auto maxV = sim.CreateInspector("clump_max_absv");
+    auto volAll = sim.CreateInspector("clump_volume");
+    auto volLeft = sim.CreateInspector("clump_volume", "return X < -0.01f;");
+    auto maxZLow = sim.CreateInspector("clump_max_z", "return Z < -0.09f;");
+    sim.Initialize();
+    sim.DoDynamics(0.05);
+    int nLeft = 0;
+    double vmax = 0, zmax = -1;
+    for (size_t o = 0; o < sim.GetNumClumps(); o++) {
+        const float3 p = sim.GetOwnerPosition((uint32_t)o), v = sim.GetOwnerVelocity((uint32_t)o);
+        nLeft += p.x < -0.01f;  // the grid has a column at x = 0: keep the boundary off it
+        vmax = std::fmax(vmax, std::sqrt((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z));
+        zmax = std::fmax(zmax, (double)p.z);
+    }
+    const float3 f0 = tFall->Pos(0), f1 = tFall->Pos(1), o18 = sim.GetOwnerPosition(18), w = tWall->Pos();
+    const float before[6] = {maxZ->GetValue(), minZ->GetValue(), maxV->GetValue(), volAll->GetValue(),
+                             volLeft->GetValue(), maxZLow->GetValue()};
+    tFall->SetPos(make_float3(0.05f, 0.05f, 0.08f), 1);
+    tFall->SetVel(make_float3(0.f, 0.f, 0.f), 1);
+    const float after = maxZ->GetValue();
+    std::string err;
+    try {
+        sim.CreateInspector("clump_volume", "X < 0")->GetValue();
+    } catch (const std::runtime_error& e) {
+        err = "region";
+    }
+    try {
+        sim.CreateInspector("clump_max_y");
+    } catch (const std::runtime_error& e) {
+        err += ",quantity";
+    }
+    std::printf("{\"max_z\": %.9g, \"min_z\": %.9g, \"max_absv\": %.9g, \"volume\": %.9g, \"volume_left\": %.9g, "
+                "\"max_z_low\": %.9g, \"max_z_after\": %.9g, \"vol\": %.9g, \"r\": %.9g, \"n\": %zu, "
+                "\"n_left\": %d, \"vmax\": %.9g, \"zmax\": %.9g, \"fall0\": [%.9g, %.9g, %.9g], "
+                "\"fall1\": [%.9g, %.9g, %.9g], \"owner18\": [%.9g, %.9g, %.9g], \"wall\": [%.9g, %.9g, %.9g], "
+                "\"wall_owner\": %u, \"errors\": \"%s\"}\n",
+                before[0], before[1], before[2], before[3], before[4], before[5], after, vol, r, sim.GetNumClumps(),
+                nLeft, vmax, zmax, f0.x, f0.y, f0.z, f1.x, f1.y, f1.z, o18.x, o18.y, o18.z, w.x, w.y, w.z,
+                tWall->obj->ownerID, err.c_str());
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc == 3 && std::string(argv[1]) == "readcsv") return read_csv(argv[2]);
+    if (argc == 2 && std::string(argv[1]) == "inspect") {
+        try {
+            return run_inspect();
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "facade_demo: %s\n", e.what());
+            return 1;
+        }
+    }
     if (argc == 5 && std::string(argv[1]) == "io") {
         try {
             return run_io(std::atoi(argv[2]), std::atof(argv[3]), argv[4]);
@@ -256,7 +332,7 @@ int main(int argc, char** argv) {
     }
     if (argc != 4) {
         std::fprintf(stderr, "usage: facade_demo <hertz|frictionless|linear|prescribed> <n_spheres> <seconds> | "
-                             "io <n_clumps> <seconds> <dir> | readcsv <file>\n");
+                             "io <n_clumps> <seconds> <dir> | inspect | readcsv <file>\n");
         return 2;
     }
     const std::string model = argv[1];

@@ -180,6 +180,8 @@ int deme_jit_compile(const char* source, char* log, uint64_t cap);
 /* The owner update kernel's source for the given strategies and rules (each nullable). */
 int deme_jit_render_owner(const char* velStrategy, const char* posStrategy, const char* familyChangeRules, char* out,
                           uint64_t cap, uint64_t* needed);
+/* The inspection kernels' source for a region (nullable: the whole domain), as deme_inspect compiles it. */
+int deme_jit_render_inspect(const char* region, char* out, uint64_t cap, uint64_t* needed);
 
 /* Adjoint of the step (no counterpart in the reference, which has no differentiation; for calibration callers that
  * fit material parameters or initial velocities to observed states). Systems of single-sphere owners (each sphere
@@ -205,6 +207,21 @@ int deme_download_wildcards(deme_ctx* ctx, float* w);
 int deme_upload_wildcards(deme_ctx* ctx, const float* w);
 int deme_get_counts(deme_ctx* ctx, deme_counts* out);
 int deme_set_time(deme_ctx* ctx, double t);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Inspectors (DEMInspector::GetValue, AuxClasses.cpp:105-110, over dT::inspectCall, dT.cpp:1660-1720).
+ * ------------------------------------------------------------------------------------------------------------- */
+/* Volume of each clump (nClumps floats; the template volumes of DEMClumpTemplate::SetVolume), summed by the
+ * "clump_volume" inspector. Call after deme_upload_templates. */
+int deme_set_owner_volumes(deme_ctx* ctx, const float* volume);
+/* Reduces a quantity over the current state into *value: "clump_max_z" (max over spheres of Z + r), "clump_min_z"
+ * (min of Z - r), "clump_max_absv" (max of the reference's estimate of a sphere centre's speed) or "clump_volume" (sum
+ * over clumps). region: NULL or blank for the whole domain, else a function body over the world position X, Y, Z
+ * (float; sphere centres, clump centres for the volume) that returns whether the entity counts, compiled once by
+ * hipRTC (AuxClasses.cpp:114-148). Unknown quantities, and regions that do not return or read none of X, Y, Z, give
+ * DEME_ERR_INVALID_ARGUMENT; with nothing counted the value is 0 (sum), -FLT_MAX (max) or FLT_MAX (min). Sums are
+ * formed in double in a fixed order; every value repeats bit for bit. Synchronises the context's stream. */
+int deme_inspect(deme_ctx* ctx, const char* quantity, const char* region, float* value);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Measurement: per-kernel HIP-event timing on the context's stream (no counterpart in the reference, which only has

@@ -108,3 +108,26 @@ def test_output_files_match_the_restated_writers_and_restart(tmp_path):
     for ra, rb in zip(a, b):
         assert ra[7] == rb[7]
         assert np.allclose([float(x) for x in ra[:7]], [float(x) for x in rb[:7]], rtol=1e-5, atol=1e-6)
+
+
+def test_inspectors_and_trackers_through_the_facade():
+    print("facade_demo inspect", flush=True)
+    out = subprocess.run([DEMO, "inspect"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    d = json.loads(out.stdout.strip().splitlines()[-1])
+    r, vol = d["r"], d["vol"]
+    assert d["n"] == 20
+    # 18 balls rest on the floor at z = -0.1 (overlaps of micrometres) while two fall from above
+    assert -0.1 - 1e-4 <= d["min_z"] <= -0.1 + 1e-6
+    assert d["max_z"] == pytest.approx(d["zmax"] + r, abs=1e-6)
+    assert d["max_absv"] == pytest.approx(d["vmax"], rel=1e-5)  # single spheres: the owner speed
+    assert d["volume"] == pytest.approx(20 * vol, rel=1e-6)
+    assert d["volume_left"] == pytest.approx(d["n_left"] * vol, rel=1e-6) and 0 < d["n_left"] < 20
+    assert d["max_z_low"] == pytest.approx(-0.1 + 2 * r, abs=1e-4)  # the resting layer only
+    # the tracker of the second batch starts at its first owner; SetPos through it moves what the inspectors see
+    assert d["fall0"] == d["owner18"]
+    assert d["max_z_after"] == pytest.approx(0.08 + r, abs=1e-6)
+    # the tracked plane is the first external object, after the 20 clumps, at its initial position
+    assert d["wall_owner"] == 20
+    assert d["wall"] == pytest.approx([0.01, 0.02, 0.03], abs=1e-6)
+    assert d["errors"] == "region,quantity"

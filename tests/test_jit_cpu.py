@@ -94,3 +94,18 @@ def test_motion_strategies_and_family_rules_compile():
     assert st == 0, log
     st, log = abi.jit_compile(abi.jit_render_owner("case 3: { vX = undefined_thing; break; }"))
     assert st == 6 and "undefined_thing" in log
+
+
+def test_shipped_inspection_kernels_are_the_template_over_the_whole_domain():
+    gen = open(os.path.join(ROOT, "dem-engine-sbel_amd", "_build", "inspect_aot.hip")).read()
+    assert gen.split("\n", 1)[1] == abi.jit_render_inspect()
+    assert "_inRegionPolicy_" not in gen and "return true;" in gen
+
+
+def test_inspector_regions_compile_and_errors_are_reported():
+    src = abi.jit_render_inspect("float r2 = X * X + Y * Y; return r2 < 0.01f && Z > -0.05f;")
+    assert "float r2 = X * X + Y * Y;" in src
+    st, log = abi.jit_compile(src)
+    assert st == 0, log
+    st, log = abi.jit_compile(abi.jit_render_inspect("return W > 0;"))
+    assert st == 6 and "W" in log
