@@ -46,6 +46,25 @@ struct DBuf {
         p = nullptr;
         cap = 0;
     }
+    // grows to hold n elements, keeping the first `keep` (synchronises the stream when it reallocates)
+    hipError_t grow_keep(size_t n, size_t keep, hipStream_t st) {
+        if (n <= cap && p) return hipSuccess;
+        DBuf<T> nb;
+        hipError_t e = nb.ensure(n);
+        if (e != hipSuccess) return e;
+        if (keep && p) {
+            e = hipMemcpyAsync(nb.p, p, keep * sizeof(T), hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                nb.release();
+                return e;
+            }
+        }
+        release();
+        p = nb.p;
+        cap = nb.cap;
+        return hipSuccess;
+    }
 };
 
 struct TimedLaunch {
@@ -91,6 +110,8 @@ struct deme_ctx {
     uint32_t maxMatSph = 0, maxMatAnal = 0, maxMat = 0;
     uint64_t nBins = 0;
     std::vector<float> hostMass;
+    std::vector<float> hostRadius;  // sphere radii (the host's copy, for the cell grid after size changes)
+    bool needDetect = false;        // clumps were appended: the next step starts with a detection pass
     // scenes of a batch: per owner scene index; analytical components grouped by scene (planeStart)
     std::vector<uint32_t> hostScene, hostAnalOwner, hostAnalScene;  // analScene empty: the owner's scene
     uint32_t nScenes = 1;
@@ -514,6 +535,8 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     c->maxMatSph = maxMat;
     c->maxMat = std::max(c->maxMatSph, c->maxMatAnal);
     c->maxRadius = maxR;
+    c->hostRadius.assign(sphRadius, sphRadius + nS);
+    c->needDetect = false;
     c->haveTemplates = true;
     c->haveState = false;
     c->havePrev = c->contactsReady = c->fresh = false;
@@ -832,6 +855,7 @@ int deme_cd_sort_remap(deme_ctx* c) {
                                    c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->bRowStart.p, c->bPart.p,
                                    c->mirror.p, c->stream));
     c->contactsReady = true;
+    c->needDetect = false;
     c->fresh = true;
     c->forcesFull = false;
     c->cdStage = 4;
@@ -974,6 +998,7 @@ int deme_cd_run(deme_ctx* c) {
                                    c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->bRowStart.p, c->bPart.p,
                                    c->mirror.p, c->stream));
     c->contactsReady = true;
+    c->needDetect = false;
     c->fresh = true;
     c->forcesFull = false;
     c->cdStage = 0;
@@ -1056,6 +1081,7 @@ static int force_pass(deme_ctx* c, bool full) {
     if (r) return r;
     if (c->jitActive && c->jitStale && (r = jit_rebuild(c))) return r;
     if (!c->contactsReady) return c->fail(DEME_ERR_NOT_READY, "contact detection must run before the force pass");
+    if (c->needDetect) return c->fail(DEME_ERR_NOT_READY, "clumps were appended: run contact detection first");
     const int cu = c->cur, pv = cu ^ 1;
     const bool remap = c->fresh;
     if (full) DEME_HIP(c, c->frc.ensure(c->C));
@@ -1150,7 +1176,7 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
     if (r) return r;
     const uint64_t f = updateFreq > 0 ? (uint64_t)updateFreq : 1;
     for (int k = 0; k < nSteps; k++) {
-        const bool detect = c->nSteps % f == 0 || !c->contactsReady;
+        const bool detect = c->nSteps % f == 0 || !c->contactsReady || c->needDetect;
         if (detect) {
             r = deme_cd_run(c);
             if (r) return r;
@@ -1672,6 +1698,203 @@ int deme_get_counts(deme_ctx* c, deme_counts* out) {
     out->nSteps = c->nSteps;
     out->time = c->time;
     out->nCells = c->nCells;
+    return DEME_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Mid-simulation changes: appended clumps, clump sizes
+// ---------------------------------------------------------------------------------------------------------------
+int deme_append_clumps(deme_ctx* c, uint32_t nNewSph, const uint32_t* sphOwner, const float* sphRadius,
+                       const float* sphRelPos, const uint8_t* sphMat, uint32_t nNew, const float* ownerMass,
+                       const float* ownerMoi, const deme_owner_state* s) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->nScenes != 1) return c->fail(DEME_ERR_UNSUPPORTED, "clumps cannot be appended to a batch of scenes");
+    if (nNew == 0) return DEME_OK;
+    if (!s || !s->voxelID || !s->locX || !s->locY || !s->locZ || !ownerMass || !ownerMoi ||
+        (nNewSph && (!sphOwner || !sphRadius || !sphRelPos || !sphMat)))
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, "appended clumps need geometry, mass, MOI and positions");
+    const uint32_t nS0 = c->nS, nO0 = c->nO, nC0 = c->nClumps, nAO = c->nO - c->nClumps;
+    if ((uint64_t)nS0 + nNewSph > 0xFFFFFFF0ull || (uint64_t)nO0 + nNew > 0xFFFFFFF0ull)
+        return c->fail(DEME_ERR_CAPACITY, "too many spheres or owners");
+    // new component spheres follow the existing ones; their owners follow the existing clumps
+    std::vector<SphereRec> hs(nNewSph);
+    std::vector<uint32_t> cnt(nNew, 0);
+    uint32_t maxMat = c->maxMatSph;
+    float maxR = c->maxRadius;
+    for (uint32_t k = 0; k < nNewSph; k++) {
+        if (sphOwner[k] >= nNew) return c->fail(DEME_ERR_INVALID_ARGUMENT, "sphere owner is not an appended clump");
+        if (k > 0 && sphOwner[k] < sphOwner[k - 1])
+            return c->fail(DEME_ERR_INVALID_ARGUMENT, "spheres must be grouped by owner in ascending order");
+        if (!std::isfinite(sphRadius[k]) || sphRadius[k] < 0.f)
+            return c->fail(DEME_ERR_INVALID_ARGUMENT, "sphere radii must be finite and non-negative");
+        if (sphMat[k] >= c->nM) return c->fail(DEME_ERR_INVALID_ARGUMENT, "a material index exceeds the material table");
+        SphereRec& h = hs[k];
+        h.owner = nC0 + sphOwner[k];
+        h.r = sphRadius[k];
+        h.rx = sphRelPos[3 * k];
+        h.ry = sphRelPos[3 * k + 1];
+        h.rz = sphRelPos[3 * k + 2];
+        h.mat = sphMat[k];
+        h.pad0 = h.pad1 = 0;
+        cnt[sphOwner[k]]++;
+        maxMat = std::max<uint32_t>(maxMat, sphMat[k]);
+        maxR = std::max(maxR, sphRadius[k]);
+    }
+    const uint32_t nS = nS0 + nNewSph, nO = nO0 + nNew, nC = nC0 + nNew;
+    // owner records: [old clumps][new clumps][external objects moved up by nNew]
+    std::vector<OwnerRec> tail(nNew + nAO);
+    std::vector<float4> tailMoi(nNew + nAO);
+    std::vector<float> accOld((size_t)6 * nO0);
+    if (nAO) {
+        DEME_HIP(c, hipMemcpyAsync(tail.data() + nNew, c->own.p + nC0, nAO * sizeof(OwnerRec), hipMemcpyDeviceToHost,
+                                   c->stream));
+        DEME_HIP(c, hipMemcpyAsync(tailMoi.data() + nNew, c->moi.p + nC0, nAO * sizeof(float4), hipMemcpyDeviceToHost,
+                                   c->stream));
+    }
+    if (nO0) DEME_HIP(c, hipMemcpyAsync(accOld.data(), c->acc.p, accOld.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<deme_anal_obj> anal(c->nA);
+    if (c->nA)
+        DEME_HIP(c, hipMemcpyAsync(anal.data(), c->anal.p, c->nA * sizeof(deme_anal_obj), hipMemcpyDeviceToHost,
+                                   c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    for (uint32_t k = 0; k < nNew; k++) {
+        OwnerRec& o = tail[k];
+        std::memset(&o, 0, sizeof(o));
+        o.voxel = s->voxelID[k];
+        o.lx = s->locX[k];
+        o.ly = s->locY[k];
+        o.lz = s->locZ[k];
+        o.fam = s->family ? s->family[k] : 0;
+        o.qw = s->oriQw ? s->oriQw[k] : 1.f;
+        o.qx = s->oriQx ? s->oriQx[k] : 0.f;
+        o.qy = s->oriQy ? s->oriQy[k] : 0.f;
+        o.qz = s->oriQz ? s->oriQz[k] : 0.f;
+        o.vx = s->vX ? s->vX[k] : 0.f;
+        o.vy = s->vY ? s->vY[k] : 0.f;
+        o.vz = s->vZ ? s->vZ[k] : 0.f;
+        o.wx = s->omgX ? s->omgX[k] : 0.f;
+        o.wy = s->omgY ? s->omgY[k] : 0.f;
+        o.wz = s->omgZ ? s->omgZ[k] : 0.f;
+        o.mass = ownerMass[k];
+        o.scene = 0;
+        tailMoi[k] = make_float4(ownerMoi[3 * k], ownerMoi[3 * k + 1], ownerMoi[3 * k + 2], 0.f);
+    }
+    // accelerations are kept component-major over all owners: re-laid for the new owner count
+    std::vector<float> accNew((size_t)6 * nO, 0.f);
+    for (int q = 0; q < 6; q++)
+        for (uint32_t o = 0; o < nO0; o++)
+            accNew[(size_t)q * nO + (o < nC0 ? o : o + nNew)] = accOld[(size_t)q * nO0 + o];
+    std::vector<uint32_t> start(nNew);
+    for (uint32_t k = 0, at = nS0; k < nNew; k++) {
+        at += cnt[k];
+        start[k] = at;  // ownerSphStart[nC0 + 1 + k]
+    }
+    for (auto& a : anal) a.owner += nNew;
+
+    DEME_HIP(c, c->sph.grow_keep(nS, nS0, c->stream));
+    DEME_HIP(c, c->own.grow_keep(nO, nC0, c->stream));
+    DEME_HIP(c, c->moi.grow_keep(nO, nC0, c->stream));
+    DEME_HIP(c, c->ownerSphStart.grow_keep(nC + 1, nC0 + 1, c->stream));
+    DEME_HIP(c, c->rowStart[c->cur].grow_keep(nS + 1, c->contactsReady ? nS0 + 1 : 0, c->stream));
+    DEME_HIP(c, c->rowStart[c->cur ^ 1].ensure(nS + 1));
+    DEME_HIP(c, c->acc.ensure((size_t)6 * nO));
+    DEME_HIP(c, c->numBins.ensure(nS));
+    DEME_HIP(c, c->numAnal.ensure(nS));
+    DEME_HIP(c, c->rec.ensure(nS));
+    DEME_HIP(c, c->rowCount.ensure(nS));
+    DEME_HIP(c, c->bCnt.ensure(nS + 1));
+    DEME_HIP(c, c->bRowStart.ensure(nS + 1));
+    DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>({c->nBins, (uint64_t)nS, 1u << 22}))));
+    if (nNewSph)
+        DEME_HIP(c, hipMemcpyAsync(c->sph.p + nS0, hs.data(), nNewSph * sizeof(SphereRec), hipMemcpyHostToDevice,
+                                   c->stream));
+    DEME_HIP(c, hipMemcpyAsync(c->own.p + nC0, tail.data(), tail.size() * sizeof(OwnerRec), hipMemcpyHostToDevice,
+                               c->stream));
+    DEME_HIP(c, hipMemcpyAsync(c->moi.p + nC0, tailMoi.data(), tailMoi.size() * sizeof(float4), hipMemcpyHostToDevice,
+                               c->stream));
+    DEME_HIP(c, hipMemcpyAsync(c->ownerSphStart.p + nC0 + 1, start.data(), nNew * 4, hipMemcpyHostToDevice, c->stream));
+    DEME_HIP(c, hipMemcpyAsync(c->acc.p, accNew.data(), accNew.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (c->nA)
+        DEME_HIP(c, hipMemcpyAsync(c->anal.p, anal.data(), c->nA * sizeof(deme_anal_obj), hipMemcpyHostToDevice,
+                                   c->stream));
+    DEME_HIP(c, hipMemsetAsync(c->rowCount.p, 0, (size_t)nS * 4, c->stream));
+    DEME_HIP(c, hipMemsetAsync(c->bCnt.p, 0, (size_t)(nS + 1) * 4, c->stream));
+    // the current list becomes the previous one at the next detection pass, which maps histories by sphere id: the
+    // appended spheres have empty rows in it
+    if (c->contactsReady)
+        DEME_LAUNCH(c, "fill", launch_fill_u32(c->rowStart[c->cur].p + nS0 + 1, nNewSph, (uint32_t)c->C, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+
+    c->hostMass.insert(c->hostMass.begin() + nC0, ownerMass, ownerMass + nNew);
+    c->hostScene.insert(c->hostScene.begin() + nC0, nNew, 0u);
+    for (auto& o : c->hostAnalOwner) o += nNew;
+    c->hostRadius.insert(c->hostRadius.end(), sphRadius, sphRadius + nNewSph);
+    bool single = c->singleSpheres && nNewSph == nNew;
+    for (uint32_t k = 0; k < nNewSph && single; k++)
+        single = sphOwner[k] == k && sphRelPos[3 * k] == 0.f && sphRelPos[3 * k + 1] == 0.f &&
+                 sphRelPos[3 * k + 2] == 0.f;
+    c->singleSpheres = single;
+    c->nS = nS;
+    c->kp.nS = nS;
+    c->nO = nO;
+    c->nClumps = nC;
+    c->maxMatSph = maxMat;
+    c->maxMat = std::max(c->maxMatSph, c->maxMatAnal);
+    if (maxR > c->maxRadius) c->nCells = 0;  // re-derive the cell grid for the larger spheres
+    c->maxRadius = maxR;
+    c->scenesDirty = true;
+    c->haveVolumes = false;
+    c->needDetect = true;
+    c->termsReady = c->forcesFull = false;
+    c->adjReady = false;
+    c->graphGen++;
+    return DEME_OK;
+}
+
+int deme_change_clump_sizes(deme_ctx* c, uint32_t n, const uint32_t* ids, const float* factors) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (n && (!ids || !factors)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "ids and factors must not be NULL");
+    // per-clump factor (1 = unchanged); a repeated id takes its last factor
+    std::vector<float> f(c->nClumps, 1.f);
+    std::vector<uint8_t> mark(c->nClumps, 0);
+    for (uint32_t k = 0; k < n; k++) {
+        if (ids[k] >= c->nClumps) return c->fail(DEME_ERR_INVALID_ARGUMENT, "an id is not a clump owner");
+        if (!std::isfinite(factors[k]) || !(factors[k] > 0.f))
+            return c->fail(DEME_ERR_INVALID_ARGUMENT, "size factors must be finite and positive");
+        f[ids[k]] = factors[k];
+        mark[ids[k]] = 1;
+    }
+    if (!n) return DEME_OK;
+    DBuf<float> df;
+    DBuf<uint8_t> dm;
+    DEME_HIP(c, df.ensure(c->nClumps));
+    hipError_t e = dm.ensure(c->nClumps);
+    if (e != hipSuccess) {
+        df.release();
+        return c->fail_hip(e, "change_clump_sizes");
+    }
+    e = hipMemcpyAsync(df.p, f.data(), c->nClumps * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dm.p, mark.data(), c->nClumps, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_scale_components(c->sph.p, c->nS, dm.p, df.p, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    df.release();
+    dm.release();
+    if (e != hipSuccess) return c->fail_hip(e, "change_clump_sizes");
+    // the host's copy of the radii, scaled by the same float products, sizes the cell grid
+    std::vector<uint32_t> owner(c->nS);
+    if (c->nS) e = hipMemcpy2D(owner.data(), 4, c->sph.p, sizeof(SphereRec), 4, c->nS, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->fail_hip(e, "change_clump_sizes");
+    float maxR = 0.f;
+    for (uint32_t i = 0; i < c->nS; i++) {
+        if (mark[owner[i]]) c->hostRadius[i] *= f[owner[i]];
+        maxR = std::max(maxR, c->hostRadius[i]);
+    }
+    if (maxR != c->maxRadius) c->nCells = 0;
+    c->maxRadius = maxR;
+    c->adjReady = false;
+    c->graphGen++;
     return DEME_OK;
 }
 

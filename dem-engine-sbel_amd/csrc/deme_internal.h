@@ -173,6 +173,9 @@ struct JitInspect {
                       hipStream_t st) const;
 };
 std::string jit_inspect_source(const std::string& region);
+// ChangeClumpSizes: scale the offsets and radii of the spheres of clumps with mark[owner] != 0 by factor[owner]
+hipError_t launch_scale_components(SphereRec* sph, uint32_t n, const uint8_t* mark, const float* factor,
+                                   hipStream_t st);
 bool jit_is_owner_key(const std::string& key);
 // built-in model bodies: 0 frictional Hertz-Mindlin (the ahead-of-time kernel's), 1 frictionless Hertz
 const char* jit_builtin_model(int which);

@@ -25,4 +25,25 @@ hipError_t launch_inspect_final(int flavor, uint32_t nPart, const double* partia
     return hipGetLastError();
 }
 
+// dTModifyComponents (DEMMiscKernels.cu:18-33): the component spheres of marked clumps scale their offsets and radii.
+__global__ void k_scale_components(SphereRec* __restrict__ sph, uint32_t n, const uint8_t* __restrict__ mark,
+                                   const float* __restrict__ factor) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = sph[i].owner;
+    if (!mark[o]) return;
+    const float f = factor[o];
+    sph[i].rx = sph[i].rx * f;
+    sph[i].ry = sph[i].ry * f;
+    sph[i].rz = sph[i].rz * f;
+    sph[i].r = sph[i].r * f;
+}
+
+hipError_t launch_scale_components(SphereRec* sph, uint32_t n, const uint8_t* mark, const float* factor,
+                                   hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale_components, dim3((n + 255) / 256), dim3(256), 0, st, sph, n, mark, factor);
+    return hipGetLastError();
+}
+
 }  // namespace deme_int

@@ -5,6 +5,7 @@ The product path is libdeme.so; nothing here computes physics. Loading fails lou
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import os
 from dataclasses import dataclass
 
@@ -82,7 +83,8 @@ EXPORTED = [
     "deme_kernel_timing", "deme_set_beta", "deme_set_cell_scale", "deme_autotune", "deme_adjoint_forward",
     "deme_adjoint_backward", "deme_clear_contacts", "deme_set_force_model", "deme_jit_build",
     "deme_builtin_force_model", "deme_jit_render", "deme_jit_compile", "deme_set_step_graphs",
-    "deme_set_keep_contact_records", "deme_set_owner_volumes", "deme_inspect",
+    "deme_set_keep_contact_records", "deme_set_owner_volumes", "deme_inspect", "deme_append_clumps",
+    "deme_change_clump_sizes",
     "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner", "deme_jit_render_inspect",
 ]
 
@@ -145,6 +147,8 @@ def load(path: str = LIB_PATH):
         "deme_set_keep_contact_records": ([vp, i32], i32),
         "deme_set_owner_volumes": ([vp, vp], i32),
         "deme_inspect": ([vp, C.c_char_p, C.c_char_p, vp], i32),
+        "deme_append_clumps": ([vp, u32, vp, vp, vp, vp, u32, vp, vp, vp], i32),
+        "deme_change_clump_sizes": ([vp, u32, vp, vp], i32),
         "deme_clear_motion_rules": ([vp], i32),
         "deme_change_family": ([vp, i32, i32], i32),
         "deme_jit_render_owner": ([C.c_char_p, C.c_char_p, C.c_char_p, vp, u64, C.POINTER(u64)], i32),
@@ -373,6 +377,51 @@ class DeviceSystem:
         reg = None if region is None else region.encode()
         self._chk(self.lib.deme_inspect(self.ctx, quantity.encode(), reg, ptr(out)))
         return np.float32(out[0])
+
+    def append_clumps(self, sph_owner, sph_radius, sph_relpos, sph_mat, mass, moi, state):
+        """UpdateClumps: append clumps (sph_owner counts from the first new clump) with their mass, MOI and state (a
+        dict of owner-state arrays). The scene description follows: new spheres after the old ones, new clumps after
+        the old clumps, external-object owners moved up."""
+        own = np.ascontiguousarray(sph_owner, np.uint32)
+        rad = np.ascontiguousarray(sph_radius, np.float32)
+        rel = np.ascontiguousarray(np.asarray(sph_relpos, np.float32).reshape(-1, 3))
+        mat = np.ascontiguousarray(sph_mat, np.uint8)
+        mass = np.ascontiguousarray(mass, np.float32)
+        moi = np.ascontiguousarray(np.asarray(moi, np.float32).reshape(-1, 3))
+        n_new = len(mass)
+        st = {k: np.ascontiguousarray(v, STATE_FIELDS[k]) for k, v in state.items() if v is not None}
+        s = make_state_struct(st)
+        self._chk(self.lib.deme_append_clumps(self.ctx, len(own), ptr(own), ptr(rad), ptr(rel), ptr(mat), n_new,
+                                              ptr(mass), ptr(moi), C.byref(s)))
+        sc = self.scene
+        c0 = sc.n_clumps
+        anal = sc.anal.copy()
+        anal["owner"] += n_new
+        init = {}
+        for k, v in sc.state.items():
+            add = st.get(k, np.zeros(n_new, STATE_FIELDS[k]) if k not in ("oriQw",) else np.ones(n_new, np.float32))
+            init[k] = np.concatenate([v[:c0], add.astype(v.dtype), v[c0:]])
+        self.scene = dataclasses.replace(
+            sc, n_spheres=sc.n_spheres + len(own), n_owners=sc.n_owners + n_new, n_clumps=c0 + n_new,
+            sph_owner=np.concatenate([sc.sph_owner, own + np.uint32(c0)]).astype(np.uint32),
+            sph_radius=np.concatenate([sc.sph_radius, rad]).astype(np.float32),
+            sph_relpos=np.concatenate([sc.sph_relpos, rel]).astype(np.float32),
+            sph_mat=np.concatenate([sc.sph_mat, mat]).astype(sc.sph_mat.dtype),
+            owner_mass=np.concatenate([sc.owner_mass[:c0], mass, sc.owner_mass[c0:]]).astype(np.float32),
+            owner_moi=np.concatenate([sc.owner_moi[:c0], moi, sc.owner_moi[c0:]]).astype(np.float32),
+            anal=anal, state=init)
+
+    def change_clump_sizes(self, ids, factors):
+        """ChangeClumpSizes: scale the component offsets and radii of the given clumps."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        f = np.ascontiguousarray(factors, np.float32)
+        self._chk(self.lib.deme_change_clump_sizes(self.ctx, len(ids), ptr(ids), ptr(f)))
+        sc = self.scene
+        per = np.ones(sc.n_clumps, np.float32)
+        per[ids] = f
+        k = per[sc.sph_owner]
+        self.scene = dataclasses.replace(sc, sph_radius=(sc.sph_radius * k).astype(np.float32),
+                                         sph_relpos=(sc.sph_relpos * k[:, None]).astype(np.float32))
 
     def keep_contact_records(self, on=True):
         """Store the per-contact records (download_forces) on the last step of every step() call."""

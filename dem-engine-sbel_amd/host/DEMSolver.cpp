@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <sstream>
 
 namespace deme {
@@ -389,6 +390,7 @@ std::shared_ptr<DEMClumpBatch> DEMSolver::AddClumps(const std::vector<std::share
 }
 
 std::shared_ptr<DEMExternObj> DEMSolver::AddExternalObject() {
+    m_ext_loads++;
     auto p = std::make_shared<DEMExternObj>();
     p->load_order = (unsigned int)cached_extern_objs.size();
     cached_extern_objs.push_back(p);
@@ -674,20 +676,12 @@ void DEMSolver::Initialize() {
         for (size_t c = 0; c < nClumps; c++) vol[c] = m_templates.at(m_clump_template[c])->volume;
         check(deme_set_owner_volumes(m_ctx, vol.data()), "deme_set_owner_volumes");
     }
-    // owners of the tracked objects: clumps in batch order, then the external objects (dT.cpp:715-741)
-    {
-        std::vector<size_t> batchStart(1, 0);
-        for (const auto& b : cached_input_clump_batches) batchStart.push_back(batchStart.back() + b->GetNumClumps());
-        for (auto& t : m_tracked_objs) {
-            if (t->kind == DEMTrackedObj::Kind::CLUMP_BATCH) {
-                t->ownerID = (uint32_t)batchStart.at(t->load_order);
-                t->nSpanOwners = batchStart.at(t->load_order + 1) - batchStart.at(t->load_order);
-            } else {
-                t->ownerID = (uint32_t)(nClumps + t->load_order);
-                t->nSpanOwners = 1;
-            }
-        }
-    }
+    m_init_clumps = nClumps;
+    m_init_templates = m_templates.size();
+    m_init_materials = m_loaded_materials.size();
+    m_init_prescriptions = m_input_family_prescription.size();
+    m_init_ext_loads = m_ext_loads;
+    assignTrackers(0);
     // motion strategies and family change rules (equipFamilyOnFlyChanges, APIPrivate.cpp:980-1003)
     std::string changeRules;
     for (const auto& fc : m_family_changes) {
@@ -791,24 +785,29 @@ void DEMSolver::DoDynamicsThenSync(double duration) {
 
 float3 DEMSolver::GetOwnerPosition(uint32_t i) {
     syncHostState();
+    i = toInternal(i);
     double X, Y, Z;
     host_voxel_to_pos(m_params, h_voxel.at(i), h_lx.at(i), h_ly.at(i), h_lz.at(i), X, Y, Z);
     return make_float3((float)(X + m_params.LBF[0]), (float)(Y + m_params.LBF[1]), (float)(Z + m_params.LBF[2]));
 }
 float3 DEMSolver::GetOwnerVelocity(uint32_t i) {
     syncHostState();
+    i = toInternal(i);
     return make_float3(h_vx.at(i), h_vy.at(i), h_vz.at(i));
 }
 unsigned int DEMSolver::GetOwnerFamily(uint32_t i) {
     syncHostState();
+    i = toInternal(i);
     return h_fam.at(i);
 }
 float3 DEMSolver::GetOwnerAngVel(uint32_t i) {
     syncHostState();
+    i = toInternal(i);
     return make_float3(h_wx.at(i), h_wy.at(i), h_wz.at(i));
 }
 float4 DEMSolver::GetOwnerOriQ(uint32_t i) {
     syncHostState();
+    i = toInternal(i);
     float4 q;
     q.w = h_qw.at(i);
     q.x = h_qx.at(i);
@@ -818,14 +817,17 @@ float4 DEMSolver::GetOwnerOriQ(uint32_t i) {
 }
 float3 DEMSolver::GetOwnerAcc(uint32_t i) {
     syncHostState();
+    i = toInternal(i);
     return make_float3(h_ax.at(i), h_ay.at(i), h_az.at(i));
 }
 float3 DEMSolver::GetOwnerAngAcc(uint32_t i) {
     syncHostState();
+    i = toInternal(i);
     return make_float3(h_alx.at(i), h_aly.at(i), h_alz.at(i));
 }
 void DEMSolver::SetOwnerPosition(uint32_t i, float3 pos) {
     syncHostState();
+    i = toInternal(i);
     // float subtraction first, as dT::setOwnerPos does (dT.cpp:1783-1790)
     const double X = pos.x - m_params.LBF[0], Y = pos.y - m_params.LBF[1], Z = pos.z - m_params.LBF[2];
     host_pos_to_voxel(m_params, X, Y, Z, h_voxel.at(i), h_lx.at(i), h_ly.at(i), h_lz.at(i));
@@ -833,6 +835,7 @@ void DEMSolver::SetOwnerPosition(uint32_t i, float3 pos) {
 }
 void DEMSolver::SetOwnerVelocity(uint32_t i, float3 v) {
     syncHostState();
+    i = toInternal(i);
     h_vx.at(i) = v.x;
     h_vy.at(i) = v.y;
     h_vz.at(i) = v.z;
@@ -840,6 +843,7 @@ void DEMSolver::SetOwnerVelocity(uint32_t i, float3 v) {
 }
 void DEMSolver::SetOwnerAngVel(uint32_t i, float3 w) {
     syncHostState();
+    i = toInternal(i);
     h_wx.at(i) = w.x;
     h_wy.at(i) = w.y;
     h_wz.at(i) = w.z;
@@ -847,11 +851,204 @@ void DEMSolver::SetOwnerAngVel(uint32_t i, float3 w) {
 }
 void DEMSolver::SetOwnerOriQ(uint32_t i, float4 q) {
     syncHostState();
+    i = toInternal(i);
     h_qw.at(i) = q.w;
     h_qx.at(i) = q.x;
     h_qy.at(i) = q.y;
     h_qz.at(i) = q.z;
     m_mirror_dirty = true;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// owner numbering, mid-simulation changes
+// ---------------------------------------------------------------------------------------------------------------
+uint32_t DEMSolver::toInternal(uint32_t u) const {
+    if (u >= nOwnerBodies) throw std::out_of_range("owner " + std::to_string(u) + " does not exist");
+    if (u < m_init_clumps) return u;
+    if (u < m_init_clumps + nExtObj) return (uint32_t)(nOwnerClumps + (u - m_init_clumps));
+    return (uint32_t)(u - nExtObj);
+}
+
+uint32_t DEMSolver::toUser(uint32_t i) const {
+    if (i < m_init_clumps) return i;
+    if (i < nOwnerClumps) return (uint32_t)(i + nExtObj);
+    return (uint32_t)(m_init_clumps + (i - nOwnerClumps));
+}
+
+// Owners of trackers not yet placed (dT.cpp:715-741): a batch's first owner follows firstOwner (the owners that
+// existed before its clumps were added) by the clumps of the cached batches before it; an external object follows
+// the clumps of the Initialize call by its load order.
+void DEMSolver::assignTrackers(size_t firstOwner) {
+    std::vector<size_t> batchStart(1, 0);
+    for (const auto& b : cached_input_clump_batches) batchStart.push_back(batchStart.back() + b->GetNumClumps());
+    for (auto& t : m_tracked_objs) {
+        if (t->ownerID != std::numeric_limits<uint32_t>::max()) continue;
+        if (t->kind == DEMTrackedObj::Kind::CLUMP_BATCH) {
+            if (t->load_order + 1 >= batchStart.size()) continue;  // its batch was cleared before being added
+            t->ownerID = (uint32_t)(firstOwner + batchStart[t->load_order]);
+            t->nSpanOwners = batchStart[t->load_order + 1] - batchStart[t->load_order];
+        } else {
+            t->ownerID = (uint32_t)(m_init_clumps + t->load_order);
+            t->nSpanOwners = 1;
+        }
+    }
+}
+
+std::shared_ptr<DEMForceModel> DEMSolver::ReadContactForceModel(const std::string& filename) {
+    std::ifstream f(filename);
+    if (!f) throw std::runtime_error("The force model file " + filename + " is not found.");
+    std::stringstream ss;
+    ss << f.rdbuf();
+    m_force_model->DefineCustomModel(ss.str());
+    return m_force_model;
+}
+
+void DEMSolver::ClearCache() {
+    cached_input_clump_batches.clear();
+    cached_extern_objs.clear();
+    m_input_no_contact_pairs.clear();
+}
+
+void DEMSolver::UpdateClumps() {
+    if (!sys_initialized)
+        throw std::runtime_error(
+            "Please call UpdateClumps only after the system is initialized, because it is for adding additional clumps "
+            "to an initialized DEM system.");
+    if (m_ext_loads != m_init_ext_loads)
+        throw std::runtime_error("UpdateClumps cannot be used after loading new analytical objects.");
+    if (m_templates.size() != m_init_templates)
+        throw std::runtime_error("UpdateClumps cannot be used after loading new clump templates.");
+    if (m_loaded_materials.size() != m_init_materials || m_input_family_prescription.size() != m_init_prescriptions)
+        throw std::runtime_error(
+            "UpdateClumps should not be used if you introduce new material types or family prescription.");
+    pushHostState();
+    syncHostState();
+    size_t nNew = 0;
+    for (const auto& b : cached_input_clump_batches) nNew += b->GetNumClumps();
+    if (nNew == 0) return;
+    const size_t nOwnersBefore = nOwnerBodies;
+    std::vector<uint32_t> sOwner;
+    std::vector<float> sR, sRel, mass, moi;
+    std::vector<uint8_t> sMat;
+    std::vector<uint64_t> vox(nNew);
+    std::vector<uint16_t> lx(nNew), ly(nNew), lz(nNew);
+    std::vector<float> q[4], v[3], w[3];
+    for (auto& a : q) a.resize(nNew);
+    for (auto& a : v) a.resize(nNew);
+    for (auto& a : w) a.resize(nNew);
+    std::vector<uint8_t> fam(nNew);
+    std::vector<uint32_t> tpl(nNew);
+    size_t k = 0;
+    for (const auto& b : cached_input_clump_batches) {
+        for (size_t i = 0; i < b->GetNumClumps(); i++, k++) {
+            const auto& t = b->types.at(i);
+            if (!t) throw std::runtime_error("A clump in an AddClumps batch has no template.");
+            tpl[k] = t->mark;
+            mass.push_back(t->mass);
+            moi.insert(moi.end(), {t->MOI.x, t->MOI.y, t->MOI.z});
+            for (unsigned int j = 0; j < t->nComp; j++) {
+                sOwner.push_back((uint32_t)k);
+                sR.push_back(t->radii[j]);
+                sRel.insert(sRel.end(), {t->relPos[j].x, t->relPos[j].y, t->relPos[j].z});
+                sMat.push_back((uint8_t)t->materials[j]->load_order);
+            }
+            const float3 c = f3sub(b->xyz[i], m_boxLBF);
+            host_pos_to_voxel(m_params, (double)c.x, (double)c.y, (double)c.z, vox[k], lx[k], ly[k], lz[k]);
+            q[0][k] = b->oriQ[i].w;
+            q[1][k] = b->oriQ[i].x;
+            q[2][k] = b->oriQ[i].y;
+            q[3][k] = b->oriQ[i].z;
+            v[0][k] = b->vel[i].x;
+            v[1][k] = b->vel[i].y;
+            v[2][k] = b->vel[i].z;
+            w[0][k] = b->angVel[i].x;
+            w[1][k] = b->angVel[i].y;
+            w[2][k] = b->angVel[i].z;
+            fam[k] = (uint8_t)b->families[i];
+        }
+    }
+    deme_owner_state st{};
+    st.voxelID = vox.data();
+    st.locX = lx.data();
+    st.locY = ly.data();
+    st.locZ = lz.data();
+    st.oriQw = q[0].data();
+    st.oriQx = q[1].data();
+    st.oriQy = q[2].data();
+    st.oriQz = q[3].data();
+    st.vX = v[0].data();
+    st.vY = v[1].data();
+    st.vZ = v[2].data();
+    st.omgX = w[0].data();
+    st.omgY = w[1].data();
+    st.omgZ = w[2].data();
+    st.family = fam.data();
+    check(deme_append_clumps(m_ctx, (uint32_t)sOwner.size(), sOwner.data(), sR.data(), sRel.data(), sMat.data(),
+                             (uint32_t)nNew, mass.data(), moi.data(), &st),
+          "deme_append_clumps");
+    // host copies follow the device layout: the new clumps after the old clumps, the external objects after them
+    const size_t at = nOwnerClumps;
+    auto ins = [&](auto& dst, const auto& src) { dst.insert(dst.begin() + at, src.begin(), src.end()); };
+    ins(h_voxel, vox);
+    ins(h_lx, lx);
+    ins(h_ly, ly);
+    ins(h_lz, lz);
+    ins(h_qw, q[0]);
+    ins(h_qx, q[1]);
+    ins(h_qy, q[2]);
+    ins(h_qz, q[3]);
+    ins(h_vx, v[0]);
+    ins(h_vy, v[1]);
+    ins(h_vz, v[2]);
+    ins(h_wx, w[0]);
+    ins(h_wy, w[1]);
+    ins(h_wz, w[2]);
+    ins(h_fam, fam);
+    for (auto* a : {&h_ax, &h_ay, &h_az, &h_alx, &h_aly, &h_alz}) a->insert(a->begin() + at, nNew, 0.f);
+    for (size_t i = 0; i < sOwner.size(); i++) {
+        m_sph_owner.push_back((uint32_t)(at + sOwner[i]));
+        m_sph_r.push_back(sR[i]);
+        m_sph_rel.push_back(make_float3(sRel[3 * i], sRel[3 * i + 1], sRel[3 * i + 2]));
+    }
+    m_clump_template.insert(m_clump_template.end(), tpl.begin(), tpl.end());
+    for (auto& o : m_anal_owner) o += (uint32_t)nNew;
+    nOwnerClumps += nNew;
+    nOwnerBodies += nNew;
+    nSpheresGM += sOwner.size();
+    std::vector<float> vol(nOwnerClumps);
+    for (size_t c = 0; c < nOwnerClumps; c++) vol[c] = m_templates.at(m_clump_template[c])->volume;
+    check(deme_set_owner_volumes(m_ctx, vol.data()), "deme_set_owner_volumes");
+    assignTrackers(nOwnersBefore);
+    m_mirror_valid = true;
+    m_mirror_dirty = false;
+}
+
+void DEMSolver::ChangeClumpSizes(const std::vector<uint32_t>& IDs, const std::vector<float>& factors) {
+    if (!sys_initialized)
+        throw std::runtime_error(
+            "ChangeClumpSizes operates on device-side arrays directly, so it requires the system to be initialized "
+            "first.");
+    if (IDs.size() != factors.size()) throw std::runtime_error("ChangeClumpSizes: IDs and factors differ in length.");
+    std::vector<uint32_t> in(IDs.size());
+    for (size_t k = 0; k < IDs.size(); k++) {
+        if (IDs[k] >= nOwnerBodies) throw std::runtime_error("ChangeClumpSizes: owner " + std::to_string(IDs[k]) +
+                                                             " does not exist.");
+        in[k] = toInternal(IDs[k]);
+    }
+    check(deme_change_clump_sizes(m_ctx, (uint32_t)in.size(), in.data(), factors.data()), "deme_change_clump_sizes");
+    // the host's geometry (output files) follows
+    std::vector<float> f(nOwnerClumps, 1.f);
+    std::vector<uint8_t> mark(nOwnerClumps, 0);
+    for (size_t k = 0; k < in.size(); k++) {
+        f[in[k]] = factors[k];
+        mark[in[k]] = 1;
+    }
+    for (size_t i = 0; i < nSpheresGM; i++) {
+        const uint32_t o = m_sph_owner[i];
+        if (!mark[o]) continue;
+        m_sph_rel[i] = make_float3(m_sph_rel[i].x * f[o], m_sph_rel[i].y * f[o], m_sph_rel[i].z * f[o]);
+        m_sph_r[i] = m_sph_r[i] * f[o];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------

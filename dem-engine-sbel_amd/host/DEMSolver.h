@@ -329,6 +329,18 @@ class DEMSolver {
     std::shared_ptr<DEMTracker> Track(std::shared_ptr<DEMClumpBatch>& obj);
     bool GetInitStatus() const { return sys_initialized; }
 
+    // ---- mid-simulation changes (API.h:120-122, 302-316) ----
+    // ReadContactForceModel: the model body from a file (a path as given).
+    std::shared_ptr<DEMForceModel> ReadContactForceModel(const std::string& filename);
+    // ClearCache: forget the cached clump batches and external objects (trackers stay valid).
+    void ClearCache();
+    // UpdateClumps: add every cached clump batch to the initialised system. Its clumps take the owner numbers after
+    // all existing owners (as the reference numbers them); contacts and their histories are kept. New templates,
+    // materials, external objects or family prescriptions since Initialize are not allowed.
+    void UpdateClumps();
+    // ChangeClumpSizes: scale the component offsets and radii of the given clumps by their factors.
+    void ChangeClumpSizes(const std::vector<uint32_t>& IDs, const std::vector<float>& factors);
+
     // ---- running (API.h:374-386) ----
     void Initialize();
     void DoDynamics(double thisCallDuration);
@@ -374,6 +386,11 @@ class DEMSolver {
     };
 
     void check(int status, const char* what) const;
+    // Owner numbers: the reference numbers clumps added by UpdateClumps after the external objects; the device keeps
+    // all clumps first. toInternal maps the reference's number to the device's, toUser back.
+    uint32_t toInternal(uint32_t u) const;
+    uint32_t toUser(uint32_t i) const;
+    void assignTrackers(size_t firstOwner);
     void syncHostState();
     void pushHostState();
     void addWorldBoundingBox();
@@ -437,6 +454,10 @@ class DEMSolver {
     std::vector<float3> m_sph_rel;
     std::vector<uint32_t> m_clump_template;
     std::vector<uint32_t> m_anal_owner;
+    // counts at Initialize: clumps (owner numbers below it are the same on both sides), and what UpdateClumps must
+    // find unchanged
+    size_t m_init_clumps = 0, m_init_templates = 0, m_init_materials = 0, m_init_prescriptions = 0;
+    size_t m_ext_loads = 0, m_init_ext_loads = 0;
 
     void writeSpheresCsv(std::string& out);
     void writeClumpsCsv(std::string& out);

@@ -343,9 +343,9 @@ void DEMSolver::writeContactsCsv(std::string& s) {
             oB = m_anal_owner.at(idB[i]);
             name = "SA";
         }
-        s += std::to_string(oA);
+        s += std::to_string(toUser(oA));
         s += ',';
-        s += std::to_string(oB);
+        s += std::to_string(toUser(oB));
         s += ',';
         s += name;
         if (cf & CNT_OUTPUT_CONTENT::FORCE) put3(s, F[3 * i], F[3 * i + 1], F[3 * i + 2]);

@@ -14,6 +14,10 @@
 //     Balls settle on the floor while two more fall from above; inspectors (whole domain and a region) and trackers
 //     (of the falling pair and of an added plane) are read, one tracked ball is moved with SetPos, and the
 //     inspectors are read again. Prints one JSON line.
+//   facade_demo update <dir>
+//     The frictionless model is read from a file (ReadContactForceModel); 18 balls settle on the floor; the cache is
+//     cleared and three more balls are added above with UpdateClumps (tracked); one resting ball grows by 2%
+//     (ChangeClumpSizes, by 2%); after the new balls land, the contact and sphere files are written into <dir>.
 //   facade_demo readcsv <file>
 //     Reads <file> with ReadClumpXyzFromCsv and ReadClumpQuatFromCsv and prints the values as float bit patterns.
 #include <cmath>
@@ -310,10 +314,64 @@ int run_inspect() {
     return 0;
 }
 
+int run_update(const std::string& dir) {
+    {
+        std::FILE* f = std::fopen((dir + "/frictionless.model").c_str(), "w");
+        if (!f) throw std::runtime_error("cannot write the model file");
+        std::fputs(deme_builtin_force_model(1), f);
+        std::fclose(f);
+    }
+    DEMSolver sim;
+    sim.InstructBoxDomainDimension(0.2f, 0.2f, 0.2f);
+    sim.SetCoordSysOrigin("center");
+    auto mat = sim.LoadMaterial({{"E", 1e7f}, {"nu", 0.3f}, {"CoR", 0.3f}, {"mu", 0.4f}, {"Crr", 0.f}});
+    sim.InstructBoxDomainBoundingBC("top_open", mat);
+    auto fm = sim.ReadContactForceModel(dir + "/frictionless.model");
+    fm->SetPerContactWildcards({});
+    const float r = 0.004f, pi = 3.14159265f, vol = 4.f / 3.f * pi * r * r * r;
+    auto ball = sim.LoadSphereType(2600.f * vol, r, mat);
+    ball->AssignName("ball");
+    std::vector<float3> rest;
+    for (int i = 0; i < 18; i++) rest.push_back(make_float3(-0.08f + 0.02f * (i % 9), -0.08f + 0.02f * (i / 9), -0.1f + r));
+    sim.AddClumps(std::vector<std::shared_ptr<DEMClumpTemplate>>(rest.size(), ball), rest);
+    sim.SetInitTimeStep(1e-5);
+    sim.SetCDUpdateFreq(1);
+    sim.SetGravitationalAcceleration(make_float3(0, 0, -9.81f));
+    sim.Initialize();
+    sim.DoDynamics(0.02);
+    sim.ClearCache();
+    std::vector<float3> more = {make_float3(0.05f, 0.05f, -0.1f + r + 0.002f), make_float3(0.07f, 0.05f, -0.1f + r + 0.003f),
+                                make_float3(0.05f, 0.07f, -0.1f + r + 0.004f)};
+    auto b2 = sim.AddClumps(std::vector<std::shared_ptr<DEMClumpTemplate>>(more.size(), ball), more);
+    auto t2 = sim.Track(b2);
+    sim.UpdateClumps();
+    const float3 start = t2->Pos(0);
+    sim.ChangeClumpSizes({0}, {1.02f});
+    sim.DoDynamics(0.05);
+    sim.SetContactOutputContent(CNT_OUTPUT_CONTENT::FORCE);
+    sim.WriteContactFile(dir + "/contacts.csv");
+    sim.WriteSphereFile(dir + "/spheres.csv");
+    const float3 p19 = sim.GetOwnerPosition(19), tp = t2->Pos(0), p0 = sim.GetOwnerPosition(0);
+    std::printf("{\"n_clumps\": %zu, \"n_owners\": %zu, \"tracker_owner\": %u, \"span\": %zu, "
+                "\"start\": [%.9g, %.9g, %.9g], \"tracker\": [%.9g, %.9g, %.9g], \"owner19\": [%.9g, %.9g, %.9g], "
+                "\"owner0\": [%.9g, %.9g, %.9g], \"r\": %.9g}\n",
+                sim.GetNumClumps(), sim.GetNumOwners(), t2->obj->ownerID, t2->obj->nSpanOwners, start.x, start.y,
+                start.z, tp.x, tp.y, tp.z, p19.x, p19.y, p19.z, p0.x, p0.y, p0.z, r);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc == 3 && std::string(argv[1]) == "readcsv") return read_csv(argv[2]);
+    if (argc == 3 && std::string(argv[1]) == "update") {
+        try {
+            return run_update(argv[2]);
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "facade_demo: %s\n", e.what());
+            return 1;
+        }
+    }
     if (argc == 2 && std::string(argv[1]) == "inspect") {
         try {
             return run_inspect();
@@ -332,7 +390,7 @@ int main(int argc, char** argv) {
     }
     if (argc != 4) {
         std::fprintf(stderr, "usage: facade_demo <hertz|frictionless|linear|prescribed> <n_spheres> <seconds> | "
-                             "io <n_clumps> <seconds> <dir> | inspect | readcsv <file>\n");
+                             "io <n_clumps> <seconds> <dir> | inspect | update <dir> | readcsv <file>\n");
         return 2;
     }
     const std::string model = argv[1];

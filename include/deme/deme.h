@@ -209,6 +209,26 @@ int deme_get_counts(deme_ctx* ctx, deme_counts* out);
 int deme_set_time(deme_ctx* ctx, double t);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Mid-simulation changes (UpdateClumps, APIPublic.cpp:792-842; ChangeClumpSizes, APIPublic.cpp:844-863 with
+ * DEMMiscKernels.cu:4-51).
+ * ------------------------------------------------------------------------------------------------------------- */
+/* Appends nNewClumps clumps to an initialised system: nNewSph component spheres (sphOwner counts from the first new
+ * clump, grouped in ascending order; radii, offsets as 3 floats each, material indices of the uploaded table), each
+ * new clump's mass and MOI (3 floats), and its state (voxelID and locX/Y/Z required; the other arrays nullable:
+ * identity orientation, zero velocities, family 0). The new spheres take the ids after the existing ones and the new
+ * clumps the owner numbers after the existing clumps; the external-object owners move up by nNewClumps. The contact
+ * list and its tangential histories are kept: the next step starts with a detection pass, which carries the
+ * histories of surviving contacts over. Clump volumes must be set again (deme_set_owner_volumes). Not for batches of
+ * scenes (DEME_ERR_UNSUPPORTED). */
+int deme_append_clumps(deme_ctx* ctx, uint32_t nNewSph, const uint32_t* sphOwner, const float* sphRadius,
+                       const float* sphRelPos, const uint8_t* sphMat, uint32_t nNewClumps, const float* ownerMass,
+                       const float* ownerMoi, const deme_owner_state* state);
+/* Scales the component offsets and radii of the given clumps (owner numbers below the clump count) by their factors
+ * (finite, positive; a repeated id takes its last factor), in place on the device. Mass and MOI are unchanged, as in
+ * the reference. The cell grid follows the new largest radius at the next detection pass. */
+int deme_change_clump_sizes(deme_ctx* ctx, uint32_t n, const uint32_t* ids, const float* factors);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Inspectors (DEMInspector::GetValue, AuxClasses.cpp:105-110, over dT::inspectCall, dT.cpp:1660-1720).
  * ------------------------------------------------------------------------------------------------------------- */
 /* Volume of each clump (nClumps floats; the template volumes of DEMClumpTemplate::SetVolume), summed by the

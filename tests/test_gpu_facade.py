@@ -131,3 +131,25 @@ def test_inspectors_and_trackers_through_the_facade():
     assert d["wall_owner"] == 20
     assert d["wall"] == pytest.approx([0.01, 0.02, 0.03], abs=1e-6)
     assert d["errors"] == "region,quantity"
+
+
+def test_update_clumps_change_sizes_and_model_file_through_the_facade(tmp_path):
+    d = str(tmp_path)
+    print(f"facade_demo update {d}", flush=True)
+    out = subprocess.run([DEMO, "update", d], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    info = json.loads(out.stdout.strip().splitlines()[-1])
+    r = info["r"]
+    # three clumps added after the 18 clumps and the box: owners 19, 20, 21, tracked from 19
+    assert info["n_clumps"] == 21 and info["n_owners"] == 22
+    assert info["tracker_owner"] == 19 and info["span"] == 3
+    assert info["tracker"] == info["owner19"]
+    assert info["start"][2] > info["tracker"][2]  # it fell
+    assert info["tracker"][2] == pytest.approx(-0.1 + r, abs=2e-5)  # and rests on the floor (frictionless model)
+    rows = [ln.split(",") for ln in open(os.path.join(d, "contacts.csv")).read().splitlines()[1:]]
+    plane = {int(a): int(b) for a, b, t, *_ in rows if t == "SA"}
+    assert {19, 20, 21} <= set(plane) and all(b == 18 for b in plane.values())  # the box keeps owner number 18
+    # ball 0 grew by 2%: radius and rest height
+    spheres = [ln.split(",") for ln in open(os.path.join(d, "spheres.csv")).read().splitlines()[1:]]
+    assert float(spheres[0][3]) == pytest.approx(1.02 * r, rel=1e-5) and float(spheres[1][3]) == pytest.approx(r)
+    assert info["owner0"][2] == pytest.approx(-0.1 + 1.02 * r, abs=2e-5)
