@@ -72,7 +72,27 @@ def build_scene(n_clumps, update_freq, beta_frac):
     return scenes.clump_settling(n_clumps=n_clumps, update_freq=update_freq, beta_frac=beta_frac)
 
 
-def run_device(sc, steps, warmup, update_freq, jit=False):
+def run_inspectors(dev, sc, calls=20):
+    """Inspector reductions over the stepped state (DEMInspector::GetValue): the clump_max_z kernel over every sphere,
+    on the whole domain and on a region compiled by hipRTC; per-launch kernel time from HIP events."""
+    out = {}
+    for label, region in (("clump_max_z", None), ("clump_max_z_region", "return Z < 0.0f && X > -0.1f;")):
+        dev.inspect("clump_max_z", region)  # compiles a region once
+        dev.reset_timing()
+        dev.enable_timing(True)
+        for _ in range(calls):
+            value = dev.inspect("clump_max_z", region)
+        dev.enable_timing(False)
+        t = dev.kernel_timing()
+        us = 1e3 * t["inspect"][0] / max(t["inspect"][1], 1)
+        algo = sc.n_spheres * 32 + sc.n_clumps * 64  # sphere records, then each owner record once
+        out[label] = {"value": float(value), "inspect_us": round(us, 2),
+                      "final_us": round(1e3 * t["inspect_final"][0] / max(t["inspect_final"][1], 1), 2),
+                      "GBps": round(algo / (us * 1e-6) / 1e9, 1), "algorithmic_bytes": algo}
+    return out
+
+
+def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False):
     from deme_amd.abi import DeviceSystem
 
     dev = DeviceSystem(sc)
@@ -91,10 +111,11 @@ def run_device(sc, steps, warmup, update_freq, jit=False):
     dev.enable_timing(False)
     timing = dev.kernel_timing()
     c = dev.counts()
+    insp = run_inspectors(dev, sc) if inspect else None
     info = dict(S=sc.n_spheres, O=sc.n_owners, P=c.nBinSphereTouches, C=c.nContacts, Cprev=c.nPrevContacts,
                 pairs=c.nContacts - c.nSphereGeoContacts, geo=c.nSphereGeoContacts, Ba=c.nActiveBins,
                 nbins=sc.params.nbX * sc.params.nbY * sc.params.nbZ, cells=c.nCells, fresh=update_freq == 1,
-                anal_blocks=(c.nContacts + 4095) // 4096)
+                anal_blocks=(c.nContacts + 4095) // 4096, inspectors=insp)
     dev.close()
     return t1 - t0, timing, info
 
@@ -237,6 +258,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-jit", action="store_true")
+    ap.add_argument("--skip-inspect", action="store_true")
     ap.add_argument("--adjoint-spheres", type=int, default=50_000)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-window", type=int, default=20)
@@ -260,7 +282,7 @@ def main():
         del src, dst
 
     scA = build_scene(args.clumps, 1, 0.0)
-    tA, timingA, infoA = run_device(scA, args.steps, args.warmup, 1)
+    tA, timingA, infoA = run_device(scA, args.steps, args.warmup, 1, inspect=not args.skip_inspect)
     value = args.clumps * args.steps / tA
 
     runB = None
@@ -337,6 +359,7 @@ def main():
                    "bin_sphere_touches": infoA["P"], "active_bins": infoA["Ba"],
                    "run_every5_beta0.2r": runB, "jit_specialised": jit, "batched_columns": batch,
                    "adjoint_drop": adjoint,
+                   "inspectors": infoA["inspectors"],
                    "kernels": kernels},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": traffic,
