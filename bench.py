@@ -38,7 +38,8 @@ def bytes_model(name, k):
         "bin_scatter": S * (32 + 4) + P * (4 + 8 + 8),
         # per-bin pair test: entries + bin ends + CD records (unique) + owner/family of hit pairs, pairs out
         "bin_pairs": P * (8 + 4) + S * 32 + Np * (8 + 8 + 8),
-        "scan_rows": S * (4 + 4 + 4),
+        # cell path: rows and mirror rows in one pass: permutation, two gathered counts, plane counts in; two starts out
+        "scan_rows": S * (4 + 4 + 4 + 4 + 4 + 4) if k.get("cells") else S * (4 + 4 + 4),
         "row_fill": Np * (8 + 4 + 4 + 8 + 5),
         "geo_fill": S * (4 + 4 + 32) + k["geo"] * 5,
         "row_finalize": S * 8 + C * 8,

@@ -47,12 +47,13 @@ __device__ __forceinline__ bool range_in_grid(const KParams& P, const BinRange& 
 
 // Membership of a linear bin id in the linear ids of a range (the reference's bins may alias when a range crosses
 // the grid's upper faces; this enumerates exactly the ids the reference would form).
-__device__ __noinline__ bool range_has_linear(const KParams& P, const BinRange& r, uint32_t lin) {
-    const uint32_t nbXY = P.nbX * P.nbY;
+// (Out of line: the rare path. Arguments by value, so the caller's ranges stay in registers.)
+__device__ __noinline__ bool range_has_linear(uint32_t nbX, uint32_t nbY, BinRange r, uint32_t lin) {
+    const uint32_t nbXY = nbX * nbY;
     for (uint32_t k = 0; k <= r.z1 - r.z0; k++)
         for (uint32_t j = 0; j <= r.y1 - r.y0; j++)
             for (uint32_t i = 0; i <= r.x1 - r.x0; i++)
-                if ((r.x0 + i) + (r.y0 + j) * P.nbX + (r.z0 + k) * nbXY == lin) return true;
+                if ((r.x0 + i) + (r.y0 + j) * nbX + (r.z0 + k) * nbXY == lin) return true;
     return false;
 }
 
@@ -281,7 +282,7 @@ __device__ __forceinline__ bool rule_member(const KParams& P, const RuleBins& R)
                    R.iz >= R.gb.z0 && R.iz <= R.gb.z1;
     }
     const uint32_t lin = R.ix + R.iy * P.nbX + R.iz * P.nbX * P.nbY;
-    return range_has_linear(P, R.ga, lin) && range_has_linear(P, R.gb, lin);
+    return range_has_linear(P.nbX, P.nbY, R.ga, lin) && range_has_linear(P.nbX, P.nbY, R.gb, lin);
 }
 
 // A: the lower id of an overlapping pair. inv: 1 / binSize.

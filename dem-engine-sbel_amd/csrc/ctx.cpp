@@ -142,6 +142,8 @@ struct deme_ctx {
     // contact's slot there
     DBuf<uint32_t> bCnt, bRowStart, bPart, mirror;
     DBuf<uint64_t> scanScratch;
+    DBuf<uint2> scanTmp;        // gathered (row, mirror-row) counts of the fused scan
+    DBuf<uint64_t> scan2Sums;   // its tile sums
     DBuf<DevCounters> ctr;
     DevCounters* hctr = nullptr;  // pinned mirror
 
@@ -409,7 +411,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->jit.release();
     c->jitOwner.release();
     for (auto& kv : c->inspectJit) kv.second.release();
-    c->inspPart.release(); c->ownerVol.release();
+    c->inspPart.release(); c->ownerVol.release(); c->scanTmp.release(); c->scan2Sums.release();
     if (c->stepExec) (void)hipGraphExecDestroy(c->stepExec);
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
@@ -912,6 +914,8 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->sortedMeta.ensure(nS));
     DEME_HIP(c, c->sorted.ensure(nS));
     DEME_HIP(c, c->planeBits.ensure(nS));
+    DEME_HIP(c, c->scanTmp.ensure(nS));
+    DEME_HIP(c, c->scan2Sums.ensure(2 * ((nS + SCAN_TILE - 1) / SCAN_TILE + 1)));
     DEME_HIP(c, c->survSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
@@ -945,12 +949,19 @@ int deme_cd_run(deme_ctx* c) {
                                       c->slabCap, c->survSlab.p, c->rowSlab.p, c->mirSlab.p, c->aCnt.p, c->bCnt.p,
                                       c->ctr.p, c->stream));
         // row lengths in id order: partner counts gathered from sorted positions, plus plane contacts; mirror rows
-        DEME_LAUNCH(c, "scan_rows",
-                    scan_u32(c->aCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1],
-                             c->scanScratch.p, c->stream, c->posOf.p));
-        DEME_LAUNCH(c, "scan_brows",
-                    scan_u32(c->bCnt.p, nullptr, nS, c->bRowStart.p, &c->ctr.p->total[2], c->scanScratch.p,
-                             c->stream, c->posOf.p));
+        if ((uint64_t)nS <= SCAN_TILE * SCAN_TILE) {
+            DEME_LAUNCH(c, "scan_rows",
+                        scan2_u32_gather(c->aCnt.p, c->numAnal.p, c->bCnt.p, c->posOf.p, nS, c->rowStart[cu].p,
+                                         c->bRowStart.p, &c->ctr.p->total[1], &c->ctr.p->total[2], c->scanTmp.p,
+                                         c->scan2Sums.p, c->stream));
+        } else {
+            DEME_LAUNCH(c, "scan_rows",
+                        scan_u32(c->aCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1],
+                                 c->scanScratch.p, c->stream, c->posOf.p));
+            DEME_LAUNCH(c, "scan_brows",
+                        scan_u32(c->bCnt.p, nullptr, nS, c->bRowStart.p, &c->ctr.p->total[2], c->scanScratch.p,
+                                 c->stream, c->posOf.p));
+        }
         r = readback_counters(c);
         if (r) return r;
         const DevCounters& h = *c->hctr;

@@ -55,6 +55,12 @@ constexpr uint32_t MAX_ANAL_OWNERS = 8;
 uint64_t scan_scratch_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
                     uint64_t* scratch, hipStream_t st, const uint32_t* idx = nullptr);
+// Two scans over one gather: out1 = scan(a1[idx[i]] + (b1 ? b1[i] : 0)), out2 = scan(a2[idx[i]]), totals at out[n]
+// and in *total1 / *total2. tmp: n uint2; sums: 2 * ceil(n / SCAN_TILE) u64. n must be at most SCAN_TILE^2
+// (hipErrorInvalidValue otherwise).
+hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32_t* a2, const uint32_t* idx,
+                            uint64_t n, uint32_t* out1, uint32_t* out2, unsigned long long* total1,
+                            unsigned long long* total2, uint2* tmp, uint64_t* sums, hipStream_t st);
 
 // cd.hip
 hipError_t launch_plane_pose(const KParams& P, const deme_anal_obj* anal, uint32_t nA, const OwnerRec* own,

@@ -122,7 +122,126 @@ hipError_t scan_u64_inplace(uint64_t* v, uint64_t n, uint64_t* scratch, hipStrea
     return hipGetLastError();
 }
 
+// ---- two scans over one gather (the row and mirror-row starts of the cell path) ----
+// Pass 1 reads each permuted pair of counts once, stores it in id order (tmp) and folds both tile sums; pass 2 scans
+// the tile sums of both components in one block; pass 3 scans the tiles from tmp (sequential reads) and adds them.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce(const uint32_t* __restrict__ a1,
+                                                             const uint32_t* __restrict__ b1,
+                                                             const uint32_t* __restrict__ a2,
+                                                             const uint32_t* __restrict__ idx, uint64_t n,
+                                                             uint2* __restrict__ tmp, uint64_t* __restrict__ sums1,
+                                                             uint64_t* __restrict__ sums2) {
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint64_t s1 = 0, s2 = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint64_t k = base + (uint64_t)i * SCAN_BLOCK + threadIdx.x;
+        if (k < n) {
+            const uint32_t p = idx[k];
+            const uint32_t v1 = a1[p] + (b1 ? b1[k] : 0u), v2 = a2[p];
+            tmp[k] = make_uint2(v1, v2);
+            s1 += v1;
+            s2 += v2;
+        }
+    }
+    uint64_t t1, t2;
+    block_excl_scan(s1, t1);
+    block_excl_scan(s2, t2);
+    if (threadIdx.x == 0) {
+        sums1[blockIdx.x] = t1;
+        sums2[blockIdx.x] = t2;
+    }
+}
+
+// One block: exclusive scans, in place, of both tile-sum arrays (nb <= SCAN_TILE entries each).
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_tiles(uint64_t* __restrict__ sums1, uint64_t* __restrict__ sums2,
+                                                            uint32_t nb) {
+    for (int c = 0; c < 2; c++) {
+        uint64_t* v = c ? sums2 : sums1;
+        const uint32_t b = threadIdx.x * SCAN_ITEMS;
+        uint64_t x[SCAN_ITEMS];
+        uint64_t s = 0;
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            x[i] = b + i < nb ? v[b + i] : 0;
+            s += x[i];
+        }
+        uint64_t tot;
+        uint64_t run = block_excl_scan(s, tot);
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            if (b + i < nb) v[b + i] = run;
+            run += x[i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restrict__ tmp, uint64_t n,
+                                                            const uint64_t* __restrict__ offs1,
+                                                            const uint64_t* __restrict__ offs2,
+                                                            uint32_t* __restrict__ out1, uint32_t* __restrict__ out2,
+                                                            unsigned long long* total1, unsigned long long* total2) {
+    // +1 padding per 16 entries: the per-lane contiguous walk hits distinct banks
+    __shared__ uint32_t t1[SCAN_TILE + SCAN_TILE / 16], t2[SCAN_TILE + SCAN_TILE / 16];
+    auto at = [](uint32_t k) { return k + (k >> 4); };
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
+        const uint2 v = base + k < n ? tmp[base + k] : make_uint2(0u, 0u);
+        t1[at(k)] = v.x;
+        t2[at(k)] = v.y;
+    }
+    __syncthreads();
+    uint64_t s1 = 0, s2 = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        s1 += t1[at(threadIdx.x * SCAN_ITEMS + i)];
+        s2 += t2[at(threadIdx.x * SCAN_ITEMS + i)];
+    }
+    uint64_t tot1, tot2;
+    const uint64_t o1 = offs1[blockIdx.x], o2 = offs2[blockIdx.x];
+    uint64_t r1 = block_excl_scan(s1, tot1) + o1;
+    uint64_t r2 = block_excl_scan(s2, tot2) + o2;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint32_t k = at(threadIdx.x * SCAN_ITEMS + i);
+        const uint32_t v1 = t1[k], v2 = t2[k];
+        t1[k] = (uint32_t)r1;
+        t2[k] = (uint32_t)r2;
+        r1 += v1;
+        r2 += v2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
+        if (base + k < n) {
+            out1[base + k] = t1[at(k)];
+            out2[base + k] = t2[at(k)];
+        }
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        out1[n] = (uint32_t)(o1 + tot1);
+        out2[n] = (uint32_t)(o2 + tot2);
+        *total1 = o1 + tot1;
+        *total2 = o2 + tot2;
+    }
+}
+
 }  // namespace
+
+hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32_t* a2, const uint32_t* idx,
+                            uint64_t n, uint32_t* out1, uint32_t* out2, unsigned long long* total1,
+                            unsigned long long* total2, uint2* tmp, uint64_t* sums, hipStream_t st) {
+    const uint64_t nb64 = n ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
+    if (nb64 > SCAN_TILE) return hipErrorInvalidValue;  // callers scan such sizes with scan_u32
+    const uint32_t nb = (uint32_t)nb64;
+    k_scan2_reduce<<<nb, SCAN_BLOCK, 0, st>>>(a1, b1, a2, idx, n, tmp, sums, sums + nb);
+    k_scan2_tiles<<<1, SCAN_BLOCK, 0, st>>>(sums, sums + nb, nb);
+    k_scan2_apply<<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2);
+    return hipGetLastError();
+}
 
 uint64_t scan_scratch_elems(uint64_t n) {
     uint64_t total = 0;
