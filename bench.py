@@ -121,6 +121,19 @@ def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False):
     return t1 - t0, timing, info
 
 
+def run_bed(steps, warmup):
+    """The large bed: 8M spheres with lognormal radii and 4 materials, packed, a top plate in a prescribed family
+    moving down at 0.05 m/s, dt = 2e-6 s, contact detection every step."""
+    from deme_amd import scenes
+
+    sc = scenes.polydisperse_bed()
+    t, timing, info = run_device(sc, steps, warmup, 1)
+    top = sorted(timing.items(), key=lambda kv: -kv[1][0])[:6]
+    return {"spheres": sc.n_spheres, "contacts": info["C"], "steps": steps,
+            "sphere_steps_per_s": sc.n_spheres * steps / t, "ms_per_step": 1e3 * t / steps,
+            "kernels_us": {k: round(1e3 * v[0] / max(v[1], 1), 1) for k, v in top}}
+
+
 def run_batch(n_scenes, steps, warmup, update_freq, beta):
     """Batched independent settling columns: one context stepping all scenes together, against the plain loop that
     steps each scene in a context of its own (same world, same physics: each scene's trajectory is identical)."""
@@ -260,6 +273,8 @@ def main():
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-jit", action="store_true")
     ap.add_argument("--skip-inspect", action="store_true")
+    ap.add_argument("--skip-bed", action="store_true")
+    ap.add_argument("--bed-steps", type=int, default=50)
     ap.add_argument("--adjoint-spheres", type=int, default=50_000)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-window", type=int, default=20)
@@ -296,6 +311,10 @@ def main():
                 "kernels_us_per_step": {k: round(1e3 * v[0] / args.steps, 2)
                                         for k, v in sorted(timingB.items(), key=lambda kv: -kv[1][0])},
                 "device_ms_per_step": round(totB / args.steps, 4)}
+
+    bed = None
+    if not args.skip_bed:
+        bed = run_bed(args.bed_steps, 10)
 
     batch = None
     if not args.skip_batch:
@@ -361,6 +380,7 @@ def main():
                    "run_every5_beta0.2r": runB, "jit_specialised": jit, "batched_columns": batch,
                    "adjoint_drop": adjoint,
                    "inspectors": infoA["inspectors"],
+                   "polydisperse_bed_8M": bed,
                    "kernels": kernels},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": traffic,
