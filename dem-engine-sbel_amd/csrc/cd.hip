@@ -336,9 +336,9 @@ __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t
                                                      const uint8_t* __restrict__ prevType, uint32_t* __restrict__ map,
                                                      const uint32_t* __restrict__ bRowStart,
                                                      const uint32_t* __restrict__ bPart,
-                                                     uint32_t* __restrict__ mirror) {
+                                                     uint32_t* __restrict__ mirror, SpecGuard guard) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nS) return;
+    if (s >= nS || spec_skip(guard)) return;
     const uint32_t ns = rowStart[s], ne = rowStart[s + 1];
     uint32_t j = prevRowStart ? prevRowStart[s] : 0;
     const uint32_t je = prevRowStart ? prevRowStart[s + 1] : 0;
@@ -404,20 +404,21 @@ __global__ __launch_bounds__(256) void k_brow_sort(uint32_t nS, const uint32_t* 
     }
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v, uint32_t* p2, uint32_t n2) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = t; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+    if (t < n2) p2[t] = v;
 }
 
 inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
 }  // namespace
 
-hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    uint32_t nb = blocks(n, 256);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st, uint32_t* p2, uint32_t n2) {
+    if (n == 0 && n2 == 0) return hipSuccess;
+    uint32_t nb = blocks(n > n2 ? n : n2, 256);
     if (nb > 65536) nb = 65536;
-    k_fill_u32<<<nb, 256, 0, st>>>(p, n, v);
+    k_fill_u32<<<nb, 256, 0, st>>>(p, n, v, p2, n2);
     return hipGetLastError();
 }
 
@@ -488,10 +489,10 @@ hipError_t launch_row_finalize(uint32_t nS, const uint32_t* rowStart, const uint
 hipError_t launch_history_map(uint32_t nS, const uint32_t* rowStart, const uint32_t* idB, const uint8_t* ctype,
                               const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
                               uint32_t* map, const uint32_t* bRowStart, const uint32_t* bPart, uint32_t* mirror,
-                              hipStream_t st) {
+                              hipStream_t st, SpecGuard guard) {
     if (nS == 0) return hipSuccess;
     k_history_map<<<blocks(nS, 256), 256, 0, st>>>(nS, rowStart, idB, ctype, prevRowStart, prevIdB, prevType, map,
-                                                   bRowStart, bPart, mirror);
+                                                   bRowStart, bPart, mirror, guard);
     return hipGetLastError();
 }
 

@@ -491,9 +491,9 @@ __global__ __launch_bounds__(256) void k_cell_rows(KParams P, const SphereRec* _
                                                    const unsigned long long* __restrict__ planeBits,
                                                    const uint32_t* __restrict__ bRowStart, uint32_t* __restrict__ idA,
                                                    uint32_t* __restrict__ idB, uint8_t* __restrict__ ctype,
-                                                   uint32_t* __restrict__ bPart) {
+                                                   uint32_t* __restrict__ bPart, SpecGuard guard) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nS) return;
+    if (s >= nS || spec_skip(guard)) return;
     const uint32_t st = rowStart[s], en = rowStart[s + 1];
     uint32_t pos = st;
     unsigned long long bits = planeBits[s];
@@ -553,10 +553,11 @@ hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, c
 hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                             const uint32_t* rowSlab, const uint32_t* mirSlab, const uint32_t* posOf,
                             const uint32_t* rowStart, const unsigned long long* planeBits, const uint32_t* bRowStart,
-                            uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* bPart, hipStream_t st) {
+                            uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* bPart, hipStream_t st,
+                            SpecGuard guard) {
     if (nS == 0) return hipSuccess;
     k_cell_rows<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, rowSlab, mirSlab, posOf, rowStart, planeBits,
-                                                 bRowStart, idA, idB, ctype, bPart);
+                                                 bRowStart, idA, idB, ctype, bPart, guard);
     return hipGetLastError();
 }
 

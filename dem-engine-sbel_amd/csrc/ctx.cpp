@@ -146,6 +146,7 @@ struct deme_ctx {
     DBuf<uint64_t> scan2Sums;   // its tile sums
     DBuf<DevCounters> ctr;
     DevCounters* hctr = nullptr;  // pinned mirror
+    hipEvent_t evCounts = nullptr;  // recorded after a detection pass's counters are copied to hctr
 
     // double-buffered contact arrays (index cur = current list, cur ^ 1 = previous list)
     DBuf<uint32_t> rowStart[2], idB[2];
@@ -373,6 +374,11 @@ int deme_ctx_create(int device, void* stream, deme_ctx** out) {
         return DEME_ERR_HIP;
     }
     std::memset(c->hctr, 0, sizeof(DevCounters));
+    if (hipEventCreateWithFlags(&c->evCounts, hipEventDisableTiming) != hipSuccess) {
+        c->evCounts = nullptr;
+        deme_ctx_destroy(c);
+        return DEME_ERR_HIP;
+    }
     *out = c;
     return DEME_OK;
 }
@@ -414,6 +420,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->inspPart.release(); c->ownerVol.release(); c->scanTmp.release(); c->scan2Sums.release();
     if (c->stepExec) (void)hipGraphExecDestroy(c->stepExec);
     if (c->hctr) (void)hipHostFree(c->hctr);
+    if (c->evCounts) (void)hipEventDestroy(c->evCounts);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
     delete c;
     return DEME_OK;
@@ -899,6 +906,22 @@ static int setup_grid(deme_ctx* c) {
     return DEME_OK;
 }
 
+// The canonical rows and mirror rows of the cell path, then the history map against the previous list.
+static int enqueue_rows(deme_ctx* c, SpecGuard guard) {
+    const int cu = c->cur, pv = cu ^ 1;
+    const uint32_t nS = c->nS;
+    DEME_LAUNCH(c, "cell_rows",
+                launch_cell_rows(c->kp, c->sph.p, c->own.p, nS, c->rowSlab.p, c->mirSlab.p, c->posOf.p,
+                                 c->rowStart[cu].p, c->planeBits.p, c->bRowStart.p, c->idA.p, c->idB[cu].p,
+                                 c->ctype[cu].p, c->bPart.p, c->stream, guard));
+    DEME_LAUNCH(c, "history_map",
+                launch_history_map(nS, c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p,
+                                   c->havePrev ? c->rowStart[pv].p : nullptr, c->havePrev ? c->idB[pv].p : nullptr,
+                                   c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->bRowStart.p, c->bPart.p,
+                                   c->mirror.p, c->stream, guard));
+    return DEME_OK;
+}
+
 int deme_cd_run(deme_ctx* c) {
     c->graphGen++;
     int r = check_ready(c);
@@ -919,9 +942,11 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->survSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
-    DEME_HIP(c, hipMemsetAsync(c->ctr.p, 0, sizeof(DevCounters), c->stream));
     DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
-    DEME_LAUNCH(c, "fill", launch_fill_u32(c->cellCount.p, c->nCells, 0, c->stream));
+    // cell counts and the pass's counters cleared by one kernel (a small memset is two fills on this runtime)
+    static_assert(sizeof(DevCounters) % 4 == 0, "counters are cleared as 32-bit words");
+    DEME_LAUNCH(c, "fill", launch_fill_u32(c->cellCount.p, c->nCells, 0, c->stream,
+                                           reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(DevCounters) / 4));
     DEME_LAUNCH(c, "cell_assign",
                 launch_cell_assign(c->kp, c->grid, c->sph.p, c->own.p, nS, c->pose.p, c->nA,
                                    c->maskAny ? c->mask.p : nullptr,
@@ -941,7 +966,8 @@ int deme_cd_run(deme_ctx* c) {
     }
     c->contactsReady = false;
     c->cdStage = 0;
-    const int cu = c->cur, pv = cu ^ 1;
+    const int cu = c->cur;
+    SpecGuard spec;
     for (int attempt = 0;; attempt++) {
         DEME_LAUNCH(c, "cell_pairs",
                     launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, c->sortedMeta.p, c->cellStart.p,
@@ -962,8 +988,21 @@ int deme_cd_run(deme_ctx* c) {
                         scan_u32(c->bCnt.p, nullptr, nS, c->bRowStart.p, &c->ctr.p->total[2], c->scanScratch.p,
                                  c->stream, c->posOf.p));
         }
-        r = readback_counters(c);
-        if (r) return r;
+        // The counters go to the host without a stream-wide wait: the rows and the history map are enqueued behind
+        // them with the buffers at hand (skipped on the device if the pass failed or the lists do not fit), so the
+        // GPU keeps working while the host reads the totals.
+        DEME_HIP(c, hipMemcpyAsync(c->hctr, c->ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+        DEME_HIP(c, hipEventRecord(c->evCounts, c->stream));
+        spec.ctr = nullptr;
+        if (c->idB[cu].p && c->ctype[cu].p && c->idA.p && c->map.p && c->mirror.p && c->bPart.p) {
+            spec.ctr = c->ctr.p;
+            spec.slabCap = c->slabCap;
+            spec.capC = std::min({c->idB[cu].cap, c->ctype[cu].cap, c->idA.cap, c->map.cap, c->mirror.cap});
+            spec.capP = c->bPart.cap;
+            r = enqueue_rows(c, spec);
+            if (r) return r;
+        }
+        DEME_HIP(c, hipEventSynchronize(c->evCounts));
         const DevCounters& h = *c->hctr;
         if (h.err) {
             c->havePrev = false;
@@ -999,15 +1038,11 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->gTerm.ensure((size_t)6 * C));
     DEME_HIP(c, c->bTerm.ensure((size_t)6 * nPairs));
     c->termsReady = false;
-    DEME_LAUNCH(c, "cell_rows",
-                launch_cell_rows(c->kp, c->sph.p, c->own.p, nS, c->rowSlab.p, c->mirSlab.p, c->posOf.p,
-                                 c->rowStart[cu].p, c->planeBits.p, c->bRowStart.p, c->idA.p, c->idB[cu].p,
-                                 c->ctype[cu].p, c->bPart.p, c->stream));
-    DEME_LAUNCH(c, "history_map",
-                launch_history_map(nS, c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p,
-                                   c->havePrev ? c->rowStart[pv].p : nullptr, c->havePrev ? c->idB[pv].p : nullptr,
-                                   c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, c->bRowStart.p, c->bPart.p,
-                                   c->mirror.p, c->stream));
+    // the rows and the map already ran unless the lists outgrew the buffers they were enqueued with
+    if (!spec.ctr || C > spec.capC || nPairs > spec.capP) {
+        r = enqueue_rows(c, SpecGuard());
+        if (r) return r;
+    }
     c->contactsReady = true;
     c->needDetect = false;
     c->fresh = true;

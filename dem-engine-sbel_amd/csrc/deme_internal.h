@@ -45,6 +45,18 @@ struct DevCounters {
 
 enum : uint32_t { ERRBIT_CAPACITY = 1u, ERRBIT_PAIR_OVERFLOW = 2u, ERRBIT_OUTSIDE = 4u };
 
+// Kernels enqueued before the host has read a detection pass's counters (so the GPU does not wait for the host)
+// exit at once, writing nothing, when the pass failed, a slab row overflowed, or the contact / mirror-row totals
+// exceed the buffers they were given; the host then redoes them. ctr == nullptr: no guard.
+struct SpecGuard {
+    const DevCounters* ctr = nullptr;
+    uint32_t slabCap = 0;
+    unsigned long long capC = 0, capP = 0;
+};
+__device__ __forceinline__ bool spec_skip(const SpecGuard& g) {
+    return g.ctr && (g.ctr->err || g.ctr->maxOcc > g.slabCap || g.ctr->total[1] > g.capC || g.ctr->total[2] > g.capP);
+}
+
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr uint64_t SCAN_TILE = (uint64_t)SCAN_BLOCK * SCAN_ITEMS;
@@ -89,7 +101,7 @@ hipError_t launch_row_finalize(uint32_t nS, const uint32_t* rowStart, const uint
 hipError_t launch_history_map(uint32_t nS, const uint32_t* rowStart, const uint32_t* idB, const uint8_t* ctype,
                               const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
                               uint32_t* map, const uint32_t* bRowStart, const uint32_t* bPart, uint32_t* mirror,
-                              hipStream_t st);
+                              hipStream_t st, SpecGuard guard = SpecGuard());
 // mirror rows from a finished contact list (counts in bCnt, consumed)
 hipError_t launch_brow_fill(uint32_t nC, uint32_t nS, const uint32_t* idA, const uint32_t* idB, const uint8_t* ctype,
                             const uint32_t* bRowStart, uint32_t* bCnt, uint32_t* bPart, hipStream_t st);
@@ -115,7 +127,8 @@ hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, c
 hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                             const uint32_t* rowSlab, const uint32_t* mirSlab, const uint32_t* posOf,
                             const uint32_t* rowStart, const unsigned long long* planeBits, const uint32_t* bRowStart,
-                            uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* bPart, hipStream_t st);
+                            uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* bPart, hipStream_t st,
+                            SpecGuard guard = SpecGuard());
 
 // dyn.hip
 // aTerm / gTerm: 6 floats per contact; bTerm: 6 floats per mirror entry; out (optional): full contact records
@@ -135,7 +148,9 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                const float* analAcc, const deme_family_rule* rules, OwnerRec* own,
                                float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate, float t,
                                hipStream_t st, OwnerRec* ownOut = nullptr /* integrated records; null: in place */);
-hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+// p[0..n) = v, and p2[0..n2) = v in the same launch (n2 <= 256 * blocks: small second ranges only)
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st, uint32_t* p2 = nullptr,
+                           uint32_t n2 = 0);
 hipError_t launch_change_family(OwnerRec* own, uint32_t nO, uint8_t from, uint8_t to, hipStream_t st);
 
 // jit_rt.cpp: the force kernel specialised at run time by hipRTC

@@ -22,7 +22,8 @@ namespace {
 
 // Row a16 for analytical owners (planes): every sphere-plane contact contributes its B-side term (stored by the
 // force kernel) to the plane's owner. Deterministic two-level reduction: lane t of a tile sums contacts t, t + 256, ...
-// (16 of them) in order, lanes combine in a fixed tree, tiles combine in a fixed tree in k_anal_final. The contact
+// (16 of them) in order, lanes combine by wave butterflies and the four waves in order, tiles combine in a fixed tree
+// in k_anal_final. The contact
 // types of a lane's 16 contacts are loaded together (coalesced across the wave), so the gathers of the few plane
 // contacts are the only dependent loads.
 constexpr uint32_t ANAL_ITEMS = 16;
@@ -34,7 +35,7 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
                                                       const deme_anal_obj* __restrict__ anal, uint32_t nA,
                                                       uint32_t firstAnal, uint32_t nAnalOwners,
                                                       float* __restrict__ partials) {
-    __shared__ float red[256][6];
+    __shared__ float wsum[4][6];
     float acc[MAX_ANAL_OWNERS][6];
 #pragma unroll
     for (uint32_t k = 0; k < MAX_ANAL_OWNERS; k++)
@@ -48,13 +49,19 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
         const uint64_t c = c0 + (uint64_t)it * 256;
         ty[it] = c < nC ? ctype[c] : (uint8_t)DEME_NOT_A_CONTACT;
     }
+    // the dependent gathers are issued a stage at a time for all the lane's contacts (component id, its owner, the
+    // term), so their latencies overlap instead of adding up per contact
+    uint32_t own[ANAL_ITEMS];
+#pragma unroll
+    for (uint32_t it = 0; it < ANAL_ITEMS; it++)
+        own[it] = ty[it] == DEME_SPHERE_PLANE_CONTACT ? idB[c0 + (uint64_t)it * 256] : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t it = 0; it < ANAL_ITEMS; it++) own[it] = own[it] < nA ? anal[own[it]].owner - firstAnal : 0xFFFFFFFFu;
 #pragma unroll
     for (uint32_t it = 0; it < ANAL_ITEMS; it++) {
         const uint64_t c = c0 + (uint64_t)it * 256;
-        if (ty[it] != DEME_SPHERE_PLANE_CONTACT) continue;
-        const uint32_t b = idB[c];
-        if (b >= nA) continue;
-        const uint32_t k = anal[b].owner - firstAnal;
+        if (own[it] == 0xFFFFFFFFu) continue;
+        const uint32_t k = own[it];
         const float2* g = reinterpret_cast<const float2*>(gTerm + c * 6);
         const float2 g0 = g[0], g1 = g[1], g2 = g[2];
         any = true;
@@ -75,19 +82,24 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
         if (threadIdx.x < nAnalOwners * 6) partials[(uint64_t)blockIdx.x * nAnalOwners * 6 + threadIdx.x] = 0.f;
         return;
     }
+    // per owner: 64-lane butterflies, then the four waves in order (a fixed order: deterministic)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (uint32_t k = 0; k < MAX_ANAL_OWNERS; k++) {
         if (k >= nAnalOwners) break;
 #pragma unroll
-        for (int c = 0; c < 6; c++) red[threadIdx.x][c] = acc[k][c];
-        __syncthreads();
-        for (uint32_t st = 128; st > 0; st >>= 1) {
-            if (threadIdx.x < st)
+        for (int c = 0; c < 6; c++) {
+            float v = acc[k][c];
 #pragma unroll
-                for (int c = 0; c < 6; c++) red[threadIdx.x][c] += red[threadIdx.x + st][c];
-            __syncthreads();
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) wsum[wave][c] = v;
         }
-        if (threadIdx.x < 6) partials[((uint64_t)blockIdx.x * nAnalOwners + k) * 6 + threadIdx.x] = red[0][threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const uint32_t c = threadIdx.x;
+            partials[((uint64_t)blockIdx.x * nAnalOwners + k) * 6 + c] =
+                ((wsum[0][c] + wsum[1][c]) + wsum[2][c]) + wsum[3][c];
+        }
         __syncthreads();
     }
 }
