@@ -155,6 +155,7 @@ struct deme_ctx {
     DBuf<uint32_t> idA, map;
     DBuf<ForceRec> frc;           // full contact records, kept by deme_force only
     DBuf<float> aTerm, bTerm, gTerm;  // owner terms: A side per contact, B side per mirror slot / per plane contact
+    DBuf<uint8_t> aOn, bOn;           // term written (the pair touches) per contact / per mirror slot
     bool forcesFull = false;
     // adjoint: checkpoints at window starts (+ the final state) and the recorded steps of one window
     struct AdjCk {
@@ -397,7 +398,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->binCount.release(); c->binStart.release(); c->numBins.release(); c->numAnal.release(); c->rec.release();
     c->binEnt.release(); c->pairs.release(); c->rowCount.release(); c->bCnt.release(); c->bRowStart.release();
     c->bPart.release(); c->mirror.release(); c->scanScratch.release(); c->ctr.release();
-    c->aTerm.release(); c->bTerm.release(); c->gTerm.release();
+    c->aTerm.release(); c->bTerm.release(); c->gTerm.release(); c->aOn.release(); c->bOn.release();
     for (int k = 0; k < 2; k++) {
         c->rowStart[k].release(); c->idB[k].release(); c->ctype[k].release(); c->hist[k].release();
     }
@@ -840,8 +841,10 @@ int deme_cd_sort_remap(deme_ctx* c) {
     DEME_HIP(c, c->mirror.ensure(C));
     DEME_HIP(c, c->bPart.ensure(c->nPairs));
     DEME_HIP(c, c->aTerm.ensure((size_t)6 * C));
+    DEME_HIP(c, c->aOn.ensure(C));
     DEME_HIP(c, c->gTerm.ensure((size_t)6 * C));
     DEME_HIP(c, c->bTerm.ensure((size_t)6 * c->nPairs));
+    DEME_HIP(c, c->bOn.ensure(c->nPairs));
     c->termsReady = false;
     DEME_LAUNCH(c, "row_fill",
                 launch_row_fill(c->pairs.p, (uint32_t)c->nPairs, c->rowStart[cu].p, c->numAnal.p, c->rowCount.p,
@@ -1035,8 +1038,10 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->mirror.ensure(C));
     DEME_HIP(c, c->bPart.ensure(nPairs));
     DEME_HIP(c, c->aTerm.ensure((size_t)6 * C));
+    DEME_HIP(c, c->aOn.ensure(C));
     DEME_HIP(c, c->gTerm.ensure((size_t)6 * C));
     DEME_HIP(c, c->bTerm.ensure((size_t)6 * nPairs));
+    DEME_HIP(c, c->bOn.ensure(nPairs));
     c->termsReady = false;
     // the rows and the map already ran unless the lists outgrew the buffers they were enqueued with
     if (!spec.ctr || C > spec.capC || nPairs > spec.capP) {
@@ -1109,6 +1114,14 @@ static int jit_rebuild(deme_ctx* c) {
     return DEME_OK;
 }
 
+// Touch flags for the owner terms: used when lists are detected with a drift margin (most pairs of such a list may
+// not touch, and their zero terms are then neither written nor read); without a margin every pair touches at
+// detection and every term is written.
+static uint8_t* term_flags(deme_ctx* c, bool bSide) {
+    if (!(c->sp.beta > 0.f)) return nullptr;
+    return bSide ? c->bOn.p : c->aOn.p;
+}
+
 // The force kernel of the context: the specialised one when a model or placeholder values were given, else the
 // ahead-of-time kernel of the built-in model.
 static hipError_t launch_force_ctx(deme_ctx* c, bool full, uint32_t nC, const uint32_t* idB, const uint8_t* ctype,
@@ -1116,10 +1129,11 @@ static hipError_t launch_force_ctx(deme_ctx* c, bool full, uint32_t nC, const ui
                                    ForceRec* out) {
     if (c->jitActive)
         return c->jit.launch(full, c->kp, nC, c->idA.p, idB, ctype, c->sph.p, own, c->moi.p, c->anal.p, c->pose.p,
-                             c->mats.p, histIn, map, histOut, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p, out,
-                             c->stream);
+                             c->mats.p, histIn, map, histOut, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p,
+                             term_flags(c, false), term_flags(c, true), out, c->stream);
     return launch_force(c->kp, nC, c->idA.p, idB, ctype, c->sph.p, own, c->moi.p, c->anal.p, c->pose.p, c->mats.p,
-                        histIn, map, histOut, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p, out, c->stream);
+                        histIn, map, histOut, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p, term_flags(c, false),
+                        term_flags(c, true), out, c->stream);
 }
 
 static int force_pass(deme_ctx* c, bool full) {
@@ -1158,11 +1172,13 @@ static int collect_and_update(deme_ctx* c, int integrate) {
     DEME_LAUNCH(c, integrate ? "owner_update" : "owner_reduce",
                 c->ownerJitActive
                     ? c->jitOwner.launch(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
-                                         c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, c->analAcc.p,
+                                         c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p,
+                                         term_flags(c, false), term_flags(c, true), c->analAcc.p,
                                          c->rules.p, c->own.p, c->own.p, c->acc.p, integrate, (float)c->time,
                                          c->stream)
                     : launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
-                                          c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, c->analAcc.p,
+                                          c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p,
+                                          term_flags(c, false), term_flags(c, true), c->analAcc.p,
                                           c->rules.p, c->own.p, c->acc.p, integrate, (float)c->time, c->stream));
     return DEME_OK;
 }
@@ -1399,6 +1415,8 @@ static int adj_restore(deme_ctx* c, const deme_ctx::AdjCk& k) {
             DEME_HIP(c, c->mirror.ensure(std::max<uint64_t>(k.C, 1)));
             DEME_HIP(c, c->bTerm.ensure(std::max<uint64_t>((uint64_t)6 * k.nPairs, 1)));
             DEME_HIP(c, c->aTerm.ensure(std::max<uint64_t>((uint64_t)6 * k.C, 1)));
+            DEME_HIP(c, c->bOn.ensure(std::max<uint64_t>(k.nPairs, 1)));
+            DEME_HIP(c, c->aOn.ensure(std::max<uint64_t>(k.C, 1)));
             DEME_HIP(c, c->gTerm.ensure(std::max<uint64_t>((uint64_t)6 * k.C, 1)));
             if (k.C) {
                 DEME_HIP(c, hipMemcpyAsync(c->idA.p, k.idA.p, k.C * 4, hipMemcpyDeviceToDevice, c->stream));
@@ -1461,7 +1479,9 @@ static int adj_window(deme_ctx* c, int n, bool record) {
                                                  c->adjHist[k].p, nullptr, c->adjHist[k + 1].p, nullptr));
         DEME_LAUNCH(c, "owner_update",
                     launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
-                                        c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, c->analAcc.p,
+                                        c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p,
+                                        term_flags(c, false), term_flags(c, true),
+                                          c->analAcc.p,
                                         c->rules.p, c->adjOwn[k].p, c->adjAcc[k].p, 1, (float)c->time, c->stream,
                                         c->adjOwn[k + 1].p));
         c->time += (double)c->sp.h;

@@ -136,6 +136,7 @@ hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, cons
                         const SphereRec* sph, const OwnerRec* own, const float4* moi, const deme_anal_obj* anal,
                         const PlanePose* pose, const deme_material* mats, const float4* histIn, const uint32_t* map,
                         float4* histOut, const uint32_t* mirror, float* aTerm, float* bTerm, float* gTerm,
+                        uint8_t* aOn, uint8_t* bOn,
                         ForceRec* out, hipStream_t st);
 hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const float* gTerm,
                                const deme_anal_obj* anal, uint32_t nA, uint32_t firstAnalOwner, uint32_t nAnalOwners,
@@ -145,6 +146,7 @@ hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint3
 hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnalOwner,
                                uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
                                const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
+                               const uint8_t* aOn, const uint8_t* bOn,
                                const float* analAcc, const deme_family_rule* rules, OwnerRec* own,
                                float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate, float t,
                                hipStream_t st, OwnerRec* ownOut = nullptr /* integrated records; null: in place */);
@@ -163,7 +165,8 @@ struct JitForce {
                       const uint8_t* ctype, const SphereRec* sph, const OwnerRec* own, const float4* moi,
                       const deme_anal_obj* anal, const PlanePose* pose, const deme_material* mats,
                       const float4* histIn, const uint32_t* map, float4* histOut, const uint32_t* mirror,
-                      float* aTerm, float* bTerm, float* gTerm, ForceRec* out, hipStream_t st) const;
+                      float* aTerm, float* bTerm, float* gTerm, uint8_t* aOn, uint8_t* bOn, ForceRec* out,
+                      hipStream_t st) const;
 };
 // the owner update kernel specialised with motion strategies / family change rules
 struct JitOwner {
@@ -173,7 +176,8 @@ struct JitOwner {
     hipError_t load(const std::vector<char>& code);
     hipError_t launch(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnal, uint32_t nAnalOwners,
                       const uint32_t* ownerSphStart, const uint32_t* rowStart, const uint32_t* bRowStart,
-                      const float* aTerm, const float* bTerm, const float* analAcc, const deme_family_rule* rules,
+                      const float* aTerm, const float* bTerm, const uint8_t* aOn, const uint8_t* bOn,
+                      const float* analAcc, const deme_family_rule* rules,
                       const OwnerRec* own, OwnerRec* ownOut, float* acc, int doIntegrate, float t,
                       hipStream_t st) const;
 };

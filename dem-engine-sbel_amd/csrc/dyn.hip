@@ -134,14 +134,15 @@ hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, cons
                         const SphereRec* sph, const OwnerRec* own, const float4* moi, const deme_anal_obj* anal,
                         const PlanePose* pose, const deme_material* mats, const float4* histIn, const uint32_t* map,
                         float4* histOut, const uint32_t* mirror, float* aTerm, float* bTerm, float* gTerm,
-                        ForceRec* out, hipStream_t st) {
+                        uint8_t* aOn, uint8_t* bOn, ForceRec* out, hipStream_t st) {
     if (nC == 0) return hipSuccess;
     if (out)
         deme_force_full<<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats, histIn,
-                                                       map, histOut, mirror, aTerm, bTerm, gTerm, out);
+                                                       map, histOut, mirror, aTerm, bTerm, gTerm, aOn, bOn, out);
     else
         deme_force_terms<<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats,
-                                                        histIn, map, histOut, mirror, aTerm, bTerm, gTerm, nullptr);
+                                                        histIn, map, histOut, mirror, aTerm, bTerm, gTerm, aOn, bOn,
+                                                        nullptr);
     return hipGetLastError();
 }
 
@@ -167,11 +168,12 @@ hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint3
 hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnalOwner,
                                uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
                                const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
-                               const float* analAcc, const deme_family_rule* rules, OwnerRec* own, float* acc,
+                               const uint8_t* aOn, const uint8_t* bOn, const float* analAcc, const deme_family_rule* rules, OwnerRec* own, float* acc,
                                int doIntegrate, float t, hipStream_t st, OwnerRec* ownOut) {
     if (nO == 0) return hipSuccess;
     deme_owner_update<<<blocks(nO, 256), 256, 0, st>>>(P, nO, nClumps, firstAnalOwner, nAnalOwners, ownerSphStart,
-                                                    rowStart, bRowStart, aTerm, bTerm, analAcc, rules, own,
+                                                    rowStart, bRowStart, aTerm, bTerm, aOn, bOn, analAcc, rules,
+                                                    own,
                                                     ownOut ? ownOut : own, acc, doIntegrate, t);
     return hipGetLastError();
 }

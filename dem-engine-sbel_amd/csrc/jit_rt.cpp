@@ -141,11 +141,12 @@ hipError_t JitForce::launch(bool wantFull, const KParams& P, uint32_t nC, const 
                             const uint8_t* ctype, const SphereRec* sph, const OwnerRec* own, const float4* moi,
                             const deme_anal_obj* anal, const PlanePose* pose, const deme_material* mats,
                             const float4* histIn, const uint32_t* map, float4* histOut, const uint32_t* mirror,
-                            float* aTerm, float* bTerm, float* gTerm, ForceRec* out, hipStream_t st) const {
+                            float* aTerm, float* bTerm, float* gTerm, uint8_t* aOn, uint8_t* bOn, ForceRec* out,
+                            hipStream_t st) const {
     if (nC == 0) return hipSuccess;
     KParams p = P;
-    void* args[] = {&p,     &nC,     &idA,    &idB,   &ctype,  &sph,   &own,   &moi,   &anal, &pose,
-                    &mats,  &histIn, &map,    &histOut, &mirror, &aTerm, &bTerm, &gTerm, &out};
+    void* args[] = {&p,    &nC,     &idA, &idB,     &ctype,  &sph,   &own,   &moi,   &anal, &pose, &mats,
+                    &histIn, &map, &histOut, &mirror, &aTerm, &bTerm, &gTerm, &aOn, &bOn, &out};
     const uint32_t grid = (nC + 255) / 256;
     return hipModuleLaunchKernel(wantFull ? full : terms, grid, 1, 1, 256, 1, 1, 0, st, args, nullptr);
 }
@@ -168,14 +169,14 @@ hipError_t JitOwner::load(const std::vector<char>& code) {
 
 hipError_t JitOwner::launch(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnal, uint32_t nAnalOwners,
                             const uint32_t* ownerSphStart, const uint32_t* rowStart, const uint32_t* bRowStart,
-                            const float* aTerm, const float* bTerm, const float* analAcc,
-                            const deme_family_rule* rules, const OwnerRec* own, OwnerRec* ownOut, float* acc,
-                            int doIntegrate, float t, hipStream_t st) const {
+                            const float* aTerm, const float* bTerm, const uint8_t* aOn, const uint8_t* bOn,
+                            const float* analAcc, const deme_family_rule* rules, const OwnerRec* own,
+                            OwnerRec* ownOut, float* acc, int doIntegrate, float t, hipStream_t st) const {
     if (nO == 0) return hipSuccess;
     KParams p = P;
-    void* args[] = {&p,     &nO,    &nClumps, &firstAnal, &nAnalOwners, &ownerSphStart, &rowStart, &bRowStart,
-                    &aTerm, &bTerm, &analAcc, &rules,     &own,         &ownOut,        &acc,      &doIntegrate,
-                    &t};
+    void* args[] = {&p,     &nO,   &nClumps, &firstAnal, &nAnalOwners, &ownerSphStart, &rowStart, &bRowStart,
+                    &aTerm, &bTerm, &aOn,    &bOn,       &analAcc,     &rules,         &own,      &ownOut,
+                    &acc,   &doIntegrate, &t};
     return hipModuleLaunchKernel(fn, (nO + 255) / 256, 1, 1, 256, 1, 1, 0, st, args, nullptr);
 }
 
