@@ -164,7 +164,8 @@ __global__ __launch_bounds__(256) void k_cell_scatter(uint32_t nS, const uint32_
                                                       const CdRec* __restrict__ rec, const uint2* __restrict__ meta,
                                                       CdRec* __restrict__ sorted, uint2* __restrict__ sortedMeta,
                                                       uint32_t* __restrict__ posOf, uint32_t* __restrict__ aCnt,
-                                                      uint32_t* __restrict__ bCnt) {
+                                                      uint32_t* __restrict__ bCnt, float beta,
+                                                      float4* __restrict__ sortedF) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
     const CdRec cr = load_cd(rec, s);
@@ -179,6 +180,12 @@ __global__ __launch_bounds__(256) void k_cell_scatter(uint32_t nS, const uint32_
     const uint4* s4 = reinterpret_cast<const uint4*>(&cr);
     d4[0] = s4[0];
     d4[1] = s4[1];
+    // the pair sweep's compact copy, when it uses one: position rounded to float, expanded radius float(r + beta)
+    if (sortedF) {
+        float rf = cr.r;
+        rf += beta;
+        sortedF[pos] = make_float4((float)cr.x, (float)cr.y, (float)cr.z, rf);
+    }
 }
 
 // Expanded radius as the reference's detection kernel forms it: float(r + beta), widened to double.
@@ -351,8 +358,10 @@ __device__ __forceinline__ void finish_pair(const KParams& P, double inv, uint32
     append_pair(bCnt, mirSlab, hiIdx, loId, nS, slabCap, ctr);
 }
 
+template <bool PRE>
 __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G, uint32_t nS, uint32_t nBlocks,
                                                            const CdRec* __restrict__ sorted,
+                                                           const float4* __restrict__ sortedF, float margin,
                                                            const uint2* __restrict__ sortedMeta,
                                                            const uint32_t* __restrict__ cellStart,
                                                            const uint8_t* __restrict__ mask, uint32_t slabCap,
@@ -374,6 +383,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
     uint32_t nc = 0;  // pairs parked in surv
     if (i < nS) {
         const CdRec me = load_cd(sorted, i);
+        const float4 mf = PRE ? sortedF[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t myOwner = sortedMeta[i].x;
         int cx, cy, cz;
         cell_of(G, me.x, me.y, me.z, cx, cy, cz);
@@ -395,7 +405,10 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
                 b = cellStart[c0];
                 e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
             }
-            // two candidates per iteration: both loads are in flight before either test
+            // two candidates per iteration: both loads are in flight before either test. With PRE, the float copies
+            // reject the pairs that are farther apart than their radii by more than the rounding margin, and only the
+            // survivors load the full records for the reference's double-precision test (worth it where most
+            // candidates are far: drift margins, wide radius spreads).
             auto test = [&](const CdRec& ot, uint32_t j) {
                 if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) return;
                 if (sortedMeta[j].x == myOwner) return;  // same clump: never a contact
@@ -409,12 +422,26 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
                 }
             };
             uint32_t j = b;
-            for (; j + 1 < e; j += 2) {
-                const CdRec o0 = load_cd(sorted, j), o1 = load_cd(sorted, j + 1);
-                test(o0, j);
-                test(o1, j + 1);
+            auto pre = [&](const float4& of) {
+                const float dx = mf.x - of.x, dy = mf.y - of.y, dz = mf.z - of.z;
+                const float lim = mf.w + of.w + margin;
+                return !(dx * dx + dy * dy + dz * dz > lim * lim);
+            };
+            if (PRE) {
+                for (; j + 1 < e; j += 2) {
+                    const float4 o0 = sortedF[j], o1 = sortedF[j + 1];
+                    if (pre(o0)) test(load_cd(sorted, j), j);
+                    if (pre(o1)) test(load_cd(sorted, j + 1), j + 1);
+                }
+                if (j < e && pre(sortedF[j])) test(load_cd(sorted, j), j);
+            } else {
+                for (; j + 1 < e; j += 2) {
+                    const CdRec o0 = load_cd(sorted, j), o1 = load_cd(sorted, j + 1);
+                    test(o0, j);
+                    test(o1, j + 1);
+                }
+                if (j < e) test(load_cd(sorted, j), j);
             }
-            if (j < e) test(load_cd(sorted, j), j);
         }
     }
     __syncthreads();
@@ -532,21 +559,28 @@ hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereR
 
 hipError_t launch_cell_scatter(uint32_t nS, const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
                                const CdRec* rec, const uint2* meta, CdRec* sorted, uint2* sortedMeta, uint32_t* posOf,
-                               uint32_t* aCnt, uint32_t* bCnt, hipStream_t st) {
+                               uint32_t* aCnt, uint32_t* bCnt, float beta, float4* sortedF, hipStream_t st) {
     if (nS == 0) return hipSuccess;
     k_cell_scatter<<<blocks(nS, 256), 256, 0, st>>>(nS, cellOf, slotOf, cellStart, rec, meta, sorted, sortedMeta,
-                                                    posOf, aCnt, bCnt);
+                                                    posOf, aCnt, bCnt, beta, sortedF);
     return hipGetLastError();
 }
 
 hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
+                             const float4* sortedF, float margin,
                              const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
                              uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
                              DevCounters* ctr, hipStream_t st) {
     if (nS == 0) return hipSuccess;
     const uint32_t nb = blocks(nS, PAIR_BLOCK);
-    k_cell_pairs<<<8 * ((nb + 7) / 8), PAIR_BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedMeta, cellStart, mask, slabCap,
-                                                            surv, rowSlab, mirSlab, aCnt, bCnt, ctr);
+    if (sortedF)
+        k_cell_pairs<true><<<8 * ((nb + 7) / 8), PAIR_BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedF, margin,
+                                                                      sortedMeta, cellStart, mask, slabCap, surv,
+                                                                      rowSlab, mirSlab, aCnt, bCnt, ctr);
+    else
+        k_cell_pairs<false><<<8 * ((nb + 7) / 8), PAIR_BLOCK, 0, st>>>(P, G, nS, nb, sorted, nullptr, 0.f,
+                                                                       sortedMeta, cellStart, mask, slabCap, surv,
+                                                                       rowSlab, mirSlab, aCnt, bCnt, ctr);
     return hipGetLastError();
 }
 

@@ -72,6 +72,12 @@ struct TimedLaunch {
     hipEvent_t s, e;
 };
 
+float mean_radius(const std::vector<float>& r) {
+    double sum = 0.0;
+    for (float x : r) sum += x;
+    return r.empty() ? 0.f : (float)(sum / (double)r.size());
+}
+
 }  // namespace
 
 struct deme_ctx {
@@ -142,6 +148,7 @@ struct deme_ctx {
     // contact's slot there
     DBuf<uint32_t> bCnt, bRowStart, bPart, mirror;
     DBuf<uint64_t> scanScratch;
+    DBuf<float4> sortedF;       // float copies of the cell-sorted records (the pair sweep's pre-reject)
     DBuf<uint2> scanTmp;        // gathered (row, mirror-row) counts of the fused scan
     DBuf<uint64_t> scan2Sums;   // its tile sums
     DBuf<DevCounters> ctr;
@@ -184,7 +191,7 @@ struct deme_ctx {
     // cell-list detection
     CellGrid grid{};
     uint64_t nCells = 0;
-    float maxRadius = 0.f;
+    float maxRadius = 0.f, meanRadius = 0.f;
     uint32_t slabCap = 32;
     double cellScale = 1.0;  // cell edge = cellScale x the largest interaction distance (>= 1)
     DBuf<uint32_t> cellCount, cellStart, cellOf, slotOf, posOf, aCnt, survSlab, rowSlab, mirSlab;
@@ -418,7 +425,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->jit.release();
     c->jitOwner.release();
     for (auto& kv : c->inspectJit) kv.second.release();
-    c->inspPart.release(); c->ownerVol.release(); c->scanTmp.release(); c->scan2Sums.release();
+    c->inspPart.release(); c->ownerVol.release(); c->scanTmp.release(); c->scan2Sums.release(); c->sortedF.release();
     if (c->stepExec) (void)hipGraphExecDestroy(c->stepExec);
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->evCounts) (void)hipEventDestroy(c->evCounts);
@@ -546,6 +553,7 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
     c->maxMat = std::max(c->maxMatSph, c->maxMatAnal);
     c->maxRadius = maxR;
     c->hostRadius.assign(sphRadius, sphRadius + nS);
+    c->meanRadius = mean_radius(c->hostRadius);
     c->needDetect = false;
     c->haveTemplates = true;
     c->haveState = false;
@@ -909,6 +917,14 @@ static int setup_grid(deme_ctx* c) {
     return DEME_OK;
 }
 
+// Rounding bound of the pair sweep's float pre-reject: coordinates (below the world's largest extent W) are rounded to
+// within 2^-24 W, so a pair's float distance errs by less than 2 * sqrt(3) * 2^-24 W plus the rounding of the float
+// arithmetic on distances far below W; 16 * 2^-24 W covers both.
+static float sweep_margin(const deme_sim_params& p) {
+    const double W = std::ldexp(p.voxelSize_exact, (int)std::max({p.nvXp2, p.nvYp2, p.nvZp2}));
+    return (float)(W * 16.0 / 16777216.0);
+}
+
 // The canonical rows and mirror rows of the cell path, then the history map against the previous list.
 static int enqueue_rows(deme_ctx* c, SpecGuard guard) {
     const int cu = c->cur, pv = cu ^ 1;
@@ -939,6 +955,11 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->meta.ensure(nS));
     DEME_HIP(c, c->sortedMeta.ensure(nS));
     DEME_HIP(c, c->sorted.ensure(nS));
+    // the float pre-reject of the pair sweep pays where most candidates are far from touching: radii spread wide
+    // enough that the cells (sized for the largest) hold many small spheres. With near-uniform radii (drift margin
+    // or not) the extra float copy costs more than the record loads it saves.
+    const bool pre = c->maxRadius > 1.5f * c->meanRadius;
+    if (pre) DEME_HIP(c, c->sortedF.ensure(nS));
     DEME_HIP(c, c->planeBits.ensure(nS));
     DEME_HIP(c, c->scanTmp.ensure(nS));
     DEME_HIP(c, c->scan2Sums.ensure(2 * ((nS + SCAN_TILE - 1) / SCAN_TILE + 1)));
@@ -960,7 +981,8 @@ int deme_cd_run(deme_ctx* c) {
                          c->stream));
     DEME_LAUNCH(c, "cell_scatter",
                 launch_cell_scatter(nS, c->cellOf.p, c->slotOf.p, c->cellStart.p, c->rec.p, c->meta.p, c->sorted.p,
-                                    c->sortedMeta.p, c->posOf.p, c->aCnt.p, c->bCnt.p, c->stream));
+                                    c->sortedMeta.p, c->posOf.p, c->aCnt.p, c->bCnt.p, c->sp.beta,
+                                    pre ? c->sortedF.p : nullptr, c->stream));
     // the current list becomes the previous one
     if (c->contactsReady) {
         c->cur ^= 1;
@@ -973,7 +995,9 @@ int deme_cd_run(deme_ctx* c) {
     SpecGuard spec;
     for (int attempt = 0;; attempt++) {
         DEME_LAUNCH(c, "cell_pairs",
-                    launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, c->sortedMeta.p, c->cellStart.p,
+                    launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, pre ? c->sortedF.p : nullptr,
+                                      sweep_margin(c->sp),
+                                      c->sortedMeta.p, c->cellStart.p,
                                       c->maskAny ? c->mask.p : nullptr,
                                       c->slabCap, c->survSlab.p, c->rowSlab.p, c->mirSlab.p, c->aCnt.p, c->bCnt.p,
                                       c->ctr.p, c->stream));
@@ -1896,6 +1920,7 @@ int deme_append_clumps(deme_ctx* c, uint32_t nNewSph, const uint32_t* sphOwner, 
     c->hostScene.insert(c->hostScene.begin() + nC0, nNew, 0u);
     for (auto& o : c->hostAnalOwner) o += nNew;
     c->hostRadius.insert(c->hostRadius.end(), sphRadius, sphRadius + nNewSph);
+    c->meanRadius = mean_radius(c->hostRadius);
     bool single = c->singleSpheres && nNewSph == nNew;
     for (uint32_t k = 0; k < nNewSph && single; k++)
         single = sphOwner[k] == k && sphRelPos[3 * k] == 0.f && sphRelPos[3 * k + 1] == 0.f &&
@@ -1959,6 +1984,7 @@ int deme_change_clump_sizes(deme_ctx* c, uint32_t n, const uint32_t* ids, const 
     }
     if (maxR != c->maxRadius) c->nCells = 0;
     c->maxRadius = maxR;
+    c->meanRadius = mean_radius(c->hostRadius);
     c->adjReady = false;
     c->graphGen++;
     return DEME_OK;

@@ -117,11 +117,14 @@ hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereR
 // are cleared for the pair pass
 hipError_t launch_cell_scatter(uint32_t nS, const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
                                const CdRec* rec, const uint2* meta, CdRec* sorted, uint2* sortedMeta, uint32_t* posOf,
-                               uint32_t* aCnt, uint32_t* bCnt, hipStream_t st);
+                               uint32_t* aCnt, uint32_t* bCnt, float beta, float4* sortedF, hipStream_t st);
 // Slabs are column-major by sorted position (entry k of sorted sphere i at slab[k * nS + i]). surv: prefilter
 // survivors; rowSlab / aCnt: higher-id partners (contact row); mirSlab / bCnt: lower-id partners (mirror row).
+// sortedF: float copies of the sorted records (position, expanded radius) for a pre-reject, or nullptr (none);
+// margin: bound on the float rounding of a pair's distance, so the float test only rejects pairs the exact test
+// would reject.
 hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
-                             const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
+                             const float4* sortedF, float margin, const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
                              uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
                              DevCounters* ctr, hipStream_t st);
 hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,

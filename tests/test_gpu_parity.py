@@ -43,6 +43,11 @@ def polydisperse():
     return scenes.polydisperse_bed(n=3000, velocity_sigma=0.05)
 
 
+def expanded_polydisperse():
+    # the lognormal bed with a drift margin of 0.2 median radii: the sweep's float pre-reject under an expanded radius
+    return scenes.polydisperse_bed(n=3000, velocity_sigma=0.05, beta=2e-4)
+
+
 def masked_families():
     # DisableContactBetweenFamilies(1, 2) on a packed bed of spheres split at random between families 1 and 2
     sc = scenes.sphere_settling(n=3000, spacing=1.98, velocity_sigma=0.05, omega_sigma=5.0, seed=1010)
@@ -52,7 +57,8 @@ def masked_families():
     return sc
 
 
-SCENES = [packed_spheres, packed_clumps, loose_box, expanded_clumps, crowded_cells, polydisperse, masked_families]
+SCENES = [packed_spheres, packed_clumps, loose_box, expanded_clumps, crowded_cells, polydisperse,
+          expanded_polydisperse, masked_families]
 
 
 def systems(make, pre_steps=0):
@@ -202,7 +208,8 @@ def test_integration_matches_oracle(make):
         assert np.abs(sd[k].astype(np.float64) - so[k]).max() <= 1e-5 * scale, k
 
 
-@pytest.mark.parametrize("make,freq", [(packed_spheres, 1), (loose_box, 1), (expanded_clumps, 5), (polydisperse, 1)])
+@pytest.mark.parametrize("make,freq", [(packed_spheres, 1), (loose_box, 1), (expanded_clumps, 5), (polydisperse, 1),
+                                       (expanded_polydisperse, 5)])
 def test_trajectory_parity(make, freq):
     import pyoracle
 
