@@ -409,9 +409,10 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
             // reject the pairs that are farther apart than their radii by more than the rounding margin, and only the
             // survivors load the full records for the reference's double-precision test (worth it where most
             // candidates are far: drift margins, wide radius spreads).
-            auto test = [&](const CdRec& ot, uint32_t j) {
+            // the candidate's owner is loaded with its record, so an overlapping pair costs no second round trip
+            auto test = [&](const CdRec& ot, uint32_t otOwner, uint32_t j) {
                 if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) return;
-                if (sortedMeta[j].x == myOwner) return;  // same clump: never a contact
+                if (otOwner == myOwner) return;  // same clump: never a contact
                 const uint32_t q = atomicAdd(&qLen, 1u);
                 if (q < PAIR_QUEUE) {
                     qI[q] = i;
@@ -430,17 +431,18 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
             if (PRE) {
                 for (; j + 1 < e; j += 2) {
                     const float4 o0 = sortedF[j], o1 = sortedF[j + 1];
-                    if (pre(o0)) test(load_cd(sorted, j), j);
-                    if (pre(o1)) test(load_cd(sorted, j + 1), j + 1);
+                    if (pre(o0)) test(load_cd(sorted, j), sortedMeta[j].x, j);
+                    if (pre(o1)) test(load_cd(sorted, j + 1), sortedMeta[j + 1].x, j + 1);
                 }
-                if (j < e && pre(sortedF[j])) test(load_cd(sorted, j), j);
+                if (j < e && pre(sortedF[j])) test(load_cd(sorted, j), sortedMeta[j].x, j);
             } else {
                 for (; j + 1 < e; j += 2) {
                     const CdRec o0 = load_cd(sorted, j), o1 = load_cd(sorted, j + 1);
-                    test(o0, j);
-                    test(o1, j + 1);
+                    const uint32_t w0 = sortedMeta[j].x, w1 = sortedMeta[j + 1].x;
+                    test(o0, w0, j);
+                    test(o1, w1, j + 1);
                 }
-                if (j < e) test(load_cd(sorted, j), j);
+                if (j < e) test(load_cd(sorted, j), sortedMeta[j].x, j);
             }
         }
     }
