@@ -99,7 +99,6 @@ __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, cons
                                                      uint32_t* __restrict__ cellOf, uint32_t* __restrict__ slotOf,
                                                      uint16_t* __restrict__ numBins, uint32_t* __restrict__ numAnal,
                                                      unsigned long long* __restrict__ planeBits,
-                                                     CdRec* __restrict__ rec, uint2* __restrict__ meta,
                                                      DevCounters* ctr) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
@@ -148,30 +147,31 @@ __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, cons
     const uint32_t cell = (uint32_t)cx + (uint32_t)cy * G.nx + (uint32_t)cz * G.nx * G.ny;
     cellOf[s] = cell;
     slotOf[s] = run_slot(cellCount, cell);
-    CdRec cr;
-    cr.x = X;
-    cr.y = Y;
-    cr.z = Z;
-    cr.r = sr.r;  // raw radius: the bin rule needs both (double)r + beta and float(r + beta)
-    cr.id = s;
-    rec[s] = cr;
-    meta[s] = make_uint2(sr.owner, o.fam);
 }
 
-__global__ __launch_bounds__(256) void k_cell_scatter(uint32_t nS, const uint32_t* __restrict__ cellOf,
+// The sorted CD record of each sphere, its position recomputed from the sphere and owner records exactly as the
+// assignment pass computed it (same function, same inputs): writing the records there and reading them back here
+// would move 240 bytes per sphere more than the recomputation's gathers.
+__global__ __launch_bounds__(256) void k_cell_scatter(KParams P, const SphereRec* __restrict__ sph,
+                                                      const OwnerRec* __restrict__ own, uint32_t nS,
+                                                      const uint32_t* __restrict__ cellOf,
                                                       const uint32_t* __restrict__ slotOf,
                                                       const uint32_t* __restrict__ cellStart,
-                                                      const CdRec* __restrict__ rec, const uint2* __restrict__ meta,
                                                       CdRec* __restrict__ sorted, uint2* __restrict__ sortedMeta,
                                                       uint32_t* __restrict__ posOf, uint32_t* __restrict__ aCnt,
                                                       uint32_t* __restrict__ bCnt, float beta,
                                                       float4* __restrict__ sortedF) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
-    const CdRec cr = load_cd(rec, s);
+    const SphereRec sr = load_sphere(sph, s);
+    const OwnerRec o = load_owner(own, sr.owner);
+    CdRec cr;
+    sphere_position(P, o, sr, cr.x, cr.y, cr.z);
+    cr.r = sr.r;  // raw radius: the bin rule needs both (double)r + beta and float(r + beta)
+    cr.id = s;
     const uint32_t pos = cellStart[cellOf[s]] + slotOf[s];
     posOf[s] = pos;
-    sortedMeta[pos] = meta[s];
+    sortedMeta[pos] = make_uint2(sr.owner, o.fam);
     // the pair pass accumulates its per-sphere counts with atomics
     aCnt[pos] = 0u;
     bCnt[pos] = 0u;
@@ -551,19 +551,19 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereRec* sph, const OwnerRec* own,
                               uint32_t nS, const PlanePose* pose, uint32_t nA, const uint8_t* mask,
                               uint32_t* cellCount, uint32_t* cellOf, uint32_t* slotOf, uint16_t* numBins,
-                              uint32_t* numAnal, unsigned long long* planeBits, CdRec* rec, uint2* meta,
-                              DevCounters* ctr, hipStream_t st) {
+                              uint32_t* numAnal, unsigned long long* planeBits, DevCounters* ctr, hipStream_t st) {
     if (nS == 0) return hipSuccess;
     k_cell_assign<<<blocks(nS, 256), 256, 0, st>>>(P, G, sph, own, nS, pose, nA, mask, cellCount, cellOf, slotOf,
-                                                   numBins, numAnal, planeBits, rec, meta, ctr);
+                                                   numBins, numAnal, planeBits, ctr);
     return hipGetLastError();
 }
 
-hipError_t launch_cell_scatter(uint32_t nS, const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
-                               const CdRec* rec, const uint2* meta, CdRec* sorted, uint2* sortedMeta, uint32_t* posOf,
-                               uint32_t* aCnt, uint32_t* bCnt, float beta, float4* sortedF, hipStream_t st) {
+hipError_t launch_cell_scatter(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                               const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
+                               CdRec* sorted, uint2* sortedMeta, uint32_t* posOf, uint32_t* aCnt, uint32_t* bCnt,
+                               float beta, float4* sortedF, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cell_scatter<<<blocks(nS, 256), 256, 0, st>>>(nS, cellOf, slotOf, cellStart, rec, meta, sorted, sortedMeta,
+    k_cell_scatter<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, cellOf, slotOf, cellStart, sorted, sortedMeta,
                                                     posOf, aCnt, bCnt, beta, sortedF);
     return hipGetLastError();
 }

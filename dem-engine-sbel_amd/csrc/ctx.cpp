@@ -952,7 +952,6 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->slotOf.ensure(nS));
     DEME_HIP(c, c->posOf.ensure(nS));
     DEME_HIP(c, c->aCnt.ensure(nS));
-    DEME_HIP(c, c->meta.ensure(nS));
     DEME_HIP(c, c->sortedMeta.ensure(nS));
     DEME_HIP(c, c->sorted.ensure(nS));
     // the float pre-reject of the pair sweep pays where most candidates are far from touching: radii spread wide
@@ -975,12 +974,12 @@ int deme_cd_run(deme_ctx* c) {
                 launch_cell_assign(c->kp, c->grid, c->sph.p, c->own.p, nS, c->pose.p, c->nA,
                                    c->maskAny ? c->mask.p : nullptr,
                                    c->cellCount.p, c->cellOf.p, c->slotOf.p, c->numBins.p, c->numAnal.p,
-                                   c->planeBits.p, c->rec.p, c->meta.p, c->ctr.p, c->stream));
+                                   c->planeBits.p, c->ctr.p, c->stream));
     DEME_LAUNCH(c, "scan_cells",
                 scan_u32(c->cellCount.p, nullptr, c->nCells, c->cellStart.p, &c->ctr.p->total[3], c->scanScratch.p,
                          c->stream));
     DEME_LAUNCH(c, "cell_scatter",
-                launch_cell_scatter(nS, c->cellOf.p, c->slotOf.p, c->cellStart.p, c->rec.p, c->meta.p, c->sorted.p,
+                launch_cell_scatter(c->kp, c->sph.p, c->own.p, nS, c->cellOf.p, c->slotOf.p, c->cellStart.p, c->sorted.p,
                                     c->sortedMeta.p, c->posOf.p, c->aCnt.p, c->bCnt.p, c->sp.beta,
                                     pre ? c->sortedF.p : nullptr, c->stream));
     // the current list becomes the previous one

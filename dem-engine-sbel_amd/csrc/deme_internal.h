@@ -111,13 +111,14 @@ hipError_t launch_brow_sort(uint32_t nS, const uint32_t* bRowStart, uint32_t* bP
 hipError_t launch_cell_assign(const KParams& P, const CellGrid& G, const SphereRec* sph, const OwnerRec* own,
                               uint32_t nS, const PlanePose* pose, uint32_t nA, const uint8_t* mask,
                               uint32_t* cellCount, uint32_t* cellOf, uint32_t* slotOf, uint16_t* numBins,
-                              uint32_t* numAnal, unsigned long long* planeBits, CdRec* rec, uint2* meta,
-                              DevCounters* ctr, hipStream_t st);
-// meta: (owner, family) per sphere; sortedMeta / sorted / posOf: the same in cell order; aCnt / bCnt (sorted order)
-// are cleared for the pair pass
-hipError_t launch_cell_scatter(uint32_t nS, const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
-                               const CdRec* rec, const uint2* meta, CdRec* sorted, uint2* sortedMeta, uint32_t* posOf,
-                               uint32_t* aCnt, uint32_t* bCnt, float beta, float4* sortedF, hipStream_t st);
+                              uint32_t* numAnal, unsigned long long* planeBits, DevCounters* ctr, hipStream_t st);
+// sorted: the CD records in cell order (positions recomputed from sph / own as the assignment computed them);
+// sortedMeta: (owner, family) in cell order; posOf: each sphere's sorted position; aCnt / bCnt (sorted order) are
+// cleared for the pair pass
+hipError_t launch_cell_scatter(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                               const uint32_t* cellOf, const uint32_t* slotOf, const uint32_t* cellStart,
+                               CdRec* sorted, uint2* sortedMeta, uint32_t* posOf, uint32_t* aCnt, uint32_t* bCnt,
+                               float beta, float4* sortedF, hipStream_t st);
 // Slabs are column-major by sorted position (entry k of sorted sphere i at slab[k * nS + i]). surv: prefilter
 // survivors; rowSlab / aCnt: higher-id partners (contact row); mirSlab / bCnt: lower-id partners (mirror row).
 // sortedF: float copies of the sorted records (position, expanded radius) for a pre-reject, or nullptr (none);
