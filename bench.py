@@ -50,11 +50,13 @@ def bytes_model(name, k):
         # per contact: ids + type, history in/out, force record out; owner and sphere records (unique footprint)
         "force": C * (9 + 16 + 16 + 48) + S * sph + O * owner_dyn + (C * 4 if k.get("fresh") else 0),
         "anal_collect": C * 5 + k["geo"] * (4 + 48),
-        # cell-list detection: per sphere template + owner pose in; CD record, cell, slot, bin count, plane count
-        # and plane bits out; one 4-byte RMW per occupied cell
-        "cell_assign": S * sph + O * owner_cd + S * (32 + 4 + 4 + 2 + 4 + 8) + Nc * 8,
+        # cell-list detection: per sphere template + owner pose in; cell, slot, bin count, plane count and plane bits
+        # out; one 4-byte RMW per occupied cell
+        "cell_assign": S * sph + O * owner_cd + S * (4 + 4 + 2 + 4 + 8) + Nc * 8,
         "scan_cells": Nc * 8,
-        "cell_scatter": S * (4 + 4 + 32 + 32) + Nc * 4,
+        # template + owner pose (the CD record recomputed), cell and slot in; sorted CD record, (owner, family),
+        # sorted position and the two cleared pair counts out; one cell start per sphere
+        "cell_scatter": S * sph + O * owner_cd + S * (4 + 4 + 32 + 8 + 4 + 8) + Nc * 4,
         # each CD record and cell boundary once, the lane's owner/family, rowCount out; per accepted partner a slab
         # entry, its owner/family and a 4-byte RMW of the owner's B-side count
         "cell_pairs": S * (32 + 4 + 4) + O * 4 + Nc * 4 + Np * (4 + 4 + 8),
