@@ -437,8 +437,11 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
                 if (j < e && pre(sortedF[j])) test(load_cd(sorted, j), sortedMeta[j].x, j);
             } else {
                 for (; j + 1 < e; j += 2) {
-                    const CdRec o0 = load_cd(sorted, j), o1 = load_cd(sorted, j + 1);
-                    const uint32_t w0 = sortedMeta[j].x, w1 = sortedMeta[j + 1].x;
+                    // one address per pair of candidates; the second record at a constant offset
+                    const CdRec* pj = sorted + j;
+                    const uint2* mj = sortedMeta + j;
+                    const CdRec o0 = load_cd(pj, 0), o1 = load_cd(pj, 1);
+                    const uint32_t w0 = mj[0].x, w1 = mj[1].x;
                     test(o0, w0, j);
                     test(o1, w1, j + 1);
                 }
