@@ -9,6 +9,9 @@
 namespace deme_int {
 namespace {
 
+// Up to this many tiles (8.4M elements) the apply pass folds each tile's offset from the raw tile sums itself.
+constexpr uint64_t SCAN_FOLD_TILES = 2048;
+
 __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, int lane) {
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -67,7 +70,8 @@ template <typename TIn, typename TOut>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const uint32_t* __restrict__ gidx,
                                                            const TIn* __restrict__ b, uint64_t n,
                                                            const uint64_t* __restrict__ offsets, TOut* out,
-                                                           unsigned long long* totalOut) {
+                                                           unsigned long long* totalOut,
+                                                           const uint64_t* __restrict__ rawSums = nullptr) {
     // +1 padding per 16 entries keeps the per-lane contiguous walk free of 64-bit bank conflicts
     __shared__ uint64_t tile[SCAN_TILE + SCAN_TILE / 16];
     auto at = [](uint32_t k) { return k + (k >> 4); };
@@ -84,7 +88,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const u
     for (int i = 0; i < SCAN_ITEMS; i++) s += tile[at(threadIdx.x * SCAN_ITEMS + i)];
     uint64_t tot;
     uint64_t run = block_excl_scan(s, tot);
-    const uint64_t off = offsets ? offsets[blockIdx.x] : 0;
+    uint64_t off = offsets ? offsets[blockIdx.x] : 0;
+    if (rawSums) {
+        // the tile's offset folded here from the unscanned tile sums of the tiles before it (fixed order)
+        uint64_t part = 0;
+        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += SCAN_BLOCK) part += rawSums[t];
+        block_excl_scan(part, off);
+    }
     run += off;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) {
@@ -269,6 +279,12 @@ hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* 
     }
     uint64_t* sums = scratch;
     k_scan_reduce<uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, idx, b, n, sums);
+    if (nb <= SCAN_FOLD_TILES) {
+        // few tiles: each tile folds its own offset from the raw tile sums, no separate scan of the sums
+        k_scan_apply<uint32_t, uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, idx, b, n, nullptr, out, totalDev,
+                                                                             sums);
+        return hipGetLastError();
+    }
     hipError_t e = scan_u64_inplace(sums, nb, scratch + nb + 1, st);
     if (e != hipSuccess) return e;
     k_scan_apply<uint32_t, uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, idx, b, n, sums, out, totalDev);
