@@ -190,7 +190,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
                                                             const uint64_t* __restrict__ offs1,
                                                             const uint64_t* __restrict__ offs2,
                                                             uint32_t* __restrict__ out1, uint32_t* __restrict__ out2,
-                                                            unsigned long long* total1, unsigned long long* total2) {
+                                                            unsigned long long* total1, unsigned long long* total2,
+                                                            bool fold) {
     // +1 padding per 16 entries: the per-lane contiguous walk hits distinct banks
     __shared__ uint32_t t1[SCAN_TILE + SCAN_TILE / 16], t2[SCAN_TILE + SCAN_TILE / 16];
     auto at = [](uint32_t k) { return k + (k >> 4); };
@@ -210,7 +211,20 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
         s2 += t2[at(threadIdx.x * SCAN_ITEMS + i)];
     }
     uint64_t tot1, tot2;
-    const uint64_t o1 = offs1[blockIdx.x], o2 = offs2[blockIdx.x];
+    uint64_t o1, o2;
+    if (fold) {
+        // offs1 / offs2 hold the raw tile sums: fold the tiles before this one (fixed order)
+        uint64_t p1 = 0, p2 = 0;
+        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += SCAN_BLOCK) {
+            p1 += offs1[t];
+            p2 += offs2[t];
+        }
+        block_excl_scan(p1, o1);
+        block_excl_scan(p2, o2);
+    } else {
+        o1 = offs1[blockIdx.x];
+        o2 = offs2[blockIdx.x];
+    }
     uint64_t r1 = block_excl_scan(s1, tot1) + o1;
     uint64_t r2 = block_excl_scan(s2, tot2) + o2;
 #pragma unroll
@@ -248,8 +262,9 @@ hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32
     if (nb64 > SCAN_TILE) return hipErrorInvalidValue;  // callers scan such sizes with scan_u32
     const uint32_t nb = (uint32_t)nb64;
     k_scan2_reduce<<<nb, SCAN_BLOCK, 0, st>>>(a1, b1, a2, idx, n, tmp, sums, sums + nb);
-    k_scan2_tiles<<<1, SCAN_BLOCK, 0, st>>>(sums, sums + nb, nb);
-    k_scan2_apply<<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2);
+    const bool fold = nb <= SCAN_FOLD_TILES;
+    if (!fold) k_scan2_tiles<<<1, SCAN_BLOCK, 0, st>>>(sums, sums + nb, nb);
+    k_scan2_apply<<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, fold);
     return hipGetLastError();
 }
 
