@@ -360,13 +360,18 @@ __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t
         map[i] = m;
         uint32_t link = DEME_NULL_MAPPING_PARTNER;
         if (t == DEME_SPHERE_SPHERE_CONTACT && b < nS) {
-            uint32_t lo = bRowStart[b], hi = bRowStart[b + 1];
-            while (lo < hi) {
-                const uint32_t mid = lo + (hi - lo) / 2;
-                if (bPart[mid] < s) lo = mid + 1;
-                else hi = mid;
+            // B's mirror row (ascending, distinct ids) is short: read it four entries per round, all four loads in
+            // flight, instead of a binary search's chain of dependent loads
+            const uint32_t lo = bRowStart[b], hi = bRowStart[b + 1];
+            for (uint32_t q = lo; q < hi && link == DEME_NULL_MAPPING_PARTNER; q += 4) {
+                uint32_t v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) v[u] = q + u < hi ? bPart[q + u] : 0xFFFFFFFFu;
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (v[u] == s && link == DEME_NULL_MAPPING_PARTNER) link = q + (uint32_t)u;
+                if (v[3] > s) break;  // past s in an ascending row
             }
-            if (lo < bRowStart[b + 1] && bPart[lo] == s) link = lo;
         }
         mirror[i] = link;
     }
