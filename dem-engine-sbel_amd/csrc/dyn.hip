@@ -26,7 +26,7 @@ namespace {
 // in k_anal_final. The contact
 // types of a lane's 16 contacts are loaded together (coalesced across the wave), so the gathers of the few plane
 // contacts are the only dependent loads.
-constexpr uint32_t ANAL_ITEMS = 16;
+constexpr uint32_t ANAL_ITEMS = 4;
 constexpr uint32_t ANAL_TILE = 256 * ANAL_ITEMS;
 
 __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_t* __restrict__ idB,
