@@ -1159,7 +1159,9 @@ static hipError_t launch_force_ctx(deme_ctx* c, bool full, uint32_t nC, const ui
                         term_flags(c, true), out, c->stream);
 }
 
-static int force_pass(deme_ctx* c, bool full) {
+// poseFresh: a detection pass has just computed the plane poses from the same owner state (the same step), so the
+// force pass reuses them.
+static int force_pass(deme_ctx* c, bool full, bool poseFresh = false) {
     int r = check_ready(c);
     if (r) return r;
     if (c->jitActive && c->jitStale && (r = jit_rebuild(c))) return r;
@@ -1168,7 +1170,8 @@ static int force_pass(deme_ctx* c, bool full) {
     const int cu = c->cur, pv = cu ^ 1;
     const bool remap = c->fresh;
     if (full) DEME_HIP(c, c->frc.ensure(c->C));
-    DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
+    if (!poseFresh)
+        DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
     DEME_LAUNCH(c, "force",
                 launch_force_ctx(c, full, (uint32_t)c->C, c->idB[cu].p, c->ctype[cu].p, c->own.p,
                                  remap ? (c->havePrev ? c->hist[pv].p : c->hist[cu].p) : c->hist[cu].p,
@@ -1273,7 +1276,7 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
         if (!detect && !c->fresh && !records && c->useGraphs && !c->timing && !c->jitStale && c->stream) {
             if ((r = step_graph(c))) return r;
         } else {
-            r = force_pass(c, records);
+            r = force_pass(c, records, detect);
             if (r) return r;
             r = collect_and_update(c, 1);
             if (r) return r;
