@@ -961,7 +961,7 @@ int deme_cd_run(deme_ctx* c) {
     if (pre) DEME_HIP(c, c->sortedF.ensure(nS));
     DEME_HIP(c, c->planeBits.ensure(nS));
     DEME_HIP(c, c->scanTmp.ensure(nS));
-    DEME_HIP(c, c->scan2Sums.ensure(2 * ((nS + SCAN_TILE - 1) / SCAN_TILE + 1)));
+    DEME_HIP(c, c->scan2Sums.ensure(2 * ((nS + SCAN2_TILE - 1) / SCAN2_TILE + 1)));
     DEME_HIP(c, c->survSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
@@ -1001,7 +1001,7 @@ int deme_cd_run(deme_ctx* c) {
                                       c->slabCap, c->survSlab.p, c->rowSlab.p, c->mirSlab.p, c->aCnt.p, c->bCnt.p,
                                       c->ctr.p, c->stream));
         // row lengths in id order: partner counts gathered from sorted positions, plus plane contacts; mirror rows
-        if ((uint64_t)nS <= SCAN_TILE * SCAN_TILE) {
+        if ((uint64_t)nS <= SCAN2_TILE * SCAN_TILE) {
             DEME_LAUNCH(c, "scan_rows",
                         scan2_u32_gather(c->aCnt.p, c->numAnal.p, c->bCnt.p, c->posOf.p, nS, c->rowStart[cu].p,
                                          c->bRowStart.p, &c->ctr.p->total[1], &c->ctr.p->total[2], c->scanTmp.p,

@@ -60,6 +60,9 @@ __device__ __forceinline__ bool spec_skip(const SpecGuard& g) {
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr uint64_t SCAN_TILE = (uint64_t)SCAN_BLOCK * SCAN_ITEMS;
+// the gathered two-component scan uses half-size tiles: twice the workgroups to cover its gathers' latency
+constexpr int SCAN2_ITEMS = 8;
+constexpr uint64_t SCAN2_TILE = (uint64_t)SCAN_BLOCK * SCAN2_ITEMS;
 constexpr uint32_t MAX_ANAL_OWNERS = 8;
 
 // scan.hip: out[0..n] = exclusive prefix sum of (a[idx ? idx[i] : i] + (b ? b[i] : 0)); out[n] = total;
@@ -68,8 +71,8 @@ uint64_t scan_scratch_elems(uint64_t n);
 hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
                     uint64_t* scratch, hipStream_t st, const uint32_t* idx = nullptr);
 // Two scans over one gather: out1 = scan(a1[idx[i]] + (b1 ? b1[i] : 0)), out2 = scan(a2[idx[i]]), totals at out[n]
-// and in *total1 / *total2. tmp: n uint2; sums: 2 * ceil(n / SCAN_TILE) u64. n must be at most SCAN_TILE^2
-// (hipErrorInvalidValue otherwise).
+// and in *total1 / *total2. tmp: n uint2; sums: 2 * ceil(n / SCAN2_TILE) u64. n must be at most
+// SCAN2_TILE * SCAN_TILE (hipErrorInvalidValue otherwise).
 hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32_t* a2, const uint32_t* idx,
                             uint64_t n, uint32_t* out1, uint32_t* out2, unsigned long long* total1,
                             unsigned long long* total2, uint2* tmp, uint64_t* sums, hipStream_t st);

@@ -141,10 +141,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce(const uint32_t* __r
                                                              const uint32_t* __restrict__ idx, uint64_t n,
                                                              uint2* __restrict__ tmp, uint64_t* __restrict__ sums1,
                                                              uint64_t* __restrict__ sums2) {
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN2_TILE;
     uint64_t s1 = 0, s2 = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
+    for (int i = 0; i < SCAN2_ITEMS; i++) {
         const uint64_t k = base + (uint64_t)i * SCAN_BLOCK + threadIdx.x;
         if (k < n) {
             const uint32_t p = idx[k];
@@ -193,11 +193,11 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
                                                             unsigned long long* total1, unsigned long long* total2,
                                                             bool fold) {
     // +1 padding per 16 entries: the per-lane contiguous walk hits distinct banks
-    __shared__ uint32_t t1[SCAN_TILE + SCAN_TILE / 16], t2[SCAN_TILE + SCAN_TILE / 16];
+    __shared__ uint32_t t1[SCAN2_TILE + SCAN2_TILE / 16], t2[SCAN2_TILE + SCAN2_TILE / 16];
     auto at = [](uint32_t k) { return k + (k >> 4); };
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN2_TILE;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
+    for (int i = 0; i < SCAN2_ITEMS; i++) {
         const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
         const uint2 v = base + k < n ? tmp[base + k] : make_uint2(0u, 0u);
         t1[at(k)] = v.x;
@@ -206,9 +206,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
     __syncthreads();
     uint64_t s1 = 0, s2 = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        s1 += t1[at(threadIdx.x * SCAN_ITEMS + i)];
-        s2 += t2[at(threadIdx.x * SCAN_ITEMS + i)];
+    for (int i = 0; i < SCAN2_ITEMS; i++) {
+        s1 += t1[at(threadIdx.x * SCAN2_ITEMS + i)];
+        s2 += t2[at(threadIdx.x * SCAN2_ITEMS + i)];
     }
     uint64_t tot1, tot2;
     uint64_t o1, o2;
@@ -228,8 +228,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
     uint64_t r1 = block_excl_scan(s1, tot1) + o1;
     uint64_t r2 = block_excl_scan(s2, tot2) + o2;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        const uint32_t k = at(threadIdx.x * SCAN_ITEMS + i);
+    for (int i = 0; i < SCAN2_ITEMS; i++) {
+        const uint32_t k = at(threadIdx.x * SCAN2_ITEMS + i);
         const uint32_t v1 = t1[k], v2 = t2[k];
         t1[k] = (uint32_t)r1;
         t2[k] = (uint32_t)r2;
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
+    for (int i = 0; i < SCAN2_ITEMS; i++) {
         const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
         if (base + k < n) {
             out1[base + k] = t1[at(k)];
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
 hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32_t* a2, const uint32_t* idx,
                             uint64_t n, uint32_t* out1, uint32_t* out2, unsigned long long* total1,
                             unsigned long long* total2, uint2* tmp, uint64_t* sums, hipStream_t st) {
-    const uint64_t nb64 = n ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
+    const uint64_t nb64 = n ? (n + SCAN2_TILE - 1) / SCAN2_TILE : 1;
     if (nb64 > SCAN_TILE) return hipErrorInvalidValue;  // callers scan such sizes with scan_u32
     const uint32_t nb = (uint32_t)nb64;
     k_scan2_reduce<<<nb, SCAN_BLOCK, 0, st>>>(a1, b1, a2, idx, n, tmp, sums, sums + nb);
