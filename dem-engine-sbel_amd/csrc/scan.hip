@@ -49,14 +49,14 @@ __device__ __forceinline__ uint64_t scan_elem(const TIn* __restrict__ a, const u
     return v;
 }
 
-template <typename TIn>
+template <typename TIn, int ITEMS = SCAN_ITEMS>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const TIn* __restrict__ a, const uint32_t* __restrict__ gidx,
                                                             const TIn* __restrict__ b, uint64_t n,
                                                             uint64_t* __restrict__ sums) {
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * (uint64_t)(SCAN_BLOCK * ITEMS);
     uint64_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
+    for (int i = 0; i < ITEMS; i++) {
         const uint64_t idx = base + (uint64_t)i * SCAN_BLOCK + threadIdx.x;
         if (idx < n) s += scan_elem(a, gidx, b, idx);
     }
@@ -66,18 +66,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const TIn* __restric
 }
 
 // tile scan: loads striped (coalesced), scans per-lane contiguous runs, stores striped.
-template <typename TIn, typename TOut>
+template <typename TIn, typename TOut, int ITEMS = SCAN_ITEMS>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const uint32_t* __restrict__ gidx,
                                                            const TIn* __restrict__ b, uint64_t n,
                                                            const uint64_t* __restrict__ offsets, TOut* out,
                                                            unsigned long long* totalOut,
                                                            const uint64_t* __restrict__ rawSums = nullptr) {
     // +1 padding per 16 entries keeps the per-lane contiguous walk free of 64-bit bank conflicts
-    __shared__ uint64_t tile[SCAN_TILE + SCAN_TILE / 16];
+    __shared__ uint64_t tile[SCAN_BLOCK * ITEMS + SCAN_BLOCK * ITEMS / 16];
     auto at = [](uint32_t k) { return k + (k >> 4); };
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * (uint64_t)(SCAN_BLOCK * ITEMS);
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
+    for (int i = 0; i < ITEMS; i++) {
         const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
         const uint64_t idx = base + k;
         tile[at(k)] = idx < n ? scan_elem(a, gidx, b, idx) : 0;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const u
     __syncthreads();
     uint64_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) s += tile[at(threadIdx.x * SCAN_ITEMS + i)];
+    for (int i = 0; i < ITEMS; i++) s += tile[at(threadIdx.x * ITEMS + i)];
     uint64_t tot;
     uint64_t run = block_excl_scan(s, tot);
     uint64_t off = offsets ? offsets[blockIdx.x] : 0;
@@ -97,15 +97,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const TIn* a, const u
     }
     run += off;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        const uint32_t k = threadIdx.x * SCAN_ITEMS + i;
+    for (int i = 0; i < ITEMS; i++) {
+        const uint32_t k = threadIdx.x * ITEMS + i;
         const uint64_t v = tile[at(k)];
         tile[at(k)] = run;
         run += v;
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
+    for (int i = 0; i < ITEMS; i++) {
         const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
         const uint64_t idx = base + k;
         if (idx < n) out[idx] = (TOut)tile[at(k)];
@@ -277,7 +277,9 @@ uint64_t scan_scratch_elems(uint64_t n) {
         if (nb <= 1) break;
         m = nb;
     }
-    return total + 16;
+    // the half-size-tile path of scan_u32 keeps one sum per tile
+    const uint64_t half = (n + SCAN2_TILE - 1) / SCAN2_TILE + 1;
+    return (total > half ? total : half) + 16;
 }
 
 hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
@@ -293,13 +295,16 @@ hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* 
         return hipGetLastError();
     }
     uint64_t* sums = scratch;
-    k_scan_reduce<uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, idx, b, n, sums);
-    if (nb <= SCAN_FOLD_TILES) {
-        // few tiles: each tile folds its own offset from the raw tile sums, no separate scan of the sums
-        k_scan_apply<uint32_t, uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, idx, b, n, nullptr, out, totalDev,
-                                                                             sums);
+    const uint64_t nbHalf = (n + SCAN2_TILE - 1) / SCAN2_TILE;
+    if (nbHalf <= SCAN_FOLD_TILES) {
+        // up to SCAN_FOLD_TILES half-size tiles: twice the workgroups of full tiles, and each tile folds its own
+        // offset from the raw tile sums (no separate scan of the sums)
+        k_scan_reduce<uint32_t, SCAN2_ITEMS><<<(uint32_t)nbHalf, SCAN_BLOCK, 0, st>>>(a, idx, b, n, sums);
+        k_scan_apply<uint32_t, uint32_t, SCAN2_ITEMS><<<(uint32_t)nbHalf, SCAN_BLOCK, 0, st>>>(a, idx, b, n, nullptr,
+                                                                                              out, totalDev, sums);
         return hipGetLastError();
     }
+    k_scan_reduce<uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, idx, b, n, sums);
     hipError_t e = scan_u64_inplace(sums, nb, scratch + nb + 1, st);
     if (e != hipSuccess) return e;
     k_scan_apply<uint32_t, uint32_t><<<(uint32_t)nb, SCAN_BLOCK, 0, st>>>(a, idx, b, n, sums, out, totalDev);
