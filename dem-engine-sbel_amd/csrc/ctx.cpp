@@ -196,7 +196,7 @@ struct deme_ctx {
     double cellScale = 1.0;  // cell edge = cellScale x the largest interaction distance (>= 1)
     DBuf<uint32_t> cellCount, cellStart, cellOf, slotOf, posOf, aCnt, survSlab, rowSlab, mirSlab;
     DBuf<CdRec> sorted;
-    DBuf<uint2> meta, sortedMeta;
+    DBuf<uint2> sortedMeta;
     DBuf<unsigned long long> planeBits;
     int cur = 0;
     bool havePrev = false, contactsReady = false, fresh = false;
@@ -412,7 +412,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->idA.release(); c->map.release(); c->frc.release(); c->analPartials.release(); c->analAcc.release();
     c->cellCount.release(); c->cellStart.release(); c->cellOf.release(); c->slotOf.release(); c->posOf.release();
     c->survSlab.release(); c->rowSlab.release(); c->mirSlab.release(); c->sorted.release(); c->planeBits.release();
-    c->aCnt.release(); c->meta.release(); c->sortedMeta.release();
+    c->aCnt.release(); c->sortedMeta.release();
     for (auto& k : c->adjCk) {
         k.own.release(); k.rowStart.release(); k.idB.release(); k.ctype.release(); k.hist.release();
         k.idA.release(); k.mirror.release(); k.bRowStart.release(); k.acc.release();
