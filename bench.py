@@ -96,7 +96,7 @@ def run_inspectors(dev, sc, calls=20):
 
 
 def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False):
-    from deme_amd.abi import DeviceSystem
+    from deme_amd.abi import ANAL_TILE, DeviceSystem
 
     dev = DeviceSystem(sc)
     if jit:
@@ -124,7 +124,7 @@ def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False):
     info = dict(S=sc.n_spheres, O=sc.n_owners, P=c.nBinSphereTouches, C=c.nContacts, Cprev=c.nPrevContacts,
                 pairs=c.nContacts - c.nSphereGeoContacts, geo=c.nSphereGeoContacts, Ba=c.nActiveBins,
                 nbins=sc.params.nbX * sc.params.nbY * sc.params.nbZ, cells=c.nCells, fresh=update_freq == 1,
-                anal_blocks=(c.nContacts + 4095) // 4096, inspectors=insp,
+                anal_blocks=(c.nContacts + ANAL_TILE - 1) // ANAL_TILE, inspectors=insp,
                 evented_ms_per_step=1e3 * (te1 - te0) / steps)
     dev.close()
     return t1 - t0, timing, info

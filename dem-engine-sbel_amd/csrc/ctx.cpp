@@ -188,6 +188,7 @@ struct deme_ctx {
     DBuf<float4> lDt[2], lDtPrev;
     bool termsReady = false;  // owner terms belong to the current contact list
     DBuf<float> analPartials, analAcc;
+    DBuf<double> clock;  // simulation time read by the owner update's motion rules (written before each such update)
     // cell-list detection
     CellGrid grid{};
     uint64_t nCells = 0;
@@ -376,7 +377,7 @@ int deme_ctx_create(int device, void* stream, deme_ctx** out) {
         }
         c->ownStream = true;
     }
-    if (c->ctr.ensure(1, 1.0) != hipSuccess ||
+    if (c->ctr.ensure(1, 1.0) != hipSuccess || c->clock.ensure(1, 1.0) != hipSuccess ||
         hipHostMalloc((void**)&c->hctr, sizeof(DevCounters), hipHostMallocDefault) != hipSuccess) {
         delete c;
         return DEME_ERR_HIP;
@@ -410,6 +411,7 @@ int deme_ctx_destroy(deme_ctx* c) {
         c->rowStart[k].release(); c->idB[k].release(); c->ctype[k].release(); c->hist[k].release();
     }
     c->idA.release(); c->map.release(); c->frc.release(); c->analPartials.release(); c->analAcc.release();
+    c->clock.release();
     c->cellCount.release(); c->cellStart.release(); c->cellOf.release(); c->slotOf.release(); c->posOf.release();
     c->survSlab.release(); c->rowSlab.release(); c->mirSlab.release(); c->sorted.release(); c->planeBits.release();
     c->aCnt.release(); c->sortedMeta.release();
@@ -1079,6 +1081,20 @@ int deme_cd_run(deme_ctx* c) {
     return DEME_OK;
 }
 
+int deme_prefix_scan_u32(deme_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* total) {
+    if ((n && !in) || !out) return c->fail(DEME_ERR_INVALID_ARGUMENT, "null device array");
+    DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(n), 1.0));
+    DEME_LAUNCH(c, "scan", scan_u32(in, nullptr, n, out, &c->ctr.p->total[0], c->scanScratch.p, c->stream));
+    if (total) {
+        unsigned long long t = 0;
+        DEME_HIP(c, hipMemcpyAsync(&t, &c->ctr.p->total[0], sizeof(t), hipMemcpyDeviceToHost, c->stream));
+        DEME_HIP(c, hipStreamSynchronize(c->stream));
+        *total = t;
+    }
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    return DEME_OK;
+}
+
 int deme_download_bin_counts(deme_ctx* c, uint16_t* nb, uint8_t* na) {
     if (c->cdStage < 1) return c->fail(DEME_ERR_NOT_READY, "no detection pass");
     if (nb && c->nS) DEME_HIP(c, hipMemcpyAsync(nb, c->numBins.p, c->nS * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream));
@@ -1184,28 +1200,33 @@ static int force_pass(deme_ctx* c, bool full, bool poseFresh = false) {
 
 int deme_force(deme_ctx* c) { return force_pass(c, true); }
 
-static int collect_and_update(deme_ctx* c, int integrate) {
+// capturing: enqueued into a graph captured for replay (the clock is then written before each replay instead)
+static int collect_and_update(deme_ctx* c, int integrate, bool capturing = false) {
     if (!c->termsReady) return c->fail(DEME_ERR_NOT_READY, "the force pass must run on the current contact list first");
     const int cu = c->cur;
     uint32_t nb = 0;
-    const uint32_t needBlocks = (uint32_t)((c->C + 4095) / 4096) + 1;
-    DEME_HIP(c, c->analPartials.ensure((size_t)needBlocks * MAX_ANAL_OWNERS * 6));
+    DEME_HIP(c, c->analPartials.ensure(std::max<size_t>((size_t)anal_blocks(c->C) * MAX_ANAL_OWNERS * 6, 1)));
     DEME_HIP(c, c->analAcc.ensure(MAX_ANAL_OWNERS * 6, 1.0));
     DEME_LAUNCH(c, "anal_collect",
                 launch_anal_collect((uint32_t)c->C, c->idB[cu].p, c->ctype[cu].p, c->gTerm.p, c->anal.p, c->nA,
-                                    c->nClumps, c->nAnalOwners, c->analPartials.p, &nb, c->stream));
+                                    c->nClumps, c->nAnalOwners, c->analPartials.p, c->analPartials.cap, &nb,
+                                    c->stream));
     DEME_LAUNCH(c, "anal_final", launch_anal_final(c->analPartials.p, nb, c->nAnalOwners, c->analAcc.p, c->stream));
+    // Motion rules read the time from the device clock; it is written here, before the update, on the stream (a step
+    // replayed from a graph gets it from set_step_clock, launched outside the graph)
+    if (integrate && c->ownerJitActive && !capturing)
+        DEME_LAUNCH(c, "set_clock", launch_set_clock(c->clock.p, c->time, c->stream));
     DEME_LAUNCH(c, integrate ? "owner_update" : "owner_reduce",
                 c->ownerJitActive
                     ? c->jitOwner.launch(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
                                          c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p,
                                          term_flags(c, false), term_flags(c, true), c->analAcc.p,
-                                         c->rules.p, c->own.p, c->own.p, c->acc.p, integrate, (float)c->time,
+                                         c->rules.p, c->own.p, c->own.p, c->acc.p, integrate, c->clock.p,
                                          c->stream)
                     : launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
                                           c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p,
                                           term_flags(c, false), term_flags(c, true), c->analAcc.p,
-                                          c->rules.p, c->own.p, c->acc.p, integrate, (float)c->time, c->stream));
+                                          c->rules.p, c->own.p, c->acc.p, integrate, c->clock.p, c->stream));
     return DEME_OK;
 }
 
@@ -1231,7 +1252,7 @@ static int step_graph(deme_ctx* c) {
         hipGraph_t g = nullptr;
         DEME_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int r = force_pass(c, false);
-        if (!r) r = collect_and_update(c, 1);
+        if (!r) r = collect_and_update(c, 1, true);
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (r) {
             if (e == hipSuccess && g) (void)hipGraphDestroy(g);
@@ -1254,6 +1275,8 @@ static int step_graph(deme_ctx* c) {
         c->capturedGen = c->graphGen;
         c->capturedKp = c->kp;
     }
+    // the time of this step for the motion rules of the replayed owner update (not a captured argument)
+    if (c->ownerJitActive) DEME_LAUNCH(c, "set_clock", launch_set_clock(c->clock.p, c->time, c->stream));
     DEME_HIP(c, hipGraphLaunch(c->stepExec, c->stream));
     return DEME_OK;
 }
@@ -1270,8 +1293,9 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
             if (r) return r;
         }
         // Between detection passes every step launches the same kernels with the same arguments (the history is
-        // updated in place; the time argument is not read by the integration), so it is captured once per pass and
-        // replayed. The step right after a detection pass (history gathered through the map) runs directly.
+        // updated in place; the time the motion rules read is written to the device clock before each replay), so it
+        // is captured once per pass and replayed. The step right after a detection pass (history gathered through the
+        // map) runs directly.
         const bool records = c->keepRecords && k == nSteps - 1;
         if (!detect && !c->fresh && !records && c->useGraphs && !c->timing && !c->jitStale && c->stream) {
             if ((r = step_graph(c))) return r;
@@ -1508,7 +1532,7 @@ static int adj_window(deme_ctx* c, int n, bool record) {
                                         c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p,
                                         term_flags(c, false), term_flags(c, true),
                                           c->analAcc.p,
-                                        c->rules.p, c->adjOwn[k].p, c->adjAcc[k].p, 1, (float)c->time, c->stream,
+                                        c->rules.p, c->adjOwn[k].p, c->adjAcc[k].p, 1, c->clock.p, c->stream,
                                         c->adjOwn[k + 1].p));
         c->time += (double)c->sp.h;
         c->nSteps++;

@@ -64,6 +64,11 @@ constexpr uint64_t SCAN_TILE = (uint64_t)SCAN_BLOCK * SCAN_ITEMS;
 constexpr int SCAN2_ITEMS = 8;
 constexpr uint64_t SCAN2_TILE = (uint64_t)SCAN_BLOCK * SCAN2_ITEMS;
 constexpr uint32_t MAX_ANAL_OWNERS = 8;
+// Plane-owner sums: each workgroup of k_anal_collect folds one tile of ANAL_TILE contacts (256 lanes x ANAL_ITEMS)
+// into nAnalOwners x 6 partials, so the partials buffer holds anal_blocks(nC) x nAnalOwners x 6 floats.
+constexpr uint32_t ANAL_ITEMS = 4;
+constexpr uint32_t ANAL_TILE = 256 * ANAL_ITEMS;
+inline uint32_t anal_blocks(uint64_t nC) { return (uint32_t)((nC + ANAL_TILE - 1) / ANAL_TILE); }
 
 // scan.hip: out[0..n] = exclusive prefix sum of (a[idx ? idx[i] : i] + (b ? b[i] : 0)); out[n] = total;
 // *totalDev = total (u64). scratch must hold scan_scratch_elems(n) u64 values.
@@ -145,9 +150,11 @@ hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, cons
                         float4* histOut, const uint32_t* mirror, float* aTerm, float* bTerm, float* gTerm,
                         uint8_t* aOn, uint8_t* bOn,
                         ForceRec* out, hipStream_t st);
+// blockPartials holds partialCap floats; a launch that would need more (anal_blocks(nC) * nAnalOwners * 6) returns
+// hipErrorInvalidValue and writes nothing
 hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const float* gTerm,
                                const deme_anal_obj* anal, uint32_t nA, uint32_t firstAnalOwner, uint32_t nAnalOwners,
-                               float* blockPartials, uint32_t* nBlocksOut, hipStream_t st);
+                               float* blockPartials, uint64_t partialCap, uint32_t* nBlocksOut, hipStream_t st);
 hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint32_t nAnalOwners, float* analAcc,
                              hipStream_t st);
 hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnalOwner,
@@ -155,8 +162,11 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
                                const uint8_t* aOn, const uint8_t* bOn,
                                const float* analAcc, const deme_family_rule* rules, OwnerRec* own,
-                               float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate, float t,
+                               float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate,
+                               const double* clock /* device: the simulation time the motion rules read */,
                                hipStream_t st, OwnerRec* ownOut = nullptr /* integrated records; null: in place */);
+// clock[0] = t (one lane; the time the owner update's motion rules read, written on the stream before the update)
+hipError_t launch_set_clock(double* clock, double t, hipStream_t st);
 // p[0..n) = v, and p2[0..n2) = v in the same launch (n2 <= 256 * blocks: small second ranges only)
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st, uint32_t* p2 = nullptr,
                            uint32_t n2 = 0);
@@ -185,7 +195,7 @@ struct JitOwner {
                       const uint32_t* ownerSphStart, const uint32_t* rowStart, const uint32_t* bRowStart,
                       const float* aTerm, const float* bTerm, const uint8_t* aOn, const uint8_t* bOn,
                       const float* analAcc, const deme_family_rule* rules,
-                      const OwnerRec* own, OwnerRec* ownOut, float* acc, int doIntegrate, float t,
+                      const OwnerRec* own, OwnerRec* ownOut, float* acc, int doIntegrate, const double* clock,
                       hipStream_t st) const;
 };
 std::string jit_owner_source(const std::vector<std::pair<std::string, std::string>>& values);

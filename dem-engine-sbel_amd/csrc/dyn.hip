@@ -21,13 +21,10 @@ using namespace deme_dev;
 namespace {
 
 // Row a16 for analytical owners (planes): every sphere-plane contact contributes its B-side term (stored by the
-// force kernel) to the plane's owner. Deterministic two-level reduction: lane t of a tile sums contacts t, t + 256, ...
-// (16 of them) in order, lanes combine by wave butterflies and the four waves in order, tiles combine in a fixed tree
-// in k_anal_final. The contact
-// types of a lane's 16 contacts are loaded together (coalesced across the wave), so the gathers of the few plane
-// contacts are the only dependent loads.
-constexpr uint32_t ANAL_ITEMS = 4;
-constexpr uint32_t ANAL_TILE = 256 * ANAL_ITEMS;
+// force kernel) to the plane's owner. Deterministic two-level reduction over tiles of ANAL_TILE contacts: lane t of a
+// tile sums contacts t, t + 256, ... (ANAL_ITEMS = 4 of them) in order, lanes combine by wave butterflies and the four
+// waves in order, tiles combine in a fixed tree in k_anal_final. The contact types of a lane's contacts are loaded
+// together (coalesced across the wave), so the gathers of the few plane contacts are the only dependent loads.
 
 __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_t* __restrict__ idB,
                                                       const uint8_t* __restrict__ ctype,
@@ -120,6 +117,11 @@ __global__ __launch_bounds__(256) void k_anal_final(const float* __restrict__ pa
     if (t == 0) analAcc[v] = red[0];
 }
 
+// The simulation time read by the owner update's motion rules (a kernel, not a copy: it may sit in a captured graph).
+__global__ void k_set_clock(double* clock, double t) {
+    if (threadIdx.x == 0) clock[0] = t;
+}
+
 // ChangeFamily (APIPublic.cpp:184-207): every owner of family `from` moves to family `to`.
 __global__ __launch_bounds__(256) void k_change_family(OwnerRec* own, uint32_t nO, uint8_t from, uint8_t to) {
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
@@ -148,11 +150,13 @@ hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, cons
 
 hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const float* gTerm,
                                const deme_anal_obj* anal, uint32_t nA, uint32_t firstAnalOwner, uint32_t nAnalOwners,
-                               float* blockPartials, uint32_t* nBlocksOut, hipStream_t st) {
-    const uint32_t nb = nC == 0 ? 0 : blocks(nC, ANAL_TILE);
+                               float* blockPartials, uint64_t partialCap, uint32_t* nBlocksOut, hipStream_t st) {
+    const uint32_t nb = anal_blocks(nC);
     *nBlocksOut = nb;
     if (nb == 0 || nAnalOwners == 0) return hipSuccess;
     if (nAnalOwners > MAX_ANAL_OWNERS) return hipErrorInvalidValue;
+    // every workgroup writes its tile's nAnalOwners x 6 partials, plane contacts or not
+    if ((uint64_t)nb * nAnalOwners * 6 > partialCap) return hipErrorInvalidValue;
     k_anal_collect<<<nb, 256, 0, st>>>(nC, idB, ctype, gTerm, anal, nA, firstAnalOwner, nAnalOwners, blockPartials);
     return hipGetLastError();
 }
@@ -169,12 +173,17 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
                                const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
                                const uint8_t* aOn, const uint8_t* bOn, const float* analAcc, const deme_family_rule* rules, OwnerRec* own, float* acc,
-                               int doIntegrate, float t, hipStream_t st, OwnerRec* ownOut) {
+                               int doIntegrate, const double* clock, hipStream_t st, OwnerRec* ownOut) {
     if (nO == 0) return hipSuccess;
     deme_owner_update<<<blocks(nO, 256), 256, 0, st>>>(P, nO, nClumps, firstAnalOwner, nAnalOwners, ownerSphStart,
                                                     rowStart, bRowStart, aTerm, bTerm, aOn, bOn, analAcc, rules,
                                                     own,
-                                                    ownOut ? ownOut : own, acc, doIntegrate, t);
+                                                    ownOut ? ownOut : own, acc, doIntegrate, clock);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_clock(double* clock, double t, hipStream_t st) {
+    k_set_clock<<<1, 64, 0, st>>>(clock, t);
     return hipGetLastError();
 }
 

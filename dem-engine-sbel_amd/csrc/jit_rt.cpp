@@ -171,12 +171,13 @@ hipError_t JitOwner::launch(const KParams& P, uint32_t nO, uint32_t nClumps, uin
                             const uint32_t* ownerSphStart, const uint32_t* rowStart, const uint32_t* bRowStart,
                             const float* aTerm, const float* bTerm, const uint8_t* aOn, const uint8_t* bOn,
                             const float* analAcc, const deme_family_rule* rules, const OwnerRec* own,
-                            OwnerRec* ownOut, float* acc, int doIntegrate, float t, hipStream_t st) const {
+                            OwnerRec* ownOut, float* acc, int doIntegrate, const double* clock,
+                            hipStream_t st) const {
     if (nO == 0) return hipSuccess;
     KParams p = P;
     void* args[] = {&p,     &nO,   &nClumps, &firstAnal, &nAnalOwners, &ownerSphStart, &rowStart, &bRowStart,
                     &aTerm, &bTerm, &aOn,    &bOn,       &analAcc,     &rules,         &own,      &ownOut,
-                    &acc,   &doIntegrate, &t};
+                    &acc,   &doIntegrate, &clock};
     return hipModuleLaunchKernel(fn, (nO + 255) / 256, 1, 1, 256, 1, 1, 0, st, args, nullptr);
 }
 

@@ -28,6 +28,9 @@ SPHERE_PLANE = 3
 NULL_MAP = 0xFFFFFFFF
 NUM_MASK_ENTRIES = 256 * 257 // 2
 NUM_FAMILIES = 256
+# contacts per workgroup tile of the plane-owner sums (ANAL_TILE in csrc/deme_internal.h): the sums write one set of
+# partials per tile
+ANAL_TILE = 1024
 INTEGRATOR = {"forward_euler": 0, "centered_difference": 1, "extended_taylor": 2}
 
 
@@ -86,6 +89,7 @@ EXPORTED = [
     "deme_set_keep_contact_records", "deme_set_owner_volumes", "deme_inspect", "deme_append_clumps",
     "deme_change_clump_sizes",
     "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner", "deme_jit_render_inspect",
+    "deme_prefix_scan_u32",
 ]
 
 _lib = None
@@ -130,6 +134,7 @@ def load(path: str = LIB_PATH):
         "deme_download_bin_counts": ([vp, vp, vp], i32),
         "deme_download_bins": ([vp, vp, vp], i32),
         "deme_download_contacts": ([vp, vp, vp, vp, vp], i32),
+        "deme_prefix_scan_u32": ([vp, vp, vp, u64, C.POINTER(u64)], i32),
         "deme_force": ([vp], i32), "deme_reduce_owner": ([vp], i32), "deme_integrate": ([vp], i32),
         "deme_step": ([vp, i32, i32], i32),
         "deme_download_forces": ([vp, vp, vp, vp, vp], i32),

@@ -102,6 +102,12 @@ int deme_download_bin_counts(deme_ctx* ctx, uint16_t* numBinsSphereTouches, uint
 int deme_download_bins(deme_ctx* ctx, uint32_t* binStart, uint32_t* binSphere);
 /* Canonical contact list: by idA; plane contacts before sphere contacts; sphere partners by ascending idB. */
 int deme_download_contacts(deme_ctx* ctx, uint32_t* idA, uint32_t* idB, uint8_t* type, uint32_t* map);
+/* The device-wide exclusive prefix sum of the detection stages (cubDEMPrefixScan, DEMCubWrappers.cu:26-41, used by
+ * contactDetection at DEMCubContactDetection.cu:81-93): d_out[i] = d_in[0] + ... + d_in[i-1] for i <= n (d_out holds
+ * n + 1 entries; d_out[n] is the total modulo 2^32), *total (host, nullable) = the total in 64 bits. d_in and d_out
+ * are device pointers (they may be the same array); the call runs on the context's stream and synchronises it. Sums
+ * are formed in 64 bits in a fixed order. */
+int deme_prefix_scan_u32(deme_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n, uint64_t* total);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Dynamics (DEMDynamicThread::calculateForces / integrateOwnerMotions, src/DEM/dT.cpp:1349-1432).

@@ -113,3 +113,13 @@ def test_encode_decode_round_trip():
     vx = v & np.uint64((1 << p.nvXp2) - 1)
     vz = v >> np.uint64(p.nvXp2 + p.nvYp2)
     assert vx.max() < 2 ** p.nvXp2 and vz.max() < 2 ** p.nvZp2
+
+
+def test_plane_sum_tile_matches_the_library():
+    # bench.py sizes the plane-owner partials' byte model with the Python constant; it must be the kernels' tile
+    from deme_amd.abi import ANAL_TILE
+
+    text = open(os.path.join(ROOT, "dem-engine-sbel_amd", "csrc", "deme_internal.h")).read()
+    items = int(re.search(r"constexpr uint32_t ANAL_ITEMS = (\d+);", text).group(1))
+    assert re.search(r"constexpr uint32_t ANAL_TILE = 256 \* ANAL_ITEMS;", text)
+    assert ANAL_TILE == 256 * items
