@@ -16,7 +16,11 @@
 //   bin counting / filling / sorting                   pinned: brute-force enumeration of AABB bins
 //   pair detection (bin rule)                          pinned: brute-force O(S^2) predicate; physical overlap set
 //   canonical order + history map                      pinned: std::sort / map join on random lists
-//   Hertz-Mindlin force                                pinned: closed-form Hertz law, Coulomb cap, Newton's 3rd law
+//   Hertz-Mindlin force                                pinned: closed-form Hertz law; normal and tangential damping,
+//                                                      tangential spring, Coulomb cap with the back-solved history,
+//                                                      rolling resistance and its collision-time switch, and the
+//                                                      two-material mixing rule against their closed forms; Newton's
+//                                                      3rd law (the FP64 restatement is held to the same closed forms)
 //   owner reduction                                    pinned: zero net force without planes; FP64 sum
 //   integration                                        pinned: free fall, constant spin, prescribed/fixed families
 //   schedule                                           pinned: resting-on-plane equilibrium depth

@@ -358,3 +358,186 @@ def test_fixed_and_prescribed_families():
     assert np.array_equal(X1[0], X0[0])  # fixed: not even gravity moves it
     t = n * np.float32(h)
     assert np.abs((X1[1] - X0[1]) - np.array([0.1, -0.05, 0.0]) * t).max() < 1e-9
+
+
+# --------------------------------------------------------------------------------------------------------------
+# a15: the rest of the Hertz-Mindlin model against its closed forms (FullHertzianForceModel.cu:37-120,
+# matProxy2ContactParam at DEMHelperKernels.cu:316-339). Two spheres on the x axis (A at the origin, B at +x, so the
+# unit normal B2A is -x and the force returned is the force on A); expected values are formed in double from the
+# model's definitions. Both the FP32 oracle (1e-4 relative: float arithmetic) and the FP64 restatement the GPU parity
+# tests use as their reference (1e-9) are held to them.
+# --------------------------------------------------------------------------------------------------------------
+def mixed(m1, m2):
+    """E*, G*, CoR, mu, Crr of a pair of materials (E, nu, CoR, mu, Crr) by the model's mixing rule."""
+    E1, n1, e1, u1, c1 = m1
+    E2, n2, e2, u2, c2 = m2
+    E_eff = 1.0 / ((1 - n1 * n1) / E1 + (1 - n2 * n2) / E2)
+    G_eff = 1.0 / (2 * (2 - n1) * (1 + n1) / E1 + 2 * (2 - n2) * (1 + n2) / E2)
+    return E_eff, G_eff, min(e1, e2), max(u1, u2), max(c1, c2)
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+def pair(delta, rA=0.005, rB=0.005, vA=(0, 0, 0), vB=(0, 0, 0), wA=(0, 0, 0), wB=(0, 0, 0),
+         matA=(E, NU, 0.5, 0.5, 0.0), matB=None, hist=None, h=1e-5):
+    """One force pass of the oracle on a two-sphere contact of overlap ~delta, with preset contact history
+    hist = (delta_tan_x, delta_tan_y, delta_tan_z, delta_time); returns the closed-form ingredients with the FP32
+    and FP64 results."""
+    from deme_amd.abi import MATERIAL_DTYPE
+
+    matB = matA if matB is None else matB
+    sc = scenes.spheres_scene([[0.0, 0.0, 0.0], [rA + rB - delta, 0.0, 0.0]], np.array([rA, rB]), (0.1, 0.1, 0.1),
+                              planes=False, g=(0, 0, 0), vel=[vA, vB], omg=[wA, wB], h=h)
+    sc.materials = np.zeros(2, MATERIAL_DTYPE)
+    sc.materials[0] = matA
+    sc.materials[1] = matB
+    sc.sph_mat = np.array([0, 1], np.uint8)
+    ora = oracle(sc)
+    ora.cd_run()
+    a, b, t, _ = ora.contacts()
+    assert list(t) == [1] and a[0] == 0 and b[0] == 1
+    if hist is not None:
+        ora.set_wildcards(np.asarray(hist, np.float32).reshape(4, 1))
+    F64, T64 = ora.forces_fp64()
+    ora.force()
+    F, T, cA, cB = ora.forces()
+    X = world_positions(sc)
+    assert X[0, 1] == X[1, 1] and X[0, 2] == X[1, 2]  # the normal is exactly -x
+    ra, rb = f32(rA), f32(rB)
+    d = ra + rb - (X[1, 0] - X[0, 0])  # the encoded overlap
+    mA, mB = (float(m) for m in sc.owner_mass[:2])
+    mats = [tuple(float(sc.materials[k][f]) for f in ("E", "nu", "CoR", "mu", "Crr")) for k in (0, 1)]
+    E_eff, G_eff, e, mu, crr = mixed(*mats)
+    R = ra * rb / (ra + rb)
+    m = mA * mB / (mA + mB)
+    Sn = 2.0 * E_eff * np.sqrt(R * d)
+    loge = np.log(e)
+    beta = loge / np.sqrt(loge * loge + np.pi ** 2)
+    cf = dict(d=d, R=R, m=m, E=E_eff, G=G_eff, e=e, mu=mu, crr=crr, beta=beta, kn=2.0 / 3.0 * Sn,
+              gn=2.0 * np.sqrt(5.0 / 6.0) * beta * np.sqrt(Sn * m), kt=8.0 * G_eff * np.sqrt(R * d), h=f32(h))
+    cf["gt"] = -2.0 * np.sqrt(5.0 / 6.0) * beta * np.sqrt(m * cf["kt"])
+    return cf, F[0].astype(np.float64), T[0].astype(np.float64), F64[0], T64[0], ora.wildcards()[:, 0]
+
+
+def close(got, want, rel):
+    return abs(got - want) <= rel * max(abs(want), 1e-30)
+
+
+@pytest.mark.parametrize("speed,cor", [(0.3, 0.5), (-0.2, 0.5), (0.3, 0.2), (0.3, 0.9)])
+def test_normal_damping(speed, cor):
+    """F_n = k_n delta + gamma_n v_n with k_n = 2/3 S_n, S_n = 2 E* sqrt(R* delta), gamma_n = 2 sqrt(5/6) beta
+    sqrt(S_n m*), beta = ln e / sqrt(ln^2 e + pi^2) and v_n = (v_A - v_B) . B2A (speed > 0: approaching)."""
+    cf, F, T, F64, _, w = pair(3e-5, vA=(speed / 2, 0, 0), vB=(-speed / 2, 0, 0), matA=(E, NU, cor, 0.5, 0.0))
+    vn = -(f32(speed / 2) + f32(speed / 2))  # velocities are stored in float
+    fn = cf["kn"] * cf["d"] + cf["gn"] * vn
+    assert abs(cf["gn"] * vn) > 0.25 * cf["kn"] * cf["d"]  # the damping term is not a rounding detail here
+    assert close(-F[0], fn, 1e-4) and abs(F[1]) < 1e-7 * abs(fn) and abs(F[2]) < 1e-7 * abs(fn)
+    assert close(-F64[0], fn, 1e-9)
+    assert np.all(T == 0) and w[0] == w[1] == w[2] == 0.0 and w[3] == np.float32(cf["h"])
+
+
+def test_tangential_spring_stiffness():
+    """Preset tangential displacement, no relative motion: F_t = -k_t delta_t with k_t = 8 G* sqrt(R* delta), and
+    G* = 1 / (2 (2 - nu)(1 + nu) / E + ...) for the pair; the history is kept (below the Coulomb cap)."""
+    d0 = 4e-6
+    cf, F, T, F64, _, w = pair(3e-5, hist=(0.0, d0, 0.0, 0.0))
+    ft = -cf["kt"] * f32(d0)
+    assert abs(ft) < cf["mu"] * cf["kn"] * cf["d"]
+    assert close(F[1], ft, 1e-4) and close(F64[1], ft, 1e-9)
+    assert close(-F[0], cf["kn"] * cf["d"], 1e-4) and F[2] == 0.0
+    assert w[1] == np.float32(d0) and w[0] == 0.0 and w[2] == 0.0
+
+
+def test_tangential_damping_on_the_first_sliding_step():
+    """From zero history, the first step of a sliding contact: delta_t = h v_t and
+    F_t = -k_t delta_t - gamma_t v_t with gamma_t = -2 sqrt(5/6) beta sqrt(m* k_t)."""
+    u = 0.005
+    cf, F, T, F64, _, w = pair(3e-5, vA=(0, u, 0), vB=(0, -u, 0))
+    vt = 2 * f32(u)
+    ft = -(cf["kt"] * cf["h"] * vt) - cf["gt"] * vt
+    assert cf["gt"] * vt > 5 * cf["kt"] * cf["h"] * vt  # the damping term dominates the first step
+    assert abs(ft) < cf["mu"] * cf["kn"] * cf["d"]
+    assert close(F[1], ft, 1e-4) and close(F64[1], ft, 1e-9)
+    assert close(w[1], cf["h"] * vt, 1e-6) and w[3] == np.float32(cf["h"])
+
+
+@pytest.mark.parametrize("u", [0.0, 0.004])
+def test_coulomb_cap_back_solves_the_history(u):
+    """A tangential force beyond mu |F_n| is scaled onto the cap and the history is back-solved from it:
+    delta_t = (F_t + gamma_t v_t) / (-k_t), with or without sliding."""
+    d0 = 1e-4
+    cf, F, T, F64, _, w = pair(3e-5, hist=(0.0, d0, 0.0, 0.0), matA=(E, NU, 0.5, 0.3, 0.0), vA=(0, u, 0),
+                               vB=(0, -u, 0))
+    fn = cf["kn"] * cf["d"]
+    vt = 2 * f32(u)
+    assert cf["kt"] * d0 > 2 * cf["mu"] * fn
+    assert close(F[1], -cf["mu"] * fn, 1e-4) and close(F64[1], -cf["mu"] * fn, 1e-9)
+    want = (cf["mu"] * fn - cf["gt"] * vt) / cf["kt"]
+    if u:
+        assert abs(cf["gt"] * vt) > 0.2 * cf["mu"] * fn  # the damping share of the back-solve matters
+    assert close(w[1], want, 1e-4)
+
+
+def t_collision(cf):
+    """The rolling-resistance switch: t_c = pi sqrt(m* / (k (1 - d^2))) with k = 4/3 E* R_e^(1/2), R_e = sqrt(R*),
+    d = g / (2 sqrt(k m*)), g = -2 sqrt(5/3 m* E*) beta R_e^(1/4) (FullHertzianForceModel.cu:70-82)."""
+    Re = np.sqrt(cf["R"])
+    k = 4.0 / 3.0 * cf["E"] * np.sqrt(Re)
+    g = -2.0 * np.sqrt(5.0 / 3.0 * cf["m"] * cf["E"]) * cf["beta"] * Re ** 0.25
+    dc = g / (2.0 * np.sqrt(k * cf["m"]))
+    assert dc < 1.0
+    return np.pi * np.sqrt(cf["m"] / (k * (1.0 - dc * dc)))
+
+
+@pytest.mark.parametrize("frac,spin", [(0.5, 40.0), (0.99, 40.0), (1.01, 40.0), (2.0, 40.0), (2.0, -25.0)])
+def test_rolling_resistance(frac, spin):
+    """Crr > 0: once the contact has lasted longer than t_c, a torque-only force Crr |F| along the rolling velocity
+    of B relative to A at the contact point (v_rot = w_B x a_B - w_A x a_A, world frame); none before. Both spheres
+    spin about z, so v_rot = -2 w a y-hat; mu = 0 keeps the tangential part out of |F|."""
+    crr, h = 0.02, 1e-5
+    cf0, *_ = pair(3e-5, matA=(E, NU, 0.5, 0.0, crr))
+    tc = t_collision(cf0)
+    dt0 = f32(frac * tc - h)  # the pass adds h first
+    cf, F, T, F64, T64, w = pair(3e-5, wA=(0, 0, spin), wB=(0, 0, spin), matA=(E, NU, 0.5, 0.0, crr),
+                                 hist=(0.0, 0.0, 0.0, dt0), h=h)
+    fn = cf["kn"] * cf["d"]
+    assert close(-F[0], fn, 1e-4) and F[1] == 0.0 and F[2] == 0.0
+    if frac < 1.0:
+        assert np.all(T == 0) and np.all(T64 == 0)
+    else:
+        want = np.array([0.0, -np.sign(spin) * cf["crr"] * fn, 0.0])  # Crr as stored (float)
+        assert np.abs(T - want).max() <= 1e-4 * crr * fn
+        assert np.abs(T64 - want).max() <= 1e-9 * crr * fn
+
+
+@pytest.mark.parametrize("swap", [False, True])
+def test_two_material_mixing(swap):
+    """E* and G* by the compliance sums of both materials, CoR the smaller, mu and Crr the larger (an exchange of min
+    and max, or of the two materials, changes the numbers); symmetric in the order of the materials."""
+    m1 = (1e7, 0.30, 0.3, 0.2, 0.01)
+    m2 = (4e7, 0.20, 0.8, 0.6, 0.03)
+    mA, mB = (m2, m1) if swap else (m1, m2)
+    rA, rB = 0.004, 0.006  # unequal radii and masses: R* and m* are the harmonic forms
+    # normal, with damping from the smaller CoR
+    cf, F, T, F64, _, _ = pair(3e-5, rA=rA, rB=rB, vA=(0.1, 0, 0), matA=mA, matB=mB)
+    E_eff = 1.0 / ((1 - 0.09) / 1e7 + (1 - 0.04) / 4e7)
+    assert close(cf["E"], E_eff, 1e-6) and cf["e"] == f32(0.3) and cf["mu"] == f32(0.6) and cf["crr"] == f32(0.03)
+    assert close(cf["R"], f32(rA) * f32(rB) / (f32(rA) + f32(rB)), 1e-12)
+    fn = cf["kn"] * cf["d"] + cf["gn"] * (-f32(0.1))
+    assert close(-F[0], fn, 1e-4) and close(-F64[0], fn, 1e-9)
+    # spring stiffness from G*
+    cf, F, T, F64, _, _ = pair(3e-5, rA=rA, rB=rB, matA=mA, matB=mB, hist=(0.0, 2e-6, 0.0, 0.0))
+    G_eff = 1.0 / (2 * 1.7 * 1.3 / 1e7 + 2 * 1.8 * 1.2 / 4e7)
+    assert close(cf["G"], G_eff, 1e-6)
+    assert close(F[1], -8.0 * G_eff * np.sqrt(cf["R"] * cf["d"]) * f32(2e-6), 1e-4)
+    # the cap from the larger mu
+    cf, F, T, F64, _, _ = pair(3e-5, rA=rA, rB=rB, matA=mA, matB=mB, hist=(0.0, 1e-4, 0.0, 0.0))
+    assert close(F[1], -f32(0.6) * cf["kn"] * cf["d"], 1e-4)
+    # rolling resistance from the larger Crr, past the collision time of the mixed contact
+    tc = t_collision(cf)
+    cf, F, T, F64, _, _ = pair(3e-5, rA=rA, rB=rB, matA=mA, matB=mB, wA=(0, 0, 30.0), wB=(0, 0, 30.0),
+                               hist=(0.0, 0.0, 0.0, f32(2 * tc)))
+    # |F| when the torque-only force is formed is the normal force's (the tangential part comes after it)
+    assert close(np.linalg.norm(T), f32(0.03) * cf["kn"] * cf["d"], 1e-4)
