@@ -95,10 +95,12 @@ def run_inspectors(dev, sc, calls=20):
     return out
 
 
-def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False):
+def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False, prepare=None):
     from deme_amd.abi import ANAL_TILE, DeviceSystem
 
     dev = DeviceSystem(sc)
+    if prepare is not None:
+        prepare(dev)
     if jit:
         dev.jit_build()  # force kernel specialised by hipRTC on this workload's constants
     dev.step(warmup, update_freq)
@@ -130,12 +132,29 @@ def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False):
     return t1 - t0, timing, info
 
 
-def run_bed(steps, warmup):
-    """The large bed: 8M spheres with lognormal radii and 4 materials, packed, a top plate in a prescribed family
-    moving down at 0.05 m/s, dt = 2e-6 s, contact detection every step."""
+def run_bed(steps, warmup, settle_steps=200):
+    """Configuration 3: 8M spheres with lognormal radii (median 1 mm, sigma 0.2, clipped to [0.5, 2] mm) and 4
+    materials packed at ~0.6 solid fraction in a 0.41 x 0.41 x 0.43 m box, a top plate in a prescribed family moving
+    down at 0.05 m/s, dt = 2e-6 s, contact detection every step. The packing's residual overlaps are relaxed by a
+    damped run on the device first (not timed)."""
     from deme_amd import scenes
 
     sc = scenes.polydisperse_bed()
+    t, timing, info = run_device(sc, steps, warmup, 1, prepare=lambda dev: scenes.settle_bed(dev, sc, settle_steps))
+    top = sorted(timing.items(), key=lambda kv: -kv[1][0])[:8]
+    return {"spheres": sc.n_spheres, "solid_fraction": round(sc.info["solid_fraction"], 4),
+            "box_m": [round(b, 4) for b in sc.info["box"]], "settle_steps": settle_steps,
+            "contacts": info["C"], "contacts_per_sphere": round(info["C"] / sc.n_spheres, 3), "steps": steps,
+            "sphere_steps_per_s": sc.n_spheres * steps / t, "ms_per_step": 1e3 * t / steps,
+            "kernels_us": {k: round(1e3 * v[0] / max(v[1], 1), 1) for k, v in top}}
+
+
+def run_small(steps, warmup):
+    """Configuration 1: 20,000 spheres (r = 5 mm) settling in a 0.3 x 0.3 x 0.6 m box, dt = 1e-5 s, contact detection
+    every step; the whole 1,000-step run is timed."""
+    from deme_amd import scenes
+
+    sc = scenes.sphere_settling()
     t, timing, info = run_device(sc, steps, warmup, 1)
     top = sorted(timing.items(), key=lambda kv: -kv[1][0])[:6]
     return {"spheres": sc.n_spheres, "contacts": info["C"], "steps": steps,
@@ -143,13 +162,15 @@ def run_bed(steps, warmup):
             "kernels_us": {k: round(1e3 * v[0] / max(v[1], 1), 1) for k, v in top}}
 
 
-def run_batch(n_scenes, steps, warmup, update_freq, beta):
+def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05):
     """Batched independent settling columns: one context stepping all scenes together, against the plain loop that
-    steps each scene in a context of its own (same world, same physics: each scene's trajectory is identical)."""
+    runs each scene's steps in a context of its own, scene after scene (each context on its own stream; same world,
+    same physics: each scene's trajectory is identical). pitch_scale: lattice pitch of the columns over the largest
+    diameter (1.05: columns in free fall at the start; 0.75: neighbours overlap, the columns carry contacts)."""
     from deme_amd import scenes
     from deme_amd.abi import DeviceSystem
 
-    sc = scenes.batched_columns(n_scenes=n_scenes, update_freq=update_freq, beta=beta)
+    sc = scenes.batched_columns(n_scenes=n_scenes, update_freq=update_freq, beta=beta, pitch_scale=pitch_scale)
     dev = DeviceSystem(sc)
     dev.step(warmup, update_freq)
     # autotuner: contact-update interval (with its drift margin for the free-fall speed bound of the columns) and
@@ -185,15 +206,15 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta):
     for d in parts:
         d.sync()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        for d in parts:
-            d.step(1, update_freq)
+    for d in parts:
+        d.step(steps, update_freq)
     for d in parts:
         d.sync()
     t_loop = time.perf_counter() - t0
     for d in parts:
         d.close()
-    return {"scenes": n_scenes, "spheres": sc.n_spheres, "contacts": contacts, "update_freq": update_freq,
+    return {"scenes": n_scenes, "spheres": sc.n_spheres, "pitch_scale": pitch_scale, "contacts": contacts,
+            "update_freq": update_freq,
             "beta_m": beta, "cell_scale": scale, "autotune_ms_per_step": table.round(4).tolist(), "steps": steps,
             "scene_steps_per_s": n_scenes * steps / t_batch, "ms_per_step": 1e3 * t_batch / steps,
             "graph_replay_ms_per_step": 1e3 * t_graph / steps,
@@ -255,16 +276,52 @@ def run_adjoint(n, steps, window, warmup_windows=1):
             "grad_E": gM[:, 0].tolist(), "grad_v0_norm": float(np.linalg.norm(gV))}
 
 
-def cpu_baseline(sc, steps, update_freq, threads):
+def cpu_threads():
+    """The job's CPU share: its affinity set, bounded by OMP_NUM_THREADS when the environment sets one."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def host_cpu():
+    """nproc, the affinity set and lscpu's model, socket and core counts of the host the baseline runs on."""
+    import subprocess
+
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(sc, steps, update_freq, threads, repeats=3):
+    """The oracle (host C++/OpenMP restatement of the reference's kernels) on `threads` threads: one warm-up step, then
+    `repeats` timed runs of `steps` steps on the same system; the median run's seconds and its per-phase seconds
+    (the reference's timer names)."""
     import pyoracle  # noqa: E402  (the oracle is the reported CPU baseline only)
 
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     ora = pyoracle.OracleSystem(sc, threads=threads)
     ora.step(1, update_freq)  # warm-up step
-    t0 = time.perf_counter()
-    ora.step(steps, update_freq)
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(repeats):
+        ora.phase_times(reset=True)
+        t0 = time.perf_counter()
+        ora.step(steps, update_freq)
+        runs.append((time.perf_counter() - t0, ora.phase_times()))
     ora.close()
-    return dt
+    runs.sort(key=lambda r: r[0])
+    dt, phases = runs[len(runs) // 2]
+    return dt, {k: v / steps for k, v in phases.items()}, [r[0] for r in runs]
 
 
 def main():
@@ -273,7 +330,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--clumps", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-run-b", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
@@ -283,7 +340,9 @@ def main():
     ap.add_argument("--skip-jit", action="store_true")
     ap.add_argument("--skip-inspect", action="store_true")
     ap.add_argument("--skip-bed", action="store_true")
-    ap.add_argument("--bed-steps", type=int, default=50)
+    ap.add_argument("--bed-steps", type=int, default=100)
+    ap.add_argument("--skip-small", action="store_true")
+    ap.add_argument("--small-steps", type=int, default=1000)
     ap.add_argument("--adjoint-spheres", type=int, default=50_000)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-window", type=int, default=20)
@@ -325,9 +384,14 @@ def main():
     if not args.skip_bed:
         bed = run_bed(args.bed_steps, 10)
 
-    batch = None
+    small = None
+    if not args.skip_small:
+        small = run_small(args.small_steps, 20)
+
+    batch = batch_loaded = None
     if not args.skip_batch:
         batch = run_batch(args.batch_scenes, args.batch_steps, 3, 1, 0.0)
+        batch_loaded = run_batch(args.batch_scenes, args.batch_steps, 3, 1, 0.0, pitch_scale=0.75)
 
     jit = None
     if not args.skip_jit:
@@ -361,13 +425,16 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        os.environ.setdefault("OMP_PROC_BIND", "close")
-        dt = cpu_baseline(scA, args.cpu_steps, 1, threads)
+        threads = cpu_threads()
+        dt, phases, runs = cpu_baseline(scA, args.cpu_steps, 1, threads)
         cpu = {"value": args.clumps * args.cpu_steps / dt, "unit": "clump-steps/s", "cores": threads,
                "kind": "port",
-               "sample": f"oracle, same 1M-clump workload, {args.cpu_steps} steps after 1 warm-up step, "
-                         f"contact detection every step, {threads} OpenMP threads"}
+               "sample": f"oracle, same 1M-clump workload, contact detection every step: median of 3 runs of "
+                         f"{args.cpu_steps} steps after 1 warm-up step, {threads} OpenMP threads (the job's CPU "
+                         f"share), OMP_PROC_BIND=close, OMP_PLACES=cores",
+               "runs_s": [round(r, 3) for r in runs],
+               "phase_s_per_step": {k: round(v, 5) for k, v in phases.items()},
+               "host": host_cpu()}
 
     line = {
         "metric": "particle-steps/s (1M clumps)",
@@ -388,9 +455,11 @@ def main():
                    "ms_per_step_with_kernel_events": round(infoA["evented_ms_per_step"], 4),
                    "bin_sphere_touches": infoA["P"], "active_bins": infoA["Ba"],
                    "run_every5_beta0.2r": runB, "jit_specialised": jit, "batched_columns": batch,
+                   "batched_columns_loaded": batch_loaded,
                    "adjoint_drop": adjoint,
                    "inspectors": infoA["inspectors"],
                    "polydisperse_bed_8M": bed,
+                   "sphere_settling_20k": small,
                    "kernels": kernels},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": traffic,

@@ -1244,6 +1244,23 @@ int deme_integrate(deme_ctx* c) {
     return collect_and_update(c, 1);
 }
 
+int deme_integrate_owners(deme_ctx* c) {
+    int r = check_ready(c);
+    if (r) return r;
+    // the owner kernel reads nothing of the contact list in this mode
+    if (c->ownerJitActive)
+        DEME_LAUNCH(c, "set_clock", launch_set_clock(c->clock.p, c->time, c->stream));
+    DEME_LAUNCH(c, "owner_integrate",
+                c->ownerJitActive
+                    ? c->jitOwner.launch(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, nullptr, nullptr,
+                                         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->rules.p, c->own.p,
+                                         c->own.p, c->acc.p, 2, c->clock.p, c->stream)
+                    : launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, nullptr, nullptr,
+                                          nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->rules.p, c->own.p,
+                                          c->acc.p, 2, c->clock.p, c->stream));
+    return DEME_OK;
+}
+
 // One non-detection step from the captured graph, capturing it first when anything it depends on changed.
 static int step_graph(deme_ctx* c) {
     const bool valid = c->stepExec && c->capturedGen == c->graphGen &&

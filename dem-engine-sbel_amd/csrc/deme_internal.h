@@ -71,8 +71,20 @@ constexpr uint32_t ANAL_TILE = 256 * ANAL_ITEMS;
 inline uint32_t anal_blocks(uint64_t nC) { return (uint32_t)((nC + ANAL_TILE - 1) / ANAL_TILE); }
 
 // scan.hip: out[0..n] = exclusive prefix sum of (a[idx ? idx[i] : i] + (b ? b[i] : 0)); out[n] = total;
-// *totalDev = total (u64). scratch must hold scan_scratch_elems(n) u64 values.
-uint64_t scan_scratch_elems(uint64_t n);
+// *totalDev = total (u64). scratch must hold scan_scratch_elems(n) u64 values: one tile sum per tile and level of the
+// full-tile path, or one per half-size tile of the folded path, whichever is larger, plus a margin.
+constexpr uint64_t scan_scratch_elems(uint64_t n) {
+    uint64_t total = 0;
+    uint64_t m = n;
+    while (true) {
+        const uint64_t nb = (m + SCAN_TILE - 1) / SCAN_TILE;
+        total += nb + 1;
+        if (nb <= 1) break;
+        m = nb;
+    }
+    const uint64_t half = (n + SCAN2_TILE - 1) / SCAN2_TILE + 1;
+    return (total > half ? total : half) + 16;
+}
 hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
                     uint64_t* scratch, hipStream_t st, const uint32_t* idx = nullptr);
 // Two scans over one gather: out1 = scan(a1[idx[i]] + (b1 ? b1[i] : 0)), out2 = scan(a2[idx[i]]), totals at out[n]

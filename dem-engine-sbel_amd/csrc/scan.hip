@@ -268,20 +268,6 @@ hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32
     return hipGetLastError();
 }
 
-uint64_t scan_scratch_elems(uint64_t n) {
-    uint64_t total = 0;
-    uint64_t m = n;
-    while (true) {
-        const uint64_t nb = (m + SCAN_TILE - 1) / SCAN_TILE;
-        total += nb + 1;
-        if (nb <= 1) break;
-        m = nb;
-    }
-    // the half-size-tile path of scan_u32 keeps one sum per tile
-    const uint64_t half = (n + SCAN2_TILE - 1) / SCAN2_TILE + 1;
-    return (total > half ? total : half) + 16;
-}
-
 hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
                     uint64_t* scratch, hipStream_t st, const uint32_t* idx) {
     const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;

@@ -89,7 +89,7 @@ EXPORTED = [
     "deme_set_keep_contact_records", "deme_set_owner_volumes", "deme_inspect", "deme_append_clumps",
     "deme_change_clump_sizes",
     "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner", "deme_jit_render_inspect",
-    "deme_prefix_scan_u32",
+    "deme_prefix_scan_u32", "deme_integrate_owners",
 ]
 
 _lib = None
@@ -136,6 +136,7 @@ def load(path: str = LIB_PATH):
         "deme_download_contacts": ([vp, vp, vp, vp, vp], i32),
         "deme_prefix_scan_u32": ([vp, vp, vp, u64, C.POINTER(u64)], i32),
         "deme_force": ([vp], i32), "deme_reduce_owner": ([vp], i32), "deme_integrate": ([vp], i32),
+        "deme_integrate_owners": ([vp], i32),
         "deme_step": ([vp, i32, i32], i32),
         "deme_download_forces": ([vp, vp, vp, vp, vp], i32),
         "deme_download_wildcards": ([vp, vp], i32), "deme_upload_wildcards": ([vp, vp], i32),
@@ -455,6 +456,7 @@ class DeviceSystem:
     def force(self): self._chk(self.lib.deme_force(self.ctx))
     def reduce_owner(self): self._chk(self.lib.deme_reduce_owner(self.ctx))
     def integrate(self): self._chk(self.lib.deme_integrate(self.ctx))
+    def integrate_owners(self): self._chk(self.lib.deme_integrate_owners(self.ctx))
     def step(self, n, update_freq=1): self._chk(self.lib.deme_step(self.ctx, int(n), int(update_freq)))
     def sync(self): self._chk(self.lib.deme_sync(self.ctx))
     def set_time(self, t): self._chk(self.lib.deme_set_time(self.ctx, float(t)))

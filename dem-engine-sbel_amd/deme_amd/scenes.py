@@ -284,32 +284,104 @@ def spheres_scene(xyz, radius, box, lbf=None, h=1e-5, beta=0.0, g=(0.0, 0.0, -9.
                    quat=to32(quat), families=families, mask=mask, rules=rules, info=dict(box=box))
 
 
+def fcc_neighbours(n, mx, my, mz):
+    """The first n sites of an FCC lattice of mx x my x mz cubic cells (four sites per cell), in z-major order, as
+    integer half-cell coordinates, with each site's 12 nearest neighbours (-1 where the lattice ends)."""
+    base = np.array([[0, 0, 0], [1, 1, 0], [1, 0, 1], [0, 1, 1]], np.int64)
+    ii, jj, kk = np.meshgrid(np.arange(mx), np.arange(my), np.arange(mz), indexing="ij")
+    cells = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1)
+    hc = (2 * cells[:, None, :] + base[None]).reshape(-1, 3)
+    hc = hc[np.lexsort((hc[:, 0], hc[:, 1], hc[:, 2]))][:n]
+    grid = np.full((2 * mx, 2 * my, 2 * mz), -1, np.int64)
+    grid[hc[:, 0], hc[:, 1], hc[:, 2]] = np.arange(n)
+    offs = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)
+            if abs(dx) + abs(dy) + abs(dz) == 2]
+    nb = np.full((n, 12), -1, np.int64)
+    dims = np.array([2 * mx, 2 * my, 2 * mz])
+    for k, o in enumerate(offs):
+        q = hc + np.array(o)
+        ok = np.all((q >= 0) & (q < dims), 1)
+        nb[ok, k] = grid[q[ok, 0], q[ok, 1], q[ok, 2]]
+    return hc, nb
+
+
+def relax_overlaps(x, r, nb, iters, step=0.4):
+    """Pushes lattice neighbours apart: each iteration moves every sphere by step x half the overlap with each of its
+    neighbours, along the line of centres (a Jacobi sweep over the fixed neighbour table; gathers only, so the result
+    does not depend on summation order). Runs on the GPU through torch when one is present."""
+    try:
+        import torch
+
+        dev = "cuda" if torch.cuda.is_available() else "cpu"
+        X = torch.from_numpy(np.ascontiguousarray(x, np.float64)).to(dev)
+        R = torch.from_numpy(np.ascontiguousarray(r, np.float64)).to(dev)
+        N = torch.from_numpy(nb).to(dev)
+        valid = N >= 0
+        Nc = torch.where(valid, N, torch.zeros_like(N))
+        for _ in range(iters):
+            d = X[:, None, :] - X[Nc]                       # n x 12 x 3, from each neighbour to the sphere
+            dist = torch.sqrt((d * d).sum(-1))
+            o = R[:, None] + R[Nc] - dist
+            w = torch.where(valid & (o > 0), step * 0.5 * o / dist.clamp_min(1e-12), torch.zeros_like(o))
+            X = X + (d * w[..., None]).sum(1)
+        out = X.cpu().numpy()
+        del X, R, N, Nc
+        if dev == "cuda":
+            torch.cuda.empty_cache()  # give the device memory back before the library allocates
+        return out
+    except ImportError:
+        X = np.array(x, np.float64)
+        valid = nb >= 0
+        Nc = np.where(valid, nb, 0)
+        for _ in range(iters):
+            d = X[:, None, :] - X[Nc]
+            dist = np.sqrt((d * d).sum(-1))
+            o = r[:, None] + r[Nc] - dist
+            w = np.where(valid & (o > 0), step * 0.5 * o / np.maximum(dist, 1e-12), 0.0)
+            X = X + (d * w[..., None]).sum(1)
+        return X
+
+
 def polydisperse_bed(n=8_000_000, seed=1003, median_r=0.001, sigma=0.2, r_clip=(0.0005, 0.002), n_mat=4,
-                     h=2e-6, plate_speed=0.05, beta=0.0, velocity_sigma=0.0, steps=500):
-    """Lognormal spheres (median 1 mm, sigma 0.2, clipped to [0.5, 2] mm) with four materials assigned at random, on a
-    jittered cubic lattice of pitch 2 x the 90th-percentile radius in an open-top box, under a top plate (its own
-    owner, a prescribed family moving down at plate_speed) one pitch above the bed."""
+                     h=2e-6, plate_speed=0.05, beta=0.0, velocity_sigma=0.0, steps=500, solid_fraction=0.6,
+                     relax_iters=12):
+    """Lognormal spheres (median 1 mm, sigma 0.2, clipped to [0.5, 2] mm) with four materials assigned at random,
+    packed at `solid_fraction` in an open-top box under a top plate (its own owner, a prescribed family moving down
+    at plate_speed) one largest diameter above the bed. At 8M spheres the bed is 0.40 x 0.40 x 0.41 m.
+
+    Packing: the spheres take the sites of a face-centred cubic lattice (in random order, with a jitter of 2% of the
+    spacing) whose spacing gives the requested solid fraction; a few sweeps pushing lattice neighbours apart then
+    spread the overlaps of the larger spheres. Residual overlaps remain (about 1.5 overlapping pairs per sphere,
+    mean depth ~5% of a radius); settle_bed() relaxes them on the device."""
     rng = np.random.default_rng(seed)
     radius = np.clip(median_r * np.exp(sigma * rng.standard_normal(n)), *r_clip).astype(np.float32)
-    pitch = 2.0 * float(np.quantile(radius, 0.9))  # the largest tenth overlap a neighbour or the floor
-    side = max(4, int(np.ceil(0.8 * n ** (1.0 / 3.0))))  # a column about twice as tall as wide
-    layers = int(np.ceil(n / (side * side)))
-    bx = side * pitch + 2 * pitch
-    bz = (layers + 3) * pitch
+    r64 = radius.astype(np.float64)
+    # FCC: volume per site a^3 / sqrt(2) for nearest-neighbour spacing a
+    a = float((4.0 / 3.0 * np.pi * np.sqrt(2.0) * np.mean(r64 ** 3) / solid_fraction) ** (1.0 / 3.0))
+    L = a * np.sqrt(2.0)
+    mx = max(2, int(np.ceil((n / (4 * 1.04)) ** (1.0 / 3.0))))  # a bed about as tall as it is wide
+    mz = int(np.ceil(n / (4 * mx * mx)))
+    hc, nb = fcc_neighbours(n, mx, mx, mz)
+    dmax = 2.0 * float(radius.max())
+    margin = dmax
+    bx = mx * L + 2 * margin
+    bed_h = mz * L
+    bz = bed_h + 3 * dmax + margin
     box = (float(f32(bx)), float(f32(bx)), float(f32(bz)))
     lbf = tuple(float(f32(-f32(b) / f32(2.0))) for b in box)
-    ii, jj, kk = np.meshgrid(np.arange(side), np.arange(side), np.arange(layers), indexing="ij")
-    order = np.lexsort((ii.ravel(), jj.ravel(), kk.ravel()))
-    i, j, k = ii.ravel()[order][:n], jj.ravel()[order][:n], kk.ravel()[order][:n]
-    xyz = np.stack([lbf[0] + 1.5 * pitch + i * pitch, lbf[1] + 1.5 * pitch + j * pitch,
-                    lbf[2] + 0.5 * pitch + k * pitch], 1)
-    xyz += rng.uniform(-0.05 * pitch, 0.05 * pitch, size=xyz.shape)
+    xyz = (hc * 0.5 + 0.25) * L + rng.uniform(-0.02 * a, 0.02 * a, size=(n, 3))
+    xyz = relax_overlaps(xyz, r64, nb, relax_iters)
+    # inside the walls: no sphere deeper into a wall than its radius allows
+    lo = np.array([margin, margin, 0.0])
+    xyz = np.maximum(xyz + lo, r64[:, None] * 0.98)
+    xyz[:, :2] = np.minimum(xyz[:, :2], box[0] - r64[:, None] * 0.98)
+    xyz = xyz + np.array(lbf)
     mat = rng.integers(0, n_mat, n).astype(np.uint8)
     mats = np.zeros(n_mat, MATERIAL_DTYPE)
     for m in range(n_mat):
         mats[m] = (1e7 * (1 + m), 0.25 + 0.02 * m, 0.4 + 0.1 * m, 0.3 + 0.1 * m, 0.0)
-    mass = (RHO * 4.0 / 3.0 * np.pi * radius.astype(np.float64) ** 3).astype(np.float32)
-    moi = (2.0 / 5.0 * mass.astype(np.float64) * radius.astype(np.float64) ** 2).astype(np.float32)
+    mass = (RHO * 4.0 / 3.0 * np.pi * r64 ** 3).astype(np.float32)
+    moi = (2.0 / 5.0 * mass.astype(np.float64) * r64 ** 2).astype(np.float32)
     params = build_sim_params(box, lbf, float(radius.min()), h, beta=beta)
     walls = box_planes(lbf, box, owner=n, material=0, top=False)
     plate = np.zeros(1, ANAL_DTYPE)
@@ -327,18 +399,36 @@ def polydisperse_bed(n=8_000_000, seed=1003, median_r=0.001, sigma=0.2, r_clip=(
     rule["vel"][:] = (0.0, 0.0, -plate_speed, 0.0, 0.0, 0.0)
     mask, rules = default_families({10: rule})
     vel, omg, q = random_motion(rng, n, velocity_sigma)
+    top = float((xyz[:, 2] + r64).max())
+    sphere_volume = float((4.0 / 3.0 * np.pi * r64 ** 3).sum())
+    fill = sphere_volume / ((mx * L) * (mx * L) * (top - lbf[2]))
     sc = _finish("polydisperse_bed", params, xyz, np.arange(n), radius, np.zeros((n, 3)), mat, mass,
                  np.repeat(moi[:, None], 3, 1), mats, anal, 2, 1, steps, vel=vel, mask=mask, rules=rules,
-                 info=dict(box=box, pitch=pitch))
+                 info=dict(box=box, spacing=a, solid_fraction=fill, bed_top=top))
     # the plate owner: centred above the bed, in the prescribed family
-    top = float(xyz[:, 2].max()) + float(radius.max()) + pitch
-    plate_xyz = np.array([[lbf[0] + box[0] / 2, lbf[1] + box[1] / 2, top]], np.float32)
+    plate_xyz = np.array([[lbf[0] + box[0] / 2, lbf[1] + box[1] / 2, top + dmax]], np.float32)
     v, lx, ly, lz = encode_positions(params, plate_xyz)
     st = sc.state
     st["voxelID"][n + 1], st["locX"][n + 1], st["locY"][n + 1], st["locZ"][n + 1] = v[0], lx[0], ly[0], lz[0]
     st["family"][n + 1] = 10
     st["vZ"][n + 1] = -plate_speed
     return sc
+
+
+def settle_bed(dev, sc, steps=400, cor=0.05):
+    """Relaxes a packed bed's residual overlaps on the device: `steps` steps with every material's restitution
+    lowered to `cor` (strongly damped contacts), then the scene's materials back and all sphere velocities zeroed.
+    The plate keeps its prescribed motion (it moves steps x h x plate speed, far less than its clearance)."""
+    damped = sc.materials.copy()
+    damped["CoR"] = np.float32(cor)
+    dev.upload_materials(damped)
+    dev.step(steps, 1)
+    dev.upload_materials(sc.materials)
+    st = dev.get_state()
+    for k in ("vX", "vY", "vZ", "omgX", "omgY", "omgZ"):
+        st[k][:sc.n_clumps] = 0.0
+    dev.set_state(st)
+    dev.clear_contacts()
 
 
 def adjoint_drop(n=50_000, seed=1005, r_range=(0.002, 0.004), n_mat=4, velocity_sigma=0.5, h=1e-5, steps=200,

@@ -119,6 +119,10 @@ int deme_force(deme_ctx* ctx);
 int deme_reduce_owner(deme_ctx* ctx);
 /* integrateOwners (DEMIntegrationKernels.cu:170-179), preceded by the owner reduction of the last forces. */
 int deme_integrate(deme_ctx* ctx);
+/* integrateOwners alone (DEMIntegrationKernels.cu:170-179): one step of every owner from the accelerations the context
+ * holds (those of the last deme_reduce_owner / deme_integrate, or uploaded with deme_upload_state), without collecting
+ * contact forces; needs no contact list. */
+int deme_integrate_owners(deme_ctx* ctx);
 /* The step loop (DEMDynamicThread::workerThread, dT.cpp:1527-1585) with a deterministic contact-update schedule:
  * detection on the state at steps k*f (f = max(update_freq, 1)), force -> collect -> integrate each step. */
 int deme_step(deme_ctx* ctx, int n_steps, int update_freq);

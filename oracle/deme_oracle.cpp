@@ -28,6 +28,7 @@
 #include "deme_oracle.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -128,6 +129,9 @@ std::string fixed17(double v) {
 // ---------------------------------------------------------------------------------------------------------------
 struct ora_ctx {
     deme_sim_params p;
+    // wall seconds per phase, under the reference's timer names (kT.h:179-181, dT.h:253-255): Discretize domain, Find
+    // contact pairs, Build history map, Calculate contact forces, Collect contact forces, Integration
+    double phase[6] = {0, 0, 0, 0, 0, 0};
     uint32_t nS = 0, nO = 0, nA = 0, nM = 0;
     // templates (flattened, ClumpCompAcqStratAllFlatten.cu / MassAcqStratFlatten.cu / MOIAcqStratFlatten.cu)
     std::vector<uint32_t> sphOwner;
@@ -453,7 +457,20 @@ void ora_get_counts(const ora_ctx* c, deme_counts* out) {
 // Rows a6-a8: getNumberOfBinsEachSphereTouches, prefix scans, populateBinSphereTouchingPairs, SortPairs, RLE
 // (DEMBinSphereKernels.cu:12-230, DEMCubContactDetection.cu:59-166)
 // ---------------------------------------------------------------------------------------------------------------
+static double wall_now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int ora_cd_bins_body(ora_ctx* c);
+
 int ora_cd_bins(ora_ctx* c) {
+    const double t0 = wall_now();
+    const int e = ora_cd_bins_body(c);
+    c->phase[0] += wall_now() - t0;
+    return e;
+}
+
+static int ora_cd_bins_body(ora_ctx* c) {
     const uint32_t nS = c->nS;
     c->numBins.assign(nS, 0);
     c->numAnal.assign(nS, 0);
@@ -648,6 +665,7 @@ void canonicalize_and_map(ora_ctx* c, std::vector<uint32_t>& A, std::vector<uint
 }  // namespace
 
 int ora_cd_pairs(ora_ctx* c) {
+    const double t0 = wall_now();
     if (!c->binsReady) return DEME_ERR_NOT_READY;
     if (c->maxInBin > DEME_MAX_SPHERES_PER_BIN) return DEME_ERR_BIN_OVERFLOW;
     const size_t nB = c->activeBin.size();
@@ -717,7 +735,10 @@ int ora_cd_pairs(ora_ctx* c) {
         }
     }
     if (A.size() > 0xFFFFFFFFull) return DEME_ERR_CAPACITY;
+    const double t1 = wall_now();
+    c->phase[1] += t1 - t0;
     canonicalize_and_map(c, A, B, T);
+    c->phase[2] += wall_now() - t1;
     return DEME_OK;
 }
 
@@ -726,7 +747,9 @@ int ora_cd(ora_ctx* c) {
     if (e) return e;
     e = ora_cd_pairs(c);
     if (e) return e;
+    const double t0 = wall_now();
     remap_wildcards(c);
+    c->phase[2] += wall_now() - t0;
     return DEME_OK;
 }
 
@@ -1376,10 +1399,17 @@ int ora_step(ora_ctx* c, int nSteps, int updateFreq) {
             int e = ora_cd(c);
             if (e) return e;
         }
+        const double t0 = wall_now();
         int e = ora_force(c);
         if (e) return e;
+        const double t1 = wall_now();
         ora_collect(c);
+        const double t2 = wall_now();
         ora_integrate(c);
+        const double t3 = wall_now();
+        c->phase[3] += t1 - t0;
+        c->phase[4] += t2 - t1;
+        c->phase[5] += t3 - t2;
         c->time += (double)c->p.h;
         c->nSteps++;
     }
@@ -1391,6 +1421,12 @@ void ora_get_time(const ora_ctx* c, double* t, uint64_t* n) {
     if (n) *n = c->nSteps;
 }
 void ora_set_time(ora_ctx* c, double t) { c->time = t; }
+void ora_phase_times(ora_ctx* c, double* out, int reset) {
+    for (int k = 0; k < 6; k++) {
+        if (out) out[k] = c->phase[k];
+        if (reset) c->phase[k] = 0.0;
+    }
+}
 
 }  // extern "C"
 

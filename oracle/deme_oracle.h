@@ -69,6 +69,9 @@ void ora_get_wildcards(const ora_ctx* c, float* w /* 4 x C, wildcard-major */);
 void ora_set_wildcards(ora_ctx* c, const float* w /* 4 x C */);
 void ora_get_time(const ora_ctx* c, double* t, uint64_t* nSteps);
 void ora_set_time(ora_ctx* c, double t);
+/* Wall seconds accumulated per phase by ora_step / ora_cd (Discretize domain, Find contact pairs, Build history map,
+ * Calculate contact forces, Collect contact forces, Integration); reset != 0 clears them after reading. */
+void ora_phase_times(ora_ctx* c, double* out6, int reset);
 
 /* FP64 evaluation of the same per-contact force for tolerance checks (same inputs, every op in double). */
 void ora_force_fp64(const ora_ctx* c, double* force /*3C*/, double* torqueOnly /*3C*/);

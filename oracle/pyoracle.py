@@ -54,6 +54,7 @@ def load():
     lib.ora_force_fp64.argtypes = [vp, vp, vp]
     lib.ora_get_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     lib.ora_set_time.argtypes = [vp, C.c_double]
+    lib.ora_phase_times.argtypes = [vp, vp, i32]
     lib.ora_figure_out_nv.argtypes = [C.c_float] * 3 + [C.POINTER(C.c_int)] * 3 + [C.POINTER(C.c_float)]
     lib.ora_literal17.argtypes = [C.c_double]
     lib.ora_literal17.restype = C.c_double
@@ -125,6 +126,15 @@ class OracleSystem:
     def integrate(self): self._chk(self.lib.ora_integrate(self.h))
     def step(self, n, update_freq=1): self._chk(self.lib.ora_step(self.h, int(n), int(update_freq)))
     def set_time(self, t): self.lib.ora_set_time(self.h, float(t))
+
+    PHASES = ("Discretize domain", "Find contact pairs", "Build history map", "Calculate contact forces",
+              "Collect contact forces", "Integration")
+
+    def phase_times(self, reset=False):
+        """Wall seconds per phase since the last reset, under the reference's timer names."""
+        out = np.zeros(6, np.float64)
+        self.lib.ora_phase_times(self.h, ptr(out), 1 if reset else 0)
+        return dict(zip(self.PHASES, out.tolist()))
 
     def bin_counts(self):
         n = self.scene.n_spheres
