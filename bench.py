@@ -95,7 +95,9 @@ def run_inspectors(dev, sc, calls=20):
     return out
 
 
-def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False, prepare=None):
+def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False, prepare=None, keep_state=False):
+    """Warm-up, then `steps` timed steps without kernel events, then as many with events (per-kernel durations).
+    keep_state: also return the final owner state (host arrays) in info["state"]."""
     from deme_amd.abi import ANAL_TILE, DeviceSystem
 
     dev = DeviceSystem(sc)
@@ -128,11 +130,13 @@ def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False, prepare
                 nbins=sc.params.nbX * sc.params.nbY * sc.params.nbZ, cells=c.nCells, fresh=update_freq == 1,
                 anal_blocks=(c.nContacts + ANAL_TILE - 1) // ANAL_TILE, inspectors=insp,
                 evented_ms_per_step=1e3 * (te1 - te0) / steps)
+    if keep_state:
+        info["state"] = dev.get_state()
     dev.close()
     return t1 - t0, timing, info
 
 
-def run_bed(steps, warmup, settle_steps=200):
+def run_bed(steps, warmup, settle_steps=200, cpu_threads=0, cpu_steps=2):
     """Configuration 3: 8M spheres with lognormal radii (median 1 mm, sigma 0.2, clipped to [0.5, 2] mm) and 4
     materials packed at ~0.6 solid fraction in a 0.41 x 0.41 x 0.43 m box, a top plate in a prescribed family moving
     down at 0.05 m/s, dt = 2e-6 s, contact detection every step. The packing's residual overlaps are relaxed by a
@@ -140,16 +144,30 @@ def run_bed(steps, warmup, settle_steps=200):
     from deme_amd import scenes
 
     sc = scenes.polydisperse_bed()
-    t, timing, info = run_device(sc, steps, warmup, 1, prepare=lambda dev: scenes.settle_bed(dev, sc, settle_steps))
+    t, timing, info = run_device(sc, steps, warmup, 1, prepare=lambda dev: scenes.settle_bed(dev, sc, settle_steps),
+                                 keep_state=cpu_threads > 0)
     top = sorted(timing.items(), key=lambda kv: -kv[1][0])[:8]
-    return {"spheres": sc.n_spheres, "solid_fraction": round(sc.info["solid_fraction"], 4),
-            "box_m": [round(b, 4) for b in sc.info["box"]], "settle_steps": settle_steps,
-            "contacts": info["C"], "contacts_per_sphere": round(info["C"] / sc.n_spheres, 3), "steps": steps,
-            "sphere_steps_per_s": sc.n_spheres * steps / t, "ms_per_step": 1e3 * t / steps,
-            "kernels_us": {k: round(1e3 * v[0] / max(v[1], 1), 1) for k, v in top}}
+    out = {"spheres": sc.n_spheres, "solid_fraction": round(sc.info["solid_fraction"], 4),
+           "box_m": [round(b, 4) for b in sc.info["box"]], "settle_steps": settle_steps,
+           "contacts": info["C"], "contacts_per_sphere": round(info["C"] / sc.n_spheres, 3), "steps": steps,
+           "sphere_steps_per_s": sc.n_spheres * steps / t, "ms_per_step": 1e3 * t / steps,
+           "kernels_us": {k: round(1e3 * v[0] / max(v[1], 1), 1) for k, v in top}}
+    if cpu_threads > 0:
+        dt, phases, runs = cpu_baseline(sc, cpu_steps, 1, cpu_threads, state=info["state"])
+        out["cpu"] = cpu_row(sc.n_spheres * cpu_steps / dt, "sphere-steps/s", cpu_threads, runs, phases,
+                             f"oracle from the device's final state, detection every step: median of 3 runs of "
+                             f"{cpu_steps} steps after 1 warm-up step")
+        out["gpu_over_cpu"] = out["sphere_steps_per_s"] / out["cpu"]["value"]
+    return out
 
 
-def run_small(steps, warmup):
+def cpu_row(value, unit, threads, runs, phases, sample):
+    return {"value": value, "unit": unit, "cores": threads, "kind": "port", "sample": sample,
+            "runs_s": [round(r, 3) for r in runs],
+            "phase_s_per_step": {k: round(v, 6) for k, v in phases.items()} if phases else None}
+
+
+def run_small(steps, warmup, cpu_threads=0):
     """Configuration 1: 20,000 spheres (r = 5 mm) settling in a 0.3 x 0.3 x 0.6 m box, dt = 1e-5 s, contact detection
     every step; the whole 1,000-step run is timed."""
     from deme_amd import scenes
@@ -157,12 +175,19 @@ def run_small(steps, warmup):
     sc = scenes.sphere_settling()
     t, timing, info = run_device(sc, steps, warmup, 1)
     top = sorted(timing.items(), key=lambda kv: -kv[1][0])[:6]
-    return {"spheres": sc.n_spheres, "contacts": info["C"], "steps": steps,
-            "sphere_steps_per_s": sc.n_spheres * steps / t, "ms_per_step": 1e3 * t / steps,
-            "kernels_us": {k: round(1e3 * v[0] / max(v[1], 1), 1) for k, v in top}}
+    out = {"spheres": sc.n_spheres, "contacts": info["C"], "steps": steps,
+           "sphere_steps_per_s": sc.n_spheres * steps / t, "ms_per_step": 1e3 * t / steps,
+           "kernels_us": {k: round(1e3 * v[0] / max(v[1], 1), 1) for k, v in top}}
+    if cpu_threads > 0:
+        dt, phases, runs = cpu_baseline(sc, steps, 1, cpu_threads)
+        out["cpu"] = cpu_row(sc.n_spheres * steps / dt, "sphere-steps/s", cpu_threads, runs, phases,
+                             f"oracle, the full {steps}-step run from the initial state (median of 3 runs after 1 "
+                             f"warm-up step)")
+        out["gpu_over_cpu"] = out["sphere_steps_per_s"] / out["cpu"]["value"]
+    return out
 
 
-def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05):
+def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05, cpu_threads=0, cpu_steps=2):
     """Batched independent settling columns: one context stepping all scenes together, against the plain loop that
     runs each scene's steps in a context of its own, scene after scene (each context on its own stream; same world,
     same physics: each scene's trajectory is identical). pitch_scale: lattice pitch of the columns over the largest
@@ -196,7 +221,9 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05):
     dev.step(steps, update_freq)
     dev.sync()
     t_graph = time.perf_counter() - t0
+    graph_stats = dev.graph_stats()
     dev.set_step_graphs(False)
+    state = dev.get_state() if cpu_threads > 0 else None
     dev.close()
     parts = [DeviceSystem(p) for p in scenes.split_batch(sc, own_world=True)]
     for d in parts:
@@ -218,11 +245,20 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05):
             "beta_m": beta, "cell_scale": scale, "autotune_ms_per_step": table.round(4).tolist(), "steps": steps,
             "scene_steps_per_s": n_scenes * steps / t_batch, "ms_per_step": 1e3 * t_batch / steps,
             "graph_replay_ms_per_step": 1e3 * t_graph / steps,
+            "graph_captures_replays_fallbacks": list(graph_stats),
             "loop_scene_steps_per_s": n_scenes * steps / t_loop, "loop_ms_per_step": 1e3 * t_loop / steps,
-            "speedup_vs_loop": t_loop / t_batch}
+            "speedup_vs_loop": t_loop / t_batch,
+            "cpu": (None if state is None else batch_cpu(sc, state, cpu_steps, cpu_threads))}
 
 
-def run_adjoint(n, steps, window, warmup_windows=1):
+def batch_cpu(sc, state, steps, threads):
+    dt, phases, runs = cpu_baseline(sc, steps, 1, threads, state=state)
+    return cpu_row(sc.n_scenes * steps / dt, "scene-steps/s", threads, runs, phases,
+                   f"oracle on the whole batch (all scenes in one system) from the device's final state, detection "
+                   f"every step: median of 3 runs of {steps} steps after 1 warm-up step")
+
+
+def run_adjoint(n, steps, window, warmup_windows=1, cpu_threads=0):
     """Differentiation workload: spheres dropped with seeded velocities onto a plane. Forward = the plain step loop
     with the contact list rebuilt every `window` steps; adjoint = deme_adjoint_forward (the same steps with a
     checkpoint per window) and deme_adjoint_backward (per window: recompute with per-step snapshots, then the reverse
@@ -269,11 +305,36 @@ def run_adjoint(n, steps, window, warmup_windows=1):
     gV, gM = dev.adjoint_backward(2.0 * (X - target) / n)
     t_bwd = time.perf_counter() - t0
     dev.close()
-    return {"spheres": n, "materials": len(sc.materials), "steps": steps, "window": window, "contacts": contacts,
-            "forward_ms_per_step": 1e3 * t_fwd / steps, "checkpointed_forward_ms_per_step": 1e3 * t_afwd / steps,
-            "backward_ms_per_step": 1e3 * t_bwd / steps, "adjoint_over_forward": t_bwd / t_fwd,
-            "gradient_eval_over_forward": (t_afwd + t_bwd) / t_fwd, "loss": loss,
-            "grad_E": gM[:, 0].tolist(), "grad_v0_norm": float(np.linalg.norm(gV))}
+    out = {"spheres": n, "materials": len(sc.materials), "steps": steps, "window": window, "contacts": contacts,
+           "forward_ms_per_step": 1e3 * t_fwd / steps, "checkpointed_forward_ms_per_step": 1e3 * t_afwd / steps,
+           "backward_ms_per_step": 1e3 * t_bwd / steps, "adjoint_over_forward": t_bwd / t_fwd,
+           "gradient_eval_over_forward": (t_afwd + t_bwd) / t_fwd, "loss": loss,
+           "grad_E": gM[:, 0].tolist(), "grad_v0_norm": float(np.linalg.norm(gV))}
+    if cpu_threads > 0:
+        out["cpu"] = cpu_adjoint(sc, steps, window, 2.0 * (X - target) / n)
+    return out
+
+
+def cpu_adjoint(sc, steps, window, dLdX, repeats=3):
+    """The FP64 host adjoint (the oracle's forward simulator and reverse sweep) on the same workload: the forward alone
+    and forward + reverse, median of `repeats` each; one thread (the restatement is serial)."""
+    import pyoracle  # noqa: E402  (the oracle is the reported CPU baseline only)
+
+    ref = pyoracle.AdjointFP64(sc, steps, window)
+    fwd, grad = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        ref.forward()
+        fwd.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        ref.gradient(dLdX)
+        grad.append(time.perf_counter() - t0)
+    tf, tg = sorted(fwd)[repeats // 2], sorted(grad)[repeats // 2]
+    return {"forward_ms_per_step": 1e3 * tf / steps, "gradient_ms_per_step": 1e3 * tg / steps,
+            "adjoint_over_forward": (tg - tf) / tf, "cores": 1, "kind": "port",
+            "sample": f"FP64 host forward simulator and reverse sweep, the full {steps} steps (windows of {window}), "
+                      f"median of {repeats}",
+            "forward_runs_s": [round(x, 3) for x in fwd], "gradient_runs_s": [round(x, 3) for x in grad]}
 
 
 def cpu_threads():
@@ -302,18 +363,22 @@ def host_cpu():
     return info
 
 
-def cpu_baseline(sc, steps, update_freq, threads, repeats=3):
+def cpu_baseline(sc, steps, update_freq, threads, repeats=3, state=None):
     """The oracle (host C++/OpenMP restatement of the reference's kernels) on `threads` threads: one warm-up step, then
-    `repeats` timed runs of `steps` steps on the same system; the median run's seconds and its per-phase seconds
-    (the reference's timer names)."""
+    `repeats` timed runs of `steps` steps, each from the same state (the scene's, or `state`); the median run's
+    seconds and its per-phase seconds (the reference's timer names)."""
     import pyoracle  # noqa: E402  (the oracle is the reported CPU baseline only)
 
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
     ora = pyoracle.OracleSystem(sc, threads=threads)
+    if state is not None:
+        ora.set_state(state)
     ora.step(1, update_freq)  # warm-up step
+    start = ora.get_state()
     runs = []
     for _ in range(repeats):
+        ora.set_state(start)
         ora.phase_times(reset=True)
         t0 = time.perf_counter()
         ora.step(steps, update_freq)
@@ -365,6 +430,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    threads = 0 if args.no_cpu else cpu_threads()
     scA = build_scene(args.clumps, 1, 0.0)
     tA, timingA, infoA = run_device(scA, args.steps, args.warmup, 1, inspect=not args.skip_inspect)
     value = args.clumps * args.steps / tA
@@ -379,19 +445,25 @@ def main():
                 "kernels_us_per_step": {k: round(1e3 * v[0] / args.steps, 2)
                                         for k, v in sorted(timingB.items(), key=lambda kv: -kv[1][0])},
                 "device_ms_per_step": round(totB / args.steps, 4)}
+        if threads:
+            dt, phases, runs = cpu_baseline(scB, 10, 5, threads)
+            runB["cpu"] = cpu_row(args.clumps * 10 / dt, "clump-steps/s", threads, runs, phases,
+                                  "oracle, same workload (detection every 5 steps, drift margin 0.2 r_min): median "
+                                  "of 3 runs of 10 steps after 1 warm-up step")
 
     bed = None
     if not args.skip_bed:
-        bed = run_bed(args.bed_steps, 10)
+        bed = run_bed(args.bed_steps, 10, cpu_threads=threads)
 
     small = None
     if not args.skip_small:
-        small = run_small(args.small_steps, 20)
+        small = run_small(args.small_steps, 20, cpu_threads=threads)
 
     batch = batch_loaded = None
     if not args.skip_batch:
         batch = run_batch(args.batch_scenes, args.batch_steps, 3, 1, 0.0)
-        batch_loaded = run_batch(args.batch_scenes, args.batch_steps, 3, 1, 0.0, pitch_scale=0.75)
+        batch_loaded = run_batch(args.batch_scenes, args.batch_steps, 3, 1, 0.0, pitch_scale=0.75,
+                                 cpu_threads=threads)
 
     jit = None
     if not args.skip_jit:
@@ -402,7 +474,7 @@ def main():
 
     adjoint = None
     if not args.skip_adjoint:
-        adjoint = run_adjoint(args.adjoint_spheres, args.adjoint_steps, args.adjoint_window)
+        adjoint = run_adjoint(args.adjoint_spheres, args.adjoint_steps, args.adjoint_window, cpu_threads=threads)
 
     # dominant kernel of the headline run
     dom = max(timingA.items(), key=lambda kv: kv[1][0])
@@ -424,8 +496,7 @@ def main():
                for k, v in sorted(timingA.items(), key=lambda kv: -kv[1][0])}
 
     cpu = None
-    if not args.no_cpu:
-        threads = cpu_threads()
+    if threads:
         dt, phases, runs = cpu_baseline(scA, args.cpu_steps, 1, threads)
         cpu = {"value": args.clumps * args.cpu_steps / dt, "unit": "clump-steps/s", "cores": threads,
                "kind": "port",

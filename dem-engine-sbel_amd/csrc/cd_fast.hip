@@ -314,9 +314,9 @@ __device__ __forceinline__ bool bin_rule_point(const KParams& P, double inv, con
 // by sorted position (entry k of sorted sphere i at slab[k * nS + i]), so the appends of a block hit neighbouring
 // addresses. Pairs beyond the queue's capacity are parked in the lane's column of `surv` and finished by that lane.
 // A column that overflows a slab is reported through maxOcc and the pass is redone with wider columns.
-constexpr uint32_t PAIR_BLOCK = 128;
+constexpr uint32_t PAIR_BLOCK = 128;  // default workgroup size (64, 128 and 256 are built)
 constexpr uint32_t PAIR_QUEUE = 1024;
-static_assert(PAIR_BLOCK <= 128 && PAIR_QUEUE < (1u << 24), "queue codes hold a 7-bit lane and a 24-bit slot");
+static_assert(PAIR_QUEUE < (1u << 23), "queue codes hold a kind bit, an 8-bit lane and a 23-bit slot");
 constexpr int PAIR_SEGS = 5;
 
 // segment k of the half shell: (dy, dz) = (0, 0), (1, 0), (-1, 1), (0, 1), (1, 1)
@@ -331,25 +331,38 @@ __device__ __forceinline__ void append_pair(uint32_t* __restrict__ cnt, uint32_t
 }
 
 // Second phase, test half: the family mask and the bin rule of an overlapping pair of different owners (sorted
-// positions i, j); on acceptance the ids of i and j.
+// positions i, j); on acceptance the ids of i and j. CHECK: the pair passed only the float sweep, so the same-owner
+// rule and the exact expanded-overlap test come first.
+template <bool CHECK>
 __device__ __forceinline__ bool accept_pair(const KParams& P, double inv, const CdRec* __restrict__ sorted,
                                             const uint2* __restrict__ sortedMeta, const uint8_t* __restrict__ mask,
                                             uint32_t i, uint32_t j, uint32_t& idI, uint32_t& idJ) {
-    if (mask && mask[mask_index(sortedMeta[i].y, sortedMeta[j].y)] != DEME_DONT_PREVENT_CONTACT) return false;
+    if (CHECK || mask) {
+        const uint2 mi = sortedMeta[i], mj = sortedMeta[j];
+        if (CHECK && mi.x == mj.x) return false;
+        if (mask && mask[mask_index(mi.y, mj.y)] != DEME_DONT_PREVENT_CONTACT) return false;
+    }
     const CdRec a = load_cd(sorted, i), b = load_cd(sorted, j);
+    if (CHECK && !expanded_overlap(expanded_radius(P, a.r), a, expanded_radius(P, b.r), b)) return false;
     idI = a.id;
     idJ = b.id;
     return a.id < b.id ? bin_rule_point(P, inv, a, b) : bin_rule_point(P, inv, b, a);
 }
 
-// Second phase for one overlapping pair of different owners (sorted positions i, j).
+// Second phase for one overlapping pair of different owners (sorted positions i, j); CHECK as above.
+template <bool CHECK>
 __device__ __forceinline__ void finish_pair(const KParams& P, double inv, uint32_t nS, const CdRec* __restrict__ sorted,
                                             const uint2* __restrict__ sortedMeta, const uint8_t* __restrict__ mask,
                                             uint32_t i, uint32_t j, uint32_t slabCap, uint32_t* rowSlab,
                                             uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt, DevCounters* ctr) {
     // (mask == nullptr: no family pair is masked)
-    if (mask && mask[mask_index(sortedMeta[i].y, sortedMeta[j].y)] != DEME_DONT_PREVENT_CONTACT) return;
+    if (CHECK || mask) {
+        const uint2 mi = sortedMeta[i], mj = sortedMeta[j];
+        if (CHECK && mi.x == mj.x) return;
+        if (mask && mask[mask_index(mi.y, mj.y)] != DEME_DONT_PREVENT_CONTACT) return;
+    }
     const CdRec a = load_cd(sorted, i), b = load_cd(sorted, j);
+    if (CHECK && !expanded_overlap(expanded_radius(P, a.r), a, expanded_radius(P, b.r), b)) return;
     const bool aLow = a.id < b.id;
     if (!(aLow ? bin_rule_point(P, inv, a, b) : bin_rule_point(P, inv, b, a))) return;
     const uint32_t loIdx = aLow ? i : j, hiIdx = aLow ? j : i;
@@ -358,8 +371,13 @@ __device__ __forceinline__ void finish_pair(const KParams& P, double inv, uint32
     append_pair(bCnt, mirSlab, hiIdx, loId, nS, slabCap, ctr);
 }
 
-template <bool PRE>
-__global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G, uint32_t nS, uint32_t nBlocks,
+// Sweep modes: the exact test on every candidate (PAIR_EXACT); a float pre-reject with the exact test on the
+// survivors (PAIR_PRE); the float test alone, with the same-owner rule and the exact test in the shared second phase
+// (PAIR_FLOAT).
+enum : int { PAIR_EXACT = 0, PAIR_PRE = 1, PAIR_FLOAT = 2 };
+
+template <int MODE, uint32_t BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_cell_pairs(KParams P, CellGrid G, uint32_t nS, uint32_t nBlocks,
                                                            const CdRec* __restrict__ sorted,
                                                            const float4* __restrict__ sortedF, float margin,
                                                            const uint2* __restrict__ sortedMeta,
@@ -370,7 +388,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
                                                            uint32_t* bCnt, DevCounters* ctr) {
     __shared__ uint32_t qI[PAIR_QUEUE], qJ[PAIR_QUEUE];
     __shared__ uint32_t qLen;
-    __shared__ uint32_t selfCnt[2][PAIR_BLOCK], selfBase[2][PAIR_BLOCK];
+    static_assert(BLOCK <= 256, "queue codes hold an 8-bit lane");
+    __shared__ uint32_t selfCnt[2][BLOCK], selfBase[2][BLOCK];
     // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so XCD k gets a contiguous share of the
     // sphere range and the rows its blocks share stay in its L2
     const uint32_t per = (nBlocks + 7) / 8;
@@ -379,10 +398,11 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
     if (lb >= nBlocks || ctr->err) return;
     if (threadIdx.x == 0) qLen = 0;
     __syncthreads();
-    const uint32_t i = lb * PAIR_BLOCK + threadIdx.x;
+    const uint32_t i = lb * BLOCK + threadIdx.x;
     uint32_t nc = 0;  // pairs parked in surv
     if (i < nS) {
         const CdRec me = load_cd(sorted, i);
+        constexpr bool PRE = MODE != PAIR_EXACT;
         const float4 mf = PRE ? sortedF[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t myOwner = sortedMeta[i].x;
         int cx, cy, cz;
@@ -392,24 +412,40 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
         const double ra = expanded_radius(P, me.r);
         const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < (int)G.nx - 1 ? cx + 1 : cx;
         uint32_t* col = surv + i;
+        // every segment's bounds are loaded before the first candidate, so their latencies overlap
+        uint32_t segB[PAIR_SEGS], segE[PAIR_SEGS];
+#pragma unroll
         for (int sg = 0; sg < PAIR_SEGS; sg++) {
             const int y = cy + seg_dy(sg), z = cz + seg_dz(sg);
+            segB[sg] = segE[sg] = 0;
             if (y < 0 || y >= (int)G.ny || z >= (int)G.nz) continue;
-            uint32_t b, e;
             if (sg == 0) {
                 // own cell after this sphere, then the next cell in x
-                b = i + 1;
-                e = cellStart[myCell + (uint32_t)(x1 - cx) + 1];
+                segB[sg] = i + 1;
+                segE[sg] = cellStart[myCell + (uint32_t)(x1 - cx) + 1];
             } else {
                 const uint32_t c0 = (uint32_t)x0 + (uint32_t)y * G.nx + (uint32_t)z * nxny;
-                b = cellStart[c0];
-                e = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
+                segB[sg] = cellStart[c0];
+                segE[sg] = cellStart[c0 + (uint32_t)(x1 - x0) + 1];
             }
+        }
+        for (int sg = 0; sg < PAIR_SEGS; sg++) {
+            const uint32_t b = segB[sg], e = segE[sg];
             // two candidates per iteration: both loads are in flight before either test. With PRE, the float copies
             // reject the pairs that are farther apart than their radii by more than the rounding margin, and only the
             // survivors load the full records for the reference's double-precision test (worth it where most
             // candidates are far: drift margins, wide radius spreads).
             // the candidate's owner is loaded with its record, so an overlapping pair costs no second round trip
+            auto push = [&](uint32_t j) {
+                const uint32_t q = atomicAdd(&qLen, 1u);
+                if (q < PAIR_QUEUE) {
+                    qI[q] = i;
+                    qJ[q] = j;
+                } else {
+                    if (nc < slabCap) col[(uint64_t)nc * nS] = j;
+                    nc++;
+                }
+            };
             auto test = [&](const CdRec& ot, uint32_t otOwner, uint32_t j) {
                 if (!expanded_overlap(ra, me, expanded_radius(P, ot.r), ot)) return;
                 if (otOwner == myOwner) return;  // same clump: never a contact
@@ -428,7 +464,18 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
                 const float lim = mf.w + of.w + margin;
                 return !(dx * dx + dy * dy + dz * dz > lim * lim);
             };
-            if (PRE) {
+            if (MODE == PAIR_FLOAT) {
+                // float copies only: 16 bytes per candidate; the few survivors get their exact tests in the queue
+                for (; j + 3 < e; j += 4) {
+                    const float4 o0 = sortedF[j], o1 = sortedF[j + 1], o2 = sortedF[j + 2], o3 = sortedF[j + 3];
+                    if (pre(o0)) push(j);
+                    if (pre(o1)) push(j + 1);
+                    if (pre(o2)) push(j + 2);
+                    if (pre(o3)) push(j + 3);
+                }
+                for (; j < e; j++)
+                    if (pre(sortedF[j])) push(j);
+            } else if (PRE) {
                 for (; j + 1 < e; j += 2) {
                     const float4 o0 = sortedF[j], o1 = sortedF[j + 1];
                     if (pre(o0)) test(load_cd(sorted, j), sortedMeta[j].x, j);
@@ -452,17 +499,17 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
     __syncthreads();
     const double inv = 1.0 / P.binSize;
     const uint32_t nq = min(qLen, PAIR_QUEUE);
-    const uint32_t blockBase = lb * PAIR_BLOCK;
+    const uint32_t blockBase = lb * BLOCK;
     selfCnt[0][threadIdx.x] = selfCnt[1][threadIdx.x] = 0;
     __syncthreads();
     // Every accepted pair makes one entry in each of the two spheres' columns. The partner's entry is appended at
     // once (global atomic); the finder's entry -- always a sphere of this block -- is counted in LDS, so each of the
     // block's spheres takes all its own entries with a single global atomic and they are written together.
     constexpr uint32_t REJECTED = 0xFFFFFFFFu;
-    for (uint32_t q = threadIdx.x; q < nq; q += PAIR_BLOCK) {
+    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
         const uint32_t si = qI[q], sj = qJ[q];
         uint32_t idI, idJ;
-        if (!accept_pair(P, inv, sorted, sortedMeta, mask, si, sj, idI, idJ)) {
+        if (!accept_pair<MODE == PAIR_FLOAT>(P, inv, sorted, sortedMeta, mask, si, sj, idI, idJ)) {
             qI[q] = REJECTED;
             continue;
         }
@@ -472,7 +519,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
         else append_pair(aCnt, rowSlab, sj, idI, nS, slabCap, ctr);
         const uint32_t local = si - blockBase;
         const uint32_t ls = atomicAdd(&selfCnt[kind][local], 1u);
-        qI[q] = (kind << 31) | (local << 24) | ls;
+        qI[q] = (kind << 31) | (local << 23) | ls;
         qJ[q] = idJ;
     }
     __syncthreads();
@@ -482,10 +529,10 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
         if (selfCnt[1][t]) selfBase[1][t] = atomicAdd(&bCnt[blockBase + t], selfCnt[1][t]);
     }
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < nq; q += PAIR_BLOCK) {
+    for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
         const uint32_t code = qI[q];
         if (code == REJECTED) continue;
-        const uint32_t kind = code >> 31, local = (code >> 24) & 0x7Fu;
+        const uint32_t kind = code >> 31, local = (code >> 23) & 0xFFu;
         const uint32_t slot = selfBase[kind][local] + (code & 0xFFFFFFu);
         if (slot < slabCap) (kind == 0 ? rowSlab : mirSlab)[(uint64_t)slot * nS + blockBase + local] = qJ[q];
         else atomicMax(&ctr->maxOcc, slot + 1);
@@ -495,8 +542,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_cell_pairs(KParams P, CellGrid G
         return;
     }
     for (uint32_t k = 0; k < nc; k++)
-        finish_pair(P, inv, nS, sorted, sortedMeta, mask, i, surv[(uint64_t)k * nS + i], slabCap, rowSlab, mirSlab,
-                    aCnt, bCnt, ctr);
+        finish_pair<MODE == PAIR_FLOAT>(P, inv, nS, sorted, sortedMeta, mask, i, surv[(uint64_t)k * nS + i], slabCap,
+                                        rowSlab, mirSlab, aCnt, bCnt, ctr);
 }
 
 // Canonical rows: plane contacts (ascending plane index) then partners (ascending id); idA for the row. Mirror rows:
@@ -571,21 +618,44 @@ hipError_t launch_cell_scatter(const KParams& P, const SphereRec* sph, const Own
     return hipGetLastError();
 }
 
+template <uint32_t BLOCK>
+static void launch_pairs_block(int mode, uint32_t grid, hipStream_t st, const KParams& P, const CellGrid& G,
+                               uint32_t nS, uint32_t nb, const CdRec* sorted, const float4* sortedF, float margin,
+                               const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask,
+                               uint32_t slabCap, uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt,
+                               uint32_t* bCnt, DevCounters* ctr) {
+    if (mode == PAIR_FLOAT)
+        k_cell_pairs<PAIR_FLOAT, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedF, margin, sortedMeta,
+                                                                cellStart, mask, slabCap, surv, rowSlab, mirSlab,
+                                                                aCnt, bCnt, ctr);
+    else if (mode == PAIR_PRE)
+        k_cell_pairs<PAIR_PRE, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedF, margin, sortedMeta,
+                                                              cellStart, mask, slabCap, surv, rowSlab, mirSlab, aCnt,
+                                                              bCnt, ctr);
+    else
+        k_cell_pairs<PAIR_EXACT, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, nullptr, 0.f, sortedMeta,
+                                                                cellStart, mask, slabCap, surv, rowSlab, mirSlab,
+                                                                aCnt, bCnt, ctr);
+}
+
 hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
-                             const float4* sortedF, float margin,
+                             const float4* sortedF, float margin, int mode, uint32_t block,
                              const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
                              uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
                              DevCounters* ctr, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    const uint32_t nb = blocks(nS, PAIR_BLOCK);
-    if (sortedF)
-        k_cell_pairs<true><<<8 * ((nb + 7) / 8), PAIR_BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedF, margin,
-                                                                      sortedMeta, cellStart, mask, slabCap, surv,
-                                                                      rowSlab, mirSlab, aCnt, bCnt, ctr);
+    if (mode != PAIR_EXACT && !sortedF) return hipErrorInvalidValue;
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    const uint32_t nb = blocks(nS, block), grid = 8 * ((nb + 7) / 8);
+    if (block == 64)
+        launch_pairs_block<64>(mode, grid, st, P, G, nS, nb, sorted, sortedF, margin, sortedMeta, cellStart, mask,
+                               slabCap, surv, rowSlab, mirSlab, aCnt, bCnt, ctr);
+    else if (block == 256)
+        launch_pairs_block<256>(mode, grid, st, P, G, nS, nb, sorted, sortedF, margin, sortedMeta, cellStart, mask,
+                                slabCap, surv, rowSlab, mirSlab, aCnt, bCnt, ctr);
     else
-        k_cell_pairs<false><<<8 * ((nb + 7) / 8), PAIR_BLOCK, 0, st>>>(P, G, nS, nb, sorted, nullptr, 0.f,
-                                                                       sortedMeta, cellStart, mask, slabCap, surv,
-                                                                       rowSlab, mirSlab, aCnt, bCnt, ctr);
+        launch_pairs_block<128>(mode, grid, st, P, G, nS, nb, sorted, sortedF, margin, sortedMeta, cellStart, mask,
+                                slabCap, surv, rowSlab, mirSlab, aCnt, bCnt, ctr);
     return hipGetLastError();
 }
 

@@ -97,6 +97,23 @@ struct deme_ctx {
     uint64_t graphGen = 1, capturedGen = 0;
     hipGraphExec_t stepExec = nullptr;
     KParams capturedKp{};
+    // A whole detection interval (the detection pass and the f steps up to the next one) captured in one graph per
+    // list parity: kernels after the pass are guarded on its counters (grids cover the buffers' capacity), so nothing
+    // needs the host between the pass and the steps. A replay whose pass failed or outgrew the buffers moves nothing;
+    // the host then runs that interval directly (regrowing) and captures again.
+    struct Cycle {
+        hipGraphExec_t exec = nullptr;
+        uint64_t gen = 0;
+        int f = 0;
+        KParams kp{};
+        SpecGuard guard{};
+        std::vector<const void*> ptrs;
+    } cycle[2];
+    uint64_t cycleReplays = 0, cycleFallbacks = 0, cycleCaptures = 0;
+    int pairMode = -1;  // pair sweep mode (launch_cell_pairs); -1: chosen from the radius spread
+    uint32_t pairBlock = 128;  // pair sweep workgroup size
+    double cycleHeadroom = 1.25;  // capacity of a captured interval: headroom x the current counts + slack
+    uint64_t cycleSlack = 1024;
     // force kernel specialised at run time (hipRTC); inactive: the ahead-of-time kernel of the built-in model
     JitForce jit;
     bool jitActive = false, jitStale = false, jitCustom = false;
@@ -429,6 +446,8 @@ int deme_ctx_destroy(deme_ctx* c) {
     for (auto& kv : c->inspectJit) kv.second.release();
     c->inspPart.release(); c->ownerVol.release(); c->scanTmp.release(); c->scan2Sums.release(); c->sortedF.release();
     if (c->stepExec) (void)hipGraphExecDestroy(c->stepExec);
+    for (auto& cy : c->cycle)
+        if (cy.exec) (void)hipGraphExecDestroy(cy.exec);
     if (c->hctr) (void)hipHostFree(c->hctr);
     if (c->evCounts) (void)hipEventDestroy(c->evCounts);
     if (c->ownStream) (void)hipStreamDestroy(c->stream);
@@ -943,11 +962,9 @@ static int enqueue_rows(deme_ctx* c, SpecGuard guard) {
     return DEME_OK;
 }
 
-int deme_cd_run(deme_ctx* c) {
-    c->graphGen++;
-    int r = check_ready(c);
-    if (r) return r;
-    r = setup_grid(c);
+// Cell grid and the per-sphere buffers of a detection pass; *mode: the pair sweep's mode (launch_cell_pairs).
+static int cd_buffers(deme_ctx* c, int* mode) {
+    int r = setup_grid(c);
     if (r) return r;
     const uint32_t nS = c->nS;
     DEME_HIP(c, c->cellOf.ensure(nS));
@@ -958,15 +975,22 @@ int deme_cd_run(deme_ctx* c) {
     DEME_HIP(c, c->sorted.ensure(nS));
     // the float pre-reject of the pair sweep pays where most candidates are far from touching: radii spread wide
     // enough that the cells (sized for the largest) hold many small spheres. With near-uniform radii (drift margin
-    // or not) the extra float copy costs more than the record loads it saves.
-    const bool pre = c->maxRadius > 1.5f * c->meanRadius;
-    if (pre) DEME_HIP(c, c->sortedF.ensure(nS));
+    // or not) the extra float copy costs more than the record loads it saves. A mode set on the context wins.
+    *mode = c->pairMode >= 0 ? c->pairMode : (c->maxRadius > 1.5f * c->meanRadius ? 1 : 0);
+    if (*mode != 0) DEME_HIP(c, c->sortedF.ensure(nS));
     DEME_HIP(c, c->planeBits.ensure(nS));
     DEME_HIP(c, c->scanTmp.ensure(nS));
     DEME_HIP(c, c->scan2Sums.ensure(2 * ((nS + SCAN2_TILE - 1) / SCAN2_TILE + 1)));
     DEME_HIP(c, c->survSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
+    return DEME_OK;
+}
+
+// The detection pass up to the sorted records: plane poses, the pass's counters and cell counts cleared, cell
+// assignment (bin counts and plane contacts on the way), the cell scan and the counting-sort scatter.
+static int enqueue_cell_pass(deme_ctx* c, int mode) {
+    const uint32_t nS = c->nS;
     DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
     // cell counts and the pass's counters cleared by one kernel (a small memset is two fills on this runtime)
     static_assert(sizeof(DevCounters) % 4 == 0, "counters are cleared as 32-bit words");
@@ -983,7 +1007,47 @@ int deme_cd_run(deme_ctx* c) {
     DEME_LAUNCH(c, "cell_scatter",
                 launch_cell_scatter(c->kp, c->sph.p, c->own.p, nS, c->cellOf.p, c->slotOf.p, c->cellStart.p, c->sorted.p,
                                     c->sortedMeta.p, c->posOf.p, c->aCnt.p, c->bCnt.p, c->sp.beta,
-                                    pre ? c->sortedF.p : nullptr, c->stream));
+                                    mode != 0 ? c->sortedF.p : nullptr, c->stream));
+    return DEME_OK;
+}
+
+// The pair sweep and the row / mirror-row scans of the list at parity cu (totals in the device counters).
+static int enqueue_pair_pass(deme_ctx* c, int mode, int cu) {
+    const uint32_t nS = c->nS;
+    DEME_LAUNCH(c, "cell_pairs",
+                launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, mode != 0 ? c->sortedF.p : nullptr,
+                                  sweep_margin(c->sp), mode, c->pairBlock,
+                                  c->sortedMeta.p, c->cellStart.p,
+                                  c->maskAny ? c->mask.p : nullptr,
+                                  c->slabCap, c->survSlab.p, c->rowSlab.p, c->mirSlab.p, c->aCnt.p, c->bCnt.p,
+                                  c->ctr.p, c->stream));
+    // row lengths in id order: partner counts gathered from sorted positions, plus plane contacts; mirror rows
+    if ((uint64_t)nS <= SCAN2_TILE * SCAN_TILE) {
+        DEME_LAUNCH(c, "scan_rows",
+                    scan2_u32_gather(c->aCnt.p, c->numAnal.p, c->bCnt.p, c->posOf.p, nS, c->rowStart[cu].p,
+                                     c->bRowStart.p, &c->ctr.p->total[1], &c->ctr.p->total[2], c->scanTmp.p,
+                                     c->scan2Sums.p, c->stream));
+    } else {
+        DEME_LAUNCH(c, "scan_rows",
+                    scan_u32(c->aCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1],
+                             c->scanScratch.p, c->stream, c->posOf.p));
+        DEME_LAUNCH(c, "scan_brows",
+                    scan_u32(c->bCnt.p, nullptr, nS, c->bRowStart.p, &c->ctr.p->total[2], c->scanScratch.p,
+                             c->stream, c->posOf.p));
+    }
+    return DEME_OK;
+}
+
+int deme_cd_run(deme_ctx* c) {
+    c->graphGen++;
+    int r = check_ready(c);
+    if (r) return r;
+    int mode = 0;
+    r = cd_buffers(c, &mode);
+    if (r) return r;
+    const uint32_t nS = c->nS;
+    r = enqueue_cell_pass(c, mode);
+    if (r) return r;
     // the current list becomes the previous one
     if (c->contactsReady) {
         c->cur ^= 1;
@@ -995,27 +1059,8 @@ int deme_cd_run(deme_ctx* c) {
     const int cu = c->cur;
     SpecGuard spec;
     for (int attempt = 0;; attempt++) {
-        DEME_LAUNCH(c, "cell_pairs",
-                    launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, pre ? c->sortedF.p : nullptr,
-                                      sweep_margin(c->sp),
-                                      c->sortedMeta.p, c->cellStart.p,
-                                      c->maskAny ? c->mask.p : nullptr,
-                                      c->slabCap, c->survSlab.p, c->rowSlab.p, c->mirSlab.p, c->aCnt.p, c->bCnt.p,
-                                      c->ctr.p, c->stream));
-        // row lengths in id order: partner counts gathered from sorted positions, plus plane contacts; mirror rows
-        if ((uint64_t)nS <= SCAN2_TILE * SCAN_TILE) {
-            DEME_LAUNCH(c, "scan_rows",
-                        scan2_u32_gather(c->aCnt.p, c->numAnal.p, c->bCnt.p, c->posOf.p, nS, c->rowStart[cu].p,
-                                         c->bRowStart.p, &c->ctr.p->total[1], &c->ctr.p->total[2], c->scanTmp.p,
-                                         c->scan2Sums.p, c->stream));
-        } else {
-            DEME_LAUNCH(c, "scan_rows",
-                        scan_u32(c->aCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1],
-                                 c->scanScratch.p, c->stream, c->posOf.p));
-            DEME_LAUNCH(c, "scan_brows",
-                        scan_u32(c->bCnt.p, nullptr, nS, c->bRowStart.p, &c->ctr.p->total[2], c->scanScratch.p,
-                                 c->stream, c->posOf.p));
-        }
+        r = enqueue_pair_pass(c, mode, cu);
+        if (r) return r;
         // The counters go to the host without a stream-wide wait: the rows and the history map are enqueued behind
         // them with the buffers at hand (skipped on the device if the pass failed or the lists do not fit), so the
         // GPU keeps working while the host reads the totals.
@@ -1165,14 +1210,28 @@ static uint8_t* term_flags(deme_ctx* c, bool bSide) {
 // ahead-of-time kernel of the built-in model.
 static hipError_t launch_force_ctx(deme_ctx* c, bool full, uint32_t nC, const uint32_t* idB, const uint8_t* ctype,
                                    const OwnerRec* own, const float4* histIn, const uint32_t* map, float4* histOut,
-                                   ForceRec* out) {
+                                   ForceRec* out, SpecGuard guard = SpecGuard()) {
     if (c->jitActive)
         return c->jit.launch(full, c->kp, nC, c->idA.p, idB, ctype, c->sph.p, own, c->moi.p, c->anal.p, c->pose.p,
                              c->mats.p, histIn, map, histOut, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p,
-                             term_flags(c, false), term_flags(c, true), out, c->stream);
+                             term_flags(c, false), term_flags(c, true), out, c->stream, guard);
     return launch_force(c->kp, nC, c->idA.p, idB, ctype, c->sph.p, own, c->moi.p, c->anal.p, c->pose.p, c->mats.p,
                         histIn, map, histOut, c->mirror.p, c->aTerm.p, c->bTerm.p, c->gTerm.p, term_flags(c, false),
-                        term_flags(c, true), out, c->stream);
+                        term_flags(c, true), out, c->stream, guard);
+}
+
+// The owner update of the context (the specialised kernel when motion strategies or rules were given): owner sums of
+// the list at parity cu, then the integration when `integrate`.
+static hipError_t launch_owner_ctx(deme_ctx* c, int integrate, int cu, SpecGuard guard = SpecGuard()) {
+    if (c->ownerJitActive)
+        return c->jitOwner.launch(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
+                                  c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, term_flags(c, false),
+                                  term_flags(c, true), c->analAcc.p, c->rules.p, c->own.p, c->own.p, c->acc.p,
+                                  integrate, c->clock.p, c->stream, guard);
+    return launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
+                               c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p, term_flags(c, false),
+                               term_flags(c, true), c->analAcc.p, c->rules.p, c->own.p, c->acc.p, integrate,
+                               c->clock.p, c->stream, nullptr, guard);
 }
 
 // poseFresh: a detection pass has just computed the plane poses from the same owner state (the same step), so the
@@ -1216,17 +1275,7 @@ static int collect_and_update(deme_ctx* c, int integrate, bool capturing = false
     // replayed from a graph gets it from set_step_clock, launched outside the graph)
     if (integrate && c->ownerJitActive && !capturing)
         DEME_LAUNCH(c, "set_clock", launch_set_clock(c->clock.p, c->time, c->stream));
-    DEME_LAUNCH(c, integrate ? "owner_update" : "owner_reduce",
-                c->ownerJitActive
-                    ? c->jitOwner.launch(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
-                                         c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p,
-                                         term_flags(c, false), term_flags(c, true), c->analAcc.p,
-                                         c->rules.p, c->own.p, c->own.p, c->acc.p, integrate, c->clock.p,
-                                         c->stream)
-                    : launch_owner_update(c->kp, c->nO, c->nClumps, c->nClumps, c->nAnalOwners, c->ownerSphStart.p,
-                                          c->rowStart[cu].p, c->bRowStart.p, c->aTerm.p, c->bTerm.p,
-                                          term_flags(c, false), term_flags(c, true), c->analAcc.p,
-                                          c->rules.p, c->own.p, c->acc.p, integrate, c->clock.p, c->stream));
+    DEME_LAUNCH(c, integrate ? "owner_update" : "owner_reduce", launch_owner_ctx(c, integrate, cu));
     return DEME_OK;
 }
 
@@ -1298,6 +1347,179 @@ static int step_graph(deme_ctx* c) {
     return DEME_OK;
 }
 
+// ---- detection intervals captured whole ----
+// Everything a captured interval of list parity p baked in (buffer addresses, sizes, grid, kernels); a replay is
+// valid only while all of it is unchanged.
+static std::vector<const void*> cycle_fingerprint(deme_ctx* c, int p) {
+    const int q = p ^ 1;
+    auto v = [](uint64_t x) { return reinterpret_cast<const void*>((uintptr_t)x); };
+    uint64_t invBits;
+    std::memcpy(&invBits, &c->grid.inv, sizeof(invBits));
+    return {c->sph.p,        c->own.p,        c->moi.p,         c->anal.p,        c->pose.p,       c->mats.p,
+            c->mask.p,       c->rules.p,      c->ownerSphStart.p, c->cellCount.p, c->cellStart.p,  c->cellOf.p,
+            c->slotOf.p,     c->posOf.p,      c->aCnt.p,        c->bCnt.p,        c->sortedMeta.p, c->sorted.p,
+            c->sortedF.p,    c->planeBits.p,  c->numBins.p,     c->numAnal.p,     c->scanTmp.p,    c->scan2Sums.p,
+            c->scanScratch.p, c->survSlab.p,  c->rowSlab.p,     c->mirSlab.p,     c->rowStart[p].p, c->rowStart[q].p,
+            c->bRowStart.p,  c->idA.p,        c->idB[p].p,      c->idB[q].p,      c->ctype[p].p,   c->ctype[q].p,
+            c->hist[p].p,    c->hist[q].p,    c->map.p,         c->mirror.p,      c->bPart.p,      c->aTerm.p,
+            c->bTerm.p,      c->gTerm.p,      c->aOn.p,         c->bOn.p,         c->analPartials.p, c->analAcc.p,
+            c->acc.p,        c->clock.p,      c->ctr.p,         c->planeStart.p,
+            reinterpret_cast<const void*>(c->jit.full), reinterpret_cast<const void*>(c->jit.terms),
+            reinterpret_cast<const void*>(c->jitOwner.fn),
+            v(c->slabCap),   v(c->nCells),    v(c->nS),         v(c->nO),         v(c->grid.nx),   v(c->grid.ny),
+            v(c->grid.nz),   v(invBits),      v(c->jitActive),  v(c->ownerJitActive), v(c->maskAny)};
+}
+
+// Buffers of the list at parity p (the one the interval's pass writes; the other parity holds the current list and
+// is left alone) and the shared per-contact buffers, sized with headroom (room for lists that grow a little between
+// passes without a new capture); the guard the interval's kernels check.
+static int cycle_capacity(deme_ctx* c, int p, SpecGuard& g) {
+    const size_t wantC = std::max<size_t>((size_t)((double)c->C * c->cycleHeadroom) + c->cycleSlack, 1);
+    const size_t wantP = std::max<size_t>((size_t)((double)c->nPairs * c->cycleHeadroom) + c->cycleSlack, 1);
+    DEME_HIP(c, c->idB[p].ensure(wantC, 1.0));
+    DEME_HIP(c, c->ctype[p].ensure(wantC, 1.0));
+    DEME_HIP(c, c->hist[p].ensure(wantC, 1.0));
+    DEME_HIP(c, c->idA.ensure(wantC, 1.0));
+    DEME_HIP(c, c->map.ensure(wantC, 1.0));
+    DEME_HIP(c, c->mirror.ensure(wantC, 1.0));
+    DEME_HIP(c, c->aTerm.ensure(6 * wantC, 1.0));
+    DEME_HIP(c, c->gTerm.ensure(6 * wantC, 1.0));
+    DEME_HIP(c, c->aOn.ensure(wantC, 1.0));
+    DEME_HIP(c, c->bPart.ensure(wantP, 1.0));
+    DEME_HIP(c, c->bTerm.ensure(6 * wantP, 1.0));
+    DEME_HIP(c, c->bOn.ensure(wantP, 1.0));
+    DEME_HIP(c, c->analPartials.ensure((size_t)std::max(anal_blocks(wantC), 1u) * MAX_ANAL_OWNERS * 6, 1.0));
+    DEME_HIP(c, c->analAcc.ensure(MAX_ANAL_OWNERS * 6, 1.0));
+    g = SpecGuard();
+    g.ctr = c->ctr.p;
+    g.slabCap = c->slabCap;
+    const uint64_t analCap = (uint64_t)(c->analPartials.cap / (MAX_ANAL_OWNERS * 6)) * ANAL_TILE;
+    g.capC = std::min<uint64_t>({c->idB[p].cap, c->ctype[p].cap, c->hist[p].cap, c->idA.cap, c->map.cap,
+                                 c->mirror.cap, c->aTerm.cap / 6, c->gTerm.cap / 6, c->aOn.cap, analCap,
+                                 0xFFFFFFFFull});
+    g.capP = std::min<uint64_t>({c->bPart.cap, c->bTerm.cap / 6, c->bOn.cap});
+    return DEME_OK;
+}
+
+// The interval's launches with the new list at parity c->cur: the detection pass, the rows and history map, then f
+// steps (the first gathers the history through the map), each advancing the device clock by h.
+static int enqueue_cycle(deme_ctx* c, int f, const SpecGuard& g, int mode) {
+    const int p = c->cur, q = p ^ 1;
+    int r = enqueue_cell_pass(c, mode);
+    if (!r) r = enqueue_pair_pass(c, mode, p);
+    if (!r) r = enqueue_rows(c, g);
+    if (r) return r;
+    const uint32_t capC = (uint32_t)g.capC;
+    for (int s = 0; s < f; s++) {
+        if (s > 0)
+            DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
+        DEME_LAUNCH(c, "force",
+                    launch_force_ctx(c, false, capC, c->idB[p].p, c->ctype[p].p, c->own.p,
+                                     s == 0 ? c->hist[q].p : c->hist[p].p, s == 0 ? c->map.p : nullptr, c->hist[p].p,
+                                     nullptr, g));
+        uint32_t nb = 0;
+        DEME_LAUNCH(c, "anal_collect",
+                    launch_anal_collect(capC, c->idB[p].p, c->ctype[p].p, c->gTerm.p, c->anal.p, c->nA, c->nClumps,
+                                        c->nAnalOwners, c->analPartials.p, c->analPartials.cap, &nb, c->stream, g));
+        DEME_LAUNCH(c, "anal_final",
+                    launch_anal_final(c->analPartials.p, nb, c->nAnalOwners, c->analAcc.p, c->stream, g));
+        DEME_LAUNCH(c, "owner_update", launch_owner_ctx(c, 1, p, g));
+        DEME_LAUNCH(c, "advance_clock", launch_advance_clock(c->clock.p, c->sp.h, c->stream));
+    }
+    return DEME_OK;
+}
+
+static int capture_cycle(deme_ctx* c, int p, int f) {
+    deme_ctx::Cycle& cy = c->cycle[p];
+    int mode = 0;
+    int r = cd_buffers(c, &mode);
+    if (r) return r;
+    SpecGuard g;
+    r = cycle_capacity(c, p, g);
+    if (r) return r;
+    const int savedCur = c->cur;
+    const bool savedPrev = c->havePrev;
+    c->cur = p;
+    c->havePrev = true;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        c->cur = savedCur;
+        c->havePrev = savedPrev;
+        return c->fail_hip(e, "hipStreamBeginCapture");
+    }
+    r = enqueue_cycle(c, f, g, mode);
+    e = hipStreamEndCapture(c->stream, &graph);
+    c->cur = savedCur;
+    c->havePrev = savedPrev;
+    if (r) {
+        if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        return r;
+    }
+    DEME_HIP(c, e);
+    if (cy.exec) {
+        (void)hipGraphExecDestroy(cy.exec);
+        cy.exec = nullptr;
+    }
+    e = hipGraphInstantiate(&cy.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        cy.exec = nullptr;
+        return c->fail_hip(e, "hipGraphInstantiate");
+    }
+    cy.gen = c->graphGen;
+    cy.f = f;
+    cy.kp = c->kp;
+    cy.guard = g;
+    cy.ptrs = cycle_fingerprint(c, p);
+    c->cycleCaptures++;
+    return DEME_OK;
+}
+
+// One detection interval of f steps from its captured graph (capturing it first when needed). *done is false when
+// the replay's pass failed or outgrew the buffers: then nothing moved and the caller runs the interval directly.
+static int run_cycle(deme_ctx* c, int f, bool* done) {
+    *done = false;
+    const int p = c->cur ^ 1;
+    deme_ctx::Cycle& cy = c->cycle[p];
+    const bool valid = cy.exec && cy.gen == c->graphGen && cy.f == f &&
+                       std::memcmp(&cy.kp, &c->kp, sizeof(KParams)) == 0 && cy.ptrs == cycle_fingerprint(c, p);
+    if (!valid) {
+        int r = capture_cycle(c, p, f);
+        if (r) return r;
+    }
+    DEME_LAUNCH(c, "set_clock", launch_set_clock(c->clock.p, c->time, c->stream));
+    DEME_HIP(c, hipGraphLaunch(cy.exec, c->stream));
+    DEME_HIP(c, hipMemcpyAsync(c->hctr, c->ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+    DEME_HIP(c, hipStreamSynchronize(c->stream));
+    const DevCounters& h = *c->hctr;
+    const SpecGuard& g = cy.guard;
+    if (h.err || h.maxOcc > g.slabCap || h.total[1] > g.capC || h.total[2] > g.capP) {
+        c->cycleFallbacks++;
+        return DEME_OK;
+    }
+    // the bookkeeping of a detection pass and f steps
+    c->cur = p;
+    c->havePrev = true;
+    c->Cprev = c->C;
+    c->C = h.total[1];
+    c->nPairs = h.total[2];
+    c->nGeo = c->C - c->nPairs;
+    c->contactsReady = true;
+    c->needDetect = false;
+    c->fresh = false;
+    c->forcesFull = false;
+    c->termsReady = true;
+    c->cdStage = 0;
+    for (int k = 0; k < f; k++) {
+        c->time += (double)c->sp.h;
+        c->nSteps++;
+    }
+    c->cycleReplays++;
+    *done = true;
+    return DEME_OK;
+}
+
 int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     int r = check_ready(c);
@@ -1305,6 +1527,17 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
     const uint64_t f = updateFreq > 0 ? (uint64_t)updateFreq : 1;
     for (int k = 0; k < nSteps; k++) {
         const bool detect = c->nSteps % f == 0 || !c->contactsReady || c->needDetect;
+        // a whole interval from its graph: due for detection, a current list to carry the history over, all f steps
+        // within this call, and no per-contact records wanted on its last step
+        if (detect && c->useGraphs && c->contactsReady && !c->needDetect && !c->timing && !c->jitStale && c->stream &&
+            c->nS > 0 && (uint64_t)(nSteps - k) >= f && f <= 1000 && !(c->keepRecords && (uint64_t)(nSteps - k) == f)) {
+            bool done = false;
+            if ((r = run_cycle(c, (int)f, &done))) return r;
+            if (done) {
+                k += (int)f - 1;
+                continue;
+            }
+        }
         if (detect) {
             r = deme_cd_run(c);
             if (r) return r;
@@ -1336,6 +1569,32 @@ int deme_set_keep_contact_records(deme_ctx* c, int on) {
 int deme_set_step_graphs(deme_ctx* c, int on) {
     c->useGraphs = on != 0;
     c->graphGen++;
+    return DEME_OK;
+}
+
+int deme_set_graph_headroom(deme_ctx* c, double factor, uint64_t slack) {
+    if (!(factor >= 1.0) || !(factor <= 16.0))
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, "graph headroom factor must lie in [1, 16]");
+    c->cycleHeadroom = factor;
+    c->cycleSlack = slack;
+    c->graphGen++;
+    return DEME_OK;
+}
+
+int deme_set_pair_sweep(deme_ctx* c, int mode, int block) {
+    if (mode < -1 || mode > 2) return c->fail(DEME_ERR_INVALID_ARGUMENT, "pair sweep mode must be -1, 0, 1 or 2");
+    if (block != 0 && block != 64 && block != 128 && block != 256)
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, "pair sweep workgroup size must be 64, 128 or 256 (0: keep)");
+    c->pairMode = mode;
+    if (block) c->pairBlock = (uint32_t)block;
+    c->graphGen++;
+    return DEME_OK;
+}
+
+int deme_graph_stats(deme_ctx* c, uint64_t out[3]) {
+    out[0] = c->cycleCaptures;
+    out[1] = c->cycleReplays;
+    out[2] = c->cycleFallbacks;
     return DEME_OK;
 }
 
@@ -1411,6 +1670,52 @@ int deme_autotune(deme_ctx* c, const int* freqs, int nFreqs, const double* cellS
     deme_set_cell_scale(c, bs);
     c->contactsReady = false;
     *bestFreq = bf;
+    return DEME_OK;
+}
+
+// Trial runs of the pair sweep's launch configurations (mode x workgroup size) on the live system at update interval
+// f, with the context's drift margin and cell scale; each trial runs trialSteps steps (rounded up to a multiple of
+// f, so every trial holds the same number of detection passes). The fastest stays configured.
+int deme_autotune_sweep(deme_ctx* c, const int* modes, int nModes, const int* blocks, int nBlocks, int trialSteps,
+                        int updateFreq, int* bestMode, int* bestBlock, double* msPerStep) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (nModes <= 0 || nBlocks <= 0 || trialSteps <= 0 || !modes || !blocks)
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, "need candidates and trial steps");
+    const int f = std::max(updateFreq, 1);
+    const int n = ((trialSteps + f - 1) / f) * f;
+    hipEvent_t e0, e1;
+    DEME_HIP(c, hipEventCreate(&e0));
+    DEME_HIP(c, hipEventCreate(&e1));
+    double best = 1e300;
+    int bm = c->pairMode, bb = (int)c->pairBlock;
+    for (int i = 0; i < nModes && !r; i++) {
+        for (int j = 0; j < nBlocks; j++) {
+            if ((r = deme_set_pair_sweep(c, modes[i], blocks[j]))) break;
+            // a pass with this configuration first (buffers sized, code loaded), then the timed steps
+            if ((r = deme_step(c, f, f))) break;
+            DEME_HIP(c, hipEventRecord(e0, c->stream));
+            if ((r = deme_step(c, n, f))) break;
+            DEME_HIP(c, hipEventRecord(e1, c->stream));
+            DEME_HIP(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            DEME_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+            const double per = (double)ms / n;
+            if (msPerStep) msPerStep[i * nBlocks + j] = per;
+            if (per < best) {
+                best = per;
+                bm = modes[i];
+                bb = blocks[j];
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r) return r;
+    r = deme_set_pair_sweep(c, bm, bb);
+    if (r) return r;
+    if (bestMode) *bestMode = bm;
+    if (bestBlock) *bestBlock = bb;
     return DEME_OK;
 }
 

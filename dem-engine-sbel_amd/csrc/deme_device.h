@@ -208,4 +208,29 @@ __device__ __forceinline__ CdRec load_cd(const CdRec* __restrict__ p, uint32_t i
     return r;
 }
 
+// Device-side counters read back by the host after a detection pass.
+struct DevCounters {
+    uint32_t pairCount;
+    uint32_t err;
+    uint32_t maxOcc;
+    uint32_t pad;
+    unsigned long long nActive;
+    unsigned long long total[4];
+};
+
+enum : uint32_t { ERRBIT_CAPACITY = 1u, ERRBIT_PAIR_OVERFLOW = 2u, ERRBIT_OUTSIDE = 4u };
+
+// Kernels enqueued before the host has read a detection pass's counters (so the GPU does not wait for the host, or
+// the whole step sits in a captured graph) exit at once, writing nothing, when the pass failed, a slab row
+// overflowed, or the contact / mirror-row totals exceed the buffers they were given; the host then redoes them.
+// Kernels over contacts then take the contact count from ctr->total[1]. ctr == nullptr: no guard.
+struct SpecGuard {
+    const DevCounters* ctr = nullptr;
+    uint32_t slabCap = 0;
+    unsigned long long capC = 0, capP = 0;
+};
+__device__ __forceinline__ bool spec_skip(const SpecGuard& g) {
+    return g.ctr && (g.ctr->err || g.ctr->maxOcc > g.slabCap || g.ctr->total[1] > g.capC || g.ctr->total[2] > g.capP);
+}
+
 }  // namespace deme_dev

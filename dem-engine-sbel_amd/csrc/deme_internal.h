@@ -12,6 +12,12 @@
 namespace deme_int {
 
 using deme_dev::CdRec;
+using deme_dev::DevCounters;
+using deme_dev::SpecGuard;
+using deme_dev::spec_skip;
+using deme_dev::ERRBIT_CAPACITY;
+using deme_dev::ERRBIT_PAIR_OVERFLOW;
+using deme_dev::ERRBIT_OUTSIDE;
 using deme_dev::ForceRec;
 using deme_dev::KParams;
 using deme_dev::OwnerRec;
@@ -32,30 +38,6 @@ struct CellGrid {
     double inv;  // 1 / cell edge
     uint32_t nx, ny, nz, nCells;
 };
-
-// Device-side counters read back by the host after a detection pass.
-struct DevCounters {
-    uint32_t pairCount;
-    uint32_t err;
-    uint32_t maxOcc;
-    uint32_t pad;
-    unsigned long long nActive;
-    unsigned long long total[4];
-};
-
-enum : uint32_t { ERRBIT_CAPACITY = 1u, ERRBIT_PAIR_OVERFLOW = 2u, ERRBIT_OUTSIDE = 4u };
-
-// Kernels enqueued before the host has read a detection pass's counters (so the GPU does not wait for the host)
-// exit at once, writing nothing, when the pass failed, a slab row overflowed, or the contact / mirror-row totals
-// exceed the buffers they were given; the host then redoes them. ctr == nullptr: no guard.
-struct SpecGuard {
-    const DevCounters* ctr = nullptr;
-    uint32_t slabCap = 0;
-    unsigned long long capC = 0, capP = 0;
-};
-__device__ __forceinline__ bool spec_skip(const SpecGuard& g) {
-    return g.ctr && (g.ctr->err || g.ctr->maxOcc > g.slabCap || g.ctr->total[1] > g.capC || g.ctr->total[2] > g.capP);
-}
 
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 16;
@@ -141,11 +123,14 @@ hipError_t launch_cell_scatter(const KParams& P, const SphereRec* sph, const Own
                                float beta, float4* sortedF, hipStream_t st);
 // Slabs are column-major by sorted position (entry k of sorted sphere i at slab[k * nS + i]). surv: prefilter
 // survivors; rowSlab / aCnt: higher-id partners (contact row); mirSlab / bCnt: lower-id partners (mirror row).
-// sortedF: float copies of the sorted records (position, expanded radius) for a pre-reject, or nullptr (none);
-// margin: bound on the float rounding of a pair's distance, so the float test only rejects pairs the exact test
-// would reject.
+// sortedF: float copies of the sorted records (position, expanded radius); margin: bound on the float rounding of a
+// pair's distance, so the float test only rejects pairs the exact test would reject. mode (sortedF needed for 1, 2):
+// 0 the exact test on every candidate; 1 a float pre-reject, the exact test on its survivors in the sweep; 2 the float
+// test alone in the sweep, the same-owner rule and the exact test in the block's shared second phase.
+// block: workgroup size, 64, 128 or 256 (hipErrorInvalidValue otherwise).
 hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
-                             const float4* sortedF, float margin, const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
+                             const float4* sortedF, float margin, int mode, uint32_t block, const uint2* sortedMeta,
+                             const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
                              uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
                              DevCounters* ctr, hipStream_t st);
 hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
@@ -161,14 +146,16 @@ hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, cons
                         const PlanePose* pose, const deme_material* mats, const float4* histIn, const uint32_t* map,
                         float4* histOut, const uint32_t* mirror, float* aTerm, float* bTerm, float* gTerm,
                         uint8_t* aOn, uint8_t* bOn,
-                        ForceRec* out, hipStream_t st);
+                        ForceRec* out, hipStream_t st, SpecGuard guard = SpecGuard());
 // blockPartials holds partialCap floats; a launch that would need more (anal_blocks(nC) * nAnalOwners * 6) returns
-// hipErrorInvalidValue and writes nothing
+// hipErrorInvalidValue and writes nothing. With a guard (a step captured with its detection pass) nC is the capacity
+// the grid covers and the kernels take the count from the pass's counters.
 hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const float* gTerm,
                                const deme_anal_obj* anal, uint32_t nA, uint32_t firstAnalOwner, uint32_t nAnalOwners,
-                               float* blockPartials, uint64_t partialCap, uint32_t* nBlocksOut, hipStream_t st);
+                               float* blockPartials, uint64_t partialCap, uint32_t* nBlocksOut, hipStream_t st,
+                               SpecGuard guard = SpecGuard());
 hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint32_t nAnalOwners, float* analAcc,
-                             hipStream_t st);
+                             hipStream_t st, SpecGuard guard = SpecGuard());
 hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, uint32_t firstAnalOwner,
                                uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
                                const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
@@ -176,9 +163,12 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                const float* analAcc, const deme_family_rule* rules, OwnerRec* own,
                                float* acc /* 6 x nO SoA: aX aY aZ alX alY alZ */, int doIntegrate,
                                const double* clock /* device: the simulation time the motion rules read */,
-                               hipStream_t st, OwnerRec* ownOut = nullptr /* integrated records; null: in place */);
+                               hipStream_t st, OwnerRec* ownOut = nullptr /* integrated records; null: in place */,
+                               SpecGuard guard = SpecGuard());
 // clock[0] = t (one lane; the time the owner update's motion rules read, written on the stream before the update)
 hipError_t launch_set_clock(double* clock, double t, hipStream_t st);
+// clock[0] += (double)h: the time step the host adds after each step, done on the device inside a captured step
+hipError_t launch_advance_clock(double* clock, float h, hipStream_t st);
 // p[0..n) = v, and p2[0..n2) = v in the same launch (n2 <= 256 * blocks: small second ranges only)
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st, uint32_t* p2 = nullptr,
                            uint32_t n2 = 0);
@@ -195,7 +185,7 @@ struct JitForce {
                       const deme_anal_obj* anal, const PlanePose* pose, const deme_material* mats,
                       const float4* histIn, const uint32_t* map, float4* histOut, const uint32_t* mirror,
                       float* aTerm, float* bTerm, float* gTerm, uint8_t* aOn, uint8_t* bOn, ForceRec* out,
-                      hipStream_t st) const;
+                      hipStream_t st, SpecGuard guard = SpecGuard()) const;
 };
 // the owner update kernel specialised with motion strategies / family change rules
 struct JitOwner {
@@ -208,7 +198,7 @@ struct JitOwner {
                       const float* aTerm, const float* bTerm, const uint8_t* aOn, const uint8_t* bOn,
                       const float* analAcc, const deme_family_rule* rules,
                       const OwnerRec* own, OwnerRec* ownOut, float* acc, int doIntegrate, const double* clock,
-                      hipStream_t st) const;
+                      hipStream_t st, SpecGuard guard = SpecGuard()) const;
 };
 std::string jit_owner_source(const std::vector<std::pair<std::string, std::string>>& values);
 

@@ -7,6 +7,8 @@
 //
 // Every float/double promotion of the reference's expressions is kept (e.g. `2. * E_cnt * sqrt_Rd` is evaluated in
 // double and rounded to float), and the device code is compiled without contraction.
+#include <algorithm>
+
 #include "deme_internal.h"
 
 // Rows a14-a15, the contact force kernel: generated at build time from jit/force_kernel.tpl with the default
@@ -31,8 +33,15 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
                                                       const float* __restrict__ gTerm,
                                                       const deme_anal_obj* __restrict__ anal, uint32_t nA,
                                                       uint32_t firstAnal, uint32_t nAnalOwners,
-                                                      float* __restrict__ partials) {
+                                                      float* __restrict__ partials, SpecGuard guard) {
     __shared__ float wsum[4][6];
+    if (guard.ctr) {
+        // captured with its detection pass: the grid covers the capacity; tiles past the pass's count write nothing
+        // (the final pass reads only the tiles of the count)
+        if (spec_skip(guard)) return;
+        nC = (uint32_t)guard.ctr->total[1];
+        if ((uint64_t)blockIdx.x * ANAL_TILE >= nC && blockIdx.x > 0) return;
+    }
     float acc[MAX_ANAL_OWNERS][6];
 #pragma unroll
     for (uint32_t k = 0; k < MAX_ANAL_OWNERS; k++)
@@ -103,8 +112,14 @@ __global__ __launch_bounds__(256) void k_anal_collect(uint32_t nC, const uint32_
 
 // One workgroup per (owner, component): lane t sums tiles t, t + 256, ... in order, then a fixed LDS tree.
 __global__ __launch_bounds__(256) void k_anal_final(const float* __restrict__ partials, uint32_t nBlocks,
-                                                    uint32_t nAnalOwners, float* __restrict__ analAcc) {
+                                                    uint32_t nAnalOwners, float* __restrict__ analAcc,
+                                                    SpecGuard guard) {
     __shared__ float red[256];
+    if (guard.ctr) {
+        if (spec_skip(guard)) return;
+        const uint64_t nb = (guard.ctr->total[1] + ANAL_TILE - 1) / ANAL_TILE;
+        nBlocks = nb > 0 ? (uint32_t)nb : 1u;  // an empty list still has its one (zero) tile
+    }
     const uint32_t v = blockIdx.x, t = threadIdx.x, stride = nAnalOwners * 6;
     float s = 0.f;
     for (uint32_t b = t; b < nBlocks; b += 256) s += partials[(uint64_t)b * stride + v];
@@ -122,6 +137,10 @@ __global__ void k_set_clock(double* clock, double t) {
     if (threadIdx.x == 0) clock[0] = t;
 }
 
+__global__ void k_advance_clock(double* clock, float h) {
+    if (threadIdx.x == 0) clock[0] += (double)h;
+}
+
 // ChangeFamily (APIPublic.cpp:184-207): every owner of family `from` moves to family `to`.
 __global__ __launch_bounds__(256) void k_change_family(OwnerRec* own, uint32_t nO, uint8_t from, uint8_t to) {
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,36 +155,40 @@ hipError_t launch_force(const KParams& P, uint32_t nC, const uint32_t* idA, cons
                         const SphereRec* sph, const OwnerRec* own, const float4* moi, const deme_anal_obj* anal,
                         const PlanePose* pose, const deme_material* mats, const float4* histIn, const uint32_t* map,
                         float4* histOut, const uint32_t* mirror, float* aTerm, float* bTerm, float* gTerm,
-                        uint8_t* aOn, uint8_t* bOn, ForceRec* out, hipStream_t st) {
+                        uint8_t* aOn, uint8_t* bOn, ForceRec* out, hipStream_t st, SpecGuard guard) {
     if (nC == 0) return hipSuccess;
     if (out)
         deme_force_full<<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats, histIn,
-                                                       map, histOut, mirror, aTerm, bTerm, gTerm, aOn, bOn, out);
+                                                       map, histOut, mirror, aTerm, bTerm, gTerm, aOn, bOn, out,
+                                                       guard);
     else
         deme_force_terms<<<blocks(nC, 256), 256, 0, st>>>(P, nC, idA, idB, ctype, sph, own, moi, anal, pose, mats,
                                                         histIn, map, histOut, mirror, aTerm, bTerm, gTerm, aOn, bOn,
-                                                        nullptr);
+                                                        nullptr, guard);
     return hipGetLastError();
 }
 
 hipError_t launch_anal_collect(uint32_t nC, const uint32_t* idB, const uint8_t* ctype, const float* gTerm,
                                const deme_anal_obj* anal, uint32_t nA, uint32_t firstAnalOwner, uint32_t nAnalOwners,
-                               float* blockPartials, uint64_t partialCap, uint32_t* nBlocksOut, hipStream_t st) {
-    const uint32_t nb = anal_blocks(nC);
+                               float* blockPartials, uint64_t partialCap, uint32_t* nBlocksOut, hipStream_t st,
+                               SpecGuard guard) {
+    // with a guard the list may come out empty: one tile still writes its zero partials for the final pass
+    const uint32_t nb = guard.ctr ? std::max(anal_blocks(nC), 1u) : anal_blocks(nC);
     *nBlocksOut = nb;
     if (nb == 0 || nAnalOwners == 0) return hipSuccess;
     if (nAnalOwners > MAX_ANAL_OWNERS) return hipErrorInvalidValue;
     // every workgroup writes its tile's nAnalOwners x 6 partials, plane contacts or not
     if ((uint64_t)nb * nAnalOwners * 6 > partialCap) return hipErrorInvalidValue;
-    k_anal_collect<<<nb, 256, 0, st>>>(nC, idB, ctype, gTerm, anal, nA, firstAnalOwner, nAnalOwners, blockPartials);
+    k_anal_collect<<<nb, 256, 0, st>>>(nC, idB, ctype, gTerm, anal, nA, firstAnalOwner, nAnalOwners, blockPartials,
+                                       guard);
     return hipGetLastError();
 }
 
 hipError_t launch_anal_final(const float* blockPartials, uint32_t nBlocks, uint32_t nAnalOwners, float* analAcc,
-                             hipStream_t st) {
+                             hipStream_t st, SpecGuard guard) {
     if (nAnalOwners == 0) return hipSuccess;
     if (nAnalOwners > MAX_ANAL_OWNERS) return hipErrorInvalidValue;
-    k_anal_final<<<nAnalOwners * 6, 256, 0, st>>>(blockPartials, nBlocks, nAnalOwners, analAcc);
+    k_anal_final<<<nAnalOwners * 6, 256, 0, st>>>(blockPartials, nBlocks, nAnalOwners, analAcc, guard);
     return hipGetLastError();
 }
 
@@ -173,17 +196,23 @@ hipError_t launch_owner_update(const KParams& P, uint32_t nO, uint32_t nClumps, 
                                uint32_t nAnalOwners, const uint32_t* ownerSphStart, const uint32_t* rowStart,
                                const uint32_t* bRowStart, const float* aTerm, const float* bTerm,
                                const uint8_t* aOn, const uint8_t* bOn, const float* analAcc, const deme_family_rule* rules, OwnerRec* own, float* acc,
-                               int doIntegrate, const double* clock, hipStream_t st, OwnerRec* ownOut) {
+                               int doIntegrate, const double* clock, hipStream_t st, OwnerRec* ownOut,
+                               SpecGuard guard) {
     if (nO == 0) return hipSuccess;
     deme_owner_update<<<blocks(nO, 256), 256, 0, st>>>(P, nO, nClumps, firstAnalOwner, nAnalOwners, ownerSphStart,
                                                     rowStart, bRowStart, aTerm, bTerm, aOn, bOn, analAcc, rules,
                                                     own,
-                                                    ownOut ? ownOut : own, acc, doIntegrate, clock);
+                                                    ownOut ? ownOut : own, acc, doIntegrate, clock, guard);
     return hipGetLastError();
 }
 
 hipError_t launch_set_clock(double* clock, double t, hipStream_t st) {
     k_set_clock<<<1, 64, 0, st>>>(clock, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_clock(double* clock, float h, hipStream_t st) {
+    k_advance_clock<<<1, 64, 0, st>>>(clock, h);
     return hipGetLastError();
 }
 

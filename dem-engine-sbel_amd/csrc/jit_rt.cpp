@@ -142,11 +142,11 @@ hipError_t JitForce::launch(bool wantFull, const KParams& P, uint32_t nC, const 
                             const deme_anal_obj* anal, const PlanePose* pose, const deme_material* mats,
                             const float4* histIn, const uint32_t* map, float4* histOut, const uint32_t* mirror,
                             float* aTerm, float* bTerm, float* gTerm, uint8_t* aOn, uint8_t* bOn, ForceRec* out,
-                            hipStream_t st) const {
+                            hipStream_t st, SpecGuard guard) const {
     if (nC == 0) return hipSuccess;
     KParams p = P;
     void* args[] = {&p,    &nC,     &idA, &idB,     &ctype,  &sph,   &own,   &moi,   &anal, &pose, &mats,
-                    &histIn, &map, &histOut, &mirror, &aTerm, &bTerm, &gTerm, &aOn, &bOn, &out};
+                    &histIn, &map, &histOut, &mirror, &aTerm, &bTerm, &gTerm, &aOn, &bOn, &out, &guard};
     const uint32_t grid = (nC + 255) / 256;
     return hipModuleLaunchKernel(wantFull ? full : terms, grid, 1, 1, 256, 1, 1, 0, st, args, nullptr);
 }
@@ -172,12 +172,12 @@ hipError_t JitOwner::launch(const KParams& P, uint32_t nO, uint32_t nClumps, uin
                             const float* aTerm, const float* bTerm, const uint8_t* aOn, const uint8_t* bOn,
                             const float* analAcc, const deme_family_rule* rules, const OwnerRec* own,
                             OwnerRec* ownOut, float* acc, int doIntegrate, const double* clock,
-                            hipStream_t st) const {
+                            hipStream_t st, SpecGuard guard) const {
     if (nO == 0) return hipSuccess;
     KParams p = P;
     void* args[] = {&p,     &nO,   &nClumps, &firstAnal, &nAnalOwners, &ownerSphStart, &rowStart, &bRowStart,
                     &aTerm, &bTerm, &aOn,    &bOn,       &analAcc,     &rules,         &own,      &ownOut,
-                    &acc,   &doIntegrate, &clock};
+                    &acc,   &doIntegrate, &clock, &guard};
     return hipModuleLaunchKernel(fn, (nO + 255) / 256, 1, 1, 256, 1, 1, 0, st, args, nullptr);
 }
 

@@ -89,7 +89,8 @@ EXPORTED = [
     "deme_set_keep_contact_records", "deme_set_owner_volumes", "deme_inspect", "deme_append_clumps",
     "deme_change_clump_sizes",
     "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner", "deme_jit_render_inspect",
-    "deme_prefix_scan_u32", "deme_integrate_owners",
+    "deme_prefix_scan_u32", "deme_integrate_owners", "deme_set_graph_headroom",
+    "deme_graph_stats", "deme_set_pair_sweep", "deme_autotune_sweep",
 ]
 
 _lib = None
@@ -135,6 +136,10 @@ def load(path: str = LIB_PATH):
         "deme_download_bins": ([vp, vp, vp], i32),
         "deme_download_contacts": ([vp, vp, vp, vp, vp], i32),
         "deme_prefix_scan_u32": ([vp, vp, vp, u64, C.POINTER(u64)], i32),
+        "deme_set_graph_headroom": ([vp, C.c_double, u64], i32),
+        "deme_graph_stats": ([vp, vp], i32),
+        "deme_set_pair_sweep": ([vp, i32, i32], i32),
+        "deme_autotune_sweep": ([vp, vp, i32, vp, i32, i32, i32, C.POINTER(i32), C.POINTER(i32), vp], i32),
         "deme_force": ([vp], i32), "deme_reduce_owner": ([vp], i32), "deme_integrate": ([vp], i32),
         "deme_integrate_owners": ([vp], i32),
         "deme_step": ([vp, i32, i32], i32),
@@ -369,8 +374,35 @@ class DeviceSystem:
         self._chk(self.lib.deme_change_family(self.ctx, int(src), int(dst)))
 
     def set_step_graphs(self, on=True):
-        """Replay the steps between detection passes from a captured HIP graph, or launch them directly (default)."""
+        """Replay whole detection intervals (and the steps outside them) from captured HIP graphs, or launch every
+        kernel directly (default)."""
         self._chk(self.lib.deme_set_step_graphs(self.ctx, 1 if on else 0))
+
+    def set_graph_headroom(self, factor, slack):
+        self._chk(self.lib.deme_set_graph_headroom(self.ctx, float(factor), int(slack)))
+
+    def set_pair_sweep(self, mode, block=0):
+        """Pair sweep of the cell-list detection: mode 0 exact, 1 float pre-reject, 2 float sweep, -1 automatic;
+        block: workgroup size 64, 128 or 256 (0 keeps it)."""
+        self._chk(self.lib.deme_set_pair_sweep(self.ctx, int(mode), int(block)))
+
+    def autotune_sweep(self, modes=(0, 1, 2), blocks=(64, 128, 256), trial_steps=10, update_freq=1):
+        """Times the pair sweep's configurations on the live system and keeps the fastest; returns
+        (mode, block, table of ms per step, modes x blocks)."""
+        m = np.asarray(modes, np.int32)
+        b = np.asarray(blocks, np.int32)
+        table = np.zeros((len(m), len(b)), np.float64)
+        bm, bb = C.c_int32(), C.c_int32()
+        self._chk(self.lib.deme_autotune_sweep(self.ctx, C.c_void_p(m.ctypes.data), len(m), C.c_void_p(b.ctypes.data),
+                                               len(b), int(trial_steps), int(update_freq), C.byref(bm), C.byref(bb),
+                                               C.c_void_p(table.ctypes.data)))
+        return bm.value, bb.value, table
+
+    def graph_stats(self):
+        """(captures, replays that advanced the system, replays that fell back to direct launches)"""
+        out = np.zeros(3, np.uint64)
+        self._chk(self.lib.deme_graph_stats(self.ctx, C.c_void_p(out.ctypes.data)))
+        return tuple(int(x) for x in out)
 
     def set_owner_volumes(self, volume):
         v = np.ascontiguousarray(volume, dtype=np.float32)

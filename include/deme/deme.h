@@ -126,11 +126,27 @@ int deme_integrate_owners(deme_ctx* ctx);
 /* The step loop (DEMDynamicThread::workerThread, dT.cpp:1527-1585) with a deterministic contact-update schedule:
  * detection on the state at steps k*f (f = max(update_freq, 1)), force -> collect -> integrate each step. */
 int deme_step(deme_ctx* ctx, int n_steps, int update_freq);
-/* Between detection passes deme_step replays each step (plane poses, force, collection, integration) from a HIP graph
- * captured once per pass (no counterpart in the reference, which launches every kernel). Off by default (direct
- * launches): the capture per detection pass costs more than the replays save unless passes are far apart. Results
- * are identical either way. Kernel timing (deme_enable_kernel_timing) runs directly. */
+/* HIP-graph replay in deme_step (no counterpart in the reference, which launches every kernel and waits on the host
+ * for each detection pass's counts). When on, each whole detection interval -- the detection pass and the update_freq
+ * steps up to the next one -- is one graph launch, captured once per list parity and replayed while the buffers it
+ * baked in stay valid: the kernels after the pass take the contact counts from device memory and skip themselves when
+ * the pass failed or its lists outgrew the captured capacity, in which case the replay changes nothing and the
+ * interval runs directly (regrowing the buffers) before the next capture. Steps outside whole intervals replay a
+ * per-step graph. Off by default (direct launches). Results are identical either way. Kernel timing
+ * (deme_enable_kernel_timing) runs directly. */
 int deme_set_step_graphs(deme_ctx* ctx, int on);
+/* Capacity a captured interval gets over the current list: factor x the contact (and mirror-row) count + slack
+ * (defaults 1.25 and 1024). */
+int deme_set_graph_headroom(deme_ctx* ctx, double factor, uint64_t slack);
+/* How the pair sweep of the cell-list detection tests its candidates, and its workgroup size (no counterpart in the
+ * reference; the contact lists are the same in every configuration): mode 0 the double-precision overlap test on
+ * every candidate; 1 a float pre-reject (rounding margin included) and the exact test on its survivors; 2 the float
+ * test alone in the sweep, the same-owner rule and the exact test in the block's shared second phase; -1 (default)
+ * mode 1 when the largest radius exceeds 1.5 x the mean, else 0. block: 64, 128 (default) or 256; 0 keeps it. */
+int deme_set_pair_sweep(deme_ctx* ctx, int mode, int block);
+/* Counters of the interval graphs since the context was created: out[0] captures, out[1] replays that advanced the
+ * system, out[2] replays that fell back to direct launches. */
+int deme_graph_stats(deme_ctx* ctx, uint64_t out[3]);
 /* When on, the last step of every deme_step call also stores the per-contact records that deme_download_forces
  * returns (force, torque-only force, contact points), as the reference's force kernel stores contactForces,
  * contactTorque_convToForce and contactPointGeometryA/B each step (DEMCalcForceKernels.cu:160-175) for its contact
@@ -149,6 +165,13 @@ int deme_set_cell_scale(deme_ctx* ctx, double scale);
  * advance the simulation. */
 int deme_autotune(deme_ctx* ctx, const int* freqs, int nFreqs, const double* cellScales, int nScales, int trialSteps,
                   float maxVel, float safety, int* bestFreq, double* msPerStep);
+/* Second stage of the autotuner (no counterpart in the reference): times trialSteps steps at update interval
+ * updateFreq (keeping the drift margin and cell scale in force) for every pair sweep configuration modes[i] x
+ * blocks[j] (deme_set_pair_sweep), keeps the fastest configured and returns it. msPerStep (optional, nModes x
+ * nBlocks) receives the timings. The trials advance the simulation; contact lists are the same in every
+ * configuration. */
+int deme_autotune_sweep(deme_ctx* ctx, const int* modes, int nModes, const int* blocks, int nBlocks, int trialSteps,
+                        int updateFreq, int* bestMode, int* bestBlock, double* msPerStep);
 
 /* ---- Force model and runtime specialisation (hipRTC in place of jitify) ----
  * The force kernel is one template with the placeholders of the reference's JIT programs (DEMCalcForceKernels.cu:

@@ -1,6 +1,8 @@
-"""Steps between detection passes replayed from a captured HIP graph give the same states, bit for bit, as the same
-steps launched directly: several detection intervals (a capture per pass and replays between them, with the eager
-detection work in between), a batch of scenes, a drift margin, and re-captures after a change of parameters."""
+"""Steps replayed from captured HIP graphs give the same states, bit for bit, as the same steps launched directly:
+whole detection intervals (the detection pass and the steps up to the next, one graph launch each) with and without a
+drift margin, calls that start and end inside intervals (per-step graphs), a batch of scenes, re-captures after a
+change of parameters, and intervals whose lists outgrow the captured capacity (the replay changes nothing and the
+interval runs directly)."""
 import numpy as np
 import pytest
 
@@ -61,4 +63,37 @@ def test_graph_replay_in_a_batch():
     a, b = _pair(sc)
     a.step(17, 4)
     b.step(17, 4)
+    _same(a, b)
+
+
+@pytest.mark.parametrize("freq,beta", [(1, 0.0), (5, 2e-4)])
+def test_whole_interval_graphs_equal_direct_launches(freq, beta):
+    sc = scenes.sphere_settling(n=4000, spacing=2.02, velocity_sigma=0.2, omega_sigma=3.0, beta=beta, seed=1013)
+    a, b = _pair(sc)
+    for chunk in (freq * 6, 3, freq * 4 + 2):
+        a.step(chunk, freq)
+        b.step(chunk, freq)
+        _same(a, b)
+        da, db = a.contacts(), b.contacts()
+        for x, y in zip(da, db):
+            assert np.array_equal(x, y)
+    captures, replays, fallbacks = a.graph_stats()
+    assert replays >= 6 and captures >= 1
+    assert a.counts().nContacts == b.counts().nContacts > 500
+    assert a.counts().nSteps == b.counts().nSteps
+
+
+def test_whole_interval_graphs_fall_back_when_lists_outgrow_them():
+    # no headroom: every pass whose list grew outgrows the capture, runs directly and is captured again
+    sc = scenes.sphere_settling(n=4000, spacing=2.02, velocity_sigma=0.3, omega_sigma=3.0, beta=2e-4, seed=1014)
+    a, b = _pair(sc)
+    a.set_graph_headroom(1.0, 0)
+    a.step(60, 3)
+    b.step(60, 3)
+    _same(a, b)
+    captures, replays, fallbacks = a.graph_stats()
+    assert fallbacks >= 1 and replays >= 1 and captures >= fallbacks
+    a.set_graph_headroom(1.25, 1024)
+    a.step(30, 3)
+    b.step(30, 3)
     _same(a, b)
