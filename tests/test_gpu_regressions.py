@@ -106,19 +106,20 @@ def test_graph_replay_sees_the_current_time(freq):
 
 
 def _scan_case(lib, ctx, n, rng):
-    import torch
+    from devmem import DeviceArray
 
     x = rng.integers(0, 1 << 16, n, dtype=np.uint32) if n else np.zeros(0, np.uint32)
     if n > 3:
         x[rng.integers(0, n, 3)] = 0xFFFFFFFF  # large values: totals beyond 32 bits
-    din = torch.from_numpy(x.view(np.int32)).to("cuda")
-    dout = torch.full((n + 1,), -1, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
-    total = C.c_uint64(0)
-    st = lib.deme_prefix_scan_u32(ctx, C.c_void_p(din.data_ptr() if n else 0), C.c_void_p(dout.data_ptr()), n,
-                                  C.byref(total))
-    assert st == DEME_OK, lib.deme_last_error(ctx)
-    got = dout.cpu().numpy().view(np.uint32)
+    din, dout = DeviceArray.of(x), DeviceArray(4 * (n + 1))
+    try:
+        total = C.c_uint64(0)
+        st = lib.deme_prefix_scan_u32(ctx, din.ptr if n else None, dout.ptr, n, C.byref(total))
+        assert st == DEME_OK, lib.deme_last_error(ctx)
+        got = dout.get(np.uint32, n + 1)
+    finally:
+        din.close()
+        dout.close()
     ref = np.zeros(n + 1, np.uint64)
     ref[1:] = np.cumsum(x, dtype=np.uint64)
     assert total.value == int(ref[-1])
