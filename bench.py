@@ -214,6 +214,8 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05, cpu_
     f32 = np.float32
     beta = float(f32(vmax) * f32(sc.params.h) * f32(update_freq) * f32(1.2)) if update_freq > 1 else 0.0
     scale = (1.0, 1.25)[int(np.argmin(table[[1, 2, 5, 10].index(update_freq)]))]
+    # second stage: the pair sweep's mode and workgroup size at that interval
+    sweep_mode, sweep_block, sweep_table = dev.autotune_sweep(trial_steps=update_freq, update_freq=update_freq)
     dev.step(warmup, update_freq)
     dev.sync()
     t0 = time.perf_counter()
@@ -221,15 +223,19 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05, cpu_
     dev.sync()
     t_batch = time.perf_counter() - t0
     contacts = dev.counts().nContacts
-    # the same steps with the non-detection steps replayed from captured HIP graphs
+    # the same steps as whole detection intervals replayed from captured HIP graphs: aligned to an interval start,
+    # with both list parities' graphs captured before the timed intervals
     dev.set_step_graphs(True)
-    dev.step(warmup, update_freq)
+    align = (-dev.counts().nSteps) % update_freq
+    dev.step(align + 2 * update_freq, update_freq)
     dev.sync()
+    gsteps = -(-steps // update_freq) * update_freq
+    s0 = dev.graph_stats()
     t0 = time.perf_counter()
-    dev.step(steps, update_freq)
+    dev.step(gsteps, update_freq)
     dev.sync()
-    t_graph = time.perf_counter() - t0
-    graph_stats = dev.graph_stats()
+    t_graph = (time.perf_counter() - t0) * steps / gsteps
+    graph_stats = tuple(b - a for a, b in zip(s0, dev.graph_stats()))
     dev.set_step_graphs(False)
     state = dev.get_state() if cpu_threads > 0 else None
     dev.close()
@@ -237,6 +243,7 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05, cpu_
     for d in parts:
         d.set_beta(beta)
         d.set_cell_scale(scale)
+        d.set_pair_sweep(sweep_mode, sweep_block)
         d.step(warmup, update_freq)
     for d in parts:
         d.sync()
@@ -252,8 +259,10 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05, cpu_
             "update_freq": update_freq,
             "beta_m": beta, "cell_scale": scale, "autotune_ms_per_step": table.round(4).tolist(), "steps": steps,
             "scene_steps_per_s": n_scenes * steps / t_batch, "ms_per_step": 1e3 * t_batch / steps,
+            "pair_sweep_mode_block": [sweep_mode, sweep_block],
+            "pair_sweep_autotune_ms_per_step": sweep_table.round(4).tolist(),
             "graph_replay_ms_per_step": 1e3 * t_graph / steps,
-            "graph_captures_replays_fallbacks": list(graph_stats),
+            "graph_captures_replays_fallbacks_in_timed_steps": list(graph_stats),
             "loop_scene_steps_per_s": n_scenes * steps / t_loop, "loop_ms_per_step": 1e3 * t_loop / steps,
             "speedup_vs_loop": t_loop / t_batch,
             "cpu": (None if state is None else batch_cpu(sc, state, cpu_steps, cpu_threads))}
@@ -431,14 +440,23 @@ def main():
         sys.exit(2)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     if args.copy_probe_mib > 0:
-        import torch
+        # a device-to-device copy of known size, through the HIP runtime the library itself uses (not torch's: a
+        # process drives the device through one runtime)
+        import ctypes as C
 
-        n = args.copy_probe_mib << 18  # float32 elements
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        from deme_amd.abi import load
+
+        load()
+        hip = C.CDLL("libamdhip64.so.7")
+        n = C.c_size_t(args.copy_probe_mib << 20)
+        src, dst = C.c_void_p(), C.c_void_p()
+        ok = hip.hipMalloc(C.byref(src), n) == 0 and hip.hipMalloc(C.byref(dst), n) == 0
+        ok = ok and hip.hipMemset(src, 1, n) == 0 and hip.hipMemcpy(dst, src, n, 3) == 0  # 3: device to device
+        ok = ok and hip.hipDeviceSynchronize() == 0
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        if not ok:
+            raise RuntimeError("copy probe failed")
 
     threads = 0 if args.no_cpu else cpu_threads()
     note("headline: building the 1M-clump scene")

@@ -328,6 +328,9 @@ __global__ __launch_bounds__(256) void k_row_finalize(uint32_t nS, const uint32_
 // new and previous rows of the same sphere (both ordered by (type group, idB)). The same pass links every
 // sphere-sphere contact (s, b) to its slot in b's mirror row (the contacts where b is the B body, ordered by A id):
 // mirror[i] = bRowStart[b] + rank of s in that row.
+// Rows up to this long take the register path of the history map.
+constexpr uint32_t HM_REG = 8;
+
 __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t* __restrict__ rowStart,
                                                      const uint32_t* __restrict__ idB,
                                                      const uint8_t* __restrict__ ctype,
@@ -342,6 +345,49 @@ __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t
     const uint32_t ns = rowStart[s], ne = rowStart[s + 1];
     uint32_t j = prevRowStart ? prevRowStart[s] : 0;
     const uint32_t je = prevRowStart ? prevRowStart[s + 1] : 0;
+    if (ne - ns <= HM_REG && je - j <= HM_REG) {
+        // short rows (nearly all): both rows loaded at once and matched in registers. A row's keys (plane or sphere,
+        // partner id) are distinct, so the entry of the previous row with the same partner and type is the merge's.
+        const uint32_t nn = ne - ns, np = je - j;
+        uint32_t nb[HM_REG], pb[HM_REG];
+        uint8_t nt[HM_REG], pt[HM_REG];
+#pragma unroll
+        for (uint32_t k = 0; k < HM_REG; k++) {
+            nb[k] = k < nn ? idB[ns + k] : 0u;
+            nt[k] = k < nn ? ctype[ns + k] : (uint8_t)DEME_NOT_A_CONTACT;
+            pb[k] = k < np ? prevIdB[j + k] : 0u;
+            pt[k] = k < np ? prevType[j + k] : (uint8_t)0xFF;
+        }
+        // the partners' mirror-row bounds, all in flight
+        uint32_t lo[HM_REG], hi[HM_REG];
+#pragma unroll
+        for (uint32_t k = 0; k < HM_REG; k++) {
+            const bool ss = k < nn && nt[k] == DEME_SPHERE_SPHERE_CONTACT && nb[k] < nS;
+            lo[k] = ss ? bRowStart[nb[k]] : 0u;
+            hi[k] = ss ? bRowStart[nb[k] + 1] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < HM_REG; k++) {
+            if (k >= nn) break;
+            uint32_t m = DEME_NULL_MAPPING_PARTNER;
+#pragma unroll
+            for (uint32_t q = 0; q < HM_REG; q++)
+                if (q < np && pb[q] == nb[k] && pt[q] == nt[k]) m = j + q;
+            map[ns + k] = m;
+            uint32_t link = DEME_NULL_MAPPING_PARTNER;
+            for (uint32_t q = lo[k]; q < hi[k] && link == DEME_NULL_MAPPING_PARTNER; q += 4) {
+                uint32_t v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) v[u] = q + u < hi[k] ? bPart[q + u] : 0xFFFFFFFFu;
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (v[u] == s && link == DEME_NULL_MAPPING_PARTNER) link = q + (uint32_t)u;
+                if (v[3] > s) break;  // past s in an ascending row
+            }
+            mirror[ns + k] = link;
+        }
+        return;
+    }
     for (uint32_t i = ns; i < ne; i++) {
         const uint32_t t = ctype[i], b = idB[i];
         const uint32_t gi = t == DEME_SPHERE_SPHERE_CONTACT ? 1u : 0u;

@@ -314,7 +314,6 @@ __device__ __forceinline__ bool bin_rule_point(const KParams& P, double inv, con
 // by sorted position (entry k of sorted sphere i at slab[k * nS + i]), so the appends of a block hit neighbouring
 // addresses. Pairs beyond the queue's capacity are parked in the lane's column of `surv` and finished by that lane.
 // A column that overflows a slab is reported through maxOcc and the pass is redone with wider columns.
-constexpr uint32_t PAIR_BLOCK = 128;  // default workgroup size (64, 128 and 256 are built)
 constexpr uint32_t PAIR_QUEUE = 1024;
 static_assert(PAIR_QUEUE < (1u << 23), "queue codes hold a kind bit, an 8-bit lane and a 23-bit slot");
 // Queue code of an accepted pair in the second phase: kind (0: the finder's contact row, 1: its mirror row) in bit
@@ -561,8 +560,28 @@ __global__ __launch_bounds__(BLOCK) void k_cell_pairs(KParams P, CellGrid G, uin
 
 // Canonical rows: plane contacts (ascending plane index) then partners (ascending id); idA for the row. Mirror rows:
 // the sphere's lower-id partners, ascending. Both come from the sphere's slab columns (sorted position).
+// A column of up to SORT_REG entries is loaded at once (all loads in flight), sorted in registers by an unrolled
+// insertion network and stored; longer columns (rare) are sorted in place.
+constexpr uint32_t SORT_REG = 8;
 __device__ __forceinline__ void sorted_copy(const uint32_t* __restrict__ src, uint32_t n, uint32_t nS,
                                             uint32_t* __restrict__ dst) {
+    if (n <= SORT_REG) {
+        uint32_t v[SORT_REG];
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_REG; k++) v[k] = k < n ? src[(uint64_t)k * nS] : 0xFFFFFFFFu;
+#pragma unroll
+        for (uint32_t i = 1; i < SORT_REG; i++)
+#pragma unroll
+            for (uint32_t j = i; j > 0; j--) {
+                const uint32_t lo = min(v[j - 1], v[j]), hi = max(v[j - 1], v[j]);
+                v[j - 1] = lo;
+                v[j] = hi;
+            }
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_REG; k++)
+            if (k < n) dst[k] = v[k];
+        return;
+    }
     for (uint32_t k = 0; k < n; k++) {
         const uint32_t v = src[(uint64_t)k * nS];
         uint32_t q = k;

@@ -111,7 +111,7 @@ struct deme_ctx {
     } cycle[2];
     uint64_t cycleReplays = 0, cycleFallbacks = 0, cycleCaptures = 0;
     int pairMode = -1;  // pair sweep mode (launch_cell_pairs); -1: chosen from the radius spread
-    uint32_t pairBlock = 128;  // pair sweep workgroup size
+    uint32_t pairBlock = 256;  // pair sweep workgroup size
     double cycleHeadroom = 1.25;  // capacity of a captured interval: headroom x the current counts + slack
     uint64_t cycleSlack = 1024;
     // force kernel specialised at run time (hipRTC); inactive: the ahead-of-time kernel of the built-in model
@@ -973,10 +973,9 @@ static int cd_buffers(deme_ctx* c, int* mode) {
     DEME_HIP(c, c->aCnt.ensure(nS));
     DEME_HIP(c, c->sortedMeta.ensure(nS));
     DEME_HIP(c, c->sorted.ensure(nS));
-    // the float pre-reject of the pair sweep pays where most candidates are far from touching: radii spread wide
-    // enough that the cells (sized for the largest) hold many small spheres. With near-uniform radii (drift margin
-    // or not) the extra float copy costs more than the record loads it saves. A mode set on the context wins.
-    *mode = c->pairMode >= 0 ? c->pairMode : (c->maxRadius > 1.5f * c->meanRadius ? 1 : 0);
+    // default: the float sweep with the exact tests in the shared phase (measured fastest on near-uniform clumps,
+    // with or without a drift margin, and on wide radius spreads); a mode set on the context wins
+    *mode = c->pairMode >= 0 ? c->pairMode : 2;
     if (*mode != 0) DEME_HIP(c, c->sortedF.ensure(nS));
     DEME_HIP(c, c->planeBits.ensure(nS));
     DEME_HIP(c, c->scanTmp.ensure(nS));

@@ -308,11 +308,13 @@ def fcc_neighbours(n, mx, my, mz):
 def relax_overlaps(x, r, nb, iters, step=0.4):
     """Pushes lattice neighbours apart: each iteration moves every sphere by step x half the overlap with each of its
     neighbours, along the line of centres (a Jacobi sweep over the fixed neighbour table; gathers only, so the result
-    does not depend on summation order). Runs on the GPU through torch when one is present."""
+    does not depend on summation order). Runs on the CPU through torch's threaded kernels when torch is present:
+    never on the GPU, since torch's wheel carries a HIP runtime of its own and a process must not drive the device
+    through two runtimes (the library's and torch's)."""
     try:
         import torch
 
-        dev = "cuda" if torch.cuda.is_available() else "cpu"
+        dev = "cpu"
         X = torch.from_numpy(np.ascontiguousarray(x, np.float64)).to(dev)
         R = torch.from_numpy(np.ascontiguousarray(r, np.float64)).to(dev)
         N = torch.from_numpy(nb).to(dev)
@@ -324,10 +326,8 @@ def relax_overlaps(x, r, nb, iters, step=0.4):
             o = R[:, None] + R[Nc] - dist
             w = torch.where(valid & (o > 0), step * 0.5 * o / dist.clamp_min(1e-12), torch.zeros_like(o))
             X = X + (d * w[..., None]).sum(1)
-        out = X.cpu().numpy()
+        out = X.numpy()
         del X, R, N, Nc
-        if dev == "cuda":
-            torch.cuda.empty_cache()  # give the device memory back before the library allocates
         return out
     except ImportError:
         X = np.array(x, np.float64)
@@ -349,9 +349,10 @@ def polydisperse_bed(n=8_000_000, seed=1003, median_r=0.001, sigma=0.2, r_clip=(
     packed at `solid_fraction` in an open-top box under a top plate (its own owner, a prescribed family moving down
     at plate_speed) one largest diameter above the bed. At 8M spheres the bed is 0.40 x 0.40 x 0.41 m.
 
-    Packing: the spheres take the sites of a face-centred cubic lattice (in random order, with a jitter of 2% of the
-    spacing) whose spacing gives the requested solid fraction; a few sweeps pushing lattice neighbours apart then
-    spread the overlaps of the larger spheres. Residual overlaps remain (about 1.5 overlapping pairs per sphere,
+    Packing: sphere i takes site i of a face-centred cubic lattice (sites numbered x fastest, then y, then z, as a grid
+    sampler emits them; radii drawn independently per site; a jitter of 2% of the spacing) whose spacing gives the
+    requested solid fraction; a few sweeps pushing lattice neighbours apart then spread the overlaps of the larger
+    spheres. Residual overlaps remain (about 1.5 overlapping pairs per sphere,
     mean depth ~5% of a radius); settle_bed() relaxes them on the device."""
     rng = np.random.default_rng(seed)
     radius = np.clip(median_r * np.exp(sigma * rng.standard_normal(n)), *r_clip).astype(np.float32)

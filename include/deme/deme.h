@@ -141,8 +141,8 @@ int deme_set_graph_headroom(deme_ctx* ctx, double factor, uint64_t slack);
 /* How the pair sweep of the cell-list detection tests its candidates, and its workgroup size (no counterpart in the
  * reference; the contact lists are the same in every configuration): mode 0 the double-precision overlap test on
  * every candidate; 1 a float pre-reject (rounding margin included) and the exact test on its survivors; 2 the float
- * test alone in the sweep, the same-owner rule and the exact test in the block's shared second phase; -1 (default)
- * mode 1 when the largest radius exceeds 1.5 x the mean, else 0. block: 64, 128 (default) or 256; 0 keeps it. */
+ * test alone in the sweep, the same-owner rule and the exact test in the block's shared second phase; -1 the default
+ * (mode 2). block: 64, 128 or 256 (default); 0 keeps it. */
 int deme_set_pair_sweep(deme_ctx* ctx, int mode, int block);
 /* Counters of the interval graphs since the context was created: out[0] captures, out[1] replays that advanced the
  * system, out[2] replays that fell back to direct launches. */

@@ -1039,6 +1039,22 @@ void ora_get_contact_forces(const ora_ctx* c, float* F, float* T, float* cA, flo
     }
 }
 
+// Per-contact records given from outside (the device's), so that the owner reduction can be compared on identical
+// inputs; the list must be the oracle's current one (n = its contact count).
+void ora_set_contact_forces(ora_ctx* c, const float* F, const float* T, const float* cA, const float* cB) {
+    const size_t n = c->idA.size();
+    c->F.resize(n);
+    c->Ftof.resize(n);
+    c->cpA.resize(n);
+    c->cpB.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        c->F[i] = {F[3 * i], F[3 * i + 1], F[3 * i + 2]};
+        c->Ftof[i] = {T[3 * i], T[3 * i + 1], T[3 * i + 2]};
+        c->cpA[i] = {cA[3 * i], cA[3 * i + 1], cA[3 * i + 2]};
+        c->cpB[i] = {cB[3 * i], cB[3 * i + 1], cB[3 * i + 2]};
+    }
+}
+
 void ora_get_wildcards(const ora_ctx* c, float* w) {
     const size_t n = c->idA.size();
     for (int k = 0; k < 4; k++)

@@ -67,6 +67,8 @@ void ora_get_contact_forces(const ora_ctx* c, float* force /*3C*/, float* torque
                             float* cpB /*3C*/);
 void ora_get_wildcards(const ora_ctx* c, float* w /* 4 x C, wildcard-major */);
 void ora_set_wildcards(ora_ctx* c, const float* w /* 4 x C */);
+void ora_set_contact_forces(ora_ctx* c, const float* force, const float* torqueOnly, const float* cpA,
+                            const float* cpB); /* 3C each, the current list's */
 void ora_get_time(const ora_ctx* c, double* t, uint64_t* nSteps);
 void ora_set_time(ora_ctx* c, double t);
 /* Wall seconds accumulated per phase by ora_step / ora_cd (Discretize domain, Find contact pairs, Build history map,

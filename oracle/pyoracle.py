@@ -51,6 +51,7 @@ def load():
     lib.ora_get_contact_forces.argtypes = [vp, vp, vp, vp, vp]
     lib.ora_get_wildcards.argtypes = [vp, vp]
     lib.ora_set_wildcards.argtypes = [vp, vp]
+    lib.ora_set_contact_forces.argtypes = [vp, vp, vp, vp, vp]
     lib.ora_force_fp64.argtypes = [vp, vp, vp]
     lib.ora_get_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     lib.ora_set_time.argtypes = [vp, C.c_double]
@@ -193,6 +194,13 @@ class OracleSystem:
         w = np.zeros((4, n), np.float32)
         self.lib.ora_get_wildcards(self.h, ptr(w))
         return w
+
+    def set_forces(self, F, T, cA, cB):
+        """Per-contact records of the current list (n x 3 float32 each), e.g. the device's."""
+        a = [np.ascontiguousarray(x, np.float32) for x in (F, T, cA, cB)]
+        n = self.counts().nContacts
+        assert all(x.shape == (n, 3) for x in a)
+        self.lib.ora_set_contact_forces(self.h, *(ptr(x) for x in a))
 
     def set_wildcards(self, w):
         w = np.ascontiguousarray(w, np.float32)

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from deme_amd import scenes
-from deme_amd.abi import ANAL_DTYPE, DEME_OK, DeviceSystem, load
+from deme_amd.abi import ANAL_DTYPE, DEME_OK, DeviceSystem, decode_positions, load
 
 pytestmark = pytest.mark.gpu
 
@@ -31,6 +31,16 @@ def eight_plane_owners(n=16000):
         extra[k]["relPos"] = np.array(p, np.float32)
         extra[k]["rot"] = np.array(nrm, np.float32)
         extra[k]["mass"] = np.float32(1e15)
+    # the side walls and the top moved in to just inside the outermost spheres, so every wall carries contacts
+    X = decode_positions(sc.params, sc.state["voxelID"][:nc], sc.state["locX"][:nc], sc.state["locY"][:nc],
+                         sc.state["locZ"][:nc])
+    r = float(sc.sph_radius.max())
+    lo, hi = X.min(0), X.max(0)
+    for k, axis, edge in ((1, 0, lo[0] - 0.8 * r), (2, 0, hi[0] + 0.8 * r), (3, 1, lo[1] - 0.8 * r),
+                          (4, 1, hi[1] + 0.8 * r), (5, 2, hi[2] + 0.8 * r)):
+        p = walls["relPos"][k].copy()
+        p[axis] = np.float32(edge)
+        walls["relPos"][k] = p
     anal = np.concatenate([walls, extra])
     anal["owner"] = nc + np.arange(len(anal), dtype=np.uint32)
     n_ext = len(anal)
@@ -58,7 +68,7 @@ def test_eight_plane_owners_sum_like_the_oracle():
     oa, ob, ot, _ = ora.contacts()
     assert np.array_equal(da, oa) and np.array_equal(db, ob) and np.array_equal(dt, ot)
     assert len(da) > 10000
-    plane_of = db[dt == 2]
+    plane_of = db[dt == 3]  # DEME_SPHERE_PLANE_CONTACT
     assert len(set(sc.anal["owner"][plane_of].tolist())) >= 6  # most plane owners carry contacts
     ora.set_wildcards(dev.wildcards())
     dev.force()
