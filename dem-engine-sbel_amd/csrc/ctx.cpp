@@ -85,6 +85,9 @@ struct deme_ctx {
     hipStream_t stream = nullptr;
     bool ownStream = false;
     std::string err;
+    // the allocation-free scan's fixed workspace on this context's device (looked up at creation)
+    uint64_t* noallocScratch = nullptr;
+    uint64_t noallocElems = 0;
 
     deme_sim_params sp{};
     KParams kp{};
@@ -381,7 +384,8 @@ int deme_ctx_create(int device, void* stream, deme_ctx** out) {
     deme_ctx* c = new (std::nothrow) deme_ctx();
     if (!c) return DEME_ERR_INVALID_ARGUMENT;
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess ||
+        noalloc_scan_scratch(device, &c->noallocScratch, &c->noallocElems) != hipSuccess) {
         delete c;
         return DEME_ERR_HIP;
     }
@@ -1136,6 +1140,23 @@ int deme_prefix_scan_u32(deme_ctx* c, const uint32_t* in, uint32_t* out, uint64_
         *total = t;
     }
     DEME_HIP(c, hipStreamSynchronize(c->stream));
+    return DEME_OK;
+}
+
+int deme_prefix_scan_u32_noalloc(deme_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* totalDev) {
+    if ((n && !in) || !out) return c->fail(DEME_ERR_INVALID_ARGUMENT, "null device array");
+    if (n > NOALLOC_SCAN_MAX)
+        return c->fail(DEME_ERR_INVALID_ARGUMENT,
+                       "deme_prefix_scan_u32_noalloc scans at most 2^28 elements (its fixed workspace); use "
+                       "deme_prefix_scan_u32 for longer arrays");
+    if (!c->noallocScratch || scan_scratch_elems(n) + 1 > c->noallocElems)
+        return c->fail(DEME_ERR_HIP, "the allocation-free scan's workspace is not available on this device");
+    // the last slot takes the total when the caller wants none (the kernels write it unconditionally otherwise)
+    unsigned long long* tot = totalDev ? reinterpret_cast<unsigned long long*>(totalDev)
+                                       : reinterpret_cast<unsigned long long*>(c->noallocScratch + c->noallocElems - 1);
+    // launched without the kernel timer (its event pool may grow)
+    const hipError_t e = scan_u32(in, nullptr, n, out, tot, c->noallocScratch, c->stream);
+    if (e != hipSuccess) return c->fail_hip(e, "scan");
     return DEME_OK;
 }
 

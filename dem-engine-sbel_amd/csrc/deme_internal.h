@@ -69,6 +69,12 @@ constexpr uint64_t scan_scratch_elems(uint64_t n) {
 }
 hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,
                     uint64_t* scratch, hipStream_t st, const uint32_t* idx = nullptr);
+// Largest scan of the allocation-free entry point: its tile sums live in a fixed device array of
+// scan_scratch_elems(NOALLOC_SCAN_MAX) + 1 u64 values (about 1 MiB) per GPU.
+constexpr uint64_t NOALLOC_SCAN_MAX = 1ull << 28;
+// Device address of that array on `device`, which must be the current device on the first call for it (the address
+// is looked up once per device and kept); *elems (nullable) = its length in u64 values.
+hipError_t noalloc_scan_scratch(int device, uint64_t** ptr, uint64_t* elems);
 // Two scans over one gather: out1 = scan(a1[idx[i]] + (b1 ? b1[i] : 0)), out2 = scan(a2[idx[i]]), totals at out[n]
 // and in *total1 / *total2. tmp: n uint2; sums: 2 * ceil(n / SCAN2_TILE) u64. n must be at most
 // SCAN2_TILE * SCAN_TILE (hipErrorInvalidValue otherwise).

@@ -4,6 +4,8 @@
 // Three phases per level, tiles of 4096 elements (256 lanes x 16 items, four wave64s): tile reduce -> scan of tile
 // sums (recursively) -> tile scan plus offset. Deterministic; sums are carried in 64 bits so overflow of the 32-bit
 // outputs is detectable from the returned total.
+#include <mutex>
+
 #include "deme_internal.h"
 
 namespace deme_int {
@@ -266,6 +268,36 @@ hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32
     if (!fold) k_scan2_tiles<<<1, SCAN_BLOCK, 0, st>>>(sums, sums + nb, nb);
     k_scan2_apply<<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, fold);
     return hipGetLastError();
+}
+
+// ---- workspace of the allocation-free scan (deme_prefix_scan_u32_noalloc) ----
+// The tile sums of scans of up to NOALLOC_SCAN_MAX elements, plus one slot for a total nobody asked for. One copy per
+// GPU, held until the process ends. In host code the name `scratch` is a host-side stand-in, not a device address:
+// noalloc_scan_scratch() looks the address up once per device and every launch takes that address.
+constexpr uint64_t NOALLOC_SCRATCH_ELEMS = scan_scratch_elems(NOALLOC_SCAN_MAX) + 1;
+static __device__ __align__(256) uint64_t scratch[NOALLOC_SCRATCH_ELEMS];
+
+hipError_t noalloc_scan_scratch(int device, uint64_t** ptr, uint64_t* elems) {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static uint64_t* table[MAX_DEVICES] = {};
+    *ptr = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!table[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;  // the lookup gives the current device's copy
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        table[device] = static_cast<uint64_t*>(p);
+    }
+    *ptr = table[device];
+    if (elems) *elems = NOALLOC_SCRATCH_ELEMS;
+    return hipSuccess;
 }
 
 hipError_t scan_u32(const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out, unsigned long long* totalDev,

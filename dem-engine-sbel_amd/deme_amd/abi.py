@@ -89,7 +89,7 @@ EXPORTED = [
     "deme_set_keep_contact_records", "deme_set_owner_volumes", "deme_inspect", "deme_append_clumps",
     "deme_change_clump_sizes",
     "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner", "deme_jit_render_inspect",
-    "deme_prefix_scan_u32", "deme_integrate_owners", "deme_set_graph_headroom",
+    "deme_prefix_scan_u32", "deme_integrate_owners", "deme_prefix_scan_u32_noalloc", "deme_set_graph_headroom",
     "deme_graph_stats", "deme_set_pair_sweep", "deme_autotune_sweep",
 ]
 
@@ -136,6 +136,7 @@ def load(path: str = LIB_PATH):
         "deme_download_bins": ([vp, vp, vp], i32),
         "deme_download_contacts": ([vp, vp, vp, vp, vp], i32),
         "deme_prefix_scan_u32": ([vp, vp, vp, u64, C.POINTER(u64)], i32),
+        "deme_prefix_scan_u32_noalloc": ([vp, vp, vp, u64, vp], i32),
         "deme_set_graph_headroom": ([vp, C.c_double, u64], i32),
         "deme_graph_stats": ([vp, vp], i32),
         "deme_set_pair_sweep": ([vp, i32, i32], i32),

@@ -108,6 +108,15 @@ int deme_download_contacts(deme_ctx* ctx, uint32_t* idA, uint32_t* idB, uint8_t*
  * are device pointers (they may be the same array); the call runs on the context's stream and synchronises it. Sums
  * are formed in 64 bits in a fixed order. */
 int deme_prefix_scan_u32(deme_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n, uint64_t* total);
+/* The same scan without allocating anything (no counterpart in the reference): for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. Its tile sums
+ * live in a fixed device workspace (about 1 MiB per GPU, looked up when the context is created and held until the
+ * process ends), so n is at most 2^28 (DEME_ERR_INVALID_ARGUMENT beyond). d_out receives n + 1 entries as above;
+ * d_total (device pointer, nullable) receives the 64-bit total. The call only enqueues on the context's stream and
+ * returns without waiting. Every call on one GPU shares that workspace: calls that overlap on one GPU (two streams
+ * or two contexts at once) race; calls ordered on one stream do not. */
+int deme_prefix_scan_u32_noalloc(deme_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
+                                 uint64_t* d_total);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Dynamics (DEMDynamicThread::calculateForces / integrateOwnerMotions, src/DEM/dT.cpp:1349-1432).
