@@ -91,20 +91,13 @@ __device__ __forceinline__ uint32_t run_slot(uint32_t* counter, uint32_t key) {
     return base + (uint32_t)__popcll(runMask & active & (upTo >> 1));
 }
 
-// Per sphere: world position, plane contacts (a10), bin counts (a6), cell assignment with its slot.
-__global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, const SphereRec* __restrict__ sph,
-                                                     const OwnerRec* __restrict__ own, uint32_t nS,
-                                                     const PlanePose* __restrict__ pose, uint32_t nA,
-                                                     const uint8_t* __restrict__ mask, uint32_t* cellCount,
-                                                     uint32_t* __restrict__ cellOf, uint32_t* __restrict__ slotOf,
-                                                     uint16_t* __restrict__ numBins, uint32_t* __restrict__ numAnal,
-                                                     unsigned long long* __restrict__ planeBits,
-                                                     DevCounters* ctr) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nS) return;
-    const SphereRec sr = load_sphere(sph, s);
-    const OwnerRec o = load_owner(own, sr.owner);
-    double X, Y, Z;
+// Per sphere, shared by the cell assignment and the per-step position pass: world position (a4, a5), bin counts and
+// range checks (a6), plane contacts (a10).
+__device__ __forceinline__ void sphere_detect(const KParams& P, const SphereRec& sr, const OwnerRec& o, uint32_t s,
+                                              const PlanePose* __restrict__ pose, uint32_t nA,
+                                              const uint8_t* __restrict__ mask, uint16_t* __restrict__ numBins,
+                                              uint32_t* __restrict__ numAnal, unsigned long long* __restrict__ planeBits,
+                                              DevCounters* ctr, double& X, double& Y, double& Z) {
     sphere_position(P, o, sr, X, Y, Z);
     double rCD = (double)sr.r;
     rCD += (double)P.beta;
@@ -142,11 +135,66 @@ __global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, cons
     }
     numAnal[s] = cnt;
     planeBits[s] = bits;
+}
+
+// Per sphere: the shared detection work, then the cell assignment with its slot.
+__global__ __launch_bounds__(256) void k_cell_assign(KParams P, CellGrid G, const SphereRec* __restrict__ sph,
+                                                     const OwnerRec* __restrict__ own, uint32_t nS,
+                                                     const PlanePose* __restrict__ pose, uint32_t nA,
+                                                     const uint8_t* __restrict__ mask, uint32_t* cellCount,
+                                                     uint32_t* __restrict__ cellOf, uint32_t* __restrict__ slotOf,
+                                                     uint16_t* __restrict__ numBins, uint32_t* __restrict__ numAnal,
+                                                     unsigned long long* __restrict__ planeBits,
+                                                     DevCounters* ctr) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nS) return;
+    const SphereRec sr = load_sphere(sph, s);
+    const OwnerRec o = load_owner(own, sr.owner);
+    double X, Y, Z;
+    sphere_detect(P, sr, o, s, pose, nA, mask, numBins, numAnal, planeBits, ctr, X, Y, Z);
     int cx, cy, cz;
     cell_of(G, X, Y, Z, cx, cy, cz);
     const uint32_t cell = (uint32_t)cx + (uint32_t)cy * G.nx + (uint32_t)cz * G.nx * G.ny;
     cellOf[s] = cell;
     slotOf[s] = run_slot(cellCount, cell);
+}
+
+// Per sphere, each step of the candidate-list path: the shared detection work, the detection record in id order and
+// (owner, family), and the largest squared displacement since the candidate lists were built (one atomic per wave).
+__global__ __launch_bounds__(256) void k_sphere_pos(KParams P, const SphereRec* __restrict__ sph,
+                                                    const OwnerRec* __restrict__ own, uint32_t nS,
+                                                    const PlanePose* __restrict__ pose, uint32_t nA,
+                                                    const uint8_t* __restrict__ mask, CdRec* __restrict__ pos,
+                                                    const CdRec* __restrict__ pos0, uint2* __restrict__ meta,
+                                                    uint16_t* __restrict__ numBins, uint32_t* __restrict__ numAnal,
+                                                    unsigned long long* __restrict__ planeBits, DevCounters* ctr) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (s < nS) {
+        const SphereRec sr = load_sphere(sph, s);
+        const OwnerRec o = load_owner(own, sr.owner);
+        CdRec cr;
+        sphere_detect(P, sr, o, s, pose, nA, mask, numBins, numAnal, planeBits, ctr, cr.x, cr.y, cr.z);
+        cr.r = sr.r;
+        cr.id = s;
+        uint4* d4 = reinterpret_cast<uint4*>(pos + s);
+        const uint4* s4 = reinterpret_cast<const uint4*>(&cr);
+        d4[0] = s4[0];
+        d4[1] = s4[1];
+        meta[s] = make_uint2(sr.owner, o.fam);
+        const CdRec c0 = load_cd(pos0, s);
+        const double dx = cr.x - c0.x, dy = cr.y - c0.y, dz = cr.z - c0.z;
+        d2 = dx * dx + dy * dy + dz * dz;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, 64));
+    __shared__ double waveMax[4];
+    if ((threadIdx.x & 63) == 0) waveMax[threadIdx.x >> 6] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = fmax(fmax(waveMax[0], waveMax[1]), fmax(waveMax[2], waveMax[3]));
+        if (m > 0.0) atomicMax(&ctr->disp2[blockIdx.x % DISP_SLOTS], (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // The sorted CD record of each sphere, its position recomputed from the sphere and owner records exactly as the
@@ -342,43 +390,48 @@ __device__ __forceinline__ void append_pair(uint32_t* __restrict__ cnt, uint32_t
     else atomicMax(&ctr->maxOcc, slot + 1);
 }
 
-// Second phase, test half: the family mask and the bin rule of an overlapping pair of different owners (sorted
-// positions i, j); on acceptance the ids of i and j. CHECK: the pair passed only the float sweep, so the same-owner
-// rule and the exact expanded-overlap test come first.
-template <bool CHECK>
+// Candidate-list predicate: the pair lies within the skin of touching, d <= r'A + r'B + skin (double precision).
+__device__ __forceinline__ bool within_skin(const KParams& P, const CdRec& A, const CdRec& B, double skin) {
+    const double dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+    const double lim = expanded_radius(P, A.r) + expanded_radius(P, B.r) + skin;
+    return !(dx * dx + dy * dy + dz * dz > lim * lim);
+}
+
+// What the shared second phase still has to check of a queued pair (sorted positions i, j): CHK_NONE, the sweep
+// checked owners and the exact overlap; CHK_FLOAT, the pair passed only the float sweep, so the same-owner rule and the
+// exact expanded-overlap test come first; CHK_CAND, a candidate-list build: the same-owner rule and the skin test only.
+enum : int { CHK_NONE = 0, CHK_FLOAT = 1, CHK_CAND = 2 };
+
+// Second phase, test half: on acceptance the ids of i and j.
+template <int CHECK>
 __device__ __forceinline__ bool accept_pair(const KParams& P, double inv, const CdRec* __restrict__ sorted,
                                             const uint2* __restrict__ sortedMeta, const uint8_t* __restrict__ mask,
-                                            uint32_t i, uint32_t j, uint32_t& idI, uint32_t& idJ) {
-    if (CHECK || mask) {
+                                            uint32_t i, uint32_t j, double skin, uint32_t& idI, uint32_t& idJ) {
+    const bool useMask = CHECK != CHK_CAND && mask;
+    if (CHECK != CHK_NONE || useMask) {
         const uint2 mi = sortedMeta[i], mj = sortedMeta[j];
-        if (CHECK && mi.x == mj.x) return false;
-        if (mask && mask[mask_index(mi.y, mj.y)] != DEME_DONT_PREVENT_CONTACT) return false;
+        if (CHECK != CHK_NONE && mi.x == mj.x) return false;
+        if (useMask && mask[mask_index(mi.y, mj.y)] != DEME_DONT_PREVENT_CONTACT) return false;
     }
     const CdRec a = load_cd(sorted, i), b = load_cd(sorted, j);
-    if (CHECK && !expanded_overlap(expanded_radius(P, a.r), a, expanded_radius(P, b.r), b)) return false;
     idI = a.id;
     idJ = b.id;
+    if (CHECK == CHK_CAND) return within_skin(P, a, b, skin);
+    if (CHECK == CHK_FLOAT && !expanded_overlap(expanded_radius(P, a.r), a, expanded_radius(P, b.r), b)) return false;
     return a.id < b.id ? bin_rule_point(P, inv, a, b) : bin_rule_point(P, inv, b, a);
 }
 
-// Second phase for one overlapping pair of different owners (sorted positions i, j); CHECK as above.
-template <bool CHECK>
+// Second phase for one queued pair (sorted positions i, j) that did not fit the queue; CHECK as above.
+template <int CHECK>
 __device__ __forceinline__ void finish_pair(const KParams& P, double inv, uint32_t nS, const CdRec* __restrict__ sorted,
                                             const uint2* __restrict__ sortedMeta, const uint8_t* __restrict__ mask,
-                                            uint32_t i, uint32_t j, uint32_t slabCap, uint32_t* rowSlab,
+                                            uint32_t i, uint32_t j, double skin, uint32_t slabCap, uint32_t* rowSlab,
                                             uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt, DevCounters* ctr) {
-    // (mask == nullptr: no family pair is masked)
-    if (CHECK || mask) {
-        const uint2 mi = sortedMeta[i], mj = sortedMeta[j];
-        if (CHECK && mi.x == mj.x) return;
-        if (mask && mask[mask_index(mi.y, mj.y)] != DEME_DONT_PREVENT_CONTACT) return;
-    }
-    const CdRec a = load_cd(sorted, i), b = load_cd(sorted, j);
-    if (CHECK && !expanded_overlap(expanded_radius(P, a.r), a, expanded_radius(P, b.r), b)) return;
-    const bool aLow = a.id < b.id;
-    if (!(aLow ? bin_rule_point(P, inv, a, b) : bin_rule_point(P, inv, b, a))) return;
-    const uint32_t loIdx = aLow ? i : j, hiIdx = aLow ? j : i;
-    const uint32_t loId = aLow ? a.id : b.id, hiId = aLow ? b.id : a.id;
+    uint32_t idI, idJ;
+    if (!accept_pair<CHECK>(P, inv, sorted, sortedMeta, mask, i, j, skin, idI, idJ)) return;
+    const bool iLow = idI < idJ;
+    const uint32_t loIdx = iLow ? i : j, hiIdx = iLow ? j : i;
+    const uint32_t loId = iLow ? idI : idJ, hiId = iLow ? idJ : idI;
     append_pair(aCnt, rowSlab, loIdx, hiId, nS, slabCap, ctr);
     append_pair(bCnt, mirSlab, hiIdx, loId, nS, slabCap, ctr);
 }
@@ -386,13 +439,16 @@ __device__ __forceinline__ void finish_pair(const KParams& P, double inv, uint32
 // Sweep modes: the exact test on every candidate (PAIR_EXACT); a float pre-reject with the exact test on the
 // survivors (PAIR_PRE); the float test alone, with the same-owner rule and the exact test in the shared second phase
 // (PAIR_FLOAT).
-enum : int { PAIR_EXACT = 0, PAIR_PRE = 1, PAIR_FLOAT = 2 };
+// PAIR_CAND builds the candidate lists of the candidate-list path: the float sweep with the skin added to its margin,
+// then in the shared phase the same-owner rule and the exact test d <= r'A + r'B + skin (no family mask, no bin rule:
+// both are applied each step to the pairs found close enough).
+enum : int { PAIR_EXACT = 0, PAIR_PRE = 1, PAIR_FLOAT = 2, PAIR_CAND = 3 };
 
 template <int MODE, uint32_t BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_cell_pairs(KParams P, CellGrid G, uint32_t nS, uint32_t nBlocks,
                                                            const CdRec* __restrict__ sorted,
                                                            const float4* __restrict__ sortedF, float margin,
-                                                           const uint2* __restrict__ sortedMeta,
+                                                           double skin, const uint2* __restrict__ sortedMeta,
                                                            const uint32_t* __restrict__ cellStart,
                                                            const uint8_t* __restrict__ mask, uint32_t slabCap,
                                                            uint32_t* __restrict__ surv, uint32_t* __restrict__ rowSlab,
@@ -476,7 +532,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_pairs(KParams P, CellGrid G, uin
                 const float lim = mf.w + of.w + margin;
                 return !(dx * dx + dy * dy + dz * dz > lim * lim);
             };
-            if (MODE == PAIR_FLOAT) {
+            if (MODE == PAIR_FLOAT || MODE == PAIR_CAND) {
                 // float copies only: 16 bytes per candidate; the few survivors get their exact tests in the queue
                 for (; j + 3 < e; j += 4) {
                     const float4 o0 = sortedF[j], o1 = sortedF[j + 1], o2 = sortedF[j + 2], o3 = sortedF[j + 3];
@@ -521,7 +577,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell_pairs(KParams P, CellGrid G, uin
     for (uint32_t q = threadIdx.x; q < nq; q += BLOCK) {
         const uint32_t si = qI[q], sj = qJ[q];
         uint32_t idI, idJ;
-        if (!accept_pair<MODE == PAIR_FLOAT>(P, inv, sorted, sortedMeta, mask, si, sj, idI, idJ)) {
+        constexpr int CHECK = MODE == PAIR_CAND ? CHK_CAND : (MODE == PAIR_FLOAT ? CHK_FLOAT : CHK_NONE);
+        if (!accept_pair<CHECK>(P, inv, sorted, sortedMeta, mask, si, sj, skin, idI, idJ)) {
             qI[q] = REJECTED;
             continue;
         }
@@ -554,8 +611,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell_pairs(KParams P, CellGrid G, uin
         return;
     }
     for (uint32_t k = 0; k < nc; k++)
-        finish_pair<MODE == PAIR_FLOAT>(P, inv, nS, sorted, sortedMeta, mask, i, surv[(uint64_t)k * nS + i], slabCap,
-                                        rowSlab, mirSlab, aCnt, bCnt, ctr);
+        finish_pair<MODE == PAIR_CAND ? CHK_CAND : (MODE == PAIR_FLOAT ? CHK_FLOAT : CHK_NONE)>(
+            P, inv, nS, sorted, sortedMeta, mask, i, surv[(uint64_t)k * nS + i], skin, slabCap, rowSlab, mirSlab, aCnt,
+            bCnt, ctr);
 }
 
 // Canonical rows: plane contacts (ascending plane index) then partners (ascending id); idA for the row. Mirror rows:
@@ -618,12 +676,77 @@ __global__ __launch_bounds__(256) void k_cell_rows(KParams P, const SphereRec* _
         pos++;
     }
     const uint32_t sp = pos;
-    const uint32_t p = posOf[s];
+    const uint32_t p = posOf ? posOf[s] : s;  // slab columns by sorted position, or by id (candidate-list path)
     sorted_copy(rowSlab + p, en - sp, nS, idB + sp);
     const uint32_t bs = bRowStart[s];
     sorted_copy(mirSlab + p, bRowStart[s + 1] - bs, nS, bPart + bs);
     for (uint32_t q = sp; q < en; q++) ctype[q] = DEME_SPHERE_SPHERE_CONTACT;
     for (uint32_t q = st; q < en; q++) idA[q] = s;
+}
+
+// Candidate lists from a candidate build: sphere s's row of higher-id partners within the skin (ascending), copied
+// from its slab column at its sorted position; the sphere's position at the build is kept for the displacement test.
+__global__ __launch_bounds__(256) void k_cand_rows(uint32_t nS, const uint32_t* __restrict__ rowSlab,
+                                                   const uint32_t* __restrict__ posOf,
+                                                   const uint32_t* __restrict__ candStart, uint32_t* __restrict__ candB,
+                                                   const CdRec* __restrict__ sorted, CdRec* __restrict__ pos0) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nS) return;
+    const uint32_t p = posOf[s];
+    const uint32_t st = candStart[s];
+    sorted_copy(rowSlab + p, candStart[s + 1] - st, nS, candB + st);
+    const uint4* q = reinterpret_cast<const uint4*>(sorted + p);
+    uint4* d = reinterpret_cast<uint4*>(pos0 + s);
+    d[0] = q[0];
+    d[1] = q[1];
+}
+
+// Each step of the candidate-list path, per sphere A (id order): the reference's predicate (the exact expanded
+// overlap, the family mask, the contact-point bin rule) on A's candidates, which hold every partner that can touch
+// while no sphere has moved more than half the skin since the build. Accepted partners go to A's slab column in
+// ascending order (A's lane owns it) and A to the partner's mirror column. If a sphere moved further the pass is
+// marked stale and does nothing; the host rebuilds the lists and repeats it.
+__global__ __launch_bounds__(256) void k_cand_test(KParams P, uint32_t nS, const CdRec* __restrict__ pos,
+                                                   const uint2* __restrict__ meta,
+                                                   const uint32_t* __restrict__ candStart,
+                                                   const uint32_t* __restrict__ candB, const uint8_t* __restrict__ mask,
+                                                   double dispLimit2, uint32_t slabCap, uint32_t* __restrict__ rowSlab,
+                                                   uint32_t* __restrict__ mirSlab, uint32_t* __restrict__ aCnt,
+                                                   uint32_t* bCnt, DevCounters* ctr) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nS || ctr->err) return;
+    if (max_disp2(*ctr) > dispLimit2) {
+        if (s == 0) atomicOr(&ctr->err, ERRBIT_STALE);
+        return;
+    }
+    const double inv = 1.0 / P.binSize;
+    const CdRec a = load_cd(pos, s);
+    const double ra = expanded_radius(P, a.r);
+    const uint32_t famA = mask ? meta[s].y : 0u;
+    const uint32_t c0 = candStart[s], c1 = candStart[s + 1];
+    uint32_t n = 0;
+    auto take = [&](uint32_t b, const CdRec& rb) {
+        if (!expanded_overlap(ra, a, expanded_radius(P, rb.r), rb)) return;
+        if (mask && mask[mask_index(famA, meta[b].y)] != DEME_DONT_PREVENT_CONTACT) return;
+        if (!bin_rule_point(P, inv, a, rb)) return;  // a has the lower id
+        if (n < slabCap) rowSlab[(uint64_t)n * nS + s] = b;
+        n++;
+        append_pair(bCnt, mirSlab, b, s, nS, slabCap, ctr);
+    };
+    uint32_t c = c0;
+    for (; c + 1 < c1; c += 2) {
+        // both candidates' records in flight before either test
+        const uint32_t b0 = candB[c], b1 = candB[c + 1];
+        const CdRec r0 = load_cd(pos, b0), r1 = load_cd(pos, b1);
+        take(b0, r0);
+        take(b1, r1);
+    }
+    if (c < c1) {
+        const uint32_t b0 = candB[c];
+        take(b0, load_cd(pos, b0));
+    }
+    aCnt[s] = n;
+    if (n > slabCap) atomicMax(&ctr->maxOcc, n);
 }
 
 inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
@@ -653,19 +776,23 @@ hipError_t launch_cell_scatter(const KParams& P, const SphereRec* sph, const Own
 template <uint32_t BLOCK>
 static void launch_pairs_block(int mode, uint32_t grid, hipStream_t st, const KParams& P, const CellGrid& G,
                                uint32_t nS, uint32_t nb, const CdRec* sorted, const float4* sortedF, float margin,
-                               const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask,
+                               double skin, const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask,
                                uint32_t slabCap, uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt,
                                uint32_t* bCnt, DevCounters* ctr) {
-    if (mode == PAIR_FLOAT)
-        k_cell_pairs<PAIR_FLOAT, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedF, margin, sortedMeta,
+    if (mode == PAIR_CAND)
+        k_cell_pairs<PAIR_CAND, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedF, margin, skin, sortedMeta,
+                                                               cellStart, nullptr, slabCap, surv, rowSlab, mirSlab,
+                                                               aCnt, bCnt, ctr);
+    else if (mode == PAIR_FLOAT)
+        k_cell_pairs<PAIR_FLOAT, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedF, margin, 0.0, sortedMeta,
                                                                 cellStart, mask, slabCap, surv, rowSlab, mirSlab,
                                                                 aCnt, bCnt, ctr);
     else if (mode == PAIR_PRE)
-        k_cell_pairs<PAIR_PRE, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedF, margin, sortedMeta,
+        k_cell_pairs<PAIR_PRE, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, sortedF, margin, 0.0, sortedMeta,
                                                               cellStart, mask, slabCap, surv, rowSlab, mirSlab, aCnt,
                                                               bCnt, ctr);
     else
-        k_cell_pairs<PAIR_EXACT, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, nullptr, 0.f, sortedMeta,
+        k_cell_pairs<PAIR_EXACT, BLOCK><<<grid, BLOCK, 0, st>>>(P, G, nS, nb, sorted, nullptr, 0.f, 0.0, sortedMeta,
                                                                 cellStart, mask, slabCap, surv, rowSlab, mirSlab,
                                                                 aCnt, bCnt, ctr);
 }
@@ -674,20 +801,50 @@ hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, c
                              const float4* sortedF, float margin, int mode, uint32_t block,
                              const uint2* sortedMeta, const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
                              uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
-                             DevCounters* ctr, hipStream_t st) {
+                             DevCounters* ctr, hipStream_t st, double skin) {
     if (nS == 0) return hipSuccess;
+    if (mode < PAIR_EXACT || mode > PAIR_CAND) return hipErrorInvalidValue;
     if (mode != PAIR_EXACT && !sortedF) return hipErrorInvalidValue;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    // the candidate build's float test covers the skin as well
+    const float m = mode == PAIR_CAND ? margin + (float)skin : margin;
     const uint32_t nb = blocks(nS, block), grid = 8 * ((nb + 7) / 8);
     if (block == 64)
-        launch_pairs_block<64>(mode, grid, st, P, G, nS, nb, sorted, sortedF, margin, sortedMeta, cellStart, mask,
+        launch_pairs_block<64>(mode, grid, st, P, G, nS, nb, sorted, sortedF, m, skin, sortedMeta, cellStart, mask,
                                slabCap, surv, rowSlab, mirSlab, aCnt, bCnt, ctr);
     else if (block == 256)
-        launch_pairs_block<256>(mode, grid, st, P, G, nS, nb, sorted, sortedF, margin, sortedMeta, cellStart, mask,
+        launch_pairs_block<256>(mode, grid, st, P, G, nS, nb, sorted, sortedF, m, skin, sortedMeta, cellStart, mask,
                                 slabCap, surv, rowSlab, mirSlab, aCnt, bCnt, ctr);
     else
-        launch_pairs_block<128>(mode, grid, st, P, G, nS, nb, sorted, sortedF, margin, sortedMeta, cellStart, mask,
+        launch_pairs_block<128>(mode, grid, st, P, G, nS, nb, sorted, sortedF, m, skin, sortedMeta, cellStart, mask,
                                 slabCap, surv, rowSlab, mirSlab, aCnt, bCnt, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                             const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos, const CdRec* pos0,
+                             uint2* meta, uint16_t* numBins, uint32_t* numAnal, unsigned long long* planeBits,
+                             DevCounters* ctr, hipStream_t st) {
+    if (nS == 0) return hipSuccess;
+    k_sphere_pos<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, pose, nA, mask, pos, pos0, meta, numBins, numAnal,
+                                                  planeBits, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* posOf, const uint32_t* candStart,
+                            uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st) {
+    if (nS == 0) return hipSuccess;
+    k_cand_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, rowSlab, posOf, candStart, candB, sorted, pos0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cand_test(const KParams& P, uint32_t nS, const CdRec* pos, const uint2* meta,
+                            const uint32_t* candStart, const uint32_t* candB, const uint8_t* mask, double dispLimit2,
+                            uint32_t slabCap, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
+                            DevCounters* ctr, hipStream_t st) {
+    if (nS == 0) return hipSuccess;
+    k_cand_test<<<blocks(nS, 256), 256, 0, st>>>(P, nS, pos, meta, candStart, candB, mask, dispLimit2, slabCap,
+                                                 rowSlab, mirSlab, aCnt, bCnt, ctr);
     return hipGetLastError();
 }
 

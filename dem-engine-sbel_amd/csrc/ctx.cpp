@@ -113,6 +113,21 @@ struct deme_ctx {
         std::vector<const void*> ptrs;
     } cycle[2];
     uint64_t cycleReplays = 0, cycleFallbacks = 0, cycleCaptures = 0;
+    // Candidate-list detection: pairs within `skin` of touching, kept from a build until some sphere has moved half
+    // the skin; each pass then tests only them. skinSet < 0: automatic (0.2 x the mean radius); 0: off (the cell
+    // sweep every pass).
+    // The automatic skin adapts at each build to the passes the previous lists lasted: fewer than 8 widen it 1.5x (up
+    // to 0.8 x the mean radius), more than 400 narrow it 0.75x (down to 0.05 x); any skin gives the same contacts.
+    double skinSet = -1.0, skinAuto = 0.0;
+    bool candValid = false, rebuildNext = true;
+    uint64_t nCand = 0, candBuilds = 0, passesSinceBuild = 0;
+    DBuf<CdRec> pos, pos0;
+    DBuf<uint2> meta;
+    DBuf<uint32_t> candStart, candB;
+    double skin() const {
+        if (skinSet >= 0.0) return skinSet;
+        return skinAuto > 0.0 ? skinAuto : 0.2 * (double)meanRadius;
+    }
     int pairMode = -1;  // pair sweep mode (launch_cell_pairs); -1: chosen from the radius spread
     uint32_t pairBlock = 256;  // pair sweep workgroup size
     double cycleHeadroom = 1.25;  // capacity of a captured interval: headroom x the current counts + slack
@@ -436,6 +451,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->cellCount.release(); c->cellStart.release(); c->cellOf.release(); c->slotOf.release(); c->posOf.release();
     c->survSlab.release(); c->rowSlab.release(); c->mirSlab.release(); c->sorted.release(); c->planeBits.release();
     c->aCnt.release(); c->sortedMeta.release();
+    c->pos.release(); c->pos0.release(); c->meta.release(); c->candStart.release(); c->candB.release();
     for (auto& k : c->adjCk) {
         k.own.release(); k.rowStart.release(); k.idB.release(); k.ctype.release(); k.hist.release();
         k.idA.release(); k.mirror.release(); k.bRowStart.release(); k.acc.release();
@@ -473,6 +489,7 @@ int deme_sync(deme_ctx* c) {
 
 int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
     c->graphGen++;
+    c->candValid = false;  // positions, radii or the margin changed: candidate lists are rebuilt
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (p->nvXp2 + p->nvYp2 + p->nvZp2 != 64) return c->fail(DEME_ERR_INVALID_ARGUMENT, "nvXp2 + nvYp2 + nvZp2 != 64");
     if (!(p->binSize > 0.0)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "bin size must be positive");
@@ -510,6 +527,7 @@ int deme_upload_templates(deme_ctx* c, uint32_t nS, const uint32_t* sphOwner, co
                           const float* sphRelPos, const uint8_t* sphMat, uint32_t nO, uint32_t nClumps,
                           const float* ownerMass, const float* ownerMoi) {
     c->graphGen++;
+    c->candValid = false;  // positions, radii or the margin changed: candidate lists are rebuilt
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (nClumps > nO) return c->fail(DEME_ERR_INVALID_ARGUMENT, "nClumps > nOwners");
     if (nO - nClumps > MAX_ANAL_OWNERS)
@@ -628,6 +646,7 @@ int deme_upload_analytical(deme_ctx* c, uint32_t n, const deme_anal_obj* objs) {
 
 int deme_upload_scenes(deme_ctx* c, uint32_t nScenes, const uint32_t* ownerScene, const uint32_t* analScene) {
     c->graphGen++;
+    c->candValid = false;  // positions, radii or the margin changed: candidate lists are rebuilt
     if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before the scenes");
     if (nScenes == 0 || !ownerScene) return c->fail(DEME_ERR_INVALID_ARGUMENT, "need at least one scene");
     for (uint32_t o = 0; o < c->nO; o++)
@@ -670,6 +689,7 @@ int deme_upload_families(deme_ctx* c, const uint8_t* mask, const deme_family_rul
 }
 
 int deme_upload_state(deme_ctx* c, const deme_owner_state* s) {
+    c->candValid = false;  // positions, radii or the margin changed: candidate lists are rebuilt
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (!c->haveTemplates) return c->fail(DEME_ERR_NOT_READY, "upload templates before the owner state");
     const uint32_t n = c->nO;
@@ -716,6 +736,7 @@ int deme_upload_state(deme_ctx* c, const deme_owner_state* s) {
 
 int deme_clear_contacts(deme_ctx* c) {
     c->graphGen++;
+    c->candValid = false;  // positions, radii or the margin changed: candidate lists are rebuilt
     c->adjReady = false;
     c->havePrev = c->contactsReady = c->fresh = false;
     c->termsReady = c->forcesFull = false;
@@ -912,7 +933,9 @@ int deme_cd_sort_remap(deme_ctx* c) {
 static int setup_grid(deme_ctx* c) {
     if (!std::isfinite(c->maxRadius) || !(c->maxRadius > 0.f))
         return c->fail(DEME_ERR_INVALID_ARGUMENT, "sphere radii must be positive and finite");
-    double edge = 2.0 * ((double)c->maxRadius + (double)c->sp.beta) * 1.0001 * c->cellScale;
+    // with candidate lists the cells serve their builds, whose interaction distance is widened by the skin
+    const double skin = c->skin() > 0.0 ? c->skin() : 0.0;
+    double edge = 2.0 * ((double)c->maxRadius + (double)c->sp.beta + 0.5 * skin) * 1.0001 * c->cellScale;
     // the grid spans the user's domain (positions beyond it fall into edge cells), at most the voxel world
     auto span = [](double voxelWorld, float box) {
         return (box > 0.f && (double)box < voxelWorld) ? (double)box : voxelWorld;
@@ -955,7 +978,8 @@ static int enqueue_rows(deme_ctx* c, SpecGuard guard) {
     const int cu = c->cur, pv = cu ^ 1;
     const uint32_t nS = c->nS;
     DEME_LAUNCH(c, "cell_rows",
-                launch_cell_rows(c->kp, c->sph.p, c->own.p, nS, c->rowSlab.p, c->mirSlab.p, c->posOf.p,
+                launch_cell_rows(c->kp, c->sph.p, c->own.p, nS, c->rowSlab.p, c->mirSlab.p,
+                                 c->skin() > 0.0 ? nullptr : c->posOf.p,
                                  c->rowStart[cu].p, c->planeBits.p, c->bRowStart.p, c->idA.p, c->idB[cu].p,
                                  c->ctype[cu].p, c->bPart.p, c->stream, guard));
     DEME_LAUNCH(c, "history_map",
@@ -1041,16 +1065,143 @@ static int enqueue_pair_pass(deme_ctx* c, int mode, int cu) {
     return DEME_OK;
 }
 
+// Rounding allowance of the candidate lists' displacement test: positions are doubles near 1 m, so a nanometre lies
+// far above their rounding and far below any skin.
+constexpr double SKIN_SLACK = 1e-9;
+
+// The candidate-list pass is stale once some sphere has moved half the skin (less the slack) since the build.
+static double stale_limit2(const deme_ctx* c) {
+    const double h = 0.5 * c->skin() - SKIN_SLACK;
+    return h > 0.0 ? h * h : 0.0;
+}
+
+// Candidate lists from the current state: the cell sweep in its build mode (pairs of different owners within the
+// skin of touching), rows of higher-id partners in id order (candStart / candB) and the build positions (pos0).
+static int build_candidates(deme_ctx* c) {
+    if (c->skinSet < 0.0) {
+        const double m = (double)c->meanRadius, cur = c->skin();
+        double next = cur;
+        if (c->candBuilds > 0 && c->passesSinceBuild < 8) next = std::min(cur * 1.5, 0.8 * m);
+        else if (c->candBuilds > 0 && c->passesSinceBuild > 400) next = std::max(cur * 0.75, 0.05 * m);
+        c->skinAuto = next;
+    }
+    c->passesSinceBuild = 0;
+    int mode = 0;
+    int r = cd_buffers(c, &mode);
+    if (r) return r;
+    const uint32_t nS = c->nS;
+    DEME_HIP(c, c->sortedF.ensure(nS));
+    DEME_HIP(c, c->candStart.ensure((size_t)nS + 1));
+    DEME_HIP(c, c->pos0.ensure(nS));
+    for (int attempt = 0;; attempt++) {
+        r = enqueue_cell_pass(c, PAIR_MODE_CAND);
+        if (r) return r;
+        DEME_LAUNCH(c, "cand_build",
+                    launch_cell_pairs(c->kp, c->grid, nS, c->sorted.p, c->sortedF.p, sweep_margin(c->sp),
+                                      PAIR_MODE_CAND, c->pairBlock, c->sortedMeta.p, c->cellStart.p, nullptr,
+                                      c->slabCap, c->survSlab.p, c->rowSlab.p, c->mirSlab.p, c->aCnt.p, c->bCnt.p,
+                                      c->ctr.p, c->stream, c->skin()));
+        DEME_LAUNCH(c, "scan_cand",
+                    scan_u32(c->aCnt.p, nullptr, nS, c->candStart.p, &c->ctr.p->total[0], c->scanScratch.p,
+                             c->stream, c->posOf.p));
+        r = readback_counters(c);
+        if (r) return r;
+        const DevCounters& h = *c->hctr;
+        if (h.err & ERRBIT_CAPACITY)
+            return c->fail(DEME_ERR_CAPACITY, "a sphere touches more than 65535 bins (binsSphereTouches_t)");
+        if (h.err) return c->fail(DEME_ERR_CAPACITY, "a sphere left the bin grid of the simulation world");
+        if (h.maxOcc <= c->slabCap) break;
+        if (attempt >= 2) return c->fail(DEME_ERR_CAPACITY, "candidate rows could not be sized");
+        c->slabCap = (uint32_t)std::min<uint64_t>(((uint64_t)h.maxOcc * 5 / 4 + 7) & ~7ull, 65535);
+        DEME_HIP(c, c->survSlab.ensure((size_t)nS * c->slabCap, 1.0));
+        DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
+        DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
+    }
+    c->nCand = c->hctr->total[0];
+    if (c->nCand > 0xFFFFFFFFull) return c->fail(DEME_ERR_CAPACITY, "candidate count exceeds 2^32 - 1");
+    DEME_HIP(c, c->candB.ensure(std::max<size_t>(c->nCand, 1)));
+    DEME_LAUNCH(c, "cand_rows",
+                launch_cand_rows(nS, c->rowSlab.p, c->posOf.p, c->candStart.p, c->candB.p, c->sorted.p, c->pos0.p,
+                                 c->stream));
+    c->candValid = true;
+    c->rebuildNext = false;
+    c->candBuilds++;
+    return DEME_OK;
+}
+
+// Buffers of a candidate-list pass (the lists themselves come from build_candidates).
+static int skin_buffers(deme_ctx* c) {
+    const uint32_t nS = c->nS;
+    DEME_HIP(c, c->pos.ensure(nS));
+    DEME_HIP(c, c->meta.ensure(nS));
+    DEME_HIP(c, c->aCnt.ensure(nS));
+    DEME_HIP(c, c->planeBits.ensure(nS));
+    DEME_HIP(c, c->scanTmp.ensure(nS));
+    DEME_HIP(c, c->scan2Sums.ensure(2 * ((nS + SCAN2_TILE - 1) / SCAN2_TILE + 1)));
+    DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>(nS, 1u << 22)), 1.0));
+    DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
+    DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
+    return DEME_OK;
+}
+
+// One candidate-list pass up to the row scans: plane poses, counters and mirror counts cleared, per-sphere
+// positions (with the displacement since the build), the predicate on the candidates, the row and mirror-row scans.
+static int enqueue_skin_pass(deme_ctx* c, int cu) {
+    const uint32_t nS = c->nS;
+    DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
+    static_assert(sizeof(DevCounters) % 4 == 0, "counters are cleared as 32-bit words");
+    DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, nS, 0, c->stream, reinterpret_cast<uint32_t*>(c->ctr.p),
+                                           sizeof(DevCounters) / 4));
+    DEME_LAUNCH(c, "sphere_pos",
+                launch_sphere_pos(c->kp, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->maskAny ? c->mask.p : nullptr,
+                                  c->pos.p, c->pos0.p, c->meta.p, c->numBins.p, c->numAnal.p, c->planeBits.p, c->ctr.p,
+                                  c->stream));
+    DEME_LAUNCH(c, "cand_test",
+                launch_cand_test(c->kp, nS, c->pos.p, c->meta.p, c->candStart.p, c->candB.p,
+                                 c->maskAny ? c->mask.p : nullptr, stale_limit2(c), c->slabCap, c->rowSlab.p,
+                                 c->mirSlab.p, c->aCnt.p, c->bCnt.p, c->ctr.p, c->stream));
+    if ((uint64_t)nS <= SCAN2_TILE * SCAN_TILE) {
+        DEME_LAUNCH(c, "scan_rows",
+                    scan2_u32_gather(c->aCnt.p, c->numAnal.p, c->bCnt.p, nullptr, nS, c->rowStart[cu].p,
+                                     c->bRowStart.p, &c->ctr.p->total[1], &c->ctr.p->total[2], c->scanTmp.p,
+                                     c->scan2Sums.p, c->stream));
+    } else {
+        DEME_LAUNCH(c, "scan_rows",
+                    scan_u32(c->aCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1], c->scanScratch.p,
+                             c->stream));
+        DEME_LAUNCH(c, "scan_brows",
+                    scan_u32(c->bCnt.p, nullptr, nS, c->bRowStart.p, &c->ctr.p->total[2], c->scanScratch.p,
+                             c->stream));
+    }
+    return DEME_OK;
+}
+
 int deme_cd_run(deme_ctx* c) {
     c->graphGen++;
     int r = check_ready(c);
     if (r) return r;
+    const bool skin = c->skin() > 0.0;
     int mode = 0;
-    r = cd_buffers(c, &mode);
-    if (r) return r;
     const uint32_t nS = c->nS;
-    r = enqueue_cell_pass(c, mode);
-    if (r) return r;
+    if (skin) {
+        DEME_HIP(c, c->numBins.ensure(nS));
+        DEME_HIP(c, c->numAnal.ensure(nS));
+        DEME_HIP(c, c->bCnt.ensure(nS));
+        DEME_HIP(c, c->bRowStart.ensure((size_t)nS + 1));
+        DEME_HIP(c, c->rowStart[c->cur ^ 1].ensure((size_t)nS + 1));
+        DEME_HIP(c, c->rowStart[c->cur].ensure((size_t)nS + 1));
+        if (!c->candValid || c->rebuildNext) {
+            r = build_candidates(c);
+            if (r) return r;
+        }
+        r = skin_buffers(c);
+        if (r) return r;
+    } else {
+        r = cd_buffers(c, &mode);
+        if (r) return r;
+        r = enqueue_cell_pass(c, mode);
+        if (r) return r;
+    }
     // the current list becomes the previous one
     if (c->contactsReady) {
         c->cur ^= 1;
@@ -1062,7 +1213,7 @@ int deme_cd_run(deme_ctx* c) {
     const int cu = c->cur;
     SpecGuard spec;
     for (int attempt = 0;; attempt++) {
-        r = enqueue_pair_pass(c, mode, cu);
+        r = skin ? enqueue_skin_pass(c, cu) : enqueue_pair_pass(c, mode, cu);
         if (r) return r;
         // The counters go to the host without a stream-wide wait: the rows and the history map are enqueued behind
         // them with the buffers at hand (skipped on the device if the pass failed or the lists do not fit), so the
@@ -1080,6 +1231,14 @@ int deme_cd_run(deme_ctx* c) {
         }
         DEME_HIP(c, hipEventSynchronize(c->evCounts));
         const DevCounters& h = *c->hctr;
+        if (h.err == ERRBIT_STALE) {
+            // a sphere moved half the skin since the build: new lists, then the pass again
+            if (attempt >= 3) return c->fail(DEME_ERR_CAPACITY, "candidate lists went stale right after a build");
+            c->candValid = false;
+            r = build_candidates(c);
+            if (r) return r;
+            continue;
+        }
         if (h.err) {
             c->havePrev = false;
             if (h.err & ERRBIT_CAPACITY)
@@ -1087,15 +1246,23 @@ int deme_cd_run(deme_ctx* c) {
             return c->fail(DEME_ERR_CAPACITY, "a sphere left the bin grid of the simulation world");
         }
         if (h.maxOcc <= c->slabCap) break;
-        if (attempt >= 2) return c->fail(DEME_ERR_CAPACITY, "partner rows could not be sized");
+        if (attempt >= 3) return c->fail(DEME_ERR_CAPACITY, "partner rows could not be sized");
         // a sphere has more partners than the slab row holds: widen the rows and redo the pass
         c->slabCap = (uint32_t)std::min<uint64_t>(((uint64_t)h.maxOcc * 5 / 4 + 7) & ~7ull, 65535);
         DEME_HIP(c, c->survSlab.ensure((size_t)nS * c->slabCap, 1.0));
         DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
         DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
-        DEME_LAUNCH(c, "fill", launch_fill_u32(c->aCnt.p, nS, 0, c->stream));
-        DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, nS, 0, c->stream));
-        DEME_HIP(c, hipMemsetAsync(&c->ctr.p->maxOcc, 0, sizeof(uint32_t), c->stream));
+        if (!skin) {
+            DEME_LAUNCH(c, "fill", launch_fill_u32(c->aCnt.p, nS, 0, c->stream));
+            DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, nS, 0, c->stream));
+            DEME_HIP(c, hipMemsetAsync(&c->ctr.p->maxOcc, 0, sizeof(uint32_t), c->stream));
+        }
+    }
+    if (skin) {
+        c->passesSinceBuild++;
+        // rebuild ahead of time when the next pass would likely find the lists stale
+        const double lim = 0.35 * c->skin();
+        c->rebuildNext = max_disp2(*c->hctr) > lim * lim;
     }
     const uint64_t C = c->hctr->total[1], nPairs = c->hctr->total[2];
     if (C > 0xFFFFFFFFull) return c->fail(DEME_ERR_CAPACITY, "contact count exceeds contactPairs_t (2^32 - 1)");
@@ -1373,8 +1540,10 @@ static int step_graph(deme_ctx* c) {
 static std::vector<const void*> cycle_fingerprint(deme_ctx* c, int p) {
     const int q = p ^ 1;
     auto v = [](uint64_t x) { return reinterpret_cast<const void*>((uintptr_t)x); };
-    uint64_t invBits;
+    uint64_t invBits, skinBits;
     std::memcpy(&invBits, &c->grid.inv, sizeof(invBits));
+    const double skin = c->skin();
+    std::memcpy(&skinBits, &skin, sizeof(skinBits));
     return {c->sph.p,        c->own.p,        c->moi.p,         c->anal.p,        c->pose.p,       c->mats.p,
             c->mask.p,       c->rules.p,      c->ownerSphStart.p, c->cellCount.p, c->cellStart.p,  c->cellOf.p,
             c->slotOf.p,     c->posOf.p,      c->aCnt.p,        c->bCnt.p,        c->sortedMeta.p, c->sorted.p,
@@ -1387,7 +1556,8 @@ static std::vector<const void*> cycle_fingerprint(deme_ctx* c, int p) {
             reinterpret_cast<const void*>(c->jit.full), reinterpret_cast<const void*>(c->jit.terms),
             reinterpret_cast<const void*>(c->jitOwner.fn),
             v(c->slabCap),   v(c->nCells),    v(c->nS),         v(c->nO),         v(c->grid.nx),   v(c->grid.ny),
-            v(c->grid.nz),   v(invBits),      v(c->jitActive),  v(c->ownerJitActive), v(c->maskAny)};
+            v(c->grid.nz),   v(invBits),      v(c->jitActive),  v(c->ownerJitActive), v(c->maskAny),
+            c->pos.p,        c->pos0.p,       c->meta.p,        c->candStart.p,   c->candB.p,      v(skinBits)};
 }
 
 // Buffers of the list at parity p (the one the interval's pass writes; the other parity holds the current list and
@@ -1425,8 +1595,13 @@ static int cycle_capacity(deme_ctx* c, int p, SpecGuard& g) {
 // steps (the first gathers the history through the map), each advancing the device clock by h.
 static int enqueue_cycle(deme_ctx* c, int f, const SpecGuard& g, int mode) {
     const int p = c->cur, q = p ^ 1;
-    int r = enqueue_cell_pass(c, mode);
-    if (!r) r = enqueue_pair_pass(c, mode, p);
+    int r;
+    if (c->skin() > 0.0) {
+        r = enqueue_skin_pass(c, p);
+    } else {
+        r = enqueue_cell_pass(c, mode);
+        if (!r) r = enqueue_pair_pass(c, mode, p);
+    }
     if (!r) r = enqueue_rows(c, g);
     if (r) return r;
     const uint32_t capC = (uint32_t)g.capC;
@@ -1452,7 +1627,7 @@ static int enqueue_cycle(deme_ctx* c, int f, const SpecGuard& g, int mode) {
 static int capture_cycle(deme_ctx* c, int p, int f) {
     deme_ctx::Cycle& cy = c->cycle[p];
     int mode = 0;
-    int r = cd_buffers(c, &mode);
+    int r = c->skin() > 0.0 ? skin_buffers(c) : cd_buffers(c, &mode);
     if (r) return r;
     SpecGuard g;
     r = cycle_capacity(c, p, g);
@@ -1516,7 +1691,13 @@ static int run_cycle(deme_ctx* c, int f, bool* done) {
     const SpecGuard& g = cy.guard;
     if (h.err || h.maxOcc > g.slabCap || h.total[1] > g.capC || h.total[2] > g.capP) {
         c->cycleFallbacks++;
+        if (h.err & ERRBIT_STALE) c->rebuildNext = true;  // the direct interval starts with new candidate lists
         return DEME_OK;
+    }
+    if (c->skin() > 0.0) {
+        c->passesSinceBuild++;
+        const double lim = 0.35 * c->skin();
+        c->rebuildNext = max_disp2(h) > lim * lim;
     }
     // the bookkeeping of a detection pass and f steps
     c->cur = p;
@@ -1549,7 +1730,9 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
         const bool detect = c->nSteps % f == 0 || !c->contactsReady || c->needDetect;
         // a whole interval from its graph: due for detection, a current list to carry the history over, all f steps
         // within this call, and no per-contact records wanted on its last step
-        if (detect && c->useGraphs && c->contactsReady && !c->needDetect && !c->timing && !c->jitStale && c->stream &&
+        const bool listsReady = c->skin() <= 0.0 || (c->candValid && !c->rebuildNext);
+        if (detect && listsReady && c->useGraphs && c->contactsReady && !c->needDetect && !c->timing && !c->jitStale &&
+            c->stream &&
             c->nS > 0 && (uint64_t)(nSteps - k) >= f && f <= 1000 && !(c->keepRecords && (uint64_t)(nSteps - k) == f)) {
             bool done = false;
             if ((r = run_cycle(c, (int)f, &done))) return r;
@@ -1601,6 +1784,23 @@ int deme_set_graph_headroom(deme_ctx* c, double factor, uint64_t slack) {
     return DEME_OK;
 }
 
+int deme_set_skin(deme_ctx* c, double skin) {
+    if (!(skin >= -1.0) || !std::isfinite(skin))
+        return c->fail(DEME_ERR_INVALID_ARGUMENT, "skin must be a finite length >= 0 (or -1: automatic)");
+    c->skinSet = skin < 0.0 ? -1.0 : skin;
+    c->skinAuto = 0.0;
+    c->candValid = false;
+    c->graphGen++;
+    return DEME_OK;
+}
+
+int deme_candidate_stats(deme_ctx* c, double* skin, uint64_t* nCandidates, uint64_t* builds) {
+    if (skin) *skin = c->skin();
+    if (nCandidates) *nCandidates = c->nCand;
+    if (builds) *builds = c->candBuilds;
+    return DEME_OK;
+}
+
 int deme_set_pair_sweep(deme_ctx* c, int mode, int block) {
     if (mode < -1 || mode > 2) return c->fail(DEME_ERR_INVALID_ARGUMENT, "pair sweep mode must be -1, 0, 1 or 2");
     if (block != 0 && block != 64 && block != 128 && block != 256)
@@ -1628,6 +1828,7 @@ int deme_set_cell_scale(deme_ctx* c, double scale) {
 
 int deme_set_beta(deme_ctx* c, float beta) {
     c->graphGen++;
+    c->candValid = false;  // positions, radii or the margin changed: candidate lists are rebuilt
     c->adjReady = false;  // recorded adjoint checkpoints no longer describe the system
     if (!(beta >= 0.f) || !std::isfinite(beta)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "beta must be >= 0");
     c->sp.beta = beta;
@@ -1787,6 +1988,7 @@ static int adj_save(deme_ctx* c, deme_ctx::AdjCk& k, bool full) {
 
 static int adj_restore(deme_ctx* c, const deme_ctx::AdjCk& k) {
     c->graphGen++;
+    c->candValid = false;  // positions, radii or the margin changed: candidate lists are rebuilt
     DEME_HIP(c, hipMemcpyAsync(c->own.p, k.own.p, c->nO * sizeof(OwnerRec), hipMemcpyDeviceToDevice, c->stream));
     const int cu = c->cur;
     if (k.haveList) {
@@ -2165,6 +2367,7 @@ int deme_get_counts(deme_ctx* c, deme_counts* out) {
 int deme_append_clumps(deme_ctx* c, uint32_t nNewSph, const uint32_t* sphOwner, const float* sphRadius,
                        const float* sphRelPos, const uint8_t* sphMat, uint32_t nNew, const float* ownerMass,
                        const float* ownerMoi, const deme_owner_state* s) {
+    c->candValid = false;  // positions, radii or the margin changed: candidate lists are rebuilt
     int r = check_ready(c);
     if (r) return r;
     if (c->nScenes != 1) return c->fail(DEME_ERR_UNSUPPORTED, "clumps cannot be appended to a batch of scenes");
@@ -2312,6 +2515,7 @@ int deme_append_clumps(deme_ctx* c, uint32_t nNewSph, const uint32_t* sphOwner, 
 }
 
 int deme_change_clump_sizes(deme_ctx* c, uint32_t n, const uint32_t* ids, const float* factors) {
+    c->candValid = false;  // positions, radii or the margin changed: candidate lists are rebuilt
     int r = check_ready(c);
     if (r) return r;
     if (n && (!ids || !factors)) return c->fail(DEME_ERR_INVALID_ARGUMENT, "ids and factors must not be NULL");

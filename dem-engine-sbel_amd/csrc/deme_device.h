@@ -216,9 +216,23 @@ struct DevCounters {
     uint32_t pad;
     unsigned long long nActive;
     unsigned long long total[4];
+    // largest squared displacement (bits of a non-negative double) of a sphere since the candidate lists were built,
+    // spread over DISP_SLOTS words (workgroup b folds into word b % DISP_SLOTS: one address per workgroup would
+    // serialize the atomics); the largest word is the maximum
+    unsigned long long disp2[64];
 };
+constexpr uint32_t DISP_SLOTS = 64;
 
-enum : uint32_t { ERRBIT_CAPACITY = 1u, ERRBIT_PAIR_OVERFLOW = 2u, ERRBIT_OUTSIDE = 4u };
+__host__ __device__ inline double max_disp2(const DevCounters& c) {
+    unsigned long long m = 0;
+    for (uint32_t k = 0; k < DISP_SLOTS; k++) m = c.disp2[k] > m ? c.disp2[k] : m;
+    double d;
+    __builtin_memcpy(&d, &m, sizeof(d));
+    return d;
+}
+
+// ERRBIT_STALE: a sphere moved too far since the candidate lists were built; they must be rebuilt and the pass redone
+enum : uint32_t { ERRBIT_CAPACITY = 1u, ERRBIT_PAIR_OVERFLOW = 2u, ERRBIT_OUTSIDE = 4u, ERRBIT_STALE = 8u };
 
 // Kernels enqueued before the host has read a detection pass's counters (so the GPU does not wait for the host, or
 // the whole step sits in a captured graph) exit at once, writing nothing, when the pass failed, a slab row

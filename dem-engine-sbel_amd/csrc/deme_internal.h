@@ -18,6 +18,7 @@ using deme_dev::spec_skip;
 using deme_dev::ERRBIT_CAPACITY;
 using deme_dev::ERRBIT_PAIR_OVERFLOW;
 using deme_dev::ERRBIT_OUTSIDE;
+using deme_dev::ERRBIT_STALE;
 using deme_dev::ForceRec;
 using deme_dev::KParams;
 using deme_dev::OwnerRec;
@@ -133,17 +134,36 @@ hipError_t launch_cell_scatter(const KParams& P, const SphereRec* sph, const Own
 // pair's distance, so the float test only rejects pairs the exact test would reject. mode (sortedF needed for 1, 2):
 // 0 the exact test on every candidate; 1 a float pre-reject, the exact test on its survivors in the sweep; 2 the float
 // test alone in the sweep, the same-owner rule and the exact test in the block's shared second phase.
-// block: workgroup size, 64, 128 or 256 (hipErrorInvalidValue otherwise).
+constexpr int PAIR_MODE_CAND = 3;
+// mode 3 (PAIR_MODE_CAND) builds candidate lists: pairs of different owners within `skin` of touching (d <= r'A + r'B + skin), no mask
+// and no bin rule. block: workgroup size, 64, 128 or 256 (hipErrorInvalidValue otherwise).
 hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, const CdRec* sorted,
                              const float4* sortedF, float margin, int mode, uint32_t block, const uint2* sortedMeta,
                              const uint32_t* cellStart, const uint8_t* mask, uint32_t slabCap,
                              uint32_t* surv, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
-                             DevCounters* ctr, hipStream_t st);
+                             DevCounters* ctr, hipStream_t st,
+                             double skin = 0.0);
 hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                             const uint32_t* rowSlab, const uint32_t* mirSlab, const uint32_t* posOf,
                             const uint32_t* rowStart, const unsigned long long* planeBits, const uint32_t* bRowStart,
                             uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* bPart, hipStream_t st,
                             SpecGuard guard = SpecGuard());
+// Candidate-list path. Per step: detection records in id order (pos) with (owner, family) (meta), the shared
+// per-sphere detection outputs, and the largest squared displacement since the lists were built into ctr->disp2.
+hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                             const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos, const CdRec* pos0,
+                             uint2* meta, uint16_t* numBins, uint32_t* numAnal, unsigned long long* planeBits,
+                             DevCounters* ctr, hipStream_t st);
+// After a candidate build: per sphere its candidate row (candStart from a scan of the build's row counts) and its
+// position at the build (pos0).
+hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* posOf, const uint32_t* candStart,
+                            uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st);
+// Per step: the reference's predicate on the candidates into id-indexed slab columns (aCnt written, bCnt counted up
+// from zero); ERRBIT_STALE instead when ctr->disp2 exceeds dispLimit2.
+hipError_t launch_cand_test(const KParams& P, uint32_t nS, const CdRec* pos, const uint2* meta,
+                            const uint32_t* candStart, const uint32_t* candB, const uint8_t* mask, double dispLimit2,
+                            uint32_t slabCap, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
+                            DevCounters* ctr, hipStream_t st);
 
 // dyn.hip
 // aTerm / gTerm: 6 floats per contact; bTerm: 6 floats per mirror entry; out (optional): full contact records

@@ -149,7 +149,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce(const uint32_t* __r
     for (int i = 0; i < SCAN2_ITEMS; i++) {
         const uint64_t k = base + (uint64_t)i * SCAN_BLOCK + threadIdx.x;
         if (k < n) {
-            const uint32_t p = idx[k];
+            const uint32_t p = idx ? idx[k] : (uint32_t)k;
             const uint32_t v1 = a1[p] + (b1 ? b1[k] : 0u), v2 = a2[p];
             tmp[k] = make_uint2(v1, v2);
             s1 += v1;

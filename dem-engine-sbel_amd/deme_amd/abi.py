@@ -90,7 +90,7 @@ EXPORTED = [
     "deme_change_clump_sizes",
     "deme_clear_motion_rules", "deme_change_family", "deme_jit_render_owner", "deme_jit_render_inspect",
     "deme_prefix_scan_u32", "deme_integrate_owners", "deme_prefix_scan_u32_noalloc", "deme_set_graph_headroom",
-    "deme_graph_stats", "deme_set_pair_sweep", "deme_autotune_sweep",
+    "deme_graph_stats", "deme_set_pair_sweep", "deme_autotune_sweep", "deme_set_skin", "deme_candidate_stats",
 ]
 
 _lib = None
@@ -140,6 +140,8 @@ def load(path: str = LIB_PATH):
         "deme_set_graph_headroom": ([vp, C.c_double, u64], i32),
         "deme_graph_stats": ([vp, vp], i32),
         "deme_set_pair_sweep": ([vp, i32, i32], i32),
+        "deme_set_skin": ([vp, C.c_double], i32),
+        "deme_candidate_stats": ([vp, vp, vp, vp], i32),
         "deme_autotune_sweep": ([vp, vp, i32, vp, i32, i32, i32, C.POINTER(i32), C.POINTER(i32), vp], i32),
         "deme_force": ([vp], i32), "deme_reduce_owner": ([vp], i32), "deme_integrate": ([vp], i32),
         "deme_integrate_owners": ([vp], i32),
@@ -381,6 +383,17 @@ class DeviceSystem:
 
     def set_graph_headroom(self, factor, slack):
         self._chk(self.lib.deme_set_graph_headroom(self.ctx, float(factor), int(slack)))
+
+    def set_skin(self, skin):
+        """Candidate-list skin in metres (0: the cell sweep every pass; -1: automatic, 0.2 x the mean radius)."""
+        self._chk(self.lib.deme_set_skin(self.ctx, float(skin)))
+
+    def candidate_stats(self):
+        """(skin in force, candidate pairs of the last build, builds so far)"""
+        sk = C.c_double()
+        n, b = C.c_uint64(), C.c_uint64()
+        self._chk(self.lib.deme_candidate_stats(self.ctx, C.byref(sk), C.byref(n), C.byref(b)))
+        return sk.value, n.value, b.value
 
     def set_pair_sweep(self, mode, block=0):
         """Pair sweep of the cell-list detection: mode 0 exact, 1 float pre-reject, 2 float sweep, -1 automatic;

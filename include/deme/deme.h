@@ -147,6 +147,18 @@ int deme_set_step_graphs(deme_ctx* ctx, int on);
 /* Capacity a captured interval gets over the current list: factor x the contact (and mirror-row) count + slack
  * (defaults 1.25 and 1024). */
 int deme_set_graph_headroom(deme_ctx* ctx, double factor, uint64_t slack);
+/* Candidate lists for contact detection (no counterpart in the reference, whose bins are re-binned every pass):
+ * a build finds every pair of different owners within `skin` (metres) of touching; until some sphere has moved half
+ * the skin since, a detection pass evaluates the reference's predicate (expanded overlap, family mask, contact-point
+ * bin rule) on those pairs only, which are all the pairs that can satisfy it -- the contact lists are the same as
+ * the full cell sweep's. The pass measures the largest displacement itself; a pass that finds the lists stale
+ * rebuilds them and repeats, and a displacement past 0.35 x skin rebuilds before the next pass. skin < 0 (default):
+ * automatic, starting at 0.2 x the mean sphere radius and adapted at each build to how many passes the previous lists
+ * lasted (widened 1.5x below 8, up to 0.8 x the mean radius; narrowed 0.75x above 400, down to 0.05 x); 0: off (the
+ * cell sweep every pass). */
+int deme_set_skin(deme_ctx* ctx, double skin);
+/* The skin in force, the candidate pairs of the last build and the number of builds so far (each nullable). */
+int deme_candidate_stats(deme_ctx* ctx, double* skin, uint64_t* nCandidates, uint64_t* builds);
 /* How the pair sweep of the cell-list detection tests its candidates, and its workgroup size (no counterpart in the
  * reference; the contact lists are the same in every configuration): mode 0 the double-precision overlap test on
  * every candidate; 1 a float pre-reject (rounding margin included) and the exact test on its survivors; 2 the float

@@ -58,3 +58,34 @@ def test_sweep_autotuner_keeps_an_equivalent_configuration():
     sa, sb = a.get_state(), b.get_state()
     for k in ("voxelID", "locX", "locY", "locZ", "vX", "vY", "vZ"):
         assert np.array_equal(sa[k], sb[k]), k
+
+
+KEYS = ("voxelID", "locX", "locY", "locZ", "oriQw", "oriQx", "oriQy", "oriQz", "vX", "vY", "vZ", "omgX", "omgY", "omgZ")
+
+
+@pytest.mark.parametrize("skin,freq,graphs", [(None, 1, False), (2e-5, 1, False), (None, 3, True), (2e-5, 2, True)])
+def test_candidate_lists_give_the_cell_sweeps_trajectory(skin, freq, graphs):
+    """Detection from candidate lists (default skin, and a skin so thin that passes find the lists stale and rebuild
+    them) against the full cell sweep every pass: identical contact lists, maps and trajectories, bit for bit, also
+    with whole intervals replayed from graphs."""
+    sc = scenes.sphere_settling(n=6000, spacing=2.02, velocity_sigma=0.5, omega_sigma=3.0, seed=1032,
+                                beta=(2e-4 if freq > 1 else 0.0))
+    a, b = DeviceSystem(sc), DeviceSystem(sc)
+    if skin is not None:
+        a.set_skin(skin)
+    b.set_skin(0.0)
+    a.set_step_graphs(graphs)
+    for chunk in (freq * 10, 7, freq * 15):
+        a.step(chunk, freq)
+        b.step(chunk, freq)
+        sa, sb = a.get_state(), b.get_state()
+        for k in KEYS:
+            assert np.array_equal(sa[k], sb[k]), (chunk, k)
+        for x, y in zip(a.contacts(), b.contacts()):
+            assert np.array_equal(x, y)
+    sk, n_cand, builds = a.candidate_stats()
+    print(f"skin {sk:.3e} m, {n_cand} candidates at the last build, {builds} builds, {a.counts().nContacts} contacts",
+          flush=True)
+    assert sk > 0 and builds >= 1 and n_cand >= a.counts().nContacts - a.counts().nSphereGeoContacts
+    if skin is not None:
+        assert builds >= 3  # the thin skin went stale and was rebuilt
