@@ -32,6 +32,7 @@ def note(msg):
 def bytes_model(name, k):
     """Algorithmic bytes per launch of each kernel (counts k of the step it ran in)."""
     S, O, P, C, Np, Ba, nb, Nc = k["S"], k["O"], k["P"], k["C"], k["pairs"], k["Ba"], k["nbins"], k["cells"]
+    Nk = k.get("cand", 0)
     owner_cd = 8 + 6 + 16 + 1          # voxel id, sub-voxel position, quaternion, family
     owner_dyn = 8 + 6 + 16 + 12 + 12 + 4  # + velocity, angular velocity, mass
     sph = 4 + 4 + 12 + 4               # owner, radius, offset, material
@@ -68,6 +69,18 @@ def bytes_model(name, k):
         "cell_pairs": S * (32 + 4 + 4) + O * 4 + Nc * 4 + Np * (4 + 4 + 8),
         "cell_rows": S * (4 + 8) + Np * 4 + C * 9,
         "fill": (Nc + O + 1) * 4 / 2,  # two launches per pass: cell counts and owner B-side counts
+        # candidate lists: per sphere template + owner pose and the build position in; CD record, (owner, family),
+        # bin and plane counts and plane bits out
+        "sphere_pos": S * sph + O * owner_cd + S * 32 + S * (32 + 8 + 2 + 4 + 8),
+        # per sphere its record, (owner, family), row bounds and count out; per candidate its id and the partner's
+        # record (the footprint of the records is the S * 32 already counted); per contact a row entry, a mirror entry
+        # and a 4-byte RMW of the partner's count
+        "cand_test": S * (32 + 8 + 4 + 4) + Nk * 4 + Np * (4 + 4 + 4),
+        # the build sweep: as the cell sweep's pass with every candidate kept
+        "cand_build": S * (32 + 4 + 4) + O * 4 + Nc * 4 + Nk * (4 + 4 + 8),
+        "scan_cand": S * 4 * 2,
+        # candidate rows copied (sorted) from the slab into id order, build positions out
+        "cand_rows": S * (4 + 4 + 32 + 32) + Nk * (4 + 4),
         "anal_final": k["anal_blocks"] * 48 * 4,
         # per owner: own record in/out, MOI, accelerations out; A rows contiguous, B lists gathered
         "owner_update": O * (64 + 64 + 16 + 24 + 8) + C * 48 + Np * (4 + 48),
@@ -130,11 +143,13 @@ def run_device(sc, steps, warmup, update_freq, jit=False, inspect=False, prepare
     dev.enable_timing(False)
     timing = dev.kernel_timing()
     c = dev.counts()
+    skin, n_cand, builds = dev.candidate_stats()
     insp = run_inspectors(dev, sc) if inspect else None
     info = dict(S=sc.n_spheres, O=sc.n_owners, P=c.nBinSphereTouches, C=c.nContacts, Cprev=c.nPrevContacts,
                 pairs=c.nContacts - c.nSphereGeoContacts, geo=c.nSphereGeoContacts, Ba=c.nActiveBins,
                 nbins=sc.params.nbX * sc.params.nbY * sc.params.nbZ, cells=c.nCells, fresh=update_freq == 1,
-                anal_blocks=(c.nContacts + ANAL_TILE - 1) // ANAL_TILE, inspectors=insp,
+                anal_blocks=(c.nContacts + ANAL_TILE - 1) // ANAL_TILE, cand=n_cand, skin=skin, cand_builds=builds,
+                inspectors=insp,
                 evented_ms_per_step=1e3 * (te1 - te0) / steps)
     if keep_state:
         info["state"] = dev.get_state()
@@ -214,8 +229,10 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05, cpu_
     f32 = np.float32
     beta = float(f32(vmax) * f32(sc.params.h) * f32(update_freq) * f32(1.2)) if update_freq > 1 else 0.0
     scale = (1.0, 1.25)[int(np.argmin(table[[1, 2, 5, 10].index(update_freq)]))]
-    # second stage: the pair sweep's mode and workgroup size at that interval
-    sweep_mode, sweep_block, sweep_table = dev.autotune_sweep(trial_steps=update_freq, update_freq=update_freq)
+    # second stage: candidate lists (automatic skin) or the cell sweep every pass, the sweep's mode and workgroup
+    # size, at that interval
+    sweep_skin, sweep_mode, sweep_block, sweep_table = dev.autotune_sweep(
+        skins=(-1.0, 0.0), modes=(0, 2), blocks=(128, 256), trial_steps=2 * update_freq, update_freq=update_freq)
     dev.step(warmup, update_freq)
     dev.sync()
     t0 = time.perf_counter()
@@ -243,6 +260,7 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05, cpu_
     for d in parts:
         d.set_beta(beta)
         d.set_cell_scale(scale)
+        d.set_skin(sweep_skin)
         d.set_pair_sweep(sweep_mode, sweep_block)
         d.step(warmup, update_freq)
     for d in parts:
@@ -259,8 +277,8 @@ def run_batch(n_scenes, steps, warmup, update_freq, beta, pitch_scale=1.05, cpu_
             "update_freq": update_freq,
             "beta_m": beta, "cell_scale": scale, "autotune_ms_per_step": table.round(4).tolist(), "steps": steps,
             "scene_steps_per_s": n_scenes * steps / t_batch, "ms_per_step": 1e3 * t_batch / steps,
-            "pair_sweep_mode_block": [sweep_mode, sweep_block],
-            "pair_sweep_autotune_ms_per_step": sweep_table.round(4).tolist(),
+            "detection_skin_mode_block": [sweep_skin, sweep_mode, sweep_block],
+            "detection_autotune_ms_per_step_skin_mode_block": sweep_table.round(4).tolist(),
             "graph_replay_ms_per_step": 1e3 * t_graph / steps,
             "graph_captures_replays_fallbacks_in_timed_steps": list(graph_stats),
             "loop_scene_steps_per_s": n_scenes * steps / t_loop, "loop_ms_per_step": 1e3 * t_loop / steps,
@@ -566,6 +584,8 @@ def main():
                    "clumps": args.clumps, "spheres": scA.n_spheres, "contacts": infoA["C"],
                    "ms_per_step_with_kernel_events": round(infoA["evented_ms_per_step"], 4),
                    "bin_sphere_touches": infoA["P"], "active_bins": infoA["Ba"],
+                   "candidate_lists": {"skin_m": infoA["skin"], "candidates": infoA["cand"],
+                                       "builds": infoA["cand_builds"]},
                    "run_every5_beta0.2r": runB, "jit_specialised": jit, "batched_columns": batch,
                    "batched_columns_loaded": batch_loaded,
                    "adjoint_drop": adjoint,

@@ -113,6 +113,9 @@ struct deme_ctx {
         std::vector<const void*> ptrs;
     } cycle[2];
     uint64_t cycleReplays = 0, cycleFallbacks = 0, cycleCaptures = 0;
+    // After a replay falls back, the next cycleSkip intervals run directly; the run doubles with each fallback in a
+    // row (up to 64 intervals) and resets on a replay that completes.
+    uint32_t cycleSkip = 0, cycleBackoff = 0;
     // Candidate-list detection: pairs within `skin` of touching, kept from a build until some sphere has moved half
     // the skin; each pass then tests only them. skinSet < 0: automatic (0.2 x the mean radius); 0: off (the cell
     // sweep every pass).
@@ -1691,9 +1694,12 @@ static int run_cycle(deme_ctx* c, int f, bool* done) {
     const SpecGuard& g = cy.guard;
     if (h.err || h.maxOcc > g.slabCap || h.total[1] > g.capC || h.total[2] > g.capP) {
         c->cycleFallbacks++;
+        c->cycleBackoff = std::min<uint32_t>(std::max<uint32_t>(2 * c->cycleBackoff, 1), 64);
+        c->cycleSkip = c->cycleBackoff;
         if (h.err & ERRBIT_STALE) c->rebuildNext = true;  // the direct interval starts with new candidate lists
         return DEME_OK;
     }
+    c->cycleBackoff = 0;
     if (c->skin() > 0.0) {
         c->passesSinceBuild++;
         const double lim = 0.35 * c->skin();
@@ -1731,9 +1737,14 @@ int deme_step(deme_ctx* c, int nSteps, int updateFreq) {
         // a whole interval from its graph: due for detection, a current list to carry the history over, all f steps
         // within this call, and no per-contact records wanted on its last step
         const bool listsReady = c->skin() <= 0.0 || (c->candValid && !c->rebuildNext);
-        if (detect && listsReady && c->useGraphs && c->contactsReady && !c->needDetect && !c->timing && !c->jitStale &&
-            c->stream &&
-            c->nS > 0 && (uint64_t)(nSteps - k) >= f && f <= 1000 && !(c->keepRecords && (uint64_t)(nSteps - k) == f)) {
+        bool cycle = detect && listsReady && c->useGraphs && c->contactsReady && !c->needDetect && !c->timing &&
+                     !c->jitStale && c->stream && c->nS > 0 && (uint64_t)(nSteps - k) >= f && f <= 1000 &&
+                     !(c->keepRecords && (uint64_t)(nSteps - k) == f);
+        if (cycle && c->cycleSkip > 0) {
+            c->cycleSkip--;
+            cycle = false;
+        }
+        if (cycle) {
             bool done = false;
             if ((r = run_cycle(c, (int)f, &done))) return r;
             if (done) {
@@ -1897,44 +1908,62 @@ int deme_autotune(deme_ctx* c, const int* freqs, int nFreqs, const double* cellS
 // Trial runs of the pair sweep's launch configurations (mode x workgroup size) on the live system at update interval
 // f, with the context's drift margin and cell scale; each trial runs trialSteps steps (rounded up to a multiple of
 // f, so every trial holds the same number of detection passes). The fastest stays configured.
-int deme_autotune_sweep(deme_ctx* c, const int* modes, int nModes, const int* blocks, int nBlocks, int trialSteps,
-                        int updateFreq, int* bestMode, int* bestBlock, double* msPerStep) {
+int deme_autotune_sweep(deme_ctx* c, const double* skins, int nSkins, const int* modes, int nModes, const int* blocks,
+                        int nBlocks, int trialSteps, int updateFreq, double* bestSkin, int* bestMode, int* bestBlock,
+                        double* msPerStep) {
     int r = check_ready(c);
     if (r) return r;
-    if (nModes <= 0 || nBlocks <= 0 || trialSteps <= 0 || !modes || !blocks)
+    if (nModes <= 0 || nBlocks <= 0 || trialSteps <= 0 || !modes || !blocks || (skins && nSkins <= 0))
         return c->fail(DEME_ERR_INVALID_ARGUMENT, "need candidates and trial steps");
+    const double keepSkin = c->skinSet;
+    if (!skins) nSkins = 1;
     const int f = std::max(updateFreq, 1);
     const int n = ((trialSteps + f - 1) / f) * f;
     hipEvent_t e0, e1;
     DEME_HIP(c, hipEventCreate(&e0));
     DEME_HIP(c, hipEventCreate(&e1));
     double best = 1e300;
+    double bs = keepSkin;
     int bm = c->pairMode, bb = (int)c->pairBlock;
-    for (int i = 0; i < nModes && !r; i++) {
-        for (int j = 0; j < nBlocks; j++) {
-            if ((r = deme_set_pair_sweep(c, modes[i], blocks[j]))) break;
-            // a pass with this configuration first (buffers sized, code loaded), then the timed steps
-            if ((r = deme_step(c, f, f))) break;
-            DEME_HIP(c, hipEventRecord(e0, c->stream));
-            if ((r = deme_step(c, n, f))) break;
-            DEME_HIP(c, hipEventRecord(e1, c->stream));
-            DEME_HIP(c, hipEventSynchronize(e1));
-            float ms = 0.f;
-            DEME_HIP(c, hipEventElapsedTime(&ms, e0, e1));
-            const double per = (double)ms / n;
-            if (msPerStep) msPerStep[i * nBlocks + j] = per;
-            if (per < best) {
-                best = per;
-                bm = modes[i];
-                bb = blocks[j];
+    for (int k = 0; k < nSkins && !r; k++) {
+        if (skins && (r = deme_set_skin(c, skins[k]))) break;
+        // with candidate lists the sweep runs only in its build mode: one mode's timings stand for all
+        const bool cand = c->skin() > 0.0;
+        for (int i = 0; i < nModes && !r; i++) {
+            for (int j = 0; j < nBlocks; j++) {
+                if (cand && i > 0) {
+                    if (msPerStep)
+                        msPerStep[((size_t)k * nModes + i) * nBlocks + j] = msPerStep[(size_t)k * nModes * nBlocks + j];
+                    continue;
+                }
+                if ((r = deme_set_pair_sweep(c, modes[i], blocks[j]))) break;
+                // a pass with this configuration first (buffers sized, lists built, code loaded), then the timed steps
+                if ((r = deme_step(c, f, f))) break;
+                DEME_HIP(c, hipEventRecord(e0, c->stream));
+                if ((r = deme_step(c, n, f))) break;
+                DEME_HIP(c, hipEventRecord(e1, c->stream));
+                DEME_HIP(c, hipEventSynchronize(e1));
+                float ms = 0.f;
+                DEME_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+                const double per = (double)ms / n;
+                if (msPerStep) msPerStep[((size_t)k * nModes + i) * nBlocks + j] = per;
+                if (per < best) {
+                    best = per;
+                    bs = skins ? skins[k] : keepSkin;
+                    bm = modes[i];
+                    bb = blocks[j];
+                }
             }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (r) return r;
+    // the adapted automatic skin is kept when the automatic skin wins its last trial
+    if (skins && !(bs < 0.0 && c->skinSet < 0.0) && (r = deme_set_skin(c, bs))) return r;
     r = deme_set_pair_sweep(c, bm, bb);
     if (r) return r;
+    if (bestSkin) *bestSkin = bs;
     if (bestMode) *bestMode = bm;
     if (bestBlock) *bestBlock = bb;
     return DEME_OK;

@@ -142,7 +142,8 @@ def load(path: str = LIB_PATH):
         "deme_set_pair_sweep": ([vp, i32, i32], i32),
         "deme_set_skin": ([vp, C.c_double], i32),
         "deme_candidate_stats": ([vp, vp, vp, vp], i32),
-        "deme_autotune_sweep": ([vp, vp, i32, vp, i32, i32, i32, C.POINTER(i32), C.POINTER(i32), vp], i32),
+        "deme_autotune_sweep": ([vp, vp, i32, vp, i32, vp, i32, i32, i32, C.POINTER(C.c_double), C.POINTER(i32),
+                                 C.POINTER(i32), vp], i32),
         "deme_force": ([vp], i32), "deme_reduce_owner": ([vp], i32), "deme_integrate": ([vp], i32),
         "deme_integrate_owners": ([vp], i32),
         "deme_step": ([vp, i32, i32], i32),
@@ -400,17 +401,22 @@ class DeviceSystem:
         block: workgroup size 64, 128 or 256 (0 keeps it)."""
         self._chk(self.lib.deme_set_pair_sweep(self.ctx, int(mode), int(block)))
 
-    def autotune_sweep(self, modes=(0, 1, 2), blocks=(64, 128, 256), trial_steps=10, update_freq=1):
-        """Times the pair sweep's configurations on the live system and keeps the fastest; returns
-        (mode, block, table of ms per step, modes x blocks)."""
+    def autotune_sweep(self, modes=(0, 1, 2), blocks=(64, 128, 256), trial_steps=10, update_freq=1, skins=None):
+        """Times the detection configurations (skins x pair sweep modes x workgroup sizes; skins None: the skin in
+        force) on the live system and keeps the fastest; returns (skin, mode, block, table of ms per step)."""
         m = np.asarray(modes, np.int32)
         b = np.asarray(blocks, np.int32)
-        table = np.zeros((len(m), len(b)), np.float64)
-        bm, bb = C.c_int32(), C.c_int32()
-        self._chk(self.lib.deme_autotune_sweep(self.ctx, C.c_void_p(m.ctypes.data), len(m), C.c_void_p(b.ctypes.data),
-                                               len(b), int(trial_steps), int(update_freq), C.byref(bm), C.byref(bb),
-                                               C.c_void_p(table.ctypes.data)))
-        return bm.value, bb.value, table
+        sk = None if skins is None else np.asarray(skins, np.float64)
+        ns = 1 if sk is None else len(sk)
+        table = np.zeros((ns, len(m), len(b)), np.float64)
+        bs, bm, bb = C.c_double(), C.c_int32(), C.c_int32()
+        self._chk(self.lib.deme_autotune_sweep(self.ctx, None if sk is None else C.c_void_p(sk.ctypes.data), ns,
+                                               C.c_void_p(m.ctypes.data), len(m), C.c_void_p(b.ctypes.data), len(b),
+                                               int(trial_steps), int(update_freq), C.byref(bs), C.byref(bm),
+                                               C.byref(bb), C.c_void_p(table.ctypes.data)))
+        if sk is None:
+            return bm.value, bb.value, table[0]
+        return bs.value, bm.value, bb.value, table
 
     def graph_stats(self):
         """(captures, replays that advanced the system, replays that fell back to direct launches)"""

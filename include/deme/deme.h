@@ -187,12 +187,13 @@ int deme_set_cell_scale(deme_ctx* ctx, double scale);
 int deme_autotune(deme_ctx* ctx, const int* freqs, int nFreqs, const double* cellScales, int nScales, int trialSteps,
                   float maxVel, float safety, int* bestFreq, double* msPerStep);
 /* Second stage of the autotuner (no counterpart in the reference): times trialSteps steps at update interval
- * updateFreq (keeping the drift margin and cell scale in force) for every pair sweep configuration modes[i] x
- * blocks[j] (deme_set_pair_sweep), keeps the fastest configured and returns it. msPerStep (optional, nModes x
- * nBlocks) receives the timings. The trials advance the simulation; contact lists are the same in every
- * configuration. */
-int deme_autotune_sweep(deme_ctx* ctx, const int* modes, int nModes, const int* blocks, int nBlocks, int trialSteps,
-                        int updateFreq, int* bestMode, int* bestBlock, double* msPerStep);
+ * updateFreq (keeping the drift margin and cell scale in force) for every detection configuration skins[k] x
+ * modes[i] x blocks[j] (deme_set_skin, deme_set_pair_sweep; skins may be NULL: the skin in force), keeps the fastest
+ * configured and returns it. msPerStep (optional, nSkins x nModes x nBlocks) receives the timings. The trials advance
+ * the simulation; contact lists are the same in every configuration. */
+int deme_autotune_sweep(deme_ctx* ctx, const double* skins, int nSkins, const int* modes, int nModes,
+                        const int* blocks, int nBlocks, int trialSteps, int updateFreq, double* bestSkin,
+                        int* bestMode, int* bestBlock, double* msPerStep);
 
 /* ---- Force model and runtime specialisation (hipRTC in place of jitify) ----
  * The force kernel is one template with the placeholders of the reference's JIT programs (DEMCalcForceKernels.cu:

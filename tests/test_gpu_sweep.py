@@ -1,8 +1,8 @@
 """The pair sweep's launch configurations (deme_set_pair_sweep: exact test, float pre-reject, float sweep with the
 exact test in the shared second phase; workgroups of 64, 128 and 256 lanes) give the oracle's contact list and
 persistent-contact map bit for bit, on clumps (near-uniform radii, same-owner pairs) and on a polydisperse bed with a
-drift margin (many candidates far from touching), and the launch-configuration autotuner keeps a configuration that
-still gives the same lists."""
+drift margin (many candidates far from touching), as do candidate lists built by the sweep; the launch-configuration
+autotuner keeps a configuration that still gives the same lists."""
 import numpy as np
 import pytest
 
@@ -36,20 +36,27 @@ def test_every_sweep_configuration_gives_the_oracle_list(kind):
     ora.cd_run()
     ref = ora.contacts()
     assert len(ref[0]) > 1000
-    for mode, block in CONFIGS:
-        print(f"{kind}: sweep mode {mode}, block {block}", flush=True)
+    # the cell sweep every pass in each configuration, then candidate lists (automatic and a fixed skin) built by
+    # the sweep's build mode at each workgroup size
+    runs = [(0.0, mode, block) for mode, block in CONFIGS] + [(skin, 2, block) for skin in (-1.0, 3e-5)
+                                                               for block in (64, 128, 256)]
+    for skin, mode, block in runs:
+        print(f"{kind}: skin {skin}, sweep mode {mode}, block {block}", flush=True)
+        dev.set_skin(skin)
         dev.set_pair_sweep(mode, block)
         dev.cd_run()  # the previous list is the same each time, so the maps must agree too
         got = dev.contacts()
         for x, y in zip(got[:3], ref[:3]):
-            assert np.array_equal(x, y), (mode, block)
+            assert np.array_equal(x, y), (skin, mode, block)
         ora.cd_run()
-        assert np.array_equal(got[3], ora.contacts()[3]), (mode, block)
+        assert np.array_equal(got[3], ora.contacts()[3]), (skin, mode, block)
 
 
 def test_sweep_autotuner_keeps_an_equivalent_configuration():
     sc, f = _scene("clumps")
     a, b = DeviceSystem(sc), DeviceSystem(sc)
+    a.set_skin(0.0)  # the cell sweep every pass, where the mode and workgroup size apply
+    b.set_skin(0.0)
     a.step(10, f)
     b.step(10, f)
     mode, block, table = a.autotune_sweep(trial_steps=4, update_freq=f)
@@ -58,6 +65,24 @@ def test_sweep_autotuner_keeps_an_equivalent_configuration():
     sa, sb = a.get_state(), b.get_state()
     for k in ("voxelID", "locX", "locY", "locZ", "vX", "vY", "vZ"):
         assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_detection_autotuner_over_skins_keeps_an_equivalent_configuration():
+    sc, f = _scene("clumps")
+    a, b = DeviceSystem(sc), DeviceSystem(sc)
+    a.step(10, f)
+    b.step(10, f)
+    skin, mode, block, table = a.autotune_sweep(skins=(-1.0, 0.0), modes=(0, 2), blocks=(128, 256), trial_steps=4,
+                                                update_freq=f)
+    assert skin in (-1.0, 0.0) and mode in (0, 2) and block in (128, 256)
+    assert table.shape == (2, 2, 2) and np.all(table > 0)
+    # with candidate lists the mode is not tried (one timing stands for both): 2 trials; the cell sweep: 4
+    assert np.array_equal(table[0, 0], table[0, 1])
+    b.step(int((2 + 4) * (4 + 1)), f)
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("voxelID", "locX", "locY", "locZ", "vX", "vY", "vZ"):
+        assert np.array_equal(sa[k], sb[k]), k
+    assert a.candidate_stats()[0] > 0.0 if skin < 0 else a.candidate_stats()[0] == 0.0
 
 
 KEYS = ("voxelID", "locX", "locY", "locZ", "oriQw", "oriQx", "oriQy", "oriQz", "vX", "vY", "vZ", "omgX", "omgY", "omgZ")
