@@ -72,15 +72,24 @@ def bytes_model(name, k):
         # candidate lists: per sphere template + owner pose and the build position in; CD record, (owner, family),
         # bin and plane counts and plane bits out
         "sphere_pos": S * sph + O * owner_cd + S * 32 + S * (32 + 8 + 2 + 4 + 8),
-        # per sphere its record, (owner, family), row bounds and count out; per candidate its id and the partner's
-        # record (the footprint of the records is the S * 32 already counted); per contact a row entry, a mirror entry
-        # and a 4-byte RMW of the partner's count
-        "cand_test": S * (32 + 8 + 4 + 4) + Nk * 4 + Np * (4 + 4 + 4),
+        # per candidate its sphere and partner ids and the touching flag out; the spheres' records once
+        "cand_flags": S * 32 + Nk * (4 + 4 + 1),
+        # per sphere its row and transposed-row bounds, both counts out; per candidate its flag (row order) and its
+        # index and flag (transposed order)
+        "cand_count": S * (4 + 4 + 4 + 4) + Nk * (1 + 4 + 1),
+        # per candidate its sphere and flag; per contact the row bounds, plane count and previous flag and contact index
+        # of its candidate in, idA, idB, type, contact index and map out
+        "cand_fill": Nk * (4 + 1) + Np * (4 + 4 + 4 + 1 + 4) + Np * (4 + 4 + 1 + 4 + 4),
+        # per sphere plane bits, transposed and mirror row bounds; per candidate its transposed index and flag; per
+        # sphere-sphere contact its contact index and mirror link out
+        "cand_mirror": S * (8 + 4 + 4) + Nk * (4 + 1) + Np * (4 + 4),
+        # the build's mirror columns sorted into id order and each entry's candidate found in its partner's row
+        "cand_transpose": S * (4 + 4 + 4) + Nk * (4 + 4 + 4),
         # the build sweep: as the cell sweep's pass with every candidate kept
         "cand_build": S * (32 + 4 + 4) + O * 4 + Nc * 4 + Nk * (4 + 4 + 8),
         "scan_cand": S * 4 * 2,
-        # candidate rows copied (sorted) from the slab into id order, build positions out
-        "cand_rows": S * (4 + 4 + 32 + 32) + Nk * (4 + 4),
+        # candidate rows copied (sorted) from the slab into id order with their sphere ids, build positions out
+        "cand_rows": S * (4 + 4 + 32 + 32) + Nk * (4 + 4 + 4),
         "anal_final": k["anal_blocks"] * 48 * 4,
         # per owner: own record in/out, MOI, accelerations out; A rows contiguous, B lists gathered
         "owner_update": O * (64 + 64 + 16 + 24 + 8) + C * 48 + Np * (4 + 48),

@@ -327,7 +327,7 @@ __global__ __launch_bounds__(256) void k_row_finalize(uint32_t nS, const uint32_
 // Row a12: persistent-contact map (buildPersistentMap, DEMHistoryMappingKernels.cu:16-60) as a merge join of the
 // new and previous rows of the same sphere (both ordered by (type group, idB)). The same pass links every
 // sphere-sphere contact (s, b) to its slot in b's mirror row (the contacts where b is the B body, ordered by A id):
-// mirror[i] = bRowStart[b] + rank of s in that row.
+// mirror[i] = bRowStart[b] + rank of s in that row (skipped when mirror is null: the map alone).
 // Rows up to this long take the register path of the history map.
 constexpr uint32_t HM_REG = 8;
 
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t
         uint32_t lo[HM_REG], hi[HM_REG];
 #pragma unroll
         for (uint32_t k = 0; k < HM_REG; k++) {
-            const bool ss = k < nn && nt[k] == DEME_SPHERE_SPHERE_CONTACT && nb[k] < nS;
+            const bool ss = mirror && k < nn && nt[k] == DEME_SPHERE_SPHERE_CONTACT && nb[k] < nS;
             lo[k] = ss ? bRowStart[nb[k]] : 0u;
             hi[k] = ss ? bRowStart[nb[k] + 1] : 0u;
         }
@@ -374,6 +374,7 @@ __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t
             for (uint32_t q = 0; q < HM_REG; q++)
                 if (q < np && pb[q] == nb[k] && pt[q] == nt[k]) m = j + q;
             map[ns + k] = m;
+            if (!mirror) continue;
             uint32_t link = DEME_NULL_MAPPING_PARTNER;
             for (uint32_t q = lo[k]; q < hi[k] && link == DEME_NULL_MAPPING_PARTNER; q += 4) {
                 uint32_t v[4];
@@ -404,6 +405,7 @@ __global__ __launch_bounds__(256) void k_history_map(uint32_t nS, const uint32_t
             break;
         }
         map[i] = m;
+        if (!mirror) continue;
         uint32_t link = DEME_NULL_MAPPING_PARTNER;
         if (t == DEME_SPHERE_SPHERE_CONTACT && b < nS) {
             // B's mirror row (ascending, distinct ids) is short: read it four entries per round, all four loads in

@@ -688,65 +688,218 @@ __global__ __launch_bounds__(256) void k_cell_rows(KParams P, const SphereRec* _
 // from its slab column at its sorted position; the sphere's position at the build is kept for the displacement test.
 __global__ __launch_bounds__(256) void k_cand_rows(uint32_t nS, const uint32_t* __restrict__ rowSlab,
                                                    const uint32_t* __restrict__ posOf,
-                                                   const uint32_t* __restrict__ candStart, uint32_t* __restrict__ candB,
-                                                   const CdRec* __restrict__ sorted, CdRec* __restrict__ pos0) {
+                                                   const uint32_t* __restrict__ candStart, uint32_t* __restrict__ candA,
+                                                   uint32_t* __restrict__ candB, const CdRec* __restrict__ sorted,
+                                                   CdRec* __restrict__ pos0) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
     const uint32_t p = posOf[s];
-    const uint32_t st = candStart[s];
-    sorted_copy(rowSlab + p, candStart[s + 1] - st, nS, candB + st);
+    const uint32_t st = candStart[s], en = candStart[s + 1];
+    sorted_copy(rowSlab + p, en - st, nS, candB + st);
+    for (uint32_t c = st; c < en; c++) candA[c] = s;
     const uint4* q = reinterpret_cast<const uint4*>(sorted + p);
     uint4* d = reinterpret_cast<uint4*>(pos0 + s);
     d[0] = q[0];
     d[1] = q[1];
 }
 
-// Each step of the candidate-list path, per sphere A (id order): the reference's predicate (the exact expanded
-// overlap, the family mask, the contact-point bin rule) on A's candidates, which hold every partner that can touch
-// while no sphere has moved more than half the skin since the build. Accepted partners go to A's slab column in
-// ascending order (A's lane owns it) and A to the partner's mirror column. If a sphere moved further the pass is
-// marked stale and does nothing; the host rebuilds the lists and repeats it.
-__global__ __launch_bounds__(256) void k_cand_test(KParams P, uint32_t nS, const CdRec* __restrict__ pos,
-                                                   const uint2* __restrict__ meta,
-                                                   const uint32_t* __restrict__ candStart,
-                                                   const uint32_t* __restrict__ candB, const uint8_t* __restrict__ mask,
-                                                   double dispLimit2, uint32_t slabCap, uint32_t* __restrict__ rowSlab,
-                                                   uint32_t* __restrict__ mirSlab, uint32_t* __restrict__ aCnt,
-                                                   uint32_t* bCnt, DevCounters* ctr) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nS || ctr->err) return;
+// The candidates of one build are indexed once (candidate c: sphere candA[c], partner candB[c] of higher id; the
+// rows candStart[A]..candStart[A + 1] ascending in partner) and transposed once (tC[tStart[B]..tStart[B + 1]]: the
+// candidates whose partner is B, ascending A). Every pass between builds marks which candidates touch (acc[c]) and
+// derives the canonical rows, the mirror rows and the persistent-contact map from those indices alone: a contact of
+// this pass that was a contact of the previous pass is the same candidate, so the map is the previous pass's contact
+// index of c, and no list is sorted, merged or searched.
+
+// Per candidate: the reference's predicate (the exact expanded overlap, the family mask, the contact-point bin rule)
+// on the pair, which holds every partner that can touch while no sphere has moved more than half the skin since the
+// build. If a sphere moved further the pass is marked stale and does nothing; the host rebuilds the lists and
+// repeats it.
+__global__ __launch_bounds__(256) void k_cand_flags(KParams P, uint32_t nK, const CdRec* __restrict__ pos,
+                                                    const uint2* __restrict__ meta,
+                                                    const uint32_t* __restrict__ candA,
+                                                    const uint32_t* __restrict__ candB,
+                                                    const uint8_t* __restrict__ mask, double dispLimit2,
+                                                    uint8_t* __restrict__ acc, DevCounters* ctr) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ctr->err) return;
+    // (the grid has at least one lane, so a build without candidates is still found stale)
     if (max_disp2(*ctr) > dispLimit2) {
-        if (s == 0) atomicOr(&ctr->err, ERRBIT_STALE);
+        if (c == 0) atomicOr(&ctr->err, ERRBIT_STALE);
         return;
     }
-    const double inv = 1.0 / P.binSize;
-    const CdRec a = load_cd(pos, s);
-    const double ra = expanded_radius(P, a.r);
-    const uint32_t famA = mask ? meta[s].y : 0u;
-    const uint32_t c0 = candStart[s], c1 = candStart[s + 1];
-    uint32_t n = 0;
-    auto take = [&](uint32_t b, const CdRec& rb) {
-        if (!expanded_overlap(ra, a, expanded_radius(P, rb.r), rb)) return;
-        if (mask && mask[mask_index(famA, meta[b].y)] != DEME_DONT_PREVENT_CONTACT) return;
-        if (!bin_rule_point(P, inv, a, rb)) return;  // a has the lower id
-        if (n < slabCap) rowSlab[(uint64_t)n * nS + s] = b;
-        n++;
-        append_pair(bCnt, mirSlab, b, s, nS, slabCap, ctr);
-    };
-    uint32_t c = c0;
-    for (; c + 1 < c1; c += 2) {
-        // both candidates' records in flight before either test
-        const uint32_t b0 = candB[c], b1 = candB[c + 1];
-        const CdRec r0 = load_cd(pos, b0), r1 = load_cd(pos, b1);
-        take(b0, r0);
-        take(b1, r1);
+    if (c >= nK) return;
+    const uint32_t ia = candA[c], ib = candB[c];
+    const CdRec a = load_cd(pos, ia), b = load_cd(pos, ib);
+    uint8_t f = 0;
+    if (expanded_overlap(expanded_radius(P, a.r), a, expanded_radius(P, b.r), b) &&
+        (!mask || mask[mask_index(meta[ia].y, meta[ib].y)] == DEME_DONT_PREVENT_CONTACT))
+        f = bin_rule_point(P, 1.0 / P.binSize, a, b) ? 1 : 0;  // a has the lower id
+    acc[c] = f;
+}
+
+// Per sphere: its row's touching candidates (aCnt) and those with it as partner (bCnt, its mirror row's length).
+__global__ __launch_bounds__(256) void k_cand_count(uint32_t nS, const uint32_t* __restrict__ candStart,
+                                                    const uint32_t* __restrict__ tStart,
+                                                    const uint32_t* __restrict__ tC,
+                                                    const uint8_t* __restrict__ acc, uint32_t* __restrict__ aCnt,
+                                                    uint32_t* __restrict__ bCnt, const DevCounters* ctr) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nS || ctr->err) return;
+    uint32_t na = 0, nb = 0;
+    const uint32_t c1 = candStart[s + 1], t1 = tStart[s + 1];
+    for (uint32_t c = candStart[s]; c < c1; c++) na += acc[c];
+    uint32_t t = tStart[s];
+    for (; t + 3 < t1; t += 4) {
+        const uint32_t q0 = tC[t], q1 = tC[t + 1], q2 = tC[t + 2], q3 = tC[t + 3];
+        nb += (uint32_t)acc[q0] + (uint32_t)acc[q1] + (uint32_t)acc[q2] + (uint32_t)acc[q3];
     }
-    if (c < c1) {
-        const uint32_t b0 = candB[c];
-        take(b0, load_cd(pos, b0));
+    for (; t < t1; t++) nb += acc[tC[t]];
+    aCnt[s] = na;
+    bCnt[s] = nb;
+}
+
+// Per touching candidate: its contact in A's canonical row (after A's plane contacts, in candidate order), idA,
+// cidx[c] and, when prevAcc is given (the previous pass used the same candidates), its persistent-contact map entry:
+// the previous pass's contact index of c, or none if c did not touch then. A wave covers 64 consecutive candidates:
+// c's rank among the touching candidates of its row is a population count of the wave's ballot, plus, for a row
+// that began before the wave, the touching candidates of the row ahead of it (counted once by lane 0).
+__global__ __launch_bounds__(256) void k_cand_fill(uint32_t nK, const uint32_t* __restrict__ candA,
+                                                   const uint32_t* __restrict__ candB,
+                                                   const uint32_t* __restrict__ candStart,
+                                                   const uint8_t* __restrict__ acc,
+                                                   const uint32_t* __restrict__ rowStart,
+                                                   const uint32_t* __restrict__ numAnal,
+                                                   const uint8_t* __restrict__ prevAcc,
+                                                   const uint32_t* __restrict__ prevCidx, uint32_t* __restrict__ cidx,
+                                                   uint32_t* __restrict__ idA, uint32_t* __restrict__ idB,
+                                                   uint8_t* __restrict__ ctype, uint32_t* __restrict__ map,
+                                                   SpecGuard guard) {
+    if (spec_skip(guard)) return;  // uniform: the whole grid leaves
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w0 = c - lane;  // the wave's first candidate
+    if (w0 >= nK) return;          // whole wave past the end
+    const bool in = c < nK;
+    const uint32_t a = in ? candA[c] : 0u;
+    const bool t = in && acc[c];
+    const unsigned long long ball = __ballot(t);
+    const uint32_t r0 = in ? candStart[a] : 0u;
+    // touching candidates of the wave's first row ahead of the wave (lane 0's row is the only one that can begin
+    // before it)
+    uint32_t ahead = 0;
+    if (lane == 0 && in) {
+        const uint32_t rr = candStart[a];
+        for (uint32_t q = rr; q < c; q++) ahead += acc[q];
     }
-    aCnt[s] = n;
-    if (n > slabCap) atomicMax(&ctr->maxOcc, n);
+    ahead = __shfl(ahead, 0, 64);
+    if (!t) return;
+    const uint32_t lo = r0 > w0 ? r0 - w0 : 0u;  // first lane of c's row in this wave
+    const unsigned long long below = (lane ? (~0ull >> (64u - lane)) : 0ull) & ~(lo ? (~0ull >> (64u - lo)) : 0ull);
+    const uint32_t k = (uint32_t)__popcll(ball & below) + (r0 < w0 ? ahead : 0u);
+    const uint32_t i = rowStart[a] + numAnal[a] + k;
+    if (i >= rowStart[a + 1]) {  // counts that disagree with the scan (not reached)
+        cidx[c] = DEME_NULL_MAPPING_PARTNER;
+        return;
+    }
+    idA[i] = a;
+    idB[i] = candB[c];
+    ctype[i] = DEME_SPHERE_SPHERE_CONTACT;
+    cidx[c] = i;
+    if (prevAcc) map[i] = prevAcc[c] ? prevCidx[c] : DEME_NULL_MAPPING_PARTNER;
+}
+
+// Per sphere: its plane contacts at the head of its canonical row (ascending plane index; the map entry, when
+// mapPlanes, is the same plane's entry in its previous row; no mirror slot), and its mirror row -- the touching
+// candidates with it as partner, ascending A -- linked from each of their contacts: mirror[contact] =
+// bRowStart[B] + rank.
+__global__ __launch_bounds__(256) void k_cand_mirror(KParams P, const SphereRec* __restrict__ sph,
+                                                     const OwnerRec* __restrict__ own, uint32_t nS,
+                                                     const uint32_t* __restrict__ rowStart,
+                                                     const unsigned long long* __restrict__ planeBits,
+                                                     const uint32_t* __restrict__ prevRowStart,
+                                                     const uint32_t* __restrict__ prevIdB,
+                                                     const uint8_t* __restrict__ prevType, bool mapPlanes,
+                                                     const uint32_t* __restrict__ tStart,
+                                                     const uint32_t* __restrict__ tC,
+                                                     const uint8_t* __restrict__ acc,
+                                                     const uint32_t* __restrict__ cidx,
+                                                     const uint32_t* __restrict__ bRowStart, uint32_t* __restrict__ idA,
+                                                     uint32_t* __restrict__ idB, uint8_t* __restrict__ ctype,
+                                                     uint32_t* __restrict__ map, uint32_t* __restrict__ mirror,
+                                                     SpecGuard guard) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nS || spec_skip(guard)) return;
+    unsigned long long bits = planeBits[s];
+    if (bits) {
+        uint32_t pos = rowStart[s];
+        const uint32_t en = rowStart[s + 1];
+        uint32_t a0 = 0, a1 = 0;
+        scene_planes(P, own[sph[s].owner].scene, a0, a1);
+        // the previous row starts with its plane contacts, ascending: a merge over them
+        uint32_t j = 0, je = 0;
+        if (mapPlanes && prevRowStart) {
+            j = prevRowStart[s];
+            je = prevRowStart[s + 1];
+        }
+        while (bits && pos < en) {
+            const uint32_t pid = a0 + (uint32_t)(__ffsll(bits) - 1);
+            bits &= bits - 1;
+            idA[pos] = s;
+            idB[pos] = pid;
+            ctype[pos] = DEME_SPHERE_PLANE_CONTACT;
+            mirror[pos] = DEME_NULL_MAPPING_PARTNER;
+            if (mapPlanes) {
+                while (j < je && prevType[j] == DEME_SPHERE_PLANE_CONTACT && prevIdB[j] < pid) j++;
+                map[pos] = (j < je && prevType[j] == DEME_SPHERE_PLANE_CONTACT && prevIdB[j] == pid)
+                               ? j
+                               : DEME_NULL_MAPPING_PARTNER;
+            }
+            pos++;
+        }
+    }
+    uint32_t q = bRowStart[s];
+    const uint32_t qe = bRowStart[s + 1], t1 = tStart[s + 1];
+    for (uint32_t t = tStart[s]; t < t1 && q < qe; t++) {
+        const uint32_t cc = tC[t];
+        if (!acc[cc]) continue;
+        const uint32_t ci = cidx[cc];
+        if (ci != DEME_NULL_MAPPING_PARTNER) mirror[ci] = q;
+        q++;
+    }
+}
+
+// After a candidate build, per sphere B: its transposed row -- the build's mirror column (the lower-id partners,
+// ascending) with each partner A replaced by the index of the candidate (A, B) in A's row. Every mirror entry was
+// appended with its row entry, so the search finds it; if it did not, the build is refused (ERRBIT_PAIR_OVERFLOW).
+__global__ __launch_bounds__(256) void k_cand_transpose(uint32_t nS, const uint32_t* __restrict__ mirSlab,
+                                                        const uint32_t* __restrict__ posOf,
+                                                        const uint32_t* __restrict__ tStart,
+                                                        const uint32_t* __restrict__ candStart,
+                                                        const uint32_t* __restrict__ candB, uint32_t* __restrict__ tC,
+                                                        DevCounters* ctr) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nS) return;
+    const uint32_t t0 = tStart[s], n = tStart[s + 1] - t0;
+    sorted_copy(mirSlab + posOf[s], n, nS, tC + t0);
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t a = tC[t0 + k];
+        uint32_t found = 0xFFFFFFFFu;
+        if (a < nS) {
+            // A's row ascends in partner id: the first entry not below s
+            uint32_t l = candStart[a], h = candStart[a + 1];
+            while (l < h) {
+                const uint32_t m = l + (h - l) / 2;
+                if (candB[m] < s) l = m + 1;
+                else h = m;
+            }
+            if (l < candStart[a + 1] && candB[l] == s) found = l;
+        }
+        if (found == 0xFFFFFFFFu) {
+            atomicOr(&ctr->err, ERRBIT_PAIR_OVERFLOW);
+            continue;  // the build is refused
+        }
+        tC[t0 + k] = found;
+    }
 }
 
 inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
@@ -832,19 +985,56 @@ hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const Owner
 }
 
 hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* posOf, const uint32_t* candStart,
-                            uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st) {
+                            uint32_t* candA, uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cand_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, rowSlab, posOf, candStart, candB, sorted, pos0);
+    k_cand_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, rowSlab, posOf, candStart, candA, candB, sorted, pos0);
     return hipGetLastError();
 }
 
-hipError_t launch_cand_test(const KParams& P, uint32_t nS, const CdRec* pos, const uint2* meta,
-                            const uint32_t* candStart, const uint32_t* candB, const uint8_t* mask, double dispLimit2,
-                            uint32_t slabCap, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
-                            DevCounters* ctr, hipStream_t st) {
+hipError_t launch_cand_flags(const KParams& P, uint32_t nK, const CdRec* pos, const uint2* meta, const uint32_t* candA,
+                             const uint32_t* candB, const uint8_t* mask, double dispLimit2, uint8_t* acc,
+                             DevCounters* ctr, hipStream_t st) {
+    k_cand_flags<<<blocks(nK > 0 ? nK : 1u, 256), 256, 0, st>>>(P, nK, pos, meta, candA, candB, mask, dispLimit2, acc,
+                                                                ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_cand_count(uint32_t nS, const uint32_t* candStart, const uint32_t* tStart, const uint32_t* tC,
+                             const uint8_t* acc, uint32_t* aCnt, uint32_t* bCnt, const DevCounters* ctr,
+                             hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cand_test<<<blocks(nS, 256), 256, 0, st>>>(P, nS, pos, meta, candStart, candB, mask, dispLimit2, slabCap,
-                                                 rowSlab, mirSlab, aCnt, bCnt, ctr);
+    k_cand_count<<<blocks(nS, 256), 256, 0, st>>>(nS, candStart, tStart, tC, acc, aCnt, bCnt, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_cand_fill(uint32_t nK, const uint32_t* candA, const uint32_t* candB, const uint32_t* candStart,
+                            const uint8_t* acc, const uint32_t* rowStart, const uint32_t* numAnal,
+                            const uint8_t* prevAcc, const uint32_t* prevCidx, uint32_t* cidx, uint32_t* idA,
+                            uint32_t* idB, uint8_t* ctype, uint32_t* map, hipStream_t st, SpecGuard guard) {
+    if (nK == 0) return hipSuccess;
+    k_cand_fill<<<blocks(nK, 256), 256, 0, st>>>(nK, candA, candB, candStart, acc, rowStart, numAnal, prevAcc,
+                                                 prevCidx, cidx, idA, idB, ctype, map, guard);
+    return hipGetLastError();
+}
+
+hipError_t launch_cand_mirror(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                              const uint32_t* rowStart, const unsigned long long* planeBits,
+                              const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
+                              bool mapPlanes, const uint32_t* tStart, const uint32_t* tC, const uint8_t* acc,
+                              const uint32_t* cidx, const uint32_t* bRowStart, uint32_t* idA, uint32_t* idB,
+                              uint8_t* ctype, uint32_t* map, uint32_t* mirror, hipStream_t st, SpecGuard guard) {
+    if (nS == 0) return hipSuccess;
+    k_cand_mirror<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, rowStart, planeBits, prevRowStart, prevIdB,
+                                                   prevType, mapPlanes, tStart, tC, acc, cidx, bRowStart, idA, idB,
+                                                   ctype, map, mirror, guard);
+    return hipGetLastError();
+}
+
+hipError_t launch_cand_transpose(uint32_t nS, const uint32_t* mirSlab, const uint32_t* posOf, const uint32_t* tStart,
+                                 const uint32_t* candStart, const uint32_t* candB, uint32_t* tC, DevCounters* ctr,
+                                 hipStream_t st) {
+    if (nS == 0) return hipSuccess;
+    k_cand_transpose<<<blocks(nS, 256), 256, 0, st>>>(nS, mirSlab, posOf, tStart, candStart, candB, tC, ctr);
     return hipGetLastError();
 }
 

@@ -119,16 +119,37 @@ struct deme_ctx {
     // Candidate-list detection: pairs within `skin` of touching, kept from a build until some sphere has moved half
     // the skin; each pass then tests only them. skinSet < 0: automatic (0.2 x the mean radius); 0: off (the cell
     // sweep every pass).
-    // The automatic skin adapts at each build to the passes the previous lists lasted: fewer than 8 widen it 1.5x (up
-    // to 0.8 x the mean radius), more than 400 narrow it 0.75x (down to 0.05 x); any skin gives the same contacts.
+    // The automatic skin (any skin gives the same contacts) adapts at each build, within 0.05 to 0.8 x the mean
+    // radius, to the cost per pass of the lists just retired: their candidates times (1 + BUILD_PASSES / the passes
+    // they lasted), a build costing about as much as BUILD_PASSES passes over its candidates. It moves 1.25x in one
+    // direction while that cost falls and turns back when it rises; lists that die within a pass widen it 1.5x, lists
+    // that outlast 400 passes narrow it 0.75x.
+    // With the automatic skin, lists that last fewer than 4 passes at the widest skin do not pay for their builds:
+    // the cell sweep then runs every pass for cellPassesLeft passes (64, doubling while that repeats, up to 4096),
+    // after which the lists are tried again.
     double skinSet = -1.0, skinAuto = 0.0;
+    double skinCostPrev = -1.0;  // cost per pass of the lists before the ones just retired (-1: none)
+    int skinDir = 1;
+    uint32_t cellPassesLeft = 0, cellBackoff = 0;
     bool candValid = false, rebuildNext = true;
     uint64_t nCand = 0, candBuilds = 0, passesSinceBuild = 0;
+    // passes the lists before the latest build lasted (~0 before the second build) and whether the pass in hand built
+    uint64_t lastLifetime = ~0ull;
+    bool builtThisPass = false;
     DBuf<CdRec> pos, pos0;
     DBuf<uint2> meta;
     DBuf<uint32_t> candStart, candB;
+    // The build's candidates: candA (the row's sphere), transposed (tC[tStart[B]..tStart[B + 1]]: indices of the
+    // candidates with partner B, ascending A). Per list parity the touching flags (candAcc) and each touching
+    // candidate's contact index (cidx).
+    // cidxGen[p]: the build (candBuilds) whose candidates candAcc[p] / cidx[p] index, 0 when the list at parity p
+    // came another way; a pass maps its history through them when that is the current build's.
+    DBuf<uint32_t> candA, tStart, tC, cidx[2];
+    DBuf<uint8_t> candAcc[2];
+    uint64_t cidxGen[2] = {0, 0};
     double skin() const {
         if (skinSet >= 0.0) return skinSet;
+        if (cellPassesLeft > 0) return 0.0;
         return skinAuto > 0.0 ? skinAuto : 0.2 * (double)meanRadius;
     }
     int pairMode = -1;  // pair sweep mode (launch_cell_pairs); -1: chosen from the radius spread
@@ -455,6 +476,11 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->survSlab.release(); c->rowSlab.release(); c->mirSlab.release(); c->sorted.release(); c->planeBits.release();
     c->aCnt.release(); c->sortedMeta.release();
     c->pos.release(); c->pos0.release(); c->meta.release(); c->candStart.release(); c->candB.release();
+    c->candA.release(); c->tStart.release(); c->tC.release();
+    for (int k = 0; k < 2; k++) {
+        c->cidx[k].release();
+        c->candAcc[k].release();
+    }
     for (auto& k : c->adjCk) {
         k.own.release(); k.rowStart.release(); k.idB.release(); k.ctype.release(); k.hist.release();
         k.idA.release(); k.mirror.release(); k.bRowStart.release(); k.acc.release();
@@ -881,6 +907,7 @@ int deme_cd_sort_remap(deme_ctx* c) {
         c->Cprev = c->C;
     }
     const int cu = c->cur, pv = cu ^ 1;
+    c->cidxGen[cu] = 0;
     DEME_LAUNCH(c, "scan_rows",
                 scan_u32(c->rowCount.p, c->numAnal.p, c->nS, c->rowStart[cu].p, &c->ctr.p->total[1], c->scanScratch.p,
                          c->stream));
@@ -977,12 +1004,36 @@ static float sweep_margin(const deme_sim_params& p) {
 }
 
 // The canonical rows and mirror rows of the cell path, then the history map against the previous list.
-static int enqueue_rows(deme_ctx* c, SpecGuard guard) {
+// skinPass: the pass ran on candidate lists (else the cell sweep).
+static int enqueue_rows(deme_ctx* c, SpecGuard guard, bool skinPass) {
     const int cu = c->cur, pv = cu ^ 1;
     const uint32_t nS = c->nS;
+    if (skinPass) {
+        // candidate-list pass: rows and links from the candidates' indices; the map too when the previous list
+        // indexes the same candidates, else (the first pass of a build) by the merge join of the rows
+        const bool mapFromCand = c->havePrev && c->cidxGen[pv] == c->candBuilds;
+        DEME_LAUNCH(c, "cand_fill",
+                    launch_cand_fill((uint32_t)c->nCand, c->candA.p, c->candB.p, c->candStart.p, c->candAcc[cu].p,
+                                     c->rowStart[cu].p, c->numAnal.p, mapFromCand ? c->candAcc[pv].p : nullptr,
+                                     c->cidx[pv].p, c->cidx[cu].p, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->map.p,
+                                     c->stream, guard));
+        DEME_LAUNCH(c, "cand_mirror",
+                    launch_cand_mirror(c->kp, c->sph.p, c->own.p, nS, c->rowStart[cu].p, c->planeBits.p,
+                                       c->havePrev ? c->rowStart[pv].p : nullptr, c->havePrev ? c->idB[pv].p : nullptr,
+                                       c->havePrev ? c->ctype[pv].p : nullptr, mapFromCand, c->tStart.p, c->tC.p,
+                                       c->candAcc[cu].p, c->cidx[cu].p, c->bRowStart.p, c->idA.p, c->idB[cu].p,
+                                       c->ctype[cu].p, c->map.p, c->mirror.p, c->stream, guard));
+        if (!mapFromCand)
+            DEME_LAUNCH(c, "history_map",
+                        launch_history_map(nS, c->rowStart[cu].p, c->idB[cu].p, c->ctype[cu].p,
+                                           c->havePrev ? c->rowStart[pv].p : nullptr,
+                                           c->havePrev ? c->idB[pv].p : nullptr,
+                                           c->havePrev ? c->ctype[pv].p : nullptr, c->map.p, nullptr, nullptr,
+                                           nullptr, c->stream, guard));
+        return DEME_OK;
+    }
     DEME_LAUNCH(c, "cell_rows",
-                launch_cell_rows(c->kp, c->sph.p, c->own.p, nS, c->rowSlab.p, c->mirSlab.p,
-                                 c->skin() > 0.0 ? nullptr : c->posOf.p,
+                launch_cell_rows(c->kp, c->sph.p, c->own.p, nS, c->rowSlab.p, c->mirSlab.p, c->posOf.p,
                                  c->rowStart[cu].p, c->planeBits.p, c->bRowStart.p, c->idA.p, c->idB[cu].p,
                                  c->ctype[cu].p, c->bPart.p, c->stream, guard));
     DEME_LAUNCH(c, "history_map",
@@ -1081,13 +1132,27 @@ static double stale_limit2(const deme_ctx* c) {
 // Candidate lists from the current state: the cell sweep in its build mode (pairs of different owners within the
 // skin of touching), rows of higher-id partners in id order (candStart / candB) and the build positions (pos0).
 static int build_candidates(deme_ctx* c) {
-    if (c->skinSet < 0.0) {
+    if (c->skinSet < 0.0 && c->candBuilds > 0 && c->candValid) {
+        // the retiring lists' lifetime and size judge their skin (lists dropped for other reasons -- a host change,
+        // a return from the cell sweep -- do not)
+        constexpr double BUILD_PASSES = 2.7;
         const double m = (double)c->meanRadius, cur = c->skin();
+        const double life = (double)std::max<uint64_t>(c->passesSinceBuild, 1);
+        const double cost = (double)std::max<uint64_t>(c->nCand, 1) * (1.0 + BUILD_PASSES / life);
         double next = cur;
-        if (c->candBuilds > 0 && c->passesSinceBuild < 8) next = std::min(cur * 1.5, 0.8 * m);
-        else if (c->candBuilds > 0 && c->passesSinceBuild > 400) next = std::max(cur * 0.75, 0.05 * m);
-        c->skinAuto = next;
+        if (c->passesSinceBuild <= 1) {
+            next = cur * 1.5;
+        } else if (c->passesSinceBuild > 400) {
+            next = cur * 0.75;
+        } else {
+            if (c->skinCostPrev >= 0.0 && cost > c->skinCostPrev) c->skinDir = -c->skinDir;
+            next = c->skinDir > 0 ? cur * 1.25 : cur * 0.8;
+        }
+        c->skinCostPrev = cost;
+        c->skinAuto = std::min(std::max(next, 0.05 * m), 0.8 * m);
     }
+    c->lastLifetime = c->candBuilds > 0 ? c->passesSinceBuild : ~0ull;
+    c->builtThisPass = true;
     c->passesSinceBuild = 0;
     int mode = 0;
     int r = cd_buffers(c, &mode);
@@ -1095,6 +1160,7 @@ static int build_candidates(deme_ctx* c) {
     const uint32_t nS = c->nS;
     DEME_HIP(c, c->sortedF.ensure(nS));
     DEME_HIP(c, c->candStart.ensure((size_t)nS + 1));
+    DEME_HIP(c, c->tStart.ensure((size_t)nS + 1));
     DEME_HIP(c, c->pos0.ensure(nS));
     for (int attempt = 0;; attempt++) {
         r = enqueue_cell_pass(c, PAIR_MODE_CAND);
@@ -1107,6 +1173,9 @@ static int build_candidates(deme_ctx* c) {
         DEME_LAUNCH(c, "scan_cand",
                     scan_u32(c->aCnt.p, nullptr, nS, c->candStart.p, &c->ctr.p->total[0], c->scanScratch.p,
                              c->stream, c->posOf.p));
+        DEME_LAUNCH(c, "scan_cand",
+                    scan_u32(c->bCnt.p, nullptr, nS, c->tStart.p, &c->ctr.p->total[2], c->scanScratch.p, c->stream,
+                             c->posOf.p));
         r = readback_counters(c);
         if (r) return r;
         const DevCounters& h = *c->hctr;
@@ -1122,10 +1191,24 @@ static int build_candidates(deme_ctx* c) {
     }
     c->nCand = c->hctr->total[0];
     if (c->nCand > 0xFFFFFFFFull) return c->fail(DEME_ERR_CAPACITY, "candidate count exceeds 2^32 - 1");
-    DEME_HIP(c, c->candB.ensure(std::max<size_t>(c->nCand, 1)));
+    if (c->hctr->total[2] != c->nCand) return c->fail(DEME_ERR_HIP, "inconsistent candidate row and mirror counts");
+    const size_t nK = std::max<size_t>(c->nCand, 1);
+    DEME_HIP(c, c->candA.ensure(nK));
+    DEME_HIP(c, c->candB.ensure(nK));
+    DEME_HIP(c, c->tC.ensure(nK));
+    for (int k = 0; k < 2; k++) {
+        DEME_HIP(c, c->candAcc[k].ensure(nK));
+        DEME_HIP(c, c->cidx[k].ensure(nK));
+    }
     DEME_LAUNCH(c, "cand_rows",
-                launch_cand_rows(nS, c->rowSlab.p, c->posOf.p, c->candStart.p, c->candB.p, c->sorted.p, c->pos0.p,
-                                 c->stream));
+                launch_cand_rows(nS, c->rowSlab.p, c->posOf.p, c->candStart.p, c->candA.p, c->candB.p, c->sorted.p,
+                                 c->pos0.p, c->stream));
+    DEME_LAUNCH(c, "cand_transpose",
+                launch_cand_transpose(nS, c->mirSlab.p, c->posOf.p, c->tStart.p, c->candStart.p, c->candB.p, c->tC.p,
+                                      c->ctr.p, c->stream));
+    r = readback_counters(c);
+    if (r) return r;
+    if (c->hctr->err) return c->fail(DEME_ERR_HIP, "candidate rows and their transpose disagree");
     c->candValid = true;
     c->rebuildNext = false;
     c->candBuilds++;
@@ -1142,27 +1225,29 @@ static int skin_buffers(deme_ctx* c) {
     DEME_HIP(c, c->scanTmp.ensure(nS));
     DEME_HIP(c, c->scan2Sums.ensure(2 * ((nS + SCAN2_TILE - 1) / SCAN2_TILE + 1)));
     DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>(nS, 1u << 22)), 1.0));
-    DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
-    DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
     return DEME_OK;
 }
 
-// One candidate-list pass up to the row scans: plane poses, counters and mirror counts cleared, per-sphere
-// positions (with the displacement since the build), the predicate on the candidates, the row and mirror-row scans.
+// One candidate-list pass up to the row scans: plane poses, counters cleared, per-sphere positions (with the
+// displacement since the build), the predicate on the candidates (touching flags, row counts), the mirror-row
+// counts, the row and mirror-row scans.
 static int enqueue_skin_pass(deme_ctx* c, int cu) {
     const uint32_t nS = c->nS;
     DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
     static_assert(sizeof(DevCounters) % 4 == 0, "counters are cleared as 32-bit words");
-    DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, nS, 0, c->stream, reinterpret_cast<uint32_t*>(c->ctr.p),
-                                           sizeof(DevCounters) / 4));
+    DEME_LAUNCH(c, "fill", launch_fill_u32(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(DevCounters) / 4, 0,
+                                           c->stream));
     DEME_LAUNCH(c, "sphere_pos",
                 launch_sphere_pos(c->kp, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->maskAny ? c->mask.p : nullptr,
                                   c->pos.p, c->pos0.p, c->meta.p, c->numBins.p, c->numAnal.p, c->planeBits.p, c->ctr.p,
                                   c->stream));
-    DEME_LAUNCH(c, "cand_test",
-                launch_cand_test(c->kp, nS, c->pos.p, c->meta.p, c->candStart.p, c->candB.p,
-                                 c->maskAny ? c->mask.p : nullptr, stale_limit2(c), c->slabCap, c->rowSlab.p,
-                                 c->mirSlab.p, c->aCnt.p, c->bCnt.p, c->ctr.p, c->stream));
+    DEME_LAUNCH(c, "cand_flags",
+                launch_cand_flags(c->kp, (uint32_t)c->nCand, c->pos.p, c->meta.p, c->candA.p, c->candB.p,
+                                  c->maskAny ? c->mask.p : nullptr, stale_limit2(c), c->candAcc[cu].p, c->ctr.p,
+                                  c->stream));
+    DEME_LAUNCH(c, "cand_count",
+                launch_cand_count(nS, c->candStart.p, c->tStart.p, c->tC.p, c->candAcc[cu].p, c->aCnt.p, c->bCnt.p,
+                                  c->ctr.p, c->stream));
     if ((uint64_t)nS <= SCAN2_TILE * SCAN_TILE) {
         DEME_LAUNCH(c, "scan_rows",
                     scan2_u32_gather(c->aCnt.p, c->numAnal.p, c->bCnt.p, nullptr, nS, c->rowStart[cu].p,
@@ -1177,6 +1262,35 @@ static int enqueue_skin_pass(deme_ctx* c, int cu) {
                              c->stream));
     }
     return DEME_OK;
+}
+
+// Bookkeeping at the end of a detection pass (candidate-list or cell sweep; disp2: the largest squared displacement
+// since the build; the pass's builds, if any, are behind it). A candidate-list pass rebuilds ahead of time when the next pass would likely find the lists
+// stale; with the automatic skin, lists that keep dying within 4 passes at the widest skin give way to the cell sweep
+// for a while, and cell sweeps count down to trying them again.
+static void end_pass(deme_ctx* c, bool skinPass, double disp2) {
+    const bool built = c->builtThisPass;
+    c->builtThisPass = false;
+    if (skinPass) {
+        c->passesSinceBuild++;
+        const double lim = 0.35 * c->skin();
+        c->rebuildNext = disp2 > lim * lim;
+        // lists about to be rebuilt after fewer than 4 passes, or just rebuilt (found stale) after fewer than 4
+        const bool dying = (c->rebuildNext && c->passesSinceBuild < 4) || (built && c->lastLifetime < 4);
+        if (dying && c->skinSet < 0.0 && c->skinAuto >= 0.8 * (double)c->meanRadius * (1.0 - 1e-9)) {
+            c->cellBackoff = std::min<uint32_t>(c->cellBackoff ? 2 * c->cellBackoff : 64, 4096);
+            c->cellPassesLeft = c->cellBackoff;
+            c->candValid = false;
+            c->graphGen++;
+        } else if (!c->rebuildNext && c->passesSinceBuild >= 4) {
+            c->cellBackoff = 0;
+        }
+    } else if (c->skinSet < 0.0 && c->cellPassesLeft > 0 && --c->cellPassesLeft == 0) {
+        c->candValid = false;
+        c->passesSinceBuild = 100;  // the lists start afresh: no lifetime to judge them by
+        c->skinCostPrev = -1.0;
+        c->graphGen++;
+    }
 }
 
 int deme_cd_run(deme_ctx* c) {
@@ -1214,6 +1328,7 @@ int deme_cd_run(deme_ctx* c) {
     c->contactsReady = false;
     c->cdStage = 0;
     const int cu = c->cur;
+    c->cidxGen[cu] = 0;
     SpecGuard spec;
     for (int attempt = 0;; attempt++) {
         r = skin ? enqueue_skin_pass(c, cu) : enqueue_pair_pass(c, mode, cu);
@@ -1229,7 +1344,7 @@ int deme_cd_run(deme_ctx* c) {
             spec.slabCap = c->slabCap;
             spec.capC = std::min({c->idB[cu].cap, c->ctype[cu].cap, c->idA.cap, c->map.cap, c->mirror.cap});
             spec.capP = c->bPart.cap;
-            r = enqueue_rows(c, spec);
+            r = enqueue_rows(c, spec, skin);
             if (r) return r;
         }
         DEME_HIP(c, hipEventSynchronize(c->evCounts));
@@ -1237,8 +1352,7 @@ int deme_cd_run(deme_ctx* c) {
         if (h.err == ERRBIT_STALE) {
             // a sphere moved half the skin since the build: new lists, then the pass again
             if (attempt >= 3) return c->fail(DEME_ERR_CAPACITY, "candidate lists went stale right after a build");
-            c->candValid = false;
-            r = build_candidates(c);
+            r = build_candidates(c);  // (the stale lists are judged by the passes they lasted)
             if (r) return r;
             continue;
         }
@@ -1261,12 +1375,7 @@ int deme_cd_run(deme_ctx* c) {
             DEME_HIP(c, hipMemsetAsync(&c->ctr.p->maxOcc, 0, sizeof(uint32_t), c->stream));
         }
     }
-    if (skin) {
-        c->passesSinceBuild++;
-        // rebuild ahead of time when the next pass would likely find the lists stale
-        const double lim = 0.35 * c->skin();
-        c->rebuildNext = max_disp2(*c->hctr) > lim * lim;
-    }
+    const double disp2 = max_disp2(*c->hctr);
     const uint64_t C = c->hctr->total[1], nPairs = c->hctr->total[2];
     if (C > 0xFFFFFFFFull) return c->fail(DEME_ERR_CAPACITY, "contact count exceeds contactPairs_t (2^32 - 1)");
     if (nPairs > C) return c->fail(DEME_ERR_HIP, "inconsistent contact and mirror-row counts");
@@ -1288,10 +1397,12 @@ int deme_cd_run(deme_ctx* c) {
     c->termsReady = false;
     // the rows and the map already ran unless the lists outgrew the buffers they were enqueued with
     if (!spec.ctr || C > spec.capC || nPairs > spec.capP) {
-        r = enqueue_rows(c, SpecGuard());
+        r = enqueue_rows(c, SpecGuard(), skin);
         if (r) return r;
     }
     c->contactsReady = true;
+    c->cidxGen[cu] = skin ? c->candBuilds : 0;
+    end_pass(c, skin, disp2);
     c->needDetect = false;
     c->fresh = true;
     c->forcesFull = false;
@@ -1560,7 +1671,9 @@ static std::vector<const void*> cycle_fingerprint(deme_ctx* c, int p) {
             reinterpret_cast<const void*>(c->jitOwner.fn),
             v(c->slabCap),   v(c->nCells),    v(c->nS),         v(c->nO),         v(c->grid.nx),   v(c->grid.ny),
             v(c->grid.nz),   v(invBits),      v(c->jitActive),  v(c->ownerJitActive), v(c->maskAny),
-            c->pos.p,        c->pos0.p,       c->meta.p,        c->candStart.p,   c->candB.p,      v(skinBits)};
+            c->pos.p,        c->pos0.p,       c->meta.p,        c->candStart.p,   c->candB.p,      v(skinBits),
+            c->tStart.p,     c->tC.p,         c->candAcc[0].p,  c->candAcc[1].p,  c->cidx[0].p,    c->cidx[1].p,
+            c->candA.p,      v(c->nCand)};
 }
 
 // Buffers of the list at parity p (the one the interval's pass writes; the other parity holds the current list and
@@ -1605,7 +1718,7 @@ static int enqueue_cycle(deme_ctx* c, int f, const SpecGuard& g, int mode) {
         r = enqueue_cell_pass(c, mode);
         if (!r) r = enqueue_pair_pass(c, mode, p);
     }
-    if (!r) r = enqueue_rows(c, g);
+    if (!r) r = enqueue_rows(c, g, c->skin() > 0.0);
     if (r) return r;
     const uint32_t capC = (uint32_t)g.capC;
     for (int s = 0; s < f; s++) {
@@ -1680,6 +1793,9 @@ static int run_cycle(deme_ctx* c, int f, bool* done) {
     *done = false;
     const int p = c->cur ^ 1;
     deme_ctx::Cycle& cy = c->cycle[p];
+    // a captured candidate-list pass maps the history through the previous list's candidate indices: they must be
+    // the current build's
+    if (c->skin() > 0.0 && c->cidxGen[c->cur] != c->candBuilds) return DEME_OK;
     const bool valid = cy.exec && cy.gen == c->graphGen && cy.f == f &&
                        std::memcmp(&cy.kp, &c->kp, sizeof(KParams)) == 0 && cy.ptrs == cycle_fingerprint(c, p);
     if (!valid) {
@@ -1700,11 +1816,8 @@ static int run_cycle(deme_ctx* c, int f, bool* done) {
         return DEME_OK;
     }
     c->cycleBackoff = 0;
-    if (c->skin() > 0.0) {
-        c->passesSinceBuild++;
-        const double lim = 0.35 * c->skin();
-        c->rebuildNext = max_disp2(h) > lim * lim;
-    }
+    c->cidxGen[p] = c->skin() > 0.0 ? c->candBuilds : 0;
+    end_pass(c, c->skin() > 0.0, max_disp2(h));
     // the bookkeeping of a detection pass and f steps
     c->cur = p;
     c->havePrev = true;
@@ -1800,6 +1913,10 @@ int deme_set_skin(deme_ctx* c, double skin) {
         return c->fail(DEME_ERR_INVALID_ARGUMENT, "skin must be a finite length >= 0 (or -1: automatic)");
     c->skinSet = skin < 0.0 ? -1.0 : skin;
     c->skinAuto = 0.0;
+    c->skinCostPrev = -1.0;
+    c->skinDir = 1;
+    c->cellPassesLeft = 0;
+    c->cellBackoff = 0;
     c->candValid = false;
     c->graphGen++;
     return DEME_OK;

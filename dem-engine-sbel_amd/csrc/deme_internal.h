@@ -154,16 +154,40 @@ hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const Owner
                              const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos, const CdRec* pos0,
                              uint2* meta, uint16_t* numBins, uint32_t* numAnal, unsigned long long* planeBits,
                              DevCounters* ctr, hipStream_t st);
-// After a candidate build: per sphere its candidate row (candStart from a scan of the build's row counts) and its
-// position at the build (pos0).
+// After a candidate build: per sphere its candidate row (candStart from a scan of the build's row counts; candA /
+// candB the row's sphere and partners) and its position at the build (pos0).
 hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* posOf, const uint32_t* candStart,
-                            uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st);
-// Per step: the reference's predicate on the candidates into id-indexed slab columns (aCnt written, bCnt counted up
-// from zero); ERRBIT_STALE instead when ctr->disp2 exceeds dispLimit2.
-hipError_t launch_cand_test(const KParams& P, uint32_t nS, const CdRec* pos, const uint2* meta,
-                            const uint32_t* candStart, const uint32_t* candB, const uint8_t* mask, double dispLimit2,
-                            uint32_t slabCap, uint32_t* rowSlab, uint32_t* mirSlab, uint32_t* aCnt, uint32_t* bCnt,
-                            DevCounters* ctr, hipStream_t st);
+                            uint32_t* candA, uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st);
+// Per step, per candidate: the reference's predicate (acc[c] = 1 when c touches); ERRBIT_STALE instead when
+// ctr->disp2 exceeds dispLimit2.
+hipError_t launch_cand_flags(const KParams& P, uint32_t nK, const CdRec* pos, const uint2* meta, const uint32_t* candA,
+                             const uint32_t* candB, const uint8_t* mask, double dispLimit2, uint8_t* acc,
+                             DevCounters* ctr, hipStream_t st);
+// Per sphere: aCnt = touching candidates of its row, bCnt = touching candidates with it as partner.
+hipError_t launch_cand_count(uint32_t nS, const uint32_t* candStart, const uint32_t* tStart, const uint32_t* tC,
+                             const uint8_t* acc, uint32_t* aCnt, uint32_t* bCnt, const DevCounters* ctr,
+                             hipStream_t st);
+// Per touching candidate: its canonical-row entry, cidx[c] and, with prevAcc / prevCidx (the previous pass over the
+// same candidates), its persistent-contact map entry.
+hipError_t launch_cand_fill(uint32_t nK, const uint32_t* candA, const uint32_t* candB, const uint32_t* candStart,
+                            const uint8_t* acc, const uint32_t* rowStart, const uint32_t* numAnal,
+                            const uint8_t* prevAcc, const uint32_t* prevCidx, uint32_t* cidx, uint32_t* idA,
+                            uint32_t* idB, uint8_t* ctype, uint32_t* map, hipStream_t st,
+                            SpecGuard guard = SpecGuard());
+// Per sphere: its plane contacts (with their map entries when mapPlanes) and the mirror links of the touching
+// candidates with it as partner.
+hipError_t launch_cand_mirror(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+                              const uint32_t* rowStart, const unsigned long long* planeBits,
+                              const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
+                              bool mapPlanes, const uint32_t* tStart, const uint32_t* tC, const uint8_t* acc,
+                              const uint32_t* cidx, const uint32_t* bRowStart, uint32_t* idA, uint32_t* idB,
+                              uint8_t* ctype, uint32_t* map, uint32_t* mirror, hipStream_t st,
+                              SpecGuard guard = SpecGuard());
+// After a build: tC[tStart[B]..] = indices of the candidates with partner B, ascending A (from the build's mirror
+// columns); ERRBIT_PAIR_OVERFLOW if a mirror entry has no row entry.
+hipError_t launch_cand_transpose(uint32_t nS, const uint32_t* mirSlab, const uint32_t* posOf, const uint32_t* tStart,
+                                 const uint32_t* candStart, const uint32_t* candB, uint32_t* tC, DevCounters* ctr,
+                                 hipStream_t st);
 
 // dyn.hip
 // aTerm / gTerm: 6 floats per contact; bTerm: 6 floats per mirror entry; out (optional): full contact records

@@ -114,3 +114,30 @@ def test_candidate_lists_give_the_cell_sweeps_trajectory(skin, freq, graphs):
     assert sk > 0 and builds >= 1 and n_cand >= a.counts().nContacts - a.counts().nSphereGeoContacts
     if skin is not None:
         assert builds >= 3  # the thin skin went stale and was rebuilt
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_automatic_skin_gives_way_to_the_cell_sweep_and_back(graphs):
+    """A sparse gas of spheres fast enough (tens of m/s, 1e-4 s between passes) that candidate lists die within a
+    pass even at the widest automatic skin: the automatic mode then runs the cell sweep for a while and tries the
+    lists again. Every switch keeps the contact lists, maps and trajectory of the cell sweep every pass, bit for
+    bit."""
+    sc = scenes.sphere_settling(n=4000, spacing=4.0, velocity_sigma=20.0, omega_sigma=3.0, seed=1033, beta=2e-4)
+    a, b = DeviceSystem(sc), DeviceSystem(sc)
+    b.set_skin(0.0)
+    a.set_step_graphs(graphs)
+    modes = []
+    for k in range(180):  # one detection pass each
+        a.step(10, 10)
+        b.step(10, 10)
+        modes.append(a.candidate_stats()[0] > 0)
+        if k % 20 == 19:
+            sa, sb = a.get_state(), b.get_state()
+            for key in KEYS:
+                assert np.array_equal(sa[key], sb[key]), (k, key)
+            for x, y in zip(a.contacts(), b.contacts()):
+                assert np.array_equal(x, y), k
+    print("candidate lists in force per pass:", "".join("L" if m else "c" for m in modes), flush=True)
+    assert modes[0] and not all(modes)
+    first_cell = modes.index(False)
+    assert any(modes[first_cell:]), "the lists were not tried again"
