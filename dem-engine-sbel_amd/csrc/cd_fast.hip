@@ -103,7 +103,7 @@ __device__ __forceinline__ void sphere_detect(const KParams& P, const SphereRec&
     rCD += (double)P.beta;
     const BinRange r = bin_range(P, X, Y, Z, rCD);
     const uint32_t nx = r.x1 - r.x0 + 1, ny = r.y1 - r.y0 + 1, nz = r.z1 - r.z0 + 1;
-    numBins[s] = (uint16_t)((int)(uint16_t)nx * (int)(uint16_t)ny * (int)(uint16_t)nz);
+    if (numBins) numBins[s] = (uint16_t)((int)(uint16_t)nx * (int)(uint16_t)ny * (int)(uint16_t)nz);
     if ((uint64_t)nx * ny * nz > 65535u) {
         atomicOr(&ctr->err, ERRBIT_CAPACITY);
     } else {
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_sphere_pos(KParams P, const SphereRec* 
         const uint4* s4 = reinterpret_cast<const uint4*>(&cr);
         d4[0] = s4[0];
         d4[1] = s4[1];
-        meta[s] = make_uint2(sr.owner, o.fam);
+        if (meta) meta[s] = make_uint2(sr.owner, o.fam);
         const CdRec c0 = load_cd(pos0, s);
         const double dx = cr.x - c0.x, dy = cr.y - c0.y, dz = cr.z - c0.z;
         d2 = dx * dx + dy * dy + dz * dz;

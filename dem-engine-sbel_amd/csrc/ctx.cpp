@@ -1237,23 +1237,25 @@ static int enqueue_skin_pass(deme_ctx* c, int cu) {
     static_assert(sizeof(DevCounters) % 4 == 0, "counters are cleared as 32-bit words");
     DEME_LAUNCH(c, "fill", launch_fill_u32(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(DevCounters) / 4, 0,
                                            c->stream));
+    // (the families for the mask only when a mask is in force; no per-sphere bin counts: this path never reads them)
     DEME_LAUNCH(c, "sphere_pos",
                 launch_sphere_pos(c->kp, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->maskAny ? c->mask.p : nullptr,
-                                  c->pos.p, c->pos0.p, c->meta.p, c->numBins.p, c->numAnal.p, c->planeBits.p, c->ctr.p,
-                                  c->stream));
+                                  c->pos.p, c->pos0.p, c->maskAny ? c->meta.p : nullptr, nullptr, c->numAnal.p,
+                                  c->planeBits.p, c->ctr.p, c->stream));
     DEME_LAUNCH(c, "cand_flags",
                 launch_cand_flags(c->kp, (uint32_t)c->nCand, c->pos.p, c->meta.p, c->candA.p, c->candB.p,
                                   c->maskAny ? c->mask.p : nullptr, stale_limit2(c), c->candAcc[cu].p, c->ctr.p,
                                   c->stream));
-    DEME_LAUNCH(c, "cand_count",
-                launch_cand_count(nS, c->candStart.p, c->tStart.p, c->tC.p, c->candAcc[cu].p, c->aCnt.p, c->bCnt.p,
-                                  c->ctr.p, c->stream));
     if ((uint64_t)nS <= SCAN2_TILE * SCAN_TILE) {
+        // the row counts formed inside the scan's first pass
         DEME_LAUNCH(c, "scan_rows",
-                    scan2_u32_gather(c->aCnt.p, c->numAnal.p, c->bCnt.p, nullptr, nS, c->rowStart[cu].p,
-                                     c->bRowStart.p, &c->ctr.p->total[1], &c->ctr.p->total[2], c->scanTmp.p,
-                                     c->scan2Sums.p, c->stream));
+                    scan2_cand_rows(c->candStart.p, c->tStart.p, c->tC.p, c->candAcc[cu].p, c->numAnal.p, nS,
+                                    c->rowStart[cu].p, c->bRowStart.p, &c->ctr.p->total[1], &c->ctr.p->total[2],
+                                    c->scanTmp.p, c->scan2Sums.p, c->ctr.p, c->stream));
     } else {
+        DEME_LAUNCH(c, "cand_count",
+                    launch_cand_count(nS, c->candStart.p, c->tStart.p, c->tC.p, c->candAcc[cu].p, c->aCnt.p,
+                                      c->bCnt.p, c->ctr.p, c->stream));
         DEME_LAUNCH(c, "scan_rows",
                     scan_u32(c->aCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1], c->scanScratch.p,
                              c->stream));

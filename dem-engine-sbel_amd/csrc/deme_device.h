@@ -208,17 +208,6 @@ __device__ __forceinline__ CdRec load_cd(const CdRec* __restrict__ p, uint32_t i
     return r;
 }
 
-// The command processor hands a launch's workgroups to the chip's XCD_COUNT XCDs in turn (workgroup b to XCD
-// b % XCD_COUNT), and each XCD has its own L2. xcd_slice_tile(b, n) renumbers the n tiles of a launch so that XCD x
-// works through the x-th contiguous slice of them: neighbouring tiles, which gather overlapping records (partners in
-// nearby rows, the same owners), then share one L2 instead of fetching the same lines into several. A bijection of
-// [0, n) for any n; it changes which workgroup computes a tile, never what the tile computes.
-constexpr uint32_t XCD_COUNT = 8;
-__device__ __forceinline__ uint32_t xcd_slice_tile(uint32_t b, uint32_t n) {
-    const uint32_t q = n / XCD_COUNT, r = n % XCD_COUNT, x = b % XCD_COUNT;
-    return x * q + (x < r ? x : r) + b / XCD_COUNT;
-}
-
 // Device-side counters read back by the host after a detection pass.
 struct DevCounters {
     uint32_t pairCount;

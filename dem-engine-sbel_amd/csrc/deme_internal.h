@@ -82,6 +82,13 @@ hipError_t noalloc_scan_scratch(int device, uint64_t** ptr, uint64_t* elems);
 hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32_t* a2, const uint32_t* idx,
                             uint64_t n, uint32_t* out1, uint32_t* out2, unsigned long long* total1,
                             unsigned long long* total2, uint2* tmp, uint64_t* sums, hipStream_t st);
+// The candidate-list pass's two row scans with the counts formed in the first pass: out1 = scan(numAnal[i] + touching
+// candidates of row i), out2 = scan(touching candidates with partner i); same sizes and limit as scan2_u32_gather.
+// When ctr->err is set (the pass failed and is discarded) pass 1 does nothing and the outputs are meaningless.
+hipError_t scan2_cand_rows(const uint32_t* candStart, const uint32_t* tStart, const uint32_t* tC, const uint8_t* acc,
+                           const uint32_t* numAnal, uint64_t n, uint32_t* out1, uint32_t* out2,
+                           unsigned long long* total1, unsigned long long* total2, uint2* tmp, uint64_t* sums,
+                           const DevCounters* ctr, hipStream_t st);
 
 // cd.hip
 hipError_t launch_plane_pose(const KParams& P, const deme_anal_obj* anal, uint32_t nA, const OwnerRec* own,

@@ -165,6 +165,43 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce(const uint32_t* __r
     }
 }
 
+// Pass 1 of the candidate-list pass's row scans: the counts are formed here, not read -- per sphere the touching
+// candidates of its row plus its plane contacts, and the touching candidates with it as partner (through its
+// transposed row) -- stored in tmp and folded into the tile sums. A failed pass (ctr->err) leaves them alone.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce_cand(const uint32_t* __restrict__ candStart,
+                                                                  const uint32_t* __restrict__ tStart,
+                                                                  const uint32_t* __restrict__ tC,
+                                                                  const uint8_t* __restrict__ acc,
+                                                                  const uint32_t* __restrict__ numAnal, uint64_t n,
+                                                                  uint2* __restrict__ tmp,
+                                                                  uint64_t* __restrict__ sums1,
+                                                                  uint64_t* __restrict__ sums2,
+                                                                  const DevCounters* ctr) {
+    if (ctr->err) return;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN2_TILE;
+    uint64_t s1 = 0, s2 = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN2_ITEMS; i++) {
+        const uint64_t k = base + (uint64_t)i * SCAN_BLOCK + threadIdx.x;
+        if (k < n) {
+            uint32_t v1 = numAnal[k], v2 = 0;
+            const uint32_t c1 = candStart[k + 1], t1 = tStart[k + 1];
+            for (uint32_t c = candStart[k]; c < c1; c++) v1 += acc[c];
+            for (uint32_t t = tStart[k]; t < t1; t++) v2 += acc[tC[t]];
+            tmp[k] = make_uint2(v1, v2);
+            s1 += v1;
+            s2 += v2;
+        }
+    }
+    uint64_t t1, t2;
+    block_excl_scan(s1, t1);
+    block_excl_scan(s2, t2);
+    if (threadIdx.x == 0) {
+        sums1[blockIdx.x] = t1;
+        sums2[blockIdx.x] = t2;
+    }
+}
+
 // One block: exclusive scans, in place, of both tile-sum arrays (nb <= SCAN_TILE entries each).
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_tiles(uint64_t* __restrict__ sums1, uint64_t* __restrict__ sums2,
                                                             uint32_t nb) {
@@ -264,6 +301,20 @@ hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32
     if (nb64 > SCAN_TILE) return hipErrorInvalidValue;  // callers scan such sizes with scan_u32
     const uint32_t nb = (uint32_t)nb64;
     k_scan2_reduce<<<nb, SCAN_BLOCK, 0, st>>>(a1, b1, a2, idx, n, tmp, sums, sums + nb);
+    const bool fold = nb <= SCAN_FOLD_TILES;
+    if (!fold) k_scan2_tiles<<<1, SCAN_BLOCK, 0, st>>>(sums, sums + nb, nb);
+    k_scan2_apply<<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, fold);
+    return hipGetLastError();
+}
+
+hipError_t scan2_cand_rows(const uint32_t* candStart, const uint32_t* tStart, const uint32_t* tC, const uint8_t* acc,
+                           const uint32_t* numAnal, uint64_t n, uint32_t* out1, uint32_t* out2,
+                           unsigned long long* total1, unsigned long long* total2, uint2* tmp, uint64_t* sums,
+                           const DevCounters* ctr, hipStream_t st) {
+    const uint64_t nb64 = n ? (n + SCAN2_TILE - 1) / SCAN2_TILE : 1;
+    if (nb64 > SCAN_TILE) return hipErrorInvalidValue;
+    const uint32_t nb = (uint32_t)nb64;
+    k_scan2_reduce_cand<<<nb, SCAN_BLOCK, 0, st>>>(candStart, tStart, tC, acc, numAnal, n, tmp, sums, sums + nb, ctr);
     const bool fold = nb <= SCAN_FOLD_TILES;
     if (!fold) k_scan2_tiles<<<1, SCAN_BLOCK, 0, st>>>(sums, sums + nb, nb);
     k_scan2_apply<<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, fold);
