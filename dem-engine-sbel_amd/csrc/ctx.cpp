@@ -120,16 +120,18 @@ struct deme_ctx {
     // the skin; each pass then tests only them. skinSet < 0: automatic (0.2 x the mean radius); 0: off (the cell
     // sweep every pass).
     // The automatic skin (any skin gives the same contacts) adapts at each build, within 0.05 to 0.8 x the mean
-    // radius, to the cost per pass of the lists just retired: their candidates times (1 + BUILD_PASSES / the passes
-    // they lasted), a build costing about as much as BUILD_PASSES passes over its candidates. It moves 1.25x in one
-    // direction while that cost falls and turns back when it rises; lists that die within a pass widen it 1.5x, lists
-    // that outlast 400 passes narrow it 0.75x.
+    // radius, to the lists just retired. Lists that die within a pass widen it 1.5x; lists that outlast 400 passes
+    // narrow it 0.75x. Lists that last 30 passes or more leave it alone (their builds cost little). Shorter-lived
+    // lists steer it by their cost per pass -- candidates times (1 + BUILD_PASSES / the passes they lasted), a build
+    // costing about as much as BUILD_PASSES passes over its candidates: it moves 1.25x (widening first when lists
+    // last fewer than 8 passes) while that cost falls by more than 2%, turns back when it rises by more than 2%, and
+    // stops when it changes less.
     // With the automatic skin, lists that last fewer than 4 passes at the widest skin do not pay for their builds:
     // the cell sweep then runs every pass for cellPassesLeft passes (64, doubling while that repeats, up to 4096),
     // after which the lists are tried again.
     double skinSet = -1.0, skinAuto = 0.0;
     double skinCostPrev = -1.0;  // cost per pass of the lists before the ones just retired (-1: none)
-    int skinDir = 1;
+    int skinDir = 0;
     uint32_t cellPassesLeft = 0, cellBackoff = 0;
     bool candValid = false, rebuildNext = true;
     uint64_t nCand = 0, candBuilds = 0, passesSinceBuild = 0;
@@ -1139,14 +1141,23 @@ static int build_candidates(deme_ctx* c) {
         const double m = (double)c->meanRadius, cur = c->skin();
         const double life = (double)std::max<uint64_t>(c->passesSinceBuild, 1);
         const double cost = (double)std::max<uint64_t>(c->nCand, 1) * (1.0 + BUILD_PASSES / life);
+        const uint64_t L = c->passesSinceBuild;
         double next = cur;
-        if (c->passesSinceBuild <= 1) {
+        if (L <= 1) {
             next = cur * 1.5;
-        } else if (c->passesSinceBuild > 400) {
+        } else if (L > 400) {
             next = cur * 0.75;
+        } else if (L >= 30) {
+            c->skinDir = 0;
         } else {
-            if (c->skinCostPrev >= 0.0 && cost > c->skinCostPrev) c->skinDir = -c->skinDir;
-            next = c->skinDir > 0 ? cur * 1.25 : cur * 0.8;
+            if (c->skinCostPrev >= 0.0 && c->skinDir != 0) {
+                const double rel = cost / c->skinCostPrev;
+                if (rel > 1.02) c->skinDir = -c->skinDir;
+                else if (rel >= 0.98) c->skinDir = 0;
+            }
+            if (c->skinDir == 0 && L < 8) c->skinDir = 1;
+            if (c->skinDir > 0) next = cur * 1.25;
+            else if (c->skinDir < 0) next = cur * 0.8;
         }
         c->skinCostPrev = cost;
         c->skinAuto = std::min(std::max(next, 0.05 * m), 0.8 * m);
@@ -1916,7 +1927,7 @@ int deme_set_skin(deme_ctx* c, double skin) {
     c->skinSet = skin < 0.0 ? -1.0 : skin;
     c->skinAuto = 0.0;
     c->skinCostPrev = -1.0;
-    c->skinDir = 1;
+    c->skinDir = 0;
     c->cellPassesLeft = 0;
     c->cellBackoff = 0;
     c->candValid = false;
