@@ -153,11 +153,14 @@ int deme_set_graph_headroom(deme_ctx* ctx, double factor, uint64_t slack);
  * bin rule) on those pairs only, which are all the pairs that can satisfy it -- the contact lists are the same as
  * the full cell sweep's. The pass measures the largest displacement itself; a pass that finds the lists stale
  * rebuilds them and repeats, and a displacement past 0.35 x skin rebuilds before the next pass. skin < 0 (default):
- * automatic, starting at 0.2 x the mean sphere radius and adapted at each build to how many passes the previous lists
- * lasted (widened 1.5x below 8, up to 0.8 x the mean radius; narrowed 0.75x above 400, down to 0.05 x); 0: off (the
- * cell sweep every pass). */
+ * automatic, starting at 0.2 x the mean sphere radius and adapted at each build, within 0.05-0.8 x, to the lists it
+ * retires (widened 1.5x when they died within a pass, narrowed 0.75x when they outlasted 400 passes, left alone from
+ * 30 passes on; lists shorter-lived than that steer it 1.25x at a time by their candidates x (1 + 2.7 / passes
+ * lasted)); when lists die within 4 passes even at the widest skin, the cell sweep runs every pass for 64 passes
+ * (doubling while that repeats, up to 4096) before lists are tried again. 0: off (the cell sweep every pass). */
 int deme_set_skin(deme_ctx* ctx, double skin);
-/* The skin in force, the candidate pairs of the last build and the number of builds so far (each nullable). */
+/* The skin in force (0 while the automatic mode runs the cell sweep), the candidate pairs of the last build and the
+ * number of builds so far (each nullable). */
 int deme_candidate_stats(deme_ctx* ctx, double* skin, uint64_t* nCandidates, uint64_t* builds);
 /* How the pair sweep of the cell-list detection tests its candidates, and its workgroup size (no counterpart in the
  * reference; the contact lists are the same in every configuration): mode 0 the double-precision overlap test on
