@@ -83,8 +83,11 @@ def bytes_model(name, k):
         # per sphere plane bits, transposed and mirror row bounds; per candidate its transposed index and flag; per
         # sphere-sphere contact its contact index and mirror link out
         "cand_mirror": S * (8 + 4 + 4) + Nk * (4 + 1) + Np * (4 + 4),
-        # the build's mirror columns sorted into id order and each entry's candidate found in its partner's row
-        "cand_transpose": S * (4 + 4 + 4) + Nk * (4 + 4 + 4),
+        # the transpose counted from the rows: per candidate its partner in, a 4-byte RMW and its place out; then
+        # placed (partner, place, row start in; index out); then each transposed row sorted in place
+        "cand_tcount": Nk * (4 + 4 + 4),
+        "cand_tscatter": Nk * (4 + 4 + 4 + 4),
+        "cand_tsort": S * 4 + Nk * (4 + 4),
         # the build sweep: as the cell sweep's pass with every candidate kept
         "cand_build": S * (32 + 4 + 4) + O * 4 + Nc * 4 + Nk * (4 + 4 + 8),
         "scan_cand": S * 4 * 2,

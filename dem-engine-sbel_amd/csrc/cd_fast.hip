@@ -433,7 +433,7 @@ __device__ __forceinline__ void finish_pair(const KParams& P, double inv, uint32
     const uint32_t loIdx = iLow ? i : j, hiIdx = iLow ? j : i;
     const uint32_t loId = iLow ? idI : idJ, hiId = iLow ? idJ : idI;
     append_pair(aCnt, rowSlab, loIdx, hiId, nS, slabCap, ctr);
-    append_pair(bCnt, mirSlab, hiIdx, loId, nS, slabCap, ctr);
+    if (CHECK != CHK_CAND) append_pair(bCnt, mirSlab, hiIdx, loId, nS, slabCap, ctr);  // (candidates: rows only)
 }
 
 // Sweep modes: the exact test on every candidate (PAIR_EXACT); a float pre-reject with the exact test on the
@@ -582,10 +582,18 @@ __global__ __launch_bounds__(BLOCK) void k_cell_pairs(KParams P, CellGrid G, uin
             qI[q] = REJECTED;
             continue;
         }
-        // i lower id: i's contact row gets j and j's mirror row gets i; else the other way round
+        // i lower id: i's contact row gets j and j's mirror row gets i; else the other way round. A candidate build
+        // keeps rows only (its transpose is counted from the rows): no mirror entries.
         const uint32_t kind = idI < idJ ? 0u : 1u;
-        if (kind == 0) append_pair(bCnt, mirSlab, sj, idI, nS, slabCap, ctr);
-        else append_pair(aCnt, rowSlab, sj, idI, nS, slabCap, ctr);
+        if (kind == 0) {
+            if (MODE != PAIR_CAND) append_pair(bCnt, mirSlab, sj, idI, nS, slabCap, ctr);
+        } else {
+            append_pair(aCnt, rowSlab, sj, idI, nS, slabCap, ctr);
+            if (MODE == PAIR_CAND) {
+                qI[q] = REJECTED;  // no mirror entry of its own
+                continue;
+            }
+        }
         const uint32_t local = si - blockBase;
         const uint32_t ls = atomicAdd(&selfCnt[kind][local], 1u);
         qI[q] = queue_code(kind, local, ls);  // ls < PAIR_QUEUE
@@ -868,37 +876,59 @@ __global__ __launch_bounds__(256) void k_cand_mirror(KParams P, const SphereRec*
     }
 }
 
-// After a candidate build, per sphere B: its transposed row -- the build's mirror column (the lower-id partners,
-// ascending) with each partner A replaced by the index of the candidate (A, B) in A's row. Every mirror entry was
-// appended with its row entry, so the search finds it; if it did not, the build is refused (ERRBIT_PAIR_OVERFLOW).
-__global__ __launch_bounds__(256) void k_cand_transpose(uint32_t nS, const uint32_t* __restrict__ mirSlab,
-                                                        const uint32_t* __restrict__ posOf,
-                                                        const uint32_t* __restrict__ tStart,
-                                                        const uint32_t* __restrict__ candStart,
-                                                        const uint32_t* __restrict__ candB, uint32_t* __restrict__ tC,
-                                                        DevCounters* ctr) {
+// The transpose of a candidate build, counted from its rows: per candidate, its place among the candidates with the
+// same partner (an atomic count per partner, in no particular order) ...
+__global__ __launch_bounds__(256) void k_cand_tcount(uint32_t nK, const uint32_t* __restrict__ candB,
+                                                     uint32_t* __restrict__ tCnt, uint32_t* __restrict__ slot) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nK) return;
+    slot[c] = atomicAdd(&tCnt[candB[c]], 1u);
+}
+
+// ... each candidate stored at that place of its partner's transposed row (tStart from a scan of the counts) ...
+__global__ __launch_bounds__(256) void k_cand_tscatter(uint32_t nK, const uint32_t* __restrict__ candB,
+                                                       const uint32_t* __restrict__ slot,
+                                                       const uint32_t* __restrict__ tStart,
+                                                       uint32_t* __restrict__ tC) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nK) return;
+    tC[tStart[candB[c]] + slot[c]] = c;
+}
+
+// ... and each transposed row sorted ascending (candidate order is row order, so ascending A), in place: rows of up to
+// SORT_REG entries in registers, longer ones by insertion.
+__global__ __launch_bounds__(256) void k_cand_tsort(uint32_t nS, const uint32_t* __restrict__ tStart,
+                                                    uint32_t* __restrict__ tC) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
     const uint32_t t0 = tStart[s], n = tStart[s + 1] - t0;
-    sorted_copy(mirSlab + posOf[s], n, nS, tC + t0);
-    for (uint32_t k = 0; k < n; k++) {
-        const uint32_t a = tC[t0 + k];
-        uint32_t found = 0xFFFFFFFFu;
-        if (a < nS) {
-            // A's row ascends in partner id: the first entry not below s
-            uint32_t l = candStart[a], h = candStart[a + 1];
-            while (l < h) {
-                const uint32_t m = l + (h - l) / 2;
-                if (candB[m] < s) l = m + 1;
-                else h = m;
+    if (n < 2) return;
+    uint32_t* row = tC + t0;
+    if (n <= SORT_REG) {
+        uint32_t v[SORT_REG];
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_REG; k++) v[k] = k < n ? row[k] : 0xFFFFFFFFu;
+#pragma unroll
+        for (uint32_t i = 1; i < SORT_REG; i++)
+#pragma unroll
+            for (uint32_t j = i; j > 0; j--) {
+                const uint32_t lo = min(v[j - 1], v[j]), hi = max(v[j - 1], v[j]);
+                v[j - 1] = lo;
+                v[j] = hi;
             }
-            if (l < candStart[a + 1] && candB[l] == s) found = l;
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_REG; k++)
+            if (k < n) row[k] = v[k];
+        return;
+    }
+    for (uint32_t k = 1; k < n; k++) {
+        const uint32_t v = row[k];
+        uint32_t q = k;
+        while (q > 0 && row[q - 1] > v) {
+            row[q] = row[q - 1];
+            q--;
         }
-        if (found == 0xFFFFFFFFu) {
-            atomicOr(&ctr->err, ERRBIT_PAIR_OVERFLOW);
-            continue;  // the build is refused
-        }
-        tC[t0 + k] = found;
+        row[q] = v;
     }
 }
 
@@ -1030,11 +1060,22 @@ hipError_t launch_cand_mirror(const KParams& P, const SphereRec* sph, const Owne
     return hipGetLastError();
 }
 
-hipError_t launch_cand_transpose(uint32_t nS, const uint32_t* mirSlab, const uint32_t* posOf, const uint32_t* tStart,
-                                 const uint32_t* candStart, const uint32_t* candB, uint32_t* tC, DevCounters* ctr,
-                                 hipStream_t st) {
+hipError_t launch_cand_tcount(uint32_t nK, const uint32_t* candB, uint32_t* tCnt, uint32_t* slot, hipStream_t st) {
+    if (nK == 0) return hipSuccess;
+    k_cand_tcount<<<blocks(nK, 256), 256, 0, st>>>(nK, candB, tCnt, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_cand_tscatter(uint32_t nK, const uint32_t* candB, const uint32_t* slot, const uint32_t* tStart,
+                                uint32_t* tC, hipStream_t st) {
+    if (nK == 0) return hipSuccess;
+    k_cand_tscatter<<<blocks(nK, 256), 256, 0, st>>>(nK, candB, slot, tStart, tC);
+    return hipGetLastError();
+}
+
+hipError_t launch_cand_tsort(uint32_t nS, const uint32_t* tStart, uint32_t* tC, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cand_transpose<<<blocks(nS, 256), 256, 0, st>>>(nS, mirSlab, posOf, tStart, candStart, candB, tC, ctr);
+    k_cand_tsort<<<blocks(nS, 256), 256, 0, st>>>(nS, tStart, tC);
     return hipGetLastError();
 }
 

@@ -1184,9 +1184,6 @@ static int build_candidates(deme_ctx* c) {
         DEME_LAUNCH(c, "scan_cand",
                     scan_u32(c->aCnt.p, nullptr, nS, c->candStart.p, &c->ctr.p->total[0], c->scanScratch.p,
                              c->stream, c->posOf.p));
-        DEME_LAUNCH(c, "scan_cand",
-                    scan_u32(c->bCnt.p, nullptr, nS, c->tStart.p, &c->ctr.p->total[2], c->scanScratch.p, c->stream,
-                             c->posOf.p));
         r = readback_counters(c);
         if (r) return r;
         const DevCounters& h = *c->hctr;
@@ -1202,7 +1199,6 @@ static int build_candidates(deme_ctx* c) {
     }
     c->nCand = c->hctr->total[0];
     if (c->nCand > 0xFFFFFFFFull) return c->fail(DEME_ERR_CAPACITY, "candidate count exceeds 2^32 - 1");
-    if (c->hctr->total[2] != c->nCand) return c->fail(DEME_ERR_HIP, "inconsistent candidate row and mirror counts");
     const size_t nK = std::max<size_t>(c->nCand, 1);
     DEME_HIP(c, c->candA.ensure(nK));
     DEME_HIP(c, c->candB.ensure(nK));
@@ -1214,12 +1210,15 @@ static int build_candidates(deme_ctx* c) {
     DEME_LAUNCH(c, "cand_rows",
                 launch_cand_rows(nS, c->rowSlab.p, c->posOf.p, c->candStart.p, c->candA.p, c->candB.p, c->sorted.p,
                                  c->pos0.p, c->stream));
-    DEME_LAUNCH(c, "cand_transpose",
-                launch_cand_transpose(nS, c->mirSlab.p, c->posOf.p, c->tStart.p, c->candStart.p, c->candB.p, c->tC.p,
-                                      c->ctr.p, c->stream));
-    r = readback_counters(c);
-    if (r) return r;
-    if (c->hctr->err) return c->fail(DEME_ERR_HIP, "candidate rows and their transpose disagree");
+    // the transpose, counted from the rows (cidx[0] holds each candidate's place until the first pass writes it)
+    const uint32_t nK32 = (uint32_t)c->nCand;
+    DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, nS, 0, c->stream));
+    DEME_LAUNCH(c, "cand_tcount", launch_cand_tcount(nK32, c->candB.p, c->bCnt.p, c->cidx[0].p, c->stream));
+    DEME_LAUNCH(c, "scan_cand",
+                scan_u32(c->bCnt.p, nullptr, nS, c->tStart.p, &c->ctr.p->total[2], c->scanScratch.p, c->stream));
+    DEME_LAUNCH(c, "cand_tscatter",
+                launch_cand_tscatter(nK32, c->candB.p, c->cidx[0].p, c->tStart.p, c->tC.p, c->stream));
+    DEME_LAUNCH(c, "cand_tsort", launch_cand_tsort(nS, c->tStart.p, c->tC.p, c->stream));
     c->candValid = true;
     c->rebuildNext = false;
     c->candBuilds++;

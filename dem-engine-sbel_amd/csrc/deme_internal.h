@@ -190,11 +190,13 @@ hipError_t launch_cand_mirror(const KParams& P, const SphereRec* sph, const Owne
                               const uint32_t* cidx, const uint32_t* bRowStart, uint32_t* idA, uint32_t* idB,
                               uint8_t* ctype, uint32_t* map, uint32_t* mirror, hipStream_t st,
                               SpecGuard guard = SpecGuard());
-// After a build: tC[tStart[B]..] = indices of the candidates with partner B, ascending A (from the build's mirror
-// columns); ERRBIT_PAIR_OVERFLOW if a mirror entry has no row entry.
-hipError_t launch_cand_transpose(uint32_t nS, const uint32_t* mirSlab, const uint32_t* posOf, const uint32_t* tStart,
-                                 const uint32_t* candStart, const uint32_t* candB, uint32_t* tC, DevCounters* ctr,
-                                 hipStream_t st);
+// After a build, the transpose from the rows: slot[c] = c's place among the candidates with its partner (tCnt counts
+// them, cleared by the caller); then, with tStart scanned from tCnt, tC[tStart[B] + slot[c]] = c; then every
+// transposed row sorted ascending (ascending A).
+hipError_t launch_cand_tcount(uint32_t nK, const uint32_t* candB, uint32_t* tCnt, uint32_t* slot, hipStream_t st);
+hipError_t launch_cand_tscatter(uint32_t nK, const uint32_t* candB, const uint32_t* slot, const uint32_t* tStart,
+                                uint32_t* tC, hipStream_t st);
+hipError_t launch_cand_tsort(uint32_t nS, const uint32_t* tStart, uint32_t* tC, hipStream_t st);
 
 // dyn.hip
 // aTerm / gTerm: 6 floats per contact; bTerm: 6 floats per mirror entry; out (optional): full contact records
