@@ -167,8 +167,12 @@ __global__ __launch_bounds__(256) void k_sphere_pos(KParams P, const SphereRec* 
                                                     const uint8_t* __restrict__ mask, CdRec* __restrict__ pos,
                                                     const CdRec* __restrict__ pos0, uint2* __restrict__ meta,
                                                     uint16_t* __restrict__ numBins, uint32_t* __restrict__ numAnal,
-                                                    unsigned long long* __restrict__ planeBits, DevCounters* ctr) {
+                                                    unsigned long long* __restrict__ planeBits, DevCounters* ctr,
+                                                    const double* __restrict__ uIn, double* __restrict__ uOut) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    // displacements since the build are measured against a common translation u (sphere 0's displacement at the
+    // previous pass; any fixed vector keeps the lists exact), so spheres that move together do not age the lists
+    const double ux = uIn[0], uy = uIn[1], uz = uIn[2];
     double d2 = 0.0;
     if (s < nS) {
         const SphereRec sr = load_sphere(sph, s);
@@ -183,7 +187,13 @@ __global__ __launch_bounds__(256) void k_sphere_pos(KParams P, const SphereRec* 
         d4[1] = s4[1];
         if (meta) meta[s] = make_uint2(sr.owner, o.fam);
         const CdRec c0 = load_cd(pos0, s);
-        const double dx = cr.x - c0.x, dy = cr.y - c0.y, dz = cr.z - c0.z;
+        const double mx = cr.x - c0.x, my = cr.y - c0.y, mz = cr.z - c0.z;
+        if (s == 0) {
+            uOut[0] = mx;
+            uOut[1] = my;
+            uOut[2] = mz;
+        }
+        const double dx = mx - ux, dy = my - uy, dz = mz - uz;
         d2 = dx * dx + dy * dy + dz * dz;
     }
 #pragma unroll
@@ -1007,10 +1017,10 @@ hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, c
 hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                              const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos, const CdRec* pos0,
                              uint2* meta, uint16_t* numBins, uint32_t* numAnal, unsigned long long* planeBits,
-                             DevCounters* ctr, hipStream_t st) {
+                             DevCounters* ctr, const double* uIn, double* uOut, hipStream_t st) {
     if (nS == 0) return hipSuccess;
     k_sphere_pos<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, pose, nA, mask, pos, pos0, meta, numBins, numAnal,
-                                                  planeBits, ctr);
+                                                  planeBits, ctr, uIn, uOut);
     return hipGetLastError();
 }
 

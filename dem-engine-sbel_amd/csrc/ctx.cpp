@@ -139,6 +139,9 @@ struct deme_ctx {
     uint64_t lastLifetime = ~0ull;
     bool builtThisPass = false;
     DBuf<CdRec> pos, pos0;
+    // the common translation the displacement test subtracts: slot p (3 doubles) written by the pass at list parity p
+    // (sphere 0's displacement since the build), read by the next pass; both zero after a build
+    DBuf<double> dispRef;
     DBuf<uint2> meta;
     DBuf<uint32_t> candStart, candB;
     // The build's candidates: candA (the row's sphere), transposed (tC[tStart[B]..tStart[B + 1]]: indices of the
@@ -478,7 +481,7 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->survSlab.release(); c->rowSlab.release(); c->mirSlab.release(); c->sorted.release(); c->planeBits.release();
     c->aCnt.release(); c->sortedMeta.release();
     c->pos.release(); c->pos0.release(); c->meta.release(); c->candStart.release(); c->candB.release();
-    c->candA.release(); c->tStart.release(); c->tC.release();
+    c->candA.release(); c->tStart.release(); c->tC.release(); c->dispRef.release();
     for (int k = 0; k < 2; k++) {
         c->cidx[k].release();
         c->candAcc[k].release();
@@ -1173,6 +1176,7 @@ static int build_candidates(deme_ctx* c) {
     DEME_HIP(c, c->candStart.ensure((size_t)nS + 1));
     DEME_HIP(c, c->tStart.ensure((size_t)nS + 1));
     DEME_HIP(c, c->pos0.ensure(nS));
+    DEME_HIP(c, c->dispRef.ensure(8, 1.0));
     for (int attempt = 0;; attempt++) {
         r = enqueue_cell_pass(c, PAIR_MODE_CAND);
         if (r) return r;
@@ -1207,6 +1211,8 @@ static int build_candidates(deme_ctx* c) {
         DEME_HIP(c, c->candAcc[k].ensure(nK));
         DEME_HIP(c, c->cidx[k].ensure(nK));
     }
+    // the build positions are the reference: no displacement, no common translation yet
+    DEME_LAUNCH(c, "fill", launch_fill_u32(reinterpret_cast<uint32_t*>(c->dispRef.p), 12, 0, c->stream));
     DEME_LAUNCH(c, "cand_rows",
                 launch_cand_rows(nS, c->rowSlab.p, c->posOf.p, c->candStart.p, c->candA.p, c->candB.p, c->sorted.p,
                                  c->pos0.p, c->stream));
@@ -1251,7 +1257,8 @@ static int enqueue_skin_pass(deme_ctx* c, int cu) {
     DEME_LAUNCH(c, "sphere_pos",
                 launch_sphere_pos(c->kp, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->maskAny ? c->mask.p : nullptr,
                                   c->pos.p, c->pos0.p, c->maskAny ? c->meta.p : nullptr, nullptr, c->numAnal.p,
-                                  c->planeBits.p, c->ctr.p, c->stream));
+                                  c->planeBits.p, c->ctr.p, c->dispRef.p + 3 * (cu ^ 1), c->dispRef.p + 3 * cu,
+                                  c->stream));
     DEME_LAUNCH(c, "cand_flags",
                 launch_cand_flags(c->kp, (uint32_t)c->nCand, c->pos.p, c->meta.p, c->candA.p, c->candB.p,
                                   c->maskAny ? c->mask.p : nullptr, stale_limit2(c), c->candAcc[cu].p, c->ctr.p,
@@ -1685,7 +1692,7 @@ static std::vector<const void*> cycle_fingerprint(deme_ctx* c, int p) {
             v(c->grid.nz),   v(invBits),      v(c->jitActive),  v(c->ownerJitActive), v(c->maskAny),
             c->pos.p,        c->pos0.p,       c->meta.p,        c->candStart.p,   c->candB.p,      v(skinBits),
             c->tStart.p,     c->tC.p,         c->candAcc[0].p,  c->candAcc[1].p,  c->cidx[0].p,    c->cidx[1].p,
-            c->candA.p,      v(c->nCand)};
+            c->candA.p,      v(c->nCand),     c->dispRef.p};
 }
 
 // Buffers of the list at parity p (the one the interval's pass writes; the other parity holds the current list and

@@ -157,10 +157,12 @@ hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerR
                             SpecGuard guard = SpecGuard());
 // Candidate-list path. Per step: detection records in id order (pos) with (owner, family) (meta), the shared
 // per-sphere detection outputs, and the largest squared displacement since the lists were built into ctr->disp2.
+// The largest squared displacement since the build is taken relative to the translation uIn (3 doubles; any vector
+// keeps the candidate lists exact); sphere 0's displacement is written to uOut for the next pass.
 hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                              const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos, const CdRec* pos0,
                              uint2* meta, uint16_t* numBins, uint32_t* numAnal, unsigned long long* planeBits,
-                             DevCounters* ctr, hipStream_t st);
+                             DevCounters* ctr, const double* uIn, double* uOut, hipStream_t st);
 // After a candidate build: per sphere its candidate row (candStart from a scan of the build's row counts; candA /
 // candB the row's sphere and partners) and its position at the build (pos0).
 hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* posOf, const uint32_t* candStart,

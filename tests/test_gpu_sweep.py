@@ -141,3 +141,32 @@ def test_automatic_skin_gives_way_to_the_cell_sweep_and_back(graphs):
     assert modes[0] and not all(modes)
     first_cell = modes.index(False)
     assert any(modes[first_cell:]), "the lists were not tried again"
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_lists_survive_a_common_translation(graphs):
+    """Every sphere drifting and falling together (0.5 m/s sideways plus gravity, clear of each other and of the
+    walls): displacements are measured against a common translation, so the candidate lists built at the start stay
+    valid over 30 passes (1.5 mm of travel against a 1 mm skin, which would otherwise be rebuilt every ~7 passes) --
+    and the contacts and trajectory stay the cell sweep's, bit for bit."""
+    sc = scenes.sphere_settling(n=4000, spacing=2.2, jitter=0.05, seed=1035)
+    a, b = DeviceSystem(sc), DeviceSystem(sc)
+    st = a.get_state()
+    st["vX"] = np.full_like(st["vX"], 0.5)
+    for d in (a, b):
+        d.set_state(st)
+        d.set_step_graphs(graphs)
+    b.set_skin(0.0)
+    a.step(300, 10)
+    b.step(300, 10)
+    sa, sb = a.get_state(), b.get_state()
+    for k in KEYS:
+        assert np.array_equal(sa[k], sb[k]), k
+    for x, y in zip(a.contacts(), b.contacts()):
+        assert np.array_equal(x, y)
+    skin, n_cand, builds = a.candidate_stats()
+    print(f"skin {skin:.3e} m, {builds} builds over 30 passes", flush=True)
+    assert skin > 0 and builds <= 2
+    if graphs:
+        captures, replays, fallbacks = a.graph_stats()
+        assert replays >= 20 and fallbacks == 0, (captures, replays, fallbacks)
