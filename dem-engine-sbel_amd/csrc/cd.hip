@@ -52,8 +52,12 @@ __device__ __forceinline__ bool plane_cd(const KParams& P, const PlanePose& pp, 
 }
 
 __global__ __launch_bounds__(64) void k_plane_pose(KParams P, const deme_anal_obj* __restrict__ anal, uint32_t nA,
-                                                   const OwnerRec* __restrict__ own, PlanePose* __restrict__ pose) {
+                                                   const OwnerRec* __restrict__ own, PlanePose* __restrict__ pose,
+                                                   uint32_t* __restrict__ clear, uint32_t nClear) {
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    // the first workgroup also clears nClear words (a detection pass's counters: one launch instead of two)
+    if (blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < nClear; k += blockDim.x) clear[k] = 0u;
     if (a >= nA) return;
     const deme_anal_obj ob = anal[a];
     const OwnerRec o = load_owner(own, ob.owner);
@@ -478,7 +482,14 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st, 
 hipError_t launch_plane_pose(const KParams& P, const deme_anal_obj* anal, uint32_t nA, const OwnerRec* own,
                              PlanePose* pose, hipStream_t st) {
     if (nA == 0) return hipSuccess;
-    k_plane_pose<<<blocks(nA, 64), 64, 0, st>>>(P, anal, nA, own, pose);
+    k_plane_pose<<<blocks(nA, 64), 64, 0, st>>>(P, anal, nA, own, pose, nullptr, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass_start(const KParams& P, const deme_anal_obj* anal, uint32_t nA, const OwnerRec* own,
+                             PlanePose* pose, uint32_t* clear, uint32_t nClear, hipStream_t st) {
+    if (nA == 0 && nClear == 0) return hipSuccess;
+    k_plane_pose<<<nA ? blocks(nA, 64) : 1u, 64, 0, st>>>(P, anal, nA, own, pose, clear, nClear);
     return hipGetLastError();
 }
 

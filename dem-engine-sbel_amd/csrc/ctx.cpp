@@ -1066,7 +1066,7 @@ static int cd_buffers(deme_ctx* c, int* mode) {
     if (*mode != 0) DEME_HIP(c, c->sortedF.ensure(nS));
     DEME_HIP(c, c->planeBits.ensure(nS));
     DEME_HIP(c, c->scanTmp.ensure(nS));
-    DEME_HIP(c, c->scan2Sums.ensure(2 * ((nS + SCAN2_TILE - 1) / SCAN2_TILE + 1)));
+    DEME_HIP(c, c->scan2Sums.ensure(scan2_sums_elems(nS)));
     DEME_HIP(c, c->survSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->rowSlab.ensure((size_t)nS * c->slabCap, 1.0));
     DEME_HIP(c, c->mirSlab.ensure((size_t)nS * c->slabCap, 1.0));
@@ -1239,7 +1239,7 @@ static int skin_buffers(deme_ctx* c) {
     DEME_HIP(c, c->aCnt.ensure(nS));
     DEME_HIP(c, c->planeBits.ensure(nS));
     DEME_HIP(c, c->scanTmp.ensure(nS));
-    DEME_HIP(c, c->scan2Sums.ensure(2 * ((nS + SCAN2_TILE - 1) / SCAN2_TILE + 1)));
+    DEME_HIP(c, c->scan2Sums.ensure(scan2_sums_elems(nS)));
     DEME_HIP(c, c->scanScratch.ensure(scan_scratch_elems(std::max<uint64_t>(nS, 1u << 22)), 1.0));
     return DEME_OK;
 }
@@ -1249,10 +1249,10 @@ static int skin_buffers(deme_ctx* c) {
 // counts, the row and mirror-row scans.
 static int enqueue_skin_pass(deme_ctx* c, int cu) {
     const uint32_t nS = c->nS;
-    DEME_LAUNCH(c, "plane_pose", launch_plane_pose(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, c->stream));
     static_assert(sizeof(DevCounters) % 4 == 0, "counters are cleared as 32-bit words");
-    DEME_LAUNCH(c, "fill", launch_fill_u32(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(DevCounters) / 4, 0,
-                                           c->stream));
+    DEME_LAUNCH(c, "plane_pose",
+                launch_pass_start(c->kp, c->anal.p, c->nA, c->own.p, c->pose.p, reinterpret_cast<uint32_t*>(c->ctr.p),
+                                  sizeof(DevCounters) / 4, c->stream));
     // (the families for the mask only when a mask is in force; no per-sphere bin counts: this path never reads them)
     DEME_LAUNCH(c, "sphere_pos",
                 launch_sphere_pos(c->kp, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->maskAny ? c->mask.p : nullptr,

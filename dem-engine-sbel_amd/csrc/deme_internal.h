@@ -46,6 +46,12 @@ constexpr uint64_t SCAN_TILE = (uint64_t)SCAN_BLOCK * SCAN_ITEMS;
 // the gathered two-component scan uses half-size tiles: twice the workgroups to cover its gathers' latency
 constexpr int SCAN2_ITEMS = 8;
 constexpr uint64_t SCAN2_TILE = (uint64_t)SCAN_BLOCK * SCAN2_ITEMS;
+// up to this many spheres the candidate-list row scan uses one sphere per lane (tiles of SCAN_BLOCK)
+constexpr uint64_t SCAN2_SMALL_N = 65536;
+// tile sums the two-component scans need for n elements (both components)
+constexpr uint64_t scan2_sums_elems(uint64_t n) {
+    return 2 * ((n <= SCAN2_SMALL_N ? (n + SCAN_BLOCK - 1) / SCAN_BLOCK : (n + SCAN2_TILE - 1) / SCAN2_TILE) + 1);
+}
 constexpr uint32_t MAX_ANAL_OWNERS = 8;
 // Plane-owner sums: each workgroup of k_anal_collect folds one tile of ANAL_TILE contacts (256 lanes x ANAL_ITEMS)
 // into nAnalOwners x 6 partials, so the partials buffer holds anal_blocks(nC) x nAnalOwners x 6 floats.
@@ -77,7 +83,7 @@ constexpr uint64_t NOALLOC_SCAN_MAX = 1ull << 28;
 // is looked up once per device and kept); *elems (nullable) = its length in u64 values.
 hipError_t noalloc_scan_scratch(int device, uint64_t** ptr, uint64_t* elems);
 // Two scans over one gather: out1 = scan(a1[idx[i]] + (b1 ? b1[i] : 0)), out2 = scan(a2[idx[i]]), totals at out[n]
-// and in *total1 / *total2. tmp: n uint2; sums: 2 * ceil(n / SCAN2_TILE) u64. n must be at most
+// and in *total1 / *total2. tmp: n uint2; sums: scan2_sums_elems(n) u64. n must be at most
 // SCAN2_TILE * SCAN_TILE (hipErrorInvalidValue otherwise).
 hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32_t* a2, const uint32_t* idx,
                             uint64_t n, uint32_t* out1, uint32_t* out2, unsigned long long* total1,
@@ -93,6 +99,9 @@ hipError_t scan2_cand_rows(const uint32_t* candStart, const uint32_t* tStart, co
 // cd.hip
 hipError_t launch_plane_pose(const KParams& P, const deme_anal_obj* anal, uint32_t nA, const OwnerRec* own,
                              PlanePose* pose, hipStream_t st);
+// The plane poses and, in the same launch, nClear words of `clear` zeroed (a pass's counters).
+hipError_t launch_pass_start(const KParams& P, const deme_anal_obj* anal, uint32_t nA, const OwnerRec* own,
+                             PlanePose* pose, uint32_t* clear, uint32_t nClear, hipStream_t st);
 hipError_t launch_sphere_bins(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                               const PlanePose* pose, uint32_t nA, const uint8_t* mask, uint32_t* binCount,
                               uint16_t* numBins, uint32_t* numAnal, CdRec* rec, DevCounters* ctr, hipStream_t st);

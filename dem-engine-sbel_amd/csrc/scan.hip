@@ -168,6 +168,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce(const uint32_t* __r
 // Pass 1 of the candidate-list pass's row scans: the counts are formed here, not read -- per sphere the touching
 // candidates of its row plus its plane contacts, and the touching candidates with it as partner (through its
 // transposed row) -- stored in tmp and folded into the tile sums. A failed pass (ctr->err) leaves them alone.
+template <int ITEMS>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce_cand(const uint32_t* __restrict__ candStart,
                                                                   const uint32_t* __restrict__ tStart,
                                                                   const uint32_t* __restrict__ tC,
@@ -178,10 +179,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce_cand(const uint32_t
                                                                   uint64_t* __restrict__ sums2,
                                                                   const DevCounters* ctr) {
     if (ctr->err) return;
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN2_TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * (uint64_t)(SCAN_BLOCK * ITEMS);
     uint64_t s1 = 0, s2 = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN2_ITEMS; i++) {
+    for (int i = 0; i < ITEMS; i++) {
         const uint64_t k = base + (uint64_t)i * SCAN_BLOCK + threadIdx.x;
         if (k < n) {
             uint32_t v1 = numAnal[k], v2 = 0;
@@ -225,6 +226,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_tiles(uint64_t* __restrict
     }
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restrict__ tmp, uint64_t n,
                                                             const uint64_t* __restrict__ offs1,
                                                             const uint64_t* __restrict__ offs2,
@@ -232,11 +234,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
                                                             unsigned long long* total1, unsigned long long* total2,
                                                             bool fold) {
     // +1 padding per 16 entries: the per-lane contiguous walk hits distinct banks
-    __shared__ uint32_t t1[SCAN2_TILE + SCAN2_TILE / 16], t2[SCAN2_TILE + SCAN2_TILE / 16];
+    constexpr uint32_t TILE = SCAN_BLOCK * ITEMS;
+    __shared__ uint32_t t1[TILE + TILE / 16], t2[TILE + TILE / 16];
     auto at = [](uint32_t k) { return k + (k >> 4); };
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN2_TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * TILE;
 #pragma unroll
-    for (int i = 0; i < SCAN2_ITEMS; i++) {
+    for (int i = 0; i < ITEMS; i++) {
         const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
         const uint2 v = base + k < n ? tmp[base + k] : make_uint2(0u, 0u);
         t1[at(k)] = v.x;
@@ -245,9 +248,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
     __syncthreads();
     uint64_t s1 = 0, s2 = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN2_ITEMS; i++) {
-        s1 += t1[at(threadIdx.x * SCAN2_ITEMS + i)];
-        s2 += t2[at(threadIdx.x * SCAN2_ITEMS + i)];
+    for (int i = 0; i < ITEMS; i++) {
+        s1 += t1[at(threadIdx.x * ITEMS + i)];
+        s2 += t2[at(threadIdx.x * ITEMS + i)];
     }
     uint64_t tot1, tot2;
     uint64_t o1, o2;
@@ -267,8 +270,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
     uint64_t r1 = block_excl_scan(s1, tot1) + o1;
     uint64_t r2 = block_excl_scan(s2, tot2) + o2;
 #pragma unroll
-    for (int i = 0; i < SCAN2_ITEMS; i++) {
-        const uint32_t k = at(threadIdx.x * SCAN2_ITEMS + i);
+    for (int i = 0; i < ITEMS; i++) {
+        const uint32_t k = at(threadIdx.x * ITEMS + i);
         const uint32_t v1 = t1[k], v2 = t2[k];
         t1[k] = (uint32_t)r1;
         t2[k] = (uint32_t)r2;
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_apply(const uint2* __restr
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < SCAN2_ITEMS; i++) {
+    for (int i = 0; i < ITEMS; i++) {
         const uint32_t k = (uint32_t)i * SCAN_BLOCK + threadIdx.x;
         if (base + k < n) {
             out1[base + k] = t1[at(k)];
@@ -303,7 +306,7 @@ hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32
     k_scan2_reduce<<<nb, SCAN_BLOCK, 0, st>>>(a1, b1, a2, idx, n, tmp, sums, sums + nb);
     const bool fold = nb <= SCAN_FOLD_TILES;
     if (!fold) k_scan2_tiles<<<1, SCAN_BLOCK, 0, st>>>(sums, sums + nb, nb);
-    k_scan2_apply<<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, fold);
+    k_scan2_apply<SCAN2_ITEMS><<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, fold);
     return hipGetLastError();
 }
 
@@ -311,13 +314,22 @@ hipError_t scan2_cand_rows(const uint32_t* candStart, const uint32_t* tStart, co
                            const uint32_t* numAnal, uint64_t n, uint32_t* out1, uint32_t* out2,
                            unsigned long long* total1, unsigned long long* total2, uint2* tmp, uint64_t* sums,
                            const DevCounters* ctr, hipStream_t st) {
-    const uint64_t nb64 = n ? (n + SCAN2_TILE - 1) / SCAN2_TILE : 1;
+    if (n <= SCAN2_SMALL_N) {
+        // small lists: one sphere per lane, so the latency chain of each lane is one row
+        const uint32_t nb = (uint32_t)(n ? (n + SCAN_BLOCK - 1) / SCAN_BLOCK : 1);
+        k_scan2_reduce_cand<1><<<nb, SCAN_BLOCK, 0, st>>>(candStart, tStart, tC, acc, numAnal, n, tmp, sums, sums + nb,
+                                                          ctr);
+        k_scan2_apply<1><<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, true);
+        return hipGetLastError();
+    }
+    const uint64_t nb64 = (n + SCAN2_TILE - 1) / SCAN2_TILE;
     if (nb64 > SCAN_TILE) return hipErrorInvalidValue;
     const uint32_t nb = (uint32_t)nb64;
-    k_scan2_reduce_cand<<<nb, SCAN_BLOCK, 0, st>>>(candStart, tStart, tC, acc, numAnal, n, tmp, sums, sums + nb, ctr);
+    k_scan2_reduce_cand<SCAN2_ITEMS><<<nb, SCAN_BLOCK, 0, st>>>(candStart, tStart, tC, acc, numAnal, n, tmp, sums,
+                                                                sums + nb, ctr);
     const bool fold = nb <= SCAN_FOLD_TILES;
     if (!fold) k_scan2_tiles<<<1, SCAN_BLOCK, 0, st>>>(sums, sums + nb, nb);
-    k_scan2_apply<<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, fold);
+    k_scan2_apply<SCAN2_ITEMS><<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, fold);
     return hipGetLastError();
 }
 
