@@ -168,12 +168,14 @@ __global__ __launch_bounds__(256) void k_sphere_pos(KParams P, const SphereRec* 
                                                     const CdRec* __restrict__ pos0, uint2* __restrict__ meta,
                                                     uint16_t* __restrict__ numBins, uint32_t* __restrict__ numAnal,
                                                     unsigned long long* __restrict__ planeBits, DevCounters* ctr,
-                                                    const double* __restrict__ uIn, double* __restrict__ uOut) {
+                                                    const double* __restrict__ uIn) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    // displacements since the build are measured against a common translation u (sphere 0's displacement at the
-    // previous pass; any fixed vector keeps the lists exact), so spheres that move together do not age the lists
+    // displacements since the build are measured against a common translation u (the midpoint of the displacement
+    // extremes of a sample -- each workgroup's first sphere -- at the previous pass; any fixed vector keeps the lists
+    // exact), so spheres that move together do not age the lists; this pass's sample extremes are folded for the next
     const double ux = uIn[0], uy = uIn[1], uz = uIn[2];
     double d2 = 0.0;
+    double m3[3] = {0.0, 0.0, 0.0};
     if (s < nS) {
         const SphereRec sr = load_sphere(sph, s);
         const OwnerRec o = load_owner(own, sr.owner);
@@ -188,11 +190,9 @@ __global__ __launch_bounds__(256) void k_sphere_pos(KParams P, const SphereRec* 
         if (meta) meta[s] = make_uint2(sr.owner, o.fam);
         const CdRec c0 = load_cd(pos0, s);
         const double mx = cr.x - c0.x, my = cr.y - c0.y, mz = cr.z - c0.z;
-        if (s == 0) {
-            uOut[0] = mx;
-            uOut[1] = my;
-            uOut[2] = mz;
-        }
+        m3[0] = mx;
+        m3[1] = my;
+        m3[2] = mz;
         const double dx = mx - ux, dy = my - uy, dz = mz - uz;
         d2 = dx * dx + dy * dy + dz * dz;
     }
@@ -202,8 +202,14 @@ __global__ __launch_bounds__(256) void k_sphere_pos(KParams P, const SphereRec* 
     if ((threadIdx.x & 63) == 0) waveMax[threadIdx.x >> 6] = d2;
     __syncthreads();
     if (threadIdx.x == 0) {
+        // (lane 0 of every workgroup holds a sphere: the grid covers nS exactly)
+        const uint32_t slot = blockIdx.x % DISP_SLOTS;
         const double m = fmax(fmax(waveMax[0], waveMax[1]), fmax(waveMax[2], waveMax[3]));
-        if (m > 0.0) atomicMax(&ctr->disp2[blockIdx.x % DISP_SLOTS], (unsigned long long)__double_as_longlong(m));
+        if (m > 0.0) atomicMax(&ctr->disp2[slot], (unsigned long long)__double_as_longlong(m));
+        for (int a = 0; a < 3; a++) {
+            atomicMax(&ctr->dispLo[a][slot], ~ordered_bits(m3[a]));
+            atomicMax(&ctr->dispHi[a][slot], ordered_bits(m3[a]));
+        }
     }
 }
 
@@ -705,13 +711,14 @@ __global__ __launch_bounds__(256) void k_cell_rows(KParams P, const SphereRec* _
 // Candidate lists from a candidate build: sphere s's row of higher-id partners within the skin (ascending), copied
 // from its slab column at its sorted position; the sphere's position at the build is kept for the displacement test.
 __global__ __launch_bounds__(256) void k_cand_rows(uint32_t nS, const uint32_t* __restrict__ rowSlab,
-                                                   const uint32_t* __restrict__ posOf,
                                                    const uint32_t* __restrict__ candStart, uint32_t* __restrict__ candA,
                                                    uint32_t* __restrict__ candB, const CdRec* __restrict__ sorted,
                                                    CdRec* __restrict__ pos0) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nS) return;
-    const uint32_t p = posOf[s];
+    // one lane per sorted position: the lanes of a wave read adjacent slab columns (coalesced) and write their
+    // spheres' rows wherever the ids put them
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nS) return;
+    const uint32_t s = sorted[p].id;
     const uint32_t st = candStart[s], en = candStart[s + 1];
     sorted_copy(rowSlab + p, en - st, nS, candB + st);
     for (uint32_t c = st; c < en; c++) candA[c] = s;
@@ -737,9 +744,22 @@ __global__ __launch_bounds__(256) void k_cand_flags(KParams P, uint32_t nK, cons
                                                     const uint32_t* __restrict__ candA,
                                                     const uint32_t* __restrict__ candB,
                                                     const uint8_t* __restrict__ mask, double dispLimit2,
-                                                    uint8_t* __restrict__ acc, DevCounters* ctr) {
+                                                    uint8_t* __restrict__ acc, DevCounters* ctr,
+                                                    double* __restrict__ uOut) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (ctr->err) return;
+    if (c == 0) {
+        // the next pass's common translation: the midpoint of this pass's displacement extremes, per axis
+        for (int a = 0; a < 3; a++) {
+            unsigned long long l = 0, h = 0;
+            for (uint32_t k = 0; k < DISP_SLOTS; k++) {
+                l = max(l, ctr->dispLo[a][k]);
+                h = max(h, ctr->dispHi[a][k]);
+            }
+            const double u = h ? 0.5 * (from_ordered_bits(~l) + from_ordered_bits(h)) : 0.0;  // (no spheres: none)
+            uOut[a] = isfinite(u) ? u : 0.0;  // (a non-finite position must not hide a displacement)
+        }
+    }
     // (the grid has at least one lane, so a build without candidates is still found stale)
     if (max_disp2(*ctr) > dispLimit2) {
         if (c == 0) atomicOr(&ctr->err, ERRBIT_STALE);
@@ -1017,25 +1037,25 @@ hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, c
 hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                              const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos, const CdRec* pos0,
                              uint2* meta, uint16_t* numBins, uint32_t* numAnal, unsigned long long* planeBits,
-                             DevCounters* ctr, const double* uIn, double* uOut, hipStream_t st) {
+                             DevCounters* ctr, const double* uIn, hipStream_t st) {
     if (nS == 0) return hipSuccess;
     k_sphere_pos<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, pose, nA, mask, pos, pos0, meta, numBins, numAnal,
-                                                  planeBits, ctr, uIn, uOut);
+                                                  planeBits, ctr, uIn);
     return hipGetLastError();
 }
 
-hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* posOf, const uint32_t* candStart,
-                            uint32_t* candA, uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st) {
+hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* candStart, uint32_t* candA,
+                            uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cand_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, rowSlab, posOf, candStart, candA, candB, sorted, pos0);
+    k_cand_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, rowSlab, candStart, candA, candB, sorted, pos0);
     return hipGetLastError();
 }
 
 hipError_t launch_cand_flags(const KParams& P, uint32_t nK, const CdRec* pos, const uint2* meta, const uint32_t* candA,
                              const uint32_t* candB, const uint8_t* mask, double dispLimit2, uint8_t* acc,
-                             DevCounters* ctr, hipStream_t st) {
+                             DevCounters* ctr, double* uOut, hipStream_t st) {
     k_cand_flags<<<blocks(nK > 0 ? nK : 1u, 256), 256, 0, st>>>(P, nK, pos, meta, candA, candB, mask, dispLimit2, acc,
-                                                                ctr);
+                                                                ctr, uOut);
     return hipGetLastError();
 }
 

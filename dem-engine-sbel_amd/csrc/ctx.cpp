@@ -140,7 +140,7 @@ struct deme_ctx {
     bool builtThisPass = false;
     DBuf<CdRec> pos, pos0;
     // the common translation the displacement test subtracts: slot p (3 doubles) written by the pass at list parity p
-    // (sphere 0's displacement since the build), read by the next pass; both zero after a build
+    // (the midpoint of its displacement extremes since the build), read by the next pass; both zero after a build
     DBuf<double> dispRef;
     DBuf<uint2> meta;
     DBuf<uint32_t> candStart, candB;
@@ -403,7 +403,7 @@ int check_ready(deme_ctx* c) {
 }
 
 int readback_counters(deme_ctx* c) {
-    DEME_HIP(c, hipMemcpyAsync(c->hctr, c->ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+    DEME_HIP(c, hipMemcpyAsync(c->hctr, c->ctr.p, DEV_COUNTERS_HOST_BYTES, hipMemcpyDeviceToHost, c->stream));
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     return DEME_OK;
 }
@@ -1214,8 +1214,8 @@ static int build_candidates(deme_ctx* c) {
     // the build positions are the reference: no displacement, no common translation yet
     DEME_LAUNCH(c, "fill", launch_fill_u32(reinterpret_cast<uint32_t*>(c->dispRef.p), 12, 0, c->stream));
     DEME_LAUNCH(c, "cand_rows",
-                launch_cand_rows(nS, c->rowSlab.p, c->posOf.p, c->candStart.p, c->candA.p, c->candB.p, c->sorted.p,
-                                 c->pos0.p, c->stream));
+                launch_cand_rows(nS, c->rowSlab.p, c->candStart.p, c->candA.p, c->candB.p, c->sorted.p, c->pos0.p,
+                                 c->stream));
     // the transpose, counted from the rows (cidx[0] holds each candidate's place until the first pass writes it)
     const uint32_t nK32 = (uint32_t)c->nCand;
     DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, nS, 0, c->stream));
@@ -1257,12 +1257,11 @@ static int enqueue_skin_pass(deme_ctx* c, int cu) {
     DEME_LAUNCH(c, "sphere_pos",
                 launch_sphere_pos(c->kp, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->maskAny ? c->mask.p : nullptr,
                                   c->pos.p, c->pos0.p, c->maskAny ? c->meta.p : nullptr, nullptr, c->numAnal.p,
-                                  c->planeBits.p, c->ctr.p, c->dispRef.p + 3 * (cu ^ 1), c->dispRef.p + 3 * cu,
-                                  c->stream));
+                                  c->planeBits.p, c->ctr.p, c->dispRef.p + 3 * (cu ^ 1), c->stream));
     DEME_LAUNCH(c, "cand_flags",
                 launch_cand_flags(c->kp, (uint32_t)c->nCand, c->pos.p, c->meta.p, c->candA.p, c->candB.p,
                                   c->maskAny ? c->mask.p : nullptr, stale_limit2(c), c->candAcc[cu].p, c->ctr.p,
-                                  c->stream));
+                                  c->dispRef.p + 3 * cu, c->stream));
     if ((uint64_t)nS <= SCAN2_TILE * SCAN_TILE) {
         // the row counts formed inside the scan's first pass
         DEME_LAUNCH(c, "scan_rows",
@@ -1355,7 +1354,7 @@ int deme_cd_run(deme_ctx* c) {
         // The counters go to the host without a stream-wide wait: the rows and the history map are enqueued behind
         // them with the buffers at hand (skipped on the device if the pass failed or the lists do not fit), so the
         // GPU keeps working while the host reads the totals.
-        DEME_HIP(c, hipMemcpyAsync(c->hctr, c->ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+        DEME_HIP(c, hipMemcpyAsync(c->hctr, c->ctr.p, DEV_COUNTERS_HOST_BYTES, hipMemcpyDeviceToHost, c->stream));
         DEME_HIP(c, hipEventRecord(c->evCounts, c->stream));
         spec.ctr = nullptr;
         if (c->idB[cu].p && c->ctype[cu].p && c->idA.p && c->map.p && c->mirror.p && c->bPart.p) {
@@ -1823,7 +1822,7 @@ static int run_cycle(deme_ctx* c, int f, bool* done) {
     }
     DEME_LAUNCH(c, "set_clock", launch_set_clock(c->clock.p, c->time, c->stream));
     DEME_HIP(c, hipGraphLaunch(cy.exec, c->stream));
-    DEME_HIP(c, hipMemcpyAsync(c->hctr, c->ctr.p, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+    DEME_HIP(c, hipMemcpyAsync(c->hctr, c->ctr.p, DEV_COUNTERS_HOST_BYTES, hipMemcpyDeviceToHost, c->stream));
     DEME_HIP(c, hipStreamSynchronize(c->stream));
     const DevCounters& h = *c->hctr;
     const SpecGuard& g = cy.guard;

@@ -220,8 +220,26 @@ struct DevCounters {
     // spread over DISP_SLOTS words (workgroup b folds into word b % DISP_SLOTS: one address per workgroup would
     // serialize the atomics); the largest word is the maximum
     unsigned long long disp2[64];
+    // per axis, the extremes of the spheres' displacements since the build (same slots; order-preserving integer
+    // images of the doubles, the minima complemented, so every word folds with atomicMax from a cleared zero)
+    unsigned long long dispLo[3][64], dispHi[3][64];
 };
 constexpr uint32_t DISP_SLOTS = 64;
+// The host reads the counters up to the displacement extremes (the device alone uses those).
+constexpr size_t DEV_COUNTERS_HOST_BYTES = __builtin_offsetof(DevCounters, dispLo);
+
+// Order-preserving image of a double as an unsigned 64-bit integer (and back).
+__host__ __device__ inline unsigned long long ordered_bits(double x) {
+    unsigned long long b;
+    __builtin_memcpy(&b, &x, sizeof(b));
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double from_ordered_bits(unsigned long long b) {
+    b = (b >> 63) ? (b & 0x7FFFFFFFFFFFFFFFull) : ~b;
+    double x;
+    __builtin_memcpy(&x, &b, sizeof(x));
+    return x;
+}
 
 __host__ __device__ inline double max_disp2(const DevCounters& c) {
     unsigned long long m = 0;

@@ -13,6 +13,7 @@ namespace deme_int {
 
 using deme_dev::CdRec;
 using deme_dev::DevCounters;
+using deme_dev::DEV_COUNTERS_HOST_BYTES;
 using deme_dev::SpecGuard;
 using deme_dev::spec_skip;
 using deme_dev::ERRBIT_CAPACITY;
@@ -167,20 +168,20 @@ hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerR
 // Candidate-list path. Per step: detection records in id order (pos) with (owner, family) (meta), the shared
 // per-sphere detection outputs, and the largest squared displacement since the lists were built into ctr->disp2.
 // The largest squared displacement since the build is taken relative to the translation uIn (3 doubles; any vector
-// keeps the candidate lists exact); sphere 0's displacement is written to uOut for the next pass.
+// keeps the candidate lists exact); the displacements' extremes per axis are folded into ctr for the next pass's.
 hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                              const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos, const CdRec* pos0,
                              uint2* meta, uint16_t* numBins, uint32_t* numAnal, unsigned long long* planeBits,
-                             DevCounters* ctr, const double* uIn, double* uOut, hipStream_t st);
+                             DevCounters* ctr, const double* uIn, hipStream_t st);
 // After a candidate build: per sphere its candidate row (candStart from a scan of the build's row counts; candA /
 // candB the row's sphere and partners) and its position at the build (pos0).
-hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* posOf, const uint32_t* candStart,
-                            uint32_t* candA, uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st);
+hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* candStart, uint32_t* candA,
+                            uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st);
 // Per step, per candidate: the reference's predicate (acc[c] = 1 when c touches); ERRBIT_STALE instead when
-// ctr->disp2 exceeds dispLimit2.
+// ctr->disp2 exceeds dispLimit2. Lane 0 also writes the next pass's common translation (3 doubles) to uOut.
 hipError_t launch_cand_flags(const KParams& P, uint32_t nK, const CdRec* pos, const uint2* meta, const uint32_t* candA,
                              const uint32_t* candB, const uint8_t* mask, double dispLimit2, uint8_t* acc,
-                             DevCounters* ctr, hipStream_t st);
+                             DevCounters* ctr, double* uOut, hipStream_t st);
 // Per sphere: aCnt = touching candidates of its row, bCnt = touching candidates with it as partner.
 hipError_t launch_cand_count(uint32_t nS, const uint32_t* candStart, const uint32_t* tStart, const uint32_t* tC,
                              const uint8_t* acc, uint32_t* aCnt, uint32_t* bCnt, const DevCounters* ctr,
