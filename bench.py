@@ -365,6 +365,7 @@ def run_adjoint(n, steps, window, warmup_windows=1, cpu_threads=0):
 def cpu_adjoint(sc, steps, window, dLdX, repeats=3):
     """The FP64 host adjoint (the oracle's forward simulator and reverse sweep) on the same workload: the forward alone
     and forward + reverse, median of `repeats` each; one thread (the restatement is serial)."""
+    os.environ.setdefault("OMP_WAIT_POLICY", "passive")  # (before the oracle's OpenMP runtime loads)
     import pyoracle  # noqa: E402  (the oracle is the reported CPU baseline only)
 
     ref = pyoracle.AdjointFP64(sc, steps, window)
@@ -415,10 +416,13 @@ def cpu_baseline(sc, steps, update_freq, threads, repeats=3, state=None):
     """The oracle (host C++/OpenMP restatement of the reference's kernels) on `threads` threads: one warm-up step, then
     `repeats` timed runs of `steps` steps, each from the same state (the scene's, or `state`); the median run's
     seconds and its per-phase seconds (the reference's timer names)."""
-    import pyoracle  # noqa: E402  (the oracle is the reported CPU baseline only)
-
+    # (set before the oracle's OpenMP runtime loads) idle workers sleep instead of spinning, so the device legs timed
+    # after a baseline keep the host's cores
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_WAIT_POLICY", "passive")
+    import pyoracle  # noqa: E402  (the oracle is the reported CPU baseline only)
+
     ora = pyoracle.OracleSystem(sc, threads=threads)
     if state is not None:
         ora.set_state(state)
@@ -573,7 +577,7 @@ def main():
                "kind": "port",
                "sample": f"oracle, same 1M-clump workload, contact detection every step: median of 3 runs of "
                          f"{args.cpu_steps} steps after 1 warm-up step, {threads} OpenMP threads (the job's CPU "
-                         f"share), OMP_PROC_BIND=close, OMP_PLACES=cores",
+                         f"share), OMP_PROC_BIND=close, OMP_PLACES=cores, OMP_WAIT_POLICY=passive",
                "runs_s": [round(r, 3) for r in runs],
                "phase_s_per_step": {k: round(v, 5) for k, v in phases.items()},
                "host": host_cpu()}
