@@ -374,7 +374,8 @@ __device__ __forceinline__ bool bin_rule_point(const KParams& P, double inv, con
 // the same-owner check and queues every surviving pair in LDS. The block then shares the queue out evenly: the family
 // mask and the contact-point bin rule (evaluated with the lower id as A, whichever lane met the pair), and each
 // accepted pair is appended to both spheres' columns: the higher id to the lower-id sphere's contact row (rowSlab,
-// count aCnt), the lower id to the higher-id sphere's mirror row (mirSlab, count bCnt). Slabs and counters are indexed
+// count aCnt), the lower id to the higher-id sphere's mirror row (mirSlab, count bCnt; a candidate build keeps the
+// rows alone). Slabs and counters are indexed
 // by sorted position (entry k of sorted sphere i at slab[k * nS + i]), so the appends of a block hit neighbouring
 // addresses. Pairs beyond the queue's capacity are parked in the lane's column of `surv` and finished by that lane.
 // A column that overflows a slab is reported through maxOcc and the pass is redone with wider columns.
@@ -700,7 +701,7 @@ __global__ __launch_bounds__(256) void k_cell_rows(KParams P, const SphereRec* _
         pos++;
     }
     const uint32_t sp = pos;
-    const uint32_t p = posOf ? posOf[s] : s;  // slab columns by sorted position, or by id (candidate-list path)
+    const uint32_t p = posOf ? posOf[s] : s;  // slab columns by sorted position (or by id when posOf is null)
     sorted_copy(rowSlab + p, en - sp, nS, idB + sp);
     const uint32_t bs = bRowStart[s];
     sorted_copy(mirSlab + p, bRowStart[s + 1] - bs, nS, bPart + bs);
