@@ -761,8 +761,19 @@ __global__ __launch_bounds__(256) void k_cand_flags(KParams P, uint32_t nK, cons
             uOut[a] = isfinite(u) ? u : 0.0;  // (a non-finite position must not hide a displacement)
         }
     }
-    // (the grid has at least one lane, so a build without candidates is still found stale)
-    if (max_disp2(*ctr) > dispLimit2) {
+    // (the grid has at least one lane, so a build without candidates is still found stale). The largest displacement
+    // is folded across the wave, one slot per lane: a loop over the slots would run on the scalar unit, which the
+    // CU's waves share.
+    static_assert(DISP_SLOTS == 64, "one displacement slot per lane of a wave");
+    unsigned long long dm = ctr->disp2[threadIdx.x & 63u];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(dm, off, 64);
+        dm = o > dm ? o : dm;
+    }
+    double dmax;
+    __builtin_memcpy(&dmax, &dm, sizeof(dmax));
+    if (dmax > dispLimit2) {
         if (c == 0) atomicOr(&ctr->err, ERRBIT_STALE);
         return;
     }
