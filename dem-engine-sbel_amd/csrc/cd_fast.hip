@@ -834,6 +834,12 @@ __global__ __launch_bounds__(256) void k_cand_fill(uint32_t nK, const uint32_t* 
     const bool t = in && acc[c];
     const unsigned long long ball = __ballot(t);
     const uint32_t r0 = in ? candStart[a] : 0u;
+    // everything a touching lane reads, requested before the lanes that do not touch leave and before the row-ahead
+    // count, so the loads overlap
+    const uint32_t rs = in ? rowStart[a] : 0u, re = in ? rowStart[a + 1] : 0u, na = in ? numAnal[a] : 0u;
+    const uint32_t b = in ? candB[c] : 0u;
+    const bool pa = in && prevAcc && prevAcc[c];
+    const uint32_t pc = in && prevAcc ? prevCidx[c] : DEME_NULL_MAPPING_PARTNER;
     // touching candidates of the wave's first row ahead of the wave (lane 0's row is the only one that can begin
     // before it)
     uint32_t ahead = 0;
@@ -846,16 +852,16 @@ __global__ __launch_bounds__(256) void k_cand_fill(uint32_t nK, const uint32_t* 
     const uint32_t lo = r0 > w0 ? r0 - w0 : 0u;  // first lane of c's row in this wave
     const unsigned long long below = (lane ? (~0ull >> (64u - lane)) : 0ull) & ~(lo ? (~0ull >> (64u - lo)) : 0ull);
     const uint32_t k = (uint32_t)__popcll(ball & below) + (r0 < w0 ? ahead : 0u);
-    const uint32_t i = rowStart[a] + numAnal[a] + k;
-    if (i >= rowStart[a + 1]) {  // counts that disagree with the scan (not reached)
+    const uint32_t i = rs + na + k;
+    if (i >= re) {  // counts that disagree with the scan (not reached)
         cidx[c] = DEME_NULL_MAPPING_PARTNER;
         return;
     }
     idA[i] = a;
-    idB[i] = candB[c];
+    idB[i] = b;
     ctype[i] = DEME_SPHERE_SPHERE_CONTACT;
     cidx[c] = i;
-    if (prevAcc) map[i] = prevAcc[c] ? prevCidx[c] : DEME_NULL_MAPPING_PARTNER;
+    if (prevAcc) map[i] = pa ? pc : DEME_NULL_MAPPING_PARTNER;
 }
 
 // Per sphere: its plane contacts at the head of its canonical row (ascending plane index; the map entry, when
@@ -880,6 +886,20 @@ __global__ __launch_bounds__(256) void k_cand_mirror(KParams P, const SphereRec*
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS || spec_skip(guard)) return;
     unsigned long long bits = planeBits[s];
+    // The mirror row's bounds and its first MIRROR_SPEC transposed entries with their flags and contact indices are
+    // loaded together before any is used (rows are short); longer rows finish in a loop.
+    constexpr uint32_t MIRROR_SPEC = 2;
+    const uint32_t t0 = tStart[s], t1 = tStart[s + 1];
+    const uint32_t qb = bRowStart[s], qe = bRowStart[s + 1];
+    uint32_t cc[MIRROR_SPEC], ci[MIRROR_SPEC];
+    bool on[MIRROR_SPEC];
+#pragma unroll
+    for (uint32_t j = 0; j < MIRROR_SPEC; j++) cc[j] = t0 + j < t1 ? tC[t0 + j] : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < MIRROR_SPEC; j++) {
+        on[j] = t0 + j < t1 && acc[cc[j]];
+        ci[j] = t0 + j < t1 ? cidx[cc[j]] : DEME_NULL_MAPPING_PARTNER;  // (used only where on)
+    }
     if (bits) {
         uint32_t pos = rowStart[s];
         const uint32_t en = rowStart[s + 1];
@@ -907,13 +927,18 @@ __global__ __launch_bounds__(256) void k_cand_mirror(KParams P, const SphereRec*
             pos++;
         }
     }
-    uint32_t q = bRowStart[s];
-    const uint32_t qe = bRowStart[s + 1], t1 = tStart[s + 1];
-    for (uint32_t t = tStart[s]; t < t1 && q < qe; t++) {
-        const uint32_t cc = tC[t];
-        if (!acc[cc]) continue;
-        const uint32_t ci = cidx[cc];
-        if (ci != DEME_NULL_MAPPING_PARTNER) mirror[ci] = q;
+    uint32_t q = qb;
+#pragma unroll
+    for (uint32_t j = 0; j < MIRROR_SPEC; j++)
+        if (on[j] && q < qe) {
+            if (ci[j] != DEME_NULL_MAPPING_PARTNER) mirror[ci[j]] = q;
+            q++;
+        }
+    for (uint32_t t = t0 + MIRROR_SPEC; t < t1 && q < qe; t++) {
+        const uint32_t c = tC[t];
+        if (!acc[c]) continue;
+        const uint32_t i = cidx[c];
+        if (i != DEME_NULL_MAPPING_PARTNER) mirror[i] = q;
         q++;
     }
 }

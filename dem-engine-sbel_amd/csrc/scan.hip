@@ -168,6 +168,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce(const uint32_t* __r
 // Pass 1 of the candidate-list pass's row scans: the counts are formed here, not read -- per sphere the touching
 // candidates of its row plus its plane contacts, and the touching candidates with it as partner (through its
 // transposed row) -- stored in tmp and folded into the tile sums. A failed pass (ctr->err) leaves them alone.
+// Rows are short (a few candidates), so a lane does not walk its ITEMS spheres one after another: it loads the row
+// bounds of all of them, then the first CAND_SPEC entries of every row and transposed row, then the flags of those,
+// and only longer rows finish in a loop -- three rounds of loads in flight instead of a chain per sphere.
+constexpr uint32_t CAND_SPEC = 2;
 template <int ITEMS>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce_cand(const uint32_t* __restrict__ candStart,
                                                                   const uint32_t* __restrict__ tStart,
@@ -180,26 +184,53 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce_cand(const uint32_t
                                                                   const DevCounters* ctr) {
     if (ctr->err) return;
     const uint64_t base = (uint64_t)blockIdx.x * (uint64_t)(SCAN_BLOCK * ITEMS);
+    // spheres in groups of G: the group's loads in flight fit the registers of eight waves per SIMD
+    constexpr int G = ITEMS < 4 ? ITEMS : 4;
+    static_assert(ITEMS % G == 0, "whole groups");
     uint64_t s1 = 0, s2 = 0;
 #pragma unroll
-    for (int i = 0; i < ITEMS; i++) {
-        const uint64_t k = base + (uint64_t)i * SCAN_BLOCK + threadIdx.x;
-        if (k < n) {
-            uint32_t v1 = numAnal[k], v2 = 0;
-            const uint32_t c1 = candStart[k + 1], t1 = tStart[k + 1];
-            for (uint32_t c = candStart[k]; c < c1; c++) v1 += acc[c];
-            for (uint32_t t = tStart[k]; t < t1; t++) v2 += acc[tC[t]];
-            tmp[k] = make_uint2(v1, v2);
-            s1 += v1;
-            s2 += v2;
+    for (int g = 0; g < ITEMS; g += G) {
+        uint32_t c0[G], c1[G], t0[G], t1[G], v1[G], v2[G];
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            const uint64_t k = base + (uint64_t)(g + i) * SCAN_BLOCK + threadIdx.x;
+            const bool in = k < n;  // (a sphere past the end has empty rows)
+            c0[i] = in ? candStart[k] : 0u;
+            c1[i] = in ? candStart[k + 1] : 0u;
+            t0[i] = in ? tStart[k] : 0u;
+            t1[i] = in ? tStart[k + 1] : 0u;
+            v1[i] = in ? numAnal[k] : 0u;
+            v2[i] = 0u;
+        }
+        uint32_t q[G][CAND_SPEC];
+#pragma unroll
+        for (int i = 0; i < G; i++)
+#pragma unroll
+            for (uint32_t j = 0; j < CAND_SPEC; j++) {
+                if (c0[i] + j < c1[i]) v1[i] += acc[c0[i] + j];
+                q[i][j] = t0[i] + j < t1[i] ? tC[t0[i] + j] : 0u;
+            }
+#pragma unroll
+        for (int i = 0; i < G; i++)
+#pragma unroll
+            for (uint32_t j = 0; j < CAND_SPEC; j++)
+                if (t0[i] + j < t1[i]) v2[i] += acc[q[i][j]];
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            for (uint32_t c = c0[i] + CAND_SPEC; c < c1[i]; c++) v1[i] += acc[c];
+            for (uint32_t t = t0[i] + CAND_SPEC; t < t1[i]; t++) v2[i] += acc[tC[t]];
+            const uint64_t k = base + (uint64_t)(g + i) * SCAN_BLOCK + threadIdx.x;
+            if (k < n) tmp[k] = make_uint2(v1[i], v2[i]);
+            s1 += v1[i];
+            s2 += v2[i];
         }
     }
-    uint64_t t1, t2;
-    block_excl_scan(s1, t1);
-    block_excl_scan(s2, t2);
+    uint64_t tot1, tot2;
+    block_excl_scan(s1, tot1);
+    block_excl_scan(s2, tot2);
     if (threadIdx.x == 0) {
-        sums1[blockIdx.x] = t1;
-        sums2[blockIdx.x] = t2;
+        sums1[blockIdx.x] = tot1;
+        sums2[blockIdx.x] = tot2;
     }
 }
 
