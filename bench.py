@@ -46,7 +46,11 @@ def bytes_model(name, k):
         # per-bin pair test: entries + bin ends + CD records (unique) + owner/family of hit pairs, pairs out
         "bin_pairs": P * (8 + 4) + S * 32 + Np * (8 + 8 + 8),
         # cell path: rows and mirror rows in one pass: permutation, two gathered counts, plane counts in; two starts out
-        "scan_rows": S * (4 + 4 + 4 + 4 + 4 + 4) if k.get("cells") else S * (4 + 4 + 4),
+        # candidate lists: per sphere its row and transposed-row bounds and plane count in, the two counts through the
+        # tile buffer, both starts out; per candidate its flag and transposed index, its flag again through the
+        # transpose; per touching sphere-sphere contact its mirror rank out
+        "scan_rows": (S * (4 + 4 + 4 + 8 + 8 + 4 + 4) + Nk * (1 + 4 + 1) + Np * 4 if Nk else
+                      S * (4 + 4 + 4 + 4 + 4 + 4) if k.get("cells") else S * (4 + 4 + 4)),
         "row_fill": Np * (8 + 4 + 4 + 8 + 5),
         "geo_fill": S * (4 + 4 + 32) + k["geo"] * 5,
         "row_finalize": S * 8 + C * 8,
@@ -77,17 +81,17 @@ def bytes_model(name, k):
         # per sphere its row and transposed-row bounds, both counts out; per candidate its flag (row order) and its
         # index and flag (transposed order)
         "cand_count": S * (4 + 4 + 4 + 4) + Nk * (1 + 4 + 1),
-        # per candidate its sphere and flag; per contact the row bounds, plane count and previous flag and contact index
-        # of its candidate in, idA, idB, type, contact index and map out
-        "cand_fill": Nk * (4 + 1) + Np * (4 + 4 + 4 + 1 + 4) + Np * (4 + 4 + 1 + 4 + 4),
-        # per sphere plane bits, transposed and mirror row bounds; per candidate its transposed index and flag; per
-        # sphere-sphere contact its contact index and mirror link out
-        "cand_mirror": S * (8 + 4 + 4) + Nk * (4 + 1) + Np * (4 + 4),
+        # per candidate its sphere and flag; per contact the row bounds, plane count, partner, previous flag and contact
+        # index, transposed slot, mirror rank and the partner's mirror row start in; idA, idB, type, mirror link,
+        # contact index and map out
+        "cand_fill": Nk * (4 + 1) + Np * (4 + 4 + 4 + 4 + 1 + 4 + 4 + 4 + 4) + Np * (4 + 4 + 1 + 4 + 4 + 4),
+        # per sphere its plane bits; per plane contact its row bound in, idA, idB, type, mirror and map out
+        "cand_planes": S * 8 + k["geo"] * (4 + 4 + 4 + 1 + 4 + 4),
         # the transpose counted from the rows: per candidate its partner in, a 4-byte RMW and its place out; then
         # placed (partner, place, row start in; index out); then each transposed row sorted in place
         "cand_tcount": Nk * (4 + 4 + 4),
         "cand_tscatter": Nk * (4 + 4 + 4 + 4),
-        "cand_tsort": S * 4 + Nk * (4 + 4),
+        "cand_tsort": S * 4 + Nk * (4 + 4 + 4),
         # the build sweep: as the cell sweep's pass with every candidate kept
         "cand_build": S * (32 + 4 + 4) + O * 4 + Nc * 4 + Nk * (4 + 4 + 8),
         "scan_cand": S * 4 * 2,

@@ -787,12 +787,14 @@ __global__ __launch_bounds__(256) void k_cand_flags(KParams P, uint32_t nK, cons
     acc[c] = f;
 }
 
-// Per sphere: its row's touching candidates (aCnt) and those with it as partner (bCnt, its mirror row's length).
+// Per sphere: its row's touching candidates (aCnt) and those with it as partner (bCnt, its mirror row's length), with
+// each touching transposed slot's rank in the mirror row (tRank).
 __global__ __launch_bounds__(256) void k_cand_count(uint32_t nS, const uint32_t* __restrict__ candStart,
                                                     const uint32_t* __restrict__ tStart,
                                                     const uint32_t* __restrict__ tC,
                                                     const uint8_t* __restrict__ acc, uint32_t* __restrict__ aCnt,
-                                                    uint32_t* __restrict__ bCnt, const DevCounters* ctr) {
+                                                    uint32_t* __restrict__ bCnt, uint32_t* __restrict__ tRank,
+                                                    const DevCounters* ctr) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS || ctr->err) return;
     uint32_t na = 0, nb = 0;
@@ -801,15 +803,21 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t nS, const uint32_t*
     uint32_t t = tStart[s];
     for (; t + 3 < t1; t += 4) {
         const uint32_t q0 = tC[t], q1 = tC[t + 1], q2 = tC[t + 2], q3 = tC[t + 3];
-        nb += (uint32_t)acc[q0] + (uint32_t)acc[q1] + (uint32_t)acc[q2] + (uint32_t)acc[q3];
+        const uint32_t f0 = acc[q0], f1 = acc[q1], f2 = acc[q2], f3 = acc[q3];
+        if (f0) tRank[t] = nb++;
+        if (f1) tRank[t + 1] = nb++;
+        if (f2) tRank[t + 2] = nb++;
+        if (f3) tRank[t + 3] = nb++;
     }
-    for (; t < t1; t++) nb += acc[tC[t]];
+    for (; t < t1; t++)
+        if (acc[tC[t]]) tRank[t] = nb++;
     aCnt[s] = na;
     bCnt[s] = nb;
 }
 
-// Per touching candidate: its contact in A's canonical row (after A's plane contacts, in candidate order), idA,
-// cidx[c] and, when prevAcc is given (the previous pass used the same candidates), its persistent-contact map entry:
+// Per touching candidate: its contact in A's canonical row (after A's plane contacts, in candidate order), idA, its
+// mirror link (B's mirror row start plus the rank the row scan gave c's transposed slot), cidx[c] and, when prevAcc
+// is given (the previous pass used the same candidates), its persistent-contact map entry:
 // the previous pass's contact index of c, or none if c did not touch then. A wave covers 64 consecutive candidates:
 // c's rank among the touching candidates of its row is a population count of the wave's ballot, plus, for a row
 // that began before the wave, the touching candidates of the row ahead of it (counted once by lane 0).
@@ -819,11 +827,14 @@ __global__ __launch_bounds__(256) void k_cand_fill(uint32_t nK, const uint32_t* 
                                                    const uint8_t* __restrict__ acc,
                                                    const uint32_t* __restrict__ rowStart,
                                                    const uint32_t* __restrict__ numAnal,
+                                                   const uint32_t* __restrict__ tPos,
+                                                   const uint32_t* __restrict__ tRank,
+                                                   const uint32_t* __restrict__ bRowStart,
                                                    const uint8_t* __restrict__ prevAcc,
                                                    const uint32_t* __restrict__ prevCidx, uint32_t* __restrict__ cidx,
                                                    uint32_t* __restrict__ idA, uint32_t* __restrict__ idB,
                                                    uint8_t* __restrict__ ctype, uint32_t* __restrict__ map,
-                                                   SpecGuard guard) {
+                                                   uint32_t* __restrict__ mirror, SpecGuard guard) {
     if (spec_skip(guard)) return;  // uniform: the whole grid leaves
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
@@ -840,6 +851,8 @@ __global__ __launch_bounds__(256) void k_cand_fill(uint32_t nK, const uint32_t* 
     const uint32_t b = in ? candB[c] : 0u;
     const bool pa = in && prevAcc && prevAcc[c];
     const uint32_t pc = in && prevAcc ? prevCidx[c] : DEME_NULL_MAPPING_PARTNER;
+    // (a lane that does not touch reads a rank this pass did not write, in bounds, and leaves before using it)
+    const uint32_t mir = in ? bRowStart[b] + tRank[tPos[c]] : 0u;
     // touching candidates of the wave's first row ahead of the wave (lane 0's row is the only one that can begin
     // before it)
     uint32_t ahead = 0;
@@ -860,86 +873,51 @@ __global__ __launch_bounds__(256) void k_cand_fill(uint32_t nK, const uint32_t* 
     idA[i] = a;
     idB[i] = b;
     ctype[i] = DEME_SPHERE_SPHERE_CONTACT;
+    mirror[i] = mir;
     cidx[c] = i;
     if (prevAcc) map[i] = pa ? pc : DEME_NULL_MAPPING_PARTNER;
 }
 
 // Per sphere: its plane contacts at the head of its canonical row (ascending plane index; the map entry, when
-// mapPlanes, is the same plane's entry in its previous row; no mirror slot), and its mirror row -- the touching
-// candidates with it as partner, ascending A -- linked from each of their contacts: mirror[contact] =
-// bRowStart[B] + rank.
-__global__ __launch_bounds__(256) void k_cand_mirror(KParams P, const SphereRec* __restrict__ sph,
+// mapPlanes, is the same plane's entry in its previous row; no mirror slot). Spheres touching no plane do nothing.
+__global__ __launch_bounds__(256) void k_cand_planes(KParams P, const SphereRec* __restrict__ sph,
                                                      const OwnerRec* __restrict__ own, uint32_t nS,
                                                      const uint32_t* __restrict__ rowStart,
                                                      const unsigned long long* __restrict__ planeBits,
                                                      const uint32_t* __restrict__ prevRowStart,
                                                      const uint32_t* __restrict__ prevIdB,
                                                      const uint8_t* __restrict__ prevType, bool mapPlanes,
-                                                     const uint32_t* __restrict__ tStart,
-                                                     const uint32_t* __restrict__ tC,
-                                                     const uint8_t* __restrict__ acc,
-                                                     const uint32_t* __restrict__ cidx,
-                                                     const uint32_t* __restrict__ bRowStart, uint32_t* __restrict__ idA,
-                                                     uint32_t* __restrict__ idB, uint8_t* __restrict__ ctype,
-                                                     uint32_t* __restrict__ map, uint32_t* __restrict__ mirror,
-                                                     SpecGuard guard) {
+                                                     uint32_t* __restrict__ idA, uint32_t* __restrict__ idB,
+                                                     uint8_t* __restrict__ ctype, uint32_t* __restrict__ map,
+                                                     uint32_t* __restrict__ mirror, SpecGuard guard) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS || spec_skip(guard)) return;
     unsigned long long bits = planeBits[s];
-    // The mirror row's bounds and its first MIRROR_SPEC transposed entries with their flags and contact indices are
-    // loaded together before any is used (rows are short); longer rows finish in a loop.
-    constexpr uint32_t MIRROR_SPEC = 2;
-    const uint32_t t0 = tStart[s], t1 = tStart[s + 1];
-    const uint32_t qb = bRowStart[s], qe = bRowStart[s + 1];
-    uint32_t cc[MIRROR_SPEC], ci[MIRROR_SPEC];
-    bool on[MIRROR_SPEC];
-#pragma unroll
-    for (uint32_t j = 0; j < MIRROR_SPEC; j++) cc[j] = t0 + j < t1 ? tC[t0 + j] : 0u;
-#pragma unroll
-    for (uint32_t j = 0; j < MIRROR_SPEC; j++) {
-        on[j] = t0 + j < t1 && acc[cc[j]];
-        ci[j] = t0 + j < t1 ? cidx[cc[j]] : DEME_NULL_MAPPING_PARTNER;  // (used only where on)
+    if (!bits) return;
+    uint32_t pos = rowStart[s];
+    const uint32_t en = rowStart[s + 1];
+    uint32_t a0 = 0, a1 = 0;
+    scene_planes(P, own[sph[s].owner].scene, a0, a1);
+    // the previous row starts with its plane contacts, ascending: a merge over them
+    uint32_t j = 0, je = 0;
+    if (mapPlanes && prevRowStart) {
+        j = prevRowStart[s];
+        je = prevRowStart[s + 1];
     }
-    if (bits) {
-        uint32_t pos = rowStart[s];
-        const uint32_t en = rowStart[s + 1];
-        uint32_t a0 = 0, a1 = 0;
-        scene_planes(P, own[sph[s].owner].scene, a0, a1);
-        // the previous row starts with its plane contacts, ascending: a merge over them
-        uint32_t j = 0, je = 0;
-        if (mapPlanes && prevRowStart) {
-            j = prevRowStart[s];
-            je = prevRowStart[s + 1];
+    while (bits && pos < en) {
+        const uint32_t pid = a0 + (uint32_t)(__ffsll(bits) - 1);
+        bits &= bits - 1;
+        idA[pos] = s;
+        idB[pos] = pid;
+        ctype[pos] = DEME_SPHERE_PLANE_CONTACT;
+        mirror[pos] = DEME_NULL_MAPPING_PARTNER;
+        if (mapPlanes) {
+            while (j < je && prevType[j] == DEME_SPHERE_PLANE_CONTACT && prevIdB[j] < pid) j++;
+            map[pos] = (j < je && prevType[j] == DEME_SPHERE_PLANE_CONTACT && prevIdB[j] == pid)
+                           ? j
+                           : DEME_NULL_MAPPING_PARTNER;
         }
-        while (bits && pos < en) {
-            const uint32_t pid = a0 + (uint32_t)(__ffsll(bits) - 1);
-            bits &= bits - 1;
-            idA[pos] = s;
-            idB[pos] = pid;
-            ctype[pos] = DEME_SPHERE_PLANE_CONTACT;
-            mirror[pos] = DEME_NULL_MAPPING_PARTNER;
-            if (mapPlanes) {
-                while (j < je && prevType[j] == DEME_SPHERE_PLANE_CONTACT && prevIdB[j] < pid) j++;
-                map[pos] = (j < je && prevType[j] == DEME_SPHERE_PLANE_CONTACT && prevIdB[j] == pid)
-                               ? j
-                               : DEME_NULL_MAPPING_PARTNER;
-            }
-            pos++;
-        }
-    }
-    uint32_t q = qb;
-#pragma unroll
-    for (uint32_t j = 0; j < MIRROR_SPEC; j++)
-        if (on[j] && q < qe) {
-            if (ci[j] != DEME_NULL_MAPPING_PARTNER) mirror[ci[j]] = q;
-            q++;
-        }
-    for (uint32_t t = t0 + MIRROR_SPEC; t < t1 && q < qe; t++) {
-        const uint32_t c = tC[t];
-        if (!acc[c]) continue;
-        const uint32_t i = cidx[c];
-        if (i != DEME_NULL_MAPPING_PARTNER) mirror[i] = q;
-        q++;
+        pos++;
     }
 }
 
@@ -963,14 +941,17 @@ __global__ __launch_bounds__(256) void k_cand_tscatter(uint32_t nK, const uint32
 }
 
 // ... and each transposed row sorted ascending (candidate order is row order, so ascending A), in place: rows of up to
-// SORT_REG entries in registers, longer ones by insertion.
+// SORT_REG entries in registers, longer ones by insertion; then each candidate's slot recorded, tPos[tC[t]] = t.
 __global__ __launch_bounds__(256) void k_cand_tsort(uint32_t nS, const uint32_t* __restrict__ tStart,
-                                                    uint32_t* __restrict__ tC) {
+                                                    uint32_t* __restrict__ tC, uint32_t* __restrict__ tPos) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nS) return;
     const uint32_t t0 = tStart[s], n = tStart[s + 1] - t0;
-    if (n < 2) return;
     uint32_t* row = tC + t0;
+    if (n < 2) {
+        if (n == 1) tPos[row[0]] = t0;
+        return;
+    }
     if (n <= SORT_REG) {
         uint32_t v[SORT_REG];
 #pragma unroll
@@ -985,7 +966,10 @@ __global__ __launch_bounds__(256) void k_cand_tsort(uint32_t nS, const uint32_t*
             }
 #pragma unroll
         for (uint32_t k = 0; k < SORT_REG; k++)
-            if (k < n) row[k] = v[k];
+            if (k < n) {
+                row[k] = v[k];
+                tPos[v[k]] = t0 + k;
+            }
         return;
     }
     for (uint32_t k = 1; k < n; k++) {
@@ -997,6 +981,7 @@ __global__ __launch_bounds__(256) void k_cand_tsort(uint32_t nS, const uint32_t*
         }
         row[q] = v;
     }
+    for (uint32_t k = 0; k < n; k++) tPos[row[k]] = t0 + k;
 }
 
 inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
@@ -1097,33 +1082,34 @@ hipError_t launch_cand_flags(const KParams& P, uint32_t nK, const CdRec* pos, co
 }
 
 hipError_t launch_cand_count(uint32_t nS, const uint32_t* candStart, const uint32_t* tStart, const uint32_t* tC,
-                             const uint8_t* acc, uint32_t* aCnt, uint32_t* bCnt, const DevCounters* ctr,
-                             hipStream_t st) {
+                             const uint8_t* acc, uint32_t* aCnt, uint32_t* bCnt, uint32_t* tRank,
+                             const DevCounters* ctr, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cand_count<<<blocks(nS, 256), 256, 0, st>>>(nS, candStart, tStart, tC, acc, aCnt, bCnt, ctr);
+    k_cand_count<<<blocks(nS, 256), 256, 0, st>>>(nS, candStart, tStart, tC, acc, aCnt, bCnt, tRank, ctr);
     return hipGetLastError();
 }
 
 hipError_t launch_cand_fill(uint32_t nK, const uint32_t* candA, const uint32_t* candB, const uint32_t* candStart,
                             const uint8_t* acc, const uint32_t* rowStart, const uint32_t* numAnal,
+                            const uint32_t* tPos, const uint32_t* tRank, const uint32_t* bRowStart,
                             const uint8_t* prevAcc, const uint32_t* prevCidx, uint32_t* cidx, uint32_t* idA,
-                            uint32_t* idB, uint8_t* ctype, uint32_t* map, hipStream_t st, SpecGuard guard) {
+                            uint32_t* idB, uint8_t* ctype, uint32_t* map, uint32_t* mirror, hipStream_t st,
+                            SpecGuard guard) {
     if (nK == 0) return hipSuccess;
-    k_cand_fill<<<blocks(nK, 256), 256, 0, st>>>(nK, candA, candB, candStart, acc, rowStart, numAnal, prevAcc,
-                                                 prevCidx, cidx, idA, idB, ctype, map, guard);
+    k_cand_fill<<<blocks(nK, 256), 256, 0, st>>>(nK, candA, candB, candStart, acc, rowStart, numAnal, tPos, tRank,
+                                                 bRowStart, prevAcc, prevCidx, cidx, idA, idB, ctype, map, mirror,
+                                                 guard);
     return hipGetLastError();
 }
 
-hipError_t launch_cand_mirror(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+hipError_t launch_cand_planes(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                               const uint32_t* rowStart, const unsigned long long* planeBits,
                               const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
-                              bool mapPlanes, const uint32_t* tStart, const uint32_t* tC, const uint8_t* acc,
-                              const uint32_t* cidx, const uint32_t* bRowStart, uint32_t* idA, uint32_t* idB,
-                              uint8_t* ctype, uint32_t* map, uint32_t* mirror, hipStream_t st, SpecGuard guard) {
+                              bool mapPlanes, uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* map,
+                              uint32_t* mirror, hipStream_t st, SpecGuard guard) {
     if (nS == 0) return hipSuccess;
-    k_cand_mirror<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, rowStart, planeBits, prevRowStart, prevIdB,
-                                                   prevType, mapPlanes, tStart, tC, acc, cidx, bRowStart, idA, idB,
-                                                   ctype, map, mirror, guard);
+    k_cand_planes<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, rowStart, planeBits, prevRowStart, prevIdB,
+                                                   prevType, mapPlanes, idA, idB, ctype, map, mirror, guard);
     return hipGetLastError();
 }
 
@@ -1140,9 +1126,9 @@ hipError_t launch_cand_tscatter(uint32_t nK, const uint32_t* candB, const uint32
     return hipGetLastError();
 }
 
-hipError_t launch_cand_tsort(uint32_t nS, const uint32_t* tStart, uint32_t* tC, hipStream_t st) {
+hipError_t launch_cand_tsort(uint32_t nS, const uint32_t* tStart, uint32_t* tC, uint32_t* tPos, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cand_tsort<<<blocks(nS, 256), 256, 0, st>>>(nS, tStart, tC);
+    k_cand_tsort<<<blocks(nS, 256), 256, 0, st>>>(nS, tStart, tC, tPos);
     return hipGetLastError();
 }
 

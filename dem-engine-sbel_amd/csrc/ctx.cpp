@@ -145,11 +145,12 @@ struct deme_ctx {
     DBuf<uint2> meta;
     DBuf<uint32_t> candStart, candB;
     // The build's candidates: candA (the row's sphere), transposed (tC[tStart[B]..tStart[B + 1]]: indices of the
-    // candidates with partner B, ascending A). Per list parity the touching flags (candAcc) and each touching
-    // candidate's contact index (cidx).
+    // candidates with partner B, ascending A; tPos[c]: c's slot there). Per list parity the touching flags (candAcc)
+    // and each touching candidate's contact index (cidx); per pass each touching transposed slot's rank in its mirror
+    // row (tRank).
     // cidxGen[p]: the build (candBuilds) whose candidates candAcc[p] / cidx[p] index, 0 when the list at parity p
     // came another way; a pass maps its history through them when that is the current build's.
-    DBuf<uint32_t> candA, tStart, tC, cidx[2];
+    DBuf<uint32_t> candA, tStart, tC, tPos, tRank, cidx[2];
     DBuf<uint8_t> candAcc[2];
     uint64_t cidxGen[2] = {0, 0};
     double skin() const {
@@ -481,7 +482,8 @@ int deme_ctx_destroy(deme_ctx* c) {
     c->survSlab.release(); c->rowSlab.release(); c->mirSlab.release(); c->sorted.release(); c->planeBits.release();
     c->aCnt.release(); c->sortedMeta.release();
     c->pos.release(); c->pos0.release(); c->meta.release(); c->candStart.release(); c->candB.release();
-    c->candA.release(); c->tStart.release(); c->tC.release(); c->dispRef.release();
+    c->candA.release(); c->tStart.release(); c->tC.release(); c->tPos.release(); c->tRank.release();
+    c->dispRef.release();
     for (int k = 0; k < 2; k++) {
         c->cidx[k].release();
         c->candAcc[k].release();
@@ -1019,14 +1021,13 @@ static int enqueue_rows(deme_ctx* c, SpecGuard guard, bool skinPass) {
         const bool mapFromCand = c->havePrev && c->cidxGen[pv] == c->candBuilds;
         DEME_LAUNCH(c, "cand_fill",
                     launch_cand_fill((uint32_t)c->nCand, c->candA.p, c->candB.p, c->candStart.p, c->candAcc[cu].p,
-                                     c->rowStart[cu].p, c->numAnal.p, mapFromCand ? c->candAcc[pv].p : nullptr,
-                                     c->cidx[pv].p, c->cidx[cu].p, c->idA.p, c->idB[cu].p, c->ctype[cu].p, c->map.p,
-                                     c->stream, guard));
-        DEME_LAUNCH(c, "cand_mirror",
-                    launch_cand_mirror(c->kp, c->sph.p, c->own.p, nS, c->rowStart[cu].p, c->planeBits.p,
+                                     c->rowStart[cu].p, c->numAnal.p, c->tPos.p, c->tRank.p, c->bRowStart.p,
+                                     mapFromCand ? c->candAcc[pv].p : nullptr, c->cidx[pv].p, c->cidx[cu].p, c->idA.p,
+                                     c->idB[cu].p, c->ctype[cu].p, c->map.p, c->mirror.p, c->stream, guard));
+        DEME_LAUNCH(c, "cand_planes",
+                    launch_cand_planes(c->kp, c->sph.p, c->own.p, nS, c->rowStart[cu].p, c->planeBits.p,
                                        c->havePrev ? c->rowStart[pv].p : nullptr, c->havePrev ? c->idB[pv].p : nullptr,
-                                       c->havePrev ? c->ctype[pv].p : nullptr, mapFromCand, c->tStart.p, c->tC.p,
-                                       c->candAcc[cu].p, c->cidx[cu].p, c->bRowStart.p, c->idA.p, c->idB[cu].p,
+                                       c->havePrev ? c->ctype[pv].p : nullptr, mapFromCand, c->idA.p, c->idB[cu].p,
                                        c->ctype[cu].p, c->map.p, c->mirror.p, c->stream, guard));
         if (!mapFromCand)
             DEME_LAUNCH(c, "history_map",
@@ -1207,6 +1208,8 @@ static int build_candidates(deme_ctx* c) {
     DEME_HIP(c, c->candA.ensure(nK));
     DEME_HIP(c, c->candB.ensure(nK));
     DEME_HIP(c, c->tC.ensure(nK));
+    DEME_HIP(c, c->tPos.ensure(nK));
+    DEME_HIP(c, c->tRank.ensure(nK));
     for (int k = 0; k < 2; k++) {
         DEME_HIP(c, c->candAcc[k].ensure(nK));
         DEME_HIP(c, c->cidx[k].ensure(nK));
@@ -1224,7 +1227,7 @@ static int build_candidates(deme_ctx* c) {
                 scan_u32(c->bCnt.p, nullptr, nS, c->tStart.p, &c->ctr.p->total[2], c->scanScratch.p, c->stream));
     DEME_LAUNCH(c, "cand_tscatter",
                 launch_cand_tscatter(nK32, c->candB.p, c->cidx[0].p, c->tStart.p, c->tC.p, c->stream));
-    DEME_LAUNCH(c, "cand_tsort", launch_cand_tsort(nS, c->tStart.p, c->tC.p, c->stream));
+    DEME_LAUNCH(c, "cand_tsort", launch_cand_tsort(nS, c->tStart.p, c->tC.p, c->tPos.p, c->stream));
     c->candValid = true;
     c->rebuildNext = false;
     c->candBuilds++;
@@ -1266,12 +1269,12 @@ static int enqueue_skin_pass(deme_ctx* c, int cu) {
         // the row counts formed inside the scan's first pass
         DEME_LAUNCH(c, "scan_rows",
                     scan2_cand_rows(c->candStart.p, c->tStart.p, c->tC.p, c->candAcc[cu].p, c->numAnal.p, nS,
-                                    c->rowStart[cu].p, c->bRowStart.p, &c->ctr.p->total[1], &c->ctr.p->total[2],
+                                    c->tRank.p, c->rowStart[cu].p, c->bRowStart.p, &c->ctr.p->total[1], &c->ctr.p->total[2],
                                     c->scanTmp.p, c->scan2Sums.p, c->ctr.p, c->stream));
     } else {
         DEME_LAUNCH(c, "cand_count",
                     launch_cand_count(nS, c->candStart.p, c->tStart.p, c->tC.p, c->candAcc[cu].p, c->aCnt.p,
-                                      c->bCnt.p, c->ctr.p, c->stream));
+                                      c->bCnt.p, c->tRank.p, c->ctr.p, c->stream));
         DEME_LAUNCH(c, "scan_rows",
                     scan_u32(c->aCnt.p, c->numAnal.p, nS, c->rowStart[cu].p, &c->ctr.p->total[1], c->scanScratch.p,
                              c->stream));
@@ -1691,7 +1694,7 @@ static std::vector<const void*> cycle_fingerprint(deme_ctx* c, int p) {
             v(c->grid.nz),   v(invBits),      v(c->jitActive),  v(c->ownerJitActive), v(c->maskAny),
             c->pos.p,        c->pos0.p,       c->meta.p,        c->candStart.p,   c->candB.p,      v(skinBits),
             c->tStart.p,     c->tC.p,         c->candAcc[0].p,  c->candAcc[1].p,  c->cidx[0].p,    c->cidx[1].p,
-            c->candA.p,      v(c->nCand),     c->dispRef.p};
+            c->candA.p,      v(c->nCand),     c->dispRef.p,     c->tPos.p,        c->tRank.p};
 }
 
 // Buffers of the list at parity p (the one the interval's pass writes; the other parity holds the current list and

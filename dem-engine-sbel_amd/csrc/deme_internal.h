@@ -91,9 +91,10 @@ hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32
                             unsigned long long* total2, uint2* tmp, uint64_t* sums, hipStream_t st);
 // The candidate-list pass's two row scans with the counts formed in the first pass: out1 = scan(numAnal[i] + touching
 // candidates of row i), out2 = scan(touching candidates with partner i); same sizes and limit as scan2_u32_gather.
+// tRank[t] of each touching transposed slot t: its rank in its mirror row (other slots are not written).
 // When ctr->err is set (the pass failed and is discarded) pass 1 does nothing and the outputs are meaningless.
 hipError_t scan2_cand_rows(const uint32_t* candStart, const uint32_t* tStart, const uint32_t* tC, const uint8_t* acc,
-                           const uint32_t* numAnal, uint64_t n, uint32_t* out1, uint32_t* out2,
+                           const uint32_t* numAnal, uint64_t n, uint32_t* tRank, uint32_t* out1, uint32_t* out2,
                            unsigned long long* total1, unsigned long long* total2, uint2* tmp, uint64_t* sums,
                            const DevCounters* ctr, hipStream_t st);
 
@@ -182,33 +183,33 @@ hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t
 hipError_t launch_cand_flags(const KParams& P, uint32_t nK, const CdRec* pos, const uint2* meta, const uint32_t* candA,
                              const uint32_t* candB, const uint8_t* mask, double dispLimit2, uint8_t* acc,
                              DevCounters* ctr, double* uOut, hipStream_t st);
-// Per sphere: aCnt = touching candidates of its row, bCnt = touching candidates with it as partner.
+// Per sphere: aCnt = touching candidates of its row, bCnt = touching candidates with it as partner; tRank as in
+// scan2_cand_rows.
 hipError_t launch_cand_count(uint32_t nS, const uint32_t* candStart, const uint32_t* tStart, const uint32_t* tC,
-                             const uint8_t* acc, uint32_t* aCnt, uint32_t* bCnt, const DevCounters* ctr,
-                             hipStream_t st);
-// Per touching candidate: its canonical-row entry, cidx[c] and, with prevAcc / prevCidx (the previous pass over the
-// same candidates), its persistent-contact map entry.
+                             const uint8_t* acc, uint32_t* aCnt, uint32_t* bCnt, uint32_t* tRank,
+                             const DevCounters* ctr, hipStream_t st);
+// Per touching candidate: its canonical-row entry, its mirror link (mirror[contact] = bRowStart[B] + tRank[tPos[c]]),
+// cidx[c] and, with prevAcc / prevCidx (the previous pass over the same candidates), its persistent-contact map entry.
 hipError_t launch_cand_fill(uint32_t nK, const uint32_t* candA, const uint32_t* candB, const uint32_t* candStart,
                             const uint8_t* acc, const uint32_t* rowStart, const uint32_t* numAnal,
+                            const uint32_t* tPos, const uint32_t* tRank, const uint32_t* bRowStart,
                             const uint8_t* prevAcc, const uint32_t* prevCidx, uint32_t* cidx, uint32_t* idA,
-                            uint32_t* idB, uint8_t* ctype, uint32_t* map, hipStream_t st,
+                            uint32_t* idB, uint8_t* ctype, uint32_t* map, uint32_t* mirror, hipStream_t st,
                             SpecGuard guard = SpecGuard());
-// Per sphere: its plane contacts (with their map entries when mapPlanes) and the mirror links of the touching
-// candidates with it as partner.
-hipError_t launch_cand_mirror(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
+// Per sphere: its plane contacts at the head of its canonical row (with their map entries when mapPlanes; no mirror
+// slot).
+hipError_t launch_cand_planes(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
                               const uint32_t* rowStart, const unsigned long long* planeBits,
                               const uint32_t* prevRowStart, const uint32_t* prevIdB, const uint8_t* prevType,
-                              bool mapPlanes, const uint32_t* tStart, const uint32_t* tC, const uint8_t* acc,
-                              const uint32_t* cidx, const uint32_t* bRowStart, uint32_t* idA, uint32_t* idB,
-                              uint8_t* ctype, uint32_t* map, uint32_t* mirror, hipStream_t st,
-                              SpecGuard guard = SpecGuard());
+                              bool mapPlanes, uint32_t* idA, uint32_t* idB, uint8_t* ctype, uint32_t* map,
+                              uint32_t* mirror, hipStream_t st, SpecGuard guard = SpecGuard());
 // After a build, the transpose from the rows: slot[c] = c's place among the candidates with its partner (tCnt counts
 // them, cleared by the caller); then, with tStart scanned from tCnt, tC[tStart[B] + slot[c]] = c; then every
-// transposed row sorted ascending (ascending A).
+// transposed row sorted ascending (ascending A), and each candidate's transposed slot recorded (tPos[tC[t]] = t).
 hipError_t launch_cand_tcount(uint32_t nK, const uint32_t* candB, uint32_t* tCnt, uint32_t* slot, hipStream_t st);
 hipError_t launch_cand_tscatter(uint32_t nK, const uint32_t* candB, const uint32_t* slot, const uint32_t* tStart,
                                 uint32_t* tC, hipStream_t st);
-hipError_t launch_cand_tsort(uint32_t nS, const uint32_t* tStart, uint32_t* tC, hipStream_t st);
+hipError_t launch_cand_tsort(uint32_t nS, const uint32_t* tStart, uint32_t* tC, uint32_t* tPos, hipStream_t st);
 
 // dyn.hip
 // aTerm / gTerm: 6 floats per contact; bTerm: 6 floats per mirror entry; out (optional): full contact records

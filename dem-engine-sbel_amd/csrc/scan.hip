@@ -167,7 +167,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce(const uint32_t* __r
 
 // Pass 1 of the candidate-list pass's row scans: the counts are formed here, not read -- per sphere the touching
 // candidates of its row plus its plane contacts, and the touching candidates with it as partner (through its
-// transposed row) -- stored in tmp and folded into the tile sums. A failed pass (ctr->err) leaves them alone.
+// transposed row) -- stored in tmp and folded into the tile sums. Each touching transposed slot t also gets its rank
+// in the mirror row, tRank[t] (the touching slots before it in its row), from which the fill pass links the contact to
+// its mirror slot (a rank by slot: consecutive slots are written together, where ranks by candidate would scatter).
+// A failed pass (ctr->err) leaves them alone.
 // Rows are short (a few candidates), so a lane does not walk its ITEMS spheres one after another: it loads the row
 // bounds of all of them, then the first CAND_SPEC entries of every row and transposed row, then the flags of those,
 // and only longer rows finish in a loop -- three rounds of loads in flight instead of a chain per sphere.
@@ -178,7 +181,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce_cand(const uint32_t
                                                                   const uint32_t* __restrict__ tC,
                                                                   const uint8_t* __restrict__ acc,
                                                                   const uint32_t* __restrict__ numAnal, uint64_t n,
-                                                                  uint2* __restrict__ tmp,
+                                                                  uint32_t* __restrict__ tRank, uint2* __restrict__ tmp,
                                                                   uint64_t* __restrict__ sums1,
                                                                   uint64_t* __restrict__ sums2,
                                                                   const DevCounters* ctr) {
@@ -210,15 +213,21 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce_cand(const uint32_t
                 if (c0[i] + j < c1[i]) v1[i] += acc[c0[i] + j];
                 q[i][j] = t0[i] + j < t1[i] ? tC[t0[i] + j] : 0u;
             }
+        uint32_t f[G][CAND_SPEC];
+#pragma unroll
+        for (int i = 0; i < G; i++)
+#pragma unroll
+            for (uint32_t j = 0; j < CAND_SPEC; j++) f[i][j] = t0[i] + j < t1[i] ? acc[q[i][j]] : 0u;
 #pragma unroll
         for (int i = 0; i < G; i++)
 #pragma unroll
             for (uint32_t j = 0; j < CAND_SPEC; j++)
-                if (t0[i] + j < t1[i]) v2[i] += acc[q[i][j]];
+                if (f[i][j]) tRank[t0[i] + j] = v2[i]++;
 #pragma unroll
         for (int i = 0; i < G; i++) {
             for (uint32_t c = c0[i] + CAND_SPEC; c < c1[i]; c++) v1[i] += acc[c];
-            for (uint32_t t = t0[i] + CAND_SPEC; t < t1[i]; t++) v2[i] += acc[tC[t]];
+            for (uint32_t t = t0[i] + CAND_SPEC; t < t1[i]; t++)
+                if (acc[tC[t]]) tRank[t] = v2[i]++;
             const uint64_t k = base + (uint64_t)(g + i) * SCAN_BLOCK + threadIdx.x;
             if (k < n) tmp[k] = make_uint2(v1[i], v2[i]);
             s1 += v1[i];
@@ -342,22 +351,22 @@ hipError_t scan2_u32_gather(const uint32_t* a1, const uint32_t* b1, const uint32
 }
 
 hipError_t scan2_cand_rows(const uint32_t* candStart, const uint32_t* tStart, const uint32_t* tC, const uint8_t* acc,
-                           const uint32_t* numAnal, uint64_t n, uint32_t* out1, uint32_t* out2,
+                           const uint32_t* numAnal, uint64_t n, uint32_t* tRank, uint32_t* out1, uint32_t* out2,
                            unsigned long long* total1, unsigned long long* total2, uint2* tmp, uint64_t* sums,
                            const DevCounters* ctr, hipStream_t st) {
     if (n <= SCAN2_SMALL_N) {
         // small lists: one sphere per lane, so the latency chain of each lane is one row
         const uint32_t nb = (uint32_t)(n ? (n + SCAN_BLOCK - 1) / SCAN_BLOCK : 1);
-        k_scan2_reduce_cand<1><<<nb, SCAN_BLOCK, 0, st>>>(candStart, tStart, tC, acc, numAnal, n, tmp, sums, sums + nb,
-                                                          ctr);
+        k_scan2_reduce_cand<1><<<nb, SCAN_BLOCK, 0, st>>>(candStart, tStart, tC, acc, numAnal, n, tRank, tmp, sums,
+                                                          sums + nb, ctr);
         k_scan2_apply<1><<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, true);
         return hipGetLastError();
     }
     const uint64_t nb64 = (n + SCAN2_TILE - 1) / SCAN2_TILE;
     if (nb64 > SCAN_TILE) return hipErrorInvalidValue;
     const uint32_t nb = (uint32_t)nb64;
-    k_scan2_reduce_cand<SCAN2_ITEMS><<<nb, SCAN_BLOCK, 0, st>>>(candStart, tStart, tC, acc, numAnal, n, tmp, sums,
-                                                                sums + nb, ctr);
+    k_scan2_reduce_cand<SCAN2_ITEMS><<<nb, SCAN_BLOCK, 0, st>>>(candStart, tStart, tC, acc, numAnal, n, tRank, tmp,
+                                                                sums, sums + nb, ctr);
     const bool fold = nb <= SCAN_FOLD_TILES;
     if (!fold) k_scan2_tiles<<<1, SCAN_BLOCK, 0, st>>>(sums, sums + nb, nb);
     k_scan2_apply<SCAN2_ITEMS><<<nb, SCAN_BLOCK, 0, st>>>(tmp, n, sums, sums + nb, out1, out2, total1, total2, fold);
