@@ -73,9 +73,9 @@ def bytes_model(name, k):
         "cell_pairs": S * (32 + 4 + 4) + O * 4 + Nc * 4 + Np * (4 + 4 + 8),
         "cell_rows": S * (4 + 8) + Np * 4 + C * 9,
         "fill": (Nc + O + 1) * 4 / 2,  # two launches per pass: cell counts and owner B-side counts
-        # candidate lists: per sphere template + owner pose and the build position in; CD record, (owner, family),
-        # bin and plane counts and plane bits out
-        "sphere_pos": S * sph + O * owner_cd + S * 32 + S * (32 + 8 + 2 + 4 + 8),
+        # candidate lists: per sphere template + owner pose and the build position (fixed point) in; CD record, plane
+        # count and plane bits out
+        "sphere_pos": S * sph + O * owner_cd + S * 12 + S * (32 + 4 + 8),
         # per candidate its sphere and partner ids and the touching flag out; the spheres' records once
         "cand_flags": S * 32 + Nk * (4 + 4 + 1),
         # per sphere its row and transposed-row bounds, both counts out; per candidate its flag (row order) and its
@@ -96,7 +96,7 @@ def bytes_model(name, k):
         "cand_build": S * (32 + 4 + 4) + O * 4 + Nc * 4 + Nk * (4 + 4 + 8),
         "scan_cand": S * 4 * 2,
         # candidate rows copied (sorted) from the slab into id order with their sphere ids, build positions out
-        "cand_rows": S * (4 + 4 + 32 + 32) + Nk * (4 + 4 + 4),
+        "cand_rows": S * (4 + 4 + 32 + 12) + Nk * (4 + 4 + 4),
         "anal_final": k["anal_blocks"] * 48 * 4,
         # per owner: own record in/out, MOI, accelerations out; A rows contiguous, B lists gathered
         "owner_update": O * (64 + 64 + 16 + 24 + 8) + C * 48 + Np * (4 + 48),

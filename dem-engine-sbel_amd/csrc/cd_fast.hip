@@ -165,7 +165,8 @@ __global__ __launch_bounds__(256) void k_sphere_pos(KParams P, const SphereRec* 
                                                     const OwnerRec* __restrict__ own, uint32_t nS,
                                                     const PlanePose* __restrict__ pose, uint32_t nA,
                                                     const uint8_t* __restrict__ mask, CdRec* __restrict__ pos,
-                                                    const CdRec* __restrict__ pos0, uint2* __restrict__ meta,
+                                                    const BuildPos* __restrict__ pos0, double unit,
+                                                    uint2* __restrict__ meta,
                                                     uint16_t* __restrict__ numBins, uint32_t* __restrict__ numAnal,
                                                     unsigned long long* __restrict__ planeBits, DevCounters* ctr,
                                                     const double* __restrict__ uIn) {
@@ -188,13 +189,20 @@ __global__ __launch_bounds__(256) void k_sphere_pos(KParams P, const SphereRec* 
         d4[0] = s4[0];
         d4[1] = s4[1];
         if (meta) meta[s] = make_uint2(sr.owner, o.fam);
-        const CdRec c0 = load_cd(pos0, s);
-        const double mx = cr.x - c0.x, my = cr.y - c0.y, mz = cr.z - c0.z;
-        m3[0] = mx;
-        m3[1] = my;
-        m3[2] = mz;
-        const double dx = mx - ux, dy = my - uy, dz = mz - uz;
-        d2 = dx * dx + dy * dy + dz * dz;
+        // the build position in fixed point (int -> double and the power-of-two scaling are exact); one that did
+        // not fit counts as an infinite displacement, so the lists are found stale
+        const BuildPos b = pos0[s];
+        if (b.x == INT32_MIN || b.y == INT32_MIN || b.z == INT32_MIN) {
+            d2 = INFINITY;
+        } else {
+            const double mx = cr.x - (double)b.x * unit, my = cr.y - (double)b.y * unit,
+                         mz = cr.z - (double)b.z * unit;
+            m3[0] = mx;
+            m3[1] = my;
+            m3[2] = mz;
+            const double dx = mx - ux, dy = my - uy, dz = mz - uz;
+            d2 = dx * dx + dy * dy + dz * dz;
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, 64));
@@ -709,24 +717,30 @@ __global__ __launch_bounds__(256) void k_cell_rows(KParams P, const SphereRec* _
     for (uint32_t q = st; q < en; q++) idA[q] = s;
 }
 
+// A build position coordinate in fixed point (scale = 1 / unit): rounded to the nearest unit, INT32_MIN when it does
+// not fit in (-2^31, 2^31) units (a NaN fails both comparisons).
+__device__ __forceinline__ int32_t build_fix(double x, double scale) {
+    const double v = rint(x * scale);
+    return (v > -2147483648.0 && v < 2147483648.0) ? (int32_t)v : INT32_MIN;
+}
+
 // Candidate lists from a candidate build: sphere s's row of higher-id partners within the skin (ascending), copied
-// from its slab column at its sorted position; the sphere's position at the build is kept for the displacement test.
+// from its slab column at its sorted position; the sphere's position at the build is kept (in fixed point) for the
+// displacement test.
 __global__ __launch_bounds__(256) void k_cand_rows(uint32_t nS, const uint32_t* __restrict__ rowSlab,
                                                    const uint32_t* __restrict__ candStart, uint32_t* __restrict__ candA,
                                                    uint32_t* __restrict__ candB, const CdRec* __restrict__ sorted,
-                                                   CdRec* __restrict__ pos0) {
+                                                   BuildPos* __restrict__ pos0, double scale) {
     // one lane per sorted position: the lanes of a wave read adjacent slab columns (coalesced) and write their
     // spheres' rows wherever the ids put them
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nS) return;
-    const uint32_t s = sorted[p].id;
+    const CdRec r = load_cd(sorted, p);
+    const uint32_t s = r.id;
     const uint32_t st = candStart[s], en = candStart[s + 1];
     sorted_copy(rowSlab + p, en - st, nS, candB + st);
     for (uint32_t c = st; c < en; c++) candA[c] = s;
-    const uint4* q = reinterpret_cast<const uint4*>(sorted + p);
-    uint4* d = reinterpret_cast<uint4*>(pos0 + s);
-    d[0] = q[0];
-    d[1] = q[1];
+    pos0[s] = BuildPos{build_fix(r.x, scale), build_fix(r.y, scale), build_fix(r.z, scale)};
 }
 
 // The candidates of one build are indexed once (candidate c: sphere candA[c], partner candB[c] of higher id; the
@@ -1057,19 +1071,19 @@ hipError_t launch_cell_pairs(const KParams& P, const CellGrid& G, uint32_t nS, c
 }
 
 hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
-                             const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos, const CdRec* pos0,
-                             uint2* meta, uint16_t* numBins, uint32_t* numAnal, unsigned long long* planeBits,
-                             DevCounters* ctr, const double* uIn, hipStream_t st) {
+                             const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos,
+                             const BuildPos* pos0, double unit, uint2* meta, uint16_t* numBins, uint32_t* numAnal,
+                             unsigned long long* planeBits, DevCounters* ctr, const double* uIn, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_sphere_pos<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, pose, nA, mask, pos, pos0, meta, numBins, numAnal,
-                                                  planeBits, ctr, uIn);
+    k_sphere_pos<<<blocks(nS, 256), 256, 0, st>>>(P, sph, own, nS, pose, nA, mask, pos, pos0, unit, meta, numBins,
+                                                  numAnal, planeBits, ctr, uIn);
     return hipGetLastError();
 }
 
 hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* candStart, uint32_t* candA,
-                            uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st) {
+                            uint32_t* candB, const CdRec* sorted, BuildPos* pos0, double scale, hipStream_t st) {
     if (nS == 0) return hipSuccess;
-    k_cand_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, rowSlab, candStart, candA, candB, sorted, pos0);
+    k_cand_rows<<<blocks(nS, 256), 256, 0, st>>>(nS, rowSlab, candStart, candA, candB, sorted, pos0, scale);
     return hipGetLastError();
 }
 

@@ -138,7 +138,10 @@ struct deme_ctx {
     // passes the lists before the latest build lasted (~0 before the second build) and whether the pass in hand built
     uint64_t lastLifetime = ~0ull;
     bool builtThisPass = false;
-    DBuf<CdRec> pos, pos0;
+    DBuf<CdRec> pos;
+    // build positions in fixed point (coordinate = q * posUnit, posUnit a power of two set at each build)
+    DBuf<BuildPos> pos0;
+    double posUnit = 0.0;
     // the common translation the displacement test subtracts: slot p (3 doubles) written by the pass at list parity p
     // (the midpoint of its displacement extremes since the build), read by the next pass; both zero after a build
     DBuf<double> dispRef;
@@ -1131,7 +1134,8 @@ constexpr double SKIN_SLACK = 1e-9;
 
 // The candidate-list pass is stale once some sphere has moved half the skin (less the slack) since the build.
 static double stale_limit2(const deme_ctx* c) {
-    const double h = 0.5 * c->skin() - SKIN_SLACK;
+    // (posUnit bounds the fixed-point rounding of a build position: sqrt(3) / 2 units)
+    const double h = 0.5 * c->skin() - SKIN_SLACK - c->posUnit;
     return h > 0.0 ? h * h : 0.0;
 }
 
@@ -1214,11 +1218,17 @@ static int build_candidates(deme_ctx* c) {
         DEME_HIP(c, c->candAcc[k].ensure(nK));
         DEME_HIP(c, c->cidx[k].ensure(nK));
     }
-    // the build positions are the reference: no displacement, no common translation yet
+    // the build positions are the reference: no displacement, no common translation yet. They are kept in fixed
+    // point with a power-of-two unit such that +-2^31 units cover twice the bin grid's largest extent.
+    {
+        const double extent = std::max({(double)c->kp.nbX, (double)c->kp.nbY, (double)c->kp.nbZ}) * c->kp.binSize;
+        const int e = std::ilogb(std::max(2.0 * extent, 1e-300)) + 1;  // 2^e > twice the extent
+        c->posUnit = std::ldexp(1.0, e - 31);
+    }
     DEME_LAUNCH(c, "fill", launch_fill_u32(reinterpret_cast<uint32_t*>(c->dispRef.p), 12, 0, c->stream));
     DEME_LAUNCH(c, "cand_rows",
                 launch_cand_rows(nS, c->rowSlab.p, c->candStart.p, c->candA.p, c->candB.p, c->sorted.p, c->pos0.p,
-                                 c->stream));
+                                 1.0 / c->posUnit, c->stream));
     // the transpose, counted from the rows (cidx[0] holds each candidate's place until the first pass writes it)
     const uint32_t nK32 = (uint32_t)c->nCand;
     DEME_LAUNCH(c, "fill", launch_fill_u32(c->bCnt.p, nS, 0, c->stream));
@@ -1259,7 +1269,8 @@ static int enqueue_skin_pass(deme_ctx* c, int cu) {
     // (the families for the mask only when a mask is in force; no per-sphere bin counts: this path never reads them)
     DEME_LAUNCH(c, "sphere_pos",
                 launch_sphere_pos(c->kp, c->sph.p, c->own.p, nS, c->pose.p, c->nA, c->maskAny ? c->mask.p : nullptr,
-                                  c->pos.p, c->pos0.p, c->maskAny ? c->meta.p : nullptr, nullptr, c->numAnal.p,
+                                  c->pos.p, c->pos0.p, c->posUnit, c->maskAny ? c->meta.p : nullptr, nullptr,
+                                  c->numAnal.p,
                                   c->planeBits.p, c->ctr.p, c->dispRef.p + 3 * (cu ^ 1), c->stream));
     DEME_LAUNCH(c, "cand_flags",
                 launch_cand_flags(c->kp, (uint32_t)c->nCand, c->pos.p, c->meta.p, c->candA.p, c->candB.p,
@@ -1675,10 +1686,11 @@ static int step_graph(deme_ctx* c) {
 static std::vector<const void*> cycle_fingerprint(deme_ctx* c, int p) {
     const int q = p ^ 1;
     auto v = [](uint64_t x) { return reinterpret_cast<const void*>((uintptr_t)x); };
-    uint64_t invBits, skinBits;
+    uint64_t invBits, skinBits, unitBits;
     std::memcpy(&invBits, &c->grid.inv, sizeof(invBits));
     const double skin = c->skin();
     std::memcpy(&skinBits, &skin, sizeof(skinBits));
+    std::memcpy(&unitBits, &c->posUnit, sizeof(unitBits));
     return {c->sph.p,        c->own.p,        c->moi.p,         c->anal.p,        c->pose.p,       c->mats.p,
             c->mask.p,       c->rules.p,      c->ownerSphStart.p, c->cellCount.p, c->cellStart.p,  c->cellOf.p,
             c->slotOf.p,     c->posOf.p,      c->aCnt.p,        c->bCnt.p,        c->sortedMeta.p, c->sorted.p,
@@ -1694,7 +1706,7 @@ static std::vector<const void*> cycle_fingerprint(deme_ctx* c, int p) {
             v(c->grid.nz),   v(invBits),      v(c->jitActive),  v(c->ownerJitActive), v(c->maskAny),
             c->pos.p,        c->pos0.p,       c->meta.p,        c->candStart.p,   c->candB.p,      v(skinBits),
             c->tStart.p,     c->tC.p,         c->candAcc[0].p,  c->candAcc[1].p,  c->cidx[0].p,    c->cidx[1].p,
-            c->candA.p,      v(c->nCand),     c->dispRef.p,     c->tPos.p,        c->tRank.p};
+            c->candA.p,      v(c->nCand),     c->dispRef.p,     c->tPos.p,        c->tRank.p,      v(unitBits)};
 }
 
 // Buffers of the list at parity p (the one the interval's pass writes; the other parity holds the current list and

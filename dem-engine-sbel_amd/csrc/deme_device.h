@@ -49,6 +49,14 @@ struct __attribute__((aligned(16))) CdRec {
 };
 static_assert(sizeof(CdRec) == 32, "CdRec must be 32 bytes");
 
+// A sphere's position at a candidate build, in fixed point: coordinate = q * unit, unit a power of two chosen so that
+// +-2^31 units cover twice the bin grid's extent (rounding error at most unit / 2 per axis). INT32_MIN on an axis marks
+// a position that did not fit (or was not a number).
+struct BuildPos {
+    int32_t x, y, z;
+};
+static_assert(sizeof(BuildPos) == 12, "BuildPos must be 12 bytes");
+
 // Per-contact outputs of the force kernel (contactForces, contactTorque_convToForce, contactPointGeometryA/B).
 struct __attribute__((aligned(16))) ForceRec {
     float fx, fy, fz;

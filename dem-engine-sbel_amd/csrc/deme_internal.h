@@ -11,6 +11,7 @@
 
 namespace deme_int {
 
+using deme_dev::BuildPos;
 using deme_dev::CdRec;
 using deme_dev::DevCounters;
 using deme_dev::DEV_COUNTERS_HOST_BYTES;
@@ -171,13 +172,14 @@ hipError_t launch_cell_rows(const KParams& P, const SphereRec* sph, const OwnerR
 // The largest squared displacement since the build is taken relative to the translation uIn (3 doubles; any vector
 // keeps the candidate lists exact); the displacements' extremes per axis are folded into ctr for the next pass's.
 hipError_t launch_sphere_pos(const KParams& P, const SphereRec* sph, const OwnerRec* own, uint32_t nS,
-                             const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos, const CdRec* pos0,
-                             uint2* meta, uint16_t* numBins, uint32_t* numAnal, unsigned long long* planeBits,
-                             DevCounters* ctr, const double* uIn, hipStream_t st);
+                             const PlanePose* pose, uint32_t nA, const uint8_t* mask, CdRec* pos,
+                             const BuildPos* pos0, double unit, uint2* meta, uint16_t* numBins, uint32_t* numAnal,
+                             unsigned long long* planeBits, DevCounters* ctr, const double* uIn, hipStream_t st);
 // After a candidate build: per sphere its candidate row (candStart from a scan of the build's row counts; candA /
-// candB the row's sphere and partners) and its position at the build (pos0).
+// candB the row's sphere and partners) and its position at the build (pos0, in fixed point: coordinate * scale
+// rounded, scale a power of two).
 hipError_t launch_cand_rows(uint32_t nS, const uint32_t* rowSlab, const uint32_t* candStart, uint32_t* candA,
-                            uint32_t* candB, const CdRec* sorted, CdRec* pos0, hipStream_t st);
+                            uint32_t* candB, const CdRec* sorted, BuildPos* pos0, double scale, hipStream_t st);
 // Per step, per candidate: the reference's predicate (acc[c] = 1 when c touches); ERRBIT_STALE instead when
 // ctr->disp2 exceeds dispLimit2. Lane 0 also writes the next pass's common translation (3 doubles) to uOut.
 hipError_t launch_cand_flags(const KParams& P, uint32_t nK, const CdRec* pos, const uint2* meta, const uint32_t* candA,
