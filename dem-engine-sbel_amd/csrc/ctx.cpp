@@ -80,6 +80,10 @@ float mean_radius(const std::vector<float>& r) {
 
 }  // namespace
 
+// Rounding allowance of the candidate lists' displacement test: positions are doubles near 1 m, so a nanometre lies
+// far above their rounding and far below any skin.
+constexpr double SKIN_SLACK = 1e-9;
+
 struct deme_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -141,7 +145,7 @@ struct deme_ctx {
     DBuf<CdRec> pos;
     // build positions in fixed point (coordinate = q * posUnit, posUnit a power of two set at each build)
     DBuf<BuildPos> pos0;
-    double posUnit = 0.0;
+    double posUnit = 0.0;  // (from the bin grid: deme_set_params)
     // the common translation the displacement test subtracts: slot p (3 doubles) written by the pass at list parity p
     // (the midpoint of its displacement extremes since the build), read by the next pass; both zero after a build
     DBuf<double> dispRef;
@@ -157,9 +161,13 @@ struct deme_ctx {
     DBuf<uint8_t> candAcc[2];
     uint64_t cidxGen[2] = {0, 0};
     double skin() const {
-        if (skinSet >= 0.0) return skinSet;
-        if (cellPassesLeft > 0) return 0.0;
-        return skinAuto > 0.0 ? skinAuto : 0.2 * (double)meanRadius;
+        double s;
+        if (skinSet >= 0.0) s = skinSet;
+        else if (cellPassesLeft > 0) s = 0.0;
+        else s = skinAuto > 0.0 ? skinAuto : 0.2 * (double)meanRadius;
+        // a skin too thin for the build positions' fixed-point unit (a world very large for its spheres) would find
+        // every pass stale: the cell sweep runs instead
+        return 0.5 * s > SKIN_SLACK + 4.0 * posUnit ? s : 0.0;
     }
     int pairMode = -1;  // pair sweep mode (launch_cell_pairs); -1: chosen from the radius spread
     uint32_t pairBlock = 256;  // pair sweep workgroup size
@@ -547,6 +555,13 @@ int deme_set_params(deme_ctx* c, const deme_sim_params* p) {
     k.nBins = (uint32_t)nb;
     k.nvX = p->nvXp2;
     k.nvY = p->nvYp2;
+    // candidate-list build positions are kept in fixed point with a power-of-two unit such that +-2^31 units cover
+    // twice the bin grid's largest extent
+    {
+        const double extent = std::max({(double)p->nbX, (double)p->nbY, (double)p->nbZ}) * p->binSize;
+        const int e = std::ilogb(std::max(2.0 * extent, 1e-300)) + 1;  // 2^e > twice the extent
+        c->posUnit = std::ldexp(1.0, e - 31);
+    }
     k.beta = p->beta;
     k.h = p->h;
     k.Gx = p->G[0];
@@ -1128,10 +1143,6 @@ static int enqueue_pair_pass(deme_ctx* c, int mode, int cu) {
     return DEME_OK;
 }
 
-// Rounding allowance of the candidate lists' displacement test: positions are doubles near 1 m, so a nanometre lies
-// far above their rounding and far below any skin.
-constexpr double SKIN_SLACK = 1e-9;
-
 // The candidate-list pass is stale once some sphere has moved half the skin (less the slack) since the build.
 static double stale_limit2(const deme_ctx* c) {
     // (posUnit bounds the fixed-point rounding of a build position: sqrt(3) / 2 units)
@@ -1218,13 +1229,7 @@ static int build_candidates(deme_ctx* c) {
         DEME_HIP(c, c->candAcc[k].ensure(nK));
         DEME_HIP(c, c->cidx[k].ensure(nK));
     }
-    // the build positions are the reference: no displacement, no common translation yet. They are kept in fixed
-    // point with a power-of-two unit such that +-2^31 units cover twice the bin grid's largest extent.
-    {
-        const double extent = std::max({(double)c->kp.nbX, (double)c->kp.nbY, (double)c->kp.nbZ}) * c->kp.binSize;
-        const int e = std::ilogb(std::max(2.0 * extent, 1e-300)) + 1;  // 2^e > twice the extent
-        c->posUnit = std::ldexp(1.0, e - 31);
-    }
+    // the build positions (in fixed point, posUnit) are the reference: no displacement, no common translation yet
     DEME_LAUNCH(c, "fill", launch_fill_u32(reinterpret_cast<uint32_t*>(c->dispRef.p), 12, 0, c->stream));
     DEME_LAUNCH(c, "cand_rows",
                 launch_cand_rows(nS, c->rowSlab.p, c->candStart.p, c->candA.p, c->candB.p, c->sorted.p, c->pos0.p,

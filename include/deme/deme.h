@@ -157,7 +157,9 @@ int deme_set_graph_headroom(deme_ctx* ctx, double factor, uint64_t slack);
  * retires (widened 1.5x when they died within a pass, narrowed 0.75x when they outlasted 400 passes, left alone from
  * 30 passes on; lists shorter-lived than that steer it 1.25x at a time by their candidates x (1 + 2.7 / passes
  * lasted)); when lists die within 4 passes even at the widest skin, the cell sweep runs every pass for 64 passes
- * (doubling while that repeats, up to 4096) before lists are tried again. 0: off (the cell sweep every pass). */
+ * (doubling while that repeats, up to 4096) before lists are tried again. 0: off (the cell sweep every pass). The
+ * build positions are kept in fixed point (a power-of-two unit u, about 2^-31 of twice the bin grid's largest
+ * extent); a skin with skin / 2 <= 1 nm + 4u also runs the cell sweep (deme_candidate_stats then reports 0). */
 int deme_set_skin(deme_ctx* ctx, double skin);
 /* The skin in force (0 while the automatic mode runs the cell sweep), the candidate pairs of the last build and the
  * number of builds so far (each nullable). */

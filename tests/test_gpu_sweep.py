@@ -143,6 +143,29 @@ def test_automatic_skin_gives_way_to_the_cell_sweep_and_back(graphs):
     assert any(modes[first_cell:]), "the lists were not tried again"
 
 
+def test_skin_below_the_build_position_resolution_runs_the_cell_sweep():
+    """Build positions are kept in fixed point (a unit near 2^-31 of twice the world's extent): a skin whose half does
+    not exceed 1 nm plus four units could never hold its lists, so the cell sweep runs instead -- no builds, a reported
+    skin of 0 -- with the cell sweep's trajectory and contacts, bit for bit."""
+    sc = scenes.sphere_settling(n=3000, spacing=2.02, velocity_sigma=0.5, seed=1036)
+    a, b = DeviceSystem(sc), DeviceSystem(sc)
+    a.set_skin(2e-9)
+    b.set_skin(0.0)
+    a.step(20, 1)
+    b.step(20, 1)
+    sa, sb = a.get_state(), b.get_state()
+    for k in KEYS:
+        assert np.array_equal(sa[k], sb[k]), k
+    for x, y in zip(a.contacts(), b.contacts()):
+        assert np.array_equal(x, y)
+    skin, n_cand, builds = a.candidate_stats()
+    assert skin == 0.0 and builds == 0
+    # a skin well above the resolution keeps its lists
+    a.set_skin(2e-5)
+    a.step(5, 1)
+    assert a.candidate_stats()[0] == 2e-5 and a.candidate_stats()[2] >= 1
+
+
 @pytest.mark.parametrize("graphs", [False, True])
 def test_lists_survive_a_common_translation(graphs):
     """Every sphere drifting and falling together (0.5 m/s sideways plus gravity, clear of each other and of the
