@@ -187,7 +187,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan2_reduce_cand(const uint32_t
                                                                   const DevCounters* ctr) {
     if (ctr->err) return;
     const uint64_t base = (uint64_t)blockIdx.x * (uint64_t)(SCAN_BLOCK * ITEMS);
-    // spheres in groups of G: the group's loads in flight fit the registers of eight waves per SIMD
+    // spheres in groups of G: with one group's loads in flight (70 VGPRs, seven waves per SIMD) a launch over 4M
+    // spheres is resident at once; all eight spheres at a time take 118 VGPRs, four waves, and run slower
     constexpr int G = ITEMS < 4 ? ITEMS : 4;
     static_assert(ITEMS % G == 0, "whole groups");
     uint64_t s1 = 0, s2 = 0;
